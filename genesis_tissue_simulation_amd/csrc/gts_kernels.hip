@@ -1,0 +1,1072 @@
+// HIP kernels and C entry points of the single-instance XPBD substep and the swept-quad cut on gfx950.
+//
+// Layout in HBM (all device buffers belong to the caller):
+//   nodes   pos / prev / vel / force as float4 rows (x, y, z, .w = inverse mass for pos): one 16-byte load per node
+//           in the tet and edge gathers, coalesced rows in the node passes;
+//   edges   one packed 32-bit descriptor per edge in colour order: node i << 4 | support << 3 | axis << 1 | live;
+//           the other end, i + stride[axis], and the rest length (the lattice spacing) follow from the lattice;
+//   tets    one packed 32-bit descriptor per tet in colour order: corner-0 node << 4 | Kuhn type << 1 | live;
+//           the four nodes and Dm^-1 follow from the type, so a tet costs 4 B of topology instead of 52 B;
+//   per-tet alpha~ and lambda pairs in colour order, so a colour pass streams contiguous ranges.
+// Each colour of the reference's greedy colourings is one launch: no two members share a node, so the Gauss-Seidel
+// pass needs no atomics and gives the same result as the serial sweep (reference: xpbd/xpbd_solver.cu:3581-3612).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/gts_xpbd.h"
+#include "gts_common.h"
+#include "gts_contact.h"
+
+#define GTS_BLOCK 256
+
+#define HIP_TRY(call)                                   \
+    do {                                                \
+        hipError_t _e = (call);                         \
+        if (_e != hipSuccess) return -(int)_e;          \
+    } while (0)
+
+namespace {
+
+inline int grid_for(int n, int block = GTS_BLOCK) { return std::max(1, (n + block - 1) / block); }
+
+struct Lattice {  // strides and per-type rest data passed by value to every kernel
+    int nx, nxy, vx, vy, vz;
+    float inv_rest[6][9];
+    float rest_len[3];
+};
+
+Lattice lattice_of(const GtsTissueView* v) {
+    Lattice L;
+    L.vx = v->vx; L.vy = v->vy; L.vz = v->vz;
+    L.nx = v->vx + 1;
+    L.nxy = (v->vx + 1) * (v->vy + 1);
+    std::memcpy(L.inv_rest, v->inv_rest, sizeof(L.inv_rest));
+    for (int a = 0; a < 3; a++) L.rest_len[a] = v->axis_rest_length[a];
+    return L;
+}
+
+__device__ __forceinline__ int axis_stride(const Lattice& L, int axis) { return axis == 0 ? 1 : (axis == 1 ? L.nx : L.nxy); }
+
+__device__ __forceinline__ f3 xyz(float4 p) { return mk3(p.x, p.y, p.z); }
+
+// ------------------------------------------------------------------------------------------------ set-up kernels
+
+__global__ void k_init_state(int N, const float* mass, const int* active, const int* pinned, const int* nbr,
+                             float4* pos, uint32_t* bend_alive, uint32_t* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counters[0] = 0u;          // topology revision
+        counters[1] = 0u;          // cut point counter
+        counters[2] = 0xffffffffu; // revision the liveness was derived at: none yet
+        counters[3] = 1u;          // stale
+    }
+    if (i >= N) return;
+    // invMass = (pinned || !active) ? 0 : 1/mass (xpbd_solver.cu:458-459); active/pinned do not change on this path.
+    float4 p = pos[i];
+    p.w = (pinned[i] != 0 || active[i] == 0) ? 0.f : 1.f / mass[i];
+    pos[i] = p;
+    // bend pair p joins slots (a, b) in lexicographic order; alive when both links exist (grid.py:257-263).
+    uint32_t m = 0u;
+    int pair = 0;
+    for (int a = 0; a < 6; a++)
+        for (int b = a + 1; b < 6; b++, pair++)
+            if (nbr[6 * i + a] >= 0 && nbr[6 * i + b] >= 0) m |= 1u << pair;
+    bend_alive[i] = m;
+}
+
+__global__ void k_set_gravity(int N, const float* mass, float gx, float gy, float gz, float4* force) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float m = mass[i];  // force = acceleration * mass (physics.cu:317-321)
+    force[i] = make_float4(gx * m, gy * m, gz * m, 0.f);
+}
+
+// k_TopologyStale (unified_vertex.cuh:859-861): liveness is re-derived only after the revision moved.
+__global__ void k_topology_stale(uint32_t* counters) {
+    uint32_t rev = counters[0];
+    counters[3] = rev != counters[2] ? 1u : 0u;
+    counters[2] = rev;
+}
+
+__global__ void k_clear_support(int n, int* support, const uint32_t* counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && counters[3]) support[i] = 0;
+}
+
+// Tet domain of k_RefreshLiveness (xpbd_solver.cu:460-465) + k_MarkTetEdgeSupport (:474-479).
+__global__ void k_refresh_tets(int T, Lattice L, const int* nbr, const int* active, uint32_t* tet_desc, int* support,
+                               const uint32_t* counters) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T || !counters[3]) return;
+    uint32_t d = tet_desc[t];
+    int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+    bool live = true;
+    int owner[3], na = 0;
+    for (int a = 0; a < 4; a++) {
+        int ca = kuhn_corner(type, a);
+        live = live && active[base + corner_offset(ca, L.nx, L.nxy)] != 0;
+        for (int b = a + 1; b < 4; b++) {
+            int cb = kuhn_corner(type, b), diff = ca ^ cb;
+            if (diff != 1 && diff != 2 && diff != 4) continue;
+            int axis = diff == 1 ? 0 : (diff == 2 ? 1 : 2);
+            int lower = ((ca >> axis) & 1) == 0 ? ca : cb;
+            if (na < 3) owner[na++] = 6 * (base + corner_offset(lower, L.nx, L.nxy)) + 2 * axis + 1;
+        }
+    }
+    for (int k = 0; k < 3; k++) live = live && nbr[owner[k]] >= 0;
+    tet_desc[t] = (d & ~1u) | (live ? 1u : 0u);
+    if (live)
+        for (int k = 0; k < 3; k++) support[owner[k]] = 1;
+}
+
+// Edge domain of k_RefreshLiveness (xpbd_solver.cu:451-453) with the tet support bit for the fallback law.
+__global__ void k_refresh_edges(int E, Lattice L, const int* nbr, const int* active, const int* support,
+                                uint32_t* edge_desc, const uint32_t* counters) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E || !counters[3]) return;
+    uint32_t d = edge_desc[e];
+    int i = (int)(d >> 4), axis = (int)((d >> 1) & 3u);
+    int j = i + axis_stride(L, axis), slot = 6 * i + 2 * axis + 1;
+    bool live = nbr[slot] >= 0 && active[i] != 0 && active[j] != 0;
+    uint32_t sup = support[slot] ? 8u : 0u;
+    edge_desc[e] = (d & ~9u) | sup | (live ? 1u : 0u);
+}
+
+// k_ComputeTetAlpha (xpbd_solver.cu:578-587): alpha~ = (1/h^2) / (mu V) and (1/h^2) / (kappa V) once per frame.
+__global__ void k_tet_alpha(int T, float invH2, const float* kd, const float* kh, float* alpha) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    alpha[t] = invH2 / kd[t];
+    alpha[T + t] = invH2 / kh[t];
+}
+
+// ------------------------------------------------------------------------------------------------ substep kernels
+
+// k_predict (xpbd_solver.cu:393-412): prev = x; for w > 0, v += h F w and x += h v.
+__global__ void k_predict(int N, float h, float4* __restrict__ pos, float4* __restrict__ prev,
+                          float4* __restrict__ vel, const float4* __restrict__ force) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float4 p = pos[i];
+    prev[i] = p;
+    if (p.w > 0.f) {
+        float4 F = force[i];
+        f3 a = mk3(F.x, F.y, F.z) * p.w;
+        float4 v0 = vel[i];
+        f3 v = mk3(v0.x, v0.y, v0.z) + a * h;
+        vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+        f3 x = xyz(p) + v * h;
+        pos[i] = make_float4(x.x, x.y, x.z, p.w);
+    }
+}
+
+// k_solve_edges (xpbd_solver.cu:484-524) over one colour.
+template <bool READ_LAMBDA, bool WRITE_LAMBDA, bool FALLBACK, bool DAMPED>
+__global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fallbackInvEh2, float omega,
+                        float betaGammaH, const uint32_t* __restrict__ edge_desc, float* __restrict__ lambda,
+                        float4* __restrict__ pos, const float4* __restrict__ prev) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    int e = off + k;
+    uint32_t d = edge_desc[e];
+    if (!(d & 1u)) return;
+    int axis = (int)((d >> 1) & 3u);
+    float restLen = L.rest_len[axis];
+    if (FALLBACK) {
+        if (d & 8u) return;
+        alphaTilde = fallbackInvEh2 / restLen;
+    }
+    int i = (int)(d >> 4), j = i + axis_stride(L, axis);
+    float4 pi = pos[i], pj = pos[j];
+    float wi = pi.w, wj = pj.w, wsum = wi + wj;
+    if (wsum <= 0.f) return;
+    f3 dd = xyz(pi) - xyz(pj);
+    float len = len3(dd);
+    if (len < 1e-9f) return;
+    f3 n = dd * (1.f / len);
+    float C = len - restLen;
+    float gamma = betaGammaH;
+    // Without Kelvin damping gamma * gradDx is zero and prev is not read (only the sign of a zero could differ).
+    float gradDx = DAMPED ? dot3(n, (xyz(pi) - xyz(prev[i])) - (xyz(pj) - xyz(prev[j]))) : 0.f;
+    float lam = READ_LAMBDA ? lambda[e] : 0.f;
+    float dl = (-C - alphaTilde * lam - gamma * gradDx) / ((1.f + gamma) * wsum + alphaTilde);
+    float applied = omega * dl;
+    if (WRITE_LAMBDA) lambda[e] = lam + applied;
+    f3 xi = xyz(pi) + n * (wi * applied);
+    f3 xj = xyz(pj) - n * (wj * applied);
+    pos[i] = make_float4(xi.x, xi.y, xi.z, wi);
+    pos[j] = make_float4(xj.x, xj.y, xj.z, wj);
+}
+
+// k_solve_tet (xpbd_solver.cu:749-829) over one colour. MODE 0 keeps the hydrostatic row alone; 1 and 2 are the
+// coupled 2x2 block, legacy and guarded.
+template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED>
+__global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float dampingOverH, float bulkDampingOverH,
+                       const uint32_t* __restrict__ tet_desc, const float* __restrict__ alpha,
+                       float* __restrict__ lambda, float4* __restrict__ pos, const float4* __restrict__ prev) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    int e = off + k;
+    uint32_t d = tet_desc[e];
+    if (!(d & 1u)) return;
+    int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+    int id[4];
+    float4 q[4];
+    for (int c = 0; c < 4; c++) {
+        id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);
+        q[c] = pos[id[c]];
+    }
+    f3 p[4] = {xyz(q[0]), xyz(q[1]), xyz(q[2]), xyz(q[3])};
+    float w[4] = {q[0].w, q[1].w, q[2].w, q[3].w};
+    f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
+    const float* B = L.inv_rest[type];
+    f3 br[3] = {mk3(B[0], B[1], B[2]), mk3(B[3], B[4], B[5]), mk3(B[6], B[7], B[8])};
+    f3 F[3];
+    for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
+    f3 cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
+    float ah = alpha[T + e];
+    float J = dot3(F[0], cof[0]);
+    f3 gh[4], gd[4];
+    for (int c = 1; c < 4; c++) {
+        f3 b = br[c - 1];
+        gh[c] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
+    }
+    float scale = 1.f + dampingOverH;
+    float lamH = READ_LAMBDA ? lambda[T + e] : 0.f;
+    if (MODE == GTS_TET_VOLUME) {
+        float CH = J - 1.f;
+        gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
+        float bulkScale = scale;
+        float effectiveAh = ah / bulkScale;
+        float vh = 0.f;
+        float schur = effectiveAh;
+        if (DAMPED)
+            for (int c = 0; c < 4; c++) vh += dot3(gh[c], p[c] - xyz(prev[id[c]]));
+        for (int c = 0; c < 4; c++) schur += w[c] * dot3(gh[c], gh[c]);
+        float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
+        if (!(schur > 0.f)) return;
+        float dlH = omega * rh / schur;
+        if (!isfinite(dlH)) return;
+        if (WRITE_LAMBDA) lambda[T + e] = lamH + dlH;
+        for (int c = 0; c < 4; c++) {
+            f3 x = p[c] + gh[c] * dlH * w[c];
+            pos[id[c]] = make_float4(x.x, x.y, x.z, w[c]);
+        }
+        return;
+    }
+    const bool safeguarded = MODE == GTS_TET_GUARDED;
+    float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    float ad = alpha[e];
+    float positiveJ = fmaxf(J, 0.05f);
+    float iso = iso_pow(positiveJ);
+    float qq = fmaxf(iso * I1 - 3.f, 0.f);
+    float CD = sqrtf(fmaxf(qq, 1e-12f));
+    float devScale = 1.f / CD;
+    float volumeGradient = J > 0.05f ? iso * I1 / (3.f * positiveJ) : 0.f;
+    float CH = safeguarded ? logf(positiveJ) + fminf(0.f, (J - 0.05f) / 0.05f) : J - 1.f;
+    for (int c = 1; c < 4; c++) {
+        f3 b = br[c - 1];
+        gd[c] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[c] * volumeGradient) * devScale;
+        if (safeguarded) gh[c] = gh[c] * (1.f / positiveJ);
+    }
+    gd[0] = (gd[1] + gd[2] + gd[3]) * -1.f;
+    gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
+    float dd = 0.f, dh = 0.f, vd = 0.f, vh = 0.f;
+    for (int c = 0; c < 4; c++) {
+        dd += w[c] * dot3(gd[c], gd[c]);
+        dh += w[c] * dot3(gd[c], gh[c]);
+        if (DAMPED) {
+            f3 dx = p[c] - xyz(prev[id[c]]);
+            vd += dot3(gd[c], dx);
+            vh += dot3(gh[c], dx);
+        }
+    }
+    float bulkScale = safeguarded ? 1.f + bulkDampingOverH : scale;
+    float effectiveAd = ad / scale, effectiveAh = ah / bulkScale;
+    float A = dd + effectiveAd, Bc = dh, gamma = Bc / A;
+    float schur = effectiveAh + effectiveAd * gamma * gamma;
+    for (int c = 0; c < 4; c++) {
+        f3 residual = gh[c] - gd[c] * gamma;
+        schur += w[c] * dot3(residual, residual);
+    }
+    float lamD = READ_LAMBDA ? lambda[e] : 0.f;
+    float rd = (-CD - ad * lamD - dampingOverH * vd) / scale;
+    float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
+    if (!(A > 0.f) || !(schur > 0.f)) return;
+    float dlH = omega * (rh - gamma * rd) / schur, dlD = (omega * rd - Bc * dlH) / A;
+    if (!isfinite(dlD) || !isfinite(dlH)) return;
+    if (safeguarded) {
+        float largest = 0.f;
+        for (int c = 0; c < 4; c++) largest = fmaxf(largest, len3((gd[c] * dlD + gh[c] * dlH) * w[c]));
+        float limit = 0.15f / fmaxf(len3(br[0]), fmaxf(len3(br[1]), len3(br[2])));
+        float accepted = largest > limit ? limit / largest : 1.f;
+        dlD *= accepted;
+        dlH *= accepted;
+    }
+    if (WRITE_LAMBDA) {
+        lambda[e] = lamD + dlD;
+        lambda[T + e] = lamH + dlH;
+    }
+    for (int c = 0; c < 4; c++) {
+        f3 x = p[c] + (gd[c] * dlD + gh[c] * dlH) * w[c];
+        pos[id[c]] = make_float4(x.x, x.y, x.z, w[c]);
+    }
+}
+
+// Friction contact of each node with the tools (friction_contact.cuh:143-169, two solve passes as at
+// xpbd_solver.cu:3663-3665), then k_update_velocity and, unless perpendicular edge damping must run between them,
+// k_apply_velscale (xpbd_solver.cu:871-919). Nodes are independent here, so one pass does all of it.
+template <bool CONTACT, bool SCALE>
+__global__ void k_finish_nodes(int N, StepShapes shapes, uint32_t armed, float alpha, float muS, float muD,
+                               float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
+                               const float4* __restrict__ prev, float4* __restrict__ vel, float* reaction) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    f3 force = mk3(0, 0, 0);
+    float contacts = 0.f;
+    if (i < N) {
+        float4 p4 = pos[i];
+        float w = p4.w;
+        f3 x = xyz(p4);
+        f3 xp = xyz(prev[i]);
+        if (CONTACT && w > 0.f) {
+            ContactRecord rec[4];
+            float lamN[4];
+            f3 lamT[4];
+            for (int kind = 0; kind < 4; kind++) {
+                rec[kind].valid = 0;
+                lamN[kind] = 0.f;
+                lamT[kind] = mk3(0, 0, 0);
+                if (((armed >> kind) & 1u) && shape_present(shapes.to[kind]))
+                    rec[kind] = trace_tool_contact(shapes.from[kind], shapes.to[kind], radius, xp, x);
+            }
+            for (int pass = 0; pass < 2; pass++)
+                for (int kind = 0; kind < 4; kind++) {
+                    const ContactRecord& c = rec[kind];
+                    if (!c.valid) continue;
+                    float C = dot3(x - c.target, c.normal), old = lamN[kind];
+                    float next = fmaxf(0.f, old + (-C - alpha * old) / (w + alpha));
+                    x = x + c.normal * (w * (next - old));
+                    lamN[kind] = next;
+                    f3 slip = x - xp - c.motion;
+                    slip = slip - c.normal * dot3(slip, c.normal);
+                    f3 oldT = lamT[kind];
+                    f3 trial = oldT - slip * (1.f / w);
+                    float len = len3(trial);
+                    f3 nextT = trial;
+                    if (len > muS * next) nextT = len > 1e-15f ? trial * (muD * next / len) : mk3(0, 0, 0);
+                    x = x + (nextT - oldT) * w;
+                    lamT[kind] = nextT;
+                }
+            for (int kind = 0; kind < 4; kind++)
+                if (rec[kind].valid) {
+                    force = force + (rec[kind].normal * lamN[kind] + lamT[kind]) * (-scaleR);
+                    if (lamN[kind] > 0.f) contacts += 1.f;
+                }
+            pos[i] = make_float4(x.x, x.y, x.z, w);
+        }
+        f3 v = w > 0.f ? (x - xp) * invH : mk3(0, 0, 0);
+        if (SCALE) v = w > 0.f ? v * velScale : mk3(0, 0, 0);
+        vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+    if (CONTACT) {
+        // reaction sum: wave reduction, one atomic per wave
+        for (int o = 32; o > 0; o >>= 1) {
+            force.x += __shfl_xor(force.x, o);
+            force.y += __shfl_xor(force.y, o);
+            force.z += __shfl_xor(force.z, o);
+            contacts += __shfl_xor(contacts, o);
+        }
+        if ((threadIdx.x & 63) == 0 && (contacts > 0.f || force.x != 0.f || force.y != 0.f || force.z != 0.f)) {
+            atomicAdd(reaction + 0, force.x);
+            atomicAdd(reaction + 1, force.y);
+            atomicAdd(reaction + 2, force.z);
+            atomicAdd(reaction + 3, contacts);
+        }
+    }
+}
+
+// k_perp_damp_edges (xpbd_solver.cu:890-910) over one colour.
+__global__ void k_perp_damp(int off, int cnt, Lattice L, float cs, float h, const uint32_t* edge_desc,
+                            const float4* pos, float4* vel) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    uint32_t d = edge_desc[off + k];
+    if (!(d & 1u)) return;
+    int axis = (int)((d >> 1) & 3u), i = (int)(d >> 4), j = i + axis_stride(L, axis);
+    float4 pi = pos[i], pj = pos[j];
+    float wi = pi.w, wj = pj.w;
+    if (wi + wj <= 0.f) return;
+    f3 dd = xyz(pi) - xyz(pj);
+    float len = len3(dd);
+    if (len < 1e-9f) return;
+    f3 n = dd * (1.f / len);
+    f3 vi = xyz(vel[i]), vj = xyz(vel[j]);
+    f3 vrel = vi - vj;
+    f3 vp = vrel - n * dot3(vrel, n);
+    f3 imp = vp * (h * cs);
+    if (wi > 0.f) { vi = vi - imp * wi; vel[i] = make_float4(vi.x, vi.y, vi.z, 0.f); }
+    if (wj > 0.f) { vj = vj + imp * wj; vel[j] = make_float4(vj.x, vj.y, vj.z, 0.f); }
+}
+
+__global__ void k_velscale(int N, float velScale, const float4* pos, float4* vel) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    f3 v = pos[i].w > 0.f ? xyz(vel[i]) * velScale : mk3(0, 0, 0);
+    vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+}
+
+// ------------------------------------------------------------------------------------------------ cut kernels
+
+struct CutToolD {
+    f3 t1v0, t1v1, t1v2, t2v0, t2v1, t2v2, ncut, lo, hi;
+    float d;
+    int valid;
+};
+
+// EdgeStencil (cut.cuh:126-137): the four voxels around a lattice edge and the edge's local id in each.
+__device__ __forceinline__ void stencil(int axis, int slot, int& dx, int& dy, int& dz, int& local) {
+    const int8_t off[3][4][3] = {{{0, 0, 0}, {0, -1, 0}, {0, -1, -1}, {0, 0, -1}},
+                                 {{0, 0, 0}, {0, 0, -1}, {-1, 0, -1}, {-1, 0, 0}},
+                                 {{0, 0, 0}, {-1, 0, 0}, {-1, -1, 0}, {0, -1, 0}}};
+    const int8_t loc[3][4] = {{0, 2, 6, 4}, {3, 7, 5, 1}, {8, 9, 10, 11}};
+    dx = off[axis][slot][0]; dy = off[axis][slot][1]; dz = off[axis][slot][2];
+    local = loc[axis][slot];
+}
+
+__device__ __forceinline__ int directed_slot(int a, int b, int vx) {  // DirectedEdgeSlot (cut.cuh:25-28)
+    int dl = b - a, ad = dl < 0 ? -dl : dl;
+    int axis = ad == 1 ? 0 : (ad == vx + 1 ? 1 : 2);
+    return 6 * a + 2 * axis + (dl > 0 ? 1 : 0);
+}
+
+__device__ __forceinline__ bool cut_hits_material(float t, float skinT, int insideA, int insideB) {
+    if (insideA && insideB) return true;
+    if (!insideA && !insideB) return false;
+    if (!(skinT >= 0.f && skinT <= 1.f)) return false;
+    return insideA ? t <= skinT : t >= skinT;
+}
+
+__device__ __forceinline__ bool moller_trumbore(f3 O, f3 D, f3 V0, f3 V1, f3 V2, float& t, float& u, float& v) {
+    f3 L1 = V1 - V0, L2 = V2 - V0, Tv = V0 - O;
+    float det = dot3(cross3(L2, D), L1);
+    t = 0.f; u = 0.f; v = 0.f;
+    if (fabsf(det) < 1e-12f) return false;
+    float inv = 1.f / det;
+    t = dot3(cross3(L2, Tv), L1) * inv;
+    u = -dot3(cross3(L2, D), Tv) * inv;
+    v = dot3(cross3(D, Tv), L1) * inv;
+    const float EPS = 1e-5f;
+    return u >= -EPS && v >= -EPS && u + v <= 1.f + EPS && t > 0.f && t <= 1.f;
+}
+
+// MarkAndEmit (cut.cu:218-254) without the emission: sets the cut bits and the directed fractions, and records
+// (t, hit) for the ordered emission pass when the edge was cut for the first time inside the material.
+__device__ void mark_edge(int e, int axis, int bx, int by, int bz, int idA, int idB, float t, f3 hit, const Lattice& L,
+                          const uint32_t* occupied, uint32_t* cut_mask, float* edge_cut_t, const float* skin,
+                          const int* inside, float4* records) {
+    bool firstCut = false, sawAny = false, occ = false;
+    for (int s = 0; s < 4; s++) {
+        int dx, dy, dz, local;
+        stencil(axis, s, dx, dy, dz, local);
+        int x = bx + dx, y = by + dy, z = bz + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= L.vx || y >= L.vy || z >= L.vz) continue;
+        int vid = x + L.vx * (y + L.vy * z);
+        if (occupied[vid] == 0u) continue;
+        occ = true;
+        uint32_t bit = 1u << local;
+        uint32_t old = atomicOr(cut_mask + vid, bit);
+        if (!sawAny) { sawAny = true; firstCut = (old & bit) == 0u; }
+    }
+    if (!(firstCut && occ)) return;
+    edge_cut_t[directed_slot(idA, idB, L.vx)] = t;
+    edge_cut_t[directed_slot(idB, idA, L.vx)] = 1.f - t;
+    if (!cut_hits_material(t, skin[directed_slot(idA, idB, L.vx)], inside[idA], inside[idB])) return;
+    records[e] = make_float4(t, hit.x, hit.y, hit.z);
+}
+
+// k_DetectCutQuad (cut.cu:421-474), rod frozen: Möller–Trumbore of each deformed lattice edge against the swept quad.
+__global__ void k_cut_quad(int Eg, Lattice L, CutToolD tool, float pad, const int* __restrict__ grid_edges,
+                           const float4* __restrict__ pos, const uint32_t* occupied, uint32_t* cut_mask,
+                           float* edge_cut_t, const float* skin, const int* inside, float4* records) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Eg) return;
+    records[e] = make_float4(-1.f, 0.f, 0.f, 0.f);
+    if (tool.valid == 0) return;
+    int4 ge = reinterpret_cast<const int4*>(grid_edges)[e];
+    int axis = ge.x;
+    int idA = ge.y + L.nx * ge.z + L.nxy * ge.w;
+    int idB = idA + axis_stride(L, axis);
+    f3 O = xyz(pos[idA]);
+    f3 D = xyz(pos[idB]) - O;
+    f3 eMin = mk3(fminf(O.x, O.x + D.x), fminf(O.y, O.y + D.y), fminf(O.z, O.z + D.z));
+    f3 eMax = mk3(fmaxf(O.x, O.x + D.x), fmaxf(O.y, O.y + D.y), fmaxf(O.z, O.z + D.z));
+    if (eMax.x < tool.lo.x - pad || eMin.x > tool.hi.x + pad || eMax.y < tool.lo.y - pad || eMin.y > tool.hi.y + pad ||
+        eMax.z < tool.lo.z - pad || eMin.z > tool.hi.z + pad)
+        return;
+    float ta, ua, va, tb, ub, vb;
+    bool h1 = moller_trumbore(O, D, tool.t1v0, tool.t1v1, tool.t1v2, ta, ua, va);
+    bool h2 = moller_trumbore(O, D, tool.t2v0, tool.t2v1, tool.t2v2, tb, ub, vb);
+    if (!h1 && !h2) return;
+    float t = (h1 && h2) ? fminf(ta, tb) : (h1 ? ta : tb);
+    t = fminf(fmaxf(t, 0.f), 1.f);
+    f3 hit = O + D * t;
+    mark_edge(e, axis, ge.y, ge.z, ge.w, idA, idB, t, hit, L, occupied, cut_mask, edge_cut_t, skin, inside, records);
+}
+
+// k_DetectCutCCD (cut.cu:486-592), tissue moving from prev_positions to pos against the static blade S->E.
+__global__ void k_cut_ccd(int Eg, Lattice L, f3 S, f3 E, f3 emitN, float pad, const int* __restrict__ grid_edges,
+                          const int* nbr, const float* __restrict__ prevp, const float4* __restrict__ pos,
+                          const uint32_t* occupied, uint32_t* cut_mask, float* edge_cut_t, const float* skin,
+                          const int* inside, float4* records, float4* normals) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Eg) return;
+    records[e] = make_float4(-1.f, 0.f, 0.f, 0.f);
+    int4 ge = reinterpret_cast<const int4*>(grid_edges)[e];
+    int axis = ge.x;
+    int idA = ge.y + L.nx * ge.z + L.nxy * ge.w;
+    int idB = idA + axis_stride(L, axis);
+    if (nbr[6 * idA + 2 * axis + 1] < 0) return;
+    f3 a0 = ld3(prevp + 3 * idA), a1 = xyz(pos[idA]), b0 = ld3(prevp + 3 * idB), b1 = xyz(pos[idB]);
+    f3 sMin = mk3(fminf(S.x, E.x) - pad, fminf(S.y, E.y) - pad, fminf(S.z, E.z) - pad);
+    f3 sMax = mk3(fmaxf(S.x, E.x) + pad, fmaxf(S.y, E.y) + pad, fmaxf(S.z, E.z) + pad);
+    f3 eMin = mk3(fminf(fminf(a0.x, a1.x), fminf(b0.x, b1.x)), fminf(fminf(a0.y, a1.y), fminf(b0.y, b1.y)),
+                  fminf(fminf(a0.z, a1.z), fminf(b0.z, b1.z)));
+    f3 eMax = mk3(fmaxf(fmaxf(a0.x, a1.x), fmaxf(b0.x, b1.x)), fmaxf(fmaxf(a0.y, a1.y), fmaxf(b0.y, b1.y)),
+                  fmaxf(fmaxf(a0.z, a1.z), fmaxf(b0.z, b1.z)));
+    if (eMax.x < sMin.x || eMin.x > sMax.x || eMax.y < sMin.y || eMin.y > sMax.y || eMax.z < sMin.z ||
+        eMin.z > sMax.z)
+        return;
+    f3 D = E - S, P0 = a0 - S, Pv = a1 - a0, Q0 = b0 - S, Qv = b1 - b0;
+    float c0 = dot3(cross3(P0, Q0), D);
+    float c1 = dot3(cross3(P0, Qv) + cross3(Pv, Q0), D);
+    float c2 = dot3(cross3(Pv, Qv), D);
+    float roots[2];
+    int nr = 0;
+    if (fabsf(c2) < 1e-14f) {
+        if (fabsf(c1) > 1e-14f) {
+            float r = -c0 / c1;
+            if (r >= 0.f && r <= 1.f) roots[nr++] = r;
+        }
+    } else {
+        float disc = c1 * c1 - 4.f * c2 * c0;
+        if (disc >= 0.f) {
+            float sq = sqrtf(disc);
+            float q = -0.5f * (c1 + (c1 >= 0.f ? sq : -sq));
+            float r0 = q / c2, r1 = fabsf(q) > 1e-20f ? c0 / q : r0;
+            if (r0 > r1) { float tmp = r0; r0 = r1; r1 = tmp; }
+            if (r0 >= 0.f && r0 <= 1.f) roots[nr++] = r0;
+            if (r1 >= 0.f && r1 <= 1.f && r1 != r0) roots[nr++] = r1;
+        }
+    }
+    for (int ri = 0; ri < nr; ri++) {
+        float tc = roots[ri];
+        f3 a = a0 + Pv * tc, b = b0 + Qv * tc, e1 = b - a, n2 = cross3(e1, D);
+        float n2l = dot3(n2, n2);
+        if (n2l < 1e-16f) continue;
+        f3 w = S - a;
+        float u = dot3(cross3(w, D), n2) / n2l;
+        float v = dot3(cross3(w, e1), n2) / n2l;
+        const float EPS = 2e-2f;
+        if (u <= 0.f || u > 1.f || v < -EPS || v > 1.f + EPS) continue;
+        f3 hit = a1 + (b1 - a1) * u;
+        f3 relV = (Pv + Qv) * 0.5f;
+        f3 nHit = cross3(D, relV);
+        float nLen = len3(nHit);
+        f3 nEmit = nLen > 1e-8f ? nHit * (1.f / nLen) : emitN;
+        normals[e] = make_float4(nEmit.x, nEmit.y, nEmit.z, 0.f);
+        mark_edge(e, axis, ge.y, ge.z, ge.w, idA, idB, u, hit, L, occupied, cut_mask, edge_cut_t, skin, inside,
+                  records);
+        break;
+    }
+}
+
+// ---- ordered stable compaction: wave ballot + LDS prefix inside a block, exclusive scan of the block totals
+
+__device__ __forceinline__ int block_prefix(bool flag, int* lds, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint64_t m = __ballot(flag);
+    int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) lds[wave] = __popcll(m);
+    __syncthreads();
+    int waveBase = 0;
+    total = 0;
+    for (int k = 0; k < nw; k++) {
+        int c = lds[k];
+        if (k < wave) waveBase += c;
+        total += c;
+    }
+    __syncthreads();
+    return waveBase + before;
+}
+
+struct FlagRecords {
+    const float4* r;
+    __device__ bool operator()(int i) const { return r[i].x >= 0.f; }
+};
+struct FlagDescLive {
+    const uint32_t* d;
+    __device__ bool operator()(int i) const { return (d[i] & 1u) != 0u; }
+};
+struct FlagNodeUsed {
+    const int* used;
+    const int* active;
+    __device__ bool operator()(int i) const { return used[i] != 0 && active[i] != 0; }
+};
+
+template <class Flag>
+__global__ void k_count_flags(int n, Flag flag, int* partial) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int total;
+    block_prefix(i < n && flag(i), lds, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Exclusive scan of the per-block totals in one workgroup; partial[nblocks] receives the grand total.
+__global__ void k_scan_partials(int nblocks, int* partial) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    int carry = 0;
+    for (int start = 0; start < nblocks; start += blockDim.x) {
+        int i = start + threadIdx.x;
+        int v = i < nblocks ? partial[i] : 0;
+        // inclusive scan in the wave
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        int x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) lds[wave] = x;
+        __syncthreads();
+        int waveBase = 0, total = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+            if (k < wave) waveBase += lds[k];
+            total += lds[k];
+        }
+        if (i < nblocks) partial[i] = carry + waveBase + x - v;
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[nblocks] = carry;
+}
+
+template <class Flag>
+__global__ void k_write_remap(int n, Flag flag, const int* partial, int* remap) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool f = i < n && flag(i);
+    int total;
+    int r = block_prefix(f, lds, total);
+    if (i < n) remap[i] = f ? partial[blockIdx.x] + r : -1;
+}
+
+// EmitCutPoint (cut.cu:186-215) for the edges marked by the detection, in lattice edge order: slot = counter +
+// 2 * (rank of the edge among the emitting ones) for the owner A and + 1 for B; the particle rotation is identity.
+__global__ void k_emit_points(int Eg, Lattice L, f3 ncutQuad, int useNormals, float toolD, float voxelL,
+                              const int* __restrict__ grid_edges, const float4* __restrict__ pos,
+                              const float4* records, const float4* normals, const int* partial, uint32_t* counters,
+                              float* points, int capacity) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    bool f = e < Eg && records[e].x >= 0.f;
+    int total;
+    int r = block_prefix(f, lds, total);
+    uint32_t base = counters[1];
+    if (f) {
+        float4 rec = records[e];
+        int4 ge = reinterpret_cast<const int4*>(grid_edges)[e];
+        int axis = ge.x;
+        int idA = ge.y + L.nx * ge.z + L.nxy * ge.w;
+        int idB = idA + axis_stride(L, axis);
+        float t = rec.x;
+        f3 hit = mk3(rec.y, rec.z, rec.w);
+        f3 ncut = ncutQuad;
+        if (useNormals) { float4 nn = normals[e]; ncut = mk3(nn.x, nn.y, nn.z); }
+        f3 axisF = mk3(axis == 0 ? 1.f : 0.f, axis == 1 ? 1.f : 0.f, axis == 2 ? 1.f : 0.f);
+        f3 restEdge = axisF * voxelL;
+        f3 along[2] = {restEdge * t, restEdge * (t - 1.f)};
+        int owners[2] = {idA, idB};
+        uint32_t slot0 = base + 2u * (uint32_t)(partial[blockIdx.x] + r);
+        for (int k = 0; k < 2; k++) {
+            int p = owners[k];
+            float side = dot3(xyz(pos[p]) - hit, ncut);
+            side = side > 0.f ? 1.f : (side < 0.f ? -1.f : 0.f);
+            if (side == 0.f) side = 1.f;
+            uint32_t slot = slot0 + (uint32_t)k;
+            if (slot >= (uint32_t)capacity) continue;
+            f3 gap = ncut * (side * (toolD * 0.5f));
+            f3 off = along[k] + gap;
+            f3 nrm = ncut * side;
+            float* cp = points + 8 * (size_t)slot;
+            cp[0] = off.x; cp[1] = off.y; cp[2] = off.z;
+            cp[3] = __int_as_float(p);
+            cp[4] = __int_as_float(e);
+            cp[5] = nrm.x; cp[6] = nrm.y; cp[7] = nrm.z;
+        }
+    }
+}
+
+__global__ void k_bump_counter(const int* partial, int nblocks, uint32_t* counters) {
+    counters[1] += 2u * (uint32_t)partial[nblocks];
+}
+
+// k_SeverLinks / SeverLink (cut.cu:719-746): part the links whose edge bit is set in any stencil voxel, kill the bend
+// pairs that cross them, count the intact links severed.
+__global__ void k_sever(int Eg, Lattice L, const int* __restrict__ grid_edges, const uint32_t* cut_mask, int* nbr,
+                        uint32_t* bend_alive, uint32_t* counters) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Eg) return;
+    int4 ge = reinterpret_cast<const int4*>(grid_edges)[e];
+    int axis = ge.x;
+    bool valid = true, cut = false;
+    for (int s = 0; s < 4; s++) {
+        int dx, dy, dz, local;
+        stencil(axis, s, dx, dy, dz, local);
+        int x = ge.y + dx, y = ge.z + dy, z = ge.w + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= L.vx || y >= L.vy || z >= L.vz) { valid = false; continue; }
+        if ((cut_mask[x + L.vx * (y + L.vy * z)] >> local) & 1u) cut = true;
+    }
+    if (!valid || !cut) return;
+    int a = ge.y + L.nx * ge.z + L.nxy * ge.w;
+    int b = a + axis_stride(L, axis);
+    int sa = 2 * axis + 1, sb = 2 * axis;
+    if (nbr[6 * a + sa] >= 0) atomicAdd(counters, 1u);
+    nbr[6 * a + sa] = -1;
+    nbr[6 * b + sb] = -1;
+    // pair mask of all bend pairs of a node that use slot s
+    uint32_t ma = 0u, mb = 0u;
+    int pair = 0;
+    for (int p = 0; p < 6; p++)
+        for (int q = p + 1; q < 6; q++, pair++) {
+            if (p == sa || q == sa) ma |= 1u << pair;
+            if (p == sb || q == sb) mb |= 1u << pair;
+        }
+    atomicAnd(bend_alive + a, ~ma);
+    atomicAnd(bend_alive + b, ~mb);
+}
+
+__global__ void k_fill_int(int n, int* x, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+__global__ void k_fill_float(int n, float* x, float v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+__global__ void k_mark_used(int T, int E, Lattice L, const uint32_t* tet_desc, const uint32_t* edge_desc, int* used) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < T) {
+        uint32_t d = tet_desc[i];
+        if (d & 1u) {
+            int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+            for (int c = 0; c < 4; c++) used[base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy)] = 1;
+        }
+    }
+    if (i < E) {
+        uint32_t d = edge_desc[i];
+        if (d & 1u) {
+            int a = (int)(d >> 4), axis = (int)((d >> 1) & 3u);
+            used[a] = 1;
+            used[a + axis_stride(L, axis)] = 1;
+        }
+    }
+}
+
+__global__ void k_copy_counts(const int* pt, int nt, const int* pe, int ne, const int* pn, int nn, int* counts) {
+    counts[0] = pt[nt];
+    counts[1] = pe[ne];
+    counts[2] = pn[nn];
+}
+
+// ------------------------------------------------------------------------------------------------ host helpers
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+ToolPose pose_of(const GtsToolState* t) {
+    ToolPose p;
+    p.shaftA = ld3(t->shaftA); p.shaftB = ld3(t->shaftB); p.shaftR = t->shaftR;
+    p.jawHinge = ld3(t->jawHinge); p.jawAxis = ld3(t->jawAxis); p.jawMid = ld3(t->jawMid);
+    p.jawNormal = ld3(t->jawNormal);
+    p.jawHalfWidth = t->jawHalfWidth; p.jawLen = t->jawLen; p.jawGap = t->jawGap; p.grasp01 = t->grasp01;
+    p.bladeA = ld3(t->bladeA); p.bladeB = ld3(t->bladeB); p.bladeR = t->bladeR;
+    return p;
+}
+
+int scan_blocks(int n) { return grid_for(n); }
+
+// scratch layout in floats: [4*Eg) detection records, [4*Eg, 8*Eg) CCD normals, then int partials
+int* partial_area(const GtsTissueView* v) { return reinterpret_cast<int*>(v->scratch + 8 * (size_t)v->n_grid_edges); }
+
+int launch_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
+    k_topology_stale<<<1, 1, 0, s>>>(v->counters);
+    k_clear_support<<<grid_for(6 * v->n_nodes), GTS_BLOCK, 0, s>>>(6 * v->n_nodes, v->slot_support, v->counters);
+    if (v->n_tets > 0)
+        k_refresh_tets<<<grid_for(v->n_tets), GTS_BLOCK, 0, s>>>(v->n_tets, L, v->nbr, v->active, v->tet_desc,
+                                                                  v->slot_support, v->counters);
+    if (v->n_edges > 0)
+        k_refresh_edges<<<grid_for(v->n_edges), GTS_BLOCK, 0, s>>>(v->n_edges, L, v->nbr, v->active, v->slot_support,
+                                                                    v->edge_desc, v->counters);
+    return -(int)hipGetLastError();
+}
+
+template <int MODE, bool DAMPED>
+void launch_tet_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
+                         float omega, float dOverH, float bOverH) {
+    dim3 g(grid_for(cnt)), b(GTS_BLOCK);
+    const bool read = it > 0, write = iters > 1;
+#define GTS_TET(R, W)                                                                                            \
+    k_tets<MODE, R, W, DAMPED><<<g, b, 0, s>>>(off, cnt, v->n_tets, L, omega, dOverH, bOverH, v->tet_desc,      \
+                                               v->tet_alpha, v->tet_lambda, (float4*)v->pos, (const float4*)v->prev)
+    if (read && write) GTS_TET(true, true);
+    else if (read) GTS_TET(true, false);
+    else if (write) GTS_TET(false, true);
+    else GTS_TET(false, false);
+#undef GTS_TET
+}
+
+template <int MODE>
+void launch_tet_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
+                       float omega, float dOverH, float bOverH) {
+    if (dOverH != 0.f || bOverH != 0.f) launch_tet_colour_d<MODE, true>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
+    else launch_tet_colour_d<MODE, false>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
+}
+
+template <bool DAMPED>
+void launch_edge_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
+                          bool fallback, float at, float fb, float omega, float bgh) {
+    dim3 g(grid_for(cnt)), b(GTS_BLOCK);
+    const bool read = it > 0, write = iters > 1;
+#define GTS_EDGE(R, W, FB)                                                                                     \
+    k_edges<R, W, FB, DAMPED><<<g, b, 0, s>>>(off, cnt, L, at, fb, omega, bgh, v->edge_desc, v->edge_lambda,  \
+                                              (float4*)v->pos, (const float4*)v->prev)
+    if (fallback) {
+        if (read && write) GTS_EDGE(true, true, true);
+        else if (read) GTS_EDGE(true, false, true);
+        else if (write) GTS_EDGE(false, true, true);
+        else GTS_EDGE(false, false, true);
+    } else {
+        if (read && write) GTS_EDGE(true, true, false);
+        else if (read) GTS_EDGE(true, false, false);
+        else if (write) GTS_EDGE(false, true, false);
+        else GTS_EDGE(false, false, false);
+    }
+#undef GTS_EDGE
+}
+
+void launch_edge_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
+                        bool fallback, float at, float fb, float omega, float bgh) {
+    if (bgh != 0.f) launch_edge_colour_d<true>(v, L, s, off, cnt, it, iters, fallback, at, fb, omega, bgh);
+    else launch_edge_colour_d<false>(v, L, s, off, cnt, it, iters, fallback, at, fb, omega, bgh);
+}
+
+int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut, int useNormals, float toolD) {
+    int Eg = v->n_grid_edges;
+    int nb = scan_blocks(Eg);
+    int* partial = partial_area(v);
+    float4* records = reinterpret_cast<float4*>(v->scratch);
+    float4* normals = records + Eg;
+    k_count_flags<<<nb, GTS_BLOCK, 0, s>>>(Eg, FlagRecords{records}, partial);
+    k_scan_partials<<<1, GTS_BLOCK, 0, s>>>(nb, partial);
+    k_emit_points<<<nb, GTS_BLOCK, 0, s>>>(Eg, L, ncut, useNormals, toolD, v->voxel_length, v->grid_edges,
+                                           (const float4*)v->pos,
+                                           records, normals, partial, v->counters, v->cut_points,
+                                           v->cut_point_capacity);
+    k_bump_counter<<<1, 1, 0, s>>>(partial, nb, v->counters);
+    k_sever<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, v->grid_edges, v->cut_mask, v->nbr, v->bend_alive, v->counters);
+    return launch_refresh(v, L, s);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------ C entry points
+
+extern "C" int gts_init_state(const GtsTissueView* v, void* stream) {
+    if (!v || v->n_nodes <= 0) return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    k_init_state<<<grid_for(v->n_nodes), GTS_BLOCK, 0, s>>>(v->n_nodes, v->mass, v->active, v->pinned, v->nbr,
+                                                            (float4*)v->pos, v->bend_alive, v->counters);
+    HIP_TRY(hipGetLastError());
+    Lattice L = lattice_of(v);
+    return launch_refresh(v, L, s);
+}
+
+extern "C" int gts_set_gravity(const GtsTissueView* v, float gx, float gy, float gz, void* stream) {
+    if (!v || !std::isfinite(gx) || !std::isfinite(gy) || !std::isfinite(gz)) return GTS_ERR_BAD_ARG;
+    k_set_gravity<<<grid_for(v->n_nodes), GTS_BLOCK, 0, S(stream)>>>(v->n_nodes, v->mass, gx, gy, gz,
+                                                                       (float4*)v->force);
+    return -(int)hipGetLastError();
+}
+
+extern "C" int gts_refresh_liveness(const GtsTissueView* v, void* stream) {
+    if (!v) return GTS_ERR_BAD_ARG;
+    Lattice L = lattice_of(v);
+    return launch_refresh(v, L, S(stream));
+}
+
+extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from,
+                        const GtsToolState* to, int tool_armed, int blade_armed, void* stream) {
+    if (!v || !p || !from || !to) return GTS_ERR_BAD_ARG;
+    if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Lattice L = lattice_of(v);
+    // xpbd_begin_frame (xpbd_solver.cu:2481-2514)
+    const int Nsub = p->substeps > 0 ? p->substeps : 1;
+    const float h = p->dt / (float)Nsub;
+    if (!(h > 0.f) || !std::isfinite(h)) return GTS_ERR_BAD_DT;
+    const float ks = p->edge_ks;
+    const float at = ks > 0.f ? 1.f / (ks * h * h) : 0.f;
+    const float velScale = (1.f - p->damping) / (1.f + p->rayleigh * h);
+    const float betaGammaH = at * p->beta_s * h;
+    const float invH2 = 1.f / (h * h);
+    const int T = v->n_tets, N = v->n_nodes;
+    { int rc_ = launch_refresh(v, L, s); if (rc_) return rc_; }
+    if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
+    const bool strain = T > 0;
+    const bool fallback = strain && ks <= 0.f;
+    const bool edges = ks > 0.f || fallback;
+    const float fb = fallback ? 1.f / (p->young * h * h) : 0.f;
+    const float dOverH = fmaxf(0.f, p->strain_damping) / h;
+    const float bOverH = h > 0.f ? p->bulk_damping / h : 0.f;
+    const int iters = p->iterations > 0 ? p->iterations : 1;
+    const ToolPose pf = pose_of(from), pt = pose_of(to);
+    const bool contact = p->contact_enabled && (tool_armed || blade_armed);
+    const uint32_t armed = (tool_armed ? 7u : 0u) | (blade_armed ? 8u : 0u);
+    const float alphaC = p->contact_compliance / (h * h);
+    const float scaleR = 1.f / (h * h * fmaxf(1.f, (float)Nsub));
+    const float invH = 1.f / h;
+    if (contact) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
+    for (int n = 0; n < Nsub; n++) {
+        // tool pose of this substep and of the previous one (xpbd_solver.cu:3467-3483)
+        float t = fminf(1.f, (float)(n + 1) / (float)Nsub);
+        float previousT = fmaxf(0.f, t - 1.f / (float)Nsub);
+        StepShapes shapes;
+        if (contact) {
+            ToolPose prevTool = lerp_tool(pf, pt, previousT), stepTool = lerp_tool(pf, pt, t);
+            for (int kind = 0; kind < 4; kind++) {
+                shapes.from[kind] = tool_shape(prevTool, kind, p->jaw_thickness, p->jaw_gap_closed);
+                shapes.to[kind] = tool_shape(stepTool, kind, p->jaw_thickness, p->jaw_gap_closed);
+            }
+        } else {
+            std::memset(&shapes, 0, sizeof(shapes));
+        }
+        k_predict<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel,
+                                                    (const float4*)v->force);
+        for (int it = 0; it < iters; it++) {
+            if (edges)
+                for (int c = 0; c < v->n_edge_colours; c++) {
+                    int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
+                    if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, iters, fallback, at, fb, p->omega, betaGammaH);
+                }
+            if (strain)
+                for (int c = 0; c < v->n_tet_colours; c++) {
+                    int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
+                    if (cnt <= 0) continue;
+                    if (p->tet_mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, iters, p->omega, dOverH, bOverH);
+                    else if (p->tet_mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, iters, p->omega, dOverH, bOverH);
+                    else launch_tet_colour<2>(v, L, s, off, cnt, it, iters, p->omega, dOverH, bOverH);
+                }
+        }
+        const bool perp = p->beta_s > 0.f;
+        dim3 g(grid_for(N)), b(GTS_BLOCK);
+        if (contact) {
+            if (perp)
+                k_finish_nodes<true, false><<<g, b, 0, s>>>(N, shapes, armed, alphaC, p->mu_s, p->mu_d,
+                                                            p->contact_radius, scaleR, invH, velScale, (float4*)v->pos,
+                                                            (const float4*)v->prev, (float4*)v->vel, v->reaction);
+            else
+                k_finish_nodes<true, true><<<g, b, 0, s>>>(N, shapes, armed, alphaC, p->mu_s, p->mu_d,
+                                                           p->contact_radius, scaleR, invH, velScale, (float4*)v->pos,
+                                                           (const float4*)v->prev, (float4*)v->vel, v->reaction);
+        } else {
+            if (perp)
+                k_finish_nodes<false, false><<<g, b, 0, s>>>(N, shapes, 0u, 0.f, 0.f, 0.f, 0.f, 0.f, invH, velScale,
+                                                             (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,
+                                                             v->reaction);
+            else
+                k_finish_nodes<false, true><<<g, b, 0, s>>>(N, shapes, 0u, 0.f, 0.f, 0.f, 0.f, 0.f, invH, velScale,
+                                                            (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,
+                                                            v->reaction);
+        }
+        if (perp) {
+            for (int c = 0; c < v->n_edge_colours; c++) {
+                int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
+                if (cnt > 0)
+                    k_perp_damp<<<grid_for(cnt), GTS_BLOCK, 0, s>>>(off, cnt, L, p->beta_s, h, v->edge_desc,
+                                                                    (const float4*)v->pos, (float4*)v->vel);
+            }
+            k_velscale<<<g, b, 0, s>>>(N, velScale, (const float4*)v->pos, (float4*)v->vel);
+        }
+    }
+    return -(int)hipGetLastError();
+}
+
+extern "C" int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream) {
+    if (!v || !tool) return GTS_ERR_BAD_ARG;
+    if (v->n_grid_edges <= 0) return GTS_OK;
+    hipStream_t s = S(stream);
+    Lattice L = lattice_of(v);
+    CutToolD t;
+    t.t1v0 = ld3(tool->t1v0); t.t1v1 = ld3(tool->t1v1); t.t1v2 = ld3(tool->t1v2);
+    t.t2v0 = ld3(tool->t2v0); t.t2v1 = ld3(tool->t2v1); t.t2v2 = ld3(tool->t2v2);
+    t.ncut = ld3(tool->ncut); t.lo = ld3(tool->aabbMin); t.hi = ld3(tool->aabbMax);
+    t.d = tool->d; t.valid = tool->valid;
+    int Eg = v->n_grid_edges;
+    float4* records = reinterpret_cast<float4*>(v->scratch);
+    k_cut_quad<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, t, v->voxel_length, v->grid_edges, (const float4*)v->pos,
+                                                  v->occupied, v->cut_mask, v->edge_cut_t, v->edge_skin_t, v->inside,
+                                                  records);
+    HIP_TRY(hipGetLastError());
+    return finish_cut(v, L, s, t.ncut, 0, t.d);
+}
+
+extern "C" int gts_detect_cut_ccd(const GtsTissueView* v, const float* Sb, const float* Eb, const float* emit_ncut,
+                                  float tool_d, const float* prev_positions, void* stream) {
+    if (!v || !Sb || !Eb || !emit_ncut || !prev_positions) return GTS_ERR_BAD_ARG;
+    if (v->n_grid_edges <= 0) return GTS_OK;
+    hipStream_t s = S(stream);
+    Lattice L = lattice_of(v);
+    int Eg = v->n_grid_edges;
+    float4* records = reinterpret_cast<float4*>(v->scratch);
+    float4* normals = records + Eg;
+    k_cut_ccd<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, ld3(Sb), ld3(Eb), ld3(emit_ncut), v->voxel_length,
+                                                 v->grid_edges, v->nbr, prev_positions, (const float4*)v->pos,
+                                                 v->occupied, v->cut_mask, v->edge_cut_t, v->edge_skin_t, v->inside,
+                                                 records, normals);
+    HIP_TRY(hipGetLastError());
+    return finish_cut(v, L, s, ld3(emit_ncut), 1, tool_d);
+}
+
+extern "C" int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap,
+                                    int32_t* node_remap, int32_t* counts, int32_t* node_used, void* stream) {
+    if (!v || !tet_remap || !edge_remap || !node_remap || !counts || !node_used) return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Lattice L = lattice_of(v);
+    { int rc_ = launch_refresh(v, L, s); if (rc_) return rc_; }
+    int T = v->n_tets, E = v->n_edges, N = v->n_nodes;
+    int* partial = partial_area(v);
+    int nbT = scan_blocks(T), nbE = scan_blocks(E), nbN = scan_blocks(N);
+    int* pT = partial;
+    int* pE = pT + nbT + 1;
+    int* pN = pE + nbE + 1;
+    k_fill_int<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, node_used, 0);
+    k_mark_used<<<grid_for(std::max(T, E)), GTS_BLOCK, 0, s>>>(T, E, L, v->tet_desc, v->edge_desc, node_used);
+    k_count_flags<<<nbT, GTS_BLOCK, 0, s>>>(T, FlagDescLive{v->tet_desc}, pT);
+    k_scan_partials<<<1, GTS_BLOCK, 0, s>>>(nbT, pT);
+    k_write_remap<<<nbT, GTS_BLOCK, 0, s>>>(T, FlagDescLive{v->tet_desc}, pT, tet_remap);
+    k_count_flags<<<nbE, GTS_BLOCK, 0, s>>>(E, FlagDescLive{v->edge_desc}, pE);
+    k_scan_partials<<<1, GTS_BLOCK, 0, s>>>(nbE, pE);
+    k_write_remap<<<nbE, GTS_BLOCK, 0, s>>>(E, FlagDescLive{v->edge_desc}, pE, edge_remap);
+    k_count_flags<<<nbN, GTS_BLOCK, 0, s>>>(N, FlagNodeUsed{node_used, v->active}, pN);
+    k_scan_partials<<<1, GTS_BLOCK, 0, s>>>(nbN, pN);
+    k_write_remap<<<nbN, GTS_BLOCK, 0, s>>>(N, FlagNodeUsed{node_used, v->active}, pN, node_remap);
+    k_copy_counts<<<1, 1, 0, s>>>(pT, nbT, pE, nbE, pN, nbN, counts);
+    return -(int)hipGetLastError();
+}
+
+extern "C" float gts_iso_pow(float J) { return iso_pow(J); }

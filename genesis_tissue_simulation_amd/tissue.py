@@ -1,0 +1,263 @@
+"""One tissue instance on the GPU: device buffers as PyTorch-ROCm tensors, computation in libgts_xpbd.so.
+
+`Tissue` is the host-side owner of everything the reference keeps in its physics / xpbd / cut modules for one
+session (`physics.cu:214-315`, `xpbd/xpbd_solver.cu:1412-1848`, `cut.cu:1105-1206`), but without process-global
+state: several instances can live side by side.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import GtsCutTool, GtsLattice, GtsStepParams, GtsTissueView, GtsToolState, GtsTopology, check, load, v3
+from .params import SolverParams
+
+TOOL_FIELDS3 = ("shaftA", "shaftB", "jawHinge", "jawAxis", "jawMid", "jawNormal", "bladeA", "bladeB")
+TOOL_FIELDS1 = ("shaftR", "jawHalfWidth", "jawLen", "jawGap", "grasp01", "bladeR")
+
+
+def tool_state(tool):
+    """GtsToolState from a dict of XpbdToolState fields (missing ones are zero)."""
+    t = GtsToolState()
+    tool = tool or {}
+    for name in TOOL_FIELDS3:
+        setattr(t, name, v3(tool.get(name, (0, 0, 0))))
+    for name in TOOL_FIELDS1:
+        setattr(t, name, float(tool.get(name, 0.0)))
+    return t
+
+
+def swept_cut_tool(prev_segment, segment, thickness):
+    """Swept plane of a straight blade moving from `prev_segment` (Sprev, Eprev) to `segment` (S, E), as the cut
+    descriptor CutToolDesc: T1 = (Sprev, Eprev, E), T2 = (Sprev, E, S), unit normal and AABB (`cut.h:25-38`)."""
+    sp, ep = (np.asarray(p, dtype=np.float32) for p in prev_segment)
+    s, e = (np.asarray(p, dtype=np.float32) for p in segment)
+    n = np.cross(ep - sp, s - sp).astype(np.float32)
+    length = float(np.linalg.norm(n))
+    valid = length > 1e-8
+    n = (n / length).astype(np.float32) if valid else np.zeros(3, dtype=np.float32)
+    pts = np.stack((sp, ep, e, s))
+    return dict(t1v0=sp, t1v1=ep, t1v2=e, t2v0=sp, t2v1=e, t2v2=s, ncut=n, d=float(thickness), valid=int(valid),
+                aabbMin=pts.min(axis=0), aabbMax=pts.max(axis=0))
+
+
+def cut_tool_struct(tool):
+    t = GtsCutTool()
+    for name in ("t1v0", "t1v1", "t1v2", "t2v0", "t2v1", "t2v2", "ncut", "aabbMin", "aabbMax"):
+        setattr(t, name, v3(tool[name]))
+    t.d = float(tool["d"])
+    t.valid = int(tool["valid"])
+    return t
+
+
+def step_params(params: SolverParams, dt):
+    p = params
+    return GtsStepParams(float(dt), int(p.substeps), int(p.iterations), int(p.tet_mode), float(p.omega),
+                         float(p.edge_ks), float(p.young), float(p.beta_s), float(p.strain_damping),
+                         float(p.bulk_damping), float(p.rayleigh), float(p.damping), int(bool(p.contact_enabled)),
+                         float(p.mu_s), float(p.mu_d), float(p.jaw_thickness), float(p.contact_radius),
+                         float(p.jaw_gap_closed), float(p.contact_compliance))
+
+
+def build_topology(grid):
+    """Host topology of a lattice via gts_build_topology; arrays in device order plus per-type rest data."""
+    lib = load()
+    nx, ny, nz = (d + 1 for d in grid.dims)
+    N = nx * ny * nz
+    V = int(np.prod(grid.dims))
+    rest = np.ascontiguousarray(grid.positions, dtype=np.float32)
+    active = np.ascontiguousarray(grid.active, dtype=np.int32)
+    nbr = np.ascontiguousarray(grid.neighbors, dtype=np.int32)
+    rest_nbr = np.ascontiguousarray(grid.rest_neighbors, dtype=np.float32)
+    lat = GtsLattice(*grid.dims, rest.ctypes.data, active.ctypes.data, nbr.ctypes.data, rest_nbr.ctypes.data)
+    edge_desc = np.zeros(3 * N, dtype=np.uint32)
+    edge_gen = np.zeros(3 * N, dtype=np.int32)
+    eoff = np.zeros(_lib.GTS_MAX_COLOURS + 1, dtype=np.int32)
+    tet_desc = np.zeros(6 * V, dtype=np.uint32)
+    tet_gen = np.zeros(6 * V, dtype=np.int32)
+    toff = np.zeros(_lib.GTS_MAX_COLOURS + 1, dtype=np.int32)
+    topo = GtsTopology(edge_desc.ctypes.data, edge_gen.ctypes.data, eoff.ctypes.data, tet_desc.ctypes.data,
+                       tet_gen.ctypes.data, toff.ctypes.data)
+    check("gts_build_topology", lib.gts_build_topology(ct.byref(lat), ct.byref(topo)))
+    E, T = topo.n_edges, topo.n_tets
+    return dict(edge_desc=edge_desc[:E].copy(), edge_gen=edge_gen[:E].copy(),
+                edge_offsets=eoff[:topo.n_edge_colours + 1].copy(), tet_desc=tet_desc[:T].copy(),
+                tet_gen=tet_gen[:T].copy(), tet_offsets=toff[:topo.n_tet_colours + 1].copy(),
+                type_inv_rest=np.array(topo.type_inv_rest, dtype=np.float64).reshape(6, 3, 3),
+                type_volume=np.array(topo.type_volume, dtype=np.float64),
+                axis_rest_length=np.array(topo.axis_rest_length, dtype=np.float32))
+
+
+def _ptr(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+def current_stream():
+    return ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Tissue:
+    """Device state of one tissue and the HIP entry points that advance and cut it."""
+
+    def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
+                 device="cuda:0", topology=None):
+        self.lib = load()
+        if not torch.cuda.is_available():
+            raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
+        self.grid, self.params, self.device = grid, params, torch.device(device)
+        topo = topology or build_topology(grid)
+        self.topology = topo
+        N = len(grid.positions)
+        V = int(np.prod(grid.dims))
+        E, T = len(topo["edge_desc"]), len(topo["tet_desc"])
+        Eg = len(grid.edges)
+        self.N, self.E, self.T, self.Eg, self.V = N, E, T, Eg, V
+        dev = self.device
+
+        def f32(a, shape=None):
+            x = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32))
+            return (x.reshape(shape) if shape else x).to(dev)
+
+        def i32(a):
+            return torch.as_tensor(np.ascontiguousarray(np.asarray(a).astype(np.int64).astype(np.int32)
+                                                        if np.asarray(a).dtype != np.uint32 else
+                                                        np.asarray(a).view(np.int32))).to(dev)
+
+        init = grid.positions if initial_positions is None else initial_positions
+        pos = np.zeros((N, 4), dtype=np.float32)
+        pos[:, :3] = init
+        self.pos = f32(pos)
+        self.prev = self.pos.clone()
+        self.vel = torch.zeros((N, 4), dtype=torch.float32, device=dev)
+        self.force = torch.zeros((N, 4), dtype=torch.float32, device=dev)
+        self.mass = f32(grid.mass)
+        self.active = i32(grid.active)
+        self.pinned = i32(grid.pinned)
+        self.inside = i32(grid.inside)
+        self.nbr = i32(grid.neighbors.reshape(-1))
+        self.bend_alive = torch.zeros(N, dtype=torch.int32, device=dev)
+        self.edge_desc = i32(topo["edge_desc"])
+        self.edge_lambda = torch.zeros(max(E, 1), dtype=torch.float32, device=dev)
+        self.slot_support = torch.zeros(6 * N, dtype=torch.int32, device=dev)
+        self.tet_desc = i32(topo["tet_desc"])
+        gen = topo["tet_gen"]
+        k_dev = np.ones(T, dtype=np.float32) if k_dev is None else np.asarray(k_dev, dtype=np.float32)
+        k_hyd = np.ones(T, dtype=np.float32) if k_hyd is None else np.asarray(k_hyd, dtype=np.float32)
+        self.tet_k_dev = f32(k_dev[gen])
+        self.tet_k_hyd = f32(k_hyd[gen])
+        self.tet_alpha = torch.zeros(2 * max(T, 1), dtype=torch.float32, device=dev)
+        self.tet_lambda = torch.zeros(2 * max(T, 1), dtype=torch.float32, device=dev)
+        self.grid_edges = i32(grid.edges.reshape(-1) if Eg else np.zeros(4, dtype=np.int32))
+        self.occupied = i32(grid.occupied)
+        self.cut_mask = torch.zeros(V, dtype=torch.int32, device=dev)
+        self.edge_cut_t = torch.full((6 * N,), -1.0, dtype=torch.float32, device=dev)
+        self.edge_skin_t = f32(grid.edge_skin_fraction())
+        self.cut_point_capacity = 4 * max(Eg, 1)
+        self.cut_points = torch.zeros(8 * self.cut_point_capacity, dtype=torch.float32, device=dev)
+        self.counters = torch.zeros(8, dtype=torch.int32, device=dev)
+        scratch = int(self.lib.gts_scratch_floats(N, E, T, Eg))
+        self.scratch = torch.zeros(scratch, dtype=torch.float32, device=dev)
+        self.reaction = torch.zeros(4, dtype=torch.float32, device=dev)
+        self.edge_offsets = np.ascontiguousarray(topo["edge_offsets"], dtype=np.int32)
+        self.tet_offsets = np.ascontiguousarray(topo["tet_offsets"], dtype=np.int32)
+        inv_rest = topo["type_inv_rest"].astype(np.float32).reshape(-1)
+        self.view = GtsTissueView(
+            N, E, T, Eg, *grid.dims, len(self.edge_offsets) - 1, len(self.tet_offsets) - 1,
+            self.edge_offsets.ctypes.data, self.tet_offsets.ctypes.data, (ct.c_float * 54)(*inv_rest),
+            v3(topo["axis_rest_length"]), float(grid.cell_size[0]),
+            *[t.data_ptr() for t in (self.pos, self.prev, self.vel, self.force, self.mass, self.active, self.pinned,
+                                     self.inside, self.nbr, self.bend_alive, self.edge_desc, self.edge_lambda,
+                                     self.slot_support, self.tet_desc, self.tet_k_dev, self.tet_k_hyd,
+                                     self.tet_alpha, self.tet_lambda, self.grid_edges, self.occupied, self.cut_mask,
+                                     self.edge_cut_t, self.edge_skin_t, self.cut_points)],
+            self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr())
+        self._tool_from = None
+        check("gts_init_state", self.lib.gts_init_state(ct.byref(self.view), current_stream()))
+
+    # ---------------------------------------------------------------- stepping
+    def set_gravity(self, g):
+        check("gts_set_gravity", self.lib.gts_set_gravity(ct.byref(self.view), *(float(x) for x in g),
+                                                          current_stream()))
+
+    def step(self, dt, tool=None, tool_from=None, tool_armed=False, blade_armed=False, params=None):
+        """One outer step; the tool pose moves from `tool_from` (default: the previous step's `tool`) to `tool`."""
+        p = step_params(params or self.params, dt)
+        to = tool_state(tool)
+        frm = tool_state(tool_from) if tool_from is not None else (self._tool_from or to)
+        check("gts_step", self.lib.gts_step(ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to),
+                                            int(tool_armed), int(blade_armed), current_stream()))
+        self._tool_from = to
+
+    def detect_cut(self, cut_tool):
+        t = cut_tool_struct(cut_tool)
+        check("gts_detect_cut", self.lib.gts_detect_cut(ct.byref(self.view), ct.byref(t), current_stream()))
+
+    def detect_cut_ccd(self, S, E, emit_ncut, tool_d, prev_positions):
+        pp = torch.as_tensor(np.ascontiguousarray(prev_positions, dtype=np.float32)).reshape(-1).to(self.device)
+        vs = [np.ascontiguousarray(v, dtype=np.float32) for v in (S, E, emit_ncut)]
+        check("gts_detect_cut_ccd", self.lib.gts_detect_cut_ccd(
+            ct.byref(self.view), *[v.ctypes.data_as(ct.c_void_p) for v in vs], float(tool_d), _ptr(pp),
+            current_stream()))
+        torch.cuda.current_stream().synchronize()
+
+    # ---------------------------------------------------------------- readback
+    def positions(self):
+        return self.pos[:, :3].cpu().numpy()
+
+    def velocities(self):
+        return self.vel[:, :3].cpu().numpy()
+
+    def inverse_mass(self):
+        return self.pos[:, 3].cpu().numpy()
+
+    def set_state(self, pos, vel):
+        self.pos[:, :3] = torch.as_tensor(np.asarray(pos, dtype=np.float32), device=self.device)
+        self.vel[:, :3] = torch.as_tensor(np.asarray(vel, dtype=np.float32), device=self.device)
+
+    def neighbors(self):
+        return self.nbr.cpu().numpy().reshape(-1, 6)
+
+    def revision(self):
+        return int(self.counters[0].item()) & 0xffffffff
+
+    def reaction_force(self):
+        r = self.reaction.cpu().numpy()
+        return r[:3].astype(np.float64), int(round(float(r[3])))
+
+    def cut_state(self):
+        counters = self.counters.cpu().numpy().view(np.uint32)
+        n = int(min(counters[1], self.cut_point_capacity))
+        raw = self.cut_points[:8 * n].cpu().numpy().reshape(-1, 8) if n else np.zeros((0, 8), dtype=np.float32)
+        points = np.zeros(n, dtype=[("offset", "<f4", 3), ("owner", "<i4"), ("edge", "<i4"), ("normal", "<f4", 3)])
+        if n:
+            points["offset"] = raw[:, 0:3]
+            points["owner"] = raw[:, 3].view(np.int32)
+            points["edge"] = raw[:, 4].view(np.int32)
+            points["normal"] = raw[:, 5:8]
+        bend = self.bend_alive.cpu().numpy().view(np.uint32)
+        bend_alive = ((bend[:, None] >> np.arange(15, dtype=np.uint32)) & 1).astype(np.int32)
+        return dict(neighbors=self.neighbors(), cut_mask=self.cut_mask.cpu().numpy().view(np.uint32),
+                    edge_cut_t=self.edge_cut_t.cpu().numpy().reshape(-1, 6), bend_alive=bend_alive,
+                    revision=int(counters[0]), cut_point_counter=int(counters[1]), cut_points=points)
+
+    def liveness(self):
+        """Tet liveness in generation order and edge liveness in colour order (as the CPU checker reports)."""
+        tet = np.zeros(self.T, dtype=np.int32)
+        tet[self.topology["tet_gen"]] = self.tet_desc.cpu().numpy() & 1
+        edge = self.edge_desc.cpu().numpy() & 1
+        return tet, edge
+
+    def compact(self):
+        """Stable remap of the live topology: (tet_remap in colour order, edge_remap, node_remap, counts)."""
+        dev = self.device
+        tr = torch.empty(max(self.T, 1), dtype=torch.int32, device=dev)
+        er = torch.empty(max(self.E, 1), dtype=torch.int32, device=dev)
+        nr = torch.empty(self.N, dtype=torch.int32, device=dev)
+        counts = torch.zeros(3, dtype=torch.int32, device=dev)
+        used = torch.empty(self.N, dtype=torch.int32, device=dev)
+        check("gts_compact_topology", self.lib.gts_compact_topology(
+            ct.byref(self.view), _ptr(tr), _ptr(er), _ptr(nr), _ptr(counts), _ptr(used), current_stream()))
+        return (tr[:self.T].cpu().numpy(), er[:self.E].cpu().numpy(), nr.cpu().numpy(), counts.cpu().numpy())

@@ -1,0 +1,146 @@
+"""The benchmark workloads: lattices, materials, tools and schedules, all synthetic and seeded.
+
+Units are the reference's native ones (cm, kg, s; gravity -9.81/0.01 cm/s^2, `native_backend.py:380`).
+
+Material mapping. The reference's tet constraint is the coupled Neo-Hookean 2x2 block of `k_solve_tet`
+(`xpbd/xpbd_solver.cu:749-829`) with deviatoric compliance 1/(mu V) and hydrostatic compliance 1/(kappa V). A
+workload's "tet compliance" c is the deviatoric compliance of each tet, and its volume compliance is c/10 (the
+kappa/mu ratio of Poisson's ratio 0.45, the reference's design value, `xpbd/xpbd_solver.h:62`). The 6144-tet block
+states edge compliance 1e-7 and volume compliance 1e-8, i.e. c = 1e-7. Without the deviatoric row an axis-edge
+lattice has no first-order shear stiffness and the block does not come to rest.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .grid import Ellipsoid, block_grid, block_nodes_for_tets, build_grid
+from .params import TET_NEOHOOKEAN, SolverParams, volume_stiffness
+
+GRAVITY = (0.0, -9.81 / 0.01, 0.0)
+VOLUME_RATIO = 10.0
+
+
+@dataclass
+class Workload:
+    name: str
+    grid: object
+    params: SolverParams
+    dt: float
+    steps: int
+    compliance: np.ndarray          # per-tet deviatoric compliance, generation order
+    initial_positions: np.ndarray
+    gravity: tuple = GRAVITY
+    blade: object = None            # callable step -> (S, E) segment, or None
+    blade_thickness: float = 0.0
+    tool: object = None             # callable step -> tool dict, or None
+    info: dict = field(default_factory=dict)
+
+    def stiffness(self):
+        """(k_dev, k_hyd) per tet in generation order: 1/c and 1/(c/10)."""
+        c = np.asarray(self.compliance, dtype=np.float64)
+        return volume_stiffness(c), volume_stiffness(c / VOLUME_RATIO)
+
+
+def tet_count(grid):
+    """Kuhn tets of the all-active cells of a lattice (six per cell)."""
+    nx, ny, nz = (d + 1 for d in grid.dims)
+    a = grid.active.reshape(nz, ny, nx) != 0
+    cells = (a[:-1, :-1, :-1] & a[:-1, :-1, 1:] & a[:-1, 1:, :-1] & a[:-1, 1:, 1:] &
+             a[1:, :-1, :-1] & a[1:, :-1, 1:] & a[1:, 1:, :-1] & a[1:, 1:, 1:])
+    return 6 * int(cells.sum())
+
+
+def jittered(grid, rng, sigma):
+    """Lattice positions plus N(0, sigma) per coordinate on every free node; pinned nodes stay exact."""
+    pos = grid.positions.astype(np.float64) + rng.normal(0.0, sigma, grid.positions.shape)
+    pos = pos.astype(np.float32)
+    pos[grid.pinned != 0] = grid.positions[grid.pinned != 0]
+    return pos
+
+
+def block_6144(seed=0, spacing=0.25, steps=200, substeps=10, iterations=1):
+    """The CPU-runnable block: 17x9x9 nodes, 6144 tets, edge compliance 1e-7, volume compliance 1e-8, bottom face
+    pinned, dt = 1/60 with 10 substeps of 1 iteration, 200 steps; a planar blade of half-width 2 cells at mid-height
+    sweeps +x at half a cell per step. Initial positions carry seeded N(0, 1e-4 cell) jitter."""
+    rng = np.random.default_rng(seed)
+    grid = block_grid((17, 9, 9), spacing)
+    L = spacing
+    params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7)
+    T = tet_count(grid)
+    zc = 4 * L
+
+    def blade(step):
+        x = -L + 0.5 * L * step
+        return (np.array((x, 4 * L, zc - 2 * L), np.float32), np.array((x, 4 * L, zc + 2 * L), np.float32))
+
+    return Workload("block-6144", grid, params, 1 / 60, steps, np.full(T, 1e-7), jittered(grid, rng, 1e-4 * L),
+                    blade=blade, blade_thickness=0.1 * L, info=dict(nodes=(17, 9, 9), spacing=L))
+
+
+def ellipsoid_spacing_for(tets, half_extents=(6.0, 3.0, 2.5)):
+    """Lattice spacing that puts about `tets` Kuhn tets in the ellipsoid's active cells."""
+    lo, hi = 0.1, 0.7
+    for _ in range(30):
+        mid = 0.5 * (lo + hi)
+        n = tet_count(build_grid(mid, shape=Ellipsoid(half_extents)))
+        if n > tets:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets=250_000):
+    """Kidney-scale single instance: an ellipsoid of semi-axes 6 x 3 x 2.5 cm rasterised at the spacing that gives
+    about 250k tets (1.65 mm; a 0.5 mm lattice would give 9M); per-tet compliance log-normal (median 1e-7,
+    sigma 0.5); a capsule grasper of radius 2 mm pressing 3 mm into the top; a blade on a seeded cubic Bezier over
+    the top surface cutting for 300 steps; 20 substeps of 2 iterations."""
+    rng = np.random.default_rng(seed)
+    shape = Ellipsoid((6.0, 3.0, 2.5))
+    h = spacing if spacing is not None else 0.1654
+    grid = build_grid(h, shape=shape)
+    T = tet_count(grid)
+    compliance = np.exp(rng.normal(np.log(1e-7), 0.5, T))
+    params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7,
+                          contact_enabled=True, contact_radius=0.0, jaw_thickness=0.1)
+    top = 3.0
+    # grasper capsule along x above the top, lowered 3 mm into the surface over the first 60 steps
+    r = 0.2
+
+    def tool(step):
+        depth = 0.3 * min(1.0, step / 60.0)
+        y = top + r - depth
+        return dict(shaftA=(-1.5, y, 1.0), shaftB=(1.5, y, 1.0), shaftR=r, jawAxis=(0, 0, 1), jawNormal=(1, 0, 0))
+
+    # blade: cubic Bezier with 4 seeded control points on the top surface; a vertical segment 0.6 cm tall
+    u = rng.uniform(-0.7, 0.7, size=(4, 2))
+    ctrl = np.stack((u[:, 0] * 6.0, np.zeros(4), u[:, 1] * 2.5), axis=1)
+    ctrl[:, 0] = np.sort(ctrl[:, 0])
+    for c in ctrl:
+        c[1] = 3.0 * np.sqrt(max(0.0, 1 - (c[0] / 6.0) ** 2 - (c[2] / 2.5) ** 2))
+
+    def blade(step):
+        s = min(1.0, step / steps)
+        b = ((1 - s) ** 3) * ctrl[0] + 3 * ((1 - s) ** 2) * s * ctrl[1] + 3 * (1 - s) * s * s * ctrl[2] + s ** 3 * ctrl[3]
+        return (np.array(b + (0, 0.3, 0), np.float32), np.array(b - (0, 0.6, 0), np.float32))
+
+    return Workload("kidney-250k", grid, params, 1 / 30, steps, compliance, jittered(grid, rng, 0.0),
+                    blade=blade, blade_thickness=0.02, tool=tool,
+                    info=dict(spacing=h, half_extents=(6.0, 3.0, 2.5)))
+
+
+def batched_blocks(count=1024, seed=0, tmin=2000, tmax=100_000, spacing=0.25):
+    """Many independent blocks: tet counts log-uniform in [tmin, tmax], 2:1:1 cells, per-instance compliance
+    log-uniform in [1e-8, 1e-6], one seeded straight cut per instance across the top face."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        t = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))
+        nodes = block_nodes_for_tets(t)
+        c = float(np.exp(rng.uniform(np.log(1e-8), np.log(1e-6))))
+        start = rng.uniform(0, 1, 2)
+        angle = rng.uniform(0, 2 * np.pi)
+        out.append(dict(nodes=nodes, compliance=c, cut_start=start, cut_angle=angle))
+    return out

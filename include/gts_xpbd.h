@@ -1,0 +1,180 @@
+/* gts_xpbd.h — C ABI of the MI355X (gfx950) XPBD tissue substep library, libgts_xpbd.so.
+ *
+ * The library computes; it allocates no device memory for a tissue. The caller owns every device buffer (the Python
+ * host side allocates them as PyTorch-ROCm tensors) and passes raw pointers and sizes in the descriptor structs
+ * below. All functions return int status: 0 = success, the reference's bands otherwise
+ * (xpbd/xpbd_solver.h:25-33 for -1000.., physics.cu:219-228 for -2000.., cut.cu:1109-1117 for -3000..), and
+ * -(int)hipError_t for a HIP runtime error. Every launch goes to the stream the caller passes.
+ *
+ * Reference interfaces replaced (paths relative to endoscopic_tissue/, native = native/vendor):
+ *   gts_build_topology       xpbd_bind edge derivation + xpbd_init colouring + xpbd_build_tets
+ *                            (native/cuda/xpbd/xpbd_solver.cu:1204-1279, :1412-1577, :1600-1848), reached from
+ *                            LCS_InitPhysics (native/src/plugin_api.cpp:105-115)
+ *   gts_set_gravity          LCS_SetGravity (plugin_api.cpp:103) -> k_SetGravity (physics.cu:317-329)
+ *   gts_step                 LCS_Step (plugin_api.cpp:125-134) -> physics_step (physics.cu:331-394)
+ *   gts_detect_cut           LCS_SetTool + LCS_DetectCut (plugin_api.cpp:145-155) -> cut_detect (cut.cu:1235-1254)
+ *   gts_detect_cut_ccd       the CCD detection of cut_ribbon_tick (cut.cu:486-592, :1303-1431)
+ *   gts_refresh_liveness     k_TopologyStale + k_RefreshLiveness (solver/material/unified_vertex.cuh:859-861,
+ *                            xpbd_solver.cu:429-479)
+ *   gts_compact_topology     no reference counterpart: stable remap of the live tets / edges / nodes
+ *   gts_batched_step         no reference counterpart (the reference is single-session): many instances, one launch
+ *   gts_adjoint_*            no reference counterpart: reverse of the substep for compliance gradients
+ * Readbacks (LCS_GetCornerPos, LCS_GetNbrIdx, LCS_GetTopologyRevision, LCS_GetCutEdgeFractions) are plain copies of
+ * the caller's own buffers and need no entry point.
+ */
+#ifndef GTS_XPBD_H
+#define GTS_XPBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GTS_ABI_VERSION 1
+
+#define GTS_OK 0
+#define GTS_ERR_NOT_READY (-1000)
+#define GTS_ERR_BAD_DT (-1002)
+#define GTS_ERR_COLOR_OVERFLOW (-1003)
+#define GTS_ERR_BAD_ARG (-1004)
+#define GTS_ERR_IRREGULAR_REST (-1005)
+#define GTS_ERR_CAPACITY (-3004)
+
+#define GTS_MAX_COLOURS 64
+
+/* Tet constraint forms of k_solve_tet (xpbd_solver.cu:749-829). */
+#define GTS_TET_VOLUME 0     /* hydrostatic row only, C = J - 1 */
+#define GTS_TET_NEOHOOKEAN 1 /* coupled deviatoric / hydrostatic block, C_H = J - 1 */
+#define GTS_TET_GUARDED 2    /* coupled block, C_H = log J + min(0,(J-.05)/.05), per-update trust region */
+
+/* Host lattice of one tissue (TissueGrid arrays, endoscopic_tissue/grid.py:41-61). */
+typedef struct GtsLattice {
+    int vx, vy, vz;         /* voxel dims; nodes (vx+1)(vy+1)(vz+1), id x + nx*(y + ny*z) */
+    const float* rest;      /* [3N] rest positions */
+    const int* active;      /* [N] */
+    const int* nbr;         /* [6N] neighbour slots -x,+x,-y,+y,-z,+z; -1 = no link */
+    const float* rest_nbr;  /* [18N] rest neighbour vectors */
+} GtsLattice;
+
+/* Topology in the device's storage order, written into caller-allocated host arrays.
+ * Edge descriptor: node i << 4 | axis << 1 (bits 0 and 3, liveness and tet support, are set on the device).
+ * Tet descriptor: corner-0 node << 4 | Kuhn type << 1 (bit 0, liveness, is set on the device).
+ * Both lists are colour-major; within a colour, in generation order. */
+typedef struct GtsTopology {
+    uint32_t* edge_desc;        /* capacity 3N */
+    int32_t* edge_gen_index;    /* colour slot -> generation index (p ascending, axis 0..2) */
+    int32_t* edge_colour_offsets; /* capacity GTS_MAX_COLOURS + 1 */
+    uint32_t* tet_desc;         /* capacity 6V */
+    int32_t* tet_gen_index;     /* colour slot -> generation index (cells z,y,x major, Kuhn type minor) */
+    int32_t* tet_colour_offsets; /* capacity GTS_MAX_COLOURS + 1 */
+    double type_inv_rest[6][3][3]; /* Dm^-1 of each Kuhn type, from the rest lattice spacing */
+    double type_volume[6];
+    float axis_rest_length[3];  /* L0 of the lattice edges along x, y, z */
+    int n_edges, n_tets, n_edge_colours, n_tet_colours;
+} GtsTopology;
+
+/* Device view of one tissue. Pointers are device pointers unless marked host. */
+typedef struct GtsTissueView {
+    int n_nodes, n_edges, n_tets, n_grid_edges;
+    int vx, vy, vz;
+    int n_edge_colours, n_tet_colours;
+    const int32_t* edge_colour_offsets; /* host, [n_edge_colours + 1] */
+    const int32_t* tet_colour_offsets;  /* host, [n_tet_colours + 1] */
+    float inv_rest[6][9];               /* Dm^-1 rows per Kuhn type, float */
+    float axis_rest_length[3];
+    float voxel_length;                 /* cut AABB pad and rest edge length of cut points */
+    /* node state: xyz + inverse mass in .w for pos; xyz for prev, vel, force */
+    float* pos;
+    float* prev;
+    float* vel;
+    float* force;
+    const float* mass;
+    const int32_t* active;
+    const int32_t* pinned;
+    const int32_t* inside;
+    int32_t* nbr;          /* [6N] live neighbour table */
+    uint32_t* bend_alive;  /* [N] bit p = bend pair p of the node is alive (grid.py:257-268) */
+    uint32_t* edge_desc;   /* [E] */
+    float* edge_lambda;    /* [E] */
+    int32_t* slot_support; /* [6N] live-tet support of each owner slot (fallback edges) */
+    uint32_t* tet_desc;    /* [T] */
+    const float* tet_k_dev;   /* [T] mu*V (or 1/compliance) */
+    const float* tet_k_hyd;   /* [T] kappa*V (or 1/compliance) */
+    float* tet_alpha;         /* [2T] per-frame alpha~: dev then hyd */
+    float* tet_lambda;        /* [2T] dev then hyd */
+    /* cut */
+    const int32_t* grid_edges;   /* [4*Eg] axis, x, y, z */
+    const uint32_t* occupied;    /* [V] */
+    uint32_t* cut_mask;          /* [V] */
+    float* edge_cut_t;           /* [6N] */
+    const float* edge_skin_t;    /* [6N] */
+    float* cut_points;           /* [8*capacity] CutPointGpu records (cut.cuh:37-46) */
+    int cut_point_capacity;
+    uint32_t* counters;          /* [8] 0 revision, 1 cut point counter, 2 refreshed revision, 3 stale flag */
+    float* scratch;              /* [4*Eg + 2*nblocks + 64] detection records and scan partials */
+    float* reaction;             /* [4] force of the tissue on the tools, contacts */
+} GtsTissueView;
+
+typedef struct GtsStepParams {
+    float dt;
+    int substeps, iterations, tet_mode;
+    float omega, edge_ks, young, beta_s, strain_damping, bulk_damping, rayleigh, damping;
+    int contact_enabled;
+    float mu_s, mu_d, jaw_thickness, contact_radius, jaw_gap_closed, contact_compliance;
+} GtsStepParams;
+
+/* XpbdToolState (xpbd_solver.h:223-229) plus a blade capsule. */
+typedef struct GtsToolState {
+    float shaftA[3], shaftB[3], shaftR;
+    float jawHinge[3], jawAxis[3], jawMid[3], jawNormal[3];
+    float jawHalfWidth, jawLen, jawGap, grasp01;
+    float bladeA[3], bladeB[3], bladeR;
+} GtsToolState;
+
+/* CutToolDesc (cut.h:30-38): swept quad T1 = (Sprev, Eprev, E), T2 = (Sprev, E, S). */
+typedef struct GtsCutTool {
+    float t1v0[3], t1v1[3], t1v2[3];
+    float t2v0[3], t2v1[3], t2v2[3];
+    float ncut[3];
+    float d;
+    int valid;
+    float aabbMin[3], aabbMax[3];
+} GtsCutTool;
+
+int gts_abi_version(void);
+/* Sizes of the structs above as this library sees them (ABI guard like LCS_PhysicsAbiVersion,
+ * plugin_api.cpp:102). out[0..5] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool. */
+int gts_struct_sizes(int* out);
+
+/* Host: derive edges, Kuhn tets and both greedy colourings from the lattice. Needs no GPU. */
+int gts_build_topology(const GtsLattice* lattice, GtsTopology* out);
+
+/* Host: J^(-2/3) as the tet kernel evaluates it (own cube-root sequence), for checks. */
+float gts_iso_pow(float J);
+/* Host: floats of device scratch a tissue view needs (detection records + scan partials). */
+long long gts_scratch_floats(int n_nodes, int n_edges, int n_tets, int n_grid_edges);
+
+/* Device: initialise the per-node inverse mass (.w of pos) and the bend masks; mark all liveness stale. */
+int gts_init_state(const GtsTissueView* v, void* stream);
+int gts_set_gravity(const GtsTissueView* v, float gx, float gy, float gz, void* stream);
+/* Device: re-derive edge / tet liveness and the fallback tet support when the topology revision moved. */
+int gts_refresh_liveness(const GtsTissueView* v, void* stream);
+/* Device: one outer step of p->substeps substeps. The tool pose moves from `from` to `to` over the step. */
+int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from, const GtsToolState* to,
+             int tool_armed, int blade_armed, void* stream);
+/* Device: swept-quad cut of the current positions, then sever the cut links and refresh liveness. */
+int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream);
+/* Device: CCD of the edges moving from prev_positions ([3N] device) to the current positions against blade S->E. */
+int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, const float* emit_ncut, float tool_d,
+                       const float* prev_positions, void* stream);
+/* Device: stable remap of the live topology. tet_remap [T] (colour order), edge_remap [E], node_remap [N],
+ * counts [3] (live tets, edges, nodes); all device. scratch as in the view. */
+int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap, int32_t* node_remap,
+                         int32_t* counts, int32_t* node_used, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GTS_XPBD_H */

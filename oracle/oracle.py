@@ -1,0 +1,250 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker of
+the HIP path; never imported by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+F = ct.c_float
+I = ct.c_int
+P = ct.c_void_p
+
+
+class OracleDesc(ct.Structure):
+    _fields_ = [("vx", I), ("vy", I), ("vz", I), ("grid_edge_count", I), ("substeps", I), ("iters", I),
+                ("tet_mode", I), ("omega", F), ("edge_ks", F), ("young", F), ("beta_s", F),
+                ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F), ("damping", F),
+                ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F), ("contact_radius", F),
+                ("jaw_gap_closed", F), ("contact_compliance", F), ("tool_voxel", F), ("cut_point_capacity", I)]
+
+
+class OracleTool(ct.Structure):
+    _fields_ = [("shaftA", F * 3), ("shaftB", F * 3), ("shaftR", F), ("jawHinge", F * 3), ("jawAxis", F * 3),
+                ("jawMid", F * 3), ("jawNormal", F * 3), ("jawHalfWidth", F), ("jawLen", F), ("jawGap", F),
+                ("grasp01", F), ("bladeA", F * 3), ("bladeB", F * 3), ("bladeR", F)]
+
+
+class OracleCutTool(ct.Structure):
+    _fields_ = [("t1v0", F * 3), ("t1v1", F * 3), ("t1v2", F * 3), ("t2v0", F * 3), ("t2v1", F * 3),
+                ("t2v2", F * 3), ("ncut", F * 3), ("d", F), ("valid", I), ("aabbMin", F * 3), ("aabbMax", F * 3)]
+
+
+CUT_POINT_DTYPE = np.dtype([("offset", "<f4", 3), ("owner", "<i4"), ("edge", "<i4"), ("normal", "<f4", 3)])
+
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so from oracle/xpbd_oracle.cpp with the committed Makefile."""
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "xpbd_oracle.cpp").stat().st_mtime:
+        subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "liboracle.so"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ct.CDLL(str(LIB_PATH))
+        for suf in ("f32", "f64"):
+            getattr(L, f"oracle_create_{suf}").restype = P
+            getattr(L, f"oracle_create_{suf}").argtypes = [ct.POINTER(OracleDesc)] + [P] * 11
+            for name in ("status", "step"):
+                getattr(L, f"oracle_{name}_{suf}").restype = I
+            getattr(L, f"oracle_step_{suf}").argtypes = [P, F]
+            getattr(L, f"oracle_status_{suf}").argtypes = [P]
+            getattr(L, f"oracle_destroy_{suf}").argtypes = [P]
+            getattr(L, f"oracle_counts_{suf}").argtypes = [P, P]
+            getattr(L, f"oracle_topology_{suf}").argtypes = [P] * 10
+            getattr(L, f"oracle_type_data_{suf}").argtypes = [P, P, P]
+            getattr(L, f"oracle_set_tet_stiffness_{suf}").argtypes = [P, P, P]
+            getattr(L, f"oracle_set_gravity_{suf}").argtypes = [P, F, F, F]
+            getattr(L, f"oracle_set_tool_{suf}").argtypes = [P, ct.POINTER(OracleTool), I, I, I]
+            getattr(L, f"oracle_detect_cut_{suf}").argtypes = [P, ct.POINTER(OracleCutTool)]
+            getattr(L, f"oracle_detect_cut_ccd_{suf}").argtypes = [P, P, P, P, F, P]
+            getattr(L, f"oracle_refresh_{suf}").argtypes = [P]
+            getattr(L, f"oracle_get_state_{suf}").argtypes = [P, P, P]
+            getattr(L, f"oracle_set_state_{suf}").argtypes = [P, P, P]
+            getattr(L, f"oracle_get_reaction_{suf}").argtypes = [P, P]
+            getattr(L, f"oracle_get_cut_{suf}").argtypes = [P, P, P, P, P, P]
+            getattr(L, f"oracle_get_cut_{suf}").restype = ct.c_uint
+            getattr(L, f"oracle_get_cut_points_{suf}").argtypes = [P, P]
+            getattr(L, f"oracle_get_liveness_{suf}").argtypes = [P, P, P]
+            getattr(L, f"oracle_compact_{suf}").argtypes = [P, P, P, P, P]
+        L.oracle_threads.argtypes = [I]
+        L.oracle_threads.restype = I
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(P)
+
+
+def _v3(values):
+    return (F * 3)(*[float(v) for v in values])
+
+
+def tool_struct(tool):
+    """OracleTool from a dict with the XpbdToolState fields (plus optional blade capsule)."""
+    t = OracleTool()
+    for name in ("shaftA", "shaftB", "jawHinge", "jawAxis", "jawMid", "jawNormal", "bladeA", "bladeB"):
+        setattr(t, name, _v3(tool.get(name, (0, 0, 0))))
+    for name in ("shaftR", "jawHalfWidth", "jawLen", "jawGap", "grasp01", "bladeR"):
+        setattr(t, name, float(tool.get(name, 0.0)))
+    return t
+
+
+def cut_tool_struct(tool):
+    t = OracleCutTool()
+    for name in ("t1v0", "t1v1", "t1v2", "t2v0", "t2v1", "t2v2", "ncut", "aabbMin", "aabbMax"):
+        setattr(t, name, _v3(tool[name]))
+    t.d = float(tool["d"])
+    t.valid = int(tool["valid"])
+    return t
+
+
+class Oracle:
+    """One tissue instance in the CPU oracle, FP32 (`double=False`) or FP64."""
+
+    def __init__(self, grid, params, initial_positions=None, double=False):
+        self.L = lib()
+        self.suf = "f64" if double else "f32"
+        self.real = np.float64 if double else np.float32
+        self.grid = grid
+        vx, vy, vz = grid.dims
+        p = params
+        d = OracleDesc(vx, vy, vz, len(grid.edges), p.substeps, p.iterations, p.tet_mode, p.omega, p.edge_ks,
+                       p.young, p.beta_s, p.strain_damping, p.bulk_damping, p.rayleigh, p.damping,
+                       int(p.contact_enabled), p.mu_s, p.mu_d, p.jaw_thickness, p.contact_radius,
+                       p.jaw_gap_closed, p.contact_compliance, float(grid.cell_size[0]),
+                       4 * len(grid.edges))
+        self.desc = d
+        rest = np.ascontiguousarray(grid.positions, dtype=np.float32)
+        pos = rest if initial_positions is None else np.ascontiguousarray(initial_positions, dtype=np.float32)
+        self._keep = [np.ascontiguousarray(a) for a in (
+            rest, pos, grid.mass.astype(np.float32), grid.pinned.astype(np.int32), grid.active.astype(np.int32),
+            grid.inside.astype(np.int32), grid.neighbors.astype(np.int32), grid.occupied.astype(np.int32),
+            grid.edges.astype(np.int32), grid.edge_skin_fraction(), grid.rest_neighbors.astype(np.float32))]
+        self.h = self._call("create", ct.byref(d), *[_ptr(a) for a in self._keep])
+        status = self._call("status", self.h)
+        if status != 0:
+            raise RuntimeError(f"oracle topology build failed with status {status}")
+        counts = np.zeros(5, dtype=np.int32)
+        self._call("counts", self.h, _ptr(counts))
+        self.N, self.E, self.T, self.edge_colours, self.tet_colours = (int(c) for c in counts)
+
+    def _call(self, name, *args):
+        return getattr(self.L, f"oracle_{name}_{self.suf}")(*args)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._call("destroy", self.h)
+            self.h = None
+
+    def topology(self):
+        ei, ej, eo = (np.zeros(self.E, dtype=np.int32) for _ in range(3))
+        ecc = np.zeros(self.edge_colours, dtype=np.int32)
+        tids = np.zeros(4 * self.T, dtype=np.int32)
+        ttype, tcolor = np.zeros(self.T, dtype=np.int32), np.zeros(self.T, dtype=np.int32)
+        towner = np.zeros(3 * self.T, dtype=np.int32)
+        tcc = np.zeros(self.tet_colours, dtype=np.int32)
+        self._call("topology", self.h, *[_ptr(a) for a in (ei, ej, eo, ecc, tids, ttype, towner, tcolor, tcc)])
+        return dict(edge_i=ei, edge_j=ej, edge_owner=eo, edge_colour_counts=ecc, tet_ids=tids.reshape(-1, 4),
+                    tet_type=ttype, tet_owner=towner.reshape(-1, 3), tet_colour=tcolor, tet_colour_counts=tcc)
+
+    def type_data(self):
+        B, V = np.zeros(54), np.zeros(6)
+        self._call("type_data", self.h, _ptr(B), _ptr(V))
+        return B.reshape(6, 3, 3), V
+
+    def set_tet_stiffness(self, k_dev, k_hyd):
+        kd = np.ascontiguousarray(k_dev, dtype=np.float32)
+        kh = np.ascontiguousarray(k_hyd, dtype=np.float32)
+        assert len(kd) == self.T and len(kh) == self.T
+        self._call("set_tet_stiffness", self.h, _ptr(kd), _ptr(kh))
+
+    def set_gravity(self, g):
+        self._call("set_gravity", self.h, *(float(v) for v in g))
+
+    def set_tool(self, tool, tool_armed=True, blade_armed=False, reset=False):
+        self._tool = tool_struct(tool)
+        self._call("set_tool", self.h, ct.byref(self._tool), int(tool_armed), int(blade_armed), int(reset))
+
+    def step(self, dt):
+        rc = self._call("step", self.h, float(dt))
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed with status {rc}")
+
+    def detect_cut(self, tool):
+        t = cut_tool_struct(tool)
+        self._call("detect_cut", self.h, ct.byref(t))
+
+    def detect_cut_ccd(self, S, E, ncut, tool_d, prev_positions):
+        pp = np.ascontiguousarray(prev_positions, dtype=np.float32)
+        args = [np.asarray(v, dtype=np.float32) for v in (S, E, ncut)]
+        self._call("detect_cut_ccd", self.h, *[_ptr(a) for a in args], float(tool_d), _ptr(pp))
+
+    def state(self):
+        pos = np.zeros((self.N, 3), dtype=self.real)
+        vel = np.zeros((self.N, 3), dtype=self.real)
+        self._call("get_state", self.h, _ptr(pos), _ptr(vel))
+        return pos, vel
+
+    def set_state(self, pos, vel):
+        p = np.ascontiguousarray(pos, dtype=np.float32)
+        v = np.ascontiguousarray(vel, dtype=np.float32)
+        self._call("set_state", self.h, _ptr(p), _ptr(v))
+
+    def reaction(self):
+        out = np.zeros(4)
+        self._call("get_reaction", self.h, _ptr(out))
+        return out[:3], int(out[3])
+
+    def cut_state(self):
+        nbr = np.zeros((self.N, 6), dtype=np.int32)
+        V = int(np.prod(self.grid.dims))
+        mask = np.zeros(V, dtype=np.uint32)
+        cut_t = np.zeros((self.N, 6), dtype=np.float32)
+        bend_alive = np.zeros((self.N, 15), dtype=np.int32)
+        counters = np.zeros(2, dtype=np.uint32)
+        n = self._call("get_cut", self.h, *[_ptr(a) for a in (nbr, mask, cut_t, bend_alive, counters)])
+        points = np.zeros(n, dtype=CUT_POINT_DTYPE)
+        if n:
+            self._call("get_cut_points", self.h, _ptr(points))
+        return dict(neighbors=nbr, cut_mask=mask, edge_cut_t=cut_t, bend_alive=bend_alive,
+                    revision=int(counters[0]), cut_point_counter=int(counters[1]), cut_points=points)
+
+    def liveness(self):
+        tet = np.zeros(self.T, dtype=np.int32)
+        edge = np.zeros(self.E, dtype=np.int32)
+        self._call("get_liveness", self.h, _ptr(tet), _ptr(edge))
+        return tet, edge
+
+    def compact(self):
+        tr = np.zeros(self.T, dtype=np.int32)
+        er = np.zeros(self.E, dtype=np.int32)
+        nr = np.zeros(self.N, dtype=np.int32)
+        counts = np.zeros(3, dtype=np.int32)
+        self._call("compact", self.h, *[_ptr(a) for a in (tr, er, nr, counts)])
+        return tr, er, nr, counts
+
+
+def set_threads(n):
+    return lib().oracle_threads(int(n))
+
+
+if os.environ.get("GTS_ORACLE_BUILD"):
+    build()

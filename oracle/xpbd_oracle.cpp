@@ -1,0 +1,1277 @@
+// CPU oracle of the endoscopic_tissue legacy XPBD substep and swept-quad cut.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker the HIP path is compared with; nothing in the product links,
+// loads or calls it. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+//
+// It is a serial restatement of the reference algorithm, templated on the real type (float = parity oracle and the
+// timed CPU baseline, double = accuracy oracle). Every function cites the reference file:line it follows; paths are
+// relative to the reference checkout (endoscopic_tissue/native/vendor/cuda unless noted). Compile with
+// -ffp-contract=off so that every product and sum is rounded on its own, as written.
+//
+// Reference structure restated here:
+//   topology     xpbd_bind edge dedup (xpbd/xpbd_solver.cu:1445-1463) + greedy edge colouring (:1213-1249);
+//                xpbd_build_tets Kuhn split, Dm^-1 per type, greedy tet colouring (:1600-1848)
+//   frame        xpbd_begin_frame (:2481-2514), k_ComputeTetAlpha (:578-587)
+//   substep      xpbd_substep legacy branch order (:3460-3721): tool lerp, k_RefreshLiveness (:429-466),
+//                fallback edge support (:469-479), k_predict (:393-412), lambda reset, k_solve_edges (:484-524),
+//                k_solve_tet (:749-829), friction tool contact (contact/friction_contact.cuh:17-169),
+//                k_update_velocity / k_perp_damp_edges / k_apply_velscale (:871-919)
+//   cut          k_DetectCutQuad (cut.cu:421-474), MollerTrumbore (cut.cuh:162-185), MarkAndEmit/EmitCutPoint
+//                (cut.cu:186-254), k_DetectCutCCD (cut.cu:486-592), SeverLink/k_SeverLinks (cut.cu:719-746)
+//   compaction   build-side stable remap of live tets / edges / nodes (no reference counterpart)
+//   adjoint      reverse of the volume-only substep (no reference counterpart; pinned by finite differences)
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+template <class R> struct V3 { R x, y, z; };
+template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <class R> inline R dot3(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline R len3(V3<R> a) { return std::sqrt(dot3(a, a)); }
+template <class R> inline V3<R> cross3(V3<R> a, V3<R> b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+template <class R> inline R comp(V3<R> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// J^(-2/3) of k_solve_tet (the reference calls powf, xpbd_solver.cu:773). The float oracle and the HIP kernel both
+// evaluate it with this fixed sequence of IEEE operations -- a bit-level cube-root seed and three Newton steps, then
+// 1/c^2 -- so the two agree bit for bit; it stays within a few ulp of the exact power for J >= 0.05. The double
+// oracle uses the library power.
+inline float isoPow(float J) {
+    uint32_t i;
+    std::memcpy(&i, &J, 4);
+    i = i / 3u + 0x2a514067u;
+    float c;
+    std::memcpy(&c, &i, 4);
+    for (int k = 0; k < 3; k++) {
+        float c2 = c * c;
+        c = c - (c2 * c - J) / (3.f * c2);
+    }
+    return 1.f / (c * c);
+}
+inline double isoPow(double J) { return std::pow(J, -2.0 / 3.0); }
+template <class R> inline R fmin_(R a, R b) { return std::fmin(a, b); }
+template <class R> inline R fmax_(R a, R b) { return std::fmax(a, b); }
+
+// Kuhn split of a cell around the diagonal 0-7, corners in bit order i = dx | dy<<1 | dz<<2 (xpbd_solver.cu:1624).
+const int KUHN[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
+// Bend slot pairs in lexicographic order (grid.py:257-260).
+const int BEND_PAIRS[15][2] = {{0, 1}, {0, 2}, {0, 3}, {0, 4}, {0, 5}, {1, 2}, {1, 3}, {1, 4},
+                               {1, 5}, {2, 3}, {2, 4}, {2, 5}, {3, 4}, {3, 5}, {4, 5}};
+
+}  // namespace
+
+extern "C" {
+
+// Solver description; every field is the value the reference derives from PhysicsDesc / XpbdParams /
+// XpbdGraspParams / the contact model (xpbd_solver.h:38-75, physics.cu:281-295, xpbd_solver.cu:2200-2210).
+struct OracleDesc {
+    int vx, vy, vz;            // voxel dims; nodes are (vx+1)(vy+1)(vz+1)
+    int grid_edge_count;       // lattice edges that may be cut (four incident voxels in bounds)
+    int substeps, iters;       // N small steps per frame, constraint iterations per substep
+    int tet_mode;              // 0: volume row only (C = J-1); 1: coupled Neo-Hookean, C_H = J-1; 2: guarded log J
+    float omega;               // SOR factor applied to lambda and positions alike
+    float edge_ks;             // > 0: every live edge with compliance 1/ks; <= 0: fallback edges only
+    float young;               // Young's modulus of the fallback edge law alpha~ = 1/(E h^2 L)
+    float beta_s;              // edge Kelvin-Voigt damping stiffness
+    float strain_damping;      // tet Kelvin damping retardation time, s
+    float bulk_damping;        // guarded-mode bulk damping time, s
+    float rayleigh;            // global velocity damping, 1/s
+    float damping;             // legacy per-substep velocity kill
+    int contact_enabled;       // friction tool contact of lattice nodes (friction_contact.cuh:143-169)
+    float mu_s, mu_d;          // Coulomb coefficients
+    float jaw_thickness;       // jaw plate thickness
+    float contact_radius;      // node radius of the swept tests
+    float jaw_gap_closed;      // closed-gap floor of the jaw boxes
+    float contact_compliance;  // contact compliance, scaled by 1/h^2
+    float tool_voxel;          // cut: lattice spacing used as AABB pad and rest edge length (voxelL)
+    int cut_point_capacity;
+};
+
+// Tool pose (XpbdToolState, xpbd_solver.h:223-229) plus the optional blade capsule; 27 floats.
+struct OracleTool {
+    float shaftA[3], shaftB[3], shaftR;
+    float jawHinge[3], jawAxis[3], jawMid[3], jawNormal[3];
+    float jawHalfWidth, jawLen, jawGap, grasp01;
+    float bladeA[3], bladeB[3], bladeR;
+};
+
+// Swept cut plane (CutToolDesc, cut.h:30-38).
+struct OracleCutTool {
+    float t1v0[3], t1v1[3], t1v2[3];
+    float t2v0[3], t2v1[3], t2v2[3];
+    float ncut[3];
+    float d;
+    int valid;
+    float aabbMin[3], aabbMax[3];
+};
+
+}  // extern "C"
+
+namespace {
+
+template <class R> struct Shape {  // ContactShape (friction_contact.cuh:3-7)
+    V3<R> center, u, n, w, half, a, b;
+    R radius;
+    int capsule;
+};
+template <class R> struct Record { V3<R> normal, target, motion; int valid; };
+
+template <class R> struct ToolR {
+    V3<R> shaftA, shaftB; R shaftR;
+    V3<R> jawHinge, jawAxis, jawMid, jawNormal;
+    R jawHalfWidth, jawLen, jawGap, grasp01;
+    V3<R> bladeA, bladeB; R bladeR;
+};
+
+template <class R> V3<R> v3(const float* p) { return mk<R>((R)p[0], (R)p[1], (R)p[2]); }
+
+template <class R> ToolR<R> toolFrom(const OracleTool& t) {
+    ToolR<R> r;
+    r.shaftA = v3<R>(t.shaftA); r.shaftB = v3<R>(t.shaftB); r.shaftR = (R)t.shaftR;
+    r.jawHinge = v3<R>(t.jawHinge); r.jawAxis = v3<R>(t.jawAxis); r.jawMid = v3<R>(t.jawMid);
+    r.jawNormal = v3<R>(t.jawNormal);
+    r.jawHalfWidth = (R)t.jawHalfWidth; r.jawLen = (R)t.jawLen; r.jawGap = (R)t.jawGap; r.grasp01 = (R)t.grasp01;
+    r.bladeA = v3<R>(t.bladeA); r.bladeB = v3<R>(t.bladeB); r.bladeR = (R)t.bladeR;
+    return r;
+}
+
+template <class R> inline V3<R> unitOr(V3<R> x, V3<R> fallback) {  // contactUnit (friction_contact.cuh:14-16)
+    R l = len3(x);
+    return l > (R)1e-8f ? x * ((R)1 / l) : fallback;
+}
+
+// contactShape (friction_contact.cuh:17-39); kind 0 shaft capsule, 1/2 jaw boxes, 3 the blade capsule.
+template <class R> Shape<R> contactShape(const ToolR<R>& t, int kind, R thickness, R minGap) {
+    Shape<R> s{};
+    s.capsule = (kind == 0 || kind == 3);
+    if (kind == 0) { s.a = t.shaftA; s.b = t.shaftB; s.radius = t.shaftR; return s; }
+    if (kind == 3) { s.a = t.bladeA; s.b = t.bladeB; s.radius = t.bladeR; return s; }
+    V3<R> u = unitOr(t.jawAxis, mk<R>(0, 0, 1));
+    V3<R> n = t.jawNormal - u * dot3(u, t.jawNormal);
+    n = unitOr(n, std::fabs(u.y) < (R)0.9f ? unitOr(cross3(u, mk<R>(0, 1, 0)), mk<R>(1, 0, 0)) : mk<R>(1, 0, 0));
+    R side = kind == 1 ? (R)1 : (R)-1;
+    R sn = fmin_<R>((R)0.95f, fmax_<R>((R)0, t.jawGap) / ((R)2 * fmax_<R>(t.jawLen, (R)1e-6f)));
+    R cs = std::sqrt((R)1 - sn * sn);
+    s.u = u * cs + n * (side * sn);
+    s.n = n * cs - u * (side * sn);
+    s.w = cross3(s.u, s.n);
+    s.half = mk<R>(t.jawLen * (R)0.5f, thickness * (R)0.5f, t.jawHalfWidth);
+    s.center = t.jawHinge + s.u * s.half.x + s.n * (side * (s.half.y + (R)0.5f * minGap));
+    return s;
+}
+
+// shapeDistance (friction_contact.cuh:40-63).
+template <class R> R shapeDistance(const Shape<R>& s, V3<R> x, V3<R>& normal, V3<R>& surface) {
+    if (s.capsule) {
+        V3<R> ab = s.b - s.a;
+        R tt = fmin_<R>((R)1, fmax_<R>((R)0, dot3(x - s.a, ab) / fmax_<R>(dot3(ab, ab), (R)1e-20f)));
+        V3<R> d = x - (s.a + ab * tt);
+        R l = len3(d);
+        V3<R> fallback = unitOr(cross3(ab, std::fabs(ab.x) < std::fabs(ab.y) ? mk<R>(1, 0, 0) : mk<R>(0, 1, 0)),
+                                mk<R>(0, 0, 1));
+        normal = l > (R)1e-6f * fmax_<R>((R)1, len3(ab)) ? d * ((R)1 / l) : fallback;
+        surface = s.a + ab * tt + normal * s.radius;
+        return l - s.radius;
+    }
+    V3<R> r = x - s.center;
+    R v[3] = {dot3(r, s.u), dot3(r, s.n), dot3(r, s.w)};
+    R h[3] = {s.half.x, s.half.y, s.half.z};
+    R q[3], cl[3];
+    for (int k = 0; k < 3; k++) { q[k] = std::fabs(v[k]) - h[k]; cl[k] = fmin_<R>(h[k], fmax_<R>(-h[k], v[k])); }
+    V3<R> closest = s.center + s.u * cl[0] + s.n * cl[1] + s.w * cl[2];
+    V3<R> d = s.u * std::copysign(fmax_<R>(q[0], (R)0), v[0]) + s.n * std::copysign(fmax_<R>(q[1], (R)0), v[1]) +
+              s.w * std::copysign(fmax_<R>(q[2], (R)0), v[2]);
+    R len = len3(d);
+    if (q[0] > 0 || q[1] > 0 || q[2] > 0) { normal = len > 0 ? d * ((R)1 / len) : s.n; surface = closest; return len; }
+    int k = q[1] > q[0] ? 1 : 0;
+    if (q[2] > q[k]) k = 2;
+    V3<R> axes[3] = {s.u, s.n, s.w};
+    normal = axes[k] * (v[k] >= 0 ? (R)1 : (R)-1);
+    surface = x - normal * q[k];
+    return q[k];
+}
+
+template <class R> Shape<R> shapeLerp(const Shape<R>& a, const Shape<R>& b, R t) {  // friction_contact.cuh:64-68
+    Shape<R> s = b;
+    s.center = a.center * ((R)1 - t) + b.center * t;
+    s.a = a.a * ((R)1 - t) + b.a * t;
+    s.b = a.b * ((R)1 - t) + b.b * t;
+    s.u = unitOr(a.u * ((R)1 - t) + b.u * t, b.u);
+    s.n = a.n * ((R)1 - t) + b.n * t;
+    s.n = unitOr(s.n - s.u * dot3(s.u, s.n), b.n);
+    s.w = cross3(s.u, s.n);
+    return s;
+}
+
+template <class R> V3<R> rotateShaftVector(const Shape<R>& from, const Shape<R>& to, V3<R> r) {  // :69-76
+    V3<R> a = from.b - from.a, b = to.b - to.a;
+    if (len3(a) < (R)1e-8f || len3(b) < (R)1e-8f) return r;
+    a = unitOr(a, mk<R>(0, 0, 1));
+    b = unitOr(b, a);
+    R c = fmin_<R>((R)1, fmax_<R>((R)-1, dot3(a, b)));
+    V3<R> v = cross3(a, b);
+    if (c < (R)-0.9999f) {
+        V3<R> u = unitOr(cross3(a, std::fabs(a.x) < (R)0.9f ? mk<R>(1, 0, 0) : mk<R>(0, 1, 0)), mk<R>(0, 0, 1));
+        return u * ((R)2 * dot3(u, r)) - r;
+    }
+    return r + cross3(v, r) + cross3(v, cross3(v, r)) * ((R)1 / ((R)1 + c));
+}
+
+template <class R> V3<R> witnessAt(const Shape<R>& from, const Shape<R>& to, V3<R> p) {  // :77-84
+    if (from.capsule) {
+        V3<R> ab = from.b - from.a;
+        R t = fmin_<R>((R)1, fmax_<R>((R)0, dot3(p - from.a, ab) / fmax_<R>(dot3(ab, ab), (R)1e-20f)));
+        return to.a + (to.b - to.a) * t + rotateShaftVector(from, to, p - from.a - ab * t);
+    }
+    V3<R> r = p - from.center;
+    return to.center + to.u * dot3(r, from.u) + to.n * dot3(r, from.n) + to.w * dot3(r, from.w);
+}
+
+// pointBoxTime (friction_contact.cuh:86-106): first time a point moving a->b comes within radius of the box.
+template <class R> bool pointBoxTime(const Shape<R>& box, R radius, V3<R> a, V3<R> b, R& time) {
+    V3<R> axes[3] = {box.u, box.n, box.w};
+    R p[3], v[3], half[3], cuts[8] = {0, 1};
+    int count = 2;
+    for (int k = 0; k < 3; k++) {
+        p[k] = dot3(a, axes[k]);
+        v[k] = dot3(b - a, axes[k]);
+        half[k] = comp(box.half, k);
+        if (std::fabs(v[k]) > (R)1e-12f)
+            for (int side = -1; side <= 1; side += 2) {
+                R t = ((R)side * half[k] - p[k]) / v[k];
+                if (t > 0 && t < 1) cuts[count++] = t;
+            }
+    }
+    for (int i = 1; i < count; i++) {
+        R x = cuts[i];
+        int j = i;
+        while (j > 0 && cuts[j - 1] > x) { cuts[j] = cuts[j - 1]; --j; }
+        cuts[j] = x;
+    }
+    for (int i = 0; i < count - 1; i++) {
+        double A = 0, B = 0, C = -(double)radius * radius;
+        R lo = cuts[i], hi = cuts[i + 1], mid = (R)0.5f * (lo + hi);
+        for (int k = 0; k < 3; k++) {
+            R q = p[k] + v[k] * mid;
+            if (std::fabs(q) <= half[k]) continue;
+            double d = p[k] - std::copysign(half[k], q);
+            A += (double)v[k] * v[k];
+            B += 2 * d * v[k];
+            C += d * d;
+        }
+        double atLo = (A * lo + B) * lo + C;
+        if (atLo <= 1e-14) { time = lo; return true; }
+        if (A <= 1e-24) continue;
+        double disc = B * B - 4 * A * C;
+        if (disc < 0) continue;
+        double root = (-B - std::sqrt(disc)) / (2 * A);
+        if (root >= lo - 1e-8 && root <= hi + 1e-8) { time = fmin_<R>(hi, fmax_<R>(lo, (R)root)); return true; }
+    }
+    return false;
+}
+
+// traceToolContact (friction_contact.cuh:107-142): swept conservative advancement of a node against a moving shape.
+template <class R> Record<R> traceToolContact(const Shape<R>& from, const Shape<R>& to, R radius, V3<R> x0, V3<R> x1) {
+    Record<R> out{};
+    V3<R> normal{}, surface{};
+    if (!to.capsule && len3(to.u - from.u) + len3(to.n - from.n) < (R)1e-7f) {
+        R time = 0;
+        if (!pointBoxTime(to, radius, x0 - from.center, x1 - to.center, time)) return out;
+        Shape<R> hit = shapeLerp(from, to, time);
+        shapeDistance(hit, x0 * ((R)1 - time) + x1 * time, normal, surface);
+        V3<R> end = witnessAt(hit, to, surface), start = witnessAt(hit, from, surface);
+        out.normal = normal; out.target = end + normal * radius; out.motion = end - start; out.valid = 1;
+        return out;
+    }
+    R speed = len3(x1 - x0);
+    if (to.capsule) speed += fmax_<R>(len3(to.a - from.a), len3(to.b - from.b));
+    else speed += len3(to.center - from.center) + (R)4.f * len3(to.half) * (len3(to.u - from.u) + len3(to.n - from.n));
+    V3<R> center0 = to.capsule ? (from.a + from.b) * (R)0.5f : from.center;
+    V3<R> center1 = to.capsule ? (to.a + to.b) * (R)0.5f : to.center;
+    R bound = to.capsule ? fmax_<R>(len3(from.b - from.a), len3(to.b - to.a)) * (R)0.5f + to.radius : len3(to.half);
+    if (len3(x0 - center0) > bound + radius + len3(x1 - x0) + len3(center1 - center0) + (R)1e-5f) return out;
+    R tol = fmax_<R>((R)1e-6f, (R)1e-4f * (bound + radius));
+    double t = 0;
+    Shape<R> hit = from;
+    bool found = false;
+    while (t < 1.) {
+        hit = shapeLerp(from, to, (R)t);
+        R d = shapeDistance(hit, x0 * ((R)1 - (R)t) + x1 * (R)t, normal, surface) - radius;
+        if (d <= tol) { found = true; break; }
+        if (speed <= (R)1e-12f) break;
+        R step = (R)0.8f * d / speed;
+        if ((R)t + step >= (R)1) {
+            t = 1; hit = to;
+            R de = shapeDistance(to, x1, normal, surface) - radius;
+            found = de <= tol;
+            break;
+        }
+        t += step;
+    }
+    if (!found) return out;
+    V3<R> end = witnessAt(hit, to, surface), start = witnessAt(hit, from, surface);
+    if (to.capsule) normal = rotateShaftVector(hit, to, normal);
+    else normal = unitOr(to.u * dot3(normal, hit.u) + to.n * dot3(normal, hit.n) + to.w * dot3(normal, hit.w), normal);
+    out.normal = normal; out.target = end + normal * radius; out.motion = end - start; out.valid = 1;
+    return out;
+}
+
+// Möller–Trumbore of the deformed edge against one swept triangle (cut.cuh:162-185).
+template <class R> bool mollerTrumbore(V3<R> O, V3<R> D, V3<R> V0, V3<R> V1, V3<R> V2, R& t, R& u, R& v) {
+    V3<R> L1 = V1 - V0, L2 = V2 - V0, T = V0 - O;
+    R det = dot3(cross3(L2, D), L1);
+    t = 0; u = 0; v = 0;
+    if (std::fabs(det) < (R)1e-12f) return false;
+    R inv = (R)1 / det;
+    t = dot3(cross3(L2, T), L1) * inv;
+    u = -dot3(cross3(L2, D), T) * inv;
+    v = dot3(cross3(D, T), L1) * inv;
+    const R EPS = (R)1e-5f;
+    return u >= -EPS && v >= -EPS && u + v <= (R)1 + EPS && t > 0 && t <= (R)1;
+}
+
+struct CutPoint {  // CutPointGpu (cut.cuh:37-46): 32 bytes, rotation taken as identity
+    float off[3];
+    int owner;
+    int edge;
+    float nrm[3];
+};
+
+template <class R> struct Tissue {
+    OracleDesc d{};
+    int N = 0, V = 0, E = 0, T = 0, nx = 0, ny = 0, nz = 0;
+    R sp[3] = {0, 0, 0};
+    // node state (AoS, as the reference's float3 buffers)
+    std::vector<V3<R>> pos, prev, vel, ext, rest;
+    std::vector<R> mass, invMass;
+    std::vector<int> active, pinned, inside, nbr, nbr0;
+    // bend pairs (grid.py BEND_DTYPE / BendPairGpuC): only (j, k, alive) are read on this path
+    std::vector<int> bendJ, bendK, bendAlive;
+    // edges, colour-ordered (xpbd_solver.cu:1230-1249)
+    std::vector<int> ei, ej, eowner, eactive, ecolorOff, ecolorCnt, egenIndex;
+    std::vector<R> erest, elambda;
+    std::vector<int> support;  // [6N] tet support of each owner slot (xpbd_solver.cu:469-479)
+    // tets, generation order (cell z,y,x major, Kuhn type minor) plus the colour-major list (:1740-1768)
+    std::vector<int> tids, ttype, towner, tactive, tcolor, tcolorOff, tcolorCnt, tcolorOrder;
+    std::vector<R> kDev, kHyd, alphaDev, alphaHyd, lamDev, lamHyd;
+    R typeB[6][3][3];
+    double typeV[6];
+    // frame constants (xpbd_begin_frame)
+    R h = 0, alphaTilde = 0, velScale = 1, betaGammaH = 0;
+    bool frameReady = false;
+    int substepIndex = 0;
+    // tool
+    ToolR<R> tool{}, toolFrom{};
+    bool toolArmed = false, bladeArmed = false;
+    R reaction[3] = {0, 0, 0};
+    int reactionContacts = 0;
+    // cut (cut.cu:1105-1206)
+    std::vector<int> gridEdges;           // [4*Eg] axis, x, y, z
+    std::vector<uint32_t> occupied, cutMask;
+    std::vector<float> edgeCutT, edgeSkinT;
+    std::vector<CutPoint> cutPoints;
+    uint32_t cutPointCounter = 0, revision = 0, refreshedRevision = 0xffffffffu;
+    int status = 0;
+
+    int cid(int x, int y, int z) const { return x + nx * (y + ny * z); }
+};
+
+// Greedy first-free colouring on a 64-bit per-vertex mask (xpbd_solver.cu:1213-1230, :1740-1751).
+template <int K>
+int greedyColour(int count, int vertices, const std::vector<int>& ids, std::vector<int>& colour, int& numColours) {
+    std::vector<unsigned long long> vmask(vertices, 0ull);
+    colour.assign(count, 0);
+    numColours = 0;
+    for (int e = 0; e < count; e++) {
+        unsigned long long used = 0;
+        for (int k = 0; k < K; k++) used |= vmask[ids[K * e + k]];
+        int c = 0;
+        while (c < 64 && (used & (1ull << c))) c++;
+        if (c >= 64) return -1003;  // XPBD_ERR_COLOR_OVERFLOW
+        colour[e] = c;
+        for (int k = 0; k < K; k++) vmask[ids[K * e + k]] |= (1ull << c);
+        if (c + 1 > numColours) numColours = c + 1;
+    }
+    return 0;
+}
+
+template <class R> bool inv3(const double m[3][3], double out[3][3]) {  // host double inverse (xpbd_solver.cu:1580-1598)
+    double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+                 m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    if (std::fabs(det) < 1e-30) return false;
+    double id = 1.0 / det;
+    out[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) * id;
+    out[0][1] = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]) * id;
+    out[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) * id;
+    out[1][0] = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]) * id;
+    out[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) * id;
+    out[1][2] = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]) * id;
+    out[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) * id;
+    out[2][1] = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]) * id;
+    out[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) * id;
+    return true;
+}
+
+template <class R> int buildTopology(Tissue<R>& s) {
+    // Edges: one per positive slot, p ascending, axis 0..2; L0 = |restNbr| (xpbd_solver.cu:1445-1463).
+    std::vector<int> ids, owner;
+    std::vector<R> rl;
+    for (int p = 0; p < s.N; p++)
+        for (int axis = 0; axis < 3; axis++) {
+            int slot = 2 * axis + 1, n = s.nbr[6 * p + slot];
+            if (n < 0) continue;
+            ids.push_back(p);
+            ids.push_back(n);
+            owner.push_back(6 * p + slot);
+        }
+    s.E = (int)owner.size();
+    std::vector<int> colour;
+    int nc = 0;
+    if (greedyColour<2>(s.E, s.N, ids, colour, nc)) return -1003;
+    s.ecolorCnt.assign(nc, 0);
+    for (int e = 0; e < s.E; e++) s.ecolorCnt[colour[e]]++;
+    s.ecolorOff.assign(nc, 0);
+    for (int c = 1; c < nc; c++) s.ecolorOff[c] = s.ecolorOff[c - 1] + s.ecolorCnt[c - 1];
+    s.ei.resize(s.E); s.ej.resize(s.E); s.eowner.resize(s.E); s.erest.resize(s.E); s.egenIndex.resize(s.E);
+    std::vector<int> cursor = s.ecolorOff;
+    for (int e = 0; e < s.E; e++) {
+        int dst = cursor[colour[e]]++;
+        s.ei[dst] = ids[2 * e];
+        s.ej[dst] = ids[2 * e + 1];
+        s.eowner[dst] = owner[e];
+        s.egenIndex[dst] = e;
+    }
+    s.eactive.assign(s.E, 1);
+    s.elambda.assign(s.E, 0);
+    s.support.assign(6 * s.N, 0);
+
+    // Kuhn tets of all-active cells; Dm^-1 and V per type from the rest lattice spacing in double
+    // (xpbd_solver.cu:1646-1660), each rest tet checked against its type (:1701, :1785-1792).
+    double spacing[3] = {((double)s.rest[s.cid(s.d.vx, 0, 0)].x - (double)s.rest[0].x) / s.d.vx,
+                         ((double)s.rest[s.cid(0, s.d.vy, 0)].y - (double)s.rest[0].y) / s.d.vy,
+                         ((double)s.rest[s.cid(0, 0, s.d.vz)].z - (double)s.rest[0].z) / s.d.vz};
+    double tB[6][3][3];
+    for (int ti = 0; ti < 6; ti++) {
+        double Dm[3][3];
+        for (int k = 1; k < 4; k++)
+            for (int a = 0; a < 3; a++) Dm[a][k - 1] = ((KUHN[ti][k] >> a) & 1) * spacing[a];
+        if (!inv3<R>(Dm, tB[ti])) return -1005;
+        double det = Dm[0][0] * (Dm[1][1] * Dm[2][2] - Dm[1][2] * Dm[2][1]) -
+                     Dm[0][1] * (Dm[1][0] * Dm[2][2] - Dm[1][2] * Dm[2][0]) +
+                     Dm[0][2] * (Dm[1][0] * Dm[2][1] - Dm[1][1] * Dm[2][0]);
+        s.typeV[ti] = std::fabs(det) / 6.0;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) s.typeB[ti][a][b] = (R)(float)tB[ti][a][b];
+    }
+    s.tids.clear(); s.ttype.clear(); s.towner.clear();
+    for (int z = 0; z < s.d.vz; z++)
+        for (int y = 0; y < s.d.vy; y++)
+            for (int x = 0; x < s.d.vx; x++) {
+                bool all = true;
+                for (int li = 0; li < 8 && all; li++)
+                    if (!s.active[s.cid(x + (li & 1), y + ((li >> 1) & 1), z + ((li >> 2) & 1))]) all = false;
+                if (!all) continue;
+                for (int ti = 0; ti < 6; ti++) {
+                    int g[4];
+                    for (int k = 0; k < 4; k++) {
+                        int li = KUHN[ti][k];
+                        g[k] = s.cid(x + (li & 1), y + ((li >> 1) & 1), z + ((li >> 2) & 1));
+                    }
+                    double e[3][3];
+                    for (int k = 0; k < 3; k++) {
+                        e[k][0] = (double)s.rest[g[k + 1]].x - (double)s.rest[g[0]].x;
+                        e[k][1] = (double)s.rest[g[k + 1]].y - (double)s.rest[g[0]].y;
+                        e[k][2] = (double)s.rest[g[k + 1]].z - (double)s.rest[g[0]].z;
+                    }
+                    double Dm[3][3] = {{e[0][0], e[1][0], e[2][0]}, {e[0][1], e[1][1], e[2][1]}, {e[0][2], e[1][2], e[2][2]}};
+                    double B[3][3];
+                    if (!inv3<R>(Dm, B)) continue;
+                    double scale = 0, dev = 0;
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) {
+                            scale = std::fmax(scale, std::fabs(tB[ti][a][b]));
+                            dev = std::fmax(dev, std::fabs(B[a][b] - tB[ti][a][b]));
+                        }
+                    if (dev > 1e-4 * scale) return -1005;
+                    int axes = 0;
+                    int own[3];
+                    for (int a = 0; a < 4 && axes < 3; a++)
+                        for (int b = a + 1; b < 4 && axes < 3; b++) {
+                            int lu = KUHN[ti][a], lv = KUHN[ti][b], diff = lu ^ lv;
+                            if (diff != 1 && diff != 2 && diff != 4) continue;
+                            int axis = diff == 1 ? 0 : (diff == 2 ? 1 : 2);
+                            int lower = ((lu >> axis) & 1) == 0 ? lu : lv;
+                            int lg = s.cid(x + (lower & 1), y + ((lower >> 1) & 1), z + ((lower >> 2) & 1));
+                            own[axes++] = 6 * lg + 2 * axis + 1;
+                        }
+                    for (int k = 0; k < 4; k++) s.tids.push_back(g[k]);
+                    s.ttype.push_back(ti);
+                    for (int k = 0; k < 3; k++) s.towner.push_back(own[k]);
+                }
+            }
+    s.T = (int)s.ttype.size();
+    int tnc = 0;
+    if (greedyColour<4>(s.T, s.N, s.tids, s.tcolor, tnc)) return -1003;
+    s.tcolorCnt.assign(tnc, 0);
+    for (int e = 0; e < s.T; e++) s.tcolorCnt[s.tcolor[e]]++;
+    s.tcolorOff.assign(tnc, 0);
+    for (int c = 1; c < tnc; c++) s.tcolorOff[c] = s.tcolorOff[c - 1] + s.tcolorCnt[c - 1];
+    s.tcolorOrder.assign(s.T, 0);
+    std::vector<int> tcur = s.tcolorOff;
+    for (int e = 0; e < s.T; e++) s.tcolorOrder[tcur[s.tcolor[e]]++] = e;
+    s.tactive.assign(s.T, 1);
+    s.kDev.assign(s.T, (R)1); s.kHyd.assign(s.T, (R)1);
+    s.alphaDev.assign(s.T, 0); s.alphaHyd.assign(s.T, 0);
+    s.lamDev.assign(s.T, 0); s.lamHyd.assign(s.T, 0);
+    return 0;
+}
+
+// k_RefreshLiveness (xpbd_solver.cu:429-466) + fallback support (:469-479); run when the topology revision moved.
+template <class R> void refreshLiveness(Tissue<R>& s) {
+    if (s.refreshedRevision == s.revision) return;
+    s.refreshedRevision = s.revision;
+    for (int e = 0; e < s.E; e++)
+        s.eactive[e] = (s.nbr[s.eowner[e]] >= 0 && s.active[s.ei[e]] != 0 && s.active[s.ej[e]] != 0) ? 1 : 0;
+    for (int t = 0; t < s.T; t++) {
+        bool live = s.nbr[s.towner[3 * t]] >= 0 && s.nbr[s.towner[3 * t + 1]] >= 0 && s.nbr[s.towner[3 * t + 2]] >= 0;
+        for (int k = 0; k < 4; k++) live = live && s.active[s.tids[4 * t + k]] != 0;
+        s.tactive[t] = live ? 1 : 0;
+    }
+    std::fill(s.support.begin(), s.support.end(), 0);
+    for (int t = 0; t < s.T; t++)
+        if (s.tactive[t])
+            for (int k = 0; k < 3; k++) s.support[s.towner[3 * t + k]] = 1;
+}
+
+// xpbd_begin_frame (xpbd_solver.cu:2481-2514) + k_ComputeTetAlpha (:578-587).
+template <class R> int beginFrame(Tissue<R>& s, R dt) {
+    int N = s.d.substeps > 0 ? s.d.substeps : 1;
+    R h = dt / (R)N;
+    if (!(h > 0) || !std::isfinite(h)) return -1002;
+    R ks = (R)s.d.edge_ks;
+    R at = ks > 0 ? (R)1 / (ks * h * h) : (R)0;
+    s.velScale = ((R)1 - (R)s.d.damping) / ((R)1 + (R)s.d.rayleigh * h);
+    s.betaGammaH = at * (R)s.d.beta_s * h;
+    s.alphaTilde = at;
+    s.h = h;
+    R invH2 = (R)1 / (h * h);
+    for (int t = 0; t < s.T; t++) {
+        s.alphaDev[t] = invH2 / s.kDev[t];
+        s.alphaHyd[t] = invH2 / s.kHyd[t];
+    }
+    s.substepIndex = 0;
+    s.frameReady = true;
+    return 0;
+}
+
+// k_solve_edges (xpbd_solver.cu:484-524) on one edge.
+template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallback, R fallbackInvEh2, bool useLambda) {
+    if (s.eactive[e] == 0) return;
+    if (fallback) {
+        if (s.support[s.eowner[e]]) return;
+        alphaTilde = fallbackInvEh2 / s.erest[e];
+    }
+    int i = s.ei[e], j = s.ej[e];
+    R wi = s.invMass[i], wj = s.invMass[j], wsum = wi + wj;
+    if (wsum <= 0) return;
+    V3<R> dd = s.pos[i] - s.pos[j];
+    R L = len3(dd);
+    if (L < (R)1e-9f) return;
+    V3<R> n = dd * ((R)1 / L);
+    R C = L - s.erest[e];
+    R gamma = s.betaGammaH;
+    R gradDx = dot3(n, (s.pos[i] - s.prev[i]) - (s.pos[j] - s.prev[j]));
+    R lam = useLambda ? s.elambda[e] : (R)0;
+    R dl = (-C - alphaTilde * lam - gamma * gradDx) / (((R)1 + gamma) * wsum + alphaTilde);
+    R applied = (R)s.d.omega * dl;
+    s.elambda[e] = lam + applied;
+    s.pos[i] = s.pos[i] + n * (wi * applied);
+    s.pos[j] = s.pos[j] - n * (wj * applied);
+}
+
+// k_solve_tet (xpbd_solver.cu:749-829) on one tet. Mode 0 keeps the hydrostatic row alone (the deviatoric row and
+// its multiplier are absent, which is the 2x2 block with gd = 0); modes 1/2 are the coupled block, legacy / guarded.
+// k_solve_tet (xpbd_solver.cu:749-829) on one tet as a pure function of its inputs. Mode 0 keeps the hydrostatic row
+// alone (the deviatoric row and its multiplier are absent: the 2x2 block with gd = 0); modes 1/2 are the coupled block,
+// legacy / guarded. Returns false where the reference returns without writing.
+template <class R>
+bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], const V3<R> br[3], R ad, R ah,
+                R& lamD, R& lamH, R dampingOverH, R bulkDampingOverH, R omega, V3<R> out[4]) {
+    V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
+    V3<R> F[3];
+    for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
+    V3<R> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
+    R J = dot3(F[0], cof[0]);
+    const bool safeguarded = mode == 2;
+    V3<R> gh[4], gd[4];
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
+    }
+    R scale = (R)1 + dampingOverH;
+    if (mode == 0) {
+        R CH = J - (R)1;
+        gh[0] = (gh[1] + gh[2] + gh[3]) * (R)-1;
+        R bulkScale = scale;
+        R effectiveAh = ah / bulkScale;
+        R vh = 0;
+        R schur = effectiveAh;
+        for (int k = 0; k < 4; k++) vh += dot3(gh[k], p[k] - pv[k]);
+        for (int k = 0; k < 4; k++) schur += w[k] * dot3(gh[k], gh[k]);
+        R rh = (-CH - ah * lamH - (bulkScale - (R)1) * vh) / bulkScale;
+        if (!(schur > 0)) return false;
+        R dlH = omega * rh / schur;
+        if (!std::isfinite(dlH)) return false;
+        lamH = lamH + dlH;
+        for (int k = 0; k < 4; k++) out[k] = p[k] + gh[k] * dlH * w[k];
+        return true;
+    }
+    R I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    R positiveJ = fmax_<R>(J, (R)0.05f);
+    R iso = isoPow(positiveJ);
+    R q = fmax_<R>(iso * I1 - (R)3, (R)0);
+    R CD = std::sqrt(fmax_<R>(q, (R)1e-12f));
+    R devScale = (R)1 / CD;
+    R volumeGradient = J > (R)0.05f ? iso * I1 / ((R)3 * positiveJ) : (R)0;
+    R CH = safeguarded ? std::log(positiveJ) + fmin_<R>((R)0, (J - (R)0.05f) / (R)0.05f) : J - (R)1;
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        gd[k] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[k] * volumeGradient) * devScale;
+        if (safeguarded) gh[k] = gh[k] * ((R)1 / positiveJ);
+    }
+    gd[0] = (gd[1] + gd[2] + gd[3]) * (R)-1;
+    gh[0] = (gh[1] + gh[2] + gh[3]) * (R)-1;
+    R dd = 0, dh = 0, vd = 0, vh = 0;
+    for (int k = 0; k < 4; k++) {
+        dd += w[k] * dot3(gd[k], gd[k]);
+        dh += w[k] * dot3(gd[k], gh[k]);
+        V3<R> dx = p[k] - pv[k];
+        vd += dot3(gd[k], dx);
+        vh += dot3(gh[k], dx);
+    }
+    R bulkScale = safeguarded ? (R)1 + bulkDampingOverH : scale;
+    R effectiveAd = ad / scale, effectiveAh = ah / bulkScale;
+    R A = dd + effectiveAd, Bc = dh, gamma = Bc / A;
+    R schur = effectiveAh + effectiveAd * gamma * gamma;
+    for (int k = 0; k < 4; k++) {
+        V3<R> residual = gh[k] - gd[k] * gamma;
+        schur += w[k] * dot3(residual, residual);
+    }
+    R rd = (-CD - ad * lamD - dampingOverH * vd) / scale;
+    R rh = (-CH - ah * lamH - (bulkScale - (R)1) * vh) / bulkScale;
+    if (!(A > 0) || !(schur > 0)) return false;
+    R dlH = omega * (rh - gamma * rd) / schur, dlD = (omega * rd - Bc * dlH) / A;
+    if (!std::isfinite(dlD) || !std::isfinite(dlH)) return false;
+    if (safeguarded) {
+        R largest = 0;
+        for (int k = 0; k < 4; k++) largest = fmax_<R>(largest, len3((gd[k] * dlD + gh[k] * dlH) * w[k]));
+        R limit = (R)0.15f / fmax_<R>(len3(br[0]), fmax_<R>(len3(br[1]), len3(br[2])));
+        R accepted = largest > limit ? limit / largest : (R)1;
+        dlD *= accepted;
+        dlH *= accepted;
+    }
+    lamD = lamD + dlD;
+    lamH = lamH + dlH;
+    for (int k = 0; k < 4; k++) out[k] = p[k] + (gd[k] * dlD + gh[k] * dlH) * w[k];
+    return true;
+}
+
+template <class R> void solveTet(Tissue<R>& s, int e, R dampingOverH, R bulkDampingOverH, bool useLambda) {
+    if (s.tactive[e] == 0) return;
+    const int* id = &s.tids[4 * e];
+    V3<R> p[4], pv[4], out[4];
+    R w[4];
+    for (int k = 0; k < 4; k++) { p[k] = s.pos[id[k]]; pv[k] = s.prev[id[k]]; w[k] = s.invMass[id[k]]; }
+    const R(*B)[3] = s.typeB[s.ttype[e]];
+    V3<R> br[3] = {mk<R>(B[0][0], B[0][1], B[0][2]), mk<R>(B[1][0], B[1][1], B[1][2]), mk<R>(B[2][0], B[2][1], B[2][2])};
+    R lamD = useLambda ? s.lamDev[e] : (R)0, lamH = useLambda ? s.lamHyd[e] : (R)0;
+    if (!tetProject<R>(s.d.tet_mode, p, pv, w, br, s.alphaDev[e], s.alphaHyd[e], lamD, lamH, dampingOverH,
+                       bulkDampingOverH, (R)s.d.omega, out))
+        return;
+    s.lamDev[e] = lamD;
+    s.lamHyd[e] = lamH;
+    for (int k = 0; k < 4; k++) s.pos[id[k]] = out[k];
+}
+
+template <class R> ToolR<R> lerpTool(const ToolR<R>& from, const ToolR<R>& to, R t) {  // xpbd_solver.cu:3467-3483
+    ToolR<R> r = to;
+    r.jawHinge = from.jawHinge * ((R)1 - t) + to.jawHinge * t;
+    r.jawAxis = from.jawAxis * ((R)1 - t) + to.jawAxis * t;
+    r.jawNormal = from.jawNormal * ((R)1 - t) + to.jawNormal * t;
+    r.shaftA = from.shaftA * ((R)1 - t) + to.shaftA * t;
+    r.shaftB = from.shaftB * ((R)1 - t) + to.shaftB * t;
+    r.jawGap = from.jawGap * ((R)1 - t) + to.jawGap * t;
+    r.bladeA = from.bladeA * ((R)1 - t) + to.bladeA * t;
+    r.bladeB = from.bladeB * ((R)1 - t) + to.bladeB * t;
+    return r;
+}
+
+// One substep of the legacy branch (xpbd_solver.cu:3460-3721 with K = 1, no grasp, no bending, no guard passes),
+// friction tool contact of lattice nodes (:3659-3665, friction_contact.cuh:143-169).
+template <class R> int substep(Tissue<R>& s) {
+    if (!s.frameReady) return -1001;
+    R h = s.h;
+    int Nsub = s.d.substeps > 0 ? s.d.substeps : 1;
+    R t = fmin_<R>((R)1, (R)(++s.substepIndex) / (R)Nsub);
+    R previousT = fmax_<R>((R)0, t - (R)1 / (R)Nsub);
+    ToolR<R> previousTool = lerpTool(s.toolFrom, s.tool, previousT);
+    ToolR<R> stepTool = lerpTool(s.toolFrom, s.tool, t);
+    refreshLiveness(s);
+    // k_predict (:393-412)
+    for (int p = 0; p < s.N; p++) {
+        s.prev[p] = s.pos[p];
+        if (s.invMass[p] > 0) {
+            V3<R> a = s.ext[p] * s.invMass[p];
+            V3<R> v = s.vel[p] + a * h;
+            s.vel[p] = v;
+            s.pos[p] = s.pos[p] + v * h;
+        }
+    }
+    const bool strain = s.T > 0;
+    const bool fallback = strain && s.d.edge_ks <= 0;
+    const bool edges = s.d.edge_ks > 0 || fallback;
+    R fallbackInvEh2 = fallback ? (R)1 / ((R)s.d.young * h * h) : (R)0;
+    R dampingOverH = fmax_<R>((R)0, (R)s.d.strain_damping) / h;
+    R bulkOverH = h > 0 ? (R)s.d.bulk_damping / h : (R)0;
+    int iters = s.d.iters > 0 ? s.d.iters : 1;
+    for (int it = 0; it < iters; it++) {
+        bool useLambda = it > 0;  // the multipliers were reset at the start of the substep (:3534-3563)
+        if (edges)
+            for (size_t c = 0; c < s.ecolorCnt.size(); c++) {
+                int off = s.ecolorOff[c], cnt = s.ecolorCnt[c];
+#pragma omp parallel for schedule(static) if (cnt > 4096)
+                for (int k = 0; k < cnt; k++) solveEdge(s, off + k, s.alphaTilde, fallback, fallbackInvEh2, useLambda);
+            }
+        if (strain)
+            for (size_t c = 0; c < s.tcolorCnt.size(); c++) {
+                int off = s.tcolorOff[c], cnt = s.tcolorCnt[c];
+#pragma omp parallel for schedule(static) if (cnt > 2048)
+                for (int k = 0; k < cnt; k++) solveTet(s, s.tcolorOrder[off + k], dampingOverH, bulkOverH, useLambda);
+            }
+    }
+    // friction contact: k_FindToolContacts x kinds, k_SolveFrictionContacts x2, k_NodeReaction (:3659-3665)
+    if (s.d.contact_enabled && (s.toolArmed || s.bladeArmed)) {
+        const int kinds = 4;
+        Shape<R> from[kinds], to[kinds];
+        for (int kind = 0; kind < kinds; kind++) {
+            from[kind] = contactShape(previousTool, kind, (R)s.d.jaw_thickness, (R)s.d.jaw_gap_closed);
+            to[kind] = contactShape(stepTool, kind, (R)s.d.jaw_thickness, (R)s.d.jaw_gap_closed);
+        }
+        R alpha = (R)s.d.contact_compliance / (h * h);
+        R muS = (R)s.d.mu_s, muD = (R)s.d.mu_d, radius = (R)s.d.contact_radius;
+        R scaleR = (R)1 / (h * h * fmax_<R>((R)1, (R)Nsub));
+        for (int i = 0; i < s.N; i++) {
+            if (!(s.invMass[i] > 0)) continue;
+            Record<R> rec[kinds];
+            R lamN[kinds];
+            V3<R> lamT[kinds];
+            for (int kind = 0; kind < kinds; kind++) {
+                rec[kind] = Record<R>{};
+                lamN[kind] = 0;
+                lamT[kind] = mk<R>(0, 0, 0);
+                bool armed = kind == 3 ? s.bladeArmed : s.toolArmed;
+                bool present = to[kind].capsule ? to[kind].radius > 0 : (to[kind].half.x > 0 && to[kind].half.z > 0);
+                if (armed && present) rec[kind] = traceToolContact(from[kind], to[kind], radius, s.prev[i], s.pos[i]);
+            }
+            R w = s.invMass[i];
+            for (int pass = 0; pass < 2; pass++)
+                for (int kind = 0; kind < kinds; kind++) {
+                    const Record<R>& c = rec[kind];
+                    if (!c.valid) continue;
+                    R C = dot3(s.pos[i] - c.target, c.normal), old = lamN[kind];
+                    R next = fmax_<R>((R)0, old + (-C - alpha * old) / (w + alpha));
+                    s.pos[i] = s.pos[i] + c.normal * (w * (next - old));
+                    lamN[kind] = next;
+                    V3<R> slip = s.pos[i] - s.prev[i] - c.motion;
+                    slip = slip - c.normal * dot3(slip, c.normal);
+                    V3<R> oldT = lamT[kind];
+                    V3<R> trial = oldT - slip * ((R)1 / w);
+                    R len = len3(trial);
+                    V3<R> nextT = trial;
+                    if (len > muS * next) nextT = len > (R)1e-15f ? trial * (muD * next / len) : mk<R>(0, 0, 0);
+                    s.pos[i] = s.pos[i] + (nextT - oldT) * w;
+                    lamT[kind] = nextT;
+                }
+            for (int kind = 0; kind < kinds; kind++)
+                if (rec[kind].valid) {  // k_NodeReaction: force of the tissue on the tool
+                    V3<R> f = (rec[kind].normal * lamN[kind] + lamT[kind]) * (-scaleR);
+                    s.reaction[0] += f.x; s.reaction[1] += f.y; s.reaction[2] += f.z;
+                    if (lamN[kind] > 0) s.reactionContacts++;
+                }
+        }
+    }
+    // k_update_velocity, [k_perp_damp_edges], k_apply_velscale (:871-919)
+    R invH = (R)1 / h;
+    for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? (s.pos[p] - s.prev[p]) * invH : mk<R>(0, 0, 0);
+    if (s.d.beta_s > 0)
+        for (size_t c = 0; c < s.ecolorCnt.size(); c++)
+            for (int k = 0; k < s.ecolorCnt[c]; k++) {
+                int e = s.ecolorOff[c] + k;
+                if (s.eactive[e] == 0) continue;
+                int i = s.ei[e], j = s.ej[e];
+                R wi = s.invMass[i], wj = s.invMass[j];
+                if (wi + wj <= 0) continue;
+                V3<R> dd = s.pos[i] - s.pos[j];
+                R L = len3(dd);
+                if (L < (R)1e-9f) continue;
+                V3<R> n = dd * ((R)1 / L);
+                V3<R> vrel = s.vel[i] - s.vel[j];
+                V3<R> vp = vrel - n * dot3(vrel, n);
+                V3<R> imp = vp * (h * (R)s.d.beta_s);
+                if (wi > 0) s.vel[i] = s.vel[i] - imp * wi;
+                if (wj > 0) s.vel[j] = s.vel[j] + imp * wj;
+            }
+    for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? s.vel[p] * s.velScale : mk<R>(0, 0, 0);
+    if (s.substepIndex >= Nsub) s.toolFrom = s.tool;
+    return 0;
+}
+
+template <class R> int step(Tissue<R>& s, R dt) {  // physics_step (physics.cu:331-394), no ribbon tick
+    int rc = beginFrame(s, dt);
+    if (rc) return rc;
+    int N = s.d.substeps > 0 ? s.d.substeps : 1;
+    s.reaction[0] = s.reaction[1] = s.reaction[2] = 0;
+    s.reactionContacts = 0;
+    for (int n = 0; n < N; n++) {
+        rc = substep(s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+// ES_VoxelOffset / ES_LocalEdge (cut.cuh:126-137).
+const int ES_OFF[3][4][3] = {{{0, 0, 0}, {0, -1, 0}, {0, -1, -1}, {0, 0, -1}},
+                             {{0, 0, 0}, {0, 0, -1}, {-1, 0, -1}, {-1, 0, 0}},
+                             {{0, 0, 0}, {-1, 0, 0}, {-1, -1, 0}, {0, -1, 0}}};
+const int ES_EDGE[3][4] = {{0, 2, 6, 4}, {3, 7, 5, 1}, {8, 9, 10, 11}};
+
+template <class R> bool voxIn(const Tissue<R>& s, int x, int y, int z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < s.d.vx && y < s.d.vy && z < s.d.vz;
+}
+template <class R> int voxId(const Tissue<R>& s, int x, int y, int z) { return x + s.d.vx * (y + s.d.vy * z); }
+
+inline int directedSlot(int a, int b, int vx) {  // DirectedEdgeSlot (cut.cuh:25-28)
+    int dlt = b - a, ad = dlt < 0 ? -dlt : dlt;
+    int axis = ad == 1 ? 0 : (ad == vx + 1 ? 1 : 2);
+    return 6 * a + 2 * axis + (dlt > 0 ? 1 : 0);
+}
+
+inline bool cutHitsMaterial(float t, float skinT, int insideA, int insideB) {  // CutHitsMaterial (cut.cuh:17-22)
+    if (insideA && insideB) return true;
+    if (!insideA && !insideB) return false;
+    if (!(skinT >= 0.f && skinT <= 1.f)) return false;
+    return insideA ? t <= skinT : t >= skinT;
+}
+
+// MarkAndEmit + EmitCutPoint (cut.cu:186-254); the particle rotation is identity on this path.
+template <class R> void markAndEmit(Tissue<R>& s, int e, int axis, const int bc[3], int idA, int idB, R t, V3<R> hit,
+                                    V3<R> ncut, R toolD) {
+    bool firstCut = false, sawAny = false, occupiedEdge = false;
+    for (int k = 0; k < 4; k++) {
+        int x = bc[0] + ES_OFF[axis][k][0], y = bc[1] + ES_OFF[axis][k][1], z = bc[2] + ES_OFF[axis][k][2];
+        if (!voxIn(s, x, y, z)) continue;
+        int vid = voxId(s, x, y, z);
+        if (s.occupied[vid] == 0u) continue;
+        occupiedEdge = true;
+        uint32_t bit = 1u << ES_EDGE[axis][k];
+        uint32_t old = s.cutMask[vid];
+        s.cutMask[vid] = old | bit;
+        if (!sawAny) { sawAny = true; firstCut = (old & bit) == 0u; }
+    }
+    if (!(firstCut && occupiedEdge)) return;
+    s.edgeCutT[directedSlot(idA, idB, s.d.vx)] = (float)t;
+    s.edgeCutT[directedSlot(idB, idA, s.d.vx)] = (float)((R)1 - t);
+    if (!cutHitsMaterial((float)t, s.edgeSkinT[directedSlot(idA, idB, s.d.vx)], s.inside[idA], s.inside[idB])) return;
+    R L = (R)s.d.tool_voxel;
+    V3<R> axisF = mk<R>(axis == 0 ? (R)1 : (R)0, axis == 1 ? (R)1 : (R)0, axis == 2 ? (R)1 : (R)0);
+    V3<R> restEdge = axisF * L;
+    V3<R> along[2] = {restEdge * t, restEdge * (t - (R)1)};
+    int owners[2] = {idA, idB};
+    for (int k = 0; k < 2; k++) {
+        int p = owners[k];
+        R side = dot3(s.pos[p] - hit, ncut);
+        side = side > 0 ? (R)1 : (side < 0 ? (R)-1 : (R)0);
+        if (side == 0) side = 1;
+        uint32_t slot = s.cutPointCounter++;
+        if (slot >= (uint32_t)s.d.cut_point_capacity) continue;
+        V3<R> gap = ncut * (side * (toolD * (R)0.5f));
+        V3<R> off = along[k] + gap;
+        V3<R> nrm = ncut * side;
+        CutPoint cp;
+        cp.off[0] = (float)off.x; cp.off[1] = (float)off.y; cp.off[2] = (float)off.z;
+        cp.owner = p;
+        cp.edge = e;
+        cp.nrm[0] = (float)nrm.x; cp.nrm[1] = (float)nrm.y; cp.nrm[2] = (float)nrm.z;
+        s.cutPoints.push_back(cp);
+    }
+}
+
+// SeverLink (cut.cu:719-731).
+template <class R> void severLink(Tissue<R>& s, int a, int b, int axis) {
+    if (s.nbr[6 * a + 2 * axis + 1] >= 0) s.revision++;
+    s.nbr[6 * a + 2 * axis + 1] = -1;
+    s.nbr[6 * b + 2 * axis] = -1;
+    for (int p = 0; p < 15; p++) {
+        int ia = 15 * a + p;
+        if (s.bendAlive[ia] != 0 && (s.bendJ[ia] == b || s.bendK[ia] == b)) s.bendAlive[ia] = 0;
+        int ib = 15 * b + p;
+        if (s.bendAlive[ib] != 0 && (s.bendJ[ib] == a || s.bendK[ib] == a)) s.bendAlive[ib] = 0;
+    }
+}
+
+template <class R> void severLinks(Tissue<R>& s) {  // k_SeverLinks (cut.cu:732-746), edges in list order
+    int Eg = s.d.grid_edge_count;
+    for (int e = 0; e < Eg; e++) {
+        int axis = s.gridEdges[4 * e], bc[3] = {s.gridEdges[4 * e + 1], s.gridEdges[4 * e + 2], s.gridEdges[4 * e + 3]};
+        bool valid = true, cut = false;
+        for (int k = 0; k < 4; k++) {
+            int x = bc[0] + ES_OFF[axis][k][0], y = bc[1] + ES_OFF[axis][k][1], z = bc[2] + ES_OFF[axis][k][2];
+            if (!voxIn(s, x, y, z)) { valid = false; continue; }
+            if ((s.cutMask[voxId(s, x, y, z)] >> ES_EDGE[axis][k]) & 1u) cut = true;
+        }
+        if (!valid || !cut) continue;
+        int a = s.cid(bc[0], bc[1], bc[2]);
+        int b = s.cid(bc[0] + (axis == 0), bc[1] + (axis == 1), bc[2] + (axis == 2));
+        severLink(s, a, b, axis);
+    }
+}
+
+// k_DetectCutQuad (cut.cu:421-474) + k_SeverLinks: cut_detect (cut.cu:1235-1254).
+template <class R> void detectCutQuad(Tissue<R>& s, const OracleCutTool& tl) {
+    if (!tl.valid) return;
+    V3<R> t1v0 = v3<R>(tl.t1v0), t1v1 = v3<R>(tl.t1v1), t1v2 = v3<R>(tl.t1v2);
+    V3<R> t2v0 = v3<R>(tl.t2v0), t2v1 = v3<R>(tl.t2v1), t2v2 = v3<R>(tl.t2v2);
+    V3<R> ncut = v3<R>(tl.ncut), lo = v3<R>(tl.aabbMin), hi = v3<R>(tl.aabbMax);
+    R pad = (R)s.d.tool_voxel;
+    int Eg = s.d.grid_edge_count;
+    for (int e = 0; e < Eg; e++) {
+        int axis = s.gridEdges[4 * e], bc[3] = {s.gridEdges[4 * e + 1], s.gridEdges[4 * e + 2], s.gridEdges[4 * e + 3]};
+        int idA = s.cid(bc[0], bc[1], bc[2]);
+        int idB = s.cid(bc[0] + (axis == 0), bc[1] + (axis == 1), bc[2] + (axis == 2));
+        V3<R> O = s.pos[idA], D = s.pos[idB] - O;
+        V3<R> eMin = mk<R>(fmin_<R>(O.x, O.x + D.x), fmin_<R>(O.y, O.y + D.y), fmin_<R>(O.z, O.z + D.z));
+        V3<R> eMax = mk<R>(fmax_<R>(O.x, O.x + D.x), fmax_<R>(O.y, O.y + D.y), fmax_<R>(O.z, O.z + D.z));
+        if (eMax.x < lo.x - pad || eMin.x > hi.x + pad || eMax.y < lo.y - pad || eMin.y > hi.y + pad ||
+            eMax.z < lo.z - pad || eMin.z > hi.z + pad)
+            continue;
+        R ta, ua, va, tb, ub, vb;
+        bool h1 = mollerTrumbore(O, D, t1v0, t1v1, t1v2, ta, ua, va);
+        bool h2 = mollerTrumbore(O, D, t2v0, t2v1, t2v2, tb, ub, vb);
+        if (!h1 && !h2) continue;
+        R t = (h1 && h2) ? fmin_<R>(ta, tb) : (h1 ? ta : tb);
+        t = fmin_<R>(fmax_<R>(t, (R)0), (R)1);
+        V3<R> hit = O + D * t;
+        markAndEmit(s, e, axis, bc, idA, idB, t, hit, ncut, (R)tl.d);
+    }
+    severLinks(s);
+}
+
+// k_DetectCutCCD (cut.cu:486-592) of the moving edges against a static blade segment S->E, then k_SeverLinks.
+template <class R> void detectCutCCD(Tissue<R>& s, V3<R> S, V3<R> E, V3<R> emitNcut, R toolD,
+                                     const std::vector<V3<R>>& prevPos) {
+    R pad = (R)s.d.tool_voxel;
+    int Eg = s.d.grid_edge_count;
+    for (int e = 0; e < Eg; e++) {
+        int axis = s.gridEdges[4 * e], bc[3] = {s.gridEdges[4 * e + 1], s.gridEdges[4 * e + 2], s.gridEdges[4 * e + 3]};
+        int idA = s.cid(bc[0], bc[1], bc[2]);
+        int idB = s.cid(bc[0] + (axis == 0), bc[1] + (axis == 1), bc[2] + (axis == 2));
+        if (s.nbr[6 * idA + 2 * axis + 1] < 0) continue;
+        V3<R> a0 = prevPos[idA], a1 = s.pos[idA], b0 = prevPos[idB], b1 = s.pos[idB];
+        V3<R> sMin = mk<R>(fmin_<R>(S.x, E.x) - pad, fmin_<R>(S.y, E.y) - pad, fmin_<R>(S.z, E.z) - pad);
+        V3<R> sMax = mk<R>(fmax_<R>(S.x, E.x) + pad, fmax_<R>(S.y, E.y) + pad, fmax_<R>(S.z, E.z) + pad);
+        V3<R> eMin = mk<R>(fmin_<R>(fmin_<R>(a0.x, a1.x), fmin_<R>(b0.x, b1.x)),
+                           fmin_<R>(fmin_<R>(a0.y, a1.y), fmin_<R>(b0.y, b1.y)),
+                           fmin_<R>(fmin_<R>(a0.z, a1.z), fmin_<R>(b0.z, b1.z)));
+        V3<R> eMax = mk<R>(fmax_<R>(fmax_<R>(a0.x, a1.x), fmax_<R>(b0.x, b1.x)),
+                           fmax_<R>(fmax_<R>(a0.y, a1.y), fmax_<R>(b0.y, b1.y)),
+                           fmax_<R>(fmax_<R>(a0.z, a1.z), fmax_<R>(b0.z, b1.z)));
+        if (eMax.x < sMin.x || eMin.x > sMax.x || eMax.y < sMin.y || eMin.y > sMax.y || eMax.z < sMin.z ||
+            eMin.z > sMax.z)
+            continue;
+        V3<R> D = E - S, P0 = a0 - S, Pv = a1 - a0, Q0 = b0 - S, Qv = b1 - b0;
+        R c0 = dot3(cross3(P0, Q0), D);
+        R c1 = dot3(cross3(P0, Qv) + cross3(Pv, Q0), D);
+        R c2 = dot3(cross3(Pv, Qv), D);
+        R roots[2];
+        int nr = 0;
+        if (std::fabs(c2) < (R)1e-14f) {
+            if (std::fabs(c1) > (R)1e-14f) {
+                R r = -c0 / c1;
+                if (r >= 0 && r <= 1) roots[nr++] = r;
+            }
+        } else {
+            R disc = c1 * c1 - (R)4 * c2 * c0;
+            if (disc >= 0) {
+                R sq = std::sqrt(disc);
+                R q = (R)-0.5f * (c1 + (c1 >= 0 ? sq : -sq));
+                R r0 = q / c2, r1 = std::fabs(q) > (R)1e-20f ? c0 / q : r0;
+                if (r0 > r1) std::swap(r0, r1);
+                if (r0 >= 0 && r0 <= 1) roots[nr++] = r0;
+                if (r1 >= 0 && r1 <= 1 && r1 != r0) roots[nr++] = r1;
+            }
+        }
+        for (int ri = 0; ri < nr; ri++) {
+            R tc = roots[ri];
+            V3<R> a = a0 + Pv * tc, b = b0 + Qv * tc, e1 = b - a, n2 = cross3(e1, D);
+            R n2l = dot3(n2, n2);
+            if (n2l < (R)1e-16f) continue;
+            V3<R> w = S - a;
+            R u = dot3(cross3(w, D), n2) / n2l;
+            R v = dot3(cross3(w, e1), n2) / n2l;
+            const R EPS = (R)2e-2f;
+            if (u <= 0 || u > 1 || v < -EPS || v > (R)1 + EPS) continue;
+            V3<R> hit = a1 + (b1 - a1) * u;
+            V3<R> relV = (Pv + Qv) * (R)0.5f;
+            V3<R> nHit = cross3(D, relV);
+            R nLen = len3(nHit);
+            V3<R> nEmit = nLen > (R)1e-8f ? nHit * ((R)1 / nLen) : emitNcut;
+            markAndEmit(s, e, axis, bc, idA, idB, u, hit, nEmit, toolD);
+            break;
+        }
+    }
+    severLinks(s);
+}
+
+// Stable compaction of the live topology: old -> new index, -1 for removed (build side, no reference counterpart).
+// Tets are numbered in colour-major order, edges in colour order, nodes in id order; a node survives when it is
+// active and some live tet or live edge references it.
+template <class R> void compact(Tissue<R>& s, int* tetRemap, int* edgeRemap, int* nodeRemap, int* counts) {
+    refreshLiveness(s);
+    std::vector<int> used(s.N, 0);
+    int nt = 0;
+    for (int k = 0; k < s.T; k++) {
+        int t = s.tcolorOrder[k];
+        if (s.tactive[t]) {
+            tetRemap[k] = nt++;
+            for (int j = 0; j < 4; j++) used[s.tids[4 * t + j]] = 1;
+        } else tetRemap[k] = -1;
+    }
+    int ne = 0;
+    for (int e = 0; e < s.E; e++) {
+        if (s.eactive[e]) {
+            edgeRemap[e] = ne++;
+            used[s.ei[e]] = used[s.ej[e]] = 1;
+        } else edgeRemap[e] = -1;
+    }
+    int nn = 0;
+    for (int p = 0; p < s.N; p++) nodeRemap[p] = (used[p] && s.active[p]) ? nn++ : -1;
+    counts[0] = nt; counts[1] = ne; counts[2] = nn;
+}
+
+template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, const float* pos, const float* mass,
+                                     const int* pinned, const int* active, const int* inside, const int* nbr,
+                                     const int* occupied, const int* gridEdges, const float* edgeSkin,
+                                     const float* restNbr) {
+    auto* s = new Tissue<R>();
+    s->d = *d;
+    s->nx = d->vx + 1; s->ny = d->vy + 1; s->nz = d->vz + 1;
+    s->N = s->nx * s->ny * s->nz;
+    s->V = d->vx * d->vy * d->vz;
+    int N = s->N;
+    s->pos.resize(N); s->prev.resize(N); s->vel.resize(N); s->ext.resize(N); s->rest.resize(N);
+    s->mass.resize(N); s->invMass.resize(N);
+    for (int i = 0; i < N; i++) {
+        s->rest[i] = v3<R>(rest + 3 * i);
+        s->pos[i] = v3<R>(pos + 3 * i);
+        s->prev[i] = s->pos[i];
+        s->vel[i] = mk<R>(0, 0, 0);
+        s->ext[i] = mk<R>(0, 0, 0);
+        s->mass[i] = (R)mass[i];
+    }
+    s->active.assign(active, active + N);
+    s->pinned.assign(pinned, pinned + N);
+    s->inside.assign(inside, inside + N);
+    s->nbr.assign(nbr, nbr + 6 * N);
+    s->nbr0 = s->nbr;
+    for (int i = 0; i < N; i++)  // invMass rule of k_RefreshLiveness; active/pinned never change on this path
+        s->invMass[i] = (s->pinned[i] != 0 || s->active[i] == 0) ? (R)0 : (R)1 / s->mass[i];
+    s->bendJ.resize(15 * N); s->bendK.resize(15 * N); s->bendAlive.resize(15 * N);
+    for (int i = 0; i < N; i++)
+        for (int p = 0; p < 15; p++) {
+            int j = s->nbr[6 * i + BEND_PAIRS[p][0]], k = s->nbr[6 * i + BEND_PAIRS[p][1]];
+            s->bendJ[15 * i + p] = j;
+            s->bendK[15 * i + p] = k;
+            s->bendAlive[15 * i + p] = (j >= 0 && k >= 0) ? 1 : 0;
+        }
+    s->gridEdges.assign(gridEdges, gridEdges + 4 * d->grid_edge_count);
+    s->occupied.assign(occupied, occupied + s->V);
+    s->cutMask.assign(s->V, 0u);
+    s->edgeCutT.assign(6 * N, -1.f);
+    s->edgeSkinT.assign(edgeSkin, edgeSkin + 6 * N);
+    s->status = buildTopology(*s);
+    for (int e = 0; e < s->E; e++) {  // L0 = sqrtf(|restNbr[6p+s]|^2) in float (xpbd_solver.cu:1458)
+        const float* r = restNbr + 3 * s->eowner[e];
+        s->erest[e] = (R)std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    }
+    return s;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------------------------
+// C entry points (ctypes). The float handle is the parity oracle and the timed CPU baseline, the double handle the
+// accuracy oracle. Inputs are float arrays as the product receives them.
+// ---------------------------------------------------------------------------------------------------------------
+#define ORACLE_API(R, SUF)                                                                                           \
+    extern "C" void* oracle_create_##SUF(const OracleDesc* d, const float* rest, const float* pos, const float* mass, \
+                                         const int* pinned, const int* active, const int* inside, const int* nbr,     \
+                                         const int* occupied, const int* gridEdges, const float* edgeSkin,            \
+                                         const float* restNbr) {                                                      \
+        return create<R>(d, rest, pos, mass, pinned, active, inside, nbr, occupied, gridEdges, edgeSkin, restNbr);    \
+    }                                                                                                                 \
+    extern "C" int oracle_status_##SUF(void* h) { return ((Tissue<R>*)h)->status; }                                   \
+    extern "C" void oracle_destroy_##SUF(void* h) { delete (Tissue<R>*)h; }                                           \
+    extern "C" void oracle_counts_##SUF(void* h, int* out) {                                                          \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        out[0] = s->N; out[1] = s->E; out[2] = s->T;                                                                  \
+        out[3] = (int)s->ecolorCnt.size(); out[4] = (int)s->tcolorCnt.size();                                         \
+    }                                                                                                                 \
+    extern "C" void oracle_topology_##SUF(void* h, int* ei, int* ej, int* eowner, int* ecolorCnt, int* tids,          \
+                                          int* ttype, int* towner, int* tcolor, int* tcolorCnt) {                     \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        for (int e = 0; e < s->E; e++) { ei[e] = s->ei[e]; ej[e] = s->ej[e]; eowner[e] = s->eowner[e]; }              \
+        for (size_t c = 0; c < s->ecolorCnt.size(); c++) ecolorCnt[c] = s->ecolorCnt[c];                              \
+        std::memcpy(tids, s->tids.data(), sizeof(int) * 4 * s->T);                                                    \
+        std::memcpy(ttype, s->ttype.data(), sizeof(int) * s->T);                                                      \
+        std::memcpy(towner, s->towner.data(), sizeof(int) * 3 * s->T);                                                \
+        std::memcpy(tcolor, s->tcolor.data(), sizeof(int) * s->T);                                                    \
+        for (size_t c = 0; c < s->tcolorCnt.size(); c++) tcolorCnt[c] = s->tcolorCnt[c];                             \
+    }                                                                                                                 \
+    extern "C" void oracle_type_data_##SUF(void* h, double* typeB, double* typeV) {                                   \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        for (int t = 0; t < 6; t++) {                                                                                 \
+            typeV[t] = s->typeV[t];                                                                                   \
+            for (int a = 0; a < 9; a++) typeB[9 * t + a] = (double)s->typeB[t][a / 3][a % 3];                         \
+        }                                                                                                             \
+    }                                                                                                                 \
+    extern "C" void oracle_set_tet_stiffness_##SUF(void* h, const float* kDev, const float* kHyd) {                   \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        for (int t = 0; t < s->T; t++) { s->kDev[t] = (R)kDev[t]; s->kHyd[t] = (R)kHyd[t]; }                          \
+    }                                                                                                                 \
+    extern "C" void oracle_set_gravity_##SUF(void* h, float gx, float gy, float gz) {                                 \
+        Tissue<R>* s = (Tissue<R>*)h; /* k_SetGravity: force = acceleration * mass (physics.cu:317-329) */            \
+        for (int i = 0; i < s->N; i++) s->ext[i] = mk<R>((R)gx, (R)gy, (R)gz) * s->mass[i];                            \
+    }                                                                                                                 \
+    extern "C" void oracle_set_tool_##SUF(void* h, const OracleTool* t, int toolArmed, int bladeArmed, int reset) {   \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        s->tool = toolFrom<R>(*t);                                                                                    \
+        if (reset) s->toolFrom = s->tool;                                                                             \
+        s->toolArmed = toolArmed != 0; s->bladeArmed = bladeArmed != 0;                                               \
+    }                                                                                                                 \
+    extern "C" int oracle_step_##SUF(void* h, float dt) { return step(*(Tissue<R>*)h, (R)dt); }                       \
+    extern "C" void oracle_detect_cut_##SUF(void* h, const OracleCutTool* t) { detectCutQuad(*(Tissue<R>*)h, *t); }   \
+    extern "C" void oracle_detect_cut_ccd_##SUF(void* h, const float* S, const float* E, const float* ncut,           \
+                                                float toolD, const float* prevPos) {                                  \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        std::vector<V3<R>> pp(s->N);                                                                                  \
+        for (int i = 0; i < s->N; i++) pp[i] = v3<R>(prevPos + 3 * i);                                                \
+        detectCutCCD(*s, v3<R>(S), v3<R>(E), v3<R>(ncut), (R)toolD, pp);                                              \
+    }                                                                                                                 \
+    extern "C" void oracle_refresh_##SUF(void* h) { refreshLiveness(*(Tissue<R>*)h); }                                \
+    extern "C" void oracle_get_state_##SUF(void* h, R* pos, R* vel) {                                                 \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        for (int i = 0; i < s->N; i++) {                                                                              \
+            pos[3 * i] = s->pos[i].x; pos[3 * i + 1] = s->pos[i].y; pos[3 * i + 2] = s->pos[i].z;                     \
+            vel[3 * i] = s->vel[i].x; vel[3 * i + 1] = s->vel[i].y; vel[3 * i + 2] = s->vel[i].z;                     \
+        }                                                                                                             \
+    }                                                                                                                 \
+    extern "C" void oracle_set_state_##SUF(void* h, const float* pos, const float* vel) {                             \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        for (int i = 0; i < s->N; i++) { s->pos[i] = v3<R>(pos + 3 * i); s->vel[i] = v3<R>(vel + 3 * i); }             \
+    }                                                                                                                 \
+    extern "C" void oracle_get_reaction_##SUF(void* h, double* out) {                                                 \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        out[0] = s->reaction[0]; out[1] = s->reaction[1]; out[2] = s->reaction[2]; out[3] = s->reactionContacts;      \
+    }                                                                                                                 \
+    extern "C" unsigned oracle_get_cut_##SUF(void* h, int* nbr, unsigned* cutMask, float* edgeCutT,                   \
+                                             int* bendAlive, unsigned* counters) {                                    \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        std::memcpy(nbr, s->nbr.data(), sizeof(int) * 6 * s->N);                                                      \
+        std::memcpy(cutMask, s->cutMask.data(), sizeof(unsigned) * s->V);                                             \
+        std::memcpy(edgeCutT, s->edgeCutT.data(), sizeof(float) * 6 * s->N);                                          \
+        std::memcpy(bendAlive, s->bendAlive.data(), sizeof(int) * 15 * s->N);                                         \
+        counters[0] = s->revision; counters[1] = s->cutPointCounter;                                                  \
+        return (unsigned)s->cutPoints.size();                                                                         \
+    }                                                                                                                 \
+    extern "C" void oracle_get_cut_points_##SUF(void* h, void* out) {                                                 \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        std::memcpy(out, s->cutPoints.data(), sizeof(CutPoint) * s->cutPoints.size());                                \
+    }                                                                                                                 \
+    extern "C" void oracle_get_liveness_##SUF(void* h, int* tetActiveGen, int* edgeActive) {                          \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        refreshLiveness(*s);                                                                                          \
+        std::memcpy(tetActiveGen, s->tactive.data(), sizeof(int) * s->T);                                             \
+        std::memcpy(edgeActive, s->eactive.data(), sizeof(int) * s->E);                                               \
+    }                                                                                                                 \
+    extern "C" void oracle_compact_##SUF(void* h, int* tetRemap, int* edgeRemap, int* nodeRemap, int* counts) {       \
+        compact(*(Tissue<R>*)h, tetRemap, edgeRemap, nodeRemap, counts);                                              \
+    }
+
+ORACLE_API(float, f32)
+ORACLE_API(double, f64)
+
+extern "C" int oracle_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Unit hooks for the checker's own tests: single projections and primitives on raw double inputs.
+// ---------------------------------------------------------------------------------------------------------------
+extern "C" int oracle_tet_project_f64(int mode, const double* p12, const double* prev12, const double* w4,
+                                      const double* B9, double ad, double ah, double* lam2, double dampingOverH,
+                                      double bulkOverH, double omega, double* out12) {
+    V3<double> p[4], pv[4], out[4], br[3];
+    double w[4];
+    for (int k = 0; k < 4; k++) {
+        p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);
+        pv[k] = mk(prev12[3 * k], prev12[3 * k + 1], prev12[3 * k + 2]);
+        w[k] = w4[k];
+    }
+    for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);
+    bool ok = tetProject<double>(mode, p, pv, w, br, ad, ah, lam2[0], lam2[1], dampingOverH, bulkOverH, omega, out);
+    for (int k = 0; k < 4; k++) { out12[3 * k] = out[k].x; out12[3 * k + 1] = out[k].y; out12[3 * k + 2] = out[k].z; }
+    return ok ? 1 : 0;
+}
+extern "C" int oracle_moller_trumbore_f64(const double* O, const double* D, const double* V, double* tuv) {
+    double t, u, v;
+    bool hit = mollerTrumbore<double>(mk(O[0], O[1], O[2]), mk(D[0], D[1], D[2]), mk(V[0], V[1], V[2]),
+                                      mk(V[3], V[4], V[5]), mk(V[6], V[7], V[8]), t, u, v);
+    tuv[0] = t; tuv[1] = u; tuv[2] = v;
+    return hit ? 1 : 0;
+}
+extern "C" float oracle_iso_pow_f32(float J) { return isoPow(J); }
+extern "C" double oracle_shape_distance_f64(const double* center, const double* u, const double* n, const double* half,
+                                           const double* a, const double* b, double radius, int capsule,
+                                           const double* x, double* normal) {
+    Shape<double> s{};
+    s.center = mk(center[0], center[1], center[2]);
+    s.u = mk(u[0], u[1], u[2]);
+    s.n = mk(n[0], n[1], n[2]);
+    s.w = cross3(s.u, s.n);
+    s.half = mk(half[0], half[1], half[2]);
+    s.a = mk(a[0], a[1], a[2]);
+    s.b = mk(b[0], b[1], b[2]);
+    s.radius = radius;
+    s.capsule = capsule;
+    V3<double> nrm, surf;
+    double d = shapeDistance(s, mk(x[0], x[1], x[2]), nrm, surf);
+    normal[0] = nrm.x; normal[1] = nrm.y; normal[2] = nrm.z;
+    return d;
+}

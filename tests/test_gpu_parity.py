@@ -1,0 +1,180 @@
+"""Parity of the HIP substep and cut with the CPU oracle on the same seeded inputs (GPU).
+
+Bars: cut topology (severed links, cut masks, directed cut fractions, bend pairs, revision, cut points) bit-exact;
+positions within 1e-5 relative (max |dx| / max |x|) after 100 substeps. The kernels and the FP32 oracle evaluate the
+same IEEE operation sequence without contraction, so positions are also checked for exact equality; the FP64 oracle
+error is reported.
+"""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.params import TET_GUARDED, TET_NEOHOOKEAN, TET_VOLUME
+from genesis_tissue_simulation_amd.workloads import block_6144, tet_count
+from genesis_tissue_simulation_amd.grid import block_grid
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-5
+
+
+def make_pair(w, params=None, double=False, positions=None):
+    from genesis_tissue_simulation_amd.tissue import Tissue
+    from oracle.oracle import Oracle
+    p = params or w.params
+    init = w.initial_positions if positions is None else positions
+    kd, kh = w.stiffness()
+    o = Oracle(w.grid, p, init, double=double)
+    o.set_tet_stiffness(kd, kh)
+    o.set_gravity(w.gravity)
+    t = Tissue(w.grid, p, kd, kh, init)
+    t.set_gravity(w.gravity)
+    return o, t
+
+
+def advance(pairs, w, steps, cut=True, start=0):
+    from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+    for s in range(start, start + steps):
+        tool = swept_cut_tool(w.blade(s), w.blade(s + 1), w.blade_thickness) if cut and w.blade else None
+        for sim in pairs:
+            sim.step(w.dt)
+            if tool is not None:
+                sim.detect_cut(tool)
+
+
+def rel_err(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)).max() /
+                 np.abs(np.asarray(b, np.float64)).max())
+
+
+def assert_cut_equal(ct, co):
+    np.testing.assert_array_equal(ct["neighbors"], co["neighbors"])
+    np.testing.assert_array_equal(ct["cut_mask"], co["cut_mask"])
+    np.testing.assert_array_equal(ct["edge_cut_t"], co["edge_cut_t"])
+    np.testing.assert_array_equal(ct["bend_alive"], co["bend_alive"])
+    assert ct["revision"] == co["revision"]
+    assert ct["cut_point_counter"] == co["cut_point_counter"]
+    np.testing.assert_array_equal(ct["cut_points"], co["cut_points"])
+
+
+def test_block_100_substeps_matches_oracle():
+    w = block_6144()
+    o, t = make_pair(w)
+    o64, _ = make_pair(w, double=True)
+    advance([o, t, o64], w, 10, cut=False)
+    po, vo = o.state()
+    pt, vt = t.positions(), t.velocities()
+    err = rel_err(pt, po)
+    err64 = rel_err(pt, o64.state()[0])
+    print(f"block 100 substeps: rel err vs FP32 oracle {err:.3e}, vs FP64 oracle {err64:.3e}")
+    assert err <= REL_TOL
+    assert err64 <= 1e-4
+    np.testing.assert_array_equal(pt, po)
+    np.testing.assert_array_equal(vt, vo)
+
+
+def test_block_cut_topology_bit_exact():
+    w = block_6144()
+    o, t = make_pair(w)
+    advance([o, t], w, 40, cut=True)
+    co, ct = o.cut_state(), t.cut_state()
+    assert co["revision"] > 0, "the blade must sever links"
+    assert_cut_equal(ct, co)
+    lo, lt = o.liveness(), t.liveness()
+    np.testing.assert_array_equal(lt[0], lo[0])
+    np.testing.assert_array_equal(lt[1], lo[1])
+    assert rel_err(t.positions(), o.state()[0]) <= REL_TOL
+    print(f"cut: revision {co['revision']}, cut points {co['cut_point_counter']}, live tets {lo[0].sum()}")
+
+
+def test_cut_unit_parity_on_identical_positions():
+    """The cut kernels on exactly the same deformed positions as the oracle (no stepping in between)."""
+    w = block_6144()
+    rng = np.random.default_rng(3)
+    pos = w.initial_positions + rng.normal(0, 0.02 * 0.25, w.initial_positions.shape).astype(np.float32)
+    o, t = make_pair(w, positions=pos)
+    from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+    for s in range(0, 36, 3):
+        tool = swept_cut_tool(w.blade(s), w.blade(s + 3), w.blade_thickness)
+        o.detect_cut(tool)
+        t.detect_cut(tool)
+    assert_cut_equal(t.cut_state(), o.cut_state())
+
+
+@pytest.mark.parametrize("mode", [TET_VOLUME, TET_NEOHOOKEAN, TET_GUARDED])
+@pytest.mark.parametrize("iterations", [1, 2])
+def test_tet_modes_and_iterations(mode, iterations):
+    w = block_6144()
+    p = w.params.with_(tet_mode=mode, iterations=iterations, strain_damping=0.01 if iterations == 2 else 0.0)
+    o, t = make_pair(w, params=p)
+    advance([o, t], w, 3, cut=False)
+    err = rel_err(t.positions(), o.state()[0])
+    print(f"mode {mode} iterations {iterations}: rel err {err:.3e}")
+    if mode == TET_GUARDED:
+        assert err <= REL_TOL  # logf is the device library's; tolerance only
+    else:
+        np.testing.assert_array_equal(t.positions(), o.state()[0])
+
+
+def test_fallback_edges_and_damping():
+    w = block_6144()
+    p = w.params.with_(edge_ks=0.0, young=60.0, beta_s=0.5, rayleigh=2.0, damping=0.01)
+    o, t = make_pair(w, params=p)
+    advance([o, t], w, 8, cut=True)
+    np.testing.assert_array_equal(t.positions(), o.state()[0])
+    np.testing.assert_array_equal(t.velocities(), o.state()[1])
+    assert_cut_equal(t.cut_state(), o.cut_state())
+
+
+def test_tool_contact_parity():
+    w = block_6144()
+    p = w.params.with_(contact_enabled=True, contact_radius=0.01, jaw_gap_closed=0.02)
+    o, t = make_pair(w, params=p)
+    L = 0.25
+    for s in range(6):
+        depth = 0.4 * L * (s + 1) / 6
+        tool = dict(shaftA=(-L, 8 * L + 0.3 * L - depth, 4 * L), shaftB=(17 * L, 8 * L + 0.3 * L - depth, 4 * L),
+                    shaftR=0.3 * L, jawHinge=(8 * L, 8 * L + 0.2 * L - depth, 2 * L), jawAxis=(0, 0, 1),
+                    jawNormal=(1, 0, 0), jawHalfWidth=0.5 * L, jawLen=2 * L, jawGap=0.2 * L,
+                    bladeA=(2 * L, 8 * L + 0.1 * L - depth, 6 * L), bladeB=(6 * L, 8 * L + 0.1 * L - depth, 6 * L),
+                    bladeR=0.1 * L)
+        o.set_tool(tool, tool_armed=True, blade_armed=True, reset=(s == 0))
+        o.step(w.dt)
+        t.step(w.dt, tool=tool, tool_armed=True, blade_armed=True)
+    err = rel_err(t.positions(), o.state()[0])
+    ro, no = o.reaction()
+    rt, nt = t.reaction_force()
+    print(f"contact: rel err {err:.3e}, reaction oracle {ro} gpu {rt}, contacts {no} {nt}")
+    assert no > 0
+    np.testing.assert_array_equal(t.positions(), o.state()[0])
+    assert nt == no
+    np.testing.assert_allclose(rt, ro, rtol=1e-4, atol=1e-6 * max(1.0, np.abs(ro).max()))
+
+
+def test_compaction_matches_oracle():
+    w = block_6144()
+    o, t = make_pair(w)
+    advance([o, t], w, 30, cut=True)
+    tro, ero, nro, co = o.compact()
+    trt, ert, nrt, ct = t.compact()
+    np.testing.assert_array_equal(trt, tro)
+    np.testing.assert_array_equal(ert, ero)
+    np.testing.assert_array_equal(nrt, nro)
+    np.testing.assert_array_equal(ct, co)
+    live = trt >= 0
+    assert np.array_equal(np.sort(trt[live]), np.arange(live.sum()))
+
+
+def test_ccd_cut_parity():
+    """CCD of edges that moved one cell along +x during the tick, against a static blade along z."""
+    w = block_6144()
+    o, t = make_pair(w)
+    L = 0.25
+    advance([o, t], w, 2, cut=False)
+    cur = t.positions()
+    prev = (cur - np.array((1.0 * L, 0.0, 0.0), np.float32)).astype(np.float32)
+    S, E = np.array((4.1 * L, 7.5 * L, 1 * L), np.float32), np.array((4.1 * L, 7.5 * L, 7 * L), np.float32)
+    o.detect_cut_ccd(S, E, (1, 0, 0), 0.02, prev)
+    t.detect_cut_ccd(S, E, (1, 0, 0), 0.02, prev)
+    co, ct = o.cut_state(), t.cut_state()
+    assert co["revision"] > 0
+    assert_cut_equal(ct, co)
