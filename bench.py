@@ -1,0 +1,168 @@
+"""Throughput of the XPBD tissue substep on one MI355X: tet-substeps/s.
+
+`python bench.py [--steps K] [--warmup W]` runs the kidney-scale single instance (an ellipsoid of ~250k Kuhn tets,
+20 substeps x 2 iterations per step, grasper contact, a Bezier blade cutting once per step) for W untimed and K
+timed steps on cuda:0, and prints one JSON line. `value` is live tets x substeps / second over the timed steps, with
+all state resident on the GPU. The dominant kernel (the tet colour pass) is timed per launch with HIP events on its
+stream over the same steps; `roofline.achieved` is its algorithmic bytes per tet update (the per-tet-substep figure
+(52 N + 8 T + 4 E)/T) over its measured duration. `cpu_baseline` is the CPU oracle of the same step, timed on this
+host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
+    """Compulsory HBM traffic of one substep per tet: node position + velocity read and write in FP32 and the inverse
+    mass (52 B/node), per-tet compliance and liveness (8 B/tet), edge liveness (4 B/edge)."""
+    return (52.0 * active_nodes + 8.0 * tets + 4.0 * edges) / tets
+
+
+def load_traffic(name):
+    path = ROOT / "profiles" / name
+    if path.exists():
+        try:
+            return json.loads(path.read_text())
+        except ValueError:
+            return None
+    return None
+
+
+def run_gpu(w, steps, warmup):
+    import ctypes as ct
+
+    import torch
+
+    from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
+
+    kd, kh = w.stiffness()
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions)
+    t.set_gravity(w.gravity)
+
+    def one(step):
+        tool = w.tool(step) if w.tool else None
+        t.step(w.dt, tool=tool, tool_armed=tool is not None)
+        if w.blade:
+            t.detect_cut(swept_cut_tool(w.blade(step), w.blade(step + 1), w.blade_thickness))
+
+    for s in range(warmup):
+        one(s)
+    torch.cuda.synchronize()
+    live0 = int(t.liveness()[0].sum())
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        one(s)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    live1 = int(t.liveness()[0].sum())
+    # Per-launch timing of the tet colour kernel with HIP events on its stream. Event records cannot sit inside the
+    # captured step graph, so these steps issue the same kernels directly; the kernels and their arguments are the
+    # same as in the graph.
+    lib = t.lib
+    lib.gts_profile_begin()
+    for s in range(warmup + steps, warmup + steps + max(1, min(steps, 3))):
+        one(s)
+    total_ms, launches, units = ct.c_double(), ct.c_longlong(), ct.c_longlong()
+    lib.gts_profile_end(ct.byref(total_ms), ct.byref(launches), ct.byref(units))
+    torch.cuda.synchronize()
+    pos = t.positions()
+    if not np.isfinite(pos).all():
+        raise RuntimeError("non-finite positions: simulation stopped")
+    return dict(elapsed=elapsed, live=(live0 + live1) / 2, tet_ms=total_ms.value, launches=launches.value,
+                units=units.value, E=t.E, T=t.T, active=int((w.grid.active != 0).sum()))
+
+
+def run_cpu(w, steps, threads):
+    from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+    from oracle import oracle as orc
+
+    orc.build()
+    used = orc.set_threads(threads)
+    kd, kh = w.stiffness()
+    o = orc.Oracle(w.grid, w.params, w.initial_positions)
+    o.set_tet_stiffness(kd, kh)
+    o.set_gravity(w.gravity)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        if w.tool:
+            o.set_tool(w.tool(s), tool_armed=True, reset=(s == 0))
+        o.step(w.dt)
+        if w.blade:
+            o.detect_cut(swept_cut_tool(w.blade(s), w.blade(s + 1), w.blade_thickness))
+    elapsed = time.perf_counter() - t0
+    live = int(o.liveness()[0].sum())
+    return dict(elapsed=elapsed, live=live, threads=used)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the tissue path runs on a single MI355X and has no multi-GPU mode "
+              "(instances shard by LPT packing across GPUs by design only)", file=sys.stderr)
+        sys.exit(2)
+
+    from genesis_tissue_simulation_amd.workloads import kidney_250k
+
+    w = kidney_250k()
+    g = run_gpu(w, args.steps, args.warmup)
+    substeps = w.params.substeps
+    value = g["live"] * substeps * args.steps / g["elapsed"]
+    b_alg = algorithmic_bytes_per_tet_substep(g["active"], g["T"], g["E"])
+    avg_launch_ms = g["tet_ms"] / max(g["launches"], 1)
+    achieved = b_alg * g["units"] / (g["tet_ms"] * 1e-3) / 1e9 if g["tet_ms"] > 0 else 0.0
+    traffic = load_traffic("tet_colour_traffic_kidney.json")
+    line = {
+        "metric": "tet-substeps/s plain vs batched vs adjoint @ <=1e-4 rel err vs FP64",
+        "value": value,
+        "unit": "tet-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * g["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "kidney-250k plain single instance", "tets": g["T"], "live_tets": g["live"],
+                   "active_nodes": g["active"], "substeps": substeps, "iterations": w.params.iterations,
+                   "dt": w.dt, "contact": "capsule grasper", "cut": "swept-quad blade, once per step"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "kernel": "k_tets (one colour of the tet pass)", "avg_launch_us": 1e3 * avg_launch_ms,
+                     "launches": g["launches"], "bytes_per_tet_update": b_alg},
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        c = run_cpu(w, args.cpu_steps, threads)
+        line["cpu_baseline"] = {
+            "value": c["live"] * substeps * args.cpu_steps / c["elapsed"], "unit": "tet-substeps/s",
+            "cores": c["threads"], "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same kidney-250k workload (CPU oracle, FP32, OpenMP over the "
+                      f"members of a colour)"}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

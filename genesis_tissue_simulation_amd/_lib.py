@@ -18,6 +18,7 @@ V3 = F * 3
 
 GTS_ABI_VERSION = 1
 GTS_MAX_COLOURS = 64
+GTS_STEP_CONST_FLOATS = 184
 STATUS = {0: "ok", -1000: "not ready", -1002: "bad dt", -1003: "colour overflow", -1004: "bad argument",
           -1005: "irregular rest lattice", -3004: "capacity exceeded"}
 
@@ -42,7 +43,8 @@ class GtsTissueView(ct.Structure):
                 ("edge_lambda", P), ("slot_support", P), ("tet_desc", P), ("tet_k_dev", P), ("tet_k_hyd", P),
                 ("tet_alpha", P), ("tet_lambda", P), ("grid_edges", P), ("occupied", P), ("cut_mask", P),
                 ("edge_cut_t", P), ("edge_skin_t", P), ("cut_points", P), ("cut_point_capacity", I),
-                ("counters", P), ("scratch", P), ("reaction", P)]
+                ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
+                ("use_graph", I)]
 
 
 class GtsStepParams(ct.Structure):
@@ -71,6 +73,7 @@ SIGNATURES = {
     "gts_struct_sizes": ((P,), I),
     "gts_build_topology": ((P, P), I),
     "gts_iso_pow": ((F,), F),
+    "gts_log": ((F,), F),
     "gts_scratch_floats": ((I, I, I, I), ct.c_longlong),
     "gts_init_state": ((P, P), I),
     "gts_set_gravity": ((P, F, F, F, P), I),
@@ -79,6 +82,8 @@ SIGNATURES = {
     "gts_detect_cut": ((P, P, P), I),
     "gts_detect_cut_ccd": ((P, P, P, P, F, P, P), I),
     "gts_compact_topology": ((P, P, P, P, P, P, P), I),
+    "gts_profile_begin": ((), I),
+    "gts_profile_end": ((P, P, P), I),
     "gts_batched_plan": ((P, I, I, P), I),
     "gts_batched_step": ((P, P, P, I, I, P), I),
     "gts_batched_lds_bytes": ((), I),

@@ -49,6 +49,25 @@ GTS_HD float iso_pow(float J) {
     return 1.f / (c * c);
 }
 
+// Natural log for the guarded bulk strain log J (the reference calls logf, xpbd/xpbd_solver.cu:780): split x = m 2^e
+// with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with s = (m-1)/(m+1) summed to s^11, e ln2 in two parts. A fixed
+// IEEE sequence, so the CPU checker reproduces it exactly; within about 1 ulp of the exact log for normal x > 0.
+GTS_HD float own_log(float x) {
+    uint32_t i = __builtin_bit_cast(uint32_t, x);
+    int e = (int)((i >> 23) & 0xffu) - 127;
+    float m = __builtin_bit_cast(float, (i & 0x7fffffu) | 0x3f800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.f) / (m + 1.f);
+    float s2 = s * s;
+    float p = s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * (0.111111112f + s2 * 0.0909090936f))));
+    float lm = 2.f * s + 2.f * s * p;
+    float fe = (float)e;
+    return fe * 0.693145752f + (fe * 1.42860677e-06f + lm);
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // Tool shapes and the swept node contact (reference: contact/friction_contact.cuh:17-169).
 // ---------------------------------------------------------------------------------------------------------------

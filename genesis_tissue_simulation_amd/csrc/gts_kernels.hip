@@ -268,7 +268,7 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
     float CD = sqrtf(fmaxf(qq, 1e-12f));
     float devScale = 1.f / CD;
     float volumeGradient = J > 0.05f ? iso * I1 / (3.f * positiveJ) : 0.f;
-    float CH = safeguarded ? logf(positiveJ) + fminf(0.f, (J - 0.05f) / 0.05f) : J - 1.f;
+    float CH = safeguarded ? own_log(positiveJ) + fminf(0.f, (J - 0.05f) / 0.05f) : J - 1.f;
     for (int c = 1; c < 4; c++) {
         f3 b = br[c - 1];
         gd[c] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[c] * volumeGradient) * devScale;
@@ -322,7 +322,7 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 // xpbd_solver.cu:3663-3665), then k_update_velocity and, unless perpendicular edge damping must run between them,
 // k_apply_velscale (xpbd_solver.cu:871-919). Nodes are independent here, so one pass does all of it.
 template <bool CONTACT, bool SCALE>
-__global__ void k_finish_nodes(int N, StepShapes shapes, uint32_t armed, float alpha, float muS, float muD,
+__global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, uint32_t armed, float alpha, float muS, float muD,
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
                                const float4* __restrict__ prev, float4* __restrict__ vel, float* reaction) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -334,6 +334,7 @@ __global__ void k_finish_nodes(int N, StepShapes shapes, uint32_t armed, float a
         f3 x = xyz(p4);
         f3 xp = xyz(prev[i]);
         if (CONTACT && w > 0.f) {
+            const StepShapes& shapes = *shapes_p;
             ContactRecord rec[4];
             float lamN[4];
             f3 lamT[4];
@@ -788,6 +789,33 @@ __global__ void k_copy_counts(const int* pt, int nt, const int* pe, int ne, cons
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Optional per-launch timing of the dominant kernel (the tet colour pass) with HIP events on the launch stream,
+// in the spirit of the reference's xpbd_profile sections (xpbd/xpbd_solver.cu:15-60).
+struct Profile {
+    bool on = false;
+    std::vector<hipEvent_t> events;  // start, stop pairs
+    std::vector<long long> units;
+    size_t used = 0;
+};
+Profile g_profile;
+
+inline void profile_mark_begin(hipStream_t s) {
+    if (!g_profile.on) return;
+    if (2 * g_profile.used + 2 > g_profile.events.size()) {
+        size_t old = g_profile.events.size();
+        g_profile.events.resize(old + 4096);
+        for (size_t k = old; k < g_profile.events.size(); k++) hipEventCreate(&g_profile.events[k]);
+    }
+    hipEventRecord(g_profile.events[2 * g_profile.used], s);
+}
+inline void profile_mark_end(hipStream_t s, long long units) {
+    if (!g_profile.on) return;
+    hipEventRecord(g_profile.events[2 * g_profile.used + 1], s);
+    if (g_profile.units.size() <= g_profile.used) g_profile.units.resize(g_profile.used + 1024);
+    g_profile.units[g_profile.used] = units;
+    g_profile.used++;
+}
+
 ToolPose pose_of(const GtsToolState* t) {
     ToolPose p;
     p.shaftA = ld3(t->shaftA); p.shaftB = ld3(t->shaftB); p.shaftR = t->shaftR;
@@ -909,100 +937,198 @@ extern "C" int gts_refresh_liveness(const GtsTissueView* v, void* stream) {
     return launch_refresh(v, L, S(stream));
 }
 
+namespace {
+
+static_assert(sizeof(StepShapes) == GTS_STEP_CONST_FLOATS * sizeof(float), "step constant layout");
+
+struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, xpbd_solver.cu:2481-2514)
+    int Nsub, iters, mode;
+    float h, at, velScale, betaGammaH, invH2, fb, dOverH, bOverH, alphaC, scaleR, invH, omega, beta_s;
+    float mu_s, mu_d, radius;
+    bool strain, fallback, edges, contact, perp;
+    uint32_t armed;
+};
+
+// Every launch of one outer step, in the order of the reference's substep (xpbd_solver.cu:3460-3721).
+void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hipStream_t s) {
+    const int T = v->n_tets, N = v->n_nodes;
+    launch_refresh(v, L, s);
+    if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, P.invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
+    if (P.contact) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
+    const StepShapes* shapes = reinterpret_cast<const StepShapes*>(v->step_consts);
+    dim3 g(grid_for(N)), b(GTS_BLOCK);
+    for (int n = 0; n < P.Nsub; n++) {
+        k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, (const float4*)v->force);
+        for (int it = 0; it < P.iters; it++) {
+            if (P.edges)
+                for (int c = 0; c < v->n_edge_colours; c++) {
+                    int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
+                    if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, P.iters, P.fallback, P.at, P.fb, P.omega,
+                                                    P.betaGammaH);
+                }
+            if (P.strain)
+                for (int c = 0; c < v->n_tet_colours; c++) {
+                    int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
+                    if (cnt <= 0) continue;
+                    profile_mark_begin(s);
+                    if (P.mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
+                    else if (P.mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
+                    else launch_tet_colour<2>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
+                    profile_mark_end(s, cnt);
+                }
+        }
+        const StepShapes* sh = shapes + n;
+#define GTS_FINISH(C, SC)                                                                                        \
+    k_finish_nodes<C, SC><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR, P.invH,   \
+                                          P.velScale, (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,  \
+                                          v->reaction)
+        if (P.contact) { if (P.perp) GTS_FINISH(true, false); else GTS_FINISH(true, true); }
+        else { if (P.perp) GTS_FINISH(false, false); else GTS_FINISH(false, true); }
+#undef GTS_FINISH
+        if (P.perp) {
+            for (int c = 0; c < v->n_edge_colours; c++) {
+                int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
+                if (cnt > 0)
+                    k_perp_damp<<<grid_for(cnt), GTS_BLOCK, 0, s>>>(off, cnt, L, P.beta_s, P.h, v->edge_desc,
+                                                                    (const float4*)v->pos, (float4*)v->vel);
+            }
+            k_velscale<<<g, b, 0, s>>>(N, P.velScale, (const float4*)v->pos, (float4*)v->vel);
+        }
+    }
+}
+
+// Captured steps, one per (view, plan) pair: replaying the graph issues the same launches with the same arguments;
+// everything that changes between steps (tool shapes, positions, liveness) lives in device memory.
+// Pinned host staging for the per-step constants: a ring of buffers, each reused only after the copy that last read
+// it has completed (its event), so the host never overwrites data still in flight.
+struct Staging {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+Staging g_staging[8];
+int g_staging_next = 0, g_staging_last = -1;
+
+int staging_slot(size_t bytes, void** out) {
+    Staging& st = g_staging[g_staging_next];
+    if (st.pending) {
+        HIP_TRY(hipEventSynchronize(st.done));
+        st.pending = false;
+    }
+    if (st.cap < bytes) {
+        if (st.host) HIP_TRY(hipHostFree(st.host));
+        st.host = nullptr;
+        HIP_TRY(hipHostMalloc(&st.host, bytes, hipHostMallocDefault));
+        st.cap = bytes;
+    }
+    if (!st.done) HIP_TRY(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    g_staging_last = g_staging_next;
+    g_staging_next = (g_staging_next + 1) % 8;
+    *out = st.host;
+    return GTS_OK;
+}
+
+int staging_done(hipStream_t s) {
+    Staging& st = g_staging[g_staging_last];
+    HIP_TRY(hipEventRecord(st.done, s));
+    st.pending = true;
+    return GTS_OK;
+}
+
+struct GraphEntry {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+};
+std::vector<GraphEntry> g_graphs;
+hipStream_t g_capture = nullptr;
+
+std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
+    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan));
+    std::memcpy(k.data(), v, sizeof(GtsTissueView));
+    std::memcpy(k.data() + sizeof(GtsTissueView), &P, sizeof(StepPlan));
+    return k;
+}
+
+int launch_step_graph(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hipStream_t s) {
+    std::vector<unsigned char> key = graph_key(v, P);
+    for (auto& e : g_graphs)
+        if (e.key == key) return -(int)hipGraphLaunch(e.exec, s);
+    if (!g_capture) HIP_TRY(hipStreamCreateWithFlags(&g_capture, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(g_capture, hipStreamCaptureModeThreadLocal));
+    enqueue_step(v, L, P, g_capture);
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamEndCapture(g_capture, &graph));
+    GraphEntry e;
+    e.key = key;
+    hipError_t err = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    HIP_TRY(err);
+    if (g_graphs.size() >= 16) {
+        hipGraphExecDestroy(g_graphs.front().exec);
+        g_graphs.erase(g_graphs.begin());
+    }
+    g_graphs.push_back(e);
+    return -(int)hipGraphLaunch(e.exec, s);
+}
+
+}  // namespace
+
 extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from,
                         const GtsToolState* to, int tool_armed, int blade_armed, void* stream) {
     if (!v || !p || !from || !to) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
     hipStream_t s = S(stream);
     Lattice L = lattice_of(v);
-    // xpbd_begin_frame (xpbd_solver.cu:2481-2514)
-    const int Nsub = p->substeps > 0 ? p->substeps : 1;
-    const float h = p->dt / (float)Nsub;
-    if (!(h > 0.f) || !std::isfinite(h)) return GTS_ERR_BAD_DT;
-    const float ks = p->edge_ks;
-    const float at = ks > 0.f ? 1.f / (ks * h * h) : 0.f;
-    const float velScale = (1.f - p->damping) / (1.f + p->rayleigh * h);
-    const float betaGammaH = at * p->beta_s * h;
-    const float invH2 = 1.f / (h * h);
-    const int T = v->n_tets, N = v->n_nodes;
-    { int rc_ = launch_refresh(v, L, s); if (rc_) return rc_; }
-    if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
-    const bool strain = T > 0;
-    const bool fallback = strain && ks <= 0.f;
-    const bool edges = ks > 0.f || fallback;
-    const float fb = fallback ? 1.f / (p->young * h * h) : 0.f;
-    const float dOverH = fmaxf(0.f, p->strain_damping) / h;
-    const float bOverH = h > 0.f ? p->bulk_damping / h : 0.f;
-    const int iters = p->iterations > 0 ? p->iterations : 1;
-    const ToolPose pf = pose_of(from), pt = pose_of(to);
-    const bool contact = p->contact_enabled && (tool_armed || blade_armed);
-    const uint32_t armed = (tool_armed ? 7u : 0u) | (blade_armed ? 8u : 0u);
-    const float alphaC = p->contact_compliance / (h * h);
-    const float scaleR = 1.f / (h * h * fmaxf(1.f, (float)Nsub));
-    const float invH = 1.f / h;
-    if (contact) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
-    for (int n = 0; n < Nsub; n++) {
-        // tool pose of this substep and of the previous one (xpbd_solver.cu:3467-3483)
-        float t = fminf(1.f, (float)(n + 1) / (float)Nsub);
-        float previousT = fmaxf(0.f, t - 1.f / (float)Nsub);
-        StepShapes shapes;
-        if (contact) {
+    StepPlan P;
+    std::memset(&P, 0, sizeof(P));
+    P.Nsub = p->substeps > 0 ? p->substeps : 1;
+    if (!v->step_consts || v->step_const_capacity < P.Nsub) return GTS_ERR_CAPACITY;
+    P.h = p->dt / (float)P.Nsub;
+    if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
+    const float ks = p->edge_ks, h = P.h;
+    P.at = ks > 0.f ? 1.f / (ks * h * h) : 0.f;
+    P.velScale = (1.f - p->damping) / (1.f + p->rayleigh * h);
+    P.betaGammaH = P.at * p->beta_s * h;
+    P.invH2 = 1.f / (h * h);
+    P.strain = v->n_tets > 0;
+    P.fallback = P.strain && ks <= 0.f;
+    P.edges = ks > 0.f || P.fallback;
+    P.fb = P.fallback ? 1.f / (p->young * h * h) : 0.f;
+    P.dOverH = fmaxf(0.f, p->strain_damping) / h;
+    P.bOverH = h > 0.f ? p->bulk_damping / h : 0.f;
+    P.iters = p->iterations > 0 ? p->iterations : 1;
+    P.mode = p->tet_mode;
+    P.omega = p->omega;
+    P.beta_s = p->beta_s;
+    P.perp = p->beta_s > 0.f;
+    P.contact = p->contact_enabled && (tool_armed || blade_armed);
+    P.armed = (tool_armed ? 7u : 0u) | (blade_armed ? 8u : 0u);
+    P.alphaC = p->contact_compliance / (h * h);
+    P.scaleR = 1.f / (h * h * fmaxf(1.f, (float)P.Nsub));
+    P.invH = 1.f / h;
+    P.mu_s = p->mu_s;
+    P.mu_d = p->mu_d;
+    P.radius = p->contact_radius;
+    // tool pose of each substep and of the one before it (xpbd_solver.cu:3467-3483), uploaded for the kernels
+    if (P.contact) {
+        StepShapes* host = nullptr;
+        { int rc_ = staging_slot(sizeof(StepShapes) * P.Nsub, reinterpret_cast<void**>(&host)); if (rc_) return rc_; }
+        const ToolPose pf = pose_of(from), pt = pose_of(to);
+        for (int n = 0; n < P.Nsub; n++) {
+            float t = fminf(1.f, (float)(n + 1) / (float)P.Nsub);
+            float previousT = fmaxf(0.f, t - 1.f / (float)P.Nsub);
             ToolPose prevTool = lerp_tool(pf, pt, previousT), stepTool = lerp_tool(pf, pt, t);
             for (int kind = 0; kind < 4; kind++) {
-                shapes.from[kind] = tool_shape(prevTool, kind, p->jaw_thickness, p->jaw_gap_closed);
-                shapes.to[kind] = tool_shape(stepTool, kind, p->jaw_thickness, p->jaw_gap_closed);
+                host[n].from[kind] = tool_shape(prevTool, kind, p->jaw_thickness, p->jaw_gap_closed);
+                host[n].to[kind] = tool_shape(stepTool, kind, p->jaw_thickness, p->jaw_gap_closed);
             }
-        } else {
-            std::memset(&shapes, 0, sizeof(shapes));
         }
-        k_predict<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel,
-                                                    (const float4*)v->force);
-        for (int it = 0; it < iters; it++) {
-            if (edges)
-                for (int c = 0; c < v->n_edge_colours; c++) {
-                    int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
-                    if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, iters, fallback, at, fb, p->omega, betaGammaH);
-                }
-            if (strain)
-                for (int c = 0; c < v->n_tet_colours; c++) {
-                    int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
-                    if (cnt <= 0) continue;
-                    if (p->tet_mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, iters, p->omega, dOverH, bOverH);
-                    else if (p->tet_mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, iters, p->omega, dOverH, bOverH);
-                    else launch_tet_colour<2>(v, L, s, off, cnt, it, iters, p->omega, dOverH, bOverH);
-                }
-        }
-        const bool perp = p->beta_s > 0.f;
-        dim3 g(grid_for(N)), b(GTS_BLOCK);
-        if (contact) {
-            if (perp)
-                k_finish_nodes<true, false><<<g, b, 0, s>>>(N, shapes, armed, alphaC, p->mu_s, p->mu_d,
-                                                            p->contact_radius, scaleR, invH, velScale, (float4*)v->pos,
-                                                            (const float4*)v->prev, (float4*)v->vel, v->reaction);
-            else
-                k_finish_nodes<true, true><<<g, b, 0, s>>>(N, shapes, armed, alphaC, p->mu_s, p->mu_d,
-                                                           p->contact_radius, scaleR, invH, velScale, (float4*)v->pos,
-                                                           (const float4*)v->prev, (float4*)v->vel, v->reaction);
-        } else {
-            if (perp)
-                k_finish_nodes<false, false><<<g, b, 0, s>>>(N, shapes, 0u, 0.f, 0.f, 0.f, 0.f, 0.f, invH, velScale,
-                                                             (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,
-                                                             v->reaction);
-            else
-                k_finish_nodes<false, true><<<g, b, 0, s>>>(N, shapes, 0u, 0.f, 0.f, 0.f, 0.f, 0.f, invH, velScale,
-                                                            (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,
-                                                            v->reaction);
-        }
-        if (perp) {
-            for (int c = 0; c < v->n_edge_colours; c++) {
-                int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
-                if (cnt > 0)
-                    k_perp_damp<<<grid_for(cnt), GTS_BLOCK, 0, s>>>(off, cnt, L, p->beta_s, h, v->edge_desc,
-                                                                    (const float4*)v->pos, (float4*)v->vel);
-            }
-            k_velscale<<<g, b, 0, s>>>(N, velScale, (const float4*)v->pos, (float4*)v->vel);
-        }
+        HIP_TRY(hipMemcpyAsync(v->step_consts, host, sizeof(StepShapes) * P.Nsub, hipMemcpyHostToDevice, s));
+        { int rc_ = staging_done(s); if (rc_) return rc_; }
     }
+    if (v->use_graph && !g_profile.on) return launch_step_graph(v, L, P, s);
+    enqueue_step(v, L, P, s);
     return -(int)hipGetLastError();
 }
 
@@ -1070,3 +1196,30 @@ extern "C" int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, 
 }
 
 extern "C" float gts_iso_pow(float J) { return iso_pow(J); }
+extern "C" float gts_log(float x) { return own_log(x); }
+
+extern "C" int gts_profile_begin(void) {
+    g_profile.on = true;
+    g_profile.used = 0;
+    return GTS_OK;
+}
+
+// Stops recording and returns the summed duration (ms), the launch count and the summed units (tets) of the recorded
+// tet colour launches. Synchronizes on the recorded events.
+extern "C" int gts_profile_end(double* total_ms, long long* launches, long long* units) {
+    g_profile.on = false;
+    double total = 0.0;
+    long long u = 0;
+    for (size_t k = 0; k < g_profile.used; k++) {
+        HIP_TRY(hipEventSynchronize(g_profile.events[2 * k + 1]));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, g_profile.events[2 * k], g_profile.events[2 * k + 1]));
+        total += ms;
+        u += g_profile.units[k];
+    }
+    if (total_ms) *total_ms = total;
+    if (launches) *launches = (long long)g_profile.used;
+    if (units) *units = u;
+    g_profile.used = 0;
+    return GTS_OK;
+}

@@ -1,0 +1,287 @@
+"""Drop-in counterpart of the reference's Python-facing tissue API (`endoscopic_tissue/native_backend.py`).
+
+`NativeTissue(grid, settings, dll_path, initial_command)` with `.step(Command) -> Observation`, `.observe()`,
+`.cut_edge_fractions()`, `.material_quality()`, `.contact_depth_m()`, `.knife_body_depth_m()`, `.close()` and the
+context manager keep the reference's names, argument meaning and error behaviour (`native_backend.py:84-196,
+:212-483`). Underneath, the legacy XPBD substep and the lattice cut run as HIP kernels on cuda:0
+(libgts_xpbd.so), with device buffers held as PyTorch-ROCm tensors.
+
+What the step does, per reference call:
+  * `LCS_SetGraspTool`: the grasper (shaft capsule + two jaw plates) from the command (`native_backend.py:341-360`);
+  * `LCS_SetGravity`: gravity ramped over 0.5 s (`:379-380`);
+  * `LCS_Step`: `Settings.substeps` substeps of the legacy branch with friction tool contact of the lattice nodes
+    (`xpbd/xpbd_solver.cu:3460-3721`), the material parameters `initialize` sends (`:289-325`): fallback edges only
+    (ks = 0), guarded Neo-Hookean tets, strain damping, bulk damping 0.01 s, Coulomb 0.6 / 0.4;
+  * the knife: a kinematic capsule whose swept surface between consecutive commands cuts the lattice
+    (`LCS_SetTool` + `LCS_DetectCut`, `plugin_api.cpp:145-155`) instead of the force-driven pressure blade.
+Out of this path: the reconstructed surface (`surface()`), scissors, grasp capture and the unified VBD/PCG branch.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .grid import METERS_PER_UNIT, TissueGrid
+from .params import TET_GUARDED, SolverParams, neohookean_stiffness
+
+# Analytic instrument dimensions (cm); the reference derives them from the instrument meshes
+# (`instrument_geometry.py`), which this path does not load.
+GRASPER_SHAFT_LENGTH = 3.0
+GRASPER_SHAFT_RADIUS = 0.4
+
+
+@dataclass(frozen=True)
+class Settings:
+    """Field-for-field the reference settings (`native_backend.py:84-116`)."""
+    dt: float = 1 / 30
+    substeps: int = 3
+    young_pa: float = 6000
+    poisson: float = 0.25
+    strain_damping_s: float = 0.04
+    density_kg_m3: float = 1050
+    jaw_length: float = 1.99
+    jaw_width: float = 0.52
+    jaw_open_gap: float = 1.0
+    jaw_axis: tuple = (0.0, -0.8, 0.6)
+    jaw_normal: tuple = (1.0, 0.0, 0.0)
+    blade_length: float = 1.6
+    blade_radius: float = 0.04
+    blade_axis: tuple = (2 ** -.5, 0.0, 2 ** -.5)
+    start_force_n: float = 0.2
+    cut_force_n: float = 0.12
+    blade_drive_stiffness: float = 1200
+    blade_mass_kg: float = 0.02
+    blade_friction: float = 0.15
+    blade_preload_n: float = 1.0
+    contact_refine_passes: int = 8
+    has_knife_body: bool = True
+    instrument_model: str = "davinci"
+    cutting_tool: str = "knife"
+    scissors_jaw_stiffness: float = 150
+    scissors_squeeze_cm: float = .2
+    contact_model: int = 2
+    # constraint iterations per substep (the reference sends 1, physics.cu:281-295)
+    iterations: int = 1
+
+
+@dataclass(frozen=True)
+class Command:
+    """Field-for-field the reference command (`native_backend.py:119-130`)."""
+    hinge: tuple = (-1.4, 3.384, -2.288)
+    closure: float = 0.0
+    blade_center: tuple = (0.2, 2.2, -0.6)
+    blade_enabled: bool = True
+    scissors_closure: float = 0.0
+    scissors_cutting: bool = False
+    scissors_pitch_deg: float = 19.9
+    scissors_roll_deg: float = 0.0
+    scissors_yaw_deg: float = 0.0
+
+
+@dataclass
+class Observation:
+    """Field-for-field the reference observation (`native_backend.py:182-196`)."""
+    nodes: np.ndarray
+    neighbors: np.ndarray
+    grip_flags: np.ndarray
+    hinge: np.ndarray
+    jaw_gap: float
+    grip_points: int
+    grip_error_m: float
+    blade: np.ndarray
+    reaction_n: np.ndarray
+    scissors: object = None
+    jaw_load_n: tuple = (0.0, 0.0)
+
+
+def solver_params(settings: Settings) -> SolverParams:
+    """The solver parameters `NativeTissue.initialize` sends (`native_backend.py:289-325`)."""
+    return SolverParams(substeps=int(settings.substeps), iterations=int(settings.iterations), tet_mode=TET_GUARDED,
+                        omega=1.0, edge_ks=0.0, young=float(settings.young_pa * METERS_PER_UNIT), beta_s=0.0,
+                        strain_damping=float(settings.strain_damping_s), bulk_damping=0.01, rayleigh=0.0,
+                        damping=0.0, contact_enabled=True, mu_s=0.6, mu_d=0.4, jaw_thickness=0.1,
+                        contact_radius=0.01, jaw_gap_closed=0.0, contact_compliance=1e-8)
+
+
+def _unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.linalg.norm(v)
+
+
+class NativeTissue:
+    """Owns one tissue on the GPU; like the reference, one session per process (close before constructing another)."""
+    active_session = False
+
+    def __init__(self, grid: TissueGrid, settings=Settings(), dll_path=None, initial_command=Command()):
+        if NativeTissue.active_session:
+            raise RuntimeError("the tissue solver is single-session; use separate processes for separate environments")
+        self.grid, self.settings = grid, settings
+        self.initial_command = initial_command
+        if settings.cutting_tool not in ("knife", "scissors"):
+            raise ValueError(f"Unknown cutting tool: {settings.cutting_tool}")
+        if settings.cutting_tool == "scissors":
+            raise ValueError("scissors are outside this path; use the knife")
+        from . import _lib
+        if dll_path is not None and Path(dll_path).resolve() != _lib.LIB_PATH.resolve():
+            raise ValueError(f"{dll_path}: this backend loads {_lib.LIB_PATH}")
+        self.dll_path = _lib.LIB_PATH
+        self.blade_axis = _unit(settings.blade_axis)
+        self.closed = True
+        NativeTissue.active_session = True
+        self.closed = False
+        try:
+            self.initialize()
+        except (RuntimeError, ValueError, OSError):
+            self.close()
+            raise
+
+    def initialize(self):
+        from .tissue import Tissue
+        g, s = self.grid, self.settings
+        self.params = solver_params(s)
+        from .tissue import build_topology
+        topo = build_topology(g)
+        tet_type = ((topo["tet_desc"] >> 1) & 7)[np.argsort(topo["tet_gen"])]
+        k_dev, k_hyd = neohookean_stiffness(s.young_pa * METERS_PER_UNIT, s.poisson, topo["type_volume"], tet_type)
+        self.tissue = Tissue(g, self.params, k_dev, k_hyd, g.positions, topology=topo)
+        self._tool = None
+        self._blade = None
+        self.command(self.initial_command)
+        self.tissue.set_gravity((0.0, 0.0, 0.0))
+        self._advance()
+        self.neighbors, self.topology_revision = None, None
+        self.tick = 0
+
+    def command(self, command: Command):
+        s = self.settings
+        if not np.isfinite((*command.hinge, *command.blade_center, command.closure)).all():
+            raise ValueError("Tool command contains non-finite values")
+        if not 0 <= command.closure <= 1 or not 0 <= command.scissors_closure <= 1:
+            raise ValueError("Jaw closure must be in [0,1]")
+        hinge = np.array(command.hinge, dtype=np.float64)
+        axis = np.array(s.jaw_axis, dtype=np.float64)
+        self._tool = dict(shaftA=hinge - GRASPER_SHAFT_LENGTH * axis, shaftB=hinge, shaftR=GRASPER_SHAFT_RADIUS,
+                          jawHinge=hinge, jawAxis=axis, jawMid=hinge + s.jaw_length / 2 * axis,
+                          jawNormal=np.array(s.jaw_normal, dtype=np.float64), jawHalfWidth=s.jaw_width / 2,
+                          jawLen=s.jaw_length, jawGap=s.jaw_open_gap * (1 - command.closure),
+                          grasp01=command.closure)
+        center = np.array(command.blade_center, dtype=np.float64)
+        offset = self.blade_axis * (s.blade_length / 2)
+        self._blade_next = (center - offset, center + offset) if command.blade_enabled else None
+        self._command = command
+
+    def _advance(self):
+        self.tissue.step(self.settings.dt, tool=self._tool, tool_armed=True)
+        nxt = self._blade_next
+        if nxt is not None and self._blade is not None:
+            from .tissue import swept_cut_tool
+            tool = swept_cut_tool(self._blade, nxt, 2 * self.settings.blade_radius)
+            if tool["valid"]:
+                self.tissue.detect_cut(tool)
+        self._blade = nxt
+
+    def step(self, command: Command):
+        if self.closed:
+            raise RuntimeError("Native session is closed")
+        self.command(command)
+        ramp = min(1.0, (self.tick + 1) * self.settings.dt / 0.5)
+        self.tissue.set_gravity((0.0, -9.81 / METERS_PER_UNIT * ramp, 0.0))
+        self._advance()
+        self.tick += 1
+        return self.observe()
+
+    def detect_cut(self, cut_tool):
+        """LCS_SetTool + LCS_DetectCut: cut the deformed lattice with a swept quad (see tissue.swept_cut_tool)."""
+        if self.closed:
+            raise RuntimeError("Native session is closed")
+        self.tissue.detect_cut(cut_tool)
+
+    def observe(self):
+        t = self.tissue
+        nodes = t.positions()
+        revision = t.revision()
+        if revision != self.topology_revision:
+            neighbors = t.neighbors()
+            neighbors.flags.writeable = False
+            self.neighbors, self.topology_revision = neighbors, revision
+        n = len(nodes)
+        blade = np.zeros(10, dtype=np.float32)
+        if self._blade is not None:
+            blade[0:3], blade[3:6] = self._blade[0], self._blade[1]
+            blade[6] = self.settings.blade_radius
+        force, contacts = t.reaction_force()
+        reaction = (force * METERS_PER_UNIT).astype(np.float32)
+        hinge = np.array(self._command.hinge, dtype=np.float32)
+        gap = float(self.settings.jaw_open_gap * (1 - self._command.closure))
+        if not np.isfinite(nodes).all() or not np.isfinite(reaction).all():
+            raise RuntimeError("Non-finite native state; simulation stopped")
+        return Observation(nodes, self.neighbors, np.zeros(n, dtype=np.int32), hinge, gap, 0, 0.0, blade, reaction)
+
+    def cut_edge_fractions(self):
+        """First blade fraction of each directed lattice edge, f32[N, 6], -1 where uncut (`cut.cu:159-162`)."""
+        return self.tissue.edge_cut_t.cpu().numpy().reshape(-1, 6)
+
+    def compact_topology(self):
+        """Stable remap of the live tets (colour order), edges and nodes after cuts, plus their counts."""
+        return self.tissue.compact()
+
+    def material_quality(self):
+        """[mean |J - 1|, min J] over the live tets of the current deformation."""
+        J = tet_determinants(self.tissue)
+        if len(J) == 0:
+            return np.zeros(2, dtype=np.float32)
+        return np.array((np.abs(J - 1).mean(), J.min()), dtype=np.float32)
+
+    def contact_depth_m(self):
+        """Deepest penetration of a lattice node into the grasper shaft capsule, in metres."""
+        t = self._tool
+        return _capsule_depth(self.tissue.positions(), t["shaftA"], t["shaftB"], t["shaftR"]) * METERS_PER_UNIT
+
+    def knife_body_depth_m(self):
+        """Deepest penetration of a lattice node into the knife capsule, in metres."""
+        if self._blade is None:
+            return 0.0
+        return _capsule_depth(self.tissue.positions(), self._blade[0], self._blade[1],
+                              self.settings.blade_radius) * METERS_PER_UNIT
+
+    def surface(self):
+        raise NotImplementedError("the reconstructed dual-contouring surface is outside the substep path")
+
+    def close(self):
+        if not self.closed:
+            self.tissue = None
+            self.closed = True
+            NativeTissue.active_session = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, exc_type, exc, traceback):
+        self.close()
+
+
+def _capsule_depth(points, a, b, r):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    ab = b - a
+    t = np.clip((points - a) @ ab / max(ab @ ab, 1e-20), 0, 1)
+    d = np.linalg.norm(points - (a + t[:, None] * ab), axis=1)
+    return float(max(0.0, (r - d).max()))
+
+
+def tet_determinants(tissue):
+    """det F of every live tet (generation order of the live ones), from the current positions."""
+    from .grid import KUHN_TETS
+    topo = tissue.topology
+    desc = tissue.tet_desc.cpu().numpy().view(np.uint32)
+    live = (desc & 1) != 0
+    base = (desc >> 4).astype(np.int64)[live]
+    ty = ((desc >> 1) & 7).astype(np.int64)[live]
+    nx, ny, _ = (d + 1 for d in tissue.grid.dims)
+    offs = np.array([[(c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nx * ny for c in row] for row in KUHN_TETS])
+    ids = base[:, None] + offs[ty]
+    x = tissue.positions().astype(np.float64)[ids]
+    Ds = np.stack((x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]), axis=2)
+    B = topo["type_inv_rest"][ty]
+    return np.linalg.det(Ds @ B)

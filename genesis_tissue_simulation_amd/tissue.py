@@ -103,7 +103,7 @@ class Tissue:
     """Device state of one tissue and the HIP entry points that advance and cut it."""
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
-                 device="cuda:0", topology=None):
+                 device="cuda:0", topology=None, use_graph=True, max_substeps=64):
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -161,6 +161,9 @@ class Tissue:
         scratch = int(self.lib.gts_scratch_floats(N, E, T, Eg))
         self.scratch = torch.zeros(scratch, dtype=torch.float32, device=dev)
         self.reaction = torch.zeros(4, dtype=torch.float32, device=dev)
+        self.max_substeps = int(max_substeps)
+        self.step_consts = torch.zeros(self.max_substeps * _lib.GTS_STEP_CONST_FLOATS, dtype=torch.float32,
+                                       device=dev)
         self.edge_offsets = np.ascontiguousarray(topo["edge_offsets"], dtype=np.int32)
         self.tet_offsets = np.ascontiguousarray(topo["tet_offsets"], dtype=np.int32)
         inv_rest = topo["type_inv_rest"].astype(np.float32).reshape(-1)
@@ -173,7 +176,8 @@ class Tissue:
                                      self.slot_support, self.tet_desc, self.tet_k_dev, self.tet_k_hyd,
                                      self.tet_alpha, self.tet_lambda, self.grid_edges, self.occupied, self.cut_mask,
                                      self.edge_cut_t, self.edge_skin_t, self.cut_points)],
-            self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr())
+            self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
+            self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)))
         self._tool_from = None
         check("gts_init_state", self.lib.gts_init_state(ct.byref(self.view), current_stream()))
 

@@ -94,12 +94,12 @@ def ellipsoid_spacing_for(tets, half_extents=(6.0, 3.0, 2.5)):
 
 def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets=250_000):
     """Kidney-scale single instance: an ellipsoid of semi-axes 6 x 3 x 2.5 cm rasterised at the spacing that gives
-    about 250k tets (1.65 mm; a 0.5 mm lattice would give 9M); per-tet compliance log-normal (median 1e-7,
+    about 250k tets (1.72 mm; a 0.5 mm lattice would give 9M); per-tet compliance log-normal (median 1e-7,
     sigma 0.5); a capsule grasper of radius 2 mm pressing 3 mm into the top; a blade on a seeded cubic Bezier over
     the top surface cutting for 300 steps; 20 substeps of 2 iterations."""
     rng = np.random.default_rng(seed)
     shape = Ellipsoid((6.0, 3.0, 2.5))
-    h = spacing if spacing is not None else 0.1654
+    h = spacing if spacing is not None else 0.1716
     grid = build_grid(h, shape=shape)
     T = tet_count(grid)
     compliance = np.exp(rng.normal(np.log(1e-7), 0.5, T))
@@ -112,7 +112,7 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
     def tool(step):
         depth = 0.3 * min(1.0, step / 60.0)
         y = top + r - depth
-        return dict(shaftA=(-1.5, y, 1.0), shaftB=(1.5, y, 1.0), shaftR=r, jawAxis=(0, 0, 1), jawNormal=(1, 0, 0))
+        return dict(shaftA=(-1.5, y, 0.0), shaftB=(1.5, y, 0.0), shaftR=r, jawAxis=(0, 0, 1), jawNormal=(1, 0, 0))
 
     # blade: cubic Bezier with 4 seeded control points on the top surface; a vertical segment 0.6 cm tall
     u = rng.uniform(-0.7, 0.7, size=(4, 2))

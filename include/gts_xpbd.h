@@ -114,7 +114,13 @@ typedef struct GtsTissueView {
     uint32_t* counters;          /* [8] 0 revision, 1 cut point counter, 2 refreshed revision, 3 stale flag */
     float* scratch;              /* [4*Eg + 2*nblocks + 64] detection records and scan partials */
     float* reaction;             /* [4] force of the tissue on the tools, contacts */
+    float* step_consts;          /* [step_const_capacity * GTS_STEP_CONST_FLOATS] per-substep tool shapes */
+    int step_const_capacity;     /* substeps the buffer holds */
+    int use_graph;               /* 1: capture the step into a hipGraph once and replay it */
 } GtsTissueView;
+
+/* Floats of one substep's tool shapes in GtsTissueView.step_consts (8 shapes of 23 floats). */
+#define GTS_STEP_CONST_FLOATS 184
 
 typedef struct GtsStepParams {
     float dt;
@@ -152,6 +158,8 @@ int gts_build_topology(const GtsLattice* lattice, GtsTopology* out);
 
 /* Host: J^(-2/3) as the tet kernel evaluates it (own cube-root sequence), for checks. */
 float gts_iso_pow(float J);
+/* Host: log x as the guarded tet kernel evaluates it (own atanh-series sequence), for checks. */
+float gts_log(float x);
 /* Host: floats of device scratch a tissue view needs (detection records + scan partials). */
 long long gts_scratch_floats(int n_nodes, int n_edges, int n_tets, int n_grid_edges);
 
@@ -172,6 +180,11 @@ int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, c
  * counts [3] (live tets, edges, nodes); all device. scratch as in the view. */
 int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap, int32_t* node_remap,
                          int32_t* counts, int32_t* node_used, void* stream);
+
+/* Timing of the dominant kernel (tet colour pass) with HIP events on its launch stream, per launch
+ * (in the spirit of LCS_ProfileEnable/LCS_ProfileRead, plugin_api.cpp:21-26). */
+int gts_profile_begin(void);
+int gts_profile_end(double* total_ms, long long* launches, long long* units);
 
 #ifdef __cplusplus
 }

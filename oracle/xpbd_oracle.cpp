@@ -60,6 +60,29 @@ inline float isoPow(float J) {
     return 1.f / (c * c);
 }
 inline double isoPow(double J) { return std::pow(J, -2.0 / 3.0); }
+
+// log of the guarded bulk strain (the reference calls logf, xpbd_solver.cu:780). The float oracle and the kernel share
+// this fixed sequence: x = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m-1)/(m+1)) summed to s^11, e ln2 in
+// two parts. The double oracle uses the library log.
+inline float guardLog(float x) {
+    uint32_t i;
+    std::memcpy(&i, &x, 4);
+    int e = (int)((i >> 23) & 0xffu) - 127;
+    uint32_t mb = (i & 0x7fffffu) | 0x3f800000u;
+    float m;
+    std::memcpy(&m, &mb, 4);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    float s = (m - 1.f) / (m + 1.f);
+    float s2 = s * s;
+    float p = s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * (0.111111112f + s2 * 0.0909090936f))));
+    float lm = 2.f * s + 2.f * s * p;
+    float fe = (float)e;
+    return fe * 0.693145752f + (fe * 1.42860677e-06f + lm);
+}
+inline double guardLog(double x) { return std::log(x); }
 template <class R> inline R fmin_(R a, R b) { return std::fmin(a, b); }
 template <class R> inline R fmax_(R a, R b) { return std::fmax(a, b); }
 
@@ -644,7 +667,7 @@ bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], con
     R CD = std::sqrt(fmax_<R>(q, (R)1e-12f));
     R devScale = (R)1 / CD;
     R volumeGradient = J > (R)0.05f ? iso * I1 / ((R)3 * positiveJ) : (R)0;
-    R CH = safeguarded ? std::log(positiveJ) + fmin_<R>((R)0, (J - (R)0.05f) / (R)0.05f) : J - (R)1;
+    R CH = safeguarded ? guardLog(positiveJ) + fmin_<R>((R)0, (J - (R)0.05f) / (R)0.05f) : J - (R)1;
     for (int k = 1; k < 4; k++) {
         V3<R> b = br[k - 1];
         gd[k] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[k] * volumeGradient) * devScale;
@@ -1257,6 +1280,7 @@ extern "C" int oracle_moller_trumbore_f64(const double* O, const double* D, cons
     return hit ? 1 : 0;
 }
 extern "C" float oracle_iso_pow_f32(float J) { return isoPow(J); }
+extern "C" float oracle_log_f32(float x) { return guardLog(x); }
 extern "C" double oracle_shape_distance_f64(const double* center, const double* u, const double* n, const double* half,
                                            const double* a, const double* b, double radius, int capsule,
                                            const double* x, double* normal) {

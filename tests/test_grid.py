@@ -1,0 +1,152 @@
+"""Lattice invariants (the reference's own CPU contract tests, `endoscopic_tissue/test_core.py:64-113, :198-236`,
+restated against this package's builder) and the benchmark lattices."""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.grid import (BEND_DTYPE, EDGE_CORNERS, ISECT_DTYPE, KUHN_TETS, Ellipsoid, Tab,
+                                                block_grid, build_grid, connectivity_table, directed_slot)
+from genesis_tissue_simulation_amd.workloads import block_6144, kidney_250k, tet_count
+
+
+def test_connectivity_exhaustive():
+    """All 4096 cut masks against an independent depth-first traversal (test_core.py:69-93)."""
+    table = connectivity_table()
+    for mask in range(4096):
+        visited, groups = set(), []
+        for start in range(8):
+            if start in visited:
+                continue
+            group, pending = set(), [start]
+            while pending:
+                node = pending.pop()
+                if node in group:
+                    continue
+                group.add(node)
+                for bit, (a, b) in enumerate(EDGE_CORNERS):
+                    if not mask & (1 << bit):
+                        if node == a:
+                            pending.append(int(b))
+                        elif node == b:
+                            pending.append(int(a))
+            groups.append(group)
+            visited.update(group)
+        assert table[mask, 0] == len(groups)
+        for label, group in enumerate(groups):
+            assert all(table[mask, node + 1] == label for node in group)
+
+
+def test_grid_boundary_and_reciprocal_neighbors():
+    """test_core.py:95-113 on the default block."""
+    grid = build_grid(0.6)
+    assert grid.intersections.dtype.itemsize == 40 and ISECT_DTYPE.itemsize == 40
+    assert grid.bends.dtype.itemsize == 32 and BEND_DTYPE.itemsize == 32
+    assert grid.counts.sum() == len(grid.intersections)
+    assert np.any(grid.pinned)
+    supported = np.all(grid.active[grid.corners] != 0, axis=1)
+    expected_mass = np.count_nonzero(supported) * (grid.spacing * .01) ** 3 * 1000
+    np.testing.assert_allclose(grid.mass.sum(), expected_mass, rtol=1e-6)
+    assert np.all(grid.mass[grid.active == 0] == 0)
+    for slot in range(6):
+        nodes = np.flatnonzero(grid.neighbors[:, slot] >= 0)
+        other = grid.neighbors[nodes, slot]
+        np.testing.assert_array_equal(grid.neighbors[other, slot ^ 1], nodes)
+        assert np.all(grid.active[nodes] & grid.active[other])
+    assert np.all((grid.intersections["t"] >= 0) & (grid.intersections["t"] <= 1))
+    distances = (np.abs(grid.intersections["position"]) - (3.0, 0.7, 2.0)).max(axis=1)
+    np.testing.assert_allclose(distances, 0, atol=1e-6)
+
+
+def test_rest_vectors_are_lattice_deltas():
+    grid = build_grid(0.5, vertical_spacing=0.25)
+    for slot in range(6):
+        nodes = np.flatnonzero(grid.neighbors[:, slot] >= 0)
+        delta = grid.positions[grid.neighbors[nodes, slot]] - grid.positions[nodes]
+        np.testing.assert_allclose(grid.rest_neighbors[nodes, slot], delta, atol=1e-6)
+
+
+def test_tab_grid_lies_on_box_union():
+    tab = Tab((3.0, .3, -1.0), (4.2, .7, 1.0), attached=0)
+    grid = build_grid(.2, density=1000, vertical_spacing=.05, tab=tab)
+    half = np.array((3.0, 0.7, 2.0))
+    lows = np.array((-half, (2.8, .3, -1.0)))
+    highs = np.array((half, (4.2, .7, 1.0)))
+    p = grid.intersections["position"].astype(np.float64)
+    on_face = np.zeros(len(p), dtype=bool)
+    for lo, hi in zip(lows, highs):
+        inside = np.all((p >= lo - 1e-5) & (p <= hi + 1e-5), axis=1)
+        face = np.any((np.abs(p - lo) < 1e-5) | (np.abs(p - hi) < 1e-5), axis=1)
+        on_face |= inside & face
+    assert on_face.all()
+
+
+def test_mass_lumping_is_a_24th_per_tet_corner():
+    """Each supported cell's six Kuhn tets put rho h^3 / 24 on each of their corners (grid.py:269-277)."""
+    grid = block_grid((4, 3, 3), 0.3, density=1000.0)
+    unit = 1000.0 * (0.3 * 0.01) ** 3 / 24
+    count = np.zeros(len(grid.positions))
+    nx, ny = 4, 3
+    for z in range(2):
+        for y in range(2):
+            for x in range(3):
+                for tet in KUHN_TETS:
+                    for c in tet:
+                        count[(x + (c & 1)) + nx * ((y + ((c >> 1) & 1)) + ny * (z + ((c >> 2) & 1)))] += 1
+    np.testing.assert_allclose(grid.mass, count * unit, rtol=1e-6)
+
+
+def test_block_6144_sizes():
+    w = block_6144()
+    g = w.grid
+    assert len(g.positions) == 17 * 9 * 9 and tet_count(g) == 6144
+    assert g.active.all() and g.inside.all()
+    assert int(g.pinned.sum()) == 17 * 9
+    assert np.all(g.positions[g.pinned != 0][:, 1] == 0)
+    # seeded jitter of 1e-4 cell on the free nodes, none on the pinned layer
+    jitter = w.initial_positions - g.positions
+    assert np.all(jitter[g.pinned != 0] == 0)
+    assert 0.5e-4 * 0.25 < jitter[g.pinned == 0].std() < 2e-4 * 0.25
+
+
+def test_grid_edges_have_four_voxels_in_bounds():
+    """Lattice edges that may be cut: exactly those with four incident voxels in bounds (cut.cuh:140-144)."""
+    g = block_grid((6, 5, 4), 0.25)
+    vx, vy, vz = g.dims
+    expect = set()
+    for axis in range(3):
+        for z in range(vz + 1):
+            for y in range(vy + 1):
+                for x in range(vx + 1):
+                    c = [x, y, z]
+                    if c[axis] >= g.dims[axis]:
+                        continue
+                    if all(0 < c[o] < g.dims[o] for o in range(3) if o != axis):
+                        expect.add((axis, x, y, z))
+    got = {tuple(int(v) for v in e) for e in g.edges}
+    assert got == expect
+
+
+def test_directed_slots():
+    g = block_grid((3, 3, 3), 0.25)
+    nx = 3
+    a = 4
+    assert directed_slot(a, a + 1, g.dims) == 6 * a + 1
+    assert directed_slot(a + 1, a, g.dims) == 6 * (a + 1) + 0
+    assert directed_slot(a, a + nx, g.dims) == 6 * a + 3
+    assert directed_slot(a, a + nx * nx, g.dims) == 6 * a + 5
+
+
+def test_ellipsoid_surface_crossings():
+    g = build_grid(0.3, shape=Ellipsoid((6.0, 3.0, 2.5)))
+    p = g.intersections["position"].astype(np.float64)
+    k = np.linalg.norm(p / np.array((6.0, 3.0, 2.5)), axis=1)
+    np.testing.assert_allclose(k, 1.0, atol=1e-4)
+    n = g.intersections["normal"].astype(np.float64)
+    assert np.all((n * p).sum(axis=1) > 0)  # outward
+    assert g.pinned.any() and np.all(g.positions[g.pinned != 0][:, 1] < -3.0 * 0.7 + 1e-6)
+
+
+@pytest.mark.slow
+def test_kidney_scale_lattice():
+    w = kidney_250k()
+    T = tet_count(w.grid)
+    assert 240_000 <= T <= 260_000

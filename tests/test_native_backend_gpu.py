@@ -1,0 +1,90 @@
+"""The drop-in NativeTissue on the GPU: API contract of the reference wrapper and parity of its step with the oracle
+driven by the same parameters and tool commands."""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.grid import build_grid
+from genesis_tissue_simulation_amd.native_backend import Command, NativeTissue, Settings, solver_params
+
+pytestmark = pytest.mark.gpu
+
+
+def commands(n):
+    out = []
+    for k in range(n):
+        out.append(Command(hinge=(-1.4, 1.2 - 0.02 * k, -0.5), closure=0.0,
+                           blade_center=(-2.0 + 0.25 * k, 0.3, 0.0), blade_enabled=True))
+    return out
+
+
+def test_contract_and_cut():
+    grid = build_grid(0.6)
+    with NativeTissue(grid) as tissue:
+        with pytest.raises(RuntimeError):
+            NativeTissue(grid)
+        with pytest.raises(ValueError):
+            tissue.step(Command(closure=2.0))
+        with pytest.raises(ValueError):
+            tissue.step(Command(hinge=(np.nan, 0, 0)))
+        obs = None
+        for cmd in commands(16):
+            obs = tissue.step(cmd)
+        assert obs.nodes.shape == (len(grid.positions), 3) and np.isfinite(obs.nodes).all()
+        assert obs.neighbors.shape == (len(grid.positions), 6) and not obs.neighbors.flags.writeable
+        assert obs.blade.shape == (10,) and obs.reaction_n.shape == (3,)
+        fractions = tissue.cut_edge_fractions()
+        assert fractions.shape == (len(grid.positions), 6)
+        cut = fractions >= 0
+        assert cut.any(), "the knife sweep must cut lattice edges"
+        assert np.all((fractions[cut] >= 0) & (fractions[cut] <= 1))
+        severed = (grid.neighbors >= 0) & (obs.neighbors < 0)
+        assert severed.any()
+        # a severed link is severed from both ends
+        nodes, slots = np.nonzero(severed)
+        other = grid.neighbors[nodes, slots]
+        assert np.all(obs.neighbors[other, slots ^ 1] < 0)
+        q = tissue.material_quality()
+        assert np.isfinite(q).all() and q[1] > 0
+    with pytest.raises(RuntimeError):
+        tissue.step(Command())
+    NativeTissue(grid).close()  # the session is free again
+
+
+def test_step_matches_oracle():
+    """The drop-in's step (guarded tets, fallback edges, strain and bulk damping, friction contact, knife cut) is the
+    same computation as the oracle given the same commands."""
+    from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+    from genesis_tissue_simulation_amd.params import neohookean_stiffness
+    from oracle.oracle import Oracle
+
+    grid = build_grid(0.6)
+    s = Settings()
+    with NativeTissue(grid, s) as tissue:
+        topo = tissue.tissue.topology
+        tet_type = ((topo["tet_desc"] >> 1) & 7)[np.argsort(topo["tet_gen"])]
+        kd, kh = neohookean_stiffness(s.young_pa * 0.01, s.poisson, topo["type_volume"], tet_type)
+        o = Oracle(grid, solver_params(s), grid.positions)
+        o.set_tet_stiffness(kd, kh)
+        # initialize: command + warm-up step at zero gravity
+        tissue_tool = tissue._tool
+        o.set_tool(tissue_tool, tool_armed=True, reset=True)
+        o.set_gravity((0, 0, 0))
+        o.step(s.dt)
+        blade = tissue._blade
+        for k, cmd in enumerate(commands(10)):
+            obs = tissue.step(cmd)
+            ramp = min(1.0, (k + 1) * s.dt / 0.5)
+            o.set_gravity((0, -981.0 * ramp, 0))
+            o.set_tool(tissue._tool, tool_armed=True)
+            o.step(s.dt)
+            nxt = tissue._blade
+            tool = swept_cut_tool(blade, nxt, 2 * s.blade_radius)
+            if tool["valid"]:
+                o.detect_cut(tool)
+            blade = nxt
+        po = o.state()[0]
+        err = float(np.abs(obs.nodes - po).max() / np.abs(po).max())
+        print(f"drop-in vs oracle: rel err {err:.3e}")
+        assert err <= 1e-5
+        np.testing.assert_array_equal(obs.nodes, po)
+        np.testing.assert_array_equal(obs.neighbors, o.cut_state()["neighbors"])
