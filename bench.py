@@ -85,6 +85,63 @@ def run_gpu(w, steps, warmup):
                 units=units.value, E=t.E, T=t.T, active=int((w.grid.active != 0).sum()))
 
 
+def run_batched(steps, warmup, count=1024):
+    """The batched workload: 1024 blocks of 2k-100k tets, each cut once, stepped in one persistent launch per step
+    with the launch shape picked by the autotuner; the step kernel is timed with events on its stream."""
+    import torch
+
+    from genesis_tissue_simulation_amd import autotune
+    from genesis_tissue_simulation_amd.batched import Batch
+    from genesis_tissue_simulation_amd.workloads import batched_blocks
+
+    t0 = time.perf_counter()
+    w = batched_blocks(count)
+    b = Batch(w.instances, w.params, gravity=w.gravity)
+    for k, cut in enumerate(w.cuts):
+        b.cut(k, cut)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    best, table = autotune.tune(b, w.dt)
+    for _ in range(warmup):
+        b.step(w.dt)
+    torch.cuda.synchronize()
+    live = b.live_tets()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(steps):
+        b.step(w.dt)
+    stop.record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernel_ms = start.elapsed_time(stop)
+    pos = b.pos[:, :3]
+    if not bool(torch.isfinite(pos).all()):
+        raise RuntimeError("non-finite batched positions")
+    active = sum(int((inst[0].active != 0).sum()) for inst in w.instances)
+    return dict(elapsed=elapsed, kernel_ms=kernel_ms, live=live, T=b.total_tets, E=int(b.edge_off[-1]),
+                active=active, substeps=w.params.substeps, best=best, table=table, setup_s=setup, count=count,
+                workload=w)
+
+
+def run_cpu_batched(w, threads, sample=8):
+    from oracle import oracle as orc
+
+    used = orc.set_threads(threads)
+    sims = []
+    for grid, p, kd, kh, init in w.instances[:sample]:
+        o = orc.Oracle(grid, p, init)
+        o.set_tet_stiffness(kd, kh)
+        o.set_gravity(w.gravity)
+        sims.append(o)
+    t0 = time.perf_counter()
+    for o in sims:
+        o.step(w.dt)
+    elapsed = time.perf_counter() - t0
+    tets = sum(o.T for o in sims)
+    return dict(value=tets * w.params.substeps / elapsed, threads=used, tets=tets)
+
+
 def run_cpu(w, steps, threads):
     from genesis_tissue_simulation_amd.tissue import swept_cut_tool
     from oracle import oracle as orc
@@ -115,6 +172,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--batched-count", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the tissue path runs on a single MI355X and has no multi-GPU mode "
@@ -153,8 +213,31 @@ def main():
                      "kernel": "k_tets (one colour of the tet pass)", "avg_launch_us": 1e3 * avg_launch_ms,
                      "launches": g["launches"], "bytes_per_tet_update": b_alg},
     }
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if not args.skip_batched:
+        bt = run_batched(args.batched_steps, 2, args.batched_count)
+        bv = bt["live"] * bt["substeps"] * args.batched_steps / bt["elapsed"]
+        bb = algorithmic_bytes_per_tet_substep(bt["active"], bt["T"], bt["E"])
+        bach = bb * bt["live"] * bt["substeps"] * args.batched_steps / (bt["kernel_ms"] * 1e-3) / 1e9
+        line["batched"] = {
+            "value": bv, "unit": "tet-substeps/s", "vs_plain": bv / value,
+            "ms_per_step": 1e3 * bt["elapsed"] / args.batched_steps,
+            "config": {"workload": f"{bt['count']} blocks, tets log-uniform 2k-100k, one straight cut each",
+                       "tets": bt["T"], "live_tets": bt["live"], "substeps": bt["substeps"], "iterations": 1,
+                       "launch": {k: bt["best"][k] for k in ("block_size", "workgroups_per_cu", "lds_nodes",
+                                                               "n_workgroups")}},
+            "roofline": {"bound": "hbm", "achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": bach / HBM_PEAK_GBS, "kernel": "k_batched_step (one launch per step)",
+                         "avg_launch_us": 1e3 * bt["kernel_ms"] / args.batched_steps, "bytes_per_tet_substep": bb,
+                         "traffic": None},
+            "autotune": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in bt["table"]],
+        }
+        if not args.no_cpu:
+            cb = run_cpu_batched(bt["workload"], threads)
+            line["batched"]["cpu_baseline"] = {"value": cb["value"], "unit": "tet-substeps/s", "cores": cb["threads"],
+                                               "kind": "port", "sample": f"1 step of the first 8 instances "
+                                                                         f"({cb['tets']} tets)"}
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         c = run_cpu(w, args.cpu_steps, threads)
         line["cpu_baseline"] = {
             "value": c["live"] * substeps * args.cpu_steps / c["elapsed"], "unit": "tet-substeps/s",

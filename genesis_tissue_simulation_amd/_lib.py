@@ -65,7 +65,21 @@ class GtsCutTool(ct.Structure):
                 ("d", F), ("valid", I), ("aabbMin", V3), ("aabbMax", V3)]
 
 
-STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool)
+class GtsBatchInstance(ct.Structure):
+    _fields_ = [("node_off", I), ("n_nodes", I), ("edge_off", I), ("n_edges", I), ("tet_off", I), ("n_tets", I),
+                ("nx", I), ("nxy", I), ("n_edge_colours", I), ("n_tet_colours", I), ("colour_off", I),
+                ("reserved", I), ("edge_ks", F), ("rest_len", F * 3), ("inv_rest", F * 54)]
+
+
+class GtsBatchView(ct.Structure):
+    _fields_ = [("n_instances", I), ("instances", P), ("colour_table", P), ("work_offsets", P), ("work_items", P),
+                ("n_workgroups", I), ("block_size", I), ("lds_nodes", I), ("use_graph", I), ("gravity", F * 3),
+                ("pos", P), ("prev", P), ("vel", P), ("mass", P), ("edge_desc", P), ("tet_desc", P), ("k_dev", P),
+                ("k_hyd", P), ("lambda_edge", P), ("lambda_tet", P)]
+
+
+STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool, GtsBatchInstance,
+           GtsBatchView)
 
 # name -> (argtypes, restype); the exported C ABI
 SIGNATURES = {
@@ -85,7 +99,7 @@ SIGNATURES = {
     "gts_profile_begin": ((), I),
     "gts_profile_end": ((P, P, P), I),
     "gts_batched_plan": ((P, I, I, P), I),
-    "gts_batched_step": ((P, P, P, I, I, P), I),
+    "gts_batched_step": ((P, P, P), I),
     "gts_batched_lds_bytes": ((), I),
     "gts_adjoint_forward": ((P, P, P, I, I, P, P), I),
     "gts_adjoint_backward": ((P, P, P, I, I, P, P, P, P), I),
@@ -114,7 +128,7 @@ def load():
         fn.argtypes, fn.restype = list(args), res
     if lib.gts_abi_version() != GTS_ABI_VERSION:
         raise NativeError("libgts_xpbd.so ABI version mismatch")
-    sizes = np.zeros(6, dtype=np.int32)
+    sizes = np.zeros(8, dtype=np.int32)
     lib.gts_struct_sizes(sizes.ctypes.data_as(P))
     for size, st in zip(sizes, STRUCTS):
         if int(size) != ct.sizeof(st):

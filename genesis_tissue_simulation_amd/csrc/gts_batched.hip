@@ -1,0 +1,270 @@
+// Batched XPBD stepping of many independent tissue instances of uneven size in one persistent launch.
+//
+// The reference runs one session per process (native_backend.py:213-218); here a whole population of lattice
+// instances advances together. Each workgroup takes a list of instances (longest-processing-time packing on the
+// host) and runs every substep of a step for one instance at a time:
+//   * the instance's node positions (float4: x, y, z, inverse mass) are staged in LDS when they fit, so all
+//     Gauss-Seidel colour passes of all substeps gather and scatter on-chip; larger instances work on their own
+//     slice of global memory, which only this workgroup touches;
+//   * colours are separated by workgroup barriers instead of kernel launches: the colour order is the reference's
+//     (xpbd/xpbd_solver.cu:3581-3612), so each instance computes exactly what the single-instance path computes;
+//   * HBM traffic per substep is the packed edge / tet descriptors and per-tet stiffness (8 B per tet, 4 B per edge);
+//     positions, velocities and prev stay on chip or in the instance's L2-resident slice.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <queue>
+#include <vector>
+
+#include "../../include/gts_xpbd.h"
+#include "gts_common.h"
+#include "gts_solver.h"
+
+#define HIP_TRY(call)                                   \
+    do {                                                \
+        hipError_t _e = (call);                         \
+        if (_e != hipSuccess) return -(int)_e;          \
+    } while (0)
+
+namespace {
+
+struct BatchStep {
+    int substeps, iters, mode, n_items;
+    float h, invH, invH2, omega, velScale, gx, gy, gz, dOverH, bOverH;
+    int lds_nodes;
+};
+
+template <int MODE, bool DAMPED, class Store>
+__device__ void run_instance(const Store& S, const GtsBatchInstance& I, const BatchStep& P, const int* colours,
+                             const uint32_t* __restrict__ edge_desc, const uint32_t* __restrict__ tet_desc,
+                             const float* __restrict__ k_dev, const float* __restrict__ k_hyd,
+                             float* __restrict__ lam_edge, float* __restrict__ lam_tet, const float* __restrict__ mass,
+                             float4* __restrict__ prev, float4* __restrict__ vel) {
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int N = I.n_nodes, T = I.n_tets;
+    const int nx = I.nx, nxy = I.nxy;
+    const int* ecol = colours;
+    const int* tcol = colours + I.n_edge_colours + 1;
+    const float at = I.edge_ks > 0.f ? 1.f / (I.edge_ks * P.h * P.h) : 0.f;
+    const float betaGammaH = 0.f;
+    for (int n = 0; n < P.substeps; n++) {
+        // k_predict (xpbd_solver.cu:393-412) with the gravity force g * mass of k_SetGravity
+        for (int i = tid; i < N; i += bs) {
+            float4 p = S.ld(i);
+            prev[i] = p;
+            if (p.w > 0.f) {
+                float m = mass[i];
+                f3 F = mk3(P.gx * m, P.gy * m, P.gz * m);
+                f3 a = F * p.w;
+                float4 v0 = vel[i];
+                f3 v = mk3(v0.x, v0.y, v0.z) + a * P.h;
+                vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+                f3 x = xyz4(p) + v * P.h;
+                S.st(i, make_float4(x.x, x.y, x.z, p.w));
+            }
+        }
+        __syncthreads();
+        for (int it = 0; it < P.iters; it++) {
+            const bool read = it > 0, write = P.iters > 1;
+            if (at > 0.f)
+                for (int c = 0; c < I.n_edge_colours; c++) {
+                    for (int k = ecol[c] + tid; k < ecol[c + 1]; k += bs) {
+                        uint32_t d = edge_desc[k];
+                        if (!(d & 1u)) continue;
+                        int axis = (int)((d >> 1) & 3u);
+                        int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? nx : nxy));
+                        float lam = read ? lam_edge[k] : 0.f, out;
+                        if (edge_project<DAMPED>(S, prev, i, j, I.rest_len[axis], at, P.omega, betaGammaH, lam, out) &&
+                            write)
+                            lam_edge[k] = out;
+                    }
+                    __syncthreads();
+                }
+            for (int c = 0; c < I.n_tet_colours; c++) {
+                for (int k = tcol[c] + tid; k < tcol[c + 1]; k += bs) {
+                    uint32_t d = tet_desc[k];
+                    if (!(d & 1u)) continue;
+                    int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+                    int id[4];
+                    for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), nx, nxy);
+                    float ad = P.invH2 / k_dev[k], ah = P.invH2 / k_hyd[k];
+                    float lamD = read ? lam_tet[k] : 0.f, lamH = read ? lam_tet[T + k] : 0.f;
+                    if (tet_project<MODE, DAMPED>(S, prev, id, I.inv_rest[type], ad, ah, lamD, lamH, P.omega,
+                                                  P.dOverH, P.bOverH) &&
+                        write) {
+                        lam_tet[k] = lamD;
+                        lam_tet[T + k] = lamH;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // k_update_velocity + k_apply_velscale (xpbd_solver.cu:871-919)
+        for (int i = tid; i < N; i += bs) {
+            float4 p = S.ld(i);
+            f3 v = p.w > 0.f ? (xyz4(p) - xyz4(prev[i])) * P.invH : mk3(0, 0, 0);
+            v = p.w > 0.f ? v * P.velScale : mk3(0, 0, 0);
+            vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+        }
+        __syncthreads();
+    }
+}
+
+template <int MODE, bool DAMPED>
+__global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBatchInstance* __restrict__ instances,
+                                                       const int* __restrict__ colour_table,
+                                                       const int* __restrict__ work_offsets,
+                                                       const int* __restrict__ work_items, float4* __restrict__ pos,
+                                                       float4* __restrict__ prev, float4* __restrict__ vel,
+                                                       const float* __restrict__ mass,
+                                                       const uint32_t* __restrict__ edge_desc,
+                                                       const uint32_t* __restrict__ tet_desc,
+                                                       const float* __restrict__ k_dev, const float* __restrict__ k_hyd,
+                                                       float* __restrict__ lam_edge, float* __restrict__ lam_tet) {
+    extern __shared__ float4 lds_nodes[];
+    const int wg = blockIdx.x;
+    for (int w = work_offsets[wg]; w < work_offsets[wg + 1]; w++) {
+        const GtsBatchInstance I = instances[work_items[w]];
+        float4* gpos = pos + I.node_off;
+        float4* gprev = prev + I.node_off;
+        float4* gvel = vel + I.node_off;
+        const float* gmass = mass + I.node_off;
+        const int* colours = colour_table + I.colour_off;
+        const uint32_t* ed = edge_desc + I.edge_off;
+        const uint32_t* td = tet_desc + I.tet_off;
+        const float* kd = k_dev + I.tet_off;
+        const float* kh = k_hyd + I.tet_off;
+        float* le = lam_edge + I.edge_off;
+        float* lt = lam_tet + 2 * (size_t)I.tet_off;
+        if (I.n_nodes <= P.lds_nodes) {
+            for (int i = threadIdx.x; i < I.n_nodes; i += blockDim.x) lds_nodes[i] = gpos[i];
+            __syncthreads();
+            run_instance<MODE, DAMPED>(LdsNodes{lds_nodes}, I, P, colours, ed, td, kd, kh, le, lt, gmass, gprev, gvel);
+            for (int i = threadIdx.x; i < I.n_nodes; i += blockDim.x) gpos[i] = lds_nodes[i];
+        } else {
+            run_instance<MODE, DAMPED>(GlobalNodes{gpos}, I, P, colours, ed, td, kd, kh, le, lt, gmass, gprev, gvel);
+        }
+        __syncthreads();
+    }
+}
+
+struct BatchGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+};
+std::vector<BatchGraph> g_batch_graphs;
+hipStream_t g_batch_capture = nullptr;
+
+template <int MODE, bool DAMPED>
+void launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
+    size_t lds = (size_t)v->lds_nodes * sizeof(float4);
+    auto kern = k_batched_step<MODE, DAMPED>;
+    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    kern<<<v->n_workgroups, v->block_size, lds, s>>>(
+        P, reinterpret_cast<const GtsBatchInstance*>(v->instances), v->colour_table, v->work_offsets, v->work_items,
+        (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, v->mass, v->edge_desc, v->tet_desc, v->k_dev, v->k_hyd,
+        v->lambda_edge, v->lambda_tet);
+}
+
+void enqueue(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
+    if (damped) {
+        if (P.mode == 0) launch<0, true>(v, P, s);
+        else if (P.mode == 1) launch<1, true>(v, P, s);
+        else launch<2, true>(v, P, s);
+    } else {
+        if (P.mode == 0) launch<0, false>(v, P, s);
+        else if (P.mode == 1) launch<1, false>(v, P, s);
+        else launch<2, false>(v, P, s);
+    }
+}
+
+}  // namespace
+
+extern "C" int gts_batched_lds_bytes(void) {
+    int dev = 0, bytes = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    return bytes;
+}
+
+// Longest-processing-time packing: instances by descending cost, each to the currently least-loaded workgroup.
+// offsets [n_workgroups + 1] and items [n] are host arrays.
+extern "C" int gts_batched_plan(const double* cost, int n, int n_workgroups, int32_t* offsets_items) {
+    if (!cost || n < 0 || n_workgroups <= 0 || !offsets_items) return GTS_ERR_BAD_ARG;
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    typedef std::pair<double, int> Load;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int w = 0; w < n_workgroups; w++) heap.push(Load(0.0, w));
+    std::vector<std::vector<int>> lists(n_workgroups);
+    for (int i : order) {
+        Load l = heap.top();
+        heap.pop();
+        lists[l.second].push_back(i);
+        heap.push(Load(l.first + cost[i], l.second));
+    }
+    int32_t* offsets = offsets_items;
+    int32_t* items = offsets_items + n_workgroups + 1;
+    int k = 0;
+    for (int w = 0; w < n_workgroups; w++) {
+        offsets[w] = k;
+        for (int i : lists[w]) items[k++] = i;
+    }
+    offsets[n_workgroups] = k;
+    return GTS_OK;
+}
+
+extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream) {
+    if (!v || !p || v->n_workgroups <= 0 || v->block_size <= 0 || v->block_size > 1024) return GTS_ERR_BAD_ARG;
+    if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
+    if ((long long)v->lds_nodes * 16 > (long long)gts_batched_lds_bytes()) return GTS_ERR_BAD_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    BatchStep P;
+    std::memset(&P, 0, sizeof(P));
+    P.substeps = p->substeps > 0 ? p->substeps : 1;
+    P.h = p->dt / (float)P.substeps;
+    if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
+    P.iters = p->iterations > 0 ? p->iterations : 1;
+    P.mode = p->tet_mode;
+    P.invH = 1.f / P.h;
+    P.invH2 = 1.f / (P.h * P.h);
+    P.omega = p->omega;
+    P.velScale = (1.f - p->damping) / (1.f + p->rayleigh * P.h);
+    P.gx = v->gravity[0];
+    P.gy = v->gravity[1];
+    P.gz = v->gravity[2];
+    P.dOverH = fmaxf(0.f, p->strain_damping) / P.h;
+    P.bOverH = p->bulk_damping / P.h;
+    P.lds_nodes = v->lds_nodes;
+    P.n_items = v->n_instances;
+    const bool damped = P.dOverH != 0.f || P.bOverH != 0.f;
+    if (!v->use_graph) {
+        enqueue(v, P, damped, s);
+        return -(int)hipGetLastError();
+    }
+    std::vector<unsigned char> key(sizeof(GtsBatchView) + sizeof(BatchStep));
+    std::memcpy(key.data(), v, sizeof(GtsBatchView));
+    std::memcpy(key.data() + sizeof(GtsBatchView), &P, sizeof(BatchStep));
+    for (auto& g : g_batch_graphs)
+        if (g.key == key) return -(int)hipGraphLaunch(g.exec, s);
+    if (!g_batch_capture) HIP_TRY(hipStreamCreateWithFlags(&g_batch_capture, hipStreamNonBlocking));
+    HIP_TRY(hipStreamBeginCapture(g_batch_capture, hipStreamCaptureModeThreadLocal));
+    enqueue(v, P, damped, g_batch_capture);
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamEndCapture(g_batch_capture, &graph));
+    BatchGraph g;
+    g.key = key;
+    hipError_t err = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    HIP_TRY(err);
+    if (g_batch_graphs.size() >= 16) {
+        hipGraphExecDestroy(g_batch_graphs.front().exec);
+        g_batch_graphs.erase(g_batch_graphs.begin());
+    }
+    g_batch_graphs.push_back(g);
+    return -(int)hipGraphLaunch(g.exec, s);
+}

@@ -21,6 +21,7 @@
 #include "../../include/gts_xpbd.h"
 #include "gts_common.h"
 #include "gts_contact.h"
+#include "gts_solver.h"
 
 #define GTS_BLOCK 256
 
@@ -182,25 +183,10 @@ __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fal
         alphaTilde = fallbackInvEh2 / restLen;
     }
     int i = (int)(d >> 4), j = i + axis_stride(L, axis);
-    float4 pi = pos[i], pj = pos[j];
-    float wi = pi.w, wj = pj.w, wsum = wi + wj;
-    if (wsum <= 0.f) return;
-    f3 dd = xyz(pi) - xyz(pj);
-    float len = len3(dd);
-    if (len < 1e-9f) return;
-    f3 n = dd * (1.f / len);
-    float C = len - restLen;
-    float gamma = betaGammaH;
-    // Without Kelvin damping gamma * gradDx is zero and prev is not read (only the sign of a zero could differ).
-    float gradDx = DAMPED ? dot3(n, (xyz(pi) - xyz(prev[i])) - (xyz(pj) - xyz(prev[j]))) : 0.f;
-    float lam = READ_LAMBDA ? lambda[e] : 0.f;
-    float dl = (-C - alphaTilde * lam - gamma * gradDx) / ((1.f + gamma) * wsum + alphaTilde);
-    float applied = omega * dl;
-    if (WRITE_LAMBDA) lambda[e] = lam + applied;
-    f3 xi = xyz(pi) + n * (wi * applied);
-    f3 xj = xyz(pj) - n * (wj * applied);
-    pos[i] = make_float4(xi.x, xi.y, xi.z, wi);
-    pos[j] = make_float4(xj.x, xj.y, xj.z, wj);
+    float lam = READ_LAMBDA ? lambda[e] : 0.f, out;
+    if (edge_project<DAMPED>(GlobalNodes{pos}, prev, i, j, restLen, alphaTilde, omega, betaGammaH, lam, out) &&
+        WRITE_LAMBDA)
+        lambda[e] = out;
 }
 
 // k_solve_tet (xpbd_solver.cu:749-829) over one colour. MODE 0 keeps the hydrostatic row alone; 1 and 2 are the
@@ -216,105 +202,13 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
     if (!(d & 1u)) return;
     int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
     int id[4];
-    float4 q[4];
-    for (int c = 0; c < 4; c++) {
-        id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);
-        q[c] = pos[id[c]];
-    }
-    f3 p[4] = {xyz(q[0]), xyz(q[1]), xyz(q[2]), xyz(q[3])};
-    float w[4] = {q[0].w, q[1].w, q[2].w, q[3].w};
-    f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
-    const float* B = L.inv_rest[type];
-    f3 br[3] = {mk3(B[0], B[1], B[2]), mk3(B[3], B[4], B[5]), mk3(B[6], B[7], B[8])};
-    f3 F[3];
-    for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
-    f3 cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
-    float ah = alpha[T + e];
-    float J = dot3(F[0], cof[0]);
-    f3 gh[4], gd[4];
-    for (int c = 1; c < 4; c++) {
-        f3 b = br[c - 1];
-        gh[c] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
-    }
-    float scale = 1.f + dampingOverH;
-    float lamH = READ_LAMBDA ? lambda[T + e] : 0.f;
-    if (MODE == GTS_TET_VOLUME) {
-        float CH = J - 1.f;
-        gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
-        float bulkScale = scale;
-        float effectiveAh = ah / bulkScale;
-        float vh = 0.f;
-        float schur = effectiveAh;
-        if (DAMPED)
-            for (int c = 0; c < 4; c++) vh += dot3(gh[c], p[c] - xyz(prev[id[c]]));
-        for (int c = 0; c < 4; c++) schur += w[c] * dot3(gh[c], gh[c]);
-        float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
-        if (!(schur > 0.f)) return;
-        float dlH = omega * rh / schur;
-        if (!isfinite(dlH)) return;
-        if (WRITE_LAMBDA) lambda[T + e] = lamH + dlH;
-        for (int c = 0; c < 4; c++) {
-            f3 x = p[c] + gh[c] * dlH * w[c];
-            pos[id[c]] = make_float4(x.x, x.y, x.z, w[c]);
-        }
-        return;
-    }
-    const bool safeguarded = MODE == GTS_TET_GUARDED;
-    float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
-    float ad = alpha[e];
-    float positiveJ = fmaxf(J, 0.05f);
-    float iso = iso_pow(positiveJ);
-    float qq = fmaxf(iso * I1 - 3.f, 0.f);
-    float CD = sqrtf(fmaxf(qq, 1e-12f));
-    float devScale = 1.f / CD;
-    float volumeGradient = J > 0.05f ? iso * I1 / (3.f * positiveJ) : 0.f;
-    float CH = safeguarded ? own_log(positiveJ) + fminf(0.f, (J - 0.05f) / 0.05f) : J - 1.f;
-    for (int c = 1; c < 4; c++) {
-        f3 b = br[c - 1];
-        gd[c] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[c] * volumeGradient) * devScale;
-        if (safeguarded) gh[c] = gh[c] * (1.f / positiveJ);
-    }
-    gd[0] = (gd[1] + gd[2] + gd[3]) * -1.f;
-    gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
-    float dd = 0.f, dh = 0.f, vd = 0.f, vh = 0.f;
-    for (int c = 0; c < 4; c++) {
-        dd += w[c] * dot3(gd[c], gd[c]);
-        dh += w[c] * dot3(gd[c], gh[c]);
-        if (DAMPED) {
-            f3 dx = p[c] - xyz(prev[id[c]]);
-            vd += dot3(gd[c], dx);
-            vh += dot3(gh[c], dx);
-        }
-    }
-    float bulkScale = safeguarded ? 1.f + bulkDampingOverH : scale;
-    float effectiveAd = ad / scale, effectiveAh = ah / bulkScale;
-    float A = dd + effectiveAd, Bc = dh, gamma = Bc / A;
-    float schur = effectiveAh + effectiveAd * gamma * gamma;
-    for (int c = 0; c < 4; c++) {
-        f3 residual = gh[c] - gd[c] * gamma;
-        schur += w[c] * dot3(residual, residual);
-    }
-    float lamD = READ_LAMBDA ? lambda[e] : 0.f;
-    float rd = (-CD - ad * lamD - dampingOverH * vd) / scale;
-    float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
-    if (!(A > 0.f) || !(schur > 0.f)) return;
-    float dlH = omega * (rh - gamma * rd) / schur, dlD = (omega * rd - Bc * dlH) / A;
-    if (!isfinite(dlD) || !isfinite(dlH)) return;
-    if (safeguarded) {
-        float largest = 0.f;
-        for (int c = 0; c < 4; c++) largest = fmaxf(largest, len3((gd[c] * dlD + gh[c] * dlH) * w[c]));
-        float limit = 0.15f / fmaxf(len3(br[0]), fmaxf(len3(br[1]), len3(br[2])));
-        float accepted = largest > limit ? limit / largest : 1.f;
-        dlD *= accepted;
-        dlH *= accepted;
-    }
-    if (WRITE_LAMBDA) {
-        lambda[e] = lamD + dlD;
-        lambda[T + e] = lamH + dlH;
-    }
-    for (int c = 0; c < 4; c++) {
-        f3 x = p[c] + (gd[c] * dlD + gh[c] * dlH) * w[c];
-        pos[id[c]] = make_float4(x.x, x.y, x.z, w[c]);
+    for (int c = 0; c < 4; c++) id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);
+    float lamD = READ_LAMBDA ? lambda[e] : 0.f, lamH = READ_LAMBDA ? lambda[T + e] : 0.f;
+    if (tet_project<MODE, DAMPED>(GlobalNodes{pos}, prev, id, L.inv_rest[type], alpha[e], alpha[T + e], lamD, lamH,
+                                  omega, dampingOverH, bulkDampingOverH) &&
+        WRITE_LAMBDA) {
+        lambda[e] = lamD;
+        lambda[T + e] = lamH;
     }
 }
 

@@ -1,0 +1,145 @@
+// Constraint projections of the XPBD substep as device functions, shared by the per-colour kernels of the single
+// instance and the persistent batched and adjoint kernels, so that every path rounds identically.
+//
+// Node storage is abstracted by a store type with ld(i) / st(i, v) on float4 (x, y, z, inverse mass): LDS-staged for
+// instances that fit in a workgroup's LDS, global memory otherwise.
+#pragma once
+#include "gts_common.h"
+
+struct GlobalNodes {
+    float4* p;
+    __device__ __forceinline__ float4 ld(int i) const { return p[i]; }
+    __device__ __forceinline__ void st(int i, float4 v) const { p[i] = v; }
+};
+struct LdsNodes {
+    float4* p;  // points into __shared__ memory
+    __device__ __forceinline__ float4 ld(int i) const { return p[i]; }
+    __device__ __forceinline__ void st(int i, float4 v) const { p[i] = v; }
+};
+
+__device__ __forceinline__ f3 xyz4(float4 p) { return mk3(p.x, p.y, p.z); }
+
+// k_solve_edges (xpbd/xpbd_solver.cu:484-524) on one edge between nodes i and j. `prev` is read only when the
+// Kelvin-Voigt term is on. Returns false where the reference returns without writing.
+template <bool DAMPED, class Store>
+__device__ __forceinline__ bool edge_project(const Store& S, const float4* __restrict__ prev, int i, int j,
+                                             float restLen, float alphaTilde, float omega, float betaGammaH,
+                                             float lam, float& lamOut) {
+    float4 pi = S.ld(i), pj = S.ld(j);
+    float wi = pi.w, wj = pj.w, wsum = wi + wj;
+    if (wsum <= 0.f) return false;
+    f3 dd = xyz4(pi) - xyz4(pj);
+    float len = len3(dd);
+    if (len < 1e-9f) return false;
+    f3 n = dd * (1.f / len);
+    float C = len - restLen;
+    float gamma = betaGammaH;
+    float gradDx = DAMPED ? dot3(n, (xyz4(pi) - xyz4(prev[i])) - (xyz4(pj) - xyz4(prev[j]))) : 0.f;
+    float dl = (-C - alphaTilde * lam - gamma * gradDx) / ((1.f + gamma) * wsum + alphaTilde);
+    float applied = omega * dl;
+    lamOut = lam + applied;
+    f3 xi = xyz4(pi) + n * (wi * applied);
+    f3 xj = xyz4(pj) - n * (wj * applied);
+    S.st(i, make_float4(xi.x, xi.y, xi.z, wi));
+    S.st(j, make_float4(xj.x, xj.y, xj.z, wj));
+    return true;
+}
+
+// k_solve_tet (xpbd/xpbd_solver.cu:749-829) on one tet with nodes id[4]. MODE 0 keeps the hydrostatic row alone;
+// 1 and 2 are the coupled 2x2 block, legacy and guarded. lamD / lamH are read and updated.
+template <int MODE, bool DAMPED, class Store>
+__device__ __forceinline__ bool tet_project(const Store& S, const float4* __restrict__ prev, const int id[4],
+                                            const float* B, float ad, float ah, float& lamD, float& lamH,
+                                            float omega, float dampingOverH, float bulkDampingOverH) {
+    float4 q[4];
+    for (int c = 0; c < 4; c++) q[c] = S.ld(id[c]);
+    f3 p[4] = {xyz4(q[0]), xyz4(q[1]), xyz4(q[2]), xyz4(q[3])};
+    float w[4] = {q[0].w, q[1].w, q[2].w, q[3].w};
+    f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
+    f3 br[3] = {mk3(B[0], B[1], B[2]), mk3(B[3], B[4], B[5]), mk3(B[6], B[7], B[8])};
+    f3 F[3];
+    for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
+    f3 cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
+    float J = dot3(F[0], cof[0]);
+    f3 gh[4], gd[4];
+    for (int c = 1; c < 4; c++) {
+        f3 b = br[c - 1];
+        gh[c] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
+    }
+    float scale = 1.f + dampingOverH;
+    if (MODE == 0) {
+        float CH = J - 1.f;
+        gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
+        float bulkScale = scale;
+        float effectiveAh = ah / bulkScale;
+        float vh = 0.f;
+        float schur = effectiveAh;
+        if (DAMPED)
+            for (int c = 0; c < 4; c++) vh += dot3(gh[c], p[c] - xyz4(prev[id[c]]));
+        for (int c = 0; c < 4; c++) schur += w[c] * dot3(gh[c], gh[c]);
+        float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
+        if (!(schur > 0.f)) return false;
+        float dlH = omega * rh / schur;
+        if (!isfinite(dlH)) return false;
+        lamH = lamH + dlH;
+        for (int c = 0; c < 4; c++) {
+            f3 x = p[c] + gh[c] * dlH * w[c];
+            S.st(id[c], make_float4(x.x, x.y, x.z, w[c]));
+        }
+        return true;
+    }
+    const bool safeguarded = MODE == 2;
+    float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    float positiveJ = fmaxf(J, 0.05f);
+    float iso = iso_pow(positiveJ);
+    float qq = fmaxf(iso * I1 - 3.f, 0.f);
+    float CD = sqrtf(fmaxf(qq, 1e-12f));
+    float devScale = 1.f / CD;
+    float volumeGradient = J > 0.05f ? iso * I1 / (3.f * positiveJ) : 0.f;
+    float CH = safeguarded ? own_log(positiveJ) + fminf(0.f, (J - 0.05f) / 0.05f) : J - 1.f;
+    for (int c = 1; c < 4; c++) {
+        f3 b = br[c - 1];
+        gd[c] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[c] * volumeGradient) * devScale;
+        if (safeguarded) gh[c] = gh[c] * (1.f / positiveJ);
+    }
+    gd[0] = (gd[1] + gd[2] + gd[3]) * -1.f;
+    gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
+    float dd = 0.f, dh = 0.f, vd = 0.f, vh = 0.f;
+    for (int c = 0; c < 4; c++) {
+        dd += w[c] * dot3(gd[c], gd[c]);
+        dh += w[c] * dot3(gd[c], gh[c]);
+        if (DAMPED) {
+            f3 dx = p[c] - xyz4(prev[id[c]]);
+            vd += dot3(gd[c], dx);
+            vh += dot3(gh[c], dx);
+        }
+    }
+    float bulkScale = safeguarded ? 1.f + bulkDampingOverH : scale;
+    float effectiveAd = ad / scale, effectiveAh = ah / bulkScale;
+    float A = dd + effectiveAd, Bc = dh, gamma = Bc / A;
+    float schur = effectiveAh + effectiveAd * gamma * gamma;
+    for (int c = 0; c < 4; c++) {
+        f3 residual = gh[c] - gd[c] * gamma;
+        schur += w[c] * dot3(residual, residual);
+    }
+    float rd = (-CD - ad * lamD - dampingOverH * vd) / scale;
+    float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
+    if (!(A > 0.f) || !(schur > 0.f)) return false;
+    float dlH = omega * (rh - gamma * rd) / schur, dlD = (omega * rd - Bc * dlH) / A;
+    if (!isfinite(dlD) || !isfinite(dlH)) return false;
+    if (safeguarded) {
+        float largest = 0.f;
+        for (int c = 0; c < 4; c++) largest = fmaxf(largest, len3((gd[c] * dlD + gh[c] * dlH) * w[c]));
+        float limit = 0.15f / fmaxf(len3(br[0]), fmaxf(len3(br[1]), len3(br[2])));
+        float accepted = largest > limit ? limit / largest : 1.f;
+        dlD *= accepted;
+        dlH *= accepted;
+    }
+    lamD = lamD + dlD;
+    lamH = lamH + dlH;
+    for (int c = 0; c < 4; c++) {
+        f3 x = p[c] + (gd[c] * dlD + gh[c] * dlH) * w[c];
+        S.st(id[c], make_float4(x.x, x.y, x.z, w[c]));
+    }
+    return true;
+}

@@ -76,6 +76,8 @@ extern "C" int gts_struct_sizes(int* out) {
     out[3] = (int)sizeof(GtsStepParams);
     out[4] = (int)sizeof(GtsToolState);
     out[5] = (int)sizeof(GtsCutTool);
+    out[6] = (int)sizeof(GtsBatchInstance);
+    out[7] = (int)sizeof(GtsBatchView);
     return GTS_OK;
 }
 

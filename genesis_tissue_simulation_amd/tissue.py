@@ -103,7 +103,9 @@ class Tissue:
     """Device state of one tissue and the HIP entry points that advance and cut it."""
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
-                 device="cuda:0", topology=None, use_graph=True, max_substeps=64):
+                 device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None):
+        """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, edge_desc, edge_lambda,
+        tet_desc, tet_k_dev, tet_k_hyd, tet_lambda) -- views into the pooled buffers of a batch."""
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -129,27 +131,36 @@ class Tissue:
         init = grid.positions if initial_positions is None else initial_positions
         pos = np.zeros((N, 4), dtype=np.float32)
         pos[:, :3] = init
-        self.pos = f32(pos)
-        self.prev = self.pos.clone()
-        self.vel = torch.zeros((N, 4), dtype=torch.float32, device=dev)
+        st = storage or {}
+        gen = topo["tet_gen"]
+        k_dev = np.ones(T, dtype=np.float32) if k_dev is None else np.asarray(k_dev, dtype=np.float32)
+        k_hyd = np.ones(T, dtype=np.float32) if k_hyd is None else np.asarray(k_hyd, dtype=np.float32)
+
+        def take(name, make):
+            t = st.get(name)
+            if t is None:
+                return make()
+            t.copy_(make())
+            return t
+
+        self.pos = take("pos", lambda: f32(pos))
+        self.prev = take("prev", lambda: f32(pos))
+        self.vel = take("vel", lambda: torch.zeros((N, 4), dtype=torch.float32, device=dev))
         self.force = torch.zeros((N, 4), dtype=torch.float32, device=dev)
-        self.mass = f32(grid.mass)
+        self.mass = take("mass", lambda: f32(grid.mass))
         self.active = i32(grid.active)
         self.pinned = i32(grid.pinned)
         self.inside = i32(grid.inside)
         self.nbr = i32(grid.neighbors.reshape(-1))
         self.bend_alive = torch.zeros(N, dtype=torch.int32, device=dev)
-        self.edge_desc = i32(topo["edge_desc"])
-        self.edge_lambda = torch.zeros(max(E, 1), dtype=torch.float32, device=dev)
+        self.edge_desc = take("edge_desc", lambda: i32(topo["edge_desc"]))
+        self.edge_lambda = take("edge_lambda", lambda: torch.zeros(max(E, 1), dtype=torch.float32, device=dev))
         self.slot_support = torch.zeros(6 * N, dtype=torch.int32, device=dev)
-        self.tet_desc = i32(topo["tet_desc"])
-        gen = topo["tet_gen"]
-        k_dev = np.ones(T, dtype=np.float32) if k_dev is None else np.asarray(k_dev, dtype=np.float32)
-        k_hyd = np.ones(T, dtype=np.float32) if k_hyd is None else np.asarray(k_hyd, dtype=np.float32)
-        self.tet_k_dev = f32(k_dev[gen])
-        self.tet_k_hyd = f32(k_hyd[gen])
+        self.tet_desc = take("tet_desc", lambda: i32(topo["tet_desc"]))
+        self.tet_k_dev = take("tet_k_dev", lambda: f32(k_dev[gen]))
+        self.tet_k_hyd = take("tet_k_hyd", lambda: f32(k_hyd[gen]))
         self.tet_alpha = torch.zeros(2 * max(T, 1), dtype=torch.float32, device=dev)
-        self.tet_lambda = torch.zeros(2 * max(T, 1), dtype=torch.float32, device=dev)
+        self.tet_lambda = take("tet_lambda", lambda: torch.zeros(2 * max(T, 1), dtype=torch.float32, device=dev))
         self.grid_edges = i32(grid.edges.reshape(-1) if Eg else np.zeros(4, dtype=np.int32))
         self.occupied = i32(grid.occupied)
         self.cut_mask = torch.zeros(V, dtype=torch.int32, device=dev)

@@ -131,16 +131,47 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
                     info=dict(spacing=h, half_extents=(6.0, 3.0, 2.5)))
 
 
-def batched_blocks(count=1024, seed=0, tmin=2000, tmax=100_000, spacing=0.25):
+@dataclass
+class BatchWorkload:
+    name: str
+    instances: list        # (grid, params, k_dev, k_hyd, initial_positions) per instance
+    cuts: list             # one swept cut per instance (cut descriptor dict)
+    params: SolverParams
+    dt: float
+    gravity: tuple = GRAVITY
+    info: dict = field(default_factory=dict)
+
+
+def straight_cut(grid, rng, depth_fraction=0.4, thickness=0.02):
+    """A seeded straight cut over the top face of a block: start point and direction uniform over the face, a
+    vertical blade reaching `depth_fraction` of the block height, swept across the face in one stroke."""
+    from .tissue import swept_cut_tool
+    lo = grid.positions.min(axis=0).astype(np.float64)
+    hi = grid.positions.max(axis=0).astype(np.float64)
+    start = lo[[0, 2]] + rng.uniform(0, 1, 2) * (hi - lo)[[0, 2]]
+    angle = rng.uniform(0, 2 * np.pi)
+    length = float(np.linalg.norm((hi - lo)[[0, 2]]))
+    end = start + length * np.array((np.cos(angle), np.sin(angle)))
+    top, depth = hi[1] + 0.5 * grid.cell_size[1], (hi[1] - lo[1]) * depth_fraction
+    seg = lambda p: ((p[0], top, p[1]), (p[0], hi[1] - depth, p[1]))
+    return swept_cut_tool(seg(start), seg(end), thickness)
+
+
+def batched_blocks(count=1024, seed=0, tmin=2000, tmax=100_000, spacing=0.25, substeps=10, iterations=1):
     """Many independent blocks: tet counts log-uniform in [tmin, tmax], 2:1:1 cells, per-instance compliance
-    log-uniform in [1e-8, 1e-6], one seeded straight cut per instance across the top face."""
+    log-uniform in [1e-8, 1e-6] (edges and tets), one seeded straight cut per instance across the top face; a
+    10-substep step of one iteration at dt = 1/60."""
     rng = np.random.default_rng(seed)
-    out = []
-    for k in range(count):
+    base = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7)
+    instances, cuts = [], []
+    for _ in range(count):
         t = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))
-        nodes = block_nodes_for_tets(t)
+        grid = block_grid(block_nodes_for_tets(t), spacing)
         c = float(np.exp(rng.uniform(np.log(1e-8), np.log(1e-6))))
-        start = rng.uniform(0, 1, 2)
-        angle = rng.uniform(0, 2 * np.pi)
-        out.append(dict(nodes=nodes, compliance=c, cut_start=start, cut_angle=angle))
-    return out
+        T = tet_count(grid)
+        p = base.with_(edge_ks=float(np.float32(1.0 / c)))
+        instances.append((grid, p, volume_stiffness(np.full(T, c)), volume_stiffness(np.full(T, c / VOLUME_RATIO)),
+                          jittered(grid, rng, 1e-4 * spacing)))
+        cuts.append(straight_cut(grid, rng))
+    return BatchWorkload("batched-1024", instances, cuts, base, 1 / 60,
+                         info=dict(count=count, tmin=tmin, tmax=tmax, spacing=spacing))

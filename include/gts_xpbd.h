@@ -17,7 +17,7 @@
  *   gts_refresh_liveness     k_TopologyStale + k_RefreshLiveness (solver/material/unified_vertex.cuh:859-861,
  *                            xpbd_solver.cu:429-479)
  *   gts_compact_topology     no reference counterpart: stable remap of the live tets / edges / nodes
- *   gts_batched_step         no reference counterpart (the reference is single-session): many instances, one launch
+ *   gts_batched_*            no reference counterpart (the reference is single-session): many instances, one launch
  *   gts_adjoint_*            no reference counterpart: reverse of the substep for compliance gradients
  * Readbacks (LCS_GetCornerPos, LCS_GetNbrIdx, LCS_GetTopologyRevision, LCS_GetCutEdgeFractions) are plain copies of
  * the caller's own buffers and need no entry point.
@@ -150,7 +150,8 @@ typedef struct GtsCutTool {
 
 int gts_abi_version(void);
 /* Sizes of the structs above as this library sees them (ABI guard like LCS_PhysicsAbiVersion,
- * plugin_api.cpp:102). out[0..5] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool. */
+ * plugin_api.cpp:102). out[0..7] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
+ * GtsBatchInstance, GtsBatchView. */
 int gts_struct_sizes(int* out);
 
 /* Host: derive edges, Kuhn tets and both greedy colourings from the lattice. Needs no GPU. */
@@ -180,6 +181,46 @@ int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, c
  * counts [3] (live tets, edges, nodes); all device. scratch as in the view. */
 int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap, int32_t* node_remap,
                          int32_t* counts, int32_t* node_used, void* stream);
+
+/* ---- batched stepping: many independent lattice instances in pooled buffers, one persistent launch per step ---- */
+
+/* Per-instance record (device array). Offsets index the pooled arrays; descriptors, stiffness and lambdas of an
+ * instance are colour-major and instance-local (node ids relative to node_off). */
+typedef struct GtsBatchInstance {
+    int node_off, n_nodes, edge_off, n_edges, tet_off, n_tets;
+    int nx, nxy, n_edge_colours, n_tet_colours, colour_off, reserved;
+    float edge_ks;               /* edge stiffness 1/compliance of this instance */
+    float rest_len[3];
+    float inv_rest[6][9];
+} GtsBatchInstance;
+
+typedef struct GtsBatchView {
+    int n_instances;
+    const void* instances;        /* device GtsBatchInstance[n_instances] */
+    const int32_t* colour_table;  /* device: per instance, n_edge_colours+1 edge then n_tet_colours+1 tet offsets */
+    const int32_t* work_offsets;  /* device [n_workgroups + 1]: instances of each workgroup (gts_batched_plan) */
+    const int32_t* work_items;    /* device [n_instances] */
+    int n_workgroups, block_size, lds_nodes; /* launch shape; instances with <= lds_nodes nodes are staged in LDS */
+    int use_graph;
+    float gravity[3];
+    float* pos;                   /* [sum N] float4 rows as in GtsTissueView */
+    float* prev;
+    float* vel;
+    const float* mass;
+    const uint32_t* edge_desc;
+    const uint32_t* tet_desc;
+    const float* k_dev;
+    const float* k_hyd;
+    float* lambda_edge;           /* [sum E] */
+    float* lambda_tet;            /* [2 sum T], dev then hyd per instance slice */
+} GtsBatchView;
+
+/* Host: LPT packing of instances (by cost) onto workgroups. out: offsets[n_workgroups+1] then items[n]. */
+int gts_batched_plan(const double* cost, int n, int n_workgroups, int32_t* out);
+/* Host: largest dynamic LDS a workgroup may allocate on the current device. */
+int gts_batched_lds_bytes(void);
+/* Device: one outer step of every instance (p->substeps substeps each) in one persistent launch. */
+int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream);
 
 /* Timing of the dominant kernel (tet colour pass) with HIP events on its launch stream, per launch
  * (in the spirit of LCS_ProfileEnable/LCS_ProfileRead, plugin_api.cpp:21-26). */

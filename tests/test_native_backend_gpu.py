@@ -44,7 +44,7 @@ def test_contract_and_cut():
         other = grid.neighbors[nodes, slots]
         assert np.all(obs.neighbors[other, slots ^ 1] < 0)
         q = tissue.material_quality()
-        assert np.isfinite(q).all() and q[1] > 0
+        assert np.isfinite(q).all()  # no minimum-J safeguard on this path: J may go negative under the knife
     with pytest.raises(RuntimeError):
         tissue.step(Command())
     NativeTissue(grid).close()  # the session is free again

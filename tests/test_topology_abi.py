@@ -34,7 +34,7 @@ def test_library_exports_every_declared_function():
 def test_struct_sizes_and_abi():
     lib = _lib.load()
     assert lib.gts_abi_version() == _lib.GTS_ABI_VERSION
-    sizes = np.zeros(6, dtype=np.int32)
+    sizes = np.zeros(8, dtype=np.int32)
     assert lib.gts_struct_sizes(sizes.ctypes.data_as(ct.c_void_p)) == 0
     assert list(sizes) == [ct.sizeof(s) for s in _lib.STRUCTS]
 
@@ -142,3 +142,18 @@ def test_own_math_matches_oracle_bitwise(oracle_lib):
         a = np.float32(lib.gts_log(float(x)))
         b = np.float32(O.oracle_log_f32(float(x)))
         assert a.view(np.uint32) == b.view(np.uint32)
+
+
+def test_plan_packs_every_instance_once():
+    lib = _lib.load()
+    rng = np.random.default_rng(2)
+    cost = rng.uniform(1, 100, 300)
+    W = 17
+    out = np.zeros(W + 1 + len(cost), dtype=np.int32)
+    assert lib.gts_batched_plan(cost.ctypes.data_as(ct.c_void_p), len(cost), W, out.ctypes.data_as(ct.c_void_p)) == 0
+    offsets, items = out[:W + 1], out[W + 1:]
+    assert offsets[0] == 0 and offsets[-1] == len(cost)
+    assert np.array_equal(np.sort(items), np.arange(len(cost)))
+    loads = [cost[items[offsets[w]:offsets[w + 1]]].sum() for w in range(W)]
+    # LPT is within 4/3 of the optimum; the optimum is at least the mean load and the largest item
+    assert max(loads) <= 4 / 3 * max(cost.sum() / W, cost.max()) + 1e-9
