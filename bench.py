@@ -124,6 +124,80 @@ def run_batched(steps, warmup, count=1024):
                 workload=w)
 
 
+def algorithmic_bytes_per_adjoint_tet_substep(nodes, tets, edges):
+    """Compulsory HBM traffic of one substep of the rollout and its reverse per tet: the forward's 52 B/node,
+    8 B/tet, 4 B/edge, plus the reverse's position and velocity adjoints read and written (48 B/node), per-tet
+    compliance and the two alpha adjoints read and written (20 B/tet) and edge liveness (4 B/edge)."""
+    return (100.0 * nodes + 28.0 * tets + 8.0 * edges) / tets
+
+
+def run_adjoint(iters, warmup, count=64, tets=20_000, nsub=50, ckpt=10):
+    """The compliance fit: `count` blocks of about `tets` tets, a rollout of `nsub` substeps under the seeded point
+    loads, gradient of the surface-node L2 loss by the adjoint kernels with checkpoints every `ckpt` substeps.
+    One iteration is a forward rollout plus its backward through torch.autograd; the adjoint kernels are timed with
+    events on their stream."""
+    import torch
+
+    from genesis_tissue_simulation_amd.adjoint import ComplianceFit
+    from genesis_tissue_simulation_amd.workloads import adjoint_instance
+
+    t0 = time.perf_counter()
+    insts = [adjoint_instance(tets=tets, seed=k) for k in range(count)]
+    fit = ComplianceFit(insts, nsub=nsub, ckpt=ckpt)
+    hidden = torch.as_tensor(np.concatenate([i.compliance_hidden for i in insts]).astype(np.float32))
+    c0 = np.concatenate([i.compliance_init for i in insts]).astype(np.float32)
+    target = fit.forward_raw(hidden).clone()
+    c = torch.tensor(c0, device="cuda", requires_grad=True)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+
+    def one():
+        c.grad = None
+        loss = fit.loss(fit.rollout(c), target)
+        loss.backward()
+        return loss
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(iters):
+        loss = one()
+    stop.record()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    if not (bool(torch.isfinite(c.grad).all()) and np.isfinite(float(loss))):
+        raise RuntimeError("non-finite adjoint gradient")
+    # end positions of block 0 at the hidden compliance against the FP64 oracle
+    x = fit.forward_raw(hidden)[:len(insts[0].grid.positions)].cpu().numpy()
+    return dict(elapsed=elapsed, kernel_ms=start.elapsed_time(stop), T=fit.sT, N=fit.sN, E=fit.sE, count=count,
+                nsub=nsub, ckpt=ckpt, setup_s=setup, loss=float(loss), inst0=insts[0],
+                hidden0=hidden[:int(fit.batch.tet_off[1])].numpy(), x0=x,
+                launch=dict(block_size=fit.batch.block_size, n_workgroups=fit.batch.n_workgroups))
+
+
+def adjoint_fp64_error(a):
+    """Relative end-position error of block 0's FP32 rollout against the FP64 oracle of the same rollout."""
+    import ctypes as ct
+
+    from oracle import oracle as orc
+
+    inst = a["inst0"]
+    L = orc.lib()
+    f = L.oracle_adjoint_f64
+    f.argtypes = [ct.c_void_p, ct.c_float, ct.c_int] + [ct.c_void_p] * 5
+    f.restype = ct.c_double
+    o = orc.Oracle(inst.grid, inst.params, inst.initial_positions, double=True)
+    ptr = lambda arr: np.ascontiguousarray(arr, dtype=np.float32).ctypes.data_as(ct.c_void_p)
+    N = len(inst.grid.positions)
+    f(o.h, np.float32(inst.dt), a["nsub"], ptr(inst.force), ptr(a["hidden0"]), ptr(inst.mask),
+      ptr(np.zeros((N, 3))), None)
+    x64 = o.state()[0]
+    return float(np.abs(a["x0"] - x64).max() / np.abs(x64).max())
+
+
 def run_cpu_batched(w, threads, sample=8):
     from oracle import oracle as orc
 
@@ -175,6 +249,8 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-count", type=int, default=1024)
+    ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--adjoint-iters", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the tissue path runs on a single MI355X and has no multi-GPU mode "
@@ -237,6 +313,24 @@ def main():
             line["batched"]["cpu_baseline"] = {"value": cb["value"], "unit": "tet-substeps/s", "cores": cb["threads"],
                                                "kind": "port", "sample": f"1 step of the first 8 instances "
                                                                          f"({cb['tets']} tets)"}
+    if not args.skip_adjoint:
+        a = run_adjoint(args.adjoint_iters, 1)
+        av = a["T"] * a["nsub"] * args.adjoint_iters / a["elapsed"]
+        ab = algorithmic_bytes_per_adjoint_tet_substep(a["N"], a["T"], a["E"])
+        aach = ab * a["T"] * a["nsub"] * args.adjoint_iters / (a["kernel_ms"] * 1e-3) / 1e9
+        line["adjoint"] = {
+            "value": av, "unit": "tet-substeps/s (rollout + gradient)", "vs_plain": av / value,
+            "ms_per_iteration": 1e3 * a["elapsed"] / args.adjoint_iters,
+            "config": {"workload": f"{a['count']} blocks of ~20k tets, compliance fit, {a['nsub']} substeps, "
+                                   f"checkpoint every {a['ckpt']}", "tets": a["T"], "substeps": a["nsub"],
+                       "checkpoint": a["ckpt"], "launch": a["launch"]},
+            "rel_err_vs_fp64": adjoint_fp64_error(a),
+            "roofline": {"bound": "hbm", "achieved": aach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": aach / HBM_PEAK_GBS,
+                         "kernel": "k_adjoint_forward + k_adjoint_backward (one launch each per iteration)",
+                         "avg_iteration_us": 1e3 * a["kernel_ms"] / args.adjoint_iters,
+                         "bytes_per_tet_substep": ab, "traffic": None},
+        }
     if not args.no_cpu:
         c = run_cpu(w, args.cpu_steps, threads)
         line["cpu_baseline"] = {

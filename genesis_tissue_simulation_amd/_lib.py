@@ -78,8 +78,14 @@ class GtsBatchView(ct.Structure):
                 ("k_hyd", P), ("lambda_edge", P), ("lambda_tet", P)]
 
 
+class GtsAdjointView(ct.Structure):
+    _fields_ = [("batch", GtsBatchView), ("total_nodes", ct.c_longlong), ("force", P), ("compliance", P),
+                ("checkpoints", P), ("segment", P), ("tet_tape", P), ("edge_tape", P), ("xbar", P), ("vbar", P),
+                ("prevbar", P), ("abar", P), ("grad", P)]
+
+
 STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool, GtsBatchInstance,
-           GtsBatchView)
+           GtsBatchView, GtsAdjointView)
 
 # name -> (argtypes, restype); the exported C ABI
 SIGNATURES = {
@@ -101,8 +107,8 @@ SIGNATURES = {
     "gts_batched_plan": ((P, I, I, P), I),
     "gts_batched_step": ((P, P, P), I),
     "gts_batched_lds_bytes": ((), I),
-    "gts_adjoint_forward": ((P, P, P, I, I, P, P), I),
-    "gts_adjoint_backward": ((P, P, P, I, I, P, P, P, P), I),
+    "gts_adjoint_forward": ((P, P, I, I, P), I),
+    "gts_adjoint_backward": ((P, P, I, I, P), I),
 }
 
 _lib = None
@@ -128,7 +134,7 @@ def load():
         fn.argtypes, fn.restype = list(args), res
     if lib.gts_abi_version() != GTS_ABI_VERSION:
         raise NativeError("libgts_xpbd.so ABI version mismatch")
-    sizes = np.zeros(8, dtype=np.int32)
+    sizes = np.zeros(len(STRUCTS), dtype=np.int32)
     lib.gts_struct_sizes(sizes.ctypes.data_as(P))
     for size, st in zip(sizes, STRUCTS):
         if int(size) != ct.sizeof(st):

@@ -1,0 +1,484 @@
+// Adjoint of the XPBD substep for per-tet compliance gradients, batched over instances (one workgroup each).
+//
+// No reference counterpart: the reference's solver has no gradient. The forward is the legacy substep with one
+// iteration, no damping and no contact (coupled Neo-Hookean tets, edges with the instance's compliance), driven by a
+// per-node force field. Checkpoints of (x, v) are kept every `ckpt` substeps; the backward pass walks the segments
+// last to first: it recomputes a segment's substep start states from its checkpoint, then for each substep in
+// reverse re-runs it forward while taping the inputs of every edge and tet projection, and reverses it colour by
+// colour. Within a colour no two projections share a node, so the adjoint scatter needs no atomics, exactly as the
+// forward sweep. The operation sequence mirrors the CPU checker's restatement, so the two agree bit for bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "../../include/gts_xpbd.h"
+#include "gts_common.h"
+#include "gts_solver.h"
+
+namespace {
+
+// The scalar fields of a GtsBatchInstance record.
+struct GtsBatchInstanceHead {
+    int node_off, n_nodes, edge_off, tet_off, n_tets, nx, nxy, n_edge_colours, n_tet_colours, colour_off;
+    float edge_ks;
+};
+
+struct AdjStep {
+    int nsub, ckpt, nckpt;
+    float h, invH, invH2, velScale;
+};
+
+// Per-instance pointers into the pooled buffers.
+struct Inst {
+    const GtsBatchInstance* rec;  // read in place: a local copy of the Dm^-1 table would be indexed from scratch
+    GtsBatchInstanceHead I;
+    float4 *pos, *prev, *vel, *xb, *vb, *pb, *ckx, *ckv, *segx, *segv, *ttape, *etape;
+    const float4* force;
+    const uint32_t *ed, *td;
+    const float* comp;
+    float* abar;
+    float* grad;
+    const int *ecol, *tcol;
+};
+
+__device__ __forceinline__ Inst bind(const GtsAdjointView& A, const AdjStep& P, int item) {
+    Inst s;
+    const GtsBatchView& B = A.batch;
+    s.rec = reinterpret_cast<const GtsBatchInstance*>(B.instances) + item;
+    s.I.node_off = s.rec->node_off;
+    s.I.n_nodes = s.rec->n_nodes;
+    s.I.edge_off = s.rec->edge_off;
+    s.I.tet_off = s.rec->tet_off;
+    s.I.n_tets = s.rec->n_tets;
+    s.I.nx = s.rec->nx;
+    s.I.nxy = s.rec->nxy;
+    s.I.n_edge_colours = s.rec->n_edge_colours;
+    s.I.n_tet_colours = s.rec->n_tet_colours;
+    s.I.colour_off = s.rec->colour_off;
+    s.I.edge_ks = s.rec->edge_ks;
+    const size_t n0 = s.I.node_off, N = s.I.n_nodes, sN = A.total_nodes;
+    s.pos = (float4*)B.pos + n0;
+    s.prev = (float4*)B.prev + n0;
+    s.vel = (float4*)B.vel + n0;
+    s.xb = (float4*)A.xbar + n0;
+    s.vb = (float4*)A.vbar + n0;
+    s.pb = (float4*)A.prevbar + n0;
+    s.force = (const float4*)A.force + n0;
+    // checkpoints: [nckpt][sum N] rows of x then v; segment states: [ckpt][sum N]
+    s.ckx = (float4*)A.checkpoints + n0;
+    s.ckv = (float4*)A.checkpoints + (size_t)P.nckpt * sN + n0;
+    s.segx = (float4*)A.segment + n0;
+    s.segv = (float4*)A.segment + (size_t)P.ckpt * sN + n0;
+    s.ttape = (float4*)A.tet_tape + 4 * (size_t)s.I.tet_off;
+    s.etape = (float4*)A.edge_tape + 2 * (size_t)s.I.edge_off;
+    s.ed = B.edge_desc + s.I.edge_off;
+    s.td = B.tet_desc + s.I.tet_off;
+    s.comp = A.compliance + s.I.tet_off;
+    s.abar = A.abar + 2 * (size_t)s.I.tet_off;
+    s.grad = A.grad + s.I.tet_off;
+    s.ecol = B.colour_table + s.I.colour_off;
+    s.tcol = s.ecol + s.I.n_edge_colours + 1;
+    (void)N;
+    return s;
+}
+
+__device__ __forceinline__ void tet_alphas(float c, float invH2, float& ad, float& ah) {
+    float kd = (float)(1.0 / (double)c), kh = (float)(1.0 / ((double)c / 10.0));
+    ad = invH2 / kd;
+    ah = invH2 / kh;
+}
+
+__device__ __forceinline__ void tet_ids(const Inst& s, uint32_t d, int id[4], int& type) {
+    int base = (int)(d >> 4);
+    type = (int)((d >> 1) & 7u);
+    for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), s.I.nx, s.I.nxy);
+}
+
+// One forward substep of the fit model; with TAPE, records the input positions of every projection.
+template <bool TAPE>
+__device__ void substep_forward(const Inst& s, const AdjStep& P) {
+    const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes;
+    GlobalNodes S{s.pos};
+    for (int i = tid; i < N; i += bs) {
+        float4 p = s.pos[i];
+        s.prev[i] = p;
+        if (p.w > 0.f) {
+            float4 F4 = s.force[i];
+            f3 a = mk3(F4.x, F4.y, F4.z) * p.w;
+            float4 v0 = s.vel[i];
+            f3 v = mk3(v0.x, v0.y, v0.z) + a * P.h;
+            s.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+            f3 x = xyz4(p) + v * P.h;
+            s.pos[i] = make_float4(x.x, x.y, x.z, p.w);
+        }
+    }
+    __syncthreads();
+    const float at = s.I.edge_ks > 0.f ? 1.f / (s.I.edge_ks * P.h * P.h) : 0.f;
+    for (int c = 0; c < s.I.n_edge_colours; c++) {
+            for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
+                uint32_t d = s.ed[k];
+                if (!(d & 1u)) continue;
+                int axis = (int)((d >> 1) & 3u);
+                int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
+                if (TAPE) {
+                    s.etape[2 * k] = s.pos[i];
+                    s.etape[2 * k + 1] = s.pos[j];
+                }
+                float out;
+                edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, 0.f, out);
+            }
+            __syncthreads();
+        }
+    for (int c = 0; c < s.I.n_tet_colours; c++) {
+        for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
+            uint32_t d = s.td[k];
+            if (!(d & 1u)) continue;
+            int id[4], type;
+            tet_ids(s, d, id, type);
+            if (TAPE)
+                for (int q = 0; q < 4; q++) s.ttape[4 * k + q] = s.pos[id[q]];
+            float ad, ah;
+            tet_alphas(s.comp[k], P.invH2, ad, ah);
+            float lamD = 0.f, lamH = 0.f;
+            tet_project<1, false>(S, s.prev, id, s.rec->inv_rest[type], ad, ah, lamD, lamH, 1.f, 0.f, 0.f);
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < N; i += bs) {
+        float4 p = s.pos[i];
+        f3 v = p.w > 0.f ? (xyz4(p) - xyz4(s.prev[i])) * P.invH : mk3(0, 0, 0);
+        v = p.w > 0.f ? v * P.velScale : mk3(0, 0, 0);
+        s.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+    __syncthreads();
+}
+
+// Reverse of the tet projection (mode 1, lambda = 0, no damping, omega = 1) at input positions p.
+__device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, float ad, float ah, const f3 ob[4],
+                            f3 pb[4], float& adb, float& ahb) {
+    f3 br[3] = {mk3(Bm[0], Bm[1], Bm[2]), mk3(Bm[3], Bm[4], Bm[5]), mk3(Bm[6], Bm[7], Bm[8])};
+    f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
+    f3 F[3];
+    for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
+    f3 cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
+    float J = dot3(F[0], cof[0]);
+    f3 gh[4], gd[4], Fb[4], G[4];
+    for (int k = 1; k < 4; k++) {
+        f3 b = br[k - 1];
+        gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
+    }
+    float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    float pJ = fmaxf(J, 0.05f);
+    float iso = iso_pow(pJ);
+    float qraw = iso * I1 - 3.f;
+    float q = fmaxf(qraw, 0.f);
+    float CD = sqrtf(fmaxf(q, 1e-12f));
+    float ds = 1.f / CD;
+    bool jlive = J > 0.05f;
+    float vg = jlive ? iso * I1 / (3.f * pJ) : 0.f;
+    float CH = J - 1.f;
+    for (int k = 1; k < 4; k++) {
+        f3 b = br[k - 1];
+        Fb[k] = F[0] * b.x + F[1] * b.y + F[2] * b.z;
+        G[k] = Fb[k] * iso - gh[k] * vg;
+        gd[k] = G[k] * ds;
+    }
+    gd[0] = (gd[1] + gd[2] + gd[3]) * -1.f;
+    gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
+    float dd = 0.f, dh = 0.f;
+    for (int k = 0; k < 4; k++) {
+        dd += w[k] * dot3(gd[k], gd[k]);
+        dh += w[k] * dot3(gd[k], gh[k]);
+    }
+    float A = dd + ad, gamma = dh / A;
+    f3 Rk[4];
+    float schur = ah + ad * gamma * gamma;
+    for (int k = 0; k < 4; k++) {
+        Rk[k] = gh[k] - gd[k] * gamma;
+        schur += w[k] * dot3(Rk[k], Rk[k]);
+    }
+    float rd = -CD, rh = -CH;
+    for (int k = 0; k < 4; k++) pb[k] = ob[k];
+    if (!(A > 0.f) || !(schur > 0.f)) return;
+    float dlH = (rh - gamma * rd) / schur, dlD = (rd - dh * dlH) / A;
+    if (!isfinite(dlD) || !isfinite(dlH)) return;
+    f3 gdb[4], ghb[4];
+    float dlDb = 0.f, dlHb = 0.f;
+    for (int k = 0; k < 4; k++) {
+        f3 u = ob[k] * w[k];
+        gdb[k] = u * dlD;
+        ghb[k] = u * dlH;
+        dlDb += dot3(u, gd[k]);
+        dlHb += dot3(u, gh[k]);
+    }
+    float rdb = dlDb / A, dhb = -dlDb * dlH / A, Ab = -dlDb * dlD / A;
+    dlHb += -dlDb * dh / A;
+    float rhb = dlHb / schur, gammab = -dlHb * rd / schur, schurb = -dlHb * dlH / schur;
+    rdb += -dlHb * gamma / schur;
+    float CDb = -rdb, CHb = -rhb;
+    ahb += schurb;
+    adb += schurb * gamma * gamma;
+    gammab += schurb * 2.f * ad * gamma;
+    for (int k = 0; k < 4; k++) {
+        f3 Rb = Rk[k] * (schurb * 2.f * w[k]);
+        ghb[k] = ghb[k] + Rb;
+        gdb[k] = gdb[k] - Rb * gamma;
+        gammab += -dot3(Rb, gd[k]);
+    }
+    dhb += gammab / A;
+    Ab += -gammab * gamma / A;
+    float ddb = Ab;
+    adb += Ab;
+    for (int k = 0; k < 4; k++) {
+        gdb[k] = gdb[k] + gd[k] * (2.f * w[k] * ddb) + gh[k] * (w[k] * dhb);
+        ghb[k] = ghb[k] + gd[k] * (w[k] * dhb);
+    }
+    for (int k = 1; k < 4; k++) {
+        gdb[k] = gdb[k] - gdb[0];
+        ghb[k] = ghb[k] - ghb[0];
+    }
+    float dsb = 0.f, isob = 0.f, vgb = 0.f;
+    f3 Fbb[4];
+    for (int k = 1; k < 4; k++) {
+        f3 Gb = gdb[k] * ds;
+        dsb += dot3(gdb[k], G[k]);
+        Fbb[k] = Gb * iso;
+        isob += dot3(Gb, Fb[k]);
+        ghb[k] = ghb[k] - Gb * vg;
+        vgb += -dot3(Gb, gh[k]);
+    }
+    f3 Fbar[3] = {mk3(0, 0, 0), mk3(0, 0, 0), mk3(0, 0, 0)};
+    for (int k = 1; k < 4; k++) {
+        f3 b = br[k - 1];
+        Fbar[0] = Fbar[0] + Fbb[k] * b.x;
+        Fbar[1] = Fbar[1] + Fbb[k] * b.y;
+        Fbar[2] = Fbar[2] + Fbb[k] * b.z;
+    }
+    CDb += -dsb / (CD * CD);
+    float qb = q > 1e-12f ? CDb / (2.f * CD) : 0.f;
+    float I1b = 0.f;
+    if (qraw > 0.f) {
+        isob += qb * I1;
+        I1b += qb * iso;
+    }
+    float pJb = 0.f;
+    if (jlive) {
+        isob += vgb * I1 / (3.f * pJ);
+        I1b += vgb * iso / (3.f * pJ);
+        pJb += -vgb * iso * I1 / (3.f * pJ * pJ);
+    }
+    pJb += isob * ((float)(-2.0 / 3.0)) * iso / pJ;
+    float Jb = CHb + (jlive ? pJb : 0.f);
+    for (int c = 0; c < 3; c++) Fbar[c] = Fbar[c] + F[c] * (2.f * I1b);
+    f3 cofb[3] = {mk3(0, 0, 0), mk3(0, 0, 0), mk3(0, 0, 0)};
+    for (int k = 1; k < 4; k++) {
+        f3 b = br[k - 1];
+        cofb[0] = cofb[0] + ghb[k] * b.x;
+        cofb[1] = cofb[1] + ghb[k] * b.y;
+        cofb[2] = cofb[2] + ghb[k] * b.z;
+    }
+    Fbar[0] = Fbar[0] + cof[0] * Jb;
+    cofb[0] = cofb[0] + F[0] * Jb;
+    Fbar[1] = Fbar[1] + cross3(F[2], cofb[0]);
+    Fbar[2] = Fbar[2] + cross3(cofb[0], F[1]);
+    Fbar[2] = Fbar[2] + cross3(F[0], cofb[1]);
+    Fbar[0] = Fbar[0] + cross3(cofb[1], F[2]);
+    Fbar[0] = Fbar[0] + cross3(F[1], cofb[2]);
+    Fbar[1] = Fbar[1] + cross3(cofb[2], F[0]);
+    for (int k = 0; k < 3; k++) {
+        f3 eb = Fbar[0] * comp3(br[k], 0) + Fbar[1] * comp3(br[k], 1) + Fbar[2] * comp3(br[k], 2);
+        pb[k + 1] = pb[k + 1] + eb;
+        pb[0] = pb[0] - eb;
+    }
+}
+
+// Reverse of the edge projection (lambda = 0, no damping, omega = 1) at inputs xi, xj.
+__device__ __forceinline__ void edge_reverse(f3 xi, f3 xj, float wi, float wj, float L0, float at, f3 oib, f3 ojb,
+                                             f3& xib, f3& xjb) {
+    xib = oib;
+    xjb = ojb;
+    float wsum = wi + wj;
+    if (wsum <= 0.f) return;
+    f3 d = xi - xj;
+    float L = len3(d);
+    if (L < 1e-9f) return;
+    f3 n = d * (1.f / L);
+    float C = L - L0;
+    float kap = 1.f / (wsum + at);
+    float dl = -C * kap;
+    f3 m = oib * wi - ojb * wj;
+    f3 nb = m * dl;
+    float dlb = dot3(m, n);
+    float Lb = -dlb * kap;
+    f3 db = (nb - n * dot3(nb, n)) * (1.f / L) + n * Lb;
+    xib = xib + db;
+    xjb = xjb - db;
+}
+
+__device__ void substep_reverse(const Inst& s, const AdjStep& P) {
+    const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes;
+    // velocity: v' = (x_end - prev) / h * velScale for w > 0
+    for (int i = tid; i < N; i += bs) {
+        float w = s.pos[i].w;
+        if (w > 0.f) {
+            f3 g = xyz4(s.vb[i]) * (P.invH * P.velScale);
+            f3 x = xyz4(s.xb[i]) + g;
+            s.xb[i] = make_float4(x.x, x.y, x.z, 0.f);
+            f3 m = g * -1.f;
+            s.pb[i] = make_float4(m.x, m.y, m.z, 0.f);
+        } else {
+            s.pb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    for (int c = s.I.n_tet_colours - 1; c >= 0; c--) {
+        for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
+            uint32_t d = s.td[k];
+            if (!(d & 1u)) continue;
+            int id[4], type;
+            tet_ids(s, d, id, type);
+            f3 p[4], ob[4], pb[4];
+            float w[4];
+            for (int q = 0; q < 4; q++) {
+                float4 t = s.ttape[4 * k + q];
+                p[q] = xyz4(t);
+                w[q] = t.w;
+                ob[q] = xyz4(s.xb[id[q]]);
+            }
+            float ad, ah, adb = s.abar[k], ahb = s.abar[s.I.n_tets + k];
+            tet_alphas(s.comp[k], P.invH2, ad, ah);
+            tet_reverse(p, w, s.rec->inv_rest[type], ad, ah, ob, pb, adb, ahb);
+            for (int q = 0; q < 4; q++) s.xb[id[q]] = make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f);
+            s.abar[k] = adb;
+            s.abar[s.I.n_tets + k] = ahb;
+        }
+        __syncthreads();
+    }
+    const float at = s.I.edge_ks > 0.f ? 1.f / (s.I.edge_ks * P.h * P.h) : 0.f;
+    for (int c = s.I.n_edge_colours - 1; c >= 0; c--) {
+            for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
+                uint32_t d = s.ed[k];
+                if (!(d & 1u)) continue;
+                int axis = (int)((d >> 1) & 3u);
+                int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
+                float4 ti = s.etape[2 * k], tj = s.etape[2 * k + 1];
+                f3 xib, xjb;
+                edge_reverse(xyz4(ti), xyz4(tj), ti.w, tj.w, s.rec->rest_len[axis], at, xyz4(s.xb[i]), xyz4(s.xb[j]),
+                             xib, xjb);
+                s.xb[i] = make_float4(xib.x, xib.y, xib.z, 0.f);
+                s.xb[j] = make_float4(xjb.x, xjb.y, xjb.z, 0.f);
+            }
+            __syncthreads();
+        }
+    // predict: xp = x + h (v + h a) for w > 0, prev = x
+    for (int i = tid; i < N; i += bs) {
+        float w = s.pos[i].w;
+        f3 g = xyz4(s.xb[i]);
+        f3 x = g + xyz4(s.pb[i]);
+        s.xb[i] = make_float4(x.x, x.y, x.z, 0.f);
+        f3 v = w > 0.f ? g * P.h : mk3(0, 0, 0);
+        s.vb[i] = make_float4(v.x, v.y, v.z, 0.f);
+    }
+    __syncthreads();
+}
+
+__device__ void save_state(const Inst& s, float4* xs, float4* vs) {
+    for (int i = threadIdx.x; i < s.I.n_nodes; i += blockDim.x) {
+        xs[i] = s.pos[i];
+        vs[i] = s.vel[i];
+    }
+    __syncthreads();
+}
+__device__ void load_state(const Inst& s, const float4* xs, const float4* vs) {
+    for (int i = threadIdx.x; i < s.I.n_nodes; i += blockDim.x) {
+        s.pos[i] = xs[i];
+        s.vel[i] = vs[i];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(1024) k_adjoint_forward(GtsAdjointView A, AdjStep P) {
+    const GtsBatchView& B = A.batch;
+    for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
+        Inst s = bind(A, P, B.work_items[w]);
+        const size_t sN = A.total_nodes;
+        for (int n = 0; n < P.nsub; n++) {
+            if (n % P.ckpt == 0) save_state(s, s.ckx + (size_t)(n / P.ckpt) * sN, s.ckv + (size_t)(n / P.ckpt) * sN);
+            substep_forward<false>(s, P);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_backward(GtsAdjointView A, AdjStep P) {
+    const GtsBatchView& B = A.batch;
+    for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
+        Inst s = bind(A, P, B.work_items[w]);
+        const size_t sN = A.total_nodes;
+        for (int k = threadIdx.x; k < 2 * s.I.n_tets; k += blockDim.x) s.abar[k] = 0.f;
+        __syncthreads();
+        for (int seg = (P.nsub - 1) / P.ckpt; seg >= 0; seg--) {
+            const int n0 = seg * P.ckpt, n1 = min(P.nsub, n0 + P.ckpt);
+            load_state(s, s.ckx + (size_t)seg * sN, s.ckv + (size_t)seg * sN);
+            for (int n = n0; n < n1; n++) {
+                save_state(s, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
+                if (n + 1 < n1) substep_forward<false>(s, P);
+            }
+            for (int n = n1 - 1; n >= n0; n--) {
+                load_state(s, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
+                substep_forward<true>(s, P);
+                substep_reverse(s, P);
+            }
+        }
+        for (int k = threadIdx.x; k < s.I.n_tets; k += blockDim.x)
+            s.grad[k] = (s.abar[k] + s.abar[s.I.n_tets + k] / 10.f) * P.invH2;
+        __syncthreads();
+    }
+}
+
+int plan_step(const GtsStepParams* p, int nsub, int ckpt, AdjStep& P) {
+    if (!p || nsub <= 0 || ckpt <= 0) return GTS_ERR_BAD_ARG;
+    if (p->tet_mode != GTS_TET_NEOHOOKEAN || p->iterations != 1 || p->strain_damping != 0.f ||
+        p->bulk_damping != 0.f || p->beta_s != 0.f || p->omega != 1.f || p->damping != 0.f || p->rayleigh != 0.f)
+        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped one-iteration Neo-Hookean substep
+    int N = p->substeps > 0 ? p->substeps : 1;
+    P.h = p->dt / (float)N;
+    if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
+    P.invH = 1.f / P.h;
+    P.invH2 = 1.f / (P.h * P.h);
+    P.velScale = 1.f;
+    P.nsub = nsub;
+    P.ckpt = ckpt;
+    P.nckpt = (nsub + ckpt - 1) / ckpt;
+    return GTS_OK;
+}
+
+int check_view(const GtsAdjointView* A) {
+    if (!A || A->batch.n_workgroups <= 0 || A->batch.block_size <= 0 || A->batch.block_size > 1024 ||
+        A->total_nodes <= 0)
+        return GTS_ERR_BAD_ARG;
+    return GTS_OK;
+}
+
+}  // namespace
+
+extern "C" int gts_adjoint_forward(const GtsAdjointView* A, const GtsStepParams* p, int nsub, int ckpt, void* stream) {
+    if (check_view(A)) return GTS_ERR_BAD_ARG;
+    AdjStep P;
+    int rc = plan_step(p, nsub, ckpt, P);
+    if (rc) return rc;
+    k_adjoint_forward<<<A->batch.n_workgroups, A->batch.block_size, 0, reinterpret_cast<hipStream_t>(stream)>>>(*A, P);
+    return -(int)hipGetLastError();
+}
+
+extern "C" int gts_adjoint_backward(const GtsAdjointView* A, const GtsStepParams* p, int nsub, int ckpt, void* stream) {
+    if (check_view(A)) return GTS_ERR_BAD_ARG;
+    AdjStep P;
+    int rc = plan_step(p, nsub, ckpt, P);
+    if (rc) return rc;
+    const int threads = A->batch.block_size < GTS_ADJOINT_BACKWARD_MAX_THREADS ? A->batch.block_size
+                                                                                : GTS_ADJOINT_BACKWARD_MAX_THREADS;
+    k_adjoint_backward<<<A->batch.n_workgroups, threads, 0, reinterpret_cast<hipStream_t>(stream)>>>(*A, P);
+    return -(int)hipGetLastError();
+}

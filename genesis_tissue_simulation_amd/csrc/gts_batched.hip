@@ -159,26 +159,28 @@ std::vector<BatchGraph> g_batch_graphs;
 hipStream_t g_batch_capture = nullptr;
 
 template <int MODE, bool DAMPED>
-void launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
+int launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
     size_t lds = (size_t)v->lds_nodes * sizeof(float4);
     auto kern = k_batched_step<MODE, DAMPED>;
-    hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -(int)e;
     kern<<<v->n_workgroups, v->block_size, lds, s>>>(
         P, reinterpret_cast<const GtsBatchInstance*>(v->instances), v->colour_table, v->work_offsets, v->work_items,
         (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, v->mass, v->edge_desc, v->tet_desc, v->k_dev, v->k_hyd,
         v->lambda_edge, v->lambda_tet);
+    return -(int)hipGetLastError();
 }
 
-void enqueue(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
+int enqueue(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
     if (damped) {
-        if (P.mode == 0) launch<0, true>(v, P, s);
-        else if (P.mode == 1) launch<1, true>(v, P, s);
-        else launch<2, true>(v, P, s);
-    } else {
-        if (P.mode == 0) launch<0, false>(v, P, s);
-        else if (P.mode == 1) launch<1, false>(v, P, s);
-        else launch<2, false>(v, P, s);
+        if (P.mode == 0) return launch<0, true>(v, P, s);
+        if (P.mode == 1) return launch<1, true>(v, P, s);
+        return launch<2, true>(v, P, s);
     }
+    if (P.mode == 0) return launch<0, false>(v, P, s);
+    if (P.mode == 1) return launch<1, false>(v, P, s);
+    return launch<2, false>(v, P, s);
 }
 
 }  // namespace
@@ -242,10 +244,7 @@ extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, v
     P.lds_nodes = v->lds_nodes;
     P.n_items = v->n_instances;
     const bool damped = P.dOverH != 0.f || P.bOverH != 0.f;
-    if (!v->use_graph) {
-        enqueue(v, P, damped, s);
-        return -(int)hipGetLastError();
-    }
+    if (!v->use_graph) return enqueue(v, P, damped, s);
     std::vector<unsigned char> key(sizeof(GtsBatchView) + sizeof(BatchStep));
     std::memcpy(key.data(), v, sizeof(GtsBatchView));
     std::memcpy(key.data() + sizeof(GtsBatchView), &P, sizeof(BatchStep));
@@ -253,9 +252,13 @@ extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, v
         if (g.key == key) return -(int)hipGraphLaunch(g.exec, s);
     if (!g_batch_capture) HIP_TRY(hipStreamCreateWithFlags(&g_batch_capture, hipStreamNonBlocking));
     HIP_TRY(hipStreamBeginCapture(g_batch_capture, hipStreamCaptureModeThreadLocal));
-    enqueue(v, P, damped, g_batch_capture);
+    int rc = enqueue(v, P, damped, g_batch_capture);
     hipGraph_t graph = nullptr;
-    HIP_TRY(hipStreamEndCapture(g_batch_capture, &graph));
+    hipError_t end = hipStreamEndCapture(g_batch_capture, &graph);
+    if (rc || end != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        return rc ? rc : -(int)end;
+    }
     BatchGraph g;
     g.key = key;
     hipError_t err = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);

@@ -78,6 +78,7 @@ extern "C" int gts_struct_sizes(int* out) {
     out[5] = (int)sizeof(GtsCutTool);
     out[6] = (int)sizeof(GtsBatchInstance);
     out[7] = (int)sizeof(GtsBatchView);
+    out[8] = (int)sizeof(GtsAdjointView);
     return GTS_OK;
 }
 

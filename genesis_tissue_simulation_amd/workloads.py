@@ -132,6 +132,38 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
 
 
 @dataclass
+class AdjointInstance:
+    grid: object
+    params: SolverParams
+    dt: float
+    initial_positions: np.ndarray
+    force: np.ndarray            # [N, 3] gravity * mass plus the point loads (native force units)
+    mask: np.ndarray             # [N] 1 on the surface nodes the loss sees
+    compliance_hidden: np.ndarray
+    compliance_init: np.ndarray
+
+
+def adjoint_instance(nodes=None, tets=20_000, seed=0, spacing=1.0, substeps=10, edge_compliance=1e-7):
+    """One block of the compliance fit: hidden per-tet compliance log-normal (median 1e-7, sigma 0.7), point loads
+    N(0, 0.5) N per component on 5% of the top nodes, loss on the free surface nodes, learnable compliance initialised
+    at the median."""
+    rng = np.random.default_rng(seed)
+    grid = block_grid(nodes or block_nodes_for_tets(tets), spacing)
+    T = tet_count(grid)
+    hidden = np.exp(rng.normal(np.log(1e-7), 0.7, T))
+    params = SolverParams(substeps=substeps, iterations=1, tet_mode=TET_NEOHOOKEAN, edge_ks=1.0 / edge_compliance)
+    ijk = grid.positions / grid.cell_size
+    top = np.flatnonzero(np.isclose(ijk[:, 1], ijk[:, 1].max()))
+    loaded = rng.choice(top, max(1, int(round(0.05 * len(top)))), replace=False)
+    force = (np.asarray(GRAVITY, dtype=np.float64)[None, :] * grid.mass[:, None].astype(np.float64))
+    force[loaded] += rng.normal(0.0, 0.5, (len(loaded), 3)) / 0.01
+    lo, hi = ijk.min(axis=0), ijk.max(axis=0)
+    surface = (np.isclose(ijk, lo) | np.isclose(ijk, hi)).any(axis=1) & (grid.pinned == 0)
+    return AdjointInstance(grid, params, 1 / 60, grid.positions.copy(), force.astype(np.float32),
+                           surface.astype(np.float32), hidden, np.full(T, 1e-7))
+
+
+@dataclass
 class BatchWorkload:
     name: str
     instances: list        # (grid, params, k_dev, k_hyd, initial_positions) per instance

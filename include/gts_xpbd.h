@@ -150,8 +150,8 @@ typedef struct GtsCutTool {
 
 int gts_abi_version(void);
 /* Sizes of the structs above as this library sees them (ABI guard like LCS_PhysicsAbiVersion,
- * plugin_api.cpp:102). out[0..7] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
- * GtsBatchInstance, GtsBatchView. */
+ * plugin_api.cpp:102). out[0..8] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
+ * GtsBatchInstance, GtsBatchView, GtsAdjointView. */
 int gts_struct_sizes(int* out);
 
 /* Host: derive edges, Kuhn tets and both greedy colourings from the lattice. Needs no GPU. */
@@ -221,6 +221,36 @@ int gts_batched_plan(const double* cost, int n, int n_workgroups, int32_t* out);
 int gts_batched_lds_bytes(void);
 /* Device: one outer step of every instance (p->substeps substeps each) in one persistent launch. */
 int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream);
+
+/* ---- adjoint of the substep for per-tet compliance gradients ----
+ * No reference counterpart (the reference solver has no gradient; its differentiable PBD path is a stub,
+ * genesis/engine/solvers/pbd_solver.py:784-815). The forward is the undamped one-iteration Neo-Hookean substep with
+ * a per-node force field and no contact or cutting; it runs on the pooled instances of a GtsBatchView (one workgroup
+ * per instance, LPT-packed). */
+typedef struct GtsAdjointView {
+    GtsBatchView batch;           /* pooled instances and launch shape; k_dev, k_hyd and the lambdas are not used */
+    long long total_nodes;        /* sum of n_nodes: row stride of the checkpoint and segment buffers */
+    const float* force;           /* [sum N] float4 rows: node force (x, y, z, unused) */
+    const float* compliance;      /* [sum T] per-tet deviatoric compliance (volume compliance / 10), colour-major */
+    float* checkpoints;           /* [2 * n_checkpoints * sum N] float4: positions, then velocities, every ckpt-th */
+    float* segment;               /* [2 * ckpt * sum N] float4: start states of the segment being reversed */
+    float* tet_tape;              /* [4 * sum T] float4: inputs of each tet projection of the substep being reversed */
+    float* edge_tape;             /* [2 * sum E] float4: inputs of each edge projection */
+    float* xbar;                  /* [sum N] float4: in dL/dx after the last substep, out dL/dx at the start */
+    float* vbar;                  /* [sum N] float4: in dL/dv after the last substep, out dL/dv at the start */
+    float* prevbar;               /* [sum N] float4 work */
+    float* abar;                  /* [2 sum T] work: dL/d alpha~_D then dL/d alpha~_H per instance slice */
+    float* grad;                  /* [sum T] out: dL/d compliance, colour-major */
+} GtsAdjointView;
+
+/* Device: `nsub` substeps of every instance from the state in batch.pos / batch.vel, storing (x, v) before every
+ * ckpt-th substep; the end state is left in batch.pos / batch.vel. */
+int gts_adjoint_forward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
+/* Device: reverse pass over the checkpoints of the last gts_adjoint_forward with the same nsub and ckpt; consumes
+ * xbar / vbar and writes grad. Leaves batch.pos / batch.vel at the state after the first substep. Runs with
+ * min(batch.block_size, GTS_ADJOINT_BACKWARD_MAX_THREADS) threads per workgroup. */
+#define GTS_ADJOINT_BACKWARD_MAX_THREADS 512
+int gts_adjoint_backward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
 
 /* Timing of the dominant kernel (tet colour pass) with HIP events on its launch stream, per launch
  * (in the spirit of LCS_ProfileEnable/LCS_ProfileRead, plugin_api.cpp:21-26). */

@@ -1299,3 +1299,348 @@ extern "C" double oracle_shape_distance_f64(const double* center, const double* 
     normal[0] = nrm.x; normal[1] = nrm.y; normal[2] = nrm.z;
     return d;
 }
+
+// ---------------------------------------------------------------------------------------------------------------
+// Adjoint of the substep for per-tet compliance gradients (no reference counterpart; pinned by finite differences).
+//
+// Forward: `nsub` substeps of the legacy branch with one iteration, no damping, no contact, omega = 1, coupled
+// Neo-Hookean tets (mode 1), node forces `force` (gravity * mass plus point loads) in place of the gravity field.
+// Per-tet compliance c gives alpha~_D = (1/h^2)/(1/c) and alpha~_H = (1/h^2)/(10/c).
+// Loss: sum over masked nodes of |x - target|^2 after the last substep. Backward: every substep in reverse, each
+// re-run forward from its stored start state with a tape of the inputs of every projection, then reversed colour
+// by colour; gradients are not taken through the max(J, .05) and max(q, 1e-12) clamps (zero subgradient).
+// ---------------------------------------------------------------------------------------------------------------
+namespace {
+
+template <class R> struct TetTape { V3<R> p[4]; };
+
+// Reverse of tetProject (mode 1, lambda = 0, no damping, omega = 1) at input positions p: given the adjoint of the
+// outputs (ob), accumulates the adjoint of the inputs (pb) and of alpha~_D / alpha~_H.
+template <class R>
+void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, R ah, const V3<R> ob[4], V3<R> pb[4],
+                       R& adb, R& ahb) {
+    V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
+    V3<R> e[3] = {e1, e2, e3};
+    V3<R> F[3];
+    for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
+    V3<R> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
+    R J = dot3(F[0], cof[0]);
+    V3<R> gh[4], gd[4], Fb[4], G[4];
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
+    }
+    R I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    R pJ = fmax_<R>(J, (R)0.05f);
+    R iso = isoPow(pJ);
+    R qraw = iso * I1 - (R)3;
+    R q = fmax_<R>(qraw, (R)0);
+    R CD = std::sqrt(fmax_<R>(q, (R)1e-12f));
+    R ds = (R)1 / CD;
+    bool jlive = J > (R)0.05f;
+    R vg = jlive ? iso * I1 / ((R)3 * pJ) : (R)0;
+    R CH = J - (R)1;
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        Fb[k] = F[0] * b.x + F[1] * b.y + F[2] * b.z;
+        G[k] = Fb[k] * iso - gh[k] * vg;
+        gd[k] = G[k] * ds;
+    }
+    gd[0] = (gd[1] + gd[2] + gd[3]) * (R)-1;
+    gh[0] = (gh[1] + gh[2] + gh[3]) * (R)-1;
+    R dd = 0, dh = 0;
+    for (int k = 0; k < 4; k++) {
+        dd += w[k] * dot3(gd[k], gd[k]);
+        dh += w[k] * dot3(gd[k], gh[k]);
+    }
+    R A = dd + ad, gamma = dh / A;
+    V3<R> Rk[4];
+    R schur = ah + ad * gamma * gamma;
+    for (int k = 0; k < 4; k++) {
+        Rk[k] = gh[k] - gd[k] * gamma;
+        schur += w[k] * dot3(Rk[k], Rk[k]);
+    }
+    R rd = -CD, rh = -CH;
+    for (int k = 0; k < 4; k++) pb[k] = ob[k];
+    if (!(A > 0) || !(schur > 0)) return;
+    R dlH = (rh - gamma * rd) / schur, dlD = (rd - dh * dlH) / A;
+    if (!std::isfinite(dlD) || !std::isfinite(dlH)) return;
+    // out_k = p_k + (gd_k dlD + gh_k dlH) w_k
+    V3<R> gdb[4], ghb[4];
+    R dlDb = 0, dlHb = 0;
+    for (int k = 0; k < 4; k++) {
+        V3<R> u = ob[k] * w[k];
+        gdb[k] = u * dlD;
+        ghb[k] = u * dlH;
+        dlDb += dot3(u, gd[k]);
+        dlHb += dot3(u, gh[k]);
+    }
+    // dlD = (rd - dh dlH) / A
+    R rdb = dlDb / A, dhb = -dlDb * dlH / A, Ab = -dlDb * dlD / A;
+    dlHb += -dlDb * dh / A;
+    // dlH = (rh - gamma rd) / schur
+    R rhb = dlHb / schur, gammab = -dlHb * rd / schur, schurb = -dlHb * dlH / schur;
+    rdb += -dlHb * gamma / schur;
+    R CDb = -rdb, CHb = -rhb;
+    // schur = ah + ad gamma^2 + sum w |gh - gamma gd|^2
+    ahb += schurb;
+    adb += schurb * gamma * gamma;
+    gammab += schurb * (R)2 * ad * gamma;
+    for (int k = 0; k < 4; k++) {
+        V3<R> Rb = Rk[k] * (schurb * (R)2 * w[k]);
+        ghb[k] = ghb[k] + Rb;
+        gdb[k] = gdb[k] - Rb * gamma;
+        gammab += -dot3(Rb, gd[k]);
+    }
+    // gamma = dh / A
+    dhb += gammab / A;
+    Ab += -gammab * gamma / A;
+    // A = dd + ad
+    R ddb = Ab;
+    adb += Ab;
+    for (int k = 0; k < 4; k++) {
+        gdb[k] = gdb[k] + gd[k] * ((R)2 * w[k] * ddb) + gh[k] * (w[k] * dhb);
+        ghb[k] = ghb[k] + gd[k] * (w[k] * dhb);
+    }
+    for (int k = 1; k < 4; k++) {
+        gdb[k] = gdb[k] - gdb[0];
+        ghb[k] = ghb[k] - ghb[0];
+    }
+    // gd_k = G_k ds, G_k = Fb_k iso - gh_k vg
+    R dsb = 0, isob = 0, vgb = 0;
+    V3<R> Fbb[4];
+    for (int k = 1; k < 4; k++) {
+        V3<R> Gb = gdb[k] * ds;
+        dsb += dot3(gdb[k], G[k]);
+        Fbb[k] = Gb * iso;
+        isob += dot3(Gb, Fb[k]);
+        ghb[k] = ghb[k] - Gb * vg;
+        vgb += -dot3(Gb, gh[k]);
+    }
+    V3<R> Fbar[3] = {mk<R>(0, 0, 0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        Fbar[0] = Fbar[0] + Fbb[k] * b.x;
+        Fbar[1] = Fbar[1] + Fbb[k] * b.y;
+        Fbar[2] = Fbar[2] + Fbb[k] * b.z;
+    }
+    // ds = 1 / CD; CD = sqrt(max(q, 1e-12)); q = max(iso I1 - 3, 0)
+    CDb += -dsb / (CD * CD);
+    R qb = q > (R)1e-12f ? CDb / ((R)2 * CD) : (R)0;
+    R I1b = 0;
+    if (qraw > 0) {
+        isob += qb * I1;
+        I1b += qb * iso;
+    }
+    R pJb = 0;
+    if (jlive) {
+        isob += vgb * I1 / ((R)3 * pJ);
+        I1b += vgb * iso / ((R)3 * pJ);
+        pJb += -vgb * iso * I1 / ((R)3 * pJ * pJ);
+    }
+    pJb += isob * ((R)(-2.0 / 3.0)) * iso / pJ;
+    R Jb = CHb + (jlive ? pJb : (R)0);
+    for (int c = 0; c < 3; c++) Fbar[c] = Fbar[c] + F[c] * ((R)2 * I1b);
+    V3<R> cofb[3] = {mk<R>(0, 0, 0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        cofb[0] = cofb[0] + ghb[k] * b.x;
+        cofb[1] = cofb[1] + ghb[k] * b.y;
+        cofb[2] = cofb[2] + ghb[k] * b.z;
+    }
+    Fbar[0] = Fbar[0] + cof[0] * Jb;
+    cofb[0] = cofb[0] + F[0] * Jb;
+    Fbar[1] = Fbar[1] + cross3(F[2], cofb[0]);
+    Fbar[2] = Fbar[2] + cross3(cofb[0], F[1]);
+    Fbar[2] = Fbar[2] + cross3(F[0], cofb[1]);
+    Fbar[0] = Fbar[0] + cross3(cofb[1], F[2]);
+    Fbar[0] = Fbar[0] + cross3(F[1], cofb[2]);
+    Fbar[1] = Fbar[1] + cross3(cofb[2], F[0]);
+    for (int k = 0; k < 3; k++) {
+        V3<R> eb = Fbar[0] * comp(br[k], 0) + Fbar[1] * comp(br[k], 1) + Fbar[2] * comp(br[k], 2);
+        pb[k + 1] = pb[k + 1] + eb;
+        pb[0] = pb[0] - eb;
+    }
+    (void)e;
+}
+
+// Reverse of solveEdge (lambda = 0, no damping, omega = 1) at inputs xi, xj.
+template <class R>
+void edgeProjectReverse(V3<R> xi, V3<R> xj, R wi, R wj, R L0, R at, V3<R> oib, V3<R> ojb, V3<R>& xib, V3<R>& xjb) {
+    xib = oib;
+    xjb = ojb;
+    R wsum = wi + wj;
+    if (wsum <= 0) return;
+    V3<R> d = xi - xj;
+    R L = len3(d);
+    if (L < (R)1e-9f) return;
+    V3<R> n = d * ((R)1 / L);
+    R C = L - L0;
+    R kap = (R)1 / (wsum + at);
+    R dl = -C * kap;
+    V3<R> m = oib * wi - ojb * wj;
+    V3<R> nb = m * dl;
+    R dlb = dot3(m, n);
+    R Lb = -dlb * kap;
+    V3<R> db = (nb - n * dot3(nb, n)) * ((R)1 / L) + n * Lb;
+    xib = xib + db;
+    xjb = xjb - db;
+}
+
+template <class R> struct AdjointRun {
+    std::vector<V3<R>> force;
+    std::vector<R> c;  // per tet (generation order)
+};
+
+// One forward substep (mode 1, iters 1) with external force field; optionally records the projection tape.
+template <class R>
+void adjointSubstep(Tissue<R>& s, const std::vector<V3<R>>& force, R h, std::vector<TetTape<R>>* tt,
+                    std::vector<V3<R>>* et) {
+    for (int p = 0; p < s.N; p++) {
+        s.prev[p] = s.pos[p];
+        if (s.invMass[p] > 0) {
+            V3<R> a = force[p] * s.invMass[p];
+            V3<R> v = s.vel[p] + a * h;
+            s.vel[p] = v;
+            s.pos[p] = s.pos[p] + v * h;
+        }
+    }
+    for (size_t c = 0; c < s.ecolorCnt.size(); c++)
+        for (int k = 0; k < s.ecolorCnt[c]; k++) {
+            int e = s.ecolorOff[c] + k;
+            if (et) { (*et)[2 * e] = s.pos[s.ei[e]]; (*et)[2 * e + 1] = s.pos[s.ej[e]]; }
+            solveEdge(s, e, s.alphaTilde, false, (R)0, false);
+        }
+    for (size_t c = 0; c < s.tcolorCnt.size(); c++)
+        for (int k = 0; k < s.tcolorCnt[c]; k++) {
+            int t = s.tcolorOrder[s.tcolorOff[c] + k];
+            if (tt)
+                for (int j = 0; j < 4; j++) (*tt)[t].p[j] = s.pos[s.tids[4 * t + j]];
+            solveTet(s, t, (R)0, (R)0, false);
+        }
+    R invH = (R)1 / h;
+    for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? (s.pos[p] - s.prev[p]) * invH : mk<R>(0, 0, 0);
+    for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? s.vel[p] * s.velScale : mk<R>(0, 0, 0);
+}
+
+template <class R>
+double adjointRun(Tissue<R>& s, R dt, int nsub, const float* force, const float* compliance, const float* mask,
+                  const float* target, R* grad) {
+    // frame constants as xpbd_begin_frame with substeps = d.substeps (h = dt / substeps)
+    int Nsub = s.d.substeps > 0 ? s.d.substeps : 1;
+    R h = dt / (R)Nsub;
+    R ks = (R)s.d.edge_ks;
+    s.alphaTilde = ks > 0 ? (R)1 / (ks * h * h) : (R)0;
+    s.velScale = (R)1;
+    s.betaGammaH = (R)0;
+    s.h = h;
+    R invH2 = (R)1 / (h * h);
+    for (int t = 0; t < s.T; t++) {
+        R kd = (R)1 / (R)compliance[t], kh = (R)1 / ((R)compliance[t] / (R)10);
+        if (sizeof(R) == 4) { kd = (R)(float)(1.0 / (double)compliance[t]); kh = (R)(float)(1.0 / ((double)compliance[t] / 10.0)); }
+        s.alphaDev[t] = invH2 / kd;
+        s.alphaHyd[t] = invH2 / kh;
+    }
+    std::vector<V3<R>> F(s.N);
+    for (int p = 0; p < s.N; p++) F[p] = v3<R>(force + 3 * p);
+    refreshLiveness(s);
+    std::vector<V3<R>> xs((size_t)(nsub + 1) * s.N), vs((size_t)(nsub + 1) * s.N);
+    for (int n = 0; n < nsub; n++) {
+        std::copy(s.pos.begin(), s.pos.end(), xs.begin() + (size_t)n * s.N);
+        std::copy(s.vel.begin(), s.vel.end(), vs.begin() + (size_t)n * s.N);
+        adjointSubstep<R>(s, F, h, nullptr, nullptr);
+    }
+    std::copy(s.pos.begin(), s.pos.end(), xs.begin() + (size_t)nsub * s.N);
+    std::copy(s.vel.begin(), s.vel.end(), vs.begin() + (size_t)nsub * s.N);
+    double loss = 0;
+    std::vector<V3<R>> xb(s.N), vb(s.N);
+    for (int p = 0; p < s.N; p++) {
+        V3<R> d = s.pos[p] - v3<R>(target + 3 * p);
+        loss += (double)mask[p] * (double)dot3(d, d);
+        xb[p] = d * ((R)2 * (R)mask[p]);
+        vb[p] = mk<R>(0, 0, 0);
+    }
+    if (!grad) return loss;
+    std::vector<R> adb(s.T, 0), ahb(s.T, 0);
+    std::vector<TetTape<R>> tt(s.T);
+    std::vector<V3<R>> et(2 * (size_t)s.E);
+    std::vector<V3<R>> prevb(s.N);
+    for (int n = nsub - 1; n >= 0; n--) {
+        std::copy(xs.begin() + (size_t)n * s.N, xs.begin() + (size_t)(n + 1) * s.N, s.pos.begin());
+        std::copy(vs.begin() + (size_t)n * s.N, vs.begin() + (size_t)(n + 1) * s.N, s.vel.begin());
+        adjointSubstep<R>(s, F, h, &tt, &et);
+        // velocity: v' = (x_end - prev) / h * velScale for w > 0
+        R invH = (R)1 / h;
+        for (int p = 0; p < s.N; p++) {
+            if (s.invMass[p] > 0) {
+                V3<R> g = vb[p] * (invH * s.velScale);
+                xb[p] = xb[p] + g;
+                prevb[p] = g * (R)-1;
+            } else prevb[p] = mk<R>(0, 0, 0);
+        }
+        // tets, colours in reverse
+        for (int c = (int)s.tcolorCnt.size() - 1; c >= 0; c--)
+            for (int k = 0; k < s.tcolorCnt[c]; k++) {
+                int t = s.tcolorOrder[s.tcolorOff[c] + k];
+                if (!s.tactive[t]) continue;
+                const int* id = &s.tids[4 * t];
+                R w[4];
+                V3<R> ob[4], pb[4];
+                for (int j = 0; j < 4; j++) { w[j] = s.invMass[id[j]]; ob[j] = xb[id[j]]; }
+                const R(*B)[3] = s.typeB[s.ttype[t]];
+                V3<R> br[3] = {mk<R>(B[0][0], B[0][1], B[0][2]), mk<R>(B[1][0], B[1][1], B[1][2]),
+                               mk<R>(B[2][0], B[2][1], B[2][2])};
+                tetProjectReverse<R>(tt[t].p, w, br, s.alphaDev[t], s.alphaHyd[t], ob, pb, adb[t], ahb[t]);
+                for (int j = 0; j < 4; j++) xb[id[j]] = pb[j];
+            }
+        for (int c = (int)s.ecolorCnt.size() - 1; c >= 0; c--)
+            for (int k = 0; k < s.ecolorCnt[c]; k++) {
+                int e = s.ecolorOff[c] + k;
+                if (!s.eactive[e]) continue;
+                int i = s.ei[e], j = s.ej[e];
+                V3<R> xib, xjb;
+                edgeProjectReverse<R>(et[2 * e], et[2 * e + 1], s.invMass[i], s.invMass[j], s.erest[e],
+                                      s.alphaTilde, xb[i], xb[j], xib, xjb);
+                xb[i] = xib;
+                xb[j] = xjb;
+            }
+        // predict: xp = x + h (v + h a) for w > 0, prev = x
+        for (int p = 0; p < s.N; p++) {
+            V3<R> g = xb[p];
+            xb[p] = g + prevb[p];
+            vb[p] = s.invMass[p] > 0 ? g * h : mk<R>(0, 0, 0);
+        }
+    }
+    for (int t = 0; t < s.T; t++) grad[t] = (adb[t] + ahb[t] / (R)10) * invH2;
+    // leave the state at the end of the forward run
+    std::copy(xs.begin() + (size_t)nsub * s.N, xs.end(), s.pos.begin());
+    std::copy(vs.begin() + (size_t)nsub * s.N, vs.end(), s.vel.begin());
+    return loss;
+}
+
+}  // namespace
+
+#define ORACLE_ADJ(R, SUF)                                                                                            \
+    extern "C" double oracle_adjoint_##SUF(void* h, float dt, int nsub, const float* force, const float* compliance,  \
+                                           const float* mask, const float* target, R* grad) {                          \
+        return adjointRun(*(Tissue<R>*)h, (R)dt, nsub, force, compliance, mask, target, grad);                        \
+    }                                                                                                                  \
+    extern "C" void oracle_tet_reverse_##SUF(const R* p12, const R* w4, const R* B9, R ad, R ah, const R* ob12,        \
+                                             R* pb12, R* abar2) {                                                      \
+        V3<R> p[4], ob[4], pb[4], br[3];                                                                               \
+        R w[4];                                                                                                        \
+        for (int k = 0; k < 4; k++) {                                                                                  \
+            p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);                                                     \
+            ob[k] = mk(ob12[3 * k], ob12[3 * k + 1], ob12[3 * k + 2]);                                                 \
+            w[k] = w4[k];                                                                                              \
+        }                                                                                                              \
+        for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);                              \
+        R adb = 0, ahb = 0;                                                                                            \
+        tetProjectReverse<R>(p, w, br, ad, ah, ob, pb, adb, ahb);                                                      \
+        for (int k = 0; k < 4; k++) { pb12[3 * k] = pb[k].x; pb12[3 * k + 1] = pb[k].y; pb12[3 * k + 2] = pb[k].z; }   \
+        abar2[0] = adb;                                                                                                \
+        abar2[1] = ahb;                                                                                                \
+    }
+
+ORACLE_ADJ(float, f32)
+ORACLE_ADJ(double, f64)

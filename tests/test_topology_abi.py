@@ -34,7 +34,7 @@ def test_library_exports_every_declared_function():
 def test_struct_sizes_and_abi():
     lib = _lib.load()
     assert lib.gts_abi_version() == _lib.GTS_ABI_VERSION
-    sizes = np.zeros(8, dtype=np.int32)
+    sizes = np.zeros(len(_lib.STRUCTS), dtype=np.int32)
     assert lib.gts_struct_sizes(sizes.ctypes.data_as(ct.c_void_p)) == 0
     assert list(sizes) == [ct.sizeof(s) for s in _lib.STRUCTS]
 
