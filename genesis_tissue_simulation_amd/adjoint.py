@@ -26,7 +26,8 @@ class ComplianceFit:
     and the rollout length. `rollout(c)` maps per-tet compliance (generation order, blocks back to back) to the node
     positions after `nsub` substeps, differentiably."""
 
-    def __init__(self, instances, nsub=50, ckpt=10, device="cuda:0", block_size=1024, workgroups_per_cu=1):
+    def __init__(self, instances, nsub=50, ckpt=10, device="cuda:0", block_size=1024, workgroups_per_cu=1,
+                 lds_nodes=None):
         if nsub <= 0 or ckpt <= 0:
             raise ValueError("nsub and ckpt must be positive")
         p0 = instances[0].params
@@ -44,7 +45,7 @@ class ComplianceFit:
         self.params = p0
         self.dt = float(instances[0].dt)
         self.batch = Batch([(i.grid, i.params, None, None, i.initial_positions) for i in instances], p0,
-                           gravity=(0.0, 0.0, 0.0), device=device, block_size=block_size,
+                           gravity=(0.0, 0.0, 0.0), device=device, block_size=block_size, lds_nodes=lds_nodes,
                            workgroups_per_cu=workgroups_per_cu, use_graph=False)
         b = self.batch
         dev = self.device
@@ -83,8 +84,11 @@ class ComplianceFit:
             self.edge_tape.data_ptr(), self.xbar.data_ptr(), self.vbar.data_ptr(), self.prevbar.data_ptr(),
             self.abar.data_ptr(), self.grad.data_ptr())
 
-    def configure(self, block_size=1024, workgroups_per_cu=1):
-        self.batch.configure(block_size, None, workgroups_per_cu)
+    def configure(self, block_size=1024, workgroups_per_cu=1, lds_nodes=None):
+        """Launch shape: threads per workgroup (the backward uses at most 512), workgroups per CU and the LDS node
+        capacity: the forward stages a block's positions in LDS when it has at most lds_nodes nodes, the backward its
+        positions and position adjoints when it has at most lds_nodes / 2."""
+        self.batch.configure(block_size, lds_nodes, workgroups_per_cu)
         self._view()
 
     @property

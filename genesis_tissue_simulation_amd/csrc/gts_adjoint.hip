@@ -26,7 +26,7 @@ struct GtsBatchInstanceHead {
 };
 
 struct AdjStep {
-    int nsub, ckpt, nckpt;
+    int nsub, ckpt, nckpt, lds_nodes;
     float h, invH, invH2, velScale;
 };
 
@@ -96,13 +96,13 @@ __device__ __forceinline__ void tet_ids(const Inst& s, uint32_t d, int id[4], in
     for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), s.I.nx, s.I.nxy);
 }
 
-// One forward substep of the fit model; with TAPE, records the input positions of every projection.
-template <bool TAPE>
-__device__ void substep_forward(const Inst& s, const AdjStep& P) {
+// One forward substep of the fit model on node positions S (LDS-staged or the instance's global slice); with TAPE,
+// records the input positions of every projection.
+template <bool TAPE, class Store>
+__device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P) {
     const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes;
-    GlobalNodes S{s.pos};
     for (int i = tid; i < N; i += bs) {
-        float4 p = s.pos[i];
+        float4 p = S.ld(i);
         s.prev[i] = p;
         if (p.w > 0.f) {
             float4 F4 = s.force[i];
@@ -111,26 +111,26 @@ __device__ void substep_forward(const Inst& s, const AdjStep& P) {
             f3 v = mk3(v0.x, v0.y, v0.z) + a * P.h;
             s.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
             f3 x = xyz4(p) + v * P.h;
-            s.pos[i] = make_float4(x.x, x.y, x.z, p.w);
+            S.st(i, make_float4(x.x, x.y, x.z, p.w));
         }
     }
     __syncthreads();
     const float at = s.I.edge_ks > 0.f ? 1.f / (s.I.edge_ks * P.h * P.h) : 0.f;
     for (int c = 0; c < s.I.n_edge_colours; c++) {
-            for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
-                uint32_t d = s.ed[k];
-                if (!(d & 1u)) continue;
-                int axis = (int)((d >> 1) & 3u);
-                int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
-                if (TAPE) {
-                    s.etape[2 * k] = s.pos[i];
-                    s.etape[2 * k + 1] = s.pos[j];
-                }
-                float out;
-                edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, 0.f, out);
+        for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
+            uint32_t d = s.ed[k];
+            if (!(d & 1u)) continue;
+            int axis = (int)((d >> 1) & 3u);
+            int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
+            if (TAPE) {
+                s.etape[2 * k] = S.ld(i);
+                s.etape[2 * k + 1] = S.ld(j);
             }
-            __syncthreads();
+            float out;
+            edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, 0.f, out);
         }
+        __syncthreads();
+    }
     for (int c = 0; c < s.I.n_tet_colours; c++) {
         for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
             uint32_t d = s.td[k];
@@ -138,7 +138,7 @@ __device__ void substep_forward(const Inst& s, const AdjStep& P) {
             int id[4], type;
             tet_ids(s, d, id, type);
             if (TAPE)
-                for (int q = 0; q < 4; q++) s.ttape[4 * k + q] = s.pos[id[q]];
+                for (int q = 0; q < 4; q++) s.ttape[4 * k + q] = S.ld(id[q]);
             float ad, ah;
             tet_alphas(s.comp[k], P.invH2, ad, ah);
             float lamD = 0.f, lamH = 0.f;
@@ -147,7 +147,7 @@ __device__ void substep_forward(const Inst& s, const AdjStep& P) {
         __syncthreads();
     }
     for (int i = tid; i < N; i += bs) {
-        float4 p = s.pos[i];
+        float4 p = S.ld(i);
         f3 v = p.w > 0.f ? (xyz4(p) - xyz4(s.prev[i])) * P.invH : mk3(0, 0, 0);
         v = p.w > 0.f ? v * P.velScale : mk3(0, 0, 0);
         s.vel[i] = make_float4(v.x, v.y, v.z, 0.f);
@@ -317,15 +317,18 @@ __device__ __forceinline__ void edge_reverse(f3 xi, f3 xj, float wi, float wj, f
     xjb = xjb - db;
 }
 
-__device__ void substep_reverse(const Inst& s, const AdjStep& P) {
+// Reverse of one substep. S holds the positions at the end of the substep (for the inverse masses), X the position
+// adjoints; the velocity adjoints and the adjoint of prev are in the instance's global slices.
+template <class Store, class XStore>
+__device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, const AdjStep& P) {
     const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes;
     // velocity: v' = (x_end - prev) / h * velScale for w > 0
     for (int i = tid; i < N; i += bs) {
-        float w = s.pos[i].w;
+        float w = S.ld(i).w;
         if (w > 0.f) {
             f3 g = xyz4(s.vb[i]) * (P.invH * P.velScale);
-            f3 x = xyz4(s.xb[i]) + g;
-            s.xb[i] = make_float4(x.x, x.y, x.z, 0.f);
+            f3 x = xyz4(X.ld(i)) + g;
+            X.st(i, make_float4(x.x, x.y, x.z, 0.f));
             f3 m = g * -1.f;
             s.pb[i] = make_float4(m.x, m.y, m.z, 0.f);
         } else {
@@ -345,12 +348,12 @@ __device__ void substep_reverse(const Inst& s, const AdjStep& P) {
                 float4 t = s.ttape[4 * k + q];
                 p[q] = xyz4(t);
                 w[q] = t.w;
-                ob[q] = xyz4(s.xb[id[q]]);
+                ob[q] = xyz4(X.ld(id[q]));
             }
             float ad, ah, adb = s.abar[k], ahb = s.abar[s.I.n_tets + k];
             tet_alphas(s.comp[k], P.invH2, ad, ah);
             tet_reverse(p, w, s.rec->inv_rest[type], ad, ah, ob, pb, adb, ahb);
-            for (int q = 0; q < 4; q++) s.xb[id[q]] = make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f);
+            for (int q = 0; q < 4; q++) X.st(id[q], make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f));
             s.abar[k] = adb;
             s.abar[s.I.n_tets + k] = ahb;
         }
@@ -358,78 +361,113 @@ __device__ void substep_reverse(const Inst& s, const AdjStep& P) {
     }
     const float at = s.I.edge_ks > 0.f ? 1.f / (s.I.edge_ks * P.h * P.h) : 0.f;
     for (int c = s.I.n_edge_colours - 1; c >= 0; c--) {
-            for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
-                uint32_t d = s.ed[k];
-                if (!(d & 1u)) continue;
-                int axis = (int)((d >> 1) & 3u);
-                int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
-                float4 ti = s.etape[2 * k], tj = s.etape[2 * k + 1];
-                f3 xib, xjb;
-                edge_reverse(xyz4(ti), xyz4(tj), ti.w, tj.w, s.rec->rest_len[axis], at, xyz4(s.xb[i]), xyz4(s.xb[j]),
-                             xib, xjb);
-                s.xb[i] = make_float4(xib.x, xib.y, xib.z, 0.f);
-                s.xb[j] = make_float4(xjb.x, xjb.y, xjb.z, 0.f);
-            }
-            __syncthreads();
+        for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
+            uint32_t d = s.ed[k];
+            if (!(d & 1u)) continue;
+            int axis = (int)((d >> 1) & 3u);
+            int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
+            float4 ti = s.etape[2 * k], tj = s.etape[2 * k + 1];
+            f3 xib, xjb;
+            edge_reverse(xyz4(ti), xyz4(tj), ti.w, tj.w, s.rec->rest_len[axis], at, xyz4(X.ld(i)), xyz4(X.ld(j)),
+                         xib, xjb);
+            X.st(i, make_float4(xib.x, xib.y, xib.z, 0.f));
+            X.st(j, make_float4(xjb.x, xjb.y, xjb.z, 0.f));
         }
+        __syncthreads();
+    }
     // predict: xp = x + h (v + h a) for w > 0, prev = x
     for (int i = tid; i < N; i += bs) {
-        float w = s.pos[i].w;
-        f3 g = xyz4(s.xb[i]);
+        float w = S.ld(i).w;
+        f3 g = xyz4(X.ld(i));
         f3 x = g + xyz4(s.pb[i]);
-        s.xb[i] = make_float4(x.x, x.y, x.z, 0.f);
+        X.st(i, make_float4(x.x, x.y, x.z, 0.f));
         f3 v = w > 0.f ? g * P.h : mk3(0, 0, 0);
         s.vb[i] = make_float4(v.x, v.y, v.z, 0.f);
     }
     __syncthreads();
 }
 
-__device__ void save_state(const Inst& s, float4* xs, float4* vs) {
+template <class Store>
+__device__ void save_state(const Inst& s, const Store& S, float4* xs, float4* vs) {
     for (int i = threadIdx.x; i < s.I.n_nodes; i += blockDim.x) {
-        xs[i] = s.pos[i];
+        xs[i] = S.ld(i);
         vs[i] = s.vel[i];
     }
     __syncthreads();
 }
-__device__ void load_state(const Inst& s, const float4* xs, const float4* vs) {
+template <class Store>
+__device__ void load_state(const Inst& s, const Store& S, const float4* xs, const float4* vs) {
     for (int i = threadIdx.x; i < s.I.n_nodes; i += blockDim.x) {
-        s.pos[i] = xs[i];
+        S.st(i, xs[i]);
         s.vel[i] = vs[i];
     }
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(1024) k_adjoint_forward(GtsAdjointView A, AdjStep P) {
-    const GtsBatchView& B = A.batch;
-    for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
-        Inst s = bind(A, P, B.work_items[w]);
-        const size_t sN = A.total_nodes;
-        for (int n = 0; n < P.nsub; n++) {
-            if (n % P.ckpt == 0) save_state(s, s.ckx + (size_t)(n / P.ckpt) * sN, s.ckv + (size_t)(n / P.ckpt) * sN);
-            substep_forward<false>(s, P);
+template <class Store>
+__device__ void forward_run(const Inst& s, const Store& S, const AdjStep& P, size_t sN) {
+    for (int n = 0; n < P.nsub; n++) {
+        if (n % P.ckpt == 0) save_state(s, S, s.ckx + (size_t)(n / P.ckpt) * sN, s.ckv + (size_t)(n / P.ckpt) * sN);
+        substep_forward<false>(s, S, P);
+    }
+}
+
+template <class Store, class XStore>
+__device__ void backward_run(const Inst& s, const Store& S, const XStore& X, const AdjStep& P, size_t sN) {
+    for (int seg = (P.nsub - 1) / P.ckpt; seg >= 0; seg--) {
+        const int n0 = seg * P.ckpt, n1 = min(P.nsub, n0 + P.ckpt);
+        load_state(s, S, s.ckx + (size_t)seg * sN, s.ckv + (size_t)seg * sN);
+        for (int n = n0; n < n1; n++) {
+            save_state(s, S, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
+            if (n + 1 < n1) substep_forward<false>(s, S, P);
+        }
+        for (int n = n1 - 1; n >= n0; n--) {
+            load_state(s, S, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
+            substep_forward<true>(s, S, P);
+            substep_reverse(s, S, X, P);
         }
     }
 }
 
-__global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_backward(GtsAdjointView A, AdjStep P) {
+// One workgroup per instance at a time. An instance whose nodes fit in the dynamic LDS (lds_nodes float4) keeps its
+// positions there for the whole rollout; a larger one works on its global slice.
+__global__ void __launch_bounds__(1024) k_adjoint_forward(GtsAdjointView A, AdjStep P) {
+    extern __shared__ float4 lds[];
     const GtsBatchView& B = A.batch;
+    const size_t sN = A.total_nodes;
     for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
         Inst s = bind(A, P, B.work_items[w]);
-        const size_t sN = A.total_nodes;
-        for (int k = threadIdx.x; k < 2 * s.I.n_tets; k += blockDim.x) s.abar[k] = 0.f;
+        const int N = s.I.n_nodes;
+        if (N <= P.lds_nodes) {
+            for (int i = threadIdx.x; i < N; i += blockDim.x) lds[i] = s.pos[i];
+            __syncthreads();
+            forward_run(s, LdsNodes{lds}, P, sN);
+            for (int i = threadIdx.x; i < N; i += blockDim.x) s.pos[i] = lds[i];
+        } else {
+            forward_run(s, GlobalNodes{s.pos}, P, sN);
+        }
         __syncthreads();
-        for (int seg = (P.nsub - 1) / P.ckpt; seg >= 0; seg--) {
-            const int n0 = seg * P.ckpt, n1 = min(P.nsub, n0 + P.ckpt);
-            load_state(s, s.ckx + (size_t)seg * sN, s.ckv + (size_t)seg * sN);
-            for (int n = n0; n < n1; n++) {
-                save_state(s, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
-                if (n + 1 < n1) substep_forward<false>(s, P);
-            }
-            for (int n = n1 - 1; n >= n0; n--) {
-                load_state(s, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
-                substep_forward<true>(s, P);
-                substep_reverse(s, P);
-            }
+    }
+}
+
+// The backward stages positions and position adjoints (2 N float4) in LDS when they fit.
+__global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_backward(GtsAdjointView A, AdjStep P) {
+    extern __shared__ float4 lds[];
+    const GtsBatchView& B = A.batch;
+    const size_t sN = A.total_nodes;
+    for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
+        Inst s = bind(A, P, B.work_items[w]);
+        const int N = s.I.n_nodes;
+        for (int k = threadIdx.x; k < 2 * s.I.n_tets; k += blockDim.x) s.abar[k] = 0.f;
+        if (2 * N <= P.lds_nodes) {
+            float4* xl = lds + N;
+            for (int i = threadIdx.x; i < N; i += blockDim.x) xl[i] = s.xb[i];
+            __syncthreads();
+            backward_run(s, LdsNodes{lds}, LdsNodes{xl}, P, sN);
+            for (int i = threadIdx.x; i < N; i += blockDim.x) s.xb[i] = xl[i];
+        } else {
+            __syncthreads();
+            backward_run(s, GlobalNodes{s.pos}, GlobalNodes{s.xb}, P, sN);
         }
         for (int k = threadIdx.x; k < s.I.n_tets; k += blockDim.x)
             s.grad[k] = (s.abar[k] + s.abar[s.I.n_tets + k] / 10.f) * P.invH2;
@@ -451,14 +489,34 @@ int plan_step(const GtsStepParams* p, int nsub, int ckpt, AdjStep& P) {
     P.nsub = nsub;
     P.ckpt = ckpt;
     P.nckpt = (nsub + ckpt - 1) / ckpt;
+    P.lds_nodes = 0;
     return GTS_OK;
 }
 
 int check_view(const GtsAdjointView* A) {
     if (!A || A->batch.n_workgroups <= 0 || A->batch.block_size <= 0 || A->batch.block_size > 1024 ||
-        A->total_nodes <= 0)
+        A->total_nodes <= 0 || A->batch.lds_nodes < 0)
         return GTS_ERR_BAD_ARG;
     return GTS_OK;
+}
+
+int lds_bytes_max() {
+    int dev = 0, bytes = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    return bytes;
+}
+
+template <class Kernel>
+int launch(Kernel kern, const GtsAdjointView* A, AdjStep& P, int threads, hipStream_t stream) {
+    size_t lds = (size_t)A->batch.lds_nodes * sizeof(float4);
+    if (lds > (size_t)lds_bytes_max()) return GTS_ERR_BAD_ARG;
+    P.lds_nodes = A->batch.lds_nodes;
+    hipError_t e =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return -(int)e;
+    kern<<<A->batch.n_workgroups, threads, lds, stream>>>(*A, P);
+    return -(int)hipGetLastError();
 }
 
 }  // namespace
@@ -468,8 +526,7 @@ extern "C" int gts_adjoint_forward(const GtsAdjointView* A, const GtsStepParams*
     AdjStep P;
     int rc = plan_step(p, nsub, ckpt, P);
     if (rc) return rc;
-    k_adjoint_forward<<<A->batch.n_workgroups, A->batch.block_size, 0, reinterpret_cast<hipStream_t>(stream)>>>(*A, P);
-    return -(int)hipGetLastError();
+    return launch(k_adjoint_forward, A, P, A->batch.block_size, reinterpret_cast<hipStream_t>(stream));
 }
 
 extern "C" int gts_adjoint_backward(const GtsAdjointView* A, const GtsStepParams* p, int nsub, int ckpt, void* stream) {
@@ -479,6 +536,5 @@ extern "C" int gts_adjoint_backward(const GtsAdjointView* A, const GtsStepParams
     if (rc) return rc;
     const int threads = A->batch.block_size < GTS_ADJOINT_BACKWARD_MAX_THREADS ? A->batch.block_size
                                                                                 : GTS_ADJOINT_BACKWARD_MAX_THREADS;
-    k_adjoint_backward<<<A->batch.n_workgroups, threads, 0, reinterpret_cast<hipStream_t>(stream)>>>(*A, P);
-    return -(int)hipGetLastError();
+    return launch(k_adjoint_backward, A, P, threads, reinterpret_cast<hipStream_t>(stream));
 }

@@ -127,7 +127,8 @@ __global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBat
     extern __shared__ float4 lds_nodes[];
     const int wg = blockIdx.x;
     for (int w = work_offsets[wg]; w < work_offsets[wg + 1]; w++) {
-        const GtsBatchInstance I = instances[work_items[w]];
+        // read the record in place: a private copy would index its Dm^-1 table out of scratch memory
+        const GtsBatchInstance& I = instances[work_items[w]];
         float4* gpos = pos + I.node_off;
         float4* gprev = prev + I.node_off;
         float4* gvel = vel + I.node_off;
@@ -256,16 +257,16 @@ extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, v
     hipGraph_t graph = nullptr;
     hipError_t end = hipStreamEndCapture(g_batch_capture, &graph);
     if (rc || end != hipSuccess) {
-        if (graph) hipGraphDestroy(graph);
+        if (graph) (void)hipGraphDestroy(graph);
         return rc ? rc : -(int)end;
     }
     BatchGraph g;
     g.key = key;
     hipError_t err = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     HIP_TRY(err);
     if (g_batch_graphs.size() >= 16) {
-        hipGraphExecDestroy(g_batch_graphs.front().exec);
+        (void)hipGraphExecDestroy(g_batch_graphs.front().exec);
         g_batch_graphs.erase(g_batch_graphs.begin());
     }
     g_batch_graphs.push_back(g);

@@ -690,21 +690,26 @@ struct Profile {
     std::vector<hipEvent_t> events;  // start, stop pairs
     std::vector<long long> units;
     size_t used = 0;
+    hipError_t error = hipSuccess;
 };
 Profile g_profile;
 
+// The first failing event call of a profiling window is kept and reported by gts_profile_end.
+inline void profile_note(hipError_t e) {
+    if (e != hipSuccess && g_profile.error == hipSuccess) g_profile.error = e;
+}
 inline void profile_mark_begin(hipStream_t s) {
     if (!g_profile.on) return;
     if (2 * g_profile.used + 2 > g_profile.events.size()) {
         size_t old = g_profile.events.size();
         g_profile.events.resize(old + 4096);
-        for (size_t k = old; k < g_profile.events.size(); k++) hipEventCreate(&g_profile.events[k]);
+        for (size_t k = old; k < g_profile.events.size(); k++) profile_note(hipEventCreate(&g_profile.events[k]));
     }
-    hipEventRecord(g_profile.events[2 * g_profile.used], s);
+    profile_note(hipEventRecord(g_profile.events[2 * g_profile.used], s));
 }
 inline void profile_mark_end(hipStream_t s, long long units) {
     if (!g_profile.on) return;
-    hipEventRecord(g_profile.events[2 * g_profile.used + 1], s);
+    profile_note(hipEventRecord(g_profile.events[2 * g_profile.used + 1], s));
     if (g_profile.units.size() <= g_profile.used) g_profile.units.resize(g_profile.used + 1024);
     g_profile.units[g_profile.used] = units;
     g_profile.used++;
@@ -956,10 +961,10 @@ int launch_step_graph(const GtsTissueView* v, const Lattice& L, const StepPlan& 
     GraphEntry e;
     e.key = key;
     hipError_t err = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     HIP_TRY(err);
     if (g_graphs.size() >= 16) {
-        hipGraphExecDestroy(g_graphs.front().exec);
+        (void)hipGraphExecDestroy(g_graphs.front().exec);
         g_graphs.erase(g_graphs.begin());
     }
     g_graphs.push_back(e);
@@ -1095,6 +1100,7 @@ extern "C" float gts_log(float x) { return own_log(x); }
 extern "C" int gts_profile_begin(void) {
     g_profile.on = true;
     g_profile.used = 0;
+    g_profile.error = hipSuccess;
     return GTS_OK;
 }
 
@@ -1102,6 +1108,12 @@ extern "C" int gts_profile_begin(void) {
 // tet colour launches. Synchronizes on the recorded events.
 extern "C" int gts_profile_end(double* total_ms, long long* launches, long long* units) {
     g_profile.on = false;
+    if (g_profile.error != hipSuccess) {
+        hipError_t e = g_profile.error;
+        g_profile.error = hipSuccess;
+        g_profile.used = 0;
+        return -(int)e;
+    }
     double total = 0.0;
     long long u = 0;
     for (size_t k = 0; k < g_profile.used; k++) {

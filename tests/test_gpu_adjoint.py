@@ -46,8 +46,12 @@ def oracle_adjoint(L, Oracle, inst, nsub, c, target, double=False):
     return loss, grad, o.state()[0]
 
 
-@pytest.mark.parametrize("scale,nsub,ckpt", [(1e4, 20, 7), (1.0, 12, 4)])
-def test_adjoint_matches_oracle_fp32(scale, nsub, ckpt):
+# The blocks have 112, 180 and 150 nodes. lds_nodes: None stages every block in LDS in both passes; 0 runs every block
+# from global memory; 160 stages the first and last block in the forward and none in the backward (which needs 2 N);
+# 300 stages all three in the forward and the first and last in the backward.
+@pytest.mark.parametrize("scale,nsub,ckpt,lds_nodes", [(1e4, 20, 7, None), (1.0, 12, 4, None), (1e4, 20, 7, 0),
+                                                       (1e4, 9, 4, 160), (1e4, 9, 4, 300)])
+def test_adjoint_matches_oracle_fp32(scale, nsub, ckpt, lds_nodes):
     import torch
 
     from genesis_tissue_simulation_amd.adjoint import ComplianceFit
@@ -55,11 +59,11 @@ def test_adjoint_matches_oracle_fp32(scale, nsub, ckpt):
 
     L = _bind(orc.lib())
     insts = blocks(scale)
-    fit = ComplianceFit(insts, nsub=nsub, ckpt=ckpt, block_size=256)
+    fit = ComplianceFit(insts, nsub=nsub, ckpt=ckpt, block_size=256, lds_nodes=lds_nodes)
     b = fit.batch
     hidden = np.concatenate([i.compliance_hidden for i in insts]).astype(np.float32)
     init = np.concatenate([i.compliance_init for i in insts]).astype(np.float32)
-    print(f"adjoint parity: blocks {SIZES} scale {scale} nsub {nsub} ckpt {ckpt}", flush=True)
+    print(f"adjoint parity: blocks {SIZES} scale {scale} nsub {nsub} ckpt {ckpt} lds_nodes {lds_nodes}", flush=True)
     target = fit.forward_raw(torch.as_tensor(hidden)).cpu().numpy().copy()
     x = fit.forward_raw(torch.as_tensor(init)).cpu().numpy().copy()
     mask = fit.mask.cpu().numpy()
