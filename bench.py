@@ -31,14 +31,84 @@ def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
     return (52.0 * active_nodes + 8.0 * tets + 4.0 * edges) / tets
 
 
-def load_traffic(name):
-    path = ROOT / "profiles" / name
-    if path.exists():
-        try:
-            return json.loads(path.read_text())
-        except ValueError:
-            return None
-    return None
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
+# kernels whose HBM bytes per dispatch the traffic file records, by the name fragment rocprofv3 reports
+TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_batched_step": "k_batched_step<", "k_adjoint_forward": "k_adjoint_forward",
+                   "k_adjoint_backward": "k_adjoint_backward", "k_stream_copy": "k_stream_copy"}
+
+
+def load_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary, or None."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    try:
+        return json.loads(TRAFFIC_FILE.read_text())["kernels"].get(kernel)
+    except (ValueError, KeyError):
+        return None
+
+
+def stream_copy(mib, repeats=5):
+    """Copies `mib` MiB with the library's streaming copy kernel `repeats` times (the known-size dispatches the
+    counter runs are scaled by) and returns the copy bandwidth in GB/s (bytes read + written per second)."""
+    import torch
+
+    from genesis_tissue_simulation_amd._lib import check, load
+    from genesis_tissue_simulation_amd.tissue import current_stream
+
+    lib = load()
+    n = mib * (1 << 20) // 16
+    src = torch.ones((n, 4), dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    check("gts_stream_copy", lib.gts_stream_copy(dst.data_ptr(), src.data_ptr(), n, current_stream()))
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(repeats):
+        check("gts_stream_copy", lib.gts_stream_copy(dst.data_ptr(), src.data_ptr(), n, current_stream()))
+    stop.record()
+    torch.cuda.synchronize()
+    return 2 * 16 * n * repeats / (start.elapsed_time(stop) * 1e-3) / 1e9
+
+
+def counter_means(path, counter):
+    """Mean value per dispatch of `counter` for each kernel of TRAFFIC_KERNELS in a rocprofv3 counter CSV."""
+    import csv
+
+    per_dispatch = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            name = row.get("Kernel_Name", "")
+            for key, frag in TRAFFIC_KERNELS.items():
+                if frag in name:
+                    d = per_dispatch.setdefault(key, {})
+                    did = row.get("Dispatch_Id") or row.get("Correlation_Id")
+                    d[did] = d.get(did, 0.0) + float(row["Counter_Value"])
+    return {k: (sum(v.values()) / len(v), len(v)) for k, v in per_dispatch.items()}
+
+
+def write_traffic(fetch_csv, write_csv, copy_mib, command):
+    """HBM bytes per dispatch of each kernel from a FETCH_SIZE and a WRITE_SIZE counter run (KiB per dispatch), each
+    scaled by the factor that makes the streaming copy of `copy_mib` MiB read and write exactly its size."""
+    fetch, write = counter_means(fetch_csv, "FETCH_SIZE"), counter_means(write_csv, "WRITE_SIZE")
+    if "k_stream_copy" not in fetch or "k_stream_copy" not in write:
+        raise SystemExit("the counter runs hold no k_stream_copy dispatch: run them with --copy-calibration-mib")
+    kib = copy_mib * 1024.0
+    f_fetch, f_write = kib / fetch["k_stream_copy"][0], kib / write["k_stream_copy"][0]
+    out = {"command": command, "counters": "FETCH_SIZE and WRITE_SIZE in KiB per dispatch, separate runs",
+           "calibration": {"kernel": "k_stream_copy", "bytes_each_way": copy_mib << 20,
+                           "fetch_kib": fetch["k_stream_copy"][0], "write_kib": write["k_stream_copy"][0],
+                           "fetch_factor": f_fetch, "write_factor": f_write},
+           "kernels": {}}
+    for k in TRAFFIC_KERNELS:
+        if k in fetch and k in write:
+            rb, wb = fetch[k][0] * f_fetch * 1024.0, write[k][0] * f_write * 1024.0
+            out["kernels"][k] = {"dispatches": fetch[k][1], "read_bytes_per_launch": rb,
+                                 "write_bytes_per_launch": wb, "bytes_per_launch": rb + wb}
+    TRAFFIC_FILE.parent.mkdir(exist_ok=True)
+    TRAFFIC_FILE.write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out, indent=1))
 
 
 def run_gpu(w, steps, warmup):
@@ -220,6 +290,12 @@ def adjoint_fp64_checks(a):
     return out
 
 
+def adjoint_traffic():
+    """HBM bytes of one forward plus one backward adjoint launch, or None."""
+    f, b = load_traffic("k_adjoint_forward"), load_traffic("k_adjoint_backward")
+    return f["bytes_per_launch"] + b["bytes_per_launch"] if f and b else None
+
+
 def run_cpu_batched(w, threads, sample=8):
     from oracle import oracle as orc
 
@@ -272,8 +348,16 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--batched-count", type=int, default=1024)
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=256,
+                    help="size of the streaming copy run first (the counter runs' known-size reference); 0 skips it")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/hbm_traffic.json from two rocprofv3 counter CSVs and exit (no GPU work)")
+    ap.add_argument("--traffic-command", default="", help="the profiled command, recorded in the traffic file")
     ap.add_argument("--adjoint-iters", type=int, default=3)
     args = ap.parse_args()
+    if args.traffic:
+        write_traffic(args.traffic[0], args.traffic[1], args.copy_calibration_mib, args.traffic_command)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the tissue path runs on a single MI355X and has no multi-GPU mode "
               "(instances shard by LPT packing across GPUs by design only)", file=sys.stderr)
@@ -281,6 +365,7 @@ def main():
 
     from genesis_tissue_simulation_amd.workloads import kidney_250k
 
+    copy_gbs = stream_copy(args.copy_calibration_mib) if args.copy_calibration_mib > 0 else None
     w = kidney_250k()
     g = run_gpu(w, args.steps, args.warmup)
     substeps = w.params.substeps
@@ -288,7 +373,7 @@ def main():
     b_alg = algorithmic_bytes_per_tet_substep(g["active"], g["T"], g["E"])
     avg_launch_ms = g["tet_ms"] / max(g["launches"], 1)
     achieved = b_alg * g["units"] / (g["tet_ms"] * 1e-3) / 1e9 if g["tet_ms"] > 0 else 0.0
-    traffic = load_traffic("tet_colour_traffic_kidney.json")
+    traffic = load_traffic("k_tets")
     line = {
         "metric": "tet-substeps/s plain vs batched vs adjoint @ <=1e-4 rel err vs FP64",
         "value": value,
@@ -307,9 +392,11 @@ def main():
                    "dt": w.dt, "contact": "capsule grasper", "cut": "swept-quad blade, once per step"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
+                     "stream_copy_GBs": copy_gbs,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": "k_tets (one colour of the tet pass)", "avg_launch_us": 1e3 * avg_launch_ms,
-                     "launches": g["launches"], "bytes_per_tet_update": b_alg},
+                     "launches": g["launches"], "bytes_per_tet_update": b_alg,
+                     "algorithmic_bytes_per_launch": b_alg * g["units"] / max(g["launches"], 1)},
     }
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.skip_batched:
@@ -327,7 +414,8 @@ def main():
             "roofline": {"bound": "hbm", "achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": bach / HBM_PEAK_GBS, "kernel": "k_batched_step (one launch per step)",
                          "avg_launch_us": 1e3 * bt["kernel_ms"] / args.batched_steps, "bytes_per_tet_substep": bb,
-                         "traffic": None},
+                         "traffic": (load_traffic("k_batched_step") or {}).get("bytes_per_launch"),
+                         "algorithmic_bytes_per_launch": bb * bt["live"] * bt["substeps"]},
             "autotune": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in bt["table"]],
         }
         if not args.no_cpu:
@@ -351,7 +439,9 @@ def main():
                          "frac": aach / HBM_PEAK_GBS,
                          "kernel": "k_adjoint_forward + k_adjoint_backward (one launch each per iteration)",
                          "avg_iteration_us": 1e3 * a["kernel_ms"] / args.adjoint_iters,
-                         "bytes_per_tet_substep": ab, "traffic": None},
+                         "bytes_per_tet_substep": ab,
+                         "traffic": adjoint_traffic(),
+                         "algorithmic_bytes_per_iteration": ab * a["T"] * a["nsub"]},
         }
         checks = adjoint_fp64_checks(a)
         line["adjoint"]["rel_err_vs_fp64"] = checks["stated"]["rel_err_positions"]

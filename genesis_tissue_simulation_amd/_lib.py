@@ -107,6 +107,7 @@ SIGNATURES = {
     "gts_batched_plan": ((P, I, I, P), I),
     "gts_batched_step": ((P, P, P), I),
     "gts_batched_lds_bytes": ((), I),
+    "gts_stream_copy": ((P, P, ct.c_longlong, P), I),
     "gts_adjoint_forward": ((P, P, I, I, P), I),
     "gts_adjoint_backward": ((P, P, I, I, P), I),
 }

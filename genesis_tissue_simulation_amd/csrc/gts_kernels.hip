@@ -10,6 +10,7 @@
 //   per-tet alpha~ and lambda pairs in colour order, so a colour pass streams contiguous ranges.
 // Each colour of the reference's greedy colourings is one launch: no two members share a node, so the Gauss-Seidel
 // pass needs no atomics and gives the same result as the serial sweep (reference: xpbd/xpbd_solver.cu:3581-3612).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -698,18 +699,28 @@ Profile g_profile;
 inline void profile_note(hipError_t e) {
     if (e != hipSuccess && g_profile.error == hipSuccess) g_profile.error = e;
 }
-inline void profile_mark_begin(hipStream_t s) {
-    if (!g_profile.on) return;
+
+template <class T> struct same_type { using type = T; };
+
+// Launches `kern` on stream s. While profiling, the launch goes through hipExtLaunchKernel with a start and a stop
+// event that the runtime stamps when the kernel itself starts and ends, so the recorded duration is the kernel's own
+// (no enqueue or dispatch latency), and `units` (tets) are credited to the launch.
+template <class... A>
+void profiled_launch(void (*kern)(A...), dim3 g, dim3 b, hipStream_t s, long long units,
+                     typename same_type<A>::type... args) {
+    if (!g_profile.on) {
+        kern<<<g, b, 0, s>>>(args...);
+        return;
+    }
     if (2 * g_profile.used + 2 > g_profile.events.size()) {
         size_t old = g_profile.events.size();
         g_profile.events.resize(old + 4096);
         for (size_t k = old; k < g_profile.events.size(); k++) profile_note(hipEventCreate(&g_profile.events[k]));
     }
-    profile_note(hipEventRecord(g_profile.events[2 * g_profile.used], s));
-}
-inline void profile_mark_end(hipStream_t s, long long units) {
-    if (!g_profile.on) return;
-    profile_note(hipEventRecord(g_profile.events[2 * g_profile.used + 1], s));
+    void* ptrs[] = {(void*)&args...};
+    profile_note(hipExtLaunchKernel(reinterpret_cast<const void*>(kern), g, b, ptrs, 0, s,
+                                    g_profile.events[2 * g_profile.used], g_profile.events[2 * g_profile.used + 1],
+                                    0));
     if (g_profile.units.size() <= g_profile.used) g_profile.units.resize(g_profile.used + 1024);
     g_profile.units[g_profile.used] = units;
     g_profile.used++;
@@ -748,8 +759,8 @@ void launch_tet_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s
     dim3 g(grid_for(cnt)), b(GTS_BLOCK);
     const bool read = it > 0, write = iters > 1;
 #define GTS_TET(R, W)                                                                                            \
-    k_tets<MODE, R, W, DAMPED><<<g, b, 0, s>>>(off, cnt, v->n_tets, L, omega, dOverH, bOverH, v->tet_desc,      \
-                                               v->tet_alpha, v->tet_lambda, (float4*)v->pos, (const float4*)v->prev)
+    profiled_launch(k_tets<MODE, R, W, DAMPED>, g, b, s, cnt, off, cnt, v->n_tets, L, omega, dOverH, bOverH,     \
+                    v->tet_desc, v->tet_alpha, v->tet_lambda, (float4*)v->pos, (const float4*)v->prev)
     if (read && write) GTS_TET(true, true);
     else if (read) GTS_TET(true, false);
     else if (write) GTS_TET(false, true);
@@ -869,11 +880,9 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
                 for (int c = 0; c < v->n_tet_colours; c++) {
                     int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
                     if (cnt <= 0) continue;
-                    profile_mark_begin(s);
                     if (P.mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
                     else if (P.mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
                     else launch_tet_colour<2>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
-                    profile_mark_end(s, cnt);
                 }
         }
         const StepShapes* sh = shapes + n;
@@ -1096,6 +1105,23 @@ extern "C" int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, 
 
 extern "C" float gts_iso_pow(float J) { return iso_pow(J); }
 extern "C" float gts_log(float x) { return own_log(x); }
+
+namespace {
+// Streaming copy of n float4: the known-size reference for HBM byte counters and the achievable copy bandwidth.
+__global__ void __launch_bounds__(256) k_stream_copy(long long n, const float4* __restrict__ src,
+                                                     float4* __restrict__ dst) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256LL) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int gts_stream_copy(void* dst, const void* src, long long n_float4, void* stream) {
+    if (!dst || !src || n_float4 <= 0) return GTS_ERR_BAD_ARG;
+    long long blocks = (n_float4 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    k_stream_copy<<<(unsigned)blocks, 256, 0, reinterpret_cast<hipStream_t>(stream)>>>(
+        n_float4, reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst));
+    return -(int)hipGetLastError();
+}
 
 extern "C" int gts_profile_begin(void) {
     g_profile.on = true;

@@ -252,6 +252,10 @@ int gts_adjoint_forward(const GtsAdjointView* a, const GtsStepParams* p, int nsu
 #define GTS_ADJOINT_BACKWARD_MAX_THREADS 512
 int gts_adjoint_backward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
 
+/* Device: copy n_float4 16-byte elements (no reference counterpart): the known-size kernel against which HBM byte
+ * counters are scaled, and the achievable copy bandwidth. */
+int gts_stream_copy(void* dst, const void* src, long long n_float4, void* stream);
+
 /* Timing of the dominant kernel (tet colour pass) with HIP events on its launch stream, per launch
  * (in the spirit of LCS_ProfileEnable/LCS_ProfileRead, plugin_api.cpp:21-26). */
 int gts_profile_begin(void);
