@@ -192,7 +192,7 @@ def run_batched(steps, warmup, count=1024):
     active = sum(int((inst[0].active != 0).sum()) for inst in w.instances)
     return dict(elapsed=elapsed, kernel_ms=kernel_ms, live=live, T=b.total_tets, E=int(b.edge_off[-1]),
                 active=active, substeps=w.params.substeps, best=best, table=table, setup_s=setup, count=count,
-                workload=w)
+                workload=w, steps_run=warmup + steps, positions0=b.positions(0))
 
 
 def algorithmic_bytes_per_adjoint_tet_substep(nodes, tets, edges):
@@ -296,22 +296,84 @@ def adjoint_traffic():
     return f["bytes_per_launch"] + b["bytes_per_launch"] if f and b else None
 
 
-def run_cpu_batched(w, threads, sample=8):
+def run_cpu_batched(w, threads, sample=64):
+    """The oracle on the first `sample` instances (after their cuts), one step each, the instances spread over the
+    threads."""
     from oracle import oracle as orc
 
     used = orc.set_threads(threads)
     sims = []
-    for grid, p, kd, kh, init in w.instances[:sample]:
+    for (grid, p, kd, kh, init), cut in zip(w.instances[:sample], w.cuts[:sample]):
         o = orc.Oracle(grid, p, init)
         o.set_tet_stiffness(kd, kh)
         o.set_gravity(w.gravity)
+        o.detect_cut(cut)
         sims.append(o)
     t0 = time.perf_counter()
-    for o in sims:
-        o.step(w.dt)
+    orc.step_many(sims, w.dt)
     elapsed = time.perf_counter() - t0
-    tets = sum(o.T for o in sims)
-    return dict(value=tets * w.params.substeps / elapsed, threads=used, tets=tets)
+    tets = sum(int(o.liveness()[0].sum()) for o in sims)
+    return dict(value=tets * w.params.substeps / elapsed, threads=used, tets=tets, sample=sample)
+
+
+def batched_fp64_error(bt, k=0):
+    """Instance k of the batched run against the oracle stepped through the same history (its cut, then the warm-up
+    and timed steps): relative position error against FP64, and whether the FP32 oracle is matched bit for bit."""
+    from oracle import oracle as orc
+
+    w = bt["workload"]
+    grid, p, kd, kh, init = w.instances[k]
+    out = {}
+    for double in (False, True):
+        o = orc.Oracle(grid, p, init, double=double)
+        o.set_tet_stiffness(kd, kh)
+        o.set_gravity(w.gravity)
+        o.detect_cut(w.cuts[k])
+        for _ in range(bt["steps_run"]):
+            o.step(w.dt)
+        out[double] = o.state()[0].astype(np.float64)
+    x = bt["positions0"].astype(np.float64)
+    return dict(instance=k, tets=int(len(bt["workload"].instances[k][2])), steps=bt["steps_run"],
+                rel_err_vs_fp64=float(np.abs(x - out[True]).max() / np.abs(out[True]).max()),
+                bit_exact_vs_fp32_oracle=bool(np.array_equal(x, out[False])))
+
+
+def plain_fp64_error(w, steps=5):
+    """A fresh GPU run of `steps` steps of the plain workload (tool, blade cut every step) against the oracle of the
+    same schedule: relative position error against FP64 and bit-exactness against FP32."""
+    import torch
+
+    from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
+    from oracle import oracle as orc
+
+    kd, kh = w.stiffness()
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions)
+    t.set_gravity(w.gravity)
+    oracles = {}
+    for double in (False, True):
+        o = orc.Oracle(w.grid, w.params, w.initial_positions, double=double)
+        o.set_tet_stiffness(kd, kh)
+        o.set_gravity(w.gravity)
+        oracles[double] = o
+    for s in range(steps):
+        tool = w.tool(s) if w.tool else None
+        t.step(w.dt, tool=tool, tool_armed=tool is not None)
+        cut = swept_cut_tool(w.blade(s), w.blade(s + 1), w.blade_thickness) if w.blade else None
+        if cut is not None:
+            t.detect_cut(cut)
+        for o in oracles.values():
+            if tool is not None:
+                o.set_tool(tool, tool_armed=True, reset=(s == 0))
+            o.step(w.dt)
+            if cut is not None:
+                o.detect_cut(cut)
+    torch.cuda.synchronize()
+    x = t.positions().astype(np.float64)
+    x64 = oracles[True].state()[0].astype(np.float64)
+    x32 = oracles[False].state()[0]
+    return dict(steps=steps, substeps=steps * w.params.substeps,
+                rel_err_vs_fp64=float(np.abs(x - x64).max() / np.abs(x64).max()),
+                bit_exact_vs_fp32_oracle=bool(np.array_equal(t.positions(), x32)))
 
 
 def run_cpu(w, steps, threads):
@@ -419,10 +481,13 @@ def main():
             "autotune": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in bt["table"]],
         }
         if not args.no_cpu:
+            line["batched"]["fp64_check"] = batched_fp64_error(bt)
             cb = run_cpu_batched(bt["workload"], threads)
             line["batched"]["cpu_baseline"] = {"value": cb["value"], "unit": "tet-substeps/s", "cores": cb["threads"],
-                                               "kind": "port", "sample": f"1 step of the first 8 instances "
-                                                                         f"({cb['tets']} tets)"}
+                                               "kind": "port",
+                                               "sample": f"1 step of the first {cb['sample']} instances after their "
+                                                         f"cuts ({cb['tets']} live tets), instances spread over the "
+                                                         f"threads"}
     if not args.skip_adjoint:
         a = run_adjoint(args.adjoint_iters, 1)
         av = a["T"] * a["nsub"] * args.adjoint_iters / a["elapsed"]
@@ -447,6 +512,7 @@ def main():
         line["adjoint"]["rel_err_vs_fp64"] = checks["stated"]["rel_err_positions"]
         line["adjoint"]["fp64_checks"] = checks
     if not args.no_cpu:
+        line["fp64_check"] = plain_fp64_error(w)
         c = run_cpu(w, args.cpu_steps, threads)
         line["cpu_baseline"] = {
             "value": c["live"] * substeps * args.cpu_steps / c["elapsed"], "unit": "tet-substeps/s",

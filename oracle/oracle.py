@@ -121,6 +121,7 @@ class Oracle:
 
     def __init__(self, grid, params, initial_positions=None, double=False):
         self.L = lib()
+        self.double = bool(double)
         self.suf = "f64" if double else "f32"
         self.real = np.float64 if double else np.float32
         self.grid = grid
@@ -240,6 +241,20 @@ class Oracle:
         counts = np.zeros(3, dtype=np.int32)
         self._call("compact", self.h, *[_ptr(a) for a in (tr, er, nr, counts)])
         return tr, er, nr, counts
+
+
+def step_many(oracles, dt):
+    """One step of every oracle in `oracles` (all FP32 or all FP64), spread over the threads by instance."""
+    if not oracles:
+        return
+    double = oracles[0].double
+    L = lib()
+    f = L.oracle_step_many_f64 if double else L.oracle_step_many_f32
+    f.argtypes = [ct.c_void_p, ct.c_int, ct.c_float]
+    f.restype = ct.c_int
+    handles = (ct.c_void_p * len(oracles))(*[o.h for o in oracles])
+    if f(handles, len(oracles), float(dt)) != 0:
+        raise RuntimeError("oracle step failed")
 
 
 def set_threads(n):

@@ -1244,6 +1244,18 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
 ORACLE_API(float, f32)
 ORACLE_API(double, f64)
 
+// One outer step of each of n independent instances, the instances spread over the threads (the colour loops inside
+// each step then run on one thread): the CPU counterpart of the batched launch.
+#define ORACLE_MANY(R, SUF)                                                                                           \
+    extern "C" int oracle_step_many_##SUF(void** h, int n, float dt) {                                               \
+        int bad = 0;                                                                                                   \
+        _Pragma("omp parallel for schedule(dynamic, 1) reduction(|| : bad)") for (int k = 0; k < n; k++) bad =         \
+            bad || step(*(Tissue<R>*)h[k], (R)dt) != 0;                                                                \
+        return bad ? -1 : 0;                                                                                           \
+    }
+ORACLE_MANY(float, f32)
+ORACLE_MANY(double, f64)
+
 extern "C" int oracle_threads(int n) {
 #ifdef _OPENMP
     if (n > 0) omp_set_num_threads(n);
