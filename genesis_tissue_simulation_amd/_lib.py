@@ -79,9 +79,10 @@ class GtsBatchView(ct.Structure):
 
 
 class GtsAdjointView(ct.Structure):
-    _fields_ = [("batch", GtsBatchView), ("total_nodes", ct.c_longlong), ("force", P), ("compliance", P),
-                ("checkpoints", P), ("segment", P), ("tet_tape", P), ("edge_tape", P), ("xbar", P), ("vbar", P),
-                ("prevbar", P), ("abar", P), ("grad", P)]
+    _fields_ = [("batch", GtsBatchView), ("total_nodes", ct.c_longlong), ("total_tets", ct.c_longlong),
+                ("total_edges", ct.c_longlong), ("force", P), ("compliance", P), ("alpha", P), ("checkpoints", P),
+                ("tet_tape", P), ("edge_tape", P), ("lambda_tape", P), ("xbar", P), ("vbar", P), ("prevbar", P),
+                ("abar", P), ("lambar", P), ("grad", P)]
 
 
 STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool, GtsBatchInstance,

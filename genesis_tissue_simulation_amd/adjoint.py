@@ -1,10 +1,11 @@
 """Per-tet compliance gradients of an XPBD rollout: the hand-written adjoint kernels behind torch.autograd.
 
-The rollout is `nsub` substeps of the undamped one-iteration substep with coupled Neo-Hookean tets (deviatoric
-compliance c per tet, volume compliance c/10) and lattice edges, driven by a per-node force field, with no contact
-and no cutting: gradients are not taken across cut events. Many blocks run together, one workgroup per block. The
-forward kernel keeps (x, v) every `ckpt` substeps; the backward kernel recomputes each segment from its checkpoint
-and reverses it substep by substep, colour by colour.
+The rollout is `nsub` substeps of the undamped substep (`params.iterations` Gauss-Seidel iterations) with coupled
+Neo-Hookean tets (deviatoric compliance c per tet, volume compliance c/10) and lattice edges, driven by a per-node
+force field, with no contact and no cutting: gradients are not taken across cut events. Many blocks run together,
+one workgroup per block. The forward kernel keeps (x, v) every `ckpt` substeps; the backward kernel re-runs each
+segment from its checkpoint with a tape of every projection's inputs and reverses it substep by substep, iteration
+by iteration, colour by colour.
 
 There is no CPU path: the library must be built and a GPU present.
 """
@@ -33,11 +34,11 @@ class ComplianceFit:
         p0 = instances[0].params
         for inst in instances:
             p = inst.params
-            if (p.tet_mode != TET_NEOHOOKEAN or p.iterations != 1 or p.substeps != p0.substeps or
-                    inst.dt != instances[0].dt or p.strain_damping or p.bulk_damping or p.beta_s or p.damping or
-                    p.rayleigh or p.omega != 1.0 or not p.edge_ks > 0):
-                raise ValueError("the adjoint covers the undamped one-iteration Neo-Hookean substep with edges; all "
-                                 "blocks must share dt and substeps")
+            if (p.tet_mode != TET_NEOHOOKEAN or p.iterations < 1 or p.iterations != p0.iterations or
+                    p.substeps != p0.substeps or inst.dt != instances[0].dt or p.strain_damping or p.bulk_damping or
+                    p.beta_s or p.damping or p.rayleigh or p.omega != 1.0 or not p.edge_ks > 0):
+                raise ValueError("the adjoint covers the undamped Neo-Hookean substep with edges; all blocks must "
+                                 "share dt, substeps and iterations")
         self.lib = load()
         self.device = torch.device(device)
         self.nsub, self.ckpt = int(nsub), int(ckpt)
@@ -66,23 +67,27 @@ class ComplianceFit:
         self.mask = torch.as_tensor(mask).to(dev)
         self.perm = torch.as_tensor(perm).to(dev)      # colour-major slot -> generation index
         f4 = lambda n: torch.zeros((max(n, 1), 4), dtype=torch.float32, device=dev)
-        self.compliance = torch.zeros(max(sT, 1), dtype=torch.float32, device=dev)
+        f1 = lambda n: torch.zeros(max(n, 1), dtype=torch.float32, device=dev)
+        slots = self.ckpt * p0.iterations
+        self.compliance = f1(sT)
+        self.alpha = f1(2 * sT)
         self.checkpoints = f4(2 * self.n_ckpt * sN)
-        self.segment = f4(2 * self.ckpt * sN)
-        self.tet_tape = f4(4 * sT)
-        self.edge_tape = f4(2 * sE)
+        self.tet_tape = f4(slots * 4 * sT)
+        self.edge_tape = f4(slots * 2 * sE)
+        self.lambda_tape = f1(slots * (2 * sT + sE) if p0.iterations > 1 else 0)
         self.xbar, self.vbar, self.prevbar = f4(sN), f4(sN), f4(sN)
-        self.abar = torch.zeros(max(2 * sT, 1), dtype=torch.float32, device=dev)
-        self.grad = torch.zeros(max(sT, 1), dtype=torch.float32, device=dev)
+        self.abar = f1(2 * sT)
+        self.lambar = f1(2 * sT + sE)
+        self.grad = f1(sT)
         self.version = 0
         self._view()
 
     def _view(self):
         self.view = GtsAdjointView(
-            self.batch.view, self.sN, self.force.data_ptr(), self.compliance.data_ptr(),
-            self.checkpoints.data_ptr(), self.segment.data_ptr(), self.tet_tape.data_ptr(),
-            self.edge_tape.data_ptr(), self.xbar.data_ptr(), self.vbar.data_ptr(), self.prevbar.data_ptr(),
-            self.abar.data_ptr(), self.grad.data_ptr())
+            self.batch.view, self.sN, self.sT, self.sE, self.force.data_ptr(), self.compliance.data_ptr(),
+            self.alpha.data_ptr(), self.checkpoints.data_ptr(), self.tet_tape.data_ptr(), self.edge_tape.data_ptr(),
+            self.lambda_tape.data_ptr(), self.xbar.data_ptr(), self.vbar.data_ptr(), self.prevbar.data_ptr(),
+            self.abar.data_ptr(), self.lambar.data_ptr(), self.grad.data_ptr())
 
     def configure(self, block_size=1024, workgroups_per_cu=1, lds_nodes=None):
         """Launch shape: threads per workgroup (the backward uses at most 512), workgroups per CU and the LDS node
@@ -117,7 +122,8 @@ class ComplianceFit:
 
     def backward_raw(self, grad_positions, grad_velocities=None):
         """Gradient of a scalar with respect to the compliance of the last forward_raw, given its gradients with
-        respect to the end positions (and velocities) [sum N, 3]; returned in generation order."""
+        respect to the end positions (and velocities) [sum N, 3]; returned in generation order. The backward reuses
+        the forward's checkpoints and per-tet alphas, so it must follow the forward it differentiates."""
         self.xbar.zero_()
         self.vbar.zero_()
         self.xbar[:self.sN, :3].copy_(grad_positions)

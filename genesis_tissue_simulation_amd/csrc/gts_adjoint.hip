@@ -1,12 +1,13 @@
 // Adjoint of the XPBD substep for per-tet compliance gradients, batched over instances (one workgroup each).
 //
-// No reference counterpart: the reference's solver has no gradient. The forward is the legacy substep with one
-// iteration, no damping and no contact (coupled Neo-Hookean tets, edges with the instance's compliance), driven by a
-// per-node force field. Checkpoints of (x, v) are kept every `ckpt` substeps; the backward pass walks the segments
-// last to first: it recomputes a segment's substep start states from its checkpoint, then for each substep in
-// reverse re-runs it forward while taping the inputs of every edge and tet projection, and reverses it colour by
-// colour. Within a colour no two projections share a node, so the adjoint scatter needs no atomics, exactly as the
-// forward sweep. The operation sequence mirrors the CPU checker's restatement, so the two agree bit for bit.
+// No reference counterpart: the reference's solver has no gradient. The forward is the legacy substep with
+// `iterations` Gauss-Seidel iterations (multipliers reset every substep), no damping and no contact (coupled
+// Neo-Hookean tets, edges with the instance's compliance), driven by a per-node force field. Checkpoints of (x, v)
+// are kept every `ckpt` substeps. The backward pass walks the segments last to first: it re-runs a segment forward
+// from its checkpoint, taping the inputs (positions and multipliers) of every projection of every substep and
+// iteration, then reverses the substeps last to first, iterations last to first and colours last to first. Within a
+// colour no two projections share a node, so the adjoint scatter needs no atomics, exactly as the forward sweep.
+// The operation sequence mirrors the CPU checker's restatement, so the two agree bit for bit.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,12 +22,12 @@ namespace {
 
 // The scalar fields of a GtsBatchInstance record.
 struct GtsBatchInstanceHead {
-    int node_off, n_nodes, edge_off, tet_off, n_tets, nx, nxy, n_edge_colours, n_tet_colours, colour_off;
+    int node_off, n_nodes, edge_off, n_edges, tet_off, n_tets, nx, nxy, n_edge_colours, n_tet_colours, colour_off;
     float edge_ks;
 };
 
 struct AdjStep {
-    int nsub, ckpt, nckpt, lds_nodes;
+    int nsub, ckpt, nckpt, iters, lds_nodes;
     float h, invH, invH2, velScale;
 };
 
@@ -34,13 +35,15 @@ struct AdjStep {
 struct Inst {
     const GtsBatchInstance* rec;  // read in place: a local copy of the Dm^-1 table would be indexed from scratch
     GtsBatchInstanceHead I;
-    float4 *pos, *prev, *vel, *xb, *vb, *pb, *ckx, *ckv, *segx, *segv, *ttape, *etape;
+    float4 *pos, *prev, *vel, *xb, *vb, *pb, *ckx, *ckv;
+    float4 *ttape, *etape;  // slot 0 of this instance; slots are tslot / eslot float4 apart
+    float *ltape_t, *ltape_e;
     const float4* force;
     const uint32_t *ed, *td;
     const float* comp;
-    float* abar;
-    float* grad;
+    float *alpha, *abar, *lamb_t, *lamb_e, *lam_t, *lam_e, *grad;
     const int *ecol, *tcol;
+    size_t tslot, eslot, lslot;
 };
 
 __device__ __forceinline__ Inst bind(const GtsAdjointView& A, const AdjStep& P, int item) {
@@ -50,6 +53,7 @@ __device__ __forceinline__ Inst bind(const GtsAdjointView& A, const AdjStep& P, 
     s.I.node_off = s.rec->node_off;
     s.I.n_nodes = s.rec->n_nodes;
     s.I.edge_off = s.rec->edge_off;
+    s.I.n_edges = s.rec->n_edges;
     s.I.tet_off = s.rec->tet_off;
     s.I.n_tets = s.rec->n_tets;
     s.I.nx = s.rec->nx;
@@ -58,7 +62,7 @@ __device__ __forceinline__ Inst bind(const GtsAdjointView& A, const AdjStep& P, 
     s.I.n_tet_colours = s.rec->n_tet_colours;
     s.I.colour_off = s.rec->colour_off;
     s.I.edge_ks = s.rec->edge_ks;
-    const size_t n0 = s.I.node_off, N = s.I.n_nodes, sN = A.total_nodes;
+    const size_t n0 = s.I.node_off, sN = A.total_nodes, sT = A.total_tets, sE = A.total_edges;
     s.pos = (float4*)B.pos + n0;
     s.prev = (float4*)B.prev + n0;
     s.vel = (float4*)B.vel + n0;
@@ -66,28 +70,30 @@ __device__ __forceinline__ Inst bind(const GtsAdjointView& A, const AdjStep& P, 
     s.vb = (float4*)A.vbar + n0;
     s.pb = (float4*)A.prevbar + n0;
     s.force = (const float4*)A.force + n0;
-    // checkpoints: [nckpt][sum N] rows of x then v; segment states: [ckpt][sum N]
+    // checkpoints: [nckpt][sum N] rows of x, then [nckpt][sum N] rows of v
     s.ckx = (float4*)A.checkpoints + n0;
     s.ckv = (float4*)A.checkpoints + (size_t)P.nckpt * sN + n0;
-    s.segx = (float4*)A.segment + n0;
-    s.segv = (float4*)A.segment + (size_t)P.ckpt * sN + n0;
+    // tapes: slot j * iters + it for substep j of the segment and iteration it
+    s.tslot = 4 * sT;
+    s.eslot = 2 * sE;
+    s.lslot = 2 * sT + sE;
     s.ttape = (float4*)A.tet_tape + 4 * (size_t)s.I.tet_off;
     s.etape = (float4*)A.edge_tape + 2 * (size_t)s.I.edge_off;
+    s.ltape_t = A.lambda_tape + 2 * (size_t)s.I.tet_off;
+    s.ltape_e = A.lambda_tape + 2 * sT + s.I.edge_off;
     s.ed = B.edge_desc + s.I.edge_off;
     s.td = B.tet_desc + s.I.tet_off;
     s.comp = A.compliance + s.I.tet_off;
+    s.alpha = A.alpha + 2 * (size_t)s.I.tet_off;
     s.abar = A.abar + 2 * (size_t)s.I.tet_off;
+    s.lamb_t = A.lambar + 2 * (size_t)s.I.tet_off;
+    s.lamb_e = A.lambar + 2 * sT + s.I.edge_off;
+    s.lam_t = B.lambda_tet + 2 * (size_t)s.I.tet_off;
+    s.lam_e = B.lambda_edge + s.I.edge_off;
     s.grad = A.grad + s.I.tet_off;
     s.ecol = B.colour_table + s.I.colour_off;
     s.tcol = s.ecol + s.I.n_edge_colours + 1;
-    (void)N;
     return s;
-}
-
-__device__ __forceinline__ void tet_alphas(float c, float invH2, float& ad, float& ah) {
-    float kd = (float)(1.0 / (double)c), kh = (float)(1.0 / ((double)c / 10.0));
-    ad = invH2 / kd;
-    ah = invH2 / kh;
 }
 
 __device__ __forceinline__ void tet_ids(const Inst& s, uint32_t d, int id[4], int& type) {
@@ -96,11 +102,27 @@ __device__ __forceinline__ void tet_ids(const Inst& s, uint32_t d, int id[4], in
     for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), s.I.nx, s.I.nxy);
 }
 
+__device__ __forceinline__ int edge_partner(const Inst& s, uint32_t d, int& axis) {
+    axis = (int)((d >> 1) & 3u);
+    return (int)(d >> 4) + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
+}
+
+// alpha~_D = (1/h^2) / (1/c) and alpha~_H = (1/h^2) / (10/c) of every tet, once per rollout.
+__device__ void compute_alphas(const Inst& s, const AdjStep& P) {
+    for (int k = threadIdx.x; k < s.I.n_tets; k += blockDim.x) {
+        float c = s.comp[k];
+        float kd = (float)(1.0 / (double)c), kh = (float)(1.0 / ((double)c / 10.0));
+        s.alpha[k] = P.invH2 / kd;
+        s.alpha[s.I.n_tets + k] = P.invH2 / kh;
+    }
+    __syncthreads();
+}
+
 // One forward substep of the fit model on node positions S (LDS-staged or the instance's global slice); with TAPE,
-// records the input positions of every projection.
+// records the inputs of every projection of every iteration in the tape slots of substep `slot`.
 template <bool TAPE, class Store>
-__device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P) {
-    const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes;
+__device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P, int slot) {
+    const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes, T = s.I.n_tets;
     for (int i = tid; i < N; i += bs) {
         float4 p = S.ld(i);
         s.prev[i] = p;
@@ -116,35 +138,53 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P)
     }
     __syncthreads();
     const float at = s.I.edge_ks > 0.f ? 1.f / (s.I.edge_ks * P.h * P.h) : 0.f;
-    for (int c = 0; c < s.I.n_edge_colours; c++) {
-        for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
-            uint32_t d = s.ed[k];
-            if (!(d & 1u)) continue;
-            int axis = (int)((d >> 1) & 3u);
-            int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
-            if (TAPE) {
-                s.etape[2 * k] = S.ld(i);
-                s.etape[2 * k + 1] = S.ld(j);
+    const bool write = P.iters > 1;
+    for (int it = 0; it < P.iters; it++) {
+        const bool read = it > 0;
+        const size_t sl = (size_t)slot * P.iters + it;
+        float4* et = s.etape + sl * s.eslot;
+        float4* tt = s.ttape + sl * s.tslot;
+        float* lte = s.ltape_e + sl * s.lslot;
+        float* ltt = s.ltape_t + sl * s.lslot;
+        for (int c = 0; c < s.I.n_edge_colours; c++) {
+            for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
+                uint32_t d = s.ed[k];
+                if (!(d & 1u)) continue;
+                int axis, i = (int)(d >> 4), j = edge_partner(s, d, axis);
+                float lam = read ? s.lam_e[k] : 0.f, out;
+                if (TAPE) {
+                    et[2 * k] = S.ld(i);
+                    et[2 * k + 1] = S.ld(j);
+                    if (write) lte[k] = lam;
+                }
+                if (edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, lam, out) && write)
+                    s.lam_e[k] = out;
             }
-            float out;
-            edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, 0.f, out);
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    for (int c = 0; c < s.I.n_tet_colours; c++) {
-        for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
-            uint32_t d = s.td[k];
-            if (!(d & 1u)) continue;
-            int id[4], type;
-            tet_ids(s, d, id, type);
-            if (TAPE)
-                for (int q = 0; q < 4; q++) s.ttape[4 * k + q] = S.ld(id[q]);
-            float ad, ah;
-            tet_alphas(s.comp[k], P.invH2, ad, ah);
-            float lamD = 0.f, lamH = 0.f;
-            tet_project<1, false>(S, s.prev, id, s.rec->inv_rest[type], ad, ah, lamD, lamH, 1.f, 0.f, 0.f);
+        for (int c = 0; c < s.I.n_tet_colours; c++) {
+            for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
+                uint32_t d = s.td[k];
+                if (!(d & 1u)) continue;
+                int id[4], type;
+                tet_ids(s, d, id, type);
+                float lamD = read ? s.lam_t[k] : 0.f, lamH = read ? s.lam_t[T + k] : 0.f;
+                if (TAPE) {
+                    for (int q = 0; q < 4; q++) tt[4 * k + q] = S.ld(id[q]);
+                    if (write) {
+                        ltt[k] = lamD;
+                        ltt[T + k] = lamH;
+                    }
+                }
+                if (tet_project<1, false>(S, s.prev, id, s.rec->inv_rest[type], s.alpha[k], s.alpha[T + k], lamD,
+                                          lamH, 1.f, 0.f, 0.f) &&
+                    write) {
+                    s.lam_t[k] = lamD;
+                    s.lam_t[T + k] = lamH;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     for (int i = tid; i < N; i += bs) {
         float4 p = S.ld(i);
@@ -155,9 +195,10 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P)
     __syncthreads();
 }
 
-// Reverse of the tet projection (mode 1, lambda = 0, no damping, omega = 1) at input positions p.
-__device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, float ad, float ah, const f3 ob[4],
-                            f3 pb[4], float& adb, float& ahb) {
+// Reverse of the tet projection (mode 1, no damping, omega = 1) at input positions p and multipliers lamD / lamH.
+// lDb / lHb hold the adjoints of the output multipliers on entry and of the input multipliers on return.
+__device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, float ad, float ah, float lamD,
+                            float lamH, const f3 ob[4], f3 pb[4], float& adb, float& ahb, float& lDb, float& lHb) {
     f3 br[3] = {mk3(Bm[0], Bm[1], Bm[2]), mk3(Bm[3], Bm[4], Bm[5]), mk3(Bm[6], Bm[7], Bm[8])};
     f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
     f3 F[3];
@@ -199,13 +240,13 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, fl
         Rk[k] = gh[k] - gd[k] * gamma;
         schur += w[k] * dot3(Rk[k], Rk[k]);
     }
-    float rd = -CD, rh = -CH;
+    float rd = -CD - ad * lamD, rh = -CH - ah * lamH;
     for (int k = 0; k < 4; k++) pb[k] = ob[k];
     if (!(A > 0.f) || !(schur > 0.f)) return;
     float dlH = (rh - gamma * rd) / schur, dlD = (rd - dh * dlH) / A;
     if (!isfinite(dlD) || !isfinite(dlH)) return;
     f3 gdb[4], ghb[4];
-    float dlDb = 0.f, dlHb = 0.f;
+    float dlDb = lDb, dlHb = lHb;
     for (int k = 0; k < 4; k++) {
         f3 u = ob[k] * w[k];
         gdb[k] = u * dlD;
@@ -218,6 +259,10 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, fl
     float rhb = dlHb / schur, gammab = -dlHb * rd / schur, schurb = -dlHb * dlH / schur;
     rdb += -dlHb * gamma / schur;
     float CDb = -rdb, CHb = -rhb;
+    adb += -lamD * rdb;
+    ahb += -lamH * rhb;
+    lDb += -ad * rdb;
+    lHb += -ah * rhb;
     ahb += schurb;
     adb += schurb * gamma * gamma;
     gammab += schurb * 2.f * ad * gamma;
@@ -294,9 +339,10 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, fl
     }
 }
 
-// Reverse of the edge projection (lambda = 0, no damping, omega = 1) at inputs xi, xj.
-__device__ __forceinline__ void edge_reverse(f3 xi, f3 xj, float wi, float wj, float L0, float at, f3 oib, f3 ojb,
-                                             f3& xib, f3& xjb) {
+// Reverse of the edge projection (no damping, omega = 1) at inputs xi, xj and multiplier lam; lb holds the adjoint
+// of the output multiplier on entry and of the input multiplier on return.
+__device__ __forceinline__ void edge_reverse(f3 xi, f3 xj, float wi, float wj, float L0, float at, float lam, f3 oib,
+                                             f3 ojb, f3& xib, f3& xjb, float& lb) {
     xib = oib;
     xjb = ojb;
     float wsum = wi + wj;
@@ -307,21 +353,22 @@ __device__ __forceinline__ void edge_reverse(f3 xi, f3 xj, float wi, float wj, f
     f3 n = d * (1.f / L);
     float C = L - L0;
     float kap = 1.f / (wsum + at);
-    float dl = -C * kap;
+    float dl = (-C - at * lam) * kap;
     f3 m = oib * wi - ojb * wj;
     f3 nb = m * dl;
-    float dlb = dot3(m, n);
+    float dlb = dot3(m, n) + lb;
     float Lb = -dlb * kap;
+    lb += -at * kap * dlb;
     f3 db = (nb - n * dot3(nb, n)) * (1.f / L) + n * Lb;
     xib = xib + db;
     xjb = xjb - db;
 }
 
-// Reverse of one substep. S holds the positions at the end of the substep (for the inverse masses), X the position
-// adjoints; the velocity adjoints and the adjoint of prev are in the instance's global slices.
+// Reverse of the substep whose tapes are in the slots of `slot`. S holds the node positions (for the inverse masses),
+// X the position adjoints; the velocity adjoints and the adjoint of prev are in the instance's global slices.
 template <class Store, class XStore>
-__device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, const AdjStep& P) {
-    const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes;
+__device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, const AdjStep& P, int slot) {
+    const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes, T = s.I.n_tets;
     // velocity: v' = (x_end - prev) / h * velScale for w > 0
     for (int i = tid; i < N; i += bs) {
         float w = S.ld(i).w;
@@ -335,45 +382,60 @@ __device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, 
             s.pb[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
+    // the multipliers of the last iteration reach nothing after the substep
+    for (int k = tid; k < 2 * T; k += bs) s.lamb_t[k] = 0.f;
+    for (int k = tid; k < s.I.n_edges; k += bs) s.lamb_e[k] = 0.f;
     __syncthreads();
-    for (int c = s.I.n_tet_colours - 1; c >= 0; c--) {
-        for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
-            uint32_t d = s.td[k];
-            if (!(d & 1u)) continue;
-            int id[4], type;
-            tet_ids(s, d, id, type);
-            f3 p[4], ob[4], pb[4];
-            float w[4];
-            for (int q = 0; q < 4; q++) {
-                float4 t = s.ttape[4 * k + q];
-                p[q] = xyz4(t);
-                w[q] = t.w;
-                ob[q] = xyz4(X.ld(id[q]));
-            }
-            float ad, ah, adb = s.abar[k], ahb = s.abar[s.I.n_tets + k];
-            tet_alphas(s.comp[k], P.invH2, ad, ah);
-            tet_reverse(p, w, s.rec->inv_rest[type], ad, ah, ob, pb, adb, ahb);
-            for (int q = 0; q < 4; q++) X.st(id[q], make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f));
-            s.abar[k] = adb;
-            s.abar[s.I.n_tets + k] = ahb;
-        }
-        __syncthreads();
-    }
     const float at = s.I.edge_ks > 0.f ? 1.f / (s.I.edge_ks * P.h * P.h) : 0.f;
-    for (int c = s.I.n_edge_colours - 1; c >= 0; c--) {
-        for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
-            uint32_t d = s.ed[k];
-            if (!(d & 1u)) continue;
-            int axis = (int)((d >> 1) & 3u);
-            int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? s.I.nx : s.I.nxy));
-            float4 ti = s.etape[2 * k], tj = s.etape[2 * k + 1];
-            f3 xib, xjb;
-            edge_reverse(xyz4(ti), xyz4(tj), ti.w, tj.w, s.rec->rest_len[axis], at, xyz4(X.ld(i)), xyz4(X.ld(j)),
-                         xib, xjb);
-            X.st(i, make_float4(xib.x, xib.y, xib.z, 0.f));
-            X.st(j, make_float4(xjb.x, xjb.y, xjb.z, 0.f));
+    const bool tapedLam = P.iters > 1;
+    for (int it = P.iters - 1; it >= 0; it--) {
+        const size_t sl = (size_t)slot * P.iters + it;
+        const float4* et = s.etape + sl * s.eslot;
+        const float4* tt = s.ttape + sl * s.tslot;
+        const float* lte = s.ltape_e + sl * s.lslot;
+        const float* ltt = s.ltape_t + sl * s.lslot;
+        for (int c = s.I.n_tet_colours - 1; c >= 0; c--) {
+            for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
+                uint32_t d = s.td[k];
+                if (!(d & 1u)) continue;
+                int id[4], type;
+                tet_ids(s, d, id, type);
+                f3 p[4], ob[4], pb[4];
+                float w[4];
+                for (int q = 0; q < 4; q++) {
+                    float4 t = tt[4 * k + q];
+                    p[q] = xyz4(t);
+                    w[q] = t.w;
+                    ob[q] = xyz4(X.ld(id[q]));
+                }
+                float lamD = tapedLam ? ltt[k] : 0.f, lamH = tapedLam ? ltt[T + k] : 0.f;
+                float adb = s.abar[k], ahb = s.abar[T + k], lDb = s.lamb_t[k], lHb = s.lamb_t[T + k];
+                tet_reverse(p, w, s.rec->inv_rest[type], s.alpha[k], s.alpha[T + k], lamD, lamH, ob, pb, adb, ahb,
+                            lDb, lHb);
+                for (int q = 0; q < 4; q++) X.st(id[q], make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f));
+                s.abar[k] = adb;
+                s.abar[T + k] = ahb;
+                s.lamb_t[k] = lDb;
+                s.lamb_t[T + k] = lHb;
+            }
+            __syncthreads();
         }
-        __syncthreads();
+        for (int c = s.I.n_edge_colours - 1; c >= 0; c--) {
+            for (int k = s.ecol[c] + tid; k < s.ecol[c + 1]; k += bs) {
+                uint32_t d = s.ed[k];
+                if (!(d & 1u)) continue;
+                int axis, i = (int)(d >> 4), j = edge_partner(s, d, axis);
+                float4 ti = et[2 * k], tj = et[2 * k + 1];
+                float lam = tapedLam ? lte[k] : 0.f, lb = s.lamb_e[k];
+                f3 xib, xjb;
+                edge_reverse(xyz4(ti), xyz4(tj), ti.w, tj.w, s.rec->rest_len[axis], at, lam, xyz4(X.ld(i)),
+                             xyz4(X.ld(j)), xib, xjb, lb);
+                X.st(i, make_float4(xib.x, xib.y, xib.z, 0.f));
+                X.st(j, make_float4(xjb.x, xjb.y, xjb.z, 0.f));
+                s.lamb_e[k] = lb;
+            }
+            __syncthreads();
+        }
     }
     // predict: xp = x + h (v + h a) for w > 0, prev = x
     for (int i = tid; i < N; i += bs) {
@@ -408,24 +470,17 @@ template <class Store>
 __device__ void forward_run(const Inst& s, const Store& S, const AdjStep& P, size_t sN) {
     for (int n = 0; n < P.nsub; n++) {
         if (n % P.ckpt == 0) save_state(s, S, s.ckx + (size_t)(n / P.ckpt) * sN, s.ckv + (size_t)(n / P.ckpt) * sN);
-        substep_forward<false>(s, S, P);
+        substep_forward<false>(s, S, P, 0);
     }
 }
 
 template <class Store, class XStore>
 __device__ void backward_run(const Inst& s, const Store& S, const XStore& X, const AdjStep& P, size_t sN) {
     for (int seg = (P.nsub - 1) / P.ckpt; seg >= 0; seg--) {
-        const int n0 = seg * P.ckpt, n1 = min(P.nsub, n0 + P.ckpt);
+        const int n0 = seg * P.ckpt, len = min(P.nsub, n0 + P.ckpt) - n0;
         load_state(s, S, s.ckx + (size_t)seg * sN, s.ckv + (size_t)seg * sN);
-        for (int n = n0; n < n1; n++) {
-            save_state(s, S, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
-            if (n + 1 < n1) substep_forward<false>(s, S, P);
-        }
-        for (int n = n1 - 1; n >= n0; n--) {
-            load_state(s, S, s.segx + (size_t)(n - n0) * sN, s.segv + (size_t)(n - n0) * sN);
-            substep_forward<true>(s, S, P);
-            substep_reverse(s, S, X, P);
-        }
+        for (int j = 0; j < len; j++) substep_forward<true>(s, S, P, j);
+        for (int j = len - 1; j >= 0; j--) substep_reverse(s, S, X, P, j);
     }
 }
 
@@ -438,6 +493,7 @@ __global__ void __launch_bounds__(1024) k_adjoint_forward(GtsAdjointView A, AdjS
     for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
         Inst s = bind(A, P, B.work_items[w]);
         const int N = s.I.n_nodes;
+        compute_alphas(s, P);
         if (N <= P.lds_nodes) {
             for (int i = threadIdx.x; i < N; i += blockDim.x) lds[i] = s.pos[i];
             __syncthreads();
@@ -477,9 +533,9 @@ __global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_ba
 
 int plan_step(const GtsStepParams* p, int nsub, int ckpt, AdjStep& P) {
     if (!p || nsub <= 0 || ckpt <= 0) return GTS_ERR_BAD_ARG;
-    if (p->tet_mode != GTS_TET_NEOHOOKEAN || p->iterations != 1 || p->strain_damping != 0.f ||
+    if (p->tet_mode != GTS_TET_NEOHOOKEAN || p->iterations < 1 || p->strain_damping != 0.f ||
         p->bulk_damping != 0.f || p->beta_s != 0.f || p->omega != 1.f || p->damping != 0.f || p->rayleigh != 0.f)
-        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped one-iteration Neo-Hookean substep
+        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped Neo-Hookean substep
     int N = p->substeps > 0 ? p->substeps : 1;
     P.h = p->dt / (float)N;
     if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
@@ -489,13 +545,14 @@ int plan_step(const GtsStepParams* p, int nsub, int ckpt, AdjStep& P) {
     P.nsub = nsub;
     P.ckpt = ckpt;
     P.nckpt = (nsub + ckpt - 1) / ckpt;
+    P.iters = p->iterations;
     P.lds_nodes = 0;
     return GTS_OK;
 }
 
 int check_view(const GtsAdjointView* A) {
     if (!A || A->batch.n_workgroups <= 0 || A->batch.block_size <= 0 || A->batch.block_size > 1024 ||
-        A->total_nodes <= 0 || A->batch.lds_nodes < 0)
+        A->total_nodes <= 0 || A->total_tets < 0 || A->total_edges < 0 || A->batch.lds_nodes < 0)
         return GTS_ERR_BAD_ARG;
     return GTS_OK;
 }

@@ -35,7 +35,16 @@ class Workload:
     blade: object = None            # callable step -> (S, E) segment, or None
     blade_thickness: float = 0.0
     tool: object = None             # callable step -> tool dict, or None
+    gravity_ramp_s: float = 0.0     # gravity ramped linearly over this time, as NativeTissue.step does
     info: dict = field(default_factory=dict)
+
+    def gravity_at(self, step):
+        """Gravity of step `step` (0-based): the reference's ramp min(1, (step + 1) dt / 0.5 s)
+        (`native_backend.py:379-380`) when gravity_ramp_s is set, else constant."""
+        if self.gravity_ramp_s <= 0:
+            return self.gravity
+        ramp = min(1.0, (step + 1) * self.dt / self.gravity_ramp_s)
+        return tuple(g * ramp for g in self.gravity)
 
     def stiffness(self):
         """(k_dev, k_hyd) per tet in generation order: 1/c and 1/(c/10)."""
@@ -96,7 +105,9 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
     """Kidney-scale single instance: an ellipsoid of semi-axes 6 x 3 x 2.5 cm rasterised at the spacing that gives
     about 250k tets (1.72 mm; a 0.5 mm lattice would give 9M); per-tet compliance log-normal (median 1e-7,
     sigma 0.5); a capsule grasper of radius 2 mm pressing 3 mm into the top; a blade on a seeded cubic Bezier over
-    the top surface cutting for 300 steps; 20 substeps of 2 iterations."""
+    the top surface cutting for 300 steps; 20 substeps of 2 iterations. Gravity ramps in over 0.5 s as in
+    NativeTissue.step: applied at once, the 30-cell-tall lattice falls onto its pinned bed faster than two
+    Gauss-Seidel sweeps per substep can carry the support up, and the bottom tets invert (in FP64 as in FP32)."""
     rng = np.random.default_rng(seed)
     shape = Ellipsoid((6.0, 3.0, 2.5))
     h = spacing if spacing is not None else 0.1716
@@ -127,7 +138,7 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
         return (np.array(b + (0, 0.3, 0), np.float32), np.array(b - (0, 0.6, 0), np.float32))
 
     return Workload("kidney-250k", grid, params, 1 / 30, steps, compliance, jittered(grid, rng, 0.0),
-                    blade=blade, blade_thickness=0.02, tool=tool,
+                    blade=blade, blade_thickness=0.02, tool=tool, gravity_ramp_s=0.5,
                     info=dict(spacing=h, half_extents=(6.0, 3.0, 2.5)))
 
 
@@ -143,7 +154,7 @@ class AdjointInstance:
     compliance_init: np.ndarray
 
 
-def adjoint_instance(nodes=None, tets=20_000, seed=0, spacing=1.0, substeps=50, edge_compliance=1e-7):
+def adjoint_instance(nodes=None, tets=20_000, seed=0, spacing=1.0, substeps=50, edge_compliance=1e-7, iterations=1):
     """One block of the compliance fit: hidden per-tet compliance log-normal (median 1e-7, sigma 0.7), point loads
     N(0, 0.5) N per component on 5% of the top nodes, loss on the free surface nodes, learnable compliance initialised
     at the median. The rollout's 50 substeps are one 1/60 s frame (substep 1/3000 s); at ten substeps per frame the
@@ -153,7 +164,8 @@ def adjoint_instance(nodes=None, tets=20_000, seed=0, spacing=1.0, substeps=50, 
     grid = block_grid(nodes or block_nodes_for_tets(tets), spacing)
     T = tet_count(grid)
     hidden = np.exp(rng.normal(np.log(1e-7), 0.7, T))
-    params = SolverParams(substeps=substeps, iterations=1, tet_mode=TET_NEOHOOKEAN, edge_ks=1.0 / edge_compliance)
+    params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN,
+                          edge_ks=1.0 / edge_compliance)
     ijk = grid.positions / grid.cell_size
     top = np.flatnonzero(np.isclose(ijk[:, 1], ijk[:, 1].max()))
     loaded = rng.choice(top, max(1, int(round(0.05 * len(top)))), replace=False)

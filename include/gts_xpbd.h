@@ -224,31 +224,38 @@ int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream
 
 /* ---- adjoint of the substep for per-tet compliance gradients ----
  * No reference counterpart (the reference solver has no gradient; its differentiable PBD path is a stub,
- * genesis/engine/solvers/pbd_solver.py:784-815). The forward is the undamped one-iteration Neo-Hookean substep with
- * a per-node force field and no contact or cutting; it runs on the pooled instances of a GtsBatchView (one workgroup
- * per instance, LPT-packed). */
+ * genesis/engine/solvers/pbd_solver.py:784-815). The forward is the undamped Neo-Hookean substep with p->iterations
+ * Gauss-Seidel iterations, a per-node force field and no contact or cutting; it runs on the pooled instances of a
+ * GtsBatchView (one workgroup per instance, LPT-packed), whose lambda_edge / lambda_tet hold the multipliers within a
+ * substep. Tape slots: ckpt * iterations. */
 typedef struct GtsAdjointView {
-    GtsBatchView batch;           /* pooled instances and launch shape; k_dev, k_hyd and the lambdas are not used */
-    long long total_nodes;        /* sum of n_nodes: row stride of the checkpoint and segment buffers */
+    GtsBatchView batch;           /* pooled instances and launch shape; k_dev and k_hyd are not used */
+    long long total_nodes;        /* sum of n_nodes: row stride of the checkpoints */
+    long long total_tets;         /* sum of n_tets: stride of the tet tape slots */
+    long long total_edges;        /* sum of n_edges: stride of the edge tape slots */
     const float* force;           /* [sum N] float4 rows: node force (x, y, z, unused) */
     const float* compliance;      /* [sum T] per-tet deviatoric compliance (volume compliance / 10), colour-major */
+    float* alpha;                 /* [2 sum T] alpha~_D then alpha~_H per instance slice, written by the forward */
     float* checkpoints;           /* [2 * n_checkpoints * sum N] float4: positions, then velocities, every ckpt-th */
-    float* segment;               /* [2 * ckpt * sum N] float4: start states of the segment being reversed */
-    float* tet_tape;              /* [4 * sum T] float4: inputs of each tet projection of the substep being reversed */
-    float* edge_tape;             /* [2 * sum E] float4: inputs of each edge projection */
+    float* tet_tape;              /* [slots * 4 * sum T] float4: positions (inverse mass in .w) entering each tet
+                                   * projection */
+    float* edge_tape;             /* [slots * 2 * sum E] float4: positions entering each edge projection */
+    float* lambda_tape;           /* [slots * (2 sum T + sum E)]: multipliers entering each projection (tets, then
+                                   * edges); used when iterations > 1 */
     float* xbar;                  /* [sum N] float4: in dL/dx after the last substep, out dL/dx at the start */
     float* vbar;                  /* [sum N] float4: in dL/dv after the last substep, out dL/dv at the start */
     float* prevbar;               /* [sum N] float4 work */
     float* abar;                  /* [2 sum T] work: dL/d alpha~_D then dL/d alpha~_H per instance slice */
+    float* lambar;                /* [2 sum T + sum E] work: multiplier adjoints (tets, then edges) */
     float* grad;                  /* [sum T] out: dL/d compliance, colour-major */
 } GtsAdjointView;
 
 /* Device: `nsub` substeps of every instance from the state in batch.pos / batch.vel, storing (x, v) before every
  * ckpt-th substep; the end state is left in batch.pos / batch.vel. */
 int gts_adjoint_forward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
-/* Device: reverse pass over the checkpoints of the last gts_adjoint_forward with the same nsub and ckpt; consumes
- * xbar / vbar and writes grad. Leaves batch.pos / batch.vel at the state after the first substep. Runs with
- * min(batch.block_size, GTS_ADJOINT_BACKWARD_MAX_THREADS) threads per workgroup. */
+/* Device: reverse pass over the checkpoints of the last gts_adjoint_forward with the same compliance, nsub, ckpt and
+ * iterations; consumes xbar / vbar and writes grad. Leaves batch.pos / batch.vel at the end of the first segment.
+ * Runs with min(batch.block_size, GTS_ADJOINT_BACKWARD_MAX_THREADS) threads per workgroup. */
 #define GTS_ADJOINT_BACKWARD_MAX_THREADS 512
 int gts_adjoint_backward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
 

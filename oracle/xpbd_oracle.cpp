@@ -1315,24 +1315,28 @@ extern "C" double oracle_shape_distance_f64(const double* center, const double* 
 // ---------------------------------------------------------------------------------------------------------------
 // Adjoint of the substep for per-tet compliance gradients (no reference counterpart; pinned by finite differences).
 //
-// Forward: `nsub` substeps of the legacy branch with one iteration, no damping, no contact, omega = 1, coupled
-// Neo-Hookean tets (mode 1), node forces `force` (gravity * mass plus point loads) in place of the gravity field.
-// Per-tet compliance c gives alpha~_D = (1/h^2)/(1/c) and alpha~_H = (1/h^2)/(10/c).
+// Forward: `nsub` substeps of the legacy branch with `iterations` Gauss-Seidel iterations (multipliers reset every
+// substep, read from the second iteration on, as solveEdge / solveTet do), no damping, no contact, omega = 1,
+// coupled Neo-Hookean tets (mode 1), node forces `force` (gravity * mass plus point loads) in place of the gravity
+// field. Per-tet compliance c gives alpha~_D = (1/h^2)/(1/c) and alpha~_H = (1/h^2)/(10/c).
 // Loss: sum over masked nodes of |x - target|^2 after the last substep. Backward: every substep in reverse, each
-// re-run forward from its stored start state with a tape of the inputs of every projection, then reversed colour
-// by colour; gradients are not taken through the max(J, .05) and max(q, 1e-12) clamps (zero subgradient).
+// re-run forward from its stored start state with a tape of the inputs (positions and multipliers) of every
+// projection, then reversed iteration by iteration and colour by colour; the multipliers carry adjoints between the
+// iterations of a substep. Gradients are not taken through the max(J, .05) and max(q, 1e-12) clamps (zero
+// subgradient).
 // ---------------------------------------------------------------------------------------------------------------
 namespace {
 
-template <class R> struct TetTape { V3<R> p[4]; };
+template <class R> struct TetTape { V3<R> p[4]; R lamD, lamH; };
+template <class R> struct EdgeTape { V3<R> xi, xj; R lam; };
 
-// Reverse of tetProject (mode 1, lambda = 0, no damping, omega = 1) at input positions p: given the adjoint of the
-// outputs (ob), accumulates the adjoint of the inputs (pb) and of alpha~_D / alpha~_H.
+// Reverse of tetProject (mode 1, no damping, omega = 1) at input positions p and multipliers lamD / lamH: given the
+// adjoints of the outputs (ob for the positions, lDb / lHb for the multipliers, replaced by the adjoints of the input
+// multipliers), accumulates the adjoint of the input positions (pb) and of alpha~_D / alpha~_H.
 template <class R>
-void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, R ah, const V3<R> ob[4], V3<R> pb[4],
-                       R& adb, R& ahb) {
+void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, R ah, R lamD, R lamH,
+                       const V3<R> ob[4], V3<R> pb[4], R& adb, R& ahb, R& lDb, R& lHb) {
     V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
-    V3<R> e[3] = {e1, e2, e3};
     V3<R> F[3];
     for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
     V3<R> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
@@ -1372,14 +1376,14 @@ void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, 
         Rk[k] = gh[k] - gd[k] * gamma;
         schur += w[k] * dot3(Rk[k], Rk[k]);
     }
-    R rd = -CD, rh = -CH;
+    R rd = -CD - ad * lamD, rh = -CH - ah * lamH;
     for (int k = 0; k < 4; k++) pb[k] = ob[k];
     if (!(A > 0) || !(schur > 0)) return;
     R dlH = (rh - gamma * rd) / schur, dlD = (rd - dh * dlH) / A;
     if (!std::isfinite(dlD) || !std::isfinite(dlH)) return;
-    // out_k = p_k + (gd_k dlD + gh_k dlH) w_k
+    // out_k = p_k + (gd_k dlD + gh_k dlH) w_k; lamD' = lamD + dlD; lamH' = lamH + dlH
     V3<R> gdb[4], ghb[4];
-    R dlDb = 0, dlHb = 0;
+    R dlDb = lDb, dlHb = lHb;
     for (int k = 0; k < 4; k++) {
         V3<R> u = ob[k] * w[k];
         gdb[k] = u * dlD;
@@ -1393,7 +1397,12 @@ void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, 
     // dlH = (rh - gamma rd) / schur
     R rhb = dlHb / schur, gammab = -dlHb * rd / schur, schurb = -dlHb * dlH / schur;
     rdb += -dlHb * gamma / schur;
+    // rd = -CD - ad lamD; rh = -CH - ah lamH
     R CDb = -rdb, CHb = -rhb;
+    adb += -lamD * rdb;
+    ahb += -lamH * rhb;
+    lDb += -ad * rdb;
+    lHb += -ah * rhb;
     // schur = ah + ad gamma^2 + sum w |gh - gamma gd|^2
     ahb += schurb;
     adb += schurb * gamma * gamma;
@@ -1473,12 +1482,13 @@ void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, 
         pb[k + 1] = pb[k + 1] + eb;
         pb[0] = pb[0] - eb;
     }
-    (void)e;
 }
 
-// Reverse of solveEdge (lambda = 0, no damping, omega = 1) at inputs xi, xj.
+// Reverse of solveEdge (no damping, omega = 1) at inputs xi, xj and multiplier lam; lb holds the adjoint of the
+// output multiplier on entry and of the input multiplier on return.
 template <class R>
-void edgeProjectReverse(V3<R> xi, V3<R> xj, R wi, R wj, R L0, R at, V3<R> oib, V3<R> ojb, V3<R>& xib, V3<R>& xjb) {
+void edgeProjectReverse(V3<R> xi, V3<R> xj, R wi, R wj, R L0, R at, R lam, V3<R> oib, V3<R> ojb, V3<R>& xib,
+                        V3<R>& xjb, R& lb) {
     xib = oib;
     xjb = ojb;
     R wsum = wi + wj;
@@ -1489,25 +1499,22 @@ void edgeProjectReverse(V3<R> xi, V3<R> xj, R wi, R wj, R L0, R at, V3<R> oib, V
     V3<R> n = d * ((R)1 / L);
     R C = L - L0;
     R kap = (R)1 / (wsum + at);
-    R dl = -C * kap;
+    R dl = (-C - at * lam) * kap;
     V3<R> m = oib * wi - ojb * wj;
     V3<R> nb = m * dl;
-    R dlb = dot3(m, n);
+    R dlb = dot3(m, n) + lb;
     R Lb = -dlb * kap;
+    lb += -at * kap * dlb;
     V3<R> db = (nb - n * dot3(nb, n)) * ((R)1 / L) + n * Lb;
     xib = xib + db;
     xjb = xjb - db;
 }
 
-template <class R> struct AdjointRun {
-    std::vector<V3<R>> force;
-    std::vector<R> c;  // per tet (generation order)
-};
-
-// One forward substep (mode 1, iters 1) with external force field; optionally records the projection tape.
+// One forward substep (mode 1, `iterations` iterations) with an external force field; optionally records the tape
+// of iteration `it` at tt[it * T + t] / et[it * E + e].
 template <class R>
-void adjointSubstep(Tissue<R>& s, const std::vector<V3<R>>& force, R h, std::vector<TetTape<R>>* tt,
-                    std::vector<V3<R>>* et) {
+void adjointSubstep(Tissue<R>& s, const std::vector<V3<R>>& force, R h, int iterations, std::vector<TetTape<R>>* tt,
+                    std::vector<EdgeTape<R>>* et) {
     for (int p = 0; p < s.N; p++) {
         s.prev[p] = s.pos[p];
         if (s.invMass[p] > 0) {
@@ -1517,27 +1524,42 @@ void adjointSubstep(Tissue<R>& s, const std::vector<V3<R>>& force, R h, std::vec
             s.pos[p] = s.pos[p] + v * h;
         }
     }
-    for (size_t c = 0; c < s.ecolorCnt.size(); c++)
-        for (int k = 0; k < s.ecolorCnt[c]; k++) {
-            int e = s.ecolorOff[c] + k;
-            if (et) { (*et)[2 * e] = s.pos[s.ei[e]]; (*et)[2 * e + 1] = s.pos[s.ej[e]]; }
-            solveEdge(s, e, s.alphaTilde, false, (R)0, false);
-        }
-    for (size_t c = 0; c < s.tcolorCnt.size(); c++)
-        for (int k = 0; k < s.tcolorCnt[c]; k++) {
-            int t = s.tcolorOrder[s.tcolorOff[c] + k];
-            if (tt)
-                for (int j = 0; j < 4; j++) (*tt)[t].p[j] = s.pos[s.tids[4 * t + j]];
-            solveTet(s, t, (R)0, (R)0, false);
-        }
+    std::fill(s.elambda.begin(), s.elambda.end(), (R)0);
+    std::fill(s.lamDev.begin(), s.lamDev.end(), (R)0);
+    std::fill(s.lamHyd.begin(), s.lamHyd.end(), (R)0);
+    for (int it = 0; it < iterations; it++) {
+        const bool useLambda = it > 0;
+        for (size_t c = 0; c < s.ecolorCnt.size(); c++)
+            for (int k = 0; k < s.ecolorCnt[c]; k++) {
+                int e = s.ecolorOff[c] + k;
+                if (et) {
+                    EdgeTape<R>& r = (*et)[(size_t)it * s.E + e];
+                    r.xi = s.pos[s.ei[e]];
+                    r.xj = s.pos[s.ej[e]];
+                    r.lam = useLambda ? s.elambda[e] : (R)0;
+                }
+                solveEdge(s, e, s.alphaTilde, false, (R)0, useLambda);
+            }
+        for (size_t c = 0; c < s.tcolorCnt.size(); c++)
+            for (int k = 0; k < s.tcolorCnt[c]; k++) {
+                int t = s.tcolorOrder[s.tcolorOff[c] + k];
+                if (tt) {
+                    TetTape<R>& r = (*tt)[(size_t)it * s.T + t];
+                    for (int j = 0; j < 4; j++) r.p[j] = s.pos[s.tids[4 * t + j]];
+                    r.lamD = useLambda ? s.lamDev[t] : (R)0;
+                    r.lamH = useLambda ? s.lamHyd[t] : (R)0;
+                }
+                solveTet(s, t, (R)0, (R)0, useLambda);
+            }
+    }
     R invH = (R)1 / h;
     for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? (s.pos[p] - s.prev[p]) * invH : mk<R>(0, 0, 0);
     for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? s.vel[p] * s.velScale : mk<R>(0, 0, 0);
 }
 
 template <class R>
-double adjointRun(Tissue<R>& s, R dt, int nsub, const float* force, const float* compliance, const float* mask,
-                  const float* target, R* grad) {
+double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* force, const float* compliance,
+                  const float* mask, const float* target, R* grad) {
     // frame constants as xpbd_begin_frame with substeps = d.substeps (h = dt / substeps)
     int Nsub = s.d.substeps > 0 ? s.d.substeps : 1;
     R h = dt / (R)Nsub;
@@ -1560,7 +1582,7 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, const float* force, const float*
     for (int n = 0; n < nsub; n++) {
         std::copy(s.pos.begin(), s.pos.end(), xs.begin() + (size_t)n * s.N);
         std::copy(s.vel.begin(), s.vel.end(), vs.begin() + (size_t)n * s.N);
-        adjointSubstep<R>(s, F, h, nullptr, nullptr);
+        adjointSubstep<R>(s, F, h, iterations, nullptr, nullptr);
     }
     std::copy(s.pos.begin(), s.pos.end(), xs.begin() + (size_t)nsub * s.N);
     std::copy(s.vel.begin(), s.vel.end(), vs.begin() + (size_t)nsub * s.N);
@@ -1572,58 +1594,69 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, const float* force, const float*
         xb[p] = d * ((R)2 * (R)mask[p]);
         vb[p] = mk<R>(0, 0, 0);
     }
-    if (!grad) return loss;
-    std::vector<R> adb(s.T, 0), ahb(s.T, 0);
-    std::vector<TetTape<R>> tt(s.T);
-    std::vector<V3<R>> et(2 * (size_t)s.E);
-    std::vector<V3<R>> prevb(s.N);
-    for (int n = nsub - 1; n >= 0; n--) {
-        std::copy(xs.begin() + (size_t)n * s.N, xs.begin() + (size_t)(n + 1) * s.N, s.pos.begin());
-        std::copy(vs.begin() + (size_t)n * s.N, vs.begin() + (size_t)(n + 1) * s.N, s.vel.begin());
-        adjointSubstep<R>(s, F, h, &tt, &et);
-        // velocity: v' = (x_end - prev) / h * velScale for w > 0
-        R invH = (R)1 / h;
-        for (int p = 0; p < s.N; p++) {
-            if (s.invMass[p] > 0) {
-                V3<R> g = vb[p] * (invH * s.velScale);
-                xb[p] = xb[p] + g;
-                prevb[p] = g * (R)-1;
-            } else prevb[p] = mk<R>(0, 0, 0);
-        }
-        // tets, colours in reverse
-        for (int c = (int)s.tcolorCnt.size() - 1; c >= 0; c--)
-            for (int k = 0; k < s.tcolorCnt[c]; k++) {
-                int t = s.tcolorOrder[s.tcolorOff[c] + k];
-                if (!s.tactive[t]) continue;
-                const int* id = &s.tids[4 * t];
-                R w[4];
-                V3<R> ob[4], pb[4];
-                for (int j = 0; j < 4; j++) { w[j] = s.invMass[id[j]]; ob[j] = xb[id[j]]; }
-                const R(*B)[3] = s.typeB[s.ttype[t]];
-                V3<R> br[3] = {mk<R>(B[0][0], B[0][1], B[0][2]), mk<R>(B[1][0], B[1][1], B[1][2]),
-                               mk<R>(B[2][0], B[2][1], B[2][2])};
-                tetProjectReverse<R>(tt[t].p, w, br, s.alphaDev[t], s.alphaHyd[t], ob, pb, adb[t], ahb[t]);
-                for (int j = 0; j < 4; j++) xb[id[j]] = pb[j];
+    if (grad) {
+        std::vector<R> adb(s.T, 0), ahb(s.T, 0), lDb(s.T), lHb(s.T), leb(s.E);
+        std::vector<TetTape<R>> tt((size_t)iterations * s.T);
+        std::vector<EdgeTape<R>> et((size_t)iterations * s.E);
+        std::vector<V3<R>> prevb(s.N);
+        for (int n = nsub - 1; n >= 0; n--) {
+            std::copy(xs.begin() + (size_t)n * s.N, xs.begin() + (size_t)(n + 1) * s.N, s.pos.begin());
+            std::copy(vs.begin() + (size_t)n * s.N, vs.begin() + (size_t)(n + 1) * s.N, s.vel.begin());
+            adjointSubstep<R>(s, F, h, iterations, &tt, &et);
+            // velocity: v' = (x_end - prev) / h * velScale for w > 0
+            R invH = (R)1 / h;
+            for (int p = 0; p < s.N; p++) {
+                if (s.invMass[p] > 0) {
+                    V3<R> g = vb[p] * (invH * s.velScale);
+                    xb[p] = xb[p] + g;
+                    prevb[p] = g * (R)-1;
+                } else prevb[p] = mk<R>(0, 0, 0);
             }
-        for (int c = (int)s.ecolorCnt.size() - 1; c >= 0; c--)
-            for (int k = 0; k < s.ecolorCnt[c]; k++) {
-                int e = s.ecolorOff[c] + k;
-                if (!s.eactive[e]) continue;
-                int i = s.ei[e], j = s.ej[e];
-                V3<R> xib, xjb;
-                edgeProjectReverse<R>(et[2 * e], et[2 * e + 1], s.invMass[i], s.invMass[j], s.erest[e],
-                                      s.alphaTilde, xb[i], xb[j], xib, xjb);
-                xb[i] = xib;
-                xb[j] = xjb;
+            // the multipliers of the last iteration reach nothing after the substep
+            std::fill(lDb.begin(), lDb.end(), (R)0);
+            std::fill(lHb.begin(), lHb.end(), (R)0);
+            std::fill(leb.begin(), leb.end(), (R)0);
+            for (int it = iterations - 1; it >= 0; it--) {
+                // tets, colours in reverse
+                for (int c = (int)s.tcolorCnt.size() - 1; c >= 0; c--)
+                    for (int k = 0; k < s.tcolorCnt[c]; k++) {
+                        int t = s.tcolorOrder[s.tcolorOff[c] + k];
+                        if (!s.tactive[t]) continue;
+                        const int* id = &s.tids[4 * t];
+                        R w[4];
+                        V3<R> ob[4], pb[4];
+                        for (int j = 0; j < 4; j++) { w[j] = s.invMass[id[j]]; ob[j] = xb[id[j]]; }
+                        const R(*B)[3] = s.typeB[s.ttype[t]];
+                        V3<R> br[3] = {mk<R>(B[0][0], B[0][1], B[0][2]), mk<R>(B[1][0], B[1][1], B[1][2]),
+                                       mk<R>(B[2][0], B[2][1], B[2][2])};
+                        const TetTape<R>& r = tt[(size_t)it * s.T + t];
+                        tetProjectReverse<R>(r.p, w, br, s.alphaDev[t], s.alphaHyd[t], r.lamD, r.lamH, ob, pb, adb[t],
+                                             ahb[t], lDb[t], lHb[t]);
+                        for (int j = 0; j < 4; j++) xb[id[j]] = pb[j];
+                    }
+                for (int c = (int)s.ecolorCnt.size() - 1; c >= 0; c--)
+                    for (int k = 0; k < s.ecolorCnt[c]; k++) {
+                        int e = s.ecolorOff[c] + k;
+                        if (!s.eactive[e]) continue;
+                        int i = s.ei[e], j = s.ej[e];
+                        const EdgeTape<R>& r = et[(size_t)it * s.E + e];
+                        V3<R> xib, xjb;
+                        edgeProjectReverse<R>(r.xi, r.xj, s.invMass[i], s.invMass[j], s.erest[e], s.alphaTilde, r.lam,
+                                              xb[i], xb[j], xib, xjb, leb[e]);
+                        xb[i] = xib;
+                        xb[j] = xjb;
+                    }
+                // the input multipliers of iteration 0 are the substep's reset zeros
             }
-        // predict: xp = x + h (v + h a) for w > 0, prev = x
-        for (int p = 0; p < s.N; p++) {
-            V3<R> g = xb[p];
-            xb[p] = g + prevb[p];
-            vb[p] = s.invMass[p] > 0 ? g * h : mk<R>(0, 0, 0);
+            // predict: xp = x + h (v + h a) for w > 0, prev = x
+            for (int p = 0; p < s.N; p++) {
+                V3<R> g = xb[p];
+                xb[p] = g + prevb[p];
+                vb[p] = s.invMass[p] > 0 ? g * h : mk<R>(0, 0, 0);
+            }
         }
+        for (int t = 0; t < s.T; t++) grad[t] = (adb[t] + ahb[t] / (R)10) * invH2;
     }
-    for (int t = 0; t < s.T; t++) grad[t] = (adb[t] + ahb[t] / (R)10) * invH2;
     // leave the state at the end of the forward run
     std::copy(xs.begin() + (size_t)nsub * s.N, xs.end(), s.pos.begin());
     std::copy(vs.begin() + (size_t)nsub * s.N, vs.end(), s.vel.begin());
@@ -1633,12 +1666,12 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, const float* force, const float*
 }  // namespace
 
 #define ORACLE_ADJ(R, SUF)                                                                                            \
-    extern "C" double oracle_adjoint_##SUF(void* h, float dt, int nsub, const float* force, const float* compliance,  \
-                                           const float* mask, const float* target, R* grad) {                          \
-        return adjointRun(*(Tissue<R>*)h, (R)dt, nsub, force, compliance, mask, target, grad);                        \
+    extern "C" double oracle_adjoint_##SUF(void* h, float dt, int nsub, int iterations, const float* force,          \
+                                           const float* compliance, const float* mask, const float* target, R* grad) { \
+        return adjointRun(*(Tissue<R>*)h, (R)dt, nsub, iterations, force, compliance, mask, target, grad);            \
     }                                                                                                                  \
-    extern "C" void oracle_tet_reverse_##SUF(const R* p12, const R* w4, const R* B9, R ad, R ah, const R* ob12,        \
-                                             R* pb12, R* abar2) {                                                      \
+    extern "C" void oracle_tet_reverse_##SUF(const R* p12, const R* w4, const R* B9, R ad, R ah, const R* lam2,        \
+                                             const R* ob12, R* lb2, R* pb12, R* abar2) {                               \
         V3<R> p[4], ob[4], pb[4], br[3];                                                                               \
         R w[4];                                                                                                        \
         for (int k = 0; k < 4; k++) {                                                                                  \
@@ -1648,7 +1681,7 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, const float* force, const float*
         }                                                                                                              \
         for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);                              \
         R adb = 0, ahb = 0;                                                                                            \
-        tetProjectReverse<R>(p, w, br, ad, ah, ob, pb, adb, ahb);                                                      \
+        tetProjectReverse<R>(p, w, br, ad, ah, lam2[0], lam2[1], ob, pb, adb, ahb, lb2[0], lb2[1]);                    \
         for (int k = 0; k < 4; k++) { pb12[3 * k] = pb[k].x; pb12[3 * k + 1] = pb[k].y; pb12[3 * k + 2] = pb[k].z; }   \
         abar2[0] = adb;                                                                                                \
         abar2[1] = ahb;                                                                                                \

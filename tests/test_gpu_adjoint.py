@@ -17,7 +17,7 @@ SIZES = [(7, 4, 4), (9, 5, 4), (5, 5, 6)]
 def _bind(L):
     for suf in ("f64", "f32"):
         f = getattr(L, f"oracle_adjoint_{suf}")
-        f.argtypes = [ct.c_void_p, ct.c_float, ct.c_int] + [ct.c_void_p] * 5
+        f.argtypes = [ct.c_void_p, ct.c_float, ct.c_int, ct.c_int] + [ct.c_void_p] * 5
         f.restype = ct.c_double
     return L
 
@@ -26,10 +26,10 @@ def P(a, dtype=np.float32):
     return np.ascontiguousarray(a, dtype=dtype).ctypes.data_as(ct.c_void_p)
 
 
-def blocks(scale, seed=0):
+def blocks(scale, seed=0, iterations=1):
     out = []
     for k, nodes in enumerate(SIZES):
-        inst = adjoint_instance(nodes=nodes, seed=seed + k, substeps=10)
+        inst = adjoint_instance(nodes=nodes, seed=seed + k, substeps=10, iterations=iterations)
         inst.compliance_hidden = inst.compliance_hidden * scale
         inst.compliance_init = inst.compliance_init * scale
         out.append(inst)
@@ -41,7 +41,7 @@ def oracle_adjoint(L, Oracle, inst, nsub, c, target, double=False):
     o = Oracle(inst.grid, inst.params, inst.initial_positions, double=double)
     grad = np.zeros(len(c), dtype=np.float64 if double else np.float32)
     f = L.oracle_adjoint_f64 if double else L.oracle_adjoint_f32
-    loss = f(o.h, np.float32(inst.dt), nsub, P(inst.force), P(c), P(inst.mask), P(target),
+    loss = f(o.h, np.float32(inst.dt), nsub, inst.params.iterations, P(inst.force), P(c), P(inst.mask), P(target),
              grad.ctypes.data_as(ct.c_void_p))
     return loss, grad, o.state()[0]
 
@@ -49,21 +49,23 @@ def oracle_adjoint(L, Oracle, inst, nsub, c, target, double=False):
 # The blocks have 112, 180 and 150 nodes. lds_nodes: None stages every block in LDS in both passes; 0 runs every block
 # from global memory; 160 stages the first and last block in the forward and none in the backward (which needs 2 N);
 # 300 stages all three in the forward and the first and last in the backward.
-@pytest.mark.parametrize("scale,nsub,ckpt,lds_nodes", [(1e4, 20, 7, None), (1.0, 12, 4, None), (1e4, 20, 7, 0),
-                                                       (1e4, 9, 4, 160), (1e4, 9, 4, 300)])
-def test_adjoint_matches_oracle_fp32(scale, nsub, ckpt, lds_nodes):
+@pytest.mark.parametrize("scale,nsub,ckpt,lds_nodes,iterations", [
+    (1e4, 20, 7, None, 1), (1.0, 12, 4, None, 1), (1e4, 20, 7, 0, 1), (1e4, 9, 4, 160, 1), (1e4, 9, 4, 300, 1),
+    (1e4, 20, 7, None, 2), (1e4, 9, 4, 0, 2), (1e4, 6, 6, 300, 3)])
+def test_adjoint_matches_oracle_fp32(scale, nsub, ckpt, lds_nodes, iterations):
     import torch
 
     from genesis_tissue_simulation_amd.adjoint import ComplianceFit
     from oracle import oracle as orc
 
     L = _bind(orc.lib())
-    insts = blocks(scale)
+    insts = blocks(scale, iterations=iterations)
     fit = ComplianceFit(insts, nsub=nsub, ckpt=ckpt, block_size=256, lds_nodes=lds_nodes)
     b = fit.batch
     hidden = np.concatenate([i.compliance_hidden for i in insts]).astype(np.float32)
     init = np.concatenate([i.compliance_init for i in insts]).astype(np.float32)
-    print(f"adjoint parity: blocks {SIZES} scale {scale} nsub {nsub} ckpt {ckpt} lds_nodes {lds_nodes}", flush=True)
+    print(f"adjoint parity: blocks {SIZES} scale {scale} nsub {nsub} ckpt {ckpt} lds_nodes {lds_nodes} "
+          f"iterations {iterations}", flush=True)
     target = fit.forward_raw(torch.as_tensor(hidden)).cpu().numpy().copy()
     x = fit.forward_raw(torch.as_tensor(init)).cpu().numpy().copy()
     mask = fit.mask.cpu().numpy()
@@ -76,8 +78,8 @@ def test_adjoint_matches_oracle_fp32(scale, nsub, ckpt, lds_nodes):
         c_k = init[t0:t1]
         # the oracle's target from the hidden compliance, then its adjoint at the initial compliance
         o = orc.Oracle(inst.grid, inst.params, inst.initial_positions)
-        L.oracle_adjoint_f32(o.h, np.float32(inst.dt), nsub, P(inst.force), P(hidden[t0:t1]), P(inst.mask),
-                             P(np.zeros((n1 - n0, 3))), None)
+        L.oracle_adjoint_f32(o.h, np.float32(inst.dt), nsub, iterations, P(inst.force), P(hidden[t0:t1]),
+                             P(inst.mask), P(np.zeros((n1 - n0, 3))), None)
         tgt_o = o.state()[0]
         np.testing.assert_array_equal(target[n0:n1], tgt_o)
         loss_o, grad_o, x_o = oracle_adjoint(L, orc.Oracle, inst, nsub, c_k, tgt_o)
@@ -128,7 +130,7 @@ def test_fp32_gradient_against_fp64_finite_differences():
     hidden = (inst.compliance_hidden * scale).astype(np.float32)
     c = (inst.compliance_init * scale).astype(np.float32)
     oh = orc.Oracle(inst.grid, inst.params, inst.initial_positions, double=True)
-    L.oracle_adjoint_f64(oh.h, np.float32(inst.dt), nsub, P(inst.force), P(hidden), P(inst.mask),
+    L.oracle_adjoint_f64(oh.h, np.float32(inst.dt), nsub, 1, P(inst.force), P(hidden), P(inst.mask),
                          P(np.zeros((N, 3))), None)
     target = oh.state()[0].astype(np.float32)
     fit = ComplianceFit([inst], nsub=nsub, ckpt=10)
@@ -137,8 +139,8 @@ def test_fp32_gradient_against_fp64_finite_differences():
 
     def loss64(cc):
         o = orc.Oracle(inst.grid, inst.params, inst.initial_positions, double=True)
-        return L.oracle_adjoint_f64(o.h, np.float32(inst.dt), nsub, P(inst.force), P(cc), P(inst.mask), P(target),
-                                    None)
+        return L.oracle_adjoint_f64(o.h, np.float32(inst.dt), nsub, 1, P(inst.force), P(cc), P(inst.mask),
+                                    P(target), None)
 
     idx = np.random.default_rng(0).choice(len(c), 32, replace=False)
     fd = []
