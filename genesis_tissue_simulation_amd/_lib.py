@@ -44,7 +44,7 @@ class GtsTissueView(ct.Structure):
                 ("tet_alpha", P), ("tet_lambda", P), ("grid_edges", P), ("occupied", P), ("cut_mask", P),
                 ("edge_cut_t", P), ("edge_skin_t", P), ("cut_points", P), ("cut_point_capacity", I),
                 ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
-                ("use_graph", I)]
+                ("use_graph", I), ("colour_block", I)]
 
 
 class GtsStepParams(ct.Structure):

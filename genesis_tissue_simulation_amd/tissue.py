@@ -103,9 +103,10 @@ class Tissue:
     """Device state of one tissue and the HIP entry points that advance and cut it."""
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
-                 device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None):
+                 device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None, colour_block=0):
         """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, edge_desc, edge_lambda,
-        tet_desc, tet_k_dev, tet_k_hyd, tet_lambda) -- views into the pooled buffers of a batch."""
+        tet_desc, tet_k_dev, tet_k_hyd, tet_lambda) -- views into the pooled buffers of a batch. `colour_block` is
+        the workgroup size of the edge / tet colour kernels (0: the library default)."""
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -188,9 +189,13 @@ class Tissue:
                                      self.tet_alpha, self.tet_lambda, self.grid_edges, self.occupied, self.cut_mask,
                                      self.edge_cut_t, self.edge_skin_t, self.cut_points)],
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
-            self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)))
+            self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
         self._tool_from = None
         check("gts_init_state", self.lib.gts_init_state(ct.byref(self.view), current_stream()))
+
+    def set_colour_block(self, threads):
+        """Workgroup size of the edge / tet colour kernels (a multiple of 64 up to 1024; 0: the default)."""
+        self.view.colour_block = int(threads)
 
     # ---------------------------------------------------------------- stepping
     def set_gravity(self, g):
