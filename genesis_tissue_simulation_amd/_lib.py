@@ -103,6 +103,7 @@ SIGNATURES = {
     "gts_detect_cut": ((P, P, P), I),
     "gts_detect_cut_ccd": ((P, P, P, P, F, P, P), I),
     "gts_compact_topology": ((P, P, P, P, P, P, P), I),
+    "gts_apply_compaction": ((P, P, P, P, P, P, P, P), I),
     "gts_profile_begin": ((), I),
     "gts_profile_end": ((P, P, P), I),
     "gts_batched_plan": ((P, I, I, P), I),

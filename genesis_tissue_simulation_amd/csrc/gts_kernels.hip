@@ -1112,6 +1112,36 @@ extern "C" int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, 
     return -(int)hipGetLastError();
 }
 
+namespace {
+// out[remap[k]] = in[k] for every entry the remap keeps (stable: remap is an exclusive scan of the live flags).
+template <class V>
+__global__ void k_gather_live(int n, const int* __restrict__ remap, const V* __restrict__ in, V* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        int r = remap[k];
+        if (r >= 0) out[r] = in[k];
+    }
+}
+}  // namespace
+
+// Moves the live tets and edges of the view into the output arrays at the positions gts_compact_topology assigned.
+// The outputs hold counts[0] tets and counts[1] edges; they must not alias the inputs.
+extern "C" int gts_apply_compaction(const GtsTissueView* v, const int32_t* tet_remap, const int32_t* edge_remap,
+                                    uint32_t* tet_desc_out, float* tet_k_dev_out, float* tet_k_hyd_out,
+                                    uint32_t* edge_desc_out, void* stream) {
+    if (!v || !tet_remap || !edge_remap || !tet_desc_out || !tet_k_dev_out || !tet_k_hyd_out || !edge_desc_out)
+        return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    const int T = v->n_tets, E = v->n_edges;
+    if (T > 0) {
+        k_gather_live<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, tet_remap, v->tet_desc, tet_desc_out);
+        k_gather_live<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, tet_remap, v->tet_k_dev, tet_k_dev_out);
+        k_gather_live<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, tet_remap, v->tet_k_hyd, tet_k_hyd_out);
+    }
+    if (E > 0) k_gather_live<<<grid_for(E), GTS_BLOCK, 0, s>>>(E, edge_remap, v->edge_desc, edge_desc_out);
+    return -(int)hipGetLastError();
+}
+
 extern "C" float gts_iso_pow(float J) { return iso_pow(J); }
 extern "C" float gts_log(float x) { return own_log(x); }
 

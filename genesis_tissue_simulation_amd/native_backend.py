@@ -223,9 +223,10 @@ class NativeTissue:
         """First blade fraction of each directed lattice edge, f32[N, 6], -1 where uncut (`cut.cu:159-162`)."""
         return self.tissue.edge_cut_t.cpu().numpy().reshape(-1, 6)
 
-    def compact_topology(self):
-        """Stable remap of the live tets (colour order), edges and nodes after cuts, plus their counts."""
-        return self.tissue.compact()
+    def compact_topology(self, apply=True):
+        """Stable remap of the live tets (colour order), edges and nodes after cuts, plus their counts; with `apply`
+        (the default) the dead tets and edges are removed from the device lists."""
+        return self.tissue.compact(apply=apply)
 
     def material_quality(self):
         """[mean |J - 1|, min J] over the live tets of the current deformation."""
@@ -271,10 +272,10 @@ def _capsule_depth(points, a, b, r):
 
 
 def tet_determinants(tissue):
-    """det F of every live tet (generation order of the live ones), from the current positions."""
+    """det F of every live tet (in the device's colour order), from the current positions."""
     from .grid import KUHN_TETS
     topo = tissue.topology
-    desc = tissue.tet_desc.cpu().numpy().view(np.uint32)
+    desc = tissue.tet_desc[:max(tissue.T, 1)].cpu().numpy().view(np.uint32)
     live = (desc & 1) != 0
     base = (desc >> 4).astype(np.int64)[live]
     ty = ((desc >> 1) & 7).astype(np.int64)[live]

@@ -118,6 +118,12 @@ class Tissue:
         E, T = len(topo["edge_desc"]), len(topo["tet_desc"])
         Eg = len(grid.edges)
         self.N, self.E, self.T, self.Eg, self.V = N, E, T, Eg, V
+        # current tet slot -> generation index and current edge slot -> original colour-order slot (compaction
+        # removes dead entries; the readbacks report over the original lists)
+        self.T0, self.E0 = T, E
+        self.tet_slot_gen = np.asarray(topo["tet_gen"], dtype=np.int64)
+        self.edge_slot_orig = np.arange(E, dtype=np.int64)
+        self.pooled = storage is not None
         dev = self.device
 
         def f32(a, shape=None):
@@ -264,14 +270,19 @@ class Tissue:
                     revision=int(counters[0]), cut_point_counter=int(counters[1]), cut_points=points)
 
     def liveness(self):
-        """Tet liveness in generation order and edge liveness in colour order (as the CPU checker reports)."""
-        tet = np.zeros(self.T, dtype=np.int32)
-        tet[self.topology["tet_gen"]] = self.tet_desc.cpu().numpy() & 1
-        edge = self.edge_desc.cpu().numpy() & 1
+        """Tet liveness in generation order and edge liveness in colour order over the original (uncompacted) lists,
+        as the CPU checker reports them; removed entries read 0."""
+        tet = np.zeros(self.T0, dtype=np.int32)
+        tet[self.tet_slot_gen] = self.tet_desc[:self.T].cpu().numpy() & 1
+        edge = np.zeros(self.E0, dtype=np.int32)
+        edge[self.edge_slot_orig] = self.edge_desc[:self.E].cpu().numpy() & 1
         return tet, edge
 
-    def compact(self):
-        """Stable remap of the live topology: (tet_remap in colour order, edge_remap, node_remap, counts)."""
+    def compact(self, apply=False):
+        """Stable remap of the live topology: (tet_remap in colour order, edge_remap, node_remap, counts), relative
+        to the current lists. With `apply`, the dead tets and edges are removed: the live ones move to their remapped
+        slots, the colour ranges shrink to their live counts and every later step and cut runs over the live lists
+        only (node storage keeps its lattice addressing)."""
         dev = self.device
         tr = torch.empty(max(self.T, 1), dtype=torch.int32, device=dev)
         er = torch.empty(max(self.E, 1), dtype=torch.int32, device=dev)
@@ -280,4 +291,33 @@ class Tissue:
         used = torch.empty(self.N, dtype=torch.int32, device=dev)
         check("gts_compact_topology", self.lib.gts_compact_topology(
             ct.byref(self.view), _ptr(tr), _ptr(er), _ptr(nr), _ptr(counts), _ptr(used), current_stream()))
-        return (tr[:self.T].cpu().numpy(), er[:self.E].cpu().numpy(), nr.cpu().numpy(), counts.cpu().numpy())
+        out = (tr[:self.T].cpu().numpy(), er[:self.E].cpu().numpy(), nr.cpu().numpy(), counts.cpu().numpy())
+        if apply:
+            self._apply_compaction(tr, er, out[0], out[1], int(out[3][0]), int(out[3][1]))
+        return out
+
+    def _apply_compaction(self, tr, er, tr_host, er_host, new_t, new_e):
+        if self.pooled:
+            raise ValueError("a tissue in a batch's pooled buffers cannot be compacted in place")
+        dev = self.device
+        tet_desc = torch.zeros(max(new_t, 1), dtype=torch.int32, device=dev)
+        k_dev = torch.zeros(max(new_t, 1), dtype=torch.float32, device=dev)
+        k_hyd = torch.zeros(max(new_t, 1), dtype=torch.float32, device=dev)
+        edge_desc = torch.zeros(max(new_e, 1), dtype=torch.int32, device=dev)
+        check("gts_apply_compaction", self.lib.gts_apply_compaction(
+            ct.byref(self.view), _ptr(tr), _ptr(er), _ptr(tet_desc), _ptr(k_dev), _ptr(k_hyd), _ptr(edge_desc),
+            current_stream()))
+        live_t, live_e = tr_host >= 0, er_host >= 0
+        kept_t = np.concatenate(([0], np.cumsum(live_t)))
+        kept_e = np.concatenate(([0], np.cumsum(live_e)))
+        self.tet_offsets = np.ascontiguousarray(kept_t[self.tet_offsets], dtype=np.int32)
+        self.edge_offsets = np.ascontiguousarray(kept_e[self.edge_offsets], dtype=np.int32)
+        self.tet_slot_gen = self.tet_slot_gen[live_t]
+        self.edge_slot_orig = self.edge_slot_orig[live_e]
+        self.tet_desc, self.tet_k_dev, self.tet_k_hyd, self.edge_desc = tet_desc, k_dev, k_hyd, edge_desc
+        self.T, self.E = new_t, new_e
+        v = self.view
+        v.n_tets, v.n_edges = new_t, new_e
+        v.tet_colour_offsets, v.edge_colour_offsets = self.tet_offsets.ctypes.data, self.edge_offsets.ctypes.data
+        v.tet_desc, v.tet_k_dev, v.tet_k_hyd = tet_desc.data_ptr(), k_dev.data_ptr(), k_hyd.data_ptr()
+        v.edge_desc = edge_desc.data_ptr()

@@ -36,6 +36,7 @@ class Workload:
     blade_thickness: float = 0.0
     tool: object = None             # callable step -> tool dict, or None
     gravity_ramp_s: float = 0.0     # gravity ramped linearly over this time, as NativeTissue.step does
+    blades: list = field(default_factory=list)  # further blades (callable step -> (S, E)), same thickness
     info: dict = field(default_factory=dict)
 
     def gravity_at(self, step):
@@ -140,6 +141,45 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
     return Workload("kidney-250k", grid, params, 1 / 30, steps, compliance, jittered(grid, rng, 0.0),
                     blade=blade, blade_thickness=0.02, tool=tool, gravity_ramp_s=0.5,
                     info=dict(spacing=h, half_extents=(6.0, 3.0, 2.5)))
+
+
+def cutting_block(nodes=(112, 56, 56), spacing=0.25, seed=0, steps=500, substeps=20, iterations=2, n_blades=3,
+                  depth_fraction=0.35):
+    """The large continuous-cutting block: 112 x 56 x 56 nodes (111 x 55 x 55 cells, 2.01M Kuhn tets; the 161 x 81 x 81
+    lattice of the same description would hold 6.14M), tet compliance 1e-7 (volume 1e-8) and edge compliance 1e-7,
+    bottom face pinned, gravity ramped over 0.5 s, 20 substeps of 2 iterations at dt = 1/60. Three seeded vertical
+    blades of depth 0.35 x the height cross the top face along seeded straight paths over `steps` steps, cutting
+    every step; the blocks are compacted every step by the caller."""
+    rng = np.random.default_rng(seed)
+    grid = block_grid(nodes, spacing)
+    T = tet_count(grid)
+    params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7)
+    lo = grid.positions.min(axis=0).astype(np.float64)
+    hi = grid.positions.max(axis=0).astype(np.float64)
+    top, depth = hi[1] + 0.5 * spacing, depth_fraction * (hi[1] - lo[1])
+    paths = []
+    for _ in range(n_blades):
+        a = lo[[0, 2]] + rng.uniform(0.05, 0.95, 2) * (hi - lo)[[0, 2]]
+        b = lo[[0, 2]] + rng.uniform(0.05, 0.95, 2) * (hi - lo)[[0, 2]]
+        paths.append((a, b))
+
+    def blade_of(path):
+        a, b = path
+
+        def blade(step):
+            p = a + (b - a) * min(1.0, step / steps)
+            return (np.array((p[0], top, p[1]), np.float32), np.array((p[0], hi[1] - depth, p[1]), np.float32))
+        return blade
+
+    blades = [blade_of(p) for p in paths]
+    return Workload("cutting-block-2m", grid, params, 1 / 60, steps, np.full(T, 1e-7),
+                    jittered(grid, rng, 1e-4 * spacing), blade=blades[0], blade_thickness=0.02,
+                    gravity_ramp_s=0.5, blades=blades[1:], info=dict(nodes=nodes, spacing=spacing, paths=paths))
+
+
+def all_blades(w):
+    """Every blade of a workload: the first, then the further ones."""
+    return ([w.blade] if w.blade else []) + list(w.blades)
 
 
 @dataclass

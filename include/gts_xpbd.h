@@ -183,6 +183,15 @@ int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, c
  * counts [3] (live tets, edges, nodes); all device. scratch as in the view. */
 int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap, int32_t* node_remap,
                          int32_t* counts, int32_t* node_used, void* stream);
+/* Device: the removal itself (no reference counterpart; the reference keeps dead tets in its static colouring):
+ * gather the live tets (descriptor, k_dev, k_hyd) and edges (descriptor) of the view into output arrays of
+ * counts[0] tets and counts[1] edges, in the order of gts_compact_topology's remaps. The caller then points the view
+ * at the outputs and updates n_tets, n_edges and the colour offsets (the live count of each colour range). Node
+ * storage keeps its lattice addressing, because descriptors locate their nodes on the lattice; node_remap is the
+ * readback numbering of the used nodes. */
+int gts_apply_compaction(const GtsTissueView* v, const int32_t* tet_remap, const int32_t* edge_remap,
+                         uint32_t* tet_desc_out, float* tet_k_dev_out, float* tet_k_hyd_out, uint32_t* edge_desc_out,
+                         void* stream);
 
 /* ---- batched stepping: many independent lattice instances in pooled buffers, one persistent launch per step ---- */
 
