@@ -50,6 +50,7 @@ class Batch:
         self.k_hyd = torch.zeros(max(sT, 1), dtype=torch.float32, device=dev)
         self.lambda_edge = torch.zeros(max(sE, 1), dtype=torch.float32, device=dev)
         self.lambda_tet = torch.zeros(max(2 * sT, 1), dtype=torch.float32, device=dev)
+        self.alpha = torch.zeros(max(2 * sT, 1), dtype=torch.float32, device=dev)
         self.tissues = []
         records = (GtsBatchInstance * len(instances))()
         colour_rows = []
@@ -109,7 +110,7 @@ class Batch:
             self.work_items.data_ptr(), n_workgroups, self.block_size, self.lds_nodes, int(bool(self.use_graph)),
             (ct.c_float * 3)(*self.gravity), self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(),
             self.mass.data_ptr(), self.edge_desc.data_ptr(), self.tet_desc.data_ptr(), self.k_dev.data_ptr(),
-            self.k_hyd.data_ptr(), self.lambda_edge.data_ptr(), self.lambda_tet.data_ptr())
+            self.k_hyd.data_ptr(), self.lambda_edge.data_ptr(), self.lambda_tet.data_ptr(), self.alpha.data_ptr())
         return self
 
     def step(self, dt, params=None):

@@ -41,8 +41,8 @@ template <int MODE, bool DAMPED, class Store>
 __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const BatchStep& P, const int* colours,
                              const uint32_t* __restrict__ edge_desc, const uint32_t* __restrict__ tet_desc,
                              const float* __restrict__ k_dev, const float* __restrict__ k_hyd,
-                             float* __restrict__ lam_edge, float* __restrict__ lam_tet, const float* __restrict__ mass,
-                             float4* __restrict__ prev, float4* __restrict__ vel) {
+                             float* __restrict__ lam_edge, float* __restrict__ lam_tet, float* __restrict__ alpha,
+                             const float* __restrict__ mass, float4* __restrict__ prev, float4* __restrict__ vel) {
     const int tid = threadIdx.x, bs = blockDim.x;
     const int N = I.n_nodes, T = I.n_tets;
     const int nx = I.nx, nxy = I.nxy;
@@ -50,6 +50,12 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
     const int* tcol = colours + I.n_edge_colours + 1;
     const float at = I.edge_ks > 0.f ? 1.f / (I.edge_ks * P.h * P.h) : 0.f;
     const float betaGammaH = 0.f;
+    // k_ComputeTetAlpha (xpbd_solver.cu:578-587) once per step, as the single-instance path does per frame
+    for (int k = tid; k < T; k += bs) {
+        alpha[k] = P.invH2 / k_dev[k];
+        alpha[T + k] = P.invH2 / k_hyd[k];
+    }
+    __syncthreads();
     for (int n = 0; n < P.substeps; n++) {
         // k_predict (xpbd_solver.cu:393-412) with the gravity force g * mass of k_SetGravity
         for (int i = tid; i < N; i += bs) {
@@ -90,7 +96,7 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                     int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
                     int id[4];
                     for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), nx, nxy);
-                    float ad = P.invH2 / k_dev[k], ah = P.invH2 / k_hyd[k];
+                    float ad = alpha[k], ah = alpha[T + k];
                     float lamD = read ? lam_tet[k] : 0.f, lamH = read ? lam_tet[T + k] : 0.f;
                     if (tet_project<MODE, DAMPED>(S, prev, id, I.inv_rest[type], ad, ah, lamD, lamH, P.omega,
                                                   P.dOverH, P.bOverH) &&
@@ -123,7 +129,8 @@ __global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBat
                                                        const uint32_t* __restrict__ edge_desc,
                                                        const uint32_t* __restrict__ tet_desc,
                                                        const float* __restrict__ k_dev, const float* __restrict__ k_hyd,
-                                                       float* __restrict__ lam_edge, float* __restrict__ lam_tet) {
+                                                       float* __restrict__ lam_edge, float* __restrict__ lam_tet,
+                                                       float* __restrict__ alpha) {
     extern __shared__ float4 lds_nodes[];
     const int wg = blockIdx.x;
     for (int w = work_offsets[wg]; w < work_offsets[wg + 1]; w++) {
@@ -140,13 +147,16 @@ __global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBat
         const float* kh = k_hyd + I.tet_off;
         float* le = lam_edge + I.edge_off;
         float* lt = lam_tet + 2 * (size_t)I.tet_off;
+        float* al = alpha + 2 * (size_t)I.tet_off;
         if (I.n_nodes <= P.lds_nodes) {
             for (int i = threadIdx.x; i < I.n_nodes; i += blockDim.x) lds_nodes[i] = gpos[i];
             __syncthreads();
-            run_instance<MODE, DAMPED>(LdsNodes{lds_nodes}, I, P, colours, ed, td, kd, kh, le, lt, gmass, gprev, gvel);
+            run_instance<MODE, DAMPED>(LdsNodes{lds_nodes}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass, gprev,
+                                       gvel);
             for (int i = threadIdx.x; i < I.n_nodes; i += blockDim.x) gpos[i] = lds_nodes[i];
         } else {
-            run_instance<MODE, DAMPED>(GlobalNodes{gpos}, I, P, colours, ed, td, kd, kh, le, lt, gmass, gprev, gvel);
+            run_instance<MODE, DAMPED>(GlobalNodes{gpos}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass, gprev,
+                                       gvel);
         }
         __syncthreads();
     }
@@ -169,7 +179,7 @@ int launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
     kern<<<v->n_workgroups, v->block_size, lds, s>>>(
         P, reinterpret_cast<const GtsBatchInstance*>(v->instances), v->colour_table, v->work_offsets, v->work_items,
         (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, v->mass, v->edge_desc, v->tet_desc, v->k_dev, v->k_hyd,
-        v->lambda_edge, v->lambda_tet);
+        v->lambda_edge, v->lambda_tet, v->alpha);
     return -(int)hipGetLastError();
 }
 

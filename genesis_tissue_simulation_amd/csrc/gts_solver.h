@@ -66,7 +66,9 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
         f3 b = br[c - 1];
         gh[c] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
     }
-    float scale = 1.f + dampingOverH;
+    // Undamped instantiations run only with both damping terms zero: the scales are then exactly 1 and the divisions
+    // by them and the damping terms drop out without changing a bit.
+    float scale = DAMPED ? 1.f + dampingOverH : 1.f;
     if (MODE == 0) {
         float CH = J - 1.f;
         gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
@@ -77,7 +79,7 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
         if (DAMPED)
             for (int c = 0; c < 4; c++) vh += dot3(gh[c], p[c] - xyz4(prev[id[c]]));
         for (int c = 0; c < 4; c++) schur += w[c] * dot3(gh[c], gh[c]);
-        float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
+        float rh = DAMPED ? (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale : -CH - ah * lamH;
         if (!(schur > 0.f)) return false;
         float dlH = omega * rh / schur;
         if (!isfinite(dlH)) return false;
@@ -114,7 +116,7 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
             vh += dot3(gh[c], dx);
         }
     }
-    float bulkScale = safeguarded ? 1.f + bulkDampingOverH : scale;
+    float bulkScale = DAMPED ? (safeguarded ? 1.f + bulkDampingOverH : scale) : 1.f;
     float effectiveAd = ad / scale, effectiveAh = ah / bulkScale;
     float A = dd + effectiveAd, Bc = dh, gamma = Bc / A;
     float schur = effectiveAh + effectiveAd * gamma * gamma;
@@ -122,8 +124,8 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
         f3 residual = gh[c] - gd[c] * gamma;
         schur += w[c] * dot3(residual, residual);
     }
-    float rd = (-CD - ad * lamD - dampingOverH * vd) / scale;
-    float rh = (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale;
+    float rd = DAMPED ? (-CD - ad * lamD - dampingOverH * vd) / scale : -CD - ad * lamD;
+    float rh = DAMPED ? (-CH - ah * lamH - (bulkScale - 1.f) * vh) / bulkScale : -CH - ah * lamH;
     if (!(A > 0.f) || !(schur > 0.f)) return false;
     float dlH = omega * (rh - gamma * rd) / schur, dlD = (omega * rd - Bc * dlH) / A;
     if (!isfinite(dlD) || !isfinite(dlH)) return false;

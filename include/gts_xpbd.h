@@ -224,6 +224,7 @@ typedef struct GtsBatchView {
     const float* k_hyd;
     float* lambda_edge;           /* [sum E] */
     float* lambda_tet;            /* [2 sum T], dev then hyd per instance slice */
+    float* alpha;                 /* [2 sum T] work: alpha~ of the step, dev then hyd per instance slice */
 } GtsBatchView;
 
 /* Host: LPT packing of instances (by cost) onto workgroups. out: offsets[n_workgroups+1] then items[n]. */
