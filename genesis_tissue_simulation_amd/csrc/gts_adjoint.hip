@@ -26,6 +26,12 @@ struct GtsBatchInstanceHead {
     float edge_ks;
 };
 
+// Per-tet inputs of a forward projection, loaded ahead of use.
+struct TetInputs {
+    uint32_t d;
+    float ad, ah, lamD, lamH;
+};
+
 struct AdjStep {
     int nsub, ckpt, nckpt, iters, lds_nodes;
     float h, invH, invH2, velScale;
@@ -162,13 +168,35 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P,
             }
             __syncthreads();
         }
+        // This thread's first tet of the next colour is loaded while the current colour computes: no projection of
+        // one colour writes another colour's descriptor, alpha or multipliers.
+        auto fetch = [&](int k, int end) {
+            TetInputs t;
+            t.d = 0u;
+            t.ad = t.ah = t.lamD = t.lamH = 0.f;
+            if (k < end) {
+                t.d = s.td[k];
+                t.ad = s.alpha[k];
+                t.ah = s.alpha[T + k];
+                if (read) {
+                    t.lamD = s.lam_t[k];
+                    t.lamH = s.lam_t[T + k];
+                }
+            }
+            return t;
+        };
+        TetInputs next = fetch(s.tcol[0] + tid, s.tcol[1]);
         for (int c = 0; c < s.I.n_tet_colours; c++) {
-            for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
-                uint32_t d = s.td[k];
+            const int k0 = s.tcol[c] + tid, end = s.tcol[c + 1];
+            const TetInputs first = next;
+            if (c + 1 < s.I.n_tet_colours) next = fetch(s.tcol[c + 1] + tid, s.tcol[c + 2]);
+            for (int k = k0; k < end; k += bs) {
+                const TetInputs in = k == k0 ? first : fetch(k, end);
+                uint32_t d = in.d;
                 if (!(d & 1u)) continue;
                 int id[4], type;
                 tet_ids(s, d, id, type);
-                float lamD = read ? s.lam_t[k] : 0.f, lamH = read ? s.lam_t[T + k] : 0.f;
+                float lamD = in.lamD, lamH = in.lamH;
                 if (TAPE) {
                     for (int q = 0; q < 4; q++) tt[4 * k + q] = S.ld(id[q]);
                     if (write) {
@@ -176,8 +204,8 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P,
                         ltt[T + k] = lamH;
                     }
                 }
-                if (tet_project<1, false>(S, s.prev, id, s.rec->inv_rest[type], s.alpha[k], s.alpha[T + k], lamD,
-                                          lamH, 1.f, 0.f, 0.f) &&
+                if (tet_project<1, false>(S, s.prev, id, s.rec->inv_rest[type], in.ad, in.ah, lamD, lamH, 1.f, 0.f,
+                                          0.f) &&
                     write) {
                     s.lam_t[k] = lamD;
                     s.lam_t[T + k] = lamH;
@@ -394,9 +422,28 @@ __device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, 
         const float4* tt = s.ttape + sl * s.tslot;
         const float* lte = s.ltape_e + sl * s.lslot;
         const float* ltt = s.ltape_t + sl * s.lslot;
-        for (int c = s.I.n_tet_colours - 1; c >= 0; c--) {
-            for (int k = s.tcol[c] + tid; k < s.tcol[c + 1]; k += bs) {
-                uint32_t d = s.td[k];
+        // The descriptor and alphas of this thread's first tet of the next (lower) colour are loaded ahead, as in
+        // the forward sweep.
+        auto fetch = [&](int k, int end) {
+            TetInputs t;
+            t.d = 0u;
+            t.ad = t.ah = t.lamD = t.lamH = 0.f;
+            if (k < end) {
+                t.d = s.td[k];
+                t.ad = s.alpha[k];
+                t.ah = s.alpha[T + k];
+            }
+            return t;
+        };
+        const int top = s.I.n_tet_colours - 1;
+        TetInputs next = top >= 0 ? fetch(s.tcol[top] + tid, s.tcol[top + 1]) : TetInputs{0u, 0.f, 0.f, 0.f, 0.f};
+        for (int c = top; c >= 0; c--) {
+            const int k0 = s.tcol[c] + tid, end = s.tcol[c + 1];
+            const TetInputs first = next;
+            if (c > 0) next = fetch(s.tcol[c - 1] + tid, s.tcol[c]);
+            for (int k = k0; k < end; k += bs) {
+                const TetInputs in = k == k0 ? first : fetch(k, end);
+                uint32_t d = in.d;
                 if (!(d & 1u)) continue;
                 int id[4], type;
                 tet_ids(s, d, id, type);
@@ -410,8 +457,7 @@ __device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, 
                 }
                 float lamD = tapedLam ? ltt[k] : 0.f, lamH = tapedLam ? ltt[T + k] : 0.f;
                 float adb = s.abar[k], ahb = s.abar[T + k], lDb = s.lamb_t[k], lHb = s.lamb_t[T + k];
-                tet_reverse(p, w, s.rec->inv_rest[type], s.alpha[k], s.alpha[T + k], lamD, lamH, ob, pb, adb, ahb,
-                            lDb, lHb);
+                tet_reverse(p, w, s.rec->inv_rest[type], in.ad, in.ah, lamD, lamH, ob, pb, adb, ahb, lDb, lHb);
                 for (int q = 0; q < 4; q++) X.st(id[q], make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f));
                 s.abar[k] = adb;
                 s.abar[T + k] = ahb;

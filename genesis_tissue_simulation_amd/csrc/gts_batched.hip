@@ -31,6 +31,12 @@
 
 namespace {
 
+// Per-tet inputs of a projection, loaded ahead of use.
+struct TetInputs {
+    uint32_t d;
+    float ad, ah, lamD, lamH;
+};
+
 struct BatchStep {
     int substeps, iters, mode, n_items;
     float h, invH, invH2, omega, velScale, gx, gy, gz, dOverH, bOverH;
@@ -89,16 +95,37 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                     }
                     __syncthreads();
                 }
+            // The per-tet inputs of this thread's first tet of the next colour are loaded while the current colour
+            // computes, hiding their latency behind it: no projection of one colour writes another colour's
+            // descriptor, alpha or multipliers.
+            auto fetch = [&](int k, int end) {
+                TetInputs t;
+                t.d = 0u;
+                t.ad = t.ah = t.lamD = t.lamH = 0.f;
+                if (k < end) {
+                    t.d = tet_desc[k];
+                    t.ad = alpha[k];
+                    t.ah = alpha[T + k];
+                    if (read) {
+                        t.lamD = lam_tet[k];
+                        t.lamH = lam_tet[T + k];
+                    }
+                }
+                return t;
+            };
+            TetInputs next = fetch(tcol[0] + tid, tcol[1]);
             for (int c = 0; c < I.n_tet_colours; c++) {
-                for (int k = tcol[c] + tid; k < tcol[c + 1]; k += bs) {
-                    uint32_t d = tet_desc[k];
-                    if (!(d & 1u)) continue;
-                    int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+                const int k0 = tcol[c] + tid, end = tcol[c + 1];
+                const TetInputs first = next;
+                if (c + 1 < I.n_tet_colours) next = fetch(tcol[c + 1] + tid, tcol[c + 2]);
+                for (int k = k0; k < end; k += bs) {
+                    const TetInputs t = k == k0 ? first : fetch(k, end);
+                    if (!(t.d & 1u)) continue;
+                    int base = (int)(t.d >> 4), type = (int)((t.d >> 1) & 7u);
                     int id[4];
                     for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), nx, nxy);
-                    float ad = alpha[k], ah = alpha[T + k];
-                    float lamD = read ? lam_tet[k] : 0.f, lamH = read ? lam_tet[T + k] : 0.f;
-                    if (tet_project<MODE, DAMPED>(S, prev, id, I.inv_rest[type], ad, ah, lamD, lamH, P.omega,
+                    float lamD = t.lamD, lamH = t.lamH;
+                    if (tet_project<MODE, DAMPED>(S, prev, id, I.inv_rest[type], t.ad, t.ah, lamD, lamH, P.omega,
                                                   P.dOverH, P.bOverH) &&
                         write) {
                         lam_tet[k] = lamD;
