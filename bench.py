@@ -34,7 +34,8 @@ def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
 # kernels whose HBM bytes per dispatch the traffic file records, by the name fragment rocprofv3 reports
 TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_batched_step": "k_batched_step<", "k_adjoint_forward": "k_adjoint_forward",
-                   "k_adjoint_backward": "k_adjoint_backward", "k_stream_copy": "k_stream_copy"}
+                   "k_adjoint_tape": "k_adjoint_tape", "k_adjoint_reverse": "k_adjoint_reverse",
+                   "k_stream_copy": "k_stream_copy"}
 
 
 def load_traffic(kernel):
@@ -291,10 +292,12 @@ def adjoint_fp64_checks(a):
     return out
 
 
-def adjoint_traffic():
-    """HBM bytes of one forward plus one backward adjoint launch, or None."""
-    f, b = load_traffic("k_adjoint_forward"), load_traffic("k_adjoint_backward")
-    return f["bytes_per_launch"] + b["bytes_per_launch"] if f and b else None
+def adjoint_traffic(segments):
+    """HBM bytes of one fit iteration (a forward launch, then a tape and a reverse launch per segment), or None."""
+    f, t, r = (load_traffic(k) for k in ("k_adjoint_forward", "k_adjoint_tape", "k_adjoint_reverse"))
+    if not (f and t and r):
+        return None
+    return f["bytes_per_launch"] + segments * (t["bytes_per_launch"] + r["bytes_per_launch"])
 
 
 def run_cutting_block(steps, warmup, adjoint_count=128):
@@ -582,10 +585,10 @@ def main():
             "rel_err_vs_fp64": None,
             "roofline": {"bound": "hbm", "achieved": aach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": aach / HBM_PEAK_GBS,
-                         "kernel": "k_adjoint_forward + k_adjoint_backward (one launch each per iteration)",
+                         "kernel": "k_adjoint_forward, then k_adjoint_tape and k_adjoint_reverse per segment",
                          "avg_iteration_us": 1e3 * a["kernel_ms"] / args.adjoint_iters,
                          "bytes_per_tet_substep": ab,
-                         "traffic": adjoint_traffic(),
+                         "traffic": adjoint_traffic(-(-a["nsub"] // a["ckpt"])),
                          "algorithmic_bytes_per_iteration": ab * a["T"] * a["nsub"]},
         }
         checks = adjoint_fp64_checks(a)

@@ -3,9 +3,10 @@
 The rollout is `nsub` substeps of the undamped substep (`params.iterations` Gauss-Seidel iterations) with coupled
 Neo-Hookean tets (deviatoric compliance c per tet, volume compliance c/10) and lattice edges, driven by a per-node
 force field, with no contact and no cutting: gradients are not taken across cut events. Many blocks run together,
-one workgroup per block. The forward kernel keeps (x, v) every `ckpt` substeps; the backward kernel re-runs each
-segment from its checkpoint with a tape of every projection's inputs and reverses it substep by substep, iteration
-by iteration, colour by colour.
+one workgroup per block. The forward kernel keeps (x, v) every `ckpt` substeps; the backward re-runs each segment
+from its checkpoint with a tape of every projection's inputs and reverses it substep by substep, iteration by
+iteration, colour by colour. The re-run of one segment runs on a side stream while the next-later segment is
+reversed, so two sets of workgroups are busy at once.
 
 There is no CPU path: the library must be built and a GPU present.
 """
@@ -68,7 +69,7 @@ class ComplianceFit:
         self.perm = torch.as_tensor(perm).to(dev)      # colour-major slot -> generation index
         f4 = lambda n: torch.zeros((max(n, 1), 4), dtype=torch.float32, device=dev)
         f1 = lambda n: torch.zeros(max(n, 1), dtype=torch.float32, device=dev)
-        slots = self.ckpt * p0.iterations
+        slots = 2 * self.ckpt * p0.iterations  # two tape halves: one re-taped while the other is reversed
         self.compliance = f1(sT)
         self.alpha = f1(2 * sT)
         self.checkpoints = f4(2 * self.n_ckpt * sN)

@@ -392,14 +392,16 @@ __device__ __forceinline__ void edge_reverse(f3 xi, f3 xj, float wi, float wj, f
     xjb = xjb - db;
 }
 
-// Reverse of the substep whose tapes are in the slots of `slot`. S holds the node positions (for the inverse masses),
-// X the position adjoints; the velocity adjoints and the adjoint of prev are in the instance's global slices.
-template <class Store, class XStore>
-__device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, const AdjStep& P, int slot) {
+// Reverse of the substep whose tapes are in the slots of `slot`. wsrc holds float4 node rows whose .w is the inverse
+// mass (the segment's checkpoint), X the position adjoints; the velocity adjoints and the adjoint of prev are in the
+// instance's global slices.
+template <class XStore>
+__device__ void substep_reverse(const Inst& s, const float4* __restrict__ wsrc, const XStore& X, const AdjStep& P,
+                                int slot) {
     const int tid = threadIdx.x, bs = blockDim.x, N = s.I.n_nodes, T = s.I.n_tets;
     // velocity: v' = (x_end - prev) / h * velScale for w > 0
     for (int i = tid; i < N; i += bs) {
-        float w = S.ld(i).w;
+        float w = wsrc[i].w;
         if (w > 0.f) {
             f3 g = xyz4(s.vb[i]) * (P.invH * P.velScale);
             f3 x = xyz4(X.ld(i)) + g;
@@ -485,7 +487,7 @@ __device__ void substep_reverse(const Inst& s, const Store& S, const XStore& X, 
     }
     // predict: xp = x + h (v + h a) for w > 0, prev = x
     for (int i = tid; i < N; i += bs) {
-        float w = S.ld(i).w;
+        float w = wsrc[i].w;
         f3 g = xyz4(X.ld(i));
         f3 x = g + xyz4(s.pb[i]);
         X.st(i, make_float4(x.x, x.y, x.z, 0.f));
@@ -520,14 +522,20 @@ __device__ void forward_run(const Inst& s, const Store& S, const AdjStep& P, siz
     }
 }
 
-template <class Store, class XStore>
-__device__ void backward_run(const Inst& s, const Store& S, const XStore& X, const AdjStep& P, size_t sN) {
-    for (int seg = (P.nsub - 1) / P.ckpt; seg >= 0; seg--) {
-        const int n0 = seg * P.ckpt, len = min(P.nsub, n0 + P.ckpt) - n0;
-        load_state(s, S, s.ckx + (size_t)seg * sN, s.ckv + (size_t)seg * sN);
-        for (int j = 0; j < len; j++) substep_forward<true>(s, S, P, j);
-        for (int j = len - 1; j >= 0; j--) substep_reverse(s, S, X, P, j);
-    }
+// Taped re-run of segment `seg` from its checkpoint into tape half `half`.
+template <class Store>
+__device__ void tape_segment(const Inst& s, const Store& S, const AdjStep& P, size_t sN, int seg, int half) {
+    const int n0 = seg * P.ckpt, len = min(P.nsub, n0 + P.ckpt) - n0;
+    load_state(s, S, s.ckx + (size_t)seg * sN, s.ckv + (size_t)seg * sN);
+    for (int j = 0; j < len; j++) substep_forward<true>(s, S, P, half * P.ckpt + j);
+}
+
+// Reverse of segment `seg` from tape half `half`.
+template <class XStore>
+__device__ void reverse_segment(const Inst& s, const XStore& X, const AdjStep& P, size_t sN, int seg, int half) {
+    const int n0 = seg * P.ckpt, len = min(P.nsub, n0 + P.ckpt) - n0;
+    const float4* w = s.ckx + (size_t)seg * sN;
+    for (int j = len - 1; j >= 0; j--) substep_reverse(s, w, X, P, half * P.ckpt + j);
 }
 
 // One workgroup per instance at a time. An instance whose nodes fit in the dynamic LDS (lds_nodes float4) keeps its
@@ -552,27 +560,46 @@ __global__ void __launch_bounds__(1024) k_adjoint_forward(GtsAdjointView A, AdjS
     }
 }
 
-// The backward stages positions and position adjoints (2 N float4) in LDS when they fit.
-__global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_backward(GtsAdjointView A, AdjStep P) {
+// Taped re-run of one segment of every instance (positions in LDS when they fit). It touches the forward state
+// (positions, prev, velocities, multipliers) and the tapes of one half only, so it can run beside the reverse of the
+// following segment, which reads the other half.
+__global__ void __launch_bounds__(1024) k_adjoint_tape(GtsAdjointView A, AdjStep P, int seg, int half) {
     extern __shared__ float4 lds[];
     const GtsBatchView& B = A.batch;
     const size_t sN = A.total_nodes;
     for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
         Inst s = bind(A, P, B.work_items[w]);
+        if (s.I.n_nodes <= P.lds_nodes) tape_segment(s, LdsNodes{lds}, P, sN, seg, half);
+        else tape_segment(s, GlobalNodes{s.pos}, P, sN, seg, half);
+        __syncthreads();
+    }
+}
+
+// Reverse of one segment of every instance (position adjoints in LDS when they fit). The last segment starts the
+// alpha adjoints at zero; segment 0 turns them into the compliance gradient.
+__global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_reverse(GtsAdjointView A, AdjStep P,
+                                                                                       int seg, int half) {
+    extern __shared__ float4 lds[];
+    const GtsBatchView& B = A.batch;
+    const size_t sN = A.total_nodes;
+    const bool first = seg == P.nckpt - 1, last = seg == 0;
+    for (int w = B.work_offsets[blockIdx.x]; w < B.work_offsets[blockIdx.x + 1]; w++) {
+        Inst s = bind(A, P, B.work_items[w]);
         const int N = s.I.n_nodes;
-        for (int k = threadIdx.x; k < 2 * s.I.n_tets; k += blockDim.x) s.abar[k] = 0.f;
-        if (2 * N <= P.lds_nodes) {
-            float4* xl = lds + N;
-            for (int i = threadIdx.x; i < N; i += blockDim.x) xl[i] = s.xb[i];
+        if (first)
+            for (int k = threadIdx.x; k < 2 * s.I.n_tets; k += blockDim.x) s.abar[k] = 0.f;
+        if (N <= P.lds_nodes) {
+            for (int i = threadIdx.x; i < N; i += blockDim.x) lds[i] = s.xb[i];
             __syncthreads();
-            backward_run(s, LdsNodes{lds}, LdsNodes{xl}, P, sN);
-            for (int i = threadIdx.x; i < N; i += blockDim.x) s.xb[i] = xl[i];
+            reverse_segment(s, LdsNodes{lds}, P, sN, seg, half);
+            for (int i = threadIdx.x; i < N; i += blockDim.x) s.xb[i] = lds[i];
         } else {
             __syncthreads();
-            backward_run(s, GlobalNodes{s.pos}, GlobalNodes{s.xb}, P, sN);
+            reverse_segment(s, GlobalNodes{s.xb}, P, sN, seg, half);
         }
-        for (int k = threadIdx.x; k < s.I.n_tets; k += blockDim.x)
-            s.grad[k] = (s.abar[k] + s.abar[s.I.n_tets + k] / 10.f) * P.invH2;
+        if (last)
+            for (int k = threadIdx.x; k < s.I.n_tets; k += blockDim.x)
+                s.grad[k] = (s.abar[k] + s.abar[s.I.n_tets + k] / 10.f) * P.invH2;
         __syncthreads();
     }
 }
@@ -610,17 +637,43 @@ int lds_bytes_max() {
     return bytes;
 }
 
-template <class Kernel>
-int launch(Kernel kern, const GtsAdjointView* A, AdjStep& P, int threads, hipStream_t stream) {
+template <class Kernel, class... Extra>
+int launch(Kernel kern, const GtsAdjointView* A, AdjStep& P, int threads, hipStream_t stream, Extra... extra) {
     size_t lds = (size_t)A->batch.lds_nodes * sizeof(float4);
     if (lds > (size_t)lds_bytes_max()) return GTS_ERR_BAD_ARG;
     P.lds_nodes = A->batch.lds_nodes;
     hipError_t e =
         hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -(int)e;
-    kern<<<A->batch.n_workgroups, threads, lds, stream>>>(*A, P);
+    kern<<<A->batch.n_workgroups, threads, lds, stream>>>(*A, P, extra...);
     return -(int)hipGetLastError();
 }
+
+// The side stream and events of the backward pipeline, created on first use (default priority).
+struct Pipeline {
+    hipStream_t side = nullptr;
+    hipEvent_t start = nullptr, taped[2] = {nullptr, nullptr}, reversed[2] = {nullptr, nullptr};
+};
+Pipeline g_pipe;
+
+int pipeline_ready() {
+    if (g_pipe.side) return GTS_OK;
+    hipError_t e = hipStreamCreateWithFlags(&g_pipe.side, hipStreamNonBlocking);
+    if (e != hipSuccess) return -(int)e;
+    hipEvent_t* all[5] = {&g_pipe.start, &g_pipe.taped[0], &g_pipe.taped[1], &g_pipe.reversed[0],
+                          &g_pipe.reversed[1]};
+    for (hipEvent_t* ev : all) {
+        e = hipEventCreateWithFlags(ev, hipEventDisableTiming);
+        if (e != hipSuccess) return -(int)e;
+    }
+    return GTS_OK;
+}
+
+#define ADJ_TRY(call)                       \
+    do {                                    \
+        hipError_t e_ = (call);             \
+        if (e_ != hipSuccess) return -(int)e_; \
+    } while (0)
 
 }  // namespace
 
@@ -632,12 +685,38 @@ extern "C" int gts_adjoint_forward(const GtsAdjointView* A, const GtsStepParams*
     return launch(k_adjoint_forward, A, P, A->batch.block_size, reinterpret_cast<hipStream_t>(stream));
 }
 
+// The segments are reversed last to first on the caller's stream. The taped re-run of segment seg - 1 (into the
+// other tape half) runs on a side stream while segment seg is reversed; events order each re-run after the reverse
+// that last read its half and each reverse after its re-run.
 extern "C" int gts_adjoint_backward(const GtsAdjointView* A, const GtsStepParams* p, int nsub, int ckpt, void* stream) {
     if (check_view(A)) return GTS_ERR_BAD_ARG;
     AdjStep P;
     int rc = plan_step(p, nsub, ckpt, P);
     if (rc) return rc;
+    rc = pipeline_ready();
+    if (rc) return rc;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream), side = g_pipe.side;
     const int threads = A->batch.block_size < GTS_ADJOINT_BACKWARD_MAX_THREADS ? A->batch.block_size
                                                                                 : GTS_ADJOINT_BACKWARD_MAX_THREADS;
-    return launch(k_adjoint_backward, A, P, threads, reinterpret_cast<hipStream_t>(stream));
+    const int last = P.nckpt - 1;
+    ADJ_TRY(hipEventRecord(g_pipe.start, s));
+    ADJ_TRY(hipStreamWaitEvent(side, g_pipe.start, 0));
+    rc = launch(k_adjoint_tape, A, P, A->batch.block_size, side, last, last & 1);
+    if (rc) return rc;
+    ADJ_TRY(hipEventRecord(g_pipe.taped[last & 1], side));
+    for (int seg = last; seg >= 0; seg--) {
+        const int half = seg & 1;
+        ADJ_TRY(hipStreamWaitEvent(s, g_pipe.taped[half], 0));
+        if (seg > 0) {
+            const int other = (seg - 1) & 1;
+            if (seg + 1 <= last) ADJ_TRY(hipStreamWaitEvent(side, g_pipe.reversed[other], 0));
+            rc = launch(k_adjoint_tape, A, P, A->batch.block_size, side, seg - 1, other);
+            if (rc) return rc;
+            ADJ_TRY(hipEventRecord(g_pipe.taped[other], side));
+        }
+        rc = launch(k_adjoint_reverse, A, P, threads, s, seg, half);
+        if (rc) return rc;
+        ADJ_TRY(hipEventRecord(g_pipe.reversed[half], s));
+    }
+    return GTS_OK;
 }

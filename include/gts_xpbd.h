@@ -239,7 +239,7 @@ int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream
  * genesis/engine/solvers/pbd_solver.py:784-815). The forward is the undamped Neo-Hookean substep with p->iterations
  * Gauss-Seidel iterations, a per-node force field and no contact or cutting; it runs on the pooled instances of a
  * GtsBatchView (one workgroup per instance, LPT-packed), whose lambda_edge / lambda_tet hold the multipliers within a
- * substep. Tape slots: ckpt * iterations. */
+ * substep. Tape slots: 2 * ckpt * iterations (two halves of ckpt * iterations). */
 typedef struct GtsAdjointView {
     GtsBatchView batch;           /* pooled instances and launch shape; k_dev and k_hyd are not used */
     long long total_nodes;        /* sum of n_nodes: row stride of the checkpoints */
@@ -266,8 +266,11 @@ typedef struct GtsAdjointView {
  * ckpt-th substep; the end state is left in batch.pos / batch.vel. */
 int gts_adjoint_forward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
 /* Device: reverse pass over the checkpoints of the last gts_adjoint_forward with the same compliance, nsub, ckpt and
- * iterations; consumes xbar / vbar and writes grad. Leaves batch.pos / batch.vel at the end of the first segment.
- * Runs with min(batch.block_size, GTS_ADJOINT_BACKWARD_MAX_THREADS) threads per workgroup. */
+ * iterations; consumes xbar / vbar and writes grad. Segments are reversed last to first on `stream`; the taped re-run
+ * of each earlier segment runs on a library-owned side stream (default priority) meanwhile, ordered by events, and
+ * all of it is complete when the work on `stream` is. Leaves batch.pos / batch.vel at the end of the first segment.
+ * The reverse runs with min(batch.block_size, GTS_ADJOINT_BACKWARD_MAX_THREADS) threads per workgroup; a block is
+ * staged in LDS when it has at most batch.lds_nodes nodes. Backward calls must not overlap one another. */
 #define GTS_ADJOINT_BACKWARD_MAX_THREADS 512
 int gts_adjoint_backward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
 

@@ -178,3 +178,50 @@ def test_ccd_cut_parity():
     co, ct = o.cut_state(), t.cut_state()
     assert co["revision"] > 0
     assert_cut_equal(ct, co)
+
+
+def test_cutting_everything_and_compacting_to_nothing():
+    """Horizontal blades through every half-cell layer of the default lattice (the reference's box-and-tab scene,
+    with its air shell, so every tissue link has a cut record) sever every vertical link: every tet dies, the
+    compaction leaves an empty tet list (and the horizontal edges), and stepping the remains (free nodes hanging on
+    edges, pinned ones still) goes on matching the oracle, which never compacts. A blade that misses the tissue
+    changes nothing."""
+    from genesis_tissue_simulation_amd.grid import build_grid
+    from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
+    from oracle.oracle import Oracle
+
+    grid = build_grid(0.6)
+    T = tet_count(grid)
+    w = block_6144()
+    params = w.params
+    kd = np.full(T, 1e7, dtype=np.float32)
+    o = Oracle(grid, params, grid.positions)
+    o.set_tet_stiffness(kd, kd * 10)
+    o.set_gravity(w.gravity)
+    t = Tissue(grid, params, kd, kd * 10, grid.positions)
+    t.set_gravity(w.gravity)
+    act = grid.active != 0
+    lo, hi = grid.positions[act].min(axis=0), grid.positions[act].max(axis=0)
+    L = float(grid.cell_size[1])
+    miss = swept_cut_tool(((lo[0] - 5, hi[1] + 5, lo[2]), (lo[0] - 5, hi[1] + 5, hi[2])),
+                          ((hi[0] + 5, hi[1] + 5, lo[2]), (hi[0] + 5, hi[1] + 5, hi[2])), 0.02)
+    for sim in (o, t):
+        sim.detect_cut(miss)
+    assert t.revision() == 0 and t.compact(apply=True)[3][0] == T
+    for y in np.arange(lo[1] + 0.5 * L, hi[1], L):
+        cut = swept_cut_tool(((lo[0] - L, y, lo[2] - L), (lo[0] - L, y, hi[2] + L)),
+                             ((hi[0] + L, y, lo[2] - L), (hi[0] + L, y, hi[2] + L)), 0.02)
+        for sim in (o, t):
+            sim.detect_cut(cut)
+    assert_cut_equal(t.cut_state(), o.cut_state())
+    tr, er, nr, counts = t.compact(apply=True)
+    _, _, _, ocounts = o.compact()
+    np.testing.assert_array_equal(counts, ocounts)
+    assert counts[0] == 0 and t.T == 0 and counts[1] > 0
+    for _ in range(3):
+        for sim in (o, t):
+            sim.step(w.dt)
+    np.testing.assert_array_equal(t.positions(), o.state()[0])
+    for a, b in zip(t.liveness(), o.liveness()):
+        np.testing.assert_array_equal(a, b)
+    assert t.compact(apply=True)[3][0] == 0
