@@ -514,11 +514,15 @@ __device__ void load_state(const Inst& s, const Store& S, const float4* xs, cons
     __syncthreads();
 }
 
+// The rollout, keeping a checkpoint before every ckpt-th substep. The last segment is taped on the way (into the
+// tape half the backward reverses first), so the backward starts with its reverse.
 template <class Store>
 __device__ void forward_run(const Inst& s, const Store& S, const AdjStep& P, size_t sN) {
+    const int last = P.nckpt - 1, n0 = last * P.ckpt, half = last & 1;
     for (int n = 0; n < P.nsub; n++) {
         if (n % P.ckpt == 0) save_state(s, S, s.ckx + (size_t)(n / P.ckpt) * sN, s.ckv + (size_t)(n / P.ckpt) * sN);
-        substep_forward<false>(s, S, P, 0);
+        if (n >= n0) substep_forward<true>(s, S, P, half * P.ckpt + (n - n0));
+        else substep_forward<false>(s, S, P, 0);
     }
 }
 
@@ -685,9 +689,9 @@ extern "C" int gts_adjoint_forward(const GtsAdjointView* A, const GtsStepParams*
     return launch(k_adjoint_forward, A, P, A->batch.block_size, reinterpret_cast<hipStream_t>(stream));
 }
 
-// The segments are reversed last to first on the caller's stream. The taped re-run of segment seg - 1 (into the
-// other tape half) runs on a side stream while segment seg is reversed; events order each re-run after the reverse
-// that last read its half and each reverse after its re-run.
+// The segments are reversed last to first on the caller's stream; the last one was taped by the forward. The taped
+// re-run of segment seg - 1 (into the other tape half) runs on a side stream while segment seg is reversed; events
+// order each re-run after the reverse that last read its half and each reverse after its re-run.
 extern "C" int gts_adjoint_backward(const GtsAdjointView* A, const GtsStepParams* p, int nsub, int ckpt, void* stream) {
     if (check_view(A)) return GTS_ERR_BAD_ARG;
     AdjStep P;
@@ -699,14 +703,12 @@ extern "C" int gts_adjoint_backward(const GtsAdjointView* A, const GtsStepParams
     const int threads = A->batch.block_size < GTS_ADJOINT_BACKWARD_MAX_THREADS ? A->batch.block_size
                                                                                 : GTS_ADJOINT_BACKWARD_MAX_THREADS;
     const int last = P.nckpt - 1;
+    // the forward taped the last segment; every later re-run follows the work already on `stream`
     ADJ_TRY(hipEventRecord(g_pipe.start, s));
     ADJ_TRY(hipStreamWaitEvent(side, g_pipe.start, 0));
-    rc = launch(k_adjoint_tape, A, P, A->batch.block_size, side, last, last & 1);
-    if (rc) return rc;
-    ADJ_TRY(hipEventRecord(g_pipe.taped[last & 1], side));
     for (int seg = last; seg >= 0; seg--) {
         const int half = seg & 1;
-        ADJ_TRY(hipStreamWaitEvent(s, g_pipe.taped[half], 0));
+        if (seg < last) ADJ_TRY(hipStreamWaitEvent(s, g_pipe.taped[half], 0));
         if (seg > 0) {
             const int other = (seg - 1) & 1;
             if (seg + 1 <= last) ADJ_TRY(hipStreamWaitEvent(side, g_pipe.reversed[other], 0));

@@ -263,7 +263,7 @@ typedef struct GtsAdjointView {
 } GtsAdjointView;
 
 /* Device: `nsub` substeps of every instance from the state in batch.pos / batch.vel, storing (x, v) before every
- * ckpt-th substep; the end state is left in batch.pos / batch.vel. */
+ * ckpt-th substep and taping the last segment for the backward; the end state is left in batch.pos / batch.vel. */
 int gts_adjoint_forward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
 /* Device: reverse pass over the checkpoints of the last gts_adjoint_forward with the same compliance, nsub, ckpt and
  * iterations; consumes xbar / vbar and writes grad. Segments are reversed last to first on `stream`; the taped re-run
