@@ -4,8 +4,8 @@
 // instances advances together. Each workgroup takes a list of instances (longest-processing-time packing on the
 // host) and runs every substep of a step for one instance at a time:
 //   * the instance's node positions (float4: x, y, z, inverse mass) are staged in LDS when they fit, so all
-//     Gauss-Seidel colour passes of all substeps gather and scatter on-chip; larger instances work on their own
-//     slice of global memory, which only this workgroup touches;
+//     Gauss-Seidel colour passes of all substeps gather and scatter on-chip; a larger instance stages as many of its
+//     nodes as fit and keeps the rest in its own slice of global memory, which only this workgroup touches;
 //   * colours are separated by workgroup barriers instead of kernel launches: the colour order is the reference's
 //     (xpbd/xpbd_solver.cu:3581-3612), so each instance computes exactly what the single-instance path computes;
 //   * HBM traffic per substep is the packed edge / tet descriptors and per-tet stiffness (8 B per tet, 4 B per edge);
@@ -175,16 +175,19 @@ __global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBat
         float* le = lam_edge + I.edge_off;
         float* lt = lam_tet + 2 * (size_t)I.tet_off;
         float* al = alpha + 2 * (size_t)I.tet_off;
-        if (I.n_nodes <= P.lds_nodes) {
-            for (int i = threadIdx.x; i < I.n_nodes; i += blockDim.x) lds_nodes[i] = gpos[i];
-            __syncthreads();
+        const int staged = min(I.n_nodes, P.lds_nodes);
+        for (int i = threadIdx.x; i < staged; i += blockDim.x) lds_nodes[i] = gpos[i];
+        __syncthreads();
+        if (I.n_nodes <= P.lds_nodes)
             run_instance<MODE, DAMPED>(LdsNodes{lds_nodes}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass, gprev,
                                        gvel);
-            for (int i = threadIdx.x; i < I.n_nodes; i += blockDim.x) gpos[i] = lds_nodes[i];
-        } else {
+        else if (staged > 0)
+            run_instance<MODE, DAMPED>(SplitNodes{lds_nodes, gpos, staged}, I, P, colours, ed, td, kd, kh, le, lt, al,
+                                       gmass, gprev, gvel);
+        else
             run_instance<MODE, DAMPED>(GlobalNodes{gpos}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass, gprev,
                                        gvel);
-        }
+        for (int i = threadIdx.x; i < staged; i += blockDim.x) gpos[i] = lds_nodes[i];
         __syncthreads();
     }
 }

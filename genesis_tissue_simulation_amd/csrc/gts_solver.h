@@ -2,7 +2,7 @@
 // instance and the persistent batched and adjoint kernels, so that every path rounds identically.
 //
 // Node storage is abstracted by a store type with ld(i) / st(i, v) on float4 (x, y, z, inverse mass): LDS-staged for
-// instances that fit in a workgroup's LDS, global memory otherwise.
+// instances that fit in a workgroup's LDS, split between LDS and global memory for larger ones, or global memory.
 #pragma once
 #include "gts_common.h"
 
@@ -15,6 +15,18 @@ struct LdsNodes {
     float4* p;  // points into __shared__ memory
     __device__ __forceinline__ float4 ld(int i) const { return p[i]; }
     __device__ __forceinline__ void st(int i, float4 v) const { p[i] = v; }
+};
+// An instance larger than the LDS budget: its first `cap` nodes are staged in LDS, the rest stay in its global slice.
+// Nodes are numbered along the lattice (x fastest, z slowest), so the tets a wave handles mostly lie on one side.
+struct SplitNodes {
+    float4* lds;
+    float4* g;
+    int cap;
+    __device__ __forceinline__ float4 ld(int i) const { return i < cap ? lds[i] : g[i]; }
+    __device__ __forceinline__ void st(int i, float4 v) const {
+        if (i < cap) lds[i] = v;
+        else g[i] = v;
+    }
 };
 
 __device__ __forceinline__ f3 xyz4(float4 p) { return mk3(p.x, p.y, p.z); }
