@@ -88,7 +88,8 @@ class Batch:
         self.configure(block_size, lds_nodes, workgroups_per_cu)
 
     def configure(self, block_size=512, lds_nodes=None, workgroups_per_cu=1, n_workgroups=None):
-        """Launch shape: threads per workgroup, LDS node capacity (instances above it run from global memory) and
+        """Launch shape: threads per workgroup, LDS node capacity (a larger instance stages that many of its nodes and
+        keeps the rest in global memory) and
         workgroup count, with the instances LPT-packed onto the workgroups."""
         max_lds = int(self.lib.gts_batched_lds_bytes())
         if lds_nodes is None:

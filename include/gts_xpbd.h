@@ -211,7 +211,8 @@ typedef struct GtsBatchView {
     const int32_t* colour_table;  /* device: per instance, n_edge_colours+1 edge then n_tet_colours+1 tet offsets */
     const int32_t* work_offsets;  /* device [n_workgroups + 1]: instances of each workgroup (gts_batched_plan) */
     const int32_t* work_items;    /* device [n_instances] */
-    int n_workgroups, block_size, lds_nodes; /* launch shape; instances with <= lds_nodes nodes are staged in LDS */
+    int n_workgroups, block_size, lds_nodes; /* launch shape; the first lds_nodes nodes of an instance are staged in LDS
+                                              * (all of them when it has no more) */
     int use_graph;
     float gravity[3];
     float* pos;                   /* [sum N] float4 rows as in GtsTissueView */
