@@ -326,7 +326,7 @@ def run_cutting_block(steps, warmup, adjoint_count=128):
         t.step(w.dt)
         for bl in all_blades(w):
             t.detect_cut(swept_cut_tool(bl(s), bl(s + 1), w.blade_thickness))
-        t.compact(apply=True)
+        t.compact(apply=True, readback=False)
 
     def gradient():
         x = fit.forward_raw(c)

@@ -966,10 +966,23 @@ std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) 
     return k;
 }
 
+// Views seen once and not captured yet.
+std::vector<std::vector<unsigned char>> g_seen;
+
+// A (view, plan) pair is captured on its second use: the first step of it runs as direct launches. Capturing pays
+// only for views that recur; a tissue whose lists are compacted after every step presents a new view every step.
 int launch_step_graph(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hipStream_t s) {
     std::vector<unsigned char> key = graph_key(v, P);
     for (auto& e : g_graphs)
         if (e.key == key) return -(int)hipGraphLaunch(e.exec, s);
+    auto seen = std::find(g_seen.begin(), g_seen.end(), key);
+    if (seen == g_seen.end()) {
+        if (g_seen.size() >= 16) g_seen.erase(g_seen.begin());
+        g_seen.push_back(key);
+        enqueue_step(v, L, P, s);
+        return -(int)hipGetLastError();
+    }
+    g_seen.erase(seen);
     if (!g_capture) HIP_TRY(hipStreamCreateWithFlags(&g_capture, hipStreamNonBlocking));
     HIP_TRY(hipStreamBeginCapture(g_capture, hipStreamCaptureModeThreadLocal));
     enqueue_step(v, L, P, g_capture);

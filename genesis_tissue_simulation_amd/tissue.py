@@ -121,10 +121,10 @@ class Tissue:
         # current tet slot -> generation index and current edge slot -> original colour-order slot (compaction
         # removes dead entries; the readbacks report over the original lists)
         self.T0, self.E0 = T, E
-        self.tet_slot_gen = np.asarray(topo["tet_gen"], dtype=np.int64)
-        self.edge_slot_orig = np.arange(E, dtype=np.int64)
-        self.pooled = storage is not None
         dev = self.device
+        self.tet_slot_gen = torch.as_tensor(np.asarray(topo["tet_gen"], dtype=np.int64)).to(dev)
+        self.edge_slot_orig = torch.arange(E, dtype=torch.int64, device=dev)
+        self.pooled = storage is not None
 
         def f32(a, shape=None):
             x = torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32))
@@ -272,17 +272,18 @@ class Tissue:
     def liveness(self):
         """Tet liveness in generation order and edge liveness in colour order over the original (uncompacted) lists,
         as the CPU checker reports them; removed entries read 0."""
-        tet = np.zeros(self.T0, dtype=np.int32)
-        tet[self.tet_slot_gen] = self.tet_desc[:self.T].cpu().numpy() & 1
-        edge = np.zeros(self.E0, dtype=np.int32)
-        edge[self.edge_slot_orig] = self.edge_desc[:self.E].cpu().numpy() & 1
-        return tet, edge
+        tet = torch.zeros(max(self.T0, 1), dtype=torch.int32, device=self.device)
+        tet[self.tet_slot_gen] = self.tet_desc[:self.T] & 1
+        edge = torch.zeros(max(self.E0, 1), dtype=torch.int32, device=self.device)
+        edge[self.edge_slot_orig] = self.edge_desc[:self.E] & 1
+        return tet[:self.T0].cpu().numpy(), edge[:self.E0].cpu().numpy()
 
-    def compact(self, apply=False):
+    def compact(self, apply=False, readback=True):
         """Stable remap of the live topology: (tet_remap in colour order, edge_remap, node_remap, counts), relative
         to the current lists. With `apply`, the dead tets and edges are removed: the live ones move to their remapped
         slots, the colour ranges shrink to their live counts and every later step and cut runs over the live lists
-        only (node storage keeps its lattice addressing)."""
+        only (node storage keeps its lattice addressing). With readback=False only the counts come back to the host
+        (the remaps stay on the device, as the first three entries of the returned tuple)."""
         dev = self.device
         tr = torch.empty(max(self.T, 1), dtype=torch.int32, device=dev)
         er = torch.empty(max(self.E, 1), dtype=torch.int32, device=dev)
@@ -291,15 +292,19 @@ class Tissue:
         used = torch.empty(self.N, dtype=torch.int32, device=dev)
         check("gts_compact_topology", self.lib.gts_compact_topology(
             ct.byref(self.view), _ptr(tr), _ptr(er), _ptr(nr), _ptr(counts), _ptr(used), current_stream()))
-        out = (tr[:self.T].cpu().numpy(), er[:self.E].cpu().numpy(), nr.cpu().numpy(), counts.cpu().numpy())
+        T, E = self.T, self.E
+        counts_h = counts.cpu().numpy()
+        out = ((tr[:T].cpu().numpy(), er[:E].cpu().numpy(), nr.cpu().numpy(), counts_h) if readback else
+               (tr[:T], er[:E], nr, counts_h))
         if apply:
-            self._apply_compaction(tr, er, out[0], out[1], int(out[3][0]), int(out[3][1]))
+            self._apply_compaction(tr, er, int(counts_h[0]), int(counts_h[1]))
         return out
 
-    def _apply_compaction(self, tr, er, tr_host, er_host, new_t, new_e):
+    def _apply_compaction(self, tr, er, new_t, new_e):
         if self.pooled:
             raise ValueError("a tissue in a batch's pooled buffers cannot be compacted in place")
         dev = self.device
+        T, E = self.T, self.E
         tet_desc = torch.zeros(max(new_t, 1), dtype=torch.int32, device=dev)
         k_dev = torch.zeros(max(new_t, 1), dtype=torch.float32, device=dev)
         k_hyd = torch.zeros(max(new_t, 1), dtype=torch.float32, device=dev)
@@ -307,11 +312,15 @@ class Tissue:
         check("gts_apply_compaction", self.lib.gts_apply_compaction(
             ct.byref(self.view), _ptr(tr), _ptr(er), _ptr(tet_desc), _ptr(k_dev), _ptr(k_hyd), _ptr(edge_desc),
             current_stream()))
-        live_t, live_e = tr_host >= 0, er_host >= 0
-        kept_t = np.concatenate(([0], np.cumsum(live_t)))
-        kept_e = np.concatenate(([0], np.cumsum(live_e)))
-        self.tet_offsets = np.ascontiguousarray(kept_t[self.tet_offsets], dtype=np.int32)
-        self.edge_offsets = np.ascontiguousarray(kept_e[self.edge_offsets], dtype=np.int32)
+        # live count before each colour boundary, and the readback order of the survivors, on the device
+        live_t, live_e = tr[:T] >= 0, er[:E] >= 0
+        kept_t = torch.cat((torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(live_t, 0)))
+        kept_e = torch.cat((torch.zeros(1, dtype=torch.int64, device=dev), torch.cumsum(live_e, 0)))
+        bounds = torch.cat((kept_t[torch.as_tensor(self.tet_offsets, dtype=torch.int64, device=dev)],
+                            kept_e[torch.as_tensor(self.edge_offsets, dtype=torch.int64, device=dev)])).cpu().numpy()
+        nt = len(self.tet_offsets)
+        self.tet_offsets = np.ascontiguousarray(bounds[:nt], dtype=np.int32)
+        self.edge_offsets = np.ascontiguousarray(bounds[nt:], dtype=np.int32)
         self.tet_slot_gen = self.tet_slot_gen[live_t]
         self.edge_slot_orig = self.edge_slot_orig[live_e]
         self.tet_desc, self.tet_k_dev, self.tet_k_hyd, self.edge_desc = tet_desc, k_dev, k_hyd, edge_desc
