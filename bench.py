@@ -117,18 +117,26 @@ def run_gpu(w, steps, warmup):
 
     import torch
 
+    from genesis_tissue_simulation_amd import autotune
     from genesis_tissue_simulation_amd._lib import check
     from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
 
     kd, kh = w.stiffness()
-    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions)
+
+    def advance(tissue, step):
+        tissue.set_gravity(w.gravity_at(step))
+        tool = w.tool(step) if w.tool else None
+        tissue.step(w.dt, tool=tool, tool_armed=tool is not None)
+        if w.blade:
+            tissue.detect_cut(swept_cut_tool(w.blade(step), w.blade(step + 1), w.blade_thickness))
+
+    # colour-kernel workgroup size, tuned on scratch copies of the tissue
+    best, table = autotune.tune_plain(lambda cb: Tissue(w.grid, w.params, kd, kh, w.initial_positions,
+                                                        colour_block=cb), advance)
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, colour_block=best["colour_block"])
 
     def one(step):
-        t.set_gravity(w.gravity_at(step))
-        tool = w.tool(step) if w.tool else None
-        t.step(w.dt, tool=tool, tool_armed=tool is not None)
-        if w.blade:
-            t.detect_cut(swept_cut_tool(w.blade(step), w.blade(step + 1), w.blade_thickness))
+        advance(t, step)
 
     for s in range(warmup):
         one(s)
@@ -154,7 +162,8 @@ def run_gpu(w, steps, warmup):
     if not np.isfinite(pos).all():
         raise RuntimeError("non-finite positions: simulation stopped")
     return dict(elapsed=elapsed, live=(live0 + live1) / 2, tet_ms=total_ms.value, launches=launches.value,
-                units=units.value, E=t.E, T=t.T, active=int((w.grid.active != 0).sum()))
+                units=units.value, E=t.E, T=t.T, active=int((w.grid.active != 0).sum()),
+                colour_block=best["colour_block"], tune=table)
 
 
 def run_batched(steps, warmup, count=1024):
@@ -534,7 +543,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": "kidney-250k plain single instance", "tets": g["T"], "live_tets": g["live"],
                    "active_nodes": g["active"], "substeps": substeps, "iterations": w.params.iterations,
-                   "dt": w.dt, "contact": "capsule grasper", "cut": "swept-quad blade, once per step"},
+                   "dt": w.dt, "contact": "capsule grasper", "cut": "swept-quad blade, once per step",
+                   "colour_block": g["colour_block"],
+                   "autotune": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                                for r in g["tune"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "stream_copy_GBs": copy_gbs,
@@ -555,7 +567,7 @@ def main():
             "config": {"workload": f"{bt['count']} blocks, tets log-uniform 2k-100k, one straight cut each",
                        "tets": bt["T"], "live_tets": bt["live"], "substeps": bt["substeps"], "iterations": 1,
                        "launch": {k: bt["best"][k] for k in ("block_size", "workgroups_per_cu", "lds_nodes",
-                                                               "n_workgroups")}},
+                                                               "n_workgroups", "substeps_per_launch")}},
             "roofline": {"bound": "hbm", "achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": bach / HBM_PEAK_GBS, "kernel": "k_batched_step (one launch per step)",
                          "avg_launch_us": 1e3 * bt["kernel_ms"] / args.batched_steps, "bytes_per_tet_substep": bb,

@@ -73,7 +73,8 @@ class GtsBatchInstance(ct.Structure):
 
 class GtsBatchView(ct.Structure):
     _fields_ = [("n_instances", I), ("instances", P), ("colour_table", P), ("work_offsets", P), ("work_items", P),
-                ("n_workgroups", I), ("block_size", I), ("lds_nodes", I), ("use_graph", I), ("gravity", F * 3),
+                ("n_workgroups", I), ("block_size", I), ("lds_nodes", I), ("use_graph", I),
+                ("substeps_per_launch", I), ("gravity", F * 3),
                 ("pos", P), ("prev", P), ("vel", P), ("mass", P), ("edge_desc", P), ("tet_desc", P), ("k_dev", P),
                 ("k_hyd", P), ("lambda_edge", P), ("lambda_tet", P), ("alpha", P)]
 

@@ -1,5 +1,7 @@
-"""Autotuner of the batched persistent launch: threads per workgroup, LDS budget per workgroup (which sets how many
-workgroups share a CU and which instances are staged in LDS) and workgroup count, measured on the actual batch.
+"""Autotuners. The batched persistent launch: threads per workgroup, LDS budget per workgroup (which sets how many
+workgroups share a CU and how much of each instance is staged in LDS), workgroup count and substeps (colours) per
+launch, measured on the actual batch. The single-instance step: threads per workgroup of the colour kernels,
+measured on a scratch copy of the tissue.
 
 Each candidate re-plans the LPT packing for its workgroup count, runs a few untimed steps (which also capture its
 step graph), then times steps between device synchronisations. The state the tuning steps advance is restored
@@ -13,13 +15,17 @@ import torch
 
 
 def candidates(max_lds_bytes):
+    """Threads per workgroup x workgroups per CU (which splits the LDS) with the whole step in one launch, and the
+    best-guess shape with one launch per substep (the colours of a substep per launch)."""
     out = []
     for wpc in (1, 2, 4):
         lds_nodes = max_lds_bytes // 16 // wpc
         for block in (256, 512, 1024):
             if block * wpc > 2048:
                 continue
-            out.append(dict(block_size=block, workgroups_per_cu=wpc, lds_nodes=lds_nodes))
+            out.append(dict(block_size=block, workgroups_per_cu=wpc, lds_nodes=lds_nodes, substeps_per_launch=0))
+    for block in (512, 1024):
+        out.append(dict(block_size=block, workgroups_per_cu=1, lds_nodes=max_lds_bytes // 16, substeps_per_launch=1))
     return out
 
 
@@ -29,7 +35,8 @@ def tune(batch, dt, steps=3, warmup=1, space=None):
     space = space or candidates(int(batch.lib.gts_batched_lds_bytes()))
     table = []
     for cand in space:
-        batch.configure(cand["block_size"], cand["lds_nodes"], cand["workgroups_per_cu"])
+        batch.configure(cand["block_size"], cand["lds_nodes"], cand["workgroups_per_cu"],
+                        substeps_per_launch=cand["substeps_per_launch"])
         for _ in range(warmup):
             batch.step(dt)
         torch.cuda.synchronize()
@@ -42,6 +49,25 @@ def tune(batch, dt, steps=3, warmup=1, space=None):
         for name, t in saved.items():
             getattr(batch, name).copy_(t)
     best = min(table, key=lambda r: r["ms_per_step"])
-    batch.configure(best["block_size"], best["lds_nodes"], best["workgroups_per_cu"])
+    batch.configure(best["block_size"], best["lds_nodes"], best["workgroups_per_cu"],
+                    substeps_per_launch=best["substeps_per_launch"])
     torch.cuda.synchronize()
     return best, table
+
+
+def tune_plain(make_tissue, step, steps=3, warmup=2, blocks=(64, 128, 256)):
+    """Colour-kernel workgroup size of the single-instance step: each candidate runs `warmup` untimed and `steps`
+    timed steps of `step(tissue, k)` on a fresh tissue from `make_tissue(colour_block)`. Returns (best, table)."""
+    table = []
+    for cb in blocks:
+        t = make_tissue(cb)
+        for k in range(warmup):
+            step(t, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(warmup, warmup + steps):
+            step(t, k)
+        torch.cuda.synchronize()
+        table.append(dict(colour_block=cb, ms_per_step=1e3 * (time.perf_counter() - t0) / steps))
+        del t
+    return min(table, key=lambda r: r["ms_per_step"]), table

@@ -87,9 +87,10 @@ class Batch:
         self.use_graph = use_graph
         self.configure(block_size, lds_nodes, workgroups_per_cu)
 
-    def configure(self, block_size=512, lds_nodes=None, workgroups_per_cu=1, n_workgroups=None):
+    def configure(self, block_size=512, lds_nodes=None, workgroups_per_cu=1, n_workgroups=None,
+                  substeps_per_launch=0):
         """Launch shape: threads per workgroup, LDS node capacity (a larger instance stages that many of its nodes and
-        keeps the rest in global memory) and
+        keeps the rest in global memory), substeps per persistent launch (0: the whole step) and
         workgroup count, with the instances LPT-packed onto the workgroups."""
         max_lds = int(self.lib.gts_batched_lds_bytes())
         if lds_nodes is None:
@@ -106,10 +107,11 @@ class Batch:
         self.work_offsets = torch.as_tensor(plan[:n_workgroups + 1].copy()).to(self.device)
         self.work_items = torch.as_tensor(plan[n_workgroups + 1:].copy()).to(self.device)
         self.block_size, self.lds_nodes, self.n_workgroups = int(block_size), int(lds_nodes), n_workgroups
+        self.substeps_per_launch = int(substeps_per_launch)
         self.view = GtsBatchView(
             len(self.tissues), self.instances.data_ptr(), self.colour_table.data_ptr(), self.work_offsets.data_ptr(),
             self.work_items.data_ptr(), n_workgroups, self.block_size, self.lds_nodes, int(bool(self.use_graph)),
-            (ct.c_float * 3)(*self.gravity), self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(),
+            self.substeps_per_launch, (ct.c_float * 3)(*self.gravity), self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(),
             self.mass.data_ptr(), self.edge_desc.data_ptr(), self.tet_desc.data_ptr(), self.k_dev.data_ptr(),
             self.k_hyd.data_ptr(), self.lambda_edge.data_ptr(), self.lambda_tet.data_ptr(), self.alpha.data_ptr())
         return self

@@ -39,6 +39,7 @@ struct TetInputs {
 
 struct BatchStep {
     int substeps, iters, mode, n_items;
+    int sub_begin, sub_end;  // the substeps this launch runs
     float h, invH, invH2, omega, velScale, gx, gy, gz, dOverH, bOverH;
     int lds_nodes;
 };
@@ -57,12 +58,14 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
     const float at = I.edge_ks > 0.f ? 1.f / (I.edge_ks * P.h * P.h) : 0.f;
     const float betaGammaH = 0.f;
     // k_ComputeTetAlpha (xpbd_solver.cu:578-587) once per step, as the single-instance path does per frame
-    for (int k = tid; k < T; k += bs) {
-        alpha[k] = P.invH2 / k_dev[k];
-        alpha[T + k] = P.invH2 / k_hyd[k];
+    if (P.sub_begin == 0) {
+        for (int k = tid; k < T; k += bs) {
+            alpha[k] = P.invH2 / k_dev[k];
+            alpha[T + k] = P.invH2 / k_hyd[k];
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int n = 0; n < P.substeps; n++) {
+    for (int n = P.sub_begin; n < P.sub_end; n++) {
         // k_predict (xpbd_solver.cu:393-412) with the gravity force g * mass of k_SetGravity
         for (int i = tid; i < N; i += bs) {
             float4 p = S.ld(i);
@@ -213,7 +216,7 @@ int launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
     return -(int)hipGetLastError();
 }
 
-int enqueue(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
+int launch_range(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
     if (damped) {
         if (P.mode == 0) return launch<0, true>(v, P, s);
         if (P.mode == 1) return launch<1, true>(v, P, s);
@@ -222,6 +225,18 @@ int enqueue(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t 
     if (P.mode == 0) return launch<0, false>(v, P, s);
     if (P.mode == 1) return launch<1, false>(v, P, s);
     return launch<2, false>(v, P, s);
+}
+
+// The step as persistent launches of substeps_per_launch substeps each (one launch when 0).
+int enqueue(const GtsBatchView* v, BatchStep P, bool damped, hipStream_t s) {
+    const int per = v->substeps_per_launch > 0 ? v->substeps_per_launch : P.substeps;
+    for (int b = 0; b < P.substeps; b += per) {
+        P.sub_begin = b;
+        P.sub_end = b + per < P.substeps ? b + per : P.substeps;
+        int rc = launch_range(v, P, damped, s);
+        if (rc) return rc;
+    }
+    return GTS_OK;
 }
 
 }  // namespace
@@ -265,6 +280,7 @@ extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, v
     if (!v || !p || v->n_workgroups <= 0 || v->block_size <= 0 || v->block_size > 1024) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
     if ((long long)v->lds_nodes * 16 > (long long)gts_batched_lds_bytes()) return GTS_ERR_BAD_ARG;
+    if (v->substeps_per_launch < 0) return GTS_ERR_BAD_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     BatchStep P;
     std::memset(&P, 0, sizeof(P));

@@ -214,6 +214,8 @@ typedef struct GtsBatchView {
     int n_workgroups, block_size, lds_nodes; /* launch shape; the first lds_nodes nodes of an instance are staged in LDS
                                               * (all of them when it has no more) */
     int use_graph;
+    int substeps_per_launch;      /* substeps each persistent launch covers (0 = the whole step in one launch): the
+                                   * colours per launch are this times the colours of a substep */
     float gravity[3];
     float* pos;                   /* [sum N] float4 rows as in GtsTissueView */
     float* prev;
@@ -232,7 +234,8 @@ typedef struct GtsBatchView {
 int gts_batched_plan(const double* cost, int n, int n_workgroups, int32_t* out);
 /* Host: largest dynamic LDS a workgroup may allocate on the current device. */
 int gts_batched_lds_bytes(void);
-/* Device: one outer step of every instance (p->substeps substeps each) in one persistent launch. */
+/* Device: one outer step of every instance (p->substeps substeps each) in persistent launches of
+ * substeps_per_launch substeps (one launch when 0). */
 int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream);
 
 /* ---- adjoint of the substep for per-tet compliance gradients ----

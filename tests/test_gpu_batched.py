@@ -27,8 +27,11 @@ def instances(seed=0, iterations=1):
     return out
 
 
-@pytest.mark.parametrize("lds_nodes,iterations", [(None, 1), (200, 1), (None, 2)])
-def test_batched_matches_oracle(lds_nodes, iterations):
+# lds_nodes 200 stages only part of every instance but the smallest; substeps_per_launch 3 splits the 10-substep step
+# into launches of 3, 3, 3 and 1 substeps.
+@pytest.mark.parametrize("lds_nodes,iterations,substeps_per_launch", [(None, 1, 0), (200, 1, 0), (None, 2, 0),
+                                                                      (200, 1, 3)])
+def test_batched_matches_oracle(lds_nodes, iterations, substeps_per_launch):
     from genesis_tissue_simulation_amd.batched import Batch
     from genesis_tissue_simulation_amd.tissue import swept_cut_tool
     from oracle.oracle import Oracle
@@ -36,6 +39,7 @@ def test_batched_matches_oracle(lds_nodes, iterations):
     insts = instances(iterations=iterations)
     base = insts[0][1]
     batch = Batch(insts, base, block_size=256, lds_nodes=lds_nodes)
+    batch.configure(256, lds_nodes, 1, substeps_per_launch=substeps_per_launch)
     oracles = []
     for g, p, kd, kh, init in insts:
         o = Oracle(g, p, init)
