@@ -66,6 +66,7 @@ __global__ void k_init_state(int N, const float* mass, const int* active, const 
         counters[1] = 0u;          // cut point counter
         counters[2] = 0xffffffffu; // revision the liveness was derived at: none yet
         counters[3] = 1u;          // stale
+        counters[4] = 0u;          // edges marked by the tear law
     }
     if (i >= N) return;
     // invMass = (pinned || !active) ? 0 : 1/mass (xpbd_solver.cu:458-459); active/pinned do not change on this path.
@@ -481,6 +482,48 @@ __global__ void k_cut_ccd(int Eg, Lattice L, f3 S, f3 E, f3 emitN, float pad, co
     }
 }
 
+// k_TearOverstretch (cut.cu:607-681): an intact link between active corners, not yet marked and with an occupied
+// stencil voxel, longer than tau times its rest length (tearLen2 scaled by |rest edge|^2 / L^2) is marked at its
+// midpoint with the plane normal to it. Each edge reads and sets only its own mask bits, so the edges are independent.
+// The grasp-ring exemption (cut.cu:655-673) is absent: only the legacy grasp fills the ring, and with an empty ring
+// neither of its branches can be taken. counters[4] counts the marked edges (the reference's dbg[15]).
+__global__ void k_tear(int Eg, Lattice L, float tearLen2, float voxelL, const int* __restrict__ grid_edges,
+                       const int* nbr, const int* active, const float4* __restrict__ pos, const float* __restrict__ rest,
+                       const uint32_t* occupied, uint32_t* cut_mask, float* edge_cut_t, const float* skin,
+                       const int* inside, float4* records, float4* normals, uint32_t* counters) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Eg) return;
+    records[e] = make_float4(-1.f, 0.f, 0.f, 0.f);
+    int4 ge = reinterpret_cast<const int4*>(grid_edges)[e];
+    int axis = ge.x;
+    int idA = ge.y + L.nx * ge.z + L.nxy * ge.w;
+    int idB = idA + axis_stride(L, axis);
+    if (nbr[6 * idA + 2 * axis + 1] < 0) return;
+    if (active[idA] == 0 || active[idB] == 0) return;
+    bool cut = false, occ = false;
+    for (int s = 0; s < 4; s++) {
+        int dx, dy, dz, local;
+        stencil(axis, s, dx, dy, dz, local);
+        int x = ge.y + dx, y = ge.z + dy, z = ge.w + dz;
+        if (x < 0 || y < 0 || z < 0 || x >= L.vx || y >= L.vy || z >= L.vz) continue;
+        int vid = x + L.vx * (y + L.vy * z);
+        if ((cut_mask[vid] >> local) & 1u) cut = true;
+        if (occupied[vid] != 0u) occ = true;
+    }
+    if (cut || !occ) return;
+    f3 A = xyz(pos[idA]), d = xyz(pos[idB]) - A;
+    float len2 = dot3(d, d);
+    f3 r = ld3(rest + 3 * idB) - ld3(rest + 3 * idA);
+    float limit = tearLen2 * (dot3(r, r) / (voxelL * voxelL));
+    if (len2 <= limit) return;
+    float len = sqrtf(len2);
+    f3 ncut = mk3(d.x / len, d.y / len, d.z / len);
+    f3 hit = A + d * 0.5f;
+    normals[e] = make_float4(ncut.x, ncut.y, ncut.z, 0.f);
+    atomicAdd(counters + 4, 1u);
+    mark_edge(e, axis, ge.y, ge.z, ge.w, idA, idB, 0.5f, hit, L, occupied, cut_mask, edge_cut_t, skin, inside, records);
+}
+
 // ---- ordered stable compaction: wave ballot + LDS prefix inside a block, exclusive scan of the block totals
 
 __device__ __forceinline__ int block_prefix(bool flag, int* lds, int& total) {
@@ -565,8 +608,8 @@ __global__ void k_write_remap(int n, Flag flag, const int* partial, int* remap) 
 // 2 * (rank of the edge among the emitting ones) for the owner A and + 1 for B; the particle rotation is identity.
 __global__ void k_emit_points(int Eg, Lattice L, f3 ncutQuad, int useNormals, float toolD, float voxelL,
                               const int* __restrict__ grid_edges, const float4* __restrict__ pos,
-                              const float4* records, const float4* normals, const int* partial, uint32_t* counters,
-                              float* points, int capacity) {
+                              const float* __restrict__ rest, const float4* records, const float4* normals,
+                              const int* partial, uint32_t* counters, float* points, int capacity) {
     __shared__ int lds[GTS_BLOCK / 64];
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     bool f = e < Eg && records[e].x >= 0.f;
@@ -583,8 +626,9 @@ __global__ void k_emit_points(int Eg, Lattice L, f3 ncutQuad, int useNormals, fl
         f3 hit = mk3(rec.y, rec.z, rec.w);
         f3 ncut = ncutQuad;
         if (useNormals) { float4 nn = normals[e]; ncut = mk3(nn.x, nn.y, nn.z); }
+        // rest edge from the corner rest positions when the view has them (materialRestDevice, cut.cu:248)
         f3 axisF = mk3(axis == 0 ? 1.f : 0.f, axis == 1 ? 1.f : 0.f, axis == 2 ? 1.f : 0.f);
-        f3 restEdge = axisF * voxelL;
+        f3 restEdge = rest ? ld3(rest + 3 * idB) - ld3(rest + 3 * idA) : axisF * voxelL;
         f3 along[2] = {restEdge * t, restEdge * (t - 1.f)};
         int owners[2] = {idA, idB};
         uint32_t slot0 = base + 2u * (uint32_t)(partial[blockIdx.x] + r);
@@ -820,9 +864,8 @@ int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut,
     k_count_flags<<<nb, GTS_BLOCK, 0, s>>>(Eg, FlagRecords{records}, partial);
     k_scan_partials<<<1, GTS_BLOCK, 0, s>>>(nb, partial);
     k_emit_points<<<nb, GTS_BLOCK, 0, s>>>(Eg, L, ncut, useNormals, toolD, v->voxel_length, v->grid_edges,
-                                           (const float4*)v->pos,
-                                           records, normals, partial, v->counters, v->cut_points,
-                                           v->cut_point_capacity);
+                                           (const float4*)v->pos, v->rest, records, normals, partial, v->counters,
+                                           v->cut_points, v->cut_point_capacity);
     k_bump_counter<<<1, 1, 0, s>>>(partial, nb, v->counters);
     k_sever<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, v->grid_edges, v->cut_mask, v->nbr, v->bend_alive, v->counters);
     return launch_refresh(v, L, s);
@@ -1096,6 +1139,24 @@ extern "C" int gts_detect_cut_ccd(const GtsTissueView* v, const float* Sb, const
                                                  records, normals);
     HIP_TRY(hipGetLastError());
     return finish_cut(v, L, s, ld3(emit_ncut), 1, tool_d);
+}
+
+extern "C" int gts_tear_overstretch(const GtsTissueView* v, float tau, float tool_d, void* stream) {
+    if (!v) return GTS_ERR_BAD_ARG;
+    if (!(tau >= 1.f) || v->n_grid_edges <= 0) return GTS_OK;  // tau < 1 or NaN: the law is off (cut.cu:1199-1202)
+    if (!v->rest) return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Lattice L = lattice_of(v);
+    int Eg = v->n_grid_edges;
+    float tl = tau * v->voxel_length;
+    float4* records = reinterpret_cast<float4*>(v->scratch);
+    float4* normals = records + Eg;
+    k_tear<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, tl * tl, v->voxel_length, v->grid_edges, v->nbr, v->active,
+                                              (const float4*)v->pos, v->rest, v->occupied, v->cut_mask,
+                                              v->edge_cut_t, v->edge_skin_t, v->inside, records, normals,
+                                              v->counters);
+    HIP_TRY(hipGetLastError());
+    return finish_cut(v, L, s, mk3(0.f, 0.f, 0.f), 1, tool_d);
 }
 
 extern "C" int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap,

@@ -173,6 +173,7 @@ class Tissue:
         self.cut_mask = torch.zeros(V, dtype=torch.int32, device=dev)
         self.edge_cut_t = torch.full((6 * N,), -1.0, dtype=torch.float32, device=dev)
         self.edge_skin_t = f32(grid.edge_skin_fraction())
+        self.rest = f32(np.asarray(grid.positions, dtype=np.float32).reshape(-1))
         self.cut_point_capacity = 4 * max(Eg, 1)
         self.cut_points = torch.zeros(8 * self.cut_point_capacity, dtype=torch.float32, device=dev)
         self.counters = torch.zeros(8, dtype=torch.int32, device=dev)
@@ -193,7 +194,7 @@ class Tissue:
                                      self.inside, self.nbr, self.bend_alive, self.edge_desc, self.edge_lambda,
                                      self.slot_support, self.tet_desc, self.tet_k_dev, self.tet_k_hyd,
                                      self.tet_alpha, self.tet_lambda, self.grid_edges, self.occupied, self.cut_mask,
-                                     self.edge_cut_t, self.edge_skin_t, self.cut_points)],
+                                     self.edge_cut_t, self.edge_skin_t, self.rest, self.cut_points)],
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
             self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
         self._tool_from = None
@@ -228,6 +229,12 @@ class Tissue:
             ct.byref(self.view), *[v.ctypes.data_as(ct.c_void_p) for v in vs], float(tool_d), _ptr(pp),
             current_stream()))
         torch.cuda.current_stream().synchronize()
+
+    def tear(self, tau, tool_d):
+        """Tear law: every intact link stretched past tau times its rest length is cut at its midpoint, then the
+        links are severed (k_TearOverstretch + k_SeverLinks). tau < 1 turns it off, as in the reference."""
+        check("gts_tear_overstretch", self.lib.gts_tear_overstretch(ct.byref(self.view), float(tau), float(tool_d),
+                                                                    current_stream()))
 
     # ---------------------------------------------------------------- readback
     def positions(self):
@@ -267,7 +274,8 @@ class Tissue:
         bend_alive = ((bend[:, None] >> np.arange(15, dtype=np.uint32)) & 1).astype(np.int32)
         return dict(neighbors=self.neighbors(), cut_mask=self.cut_mask.cpu().numpy().view(np.uint32),
                     edge_cut_t=self.edge_cut_t.cpu().numpy().reshape(-1, 6), bend_alive=bend_alive,
-                    revision=int(counters[0]), cut_point_counter=int(counters[1]), cut_points=points)
+                    revision=int(counters[0]), cut_point_counter=int(counters[1]), tear_count=int(counters[4]),
+                    cut_points=points)
 
     def liveness(self):
         """Tet liveness in generation order and edge liveness in colour order over the original (uncompacted) lists,

@@ -109,9 +109,12 @@ typedef struct GtsTissueView {
     uint32_t* cut_mask;          /* [V] */
     float* edge_cut_t;           /* [6N] */
     const float* edge_skin_t;    /* [6N] */
+    const float* rest;           /* [3N] corner rest positions (recon_corner_rest, dc_recon.cu:671): rest edge of cut
+                                  * points and tear-law scale */
     float* cut_points;           /* [8*capacity] CutPointGpu records (cut.cuh:37-46) */
     int cut_point_capacity;
-    uint32_t* counters;          /* [8] 0 revision, 1 cut point counter, 2 refreshed revision, 3 stale flag */
+    uint32_t* counters;          /* [8] 0 revision, 1 cut point counter, 2 refreshed revision, 3 stale flag,
+                                  * 4 edges marked by the tear law */
     float* scratch;              /* [4*Eg + 2*nblocks + 64] detection records and scan partials */
     float* reaction;             /* [4] force of the tissue on the tools, contacts */
     float* step_consts;          /* [step_const_capacity * GTS_STEP_CONST_FLOATS] per-substep tool shapes */
@@ -179,6 +182,12 @@ int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream)
 /* Device: CCD of the edges moving from prev_positions ([3N] device) to the current positions against blade S->E. */
 int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, const float* emit_ncut, float tool_d,
                        const float* prev_positions, void* stream);
+/* Device: tear law (k_TearOverstretch, cut.cu:607-681, as cut_ribbon_tick runs it before k_SeverLinks,
+ * cut.cu:1397-1405): every intact link between active corners that is not yet cut, has an occupied stencil voxel and
+ * is longer than tau times its rest length is cut at its midpoint (cut points with tool gap tool_d); then the links are
+ * severed and liveness refreshed. tau < 1 or NaN: no-op (cut.cu:1199-1202). Needs v->rest. Call it after the tick's
+ * detection; the grasp-ring exemption of the reference is inert on this path (no grasp ring). */
+int gts_tear_overstretch(const GtsTissueView* v, float tau, float tool_d, void* stream);
 /* Device: stable remap of the live topology. tet_remap [T] (colour order), edge_remap [E], node_remap [N],
  * counts [3] (live tets, edges, nodes); all device. scratch as in the view. */
 int gts_compact_topology(const GtsTissueView* v, int32_t* tet_remap, int32_t* edge_remap, int32_t* node_remap,

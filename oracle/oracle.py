@@ -74,6 +74,7 @@ def lib():
             getattr(L, f"oracle_detect_cut_{suf}").argtypes = [P, ct.POINTER(OracleCutTool)]
             getattr(L, f"oracle_detect_cut_ccd_{suf}").argtypes = [P, P, P, P, F, P]
             getattr(L, f"oracle_refresh_{suf}").argtypes = [P]
+            getattr(L, f"oracle_tear_{suf}").argtypes = [P, F, F]
             getattr(L, f"oracle_get_state_{suf}").argtypes = [P, P, P]
             getattr(L, f"oracle_set_state_{suf}").argtypes = [P, P, P]
             getattr(L, f"oracle_get_reaction_{suf}").argtypes = [P, P]
@@ -198,6 +199,11 @@ class Oracle:
         args = [np.asarray(v, dtype=np.float32) for v in (S, E, ncut)]
         self._call("detect_cut_ccd", self.h, *[_ptr(a) for a in args], float(tool_d), _ptr(pp))
 
+    def tear(self, tau, tool_d):
+        """Tear law (k_TearOverstretch + k_SeverLinks): links stretched past tau times their rest length part at
+        their midpoint."""
+        self._call("tear", self.h, float(tau), float(tool_d))
+
     def state(self):
         pos = np.zeros((self.N, 3), dtype=self.real)
         vel = np.zeros((self.N, 3), dtype=self.real)
@@ -220,13 +226,14 @@ class Oracle:
         mask = np.zeros(V, dtype=np.uint32)
         cut_t = np.zeros((self.N, 6), dtype=np.float32)
         bend_alive = np.zeros((self.N, 15), dtype=np.int32)
-        counters = np.zeros(2, dtype=np.uint32)
+        counters = np.zeros(3, dtype=np.uint32)
         n = self._call("get_cut", self.h, *[_ptr(a) for a in (nbr, mask, cut_t, bend_alive, counters)])
         points = np.zeros(n, dtype=CUT_POINT_DTYPE)
         if n:
             self._call("get_cut_points", self.h, _ptr(points))
         return dict(neighbors=nbr, cut_mask=mask, edge_cut_t=cut_t, bend_alive=bend_alive,
-                    revision=int(counters[0]), cut_point_counter=int(counters[1]), cut_points=points)
+                    revision=int(counters[0]), cut_point_counter=int(counters[1]), tear_count=int(counters[2]),
+                    cut_points=points)
 
     def liveness(self):
         tet = np.zeros(self.T, dtype=np.int32)

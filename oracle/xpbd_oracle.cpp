@@ -405,6 +405,7 @@ template <class R> struct Tissue {
     std::vector<float> edgeCutT, edgeSkinT;
     std::vector<CutPoint> cutPoints;
     uint32_t cutPointCounter = 0, revision = 0, refreshedRevision = 0xffffffffu;
+    uint32_t tearCount = 0;  // edges the tear law marked (the reference's dbg[15])
     int status = 0;
 
     int cid(int x, int y, int z) const { return x + nx * (y + ny * z); }
@@ -916,9 +917,12 @@ template <class R> void markAndEmit(Tissue<R>& s, int e, int axis, const int bc[
     s.edgeCutT[directedSlot(idA, idB, s.d.vx)] = (float)t;
     s.edgeCutT[directedSlot(idB, idA, s.d.vx)] = (float)((R)1 - t);
     if (!cutHitsMaterial((float)t, s.edgeSkinT[directedSlot(idA, idB, s.d.vx)], s.inside[idA], s.inside[idB])) return;
-    R L = (R)s.d.tool_voxel;
-    V3<R> axisF = mk<R>(axis == 0 ? (R)1 : (R)0, axis == 1 ? (R)1 : (R)0, axis == 2 ? (R)1 : (R)0);
-    V3<R> restEdge = axisF * L;
+    // rest edge from the corner rest positions (materialRestDevice = recon_corner_rest, cut.cu:248, :1181-1183),
+    // evaluated in float as the reference's float3 buffer holds them
+    V3<float> ra = mk<float>((float)s.rest[idA].x, (float)s.rest[idA].y, (float)s.rest[idA].z);
+    V3<float> rb = mk<float>((float)s.rest[idB].x, (float)s.rest[idB].y, (float)s.rest[idB].z);
+    V3<float> rf = rb - ra;
+    V3<R> restEdge = mk<R>((R)rf.x, (R)rf.y, (R)rf.z);
     V3<R> along[2] = {restEdge * t, restEdge * (t - (R)1)};
     int owners[2] = {idA, idB};
     for (int k = 0; k < 2; k++) {
@@ -968,6 +972,48 @@ template <class R> void severLinks(Tissue<R>& s) {  // k_SeverLinks (cut.cu:732-
         int b = s.cid(bc[0] + (axis == 0), bc[1] + (axis == 1), bc[2] + (axis == 2));
         severLink(s, a, b, axis);
     }
+}
+
+// k_TearOverstretch (cut.cu:607-681), then k_SeverLinks, as cut_ribbon_tick runs them (cut.cu:1397-1405): an intact
+// link between active corners, not yet marked and with an occupied stencil voxel, that is longer than tau times its
+// rest length is marked at its midpoint with the plane normal to it. tau < 1 or NaN disables the law (cut.cu:1199-1202).
+// The grasp-ring exemption (cut.cu:655-673) is left out: only the legacy grasp fills the ring, so on this path the
+// ring is empty, edgeWasExempt never becomes 1 and both exemption branches are unreachable. Each edge reads and sets
+// only its own mask bits, so the serial edge order decides nothing but the cut point slots.
+template <class R> void tearOverstretch(Tissue<R>& s, float tau, R toolD) {
+    if (!(tau >= 1.0f)) return;
+    float voxelL = s.d.tool_voxel;
+    float tearLen2 = (tau * voxelL) * (tau * voxelL);
+    int Eg = s.d.grid_edge_count;
+    for (int e = 0; e < Eg; e++) {
+        int axis = s.gridEdges[4 * e], bc[3] = {s.gridEdges[4 * e + 1], s.gridEdges[4 * e + 2], s.gridEdges[4 * e + 3]};
+        int idA = s.cid(bc[0], bc[1], bc[2]);
+        int idB = s.cid(bc[0] + (axis == 0), bc[1] + (axis == 1), bc[2] + (axis == 2));
+        if (s.nbr[6 * idA + 2 * axis + 1] < 0) continue;
+        if (s.active[idA] == 0 || s.active[idB] == 0) continue;
+        bool cut = false, anyOcc = false;
+        for (int k = 0; k < 4; k++) {
+            int x = bc[0] + ES_OFF[axis][k][0], y = bc[1] + ES_OFF[axis][k][1], z = bc[2] + ES_OFF[axis][k][2];
+            if (!voxIn(s, x, y, z)) continue;
+            int vid = voxId(s, x, y, z);
+            if ((s.cutMask[vid] >> ES_EDGE[axis][k]) & 1u) cut = true;
+            if (s.occupied[vid] != 0u) anyOcc = true;
+        }
+        if (cut || !anyOcc) continue;
+        V3<R> A = s.pos[idA], d = s.pos[idB] - A;
+        R len2 = dot3(d, d);
+        V3<float> ra = mk<float>((float)s.rest[idA].x, (float)s.rest[idA].y, (float)s.rest[idA].z);
+        V3<float> rb = mk<float>((float)s.rest[idB].x, (float)s.rest[idB].y, (float)s.rest[idB].z);
+        V3<float> r = rb - ra;
+        float limit = tearLen2 * (dot3(r, r) / (voxelL * voxelL));
+        if (len2 <= (R)limit) continue;
+        R len = std::sqrt(len2);
+        V3<R> ncut = mk<R>(d.x / len, d.y / len, d.z / len);
+        V3<R> hit = A + d * (R)0.5f;
+        s.tearCount++;
+        markAndEmit(s, e, axis, bc, idA, idB, (R)0.5f, hit, ncut, toolD);
+    }
+    severLinks(s);
 }
 
 // k_DetectCutQuad (cut.cu:421-474) + k_SeverLinks: cut_detect (cut.cu:1235-1254).
@@ -1201,6 +1247,7 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         for (int i = 0; i < s->N; i++) pp[i] = v3<R>(prevPos + 3 * i);                                                \
         detectCutCCD(*s, v3<R>(S), v3<R>(E), v3<R>(ncut), (R)toolD, pp);                                              \
     }                                                                                                                 \
+    extern "C" void oracle_tear_##SUF(void* h, float tau, float toolD) { tearOverstretch(*(Tissue<R>*)h, tau, (R)toolD); } \
     extern "C" void oracle_refresh_##SUF(void* h) { refreshLiveness(*(Tissue<R>*)h); }                                \
     extern "C" void oracle_get_state_##SUF(void* h, R* pos, R* vel) {                                                 \
         Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
@@ -1224,7 +1271,7 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         std::memcpy(cutMask, s->cutMask.data(), sizeof(unsigned) * s->V);                                             \
         std::memcpy(edgeCutT, s->edgeCutT.data(), sizeof(float) * 6 * s->N);                                          \
         std::memcpy(bendAlive, s->bendAlive.data(), sizeof(int) * 15 * s->N);                                         \
-        counters[0] = s->revision; counters[1] = s->cutPointCounter;                                                  \
+        counters[0] = s->revision; counters[1] = s->cutPointCounter; counters[2] = s->tearCount;                      \
         return (unsigned)s->cutPoints.size();                                                                         \
     }                                                                                                                 \
     extern "C" void oracle_get_cut_points_##SUF(void* h, void* out) {                                                 \

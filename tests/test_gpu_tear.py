@@ -1,0 +1,96 @@
+"""Parity of the HIP tear law (gts_tear_overstretch) with the oracle's k_TearOverstretch restatement (GPU).
+
+Bar: bit-exact cut topology (severed links, cut masks, directed cut fractions, bend pairs, revision, cut points, the
+tear count) and liveness, on static strain fields and over a stepped sequence where the blade and the tear law act
+in the same tick, positions staying bit-exact too.
+"""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.grid import block_grid
+from genesis_tissue_simulation_amd.params import SolverParams
+from genesis_tissue_simulation_amd.workloads import block_6144
+from tests.test_tear_oracle import expected_tears, inactive_and_empty
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(g, x, params=None, stiffness=None):
+    from genesis_tissue_simulation_amd.tissue import Tissue
+    from oracle.oracle import Oracle
+    p = params or SolverParams()
+    o = Oracle(g, p, x)
+    kd, kh = stiffness if stiffness is not None else (None, None)
+    if kd is not None:
+        o.set_tet_stiffness(kd, kh)
+    t = Tissue(g, p, kd, kh, x)
+    return o, t
+
+
+def _assert_same(t, o):
+    ct, co = t.cut_state(), o.cut_state()
+    for k in ("neighbors", "cut_mask", "edge_cut_t", "bend_alive", "cut_points"):
+        np.testing.assert_array_equal(ct[k], co[k], err_msg=k)
+    for k in ("revision", "cut_point_counter", "tear_count"):
+        assert ct[k] == co[k], k
+    for a, b in zip(t.liveness(), o.liveness()):
+        np.testing.assert_array_equal(a, b)
+    return co
+
+
+@pytest.mark.parametrize("nodes,seed,tau", [((9, 6, 5), 7, 1.3), ((17, 9, 9), 11, 1.2), ((17, 9, 9), 12, 1.5)])
+def test_random_strain(nodes, seed, tau):
+    g = block_grid(nodes=nodes, spacing=0.25)
+    rng = np.random.default_rng(seed)
+    x = (g.positions + rng.uniform(-0.35, 0.35, g.positions.shape) * 0.25).astype(np.float32)
+    print(f"tear parity: block {nodes}, seed {seed}, tau {tau}", flush=True)
+    o, t = _pair(g, x)
+    o.tear(tau, 0.02)
+    t.tear(tau, 0.02)
+    co = _assert_same(t, o)
+    sure, _ = expected_tears(g, x, tau, g.neighbors)
+    assert co["tear_count"] == len(sure) > 0
+
+
+def test_inactive_corners_and_empty_voxels():
+    g = inactive_and_empty()
+    x = (np.asarray(g.positions, np.float64) * 2.0).astype(np.float32)
+    print("tear parity: inactive corners and empty voxels", flush=True)
+    o, t = _pair(g, x)
+    o.tear(1.5, 0.01)
+    t.tear(1.5, 0.01)
+    assert _assert_same(t, o)["tear_count"] > 0
+
+
+@pytest.mark.parametrize("tau", [0.5, float("nan")])
+def test_off_below_one(tau):
+    g = block_grid(nodes=(5, 4, 4), spacing=0.25)
+    x = (np.asarray(g.positions, np.float64) * 3.0).astype(np.float32)
+    o, t = _pair(g, x)
+    t.tear(tau, 0.01)
+    o.tear(tau, 0.01)
+    assert _assert_same(t, o)["revision"] == 0
+
+
+def test_blade_and_tear_in_the_same_tick_over_steps():
+    """A strongly pre-strained block falls under gravity while the blade sweeps: each tick steps, cuts along the
+    blade, then applies the tear law; every tick is compared whole."""
+    from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+    w = block_6144()
+    rng = np.random.default_rng(5)
+    x = (w.grid.positions + rng.uniform(-0.3, 0.3, w.grid.positions.shape) * 0.25).astype(np.float32)
+    o, t = _pair(w.grid, x, w.params, w.stiffness())
+    for sim in (o, t):
+        sim.set_gravity(w.gravity)
+    total = 0
+    for s in range(12):
+        print(f"tear parity: block_6144 tick {s}", flush=True)
+        tool = swept_cut_tool(w.blade(s), w.blade(s + 1), w.blade_thickness)
+        for sim in (o, t):
+            sim.step(w.dt)
+            sim.detect_cut(tool)
+            sim.tear(1.25, w.blade_thickness)
+        co = _assert_same(t, o)
+        np.testing.assert_array_equal(t.positions(), o.state()[0])
+        total = co["tear_count"]
+    assert total > 0

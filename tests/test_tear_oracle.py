@@ -1,0 +1,199 @@
+"""Pins of the oracle's tear law (k_TearOverstretch + k_SeverLinks, cut.cu:607-681, :732-746) against what its
+definition fixes (no GPU).
+
+The reference's only test of the law is the native self-test `cut_selftest_tear_exempt` (cut.cu:1552-1613): a fresh
+edge stretched to 10 L with tau = 2 tears exactly once; its other three cases exercise the grasp-ring exemption,
+which is inert on this path (the ring is filled only by the legacy grasp). Here the base law is checked on whole
+lattices against an independent selection of the overstretched links in float64, and the cut points against the
+midpoint / edge-normal geometry the law prescribes.
+"""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.grid import block_grid, build_grid
+from genesis_tissue_simulation_amd.params import SolverParams
+from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+
+
+def _links(g):
+    """(edge index, A, B, axis) of every lattice edge in list order."""
+    vx, vy, vz = g.dims
+    nx, ny = vx + 1, vy + 1
+    out = []
+    for e, (axis, x, y, z) in enumerate(g.edges):
+        a = x + nx * (y + ny * z)
+        b = a + (1, nx, nx * ny)[axis]
+        out.append((e, a, b, int(axis), (int(x), int(y), int(z))))
+    return out
+
+
+def _any_occupied(g, axis, base):
+    """An occupied voxel among the four that share the edge (the voxels whose closure contains it)."""
+    vx, vy, vz = g.dims
+    x, y, z = base
+    others = [k for k in range(3) if k != axis]
+    for da in (0, 1):
+        for db in (0, 1):
+            c = [x, y, z]
+            c[others[0]] -= da
+            c[others[1]] -= db
+            if 0 <= c[0] < vx and 0 <= c[1] < vy and 0 <= c[2] < vz and g.occupied[c[0] + vx * (c[1] + vy * c[2])]:
+                return True
+    return False
+
+
+def expected_tears(g, pos, tau, nbr):
+    """Links that must tear: intact (in `nbr`), both corners active, an occupied voxel around them, and longer than
+    tau times their rest length, evaluated in float64; links within 1e-5 of the threshold are returned apart."""
+    x = np.asarray(pos, np.float64)
+    rest = np.asarray(g.positions, np.float32).astype(np.float64)
+    sure, close = [], []
+    for e, a, b, axis, base in _links(g):
+        if nbr[a, 2 * axis + 1] < 0 or not (g.active[a] and g.active[b]) or not _any_occupied(g, axis, base):
+            continue
+        ratio = np.linalg.norm(x[b] - x[a]) / (tau * np.linalg.norm(rest[b] - rest[a]))
+        if abs(ratio - 1.0) < 1e-5:
+            close.append(e)
+        elif ratio > 1.0:
+            sure.append(e)
+    return sure, close
+
+
+def _severed(g, cs):
+    return {e for e, a, b, axis, _ in _links(g) if g.neighbors[a, 2 * axis + 1] >= 0 and
+            cs["neighbors"][a, 2 * axis + 1] < 0}
+
+
+def _oracle(oracle_lib, g, pos=None):
+    return oracle_lib.Oracle(g, SolverParams(), pos)
+
+
+def test_uniform_stretch_tears_every_link_along_the_stretch(oracle_lib):
+    g = block_grid(nodes=(5, 4, 4), spacing=0.25)
+    x = np.asarray(g.positions, np.float32).copy()
+    x[:, 0] *= np.float32(1.5)
+    o = _oracle(oracle_lib, g, x)
+    o.tear(1.6, 0.01)                       # 1.5 L < 1.6 L: nothing tears
+    assert o.cut_state()["revision"] == 0 and o.cut_state()["tear_count"] == 0
+    o.tear(1.4, 0.01)
+    cs = o.cut_state()
+    # the cut edge list holds the interior edges (all four voxels around them in bounds): 4 x 2 x 2 along x
+    x_links = {e for e, a, b, axis, _ in _links(g) if axis == 0}
+    assert _severed(g, cs) == x_links and len(x_links) == 16
+    assert cs["tear_count"] == cs["revision"] == 16 and cs["cut_point_counter"] == 32
+    for e, a, b, axis, _ in _links(g):
+        if axis == 0:
+            assert cs["edge_cut_t"][a, 1] == 0.5 and cs["edge_cut_t"][b, 0] == 0.5
+    tet_live, _ = o.liveness()
+    assert 0 < tet_live.sum() < o.T
+    o.tear(1.4, 0.01)                       # severed links do not tear again
+    assert o.cut_state()["tear_count"] == 16
+
+
+def test_single_edge_stretched_ten_voxels_tears_once(oracle_lib):
+    """The base-law case of cut_selftest_tear_exempt (cut.cu:1595-1598): tau = 2, one edge 10 L long, only its two
+    corners active, one occupied voxel beside it."""
+    g = block_grid(nodes=(5, 5, 5), spacing=0.2)
+    nx = ny = 5
+    a = 1 + nx * (1 + ny * 1)
+    b = a + 1
+    g.active[:] = 0
+    g.active[[a, b]] = 1
+    g.occupied[:] = 0
+    g.occupied[1 + 4 * (1 + 4 * 1)] = 1
+    x = np.asarray(g.positions, np.float32).copy()
+    x[b] = x[a] + np.float32([10 * 0.2, 0.0, 0.0])
+    o = _oracle(oracle_lib, g, x)
+    o.tear(2.0, 0.2)
+    cs = o.cut_state()
+    assert cs["tear_count"] == 1 and cs["revision"] == 1
+    assert cs["neighbors"][a, 1] == -1 and cs["neighbors"][b, 0] == -1
+
+
+def test_random_strain_tears_exactly_the_overstretched_links(oracle_lib):
+    g = block_grid(nodes=(9, 6, 5), spacing=0.25)
+    rng = np.random.default_rng(7)
+    x = (g.positions + rng.uniform(-0.35, 0.35, g.positions.shape) * 0.25).astype(np.float32)
+    o = _oracle(oracle_lib, g, x)
+    tau, tool_d = 1.3, 0.02
+    sure, close = expected_tears(g, x, tau, g.neighbors)
+    assert not close and 0 < len(sure) < len(g.edges) // 2
+    o.tear(tau, tool_d)
+    cs = o.cut_state()
+    assert _severed(g, cs) == set(sure)
+    assert cs["tear_count"] == cs["revision"] == len(sure)
+    # two cut points per torn edge, in edge-list order, at the midpoint with the plane normal to the edge: the owner
+    # A lies on the negative side, B on the positive one
+    pts = cs["cut_points"]
+    assert cs["cut_point_counter"] == len(pts) == 2 * len(sure)
+    links = _links(g)
+    rest = np.asarray(g.positions, np.float64)
+    for k, e in enumerate(sorted(sure)):
+        _, a, b, axis, _ = links[e]
+        d = x[b].astype(np.float64) - x[a]
+        n = d / np.linalg.norm(d)
+        r = rest[b] - rest[a]
+        pa, pb = pts[2 * k], pts[2 * k + 1]
+        assert (pa["owner"], pa["edge"], pb["owner"], pb["edge"]) == (a, e, b, e)
+        np.testing.assert_allclose(pa["normal"], -n, atol=1e-6)
+        np.testing.assert_allclose(pb["normal"], n, atol=1e-6)
+        np.testing.assert_allclose(pa["offset"], 0.5 * r - 0.5 * tool_d * n, atol=1e-6)
+        np.testing.assert_allclose(pb["offset"], -0.5 * r + 0.5 * tool_d * n, atol=1e-6)
+        assert cs["edge_cut_t"][a, 2 * axis + 1] == 0.5 and cs["edge_cut_t"][b, 2 * axis] == 0.5
+
+
+def test_links_already_cut_by_the_blade_do_not_tear_again(oracle_lib):
+    g = block_grid(nodes=(9, 6, 5), spacing=0.25)
+    L = 0.25
+    o = _oracle(oracle_lib, g)
+    o.detect_cut(swept_cut_tool(((-L, 2.5 * L, -L), (-L, 2.5 * L, 5 * L)), ((9 * L, 2.5 * L, -L), (9 * L, 2.5 * L, 5 * L)),
+                                0.01))
+    cs0 = o.cut_state()
+    blade = _severed(g, cs0)
+    assert blade
+    x = np.asarray(g.positions, np.float32).copy()
+    x[:, 1] *= np.float32(1.5)              # every y link, the blade's among them, is overstretched
+    o.set_state(x, np.zeros_like(x))
+    sure, close = expected_tears(g, x, 1.2, cs0["neighbors"])
+    assert not close and not blade & set(sure)
+    o.tear(1.2, 0.01)
+    cs = o.cut_state()
+    assert cs["tear_count"] == len(sure)
+    assert _severed(g, cs) == blade | set(sure)
+    assert cs["cut_point_counter"] == cs0["cut_point_counter"] + 2 * len(sure)
+
+
+def inactive_and_empty(seed=3):
+    """A block whose links survive with some corners inactive and some voxels empty."""
+    g = block_grid(nodes=(9, 6, 5), spacing=0.25)
+    rng = np.random.default_rng(seed)
+    g.active[rng.random(len(g.active)) < 0.1] = 0
+    g.occupied[rng.random(len(g.occupied)) < 0.6] = 0
+    return g
+
+
+def test_inactive_corners_and_empty_voxels_never_tear(oracle_lib):
+    g = inactive_and_empty()
+    x = (np.asarray(g.positions, np.float64) * 2.0).astype(np.float32)   # every link doubles
+    o = _oracle(oracle_lib, g, x)
+    sure, close = expected_tears(g, x, 1.5, g.neighbors)
+    assert not close
+    o.tear(1.5, 0.01)
+    cs = o.cut_state()
+    got = _severed(g, cs)
+    assert got == set(sure) and cs["tear_count"] == len(sure)
+    links = _links(g)
+    inactive = {e for e, a, b, _, _ in links if not (g.active[a] and g.active[b])}
+    empty = {e for e, a, b, axis, base in links if not _any_occupied(g, axis, base)}
+    assert inactive - empty and empty - inactive and not got & (inactive | empty)
+    assert len(got) == len(links) - len(inactive | empty)
+
+
+@pytest.mark.parametrize("tau", [0.99, 0.0, float("nan"), -3.0])
+def test_tau_below_one_or_nan_is_off(oracle_lib, tau):
+    g = block_grid(nodes=(5, 4, 4), spacing=0.25)
+    x = (np.asarray(g.positions, np.float64) * 3.0).astype(np.float32)
+    o = _oracle(oracle_lib, g, x)
+    o.tear(tau, 0.01)
+    cs = o.cut_state()
+    assert cs["revision"] == 0 and cs["tear_count"] == 0 and cs["cut_point_counter"] == 0
