@@ -48,8 +48,8 @@ def test_random_strain(nodes, seed, tau):
     o.tear(tau, 0.02)
     t.tear(tau, 0.02)
     co = _assert_same(t, o)
-    sure, _ = expected_tears(g, x, tau, g.neighbors)
-    assert co["tear_count"] == len(sure) > 0
+    sure, close = expected_tears(g, x, tau, g.neighbors)   # links within 1e-5 of the threshold may go either way
+    assert len(sure) <= co["tear_count"] <= len(sure) + len(close) and co["tear_count"] > 0
 
 
 def test_inactive_corners_and_empty_voxels():
