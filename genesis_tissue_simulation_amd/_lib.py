@@ -86,8 +86,23 @@ class GtsAdjointView(ct.Structure):
                 ("abar", P), ("lambar", P), ("grad", P)]
 
 
+GRASP_FIELDS = ("capture", "release", "closed_gap", "core_fraction", "friction", "contact_compliance", "skin",
+                "ccd_velocity", "ccd_displacement", "ccd_omega", "grasp_omega", "core_compliance", "break_force",
+                "grasp_displacement")
+
+
+class GtsGraspParams(ct.Structure):
+    _fields_ = [(name, F) for name in GRASP_FIELDS] + [("exempt_rings", I)]
+
+
+class GtsGraspState(ct.Structure):
+    _fields_ = [("grasped", P), ("core", P), ("core_offset", P), ("anchor", P), ("slab_lambda", P),
+                ("core_lambda", P), ("core_force_ema", P), ("gate_lambda", P), ("capture_scratch", P),
+                ("grasping", I)]
+
+
 STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool, GtsBatchInstance,
-           GtsBatchView, GtsAdjointView)
+           GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState)
 
 # name -> (argtypes, restype); the exported C ABI
 SIGNATURES = {
@@ -104,6 +119,8 @@ SIGNATURES = {
     "gts_detect_cut": ((P, P, P), I),
     "gts_detect_cut_ccd": ((P, P, P, P, F, P, P), I),
     "gts_tear_overstretch": ((P, F, F, P), I),
+    "gts_grasp_set_tool": ((P, P, P, P, P, P), I),
+    "gts_step_legacy": ((P, P, P, P, P, P, I, P), I),
     "gts_compact_topology": ((P, P, P, P, P, P, P), I),
     "gts_apply_compaction": ((P, P, P, P, P, P, P, P), I),
     "gts_profile_begin": ((), I),

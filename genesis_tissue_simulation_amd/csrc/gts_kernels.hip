@@ -317,6 +317,224 @@ __global__ void k_velscale(int N, float velScale, const float4* pos, float4* vel
     vel[i] = make_float4(v.x, v.y, v.z, 0.f);
 }
 
+// ------------------------------------------------------------------------------------------------ legacy grasp kernels
+
+// The jaw frame of one substep's interpolated pose as xpbd_substep derives it for the grasp kernels
+// (xpbd_solver.cu:3572-3580): unit axis, the normal made orthogonal to it, w = axis x normal, half the floored gap.
+struct GraspFrame {
+    f3 hinge, axis, normal, w, shaftA, shaftB;
+    float shaftR, jawLen, jawHalfWidth, gapHalf;
+};
+
+// The legacy grasp state and the constants of one step, passed by value.
+struct GraspArgs {
+    int32_t* grasped;
+    int32_t* core;
+    const float* off;   // [3N] (u, n, w)
+    float* anchor;      // [2N] (u, w)
+    float* slab;
+    float* slabPrev;
+    float4* coreLam;
+    float* ema;
+    float* jawLam;
+    float* shaftLam;
+    float omega, contactAlpha, mu, maxDisp, coreAlpha, breakForce, kEMA, ccdOmega, ccdVel, ccdMaxDisp;
+};
+
+// Per-substep reset of the legacy multipliers (xpbd_solver.cu:3538-3545): the CCD gate's while the tool is armed, the
+// grasp's while grasping. A fill kernel, so that a captured step graph replays it exactly.
+__global__ void k_grasp_reset(int N, int gate, int grasp, GraspArgs a) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    if (gate) { a.jawLam[c] = 0.f; a.shaftLam[c] = 0.f; }
+    if (grasp) { a.slab[c] = 0.f; a.coreLam[c] = make_float4(0.f, 0.f, 0.f, 0.f); }
+}
+
+// k_SolveSlabCoulomb (xpbd_solver.cu:1131-1199) on the ring corners (grasped, not core) and k_SolveCorePin (:960-977)
+// on the core corners. Each corner reads and writes only itself, so the two launches of the reference fuse into one.
+__global__ void k_grasp_pass(int N, const GraspFrame* __restrict__ frame, float h, GraspArgs a, float4* pos) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    const int ring = a.grasped[c] != 0 && a.core[c] == 0, core = a.core[c] != 0;
+    if (!ring && !core) return;
+    float4 p4 = pos[c];
+    const float wc = p4.w;
+    if (!(wc > 0.f)) return;
+    const GraspFrame f = *frame;
+    f3 x = xyz(p4);
+    if (ring) {
+        f3 rel = x - f.hinge;
+        float u = dot3(rel, f.axis), v = dot3(rel, f.w), n = dot3(rel, f.normal);
+        if (u >= 0.f && u <= f.jawLen && fabsf(v) <= f.jawHalfWidth) {
+            float absN = fabsf(n);
+            if (absN > f.gapHalf) {
+                float Ctrue = f.gapHalf - absN;
+                float sgn = n >= 0.f ? 1.f : -1.f;
+                float denom = wc + a.contactAlpha;
+                float dLambda = (-Ctrue - a.contactAlpha * a.slab[c]) / denom;
+                float applied = a.omega * dLambda;
+                float corrMag = wc * applied;
+                if (fabsf(corrMag) > a.maxDisp) {
+                    float capScale = a.maxDisp / fabsf(corrMag);
+                    applied *= capScale;
+                    corrMag *= capScale;
+                }
+                a.slab[c] += applied;
+                x = x - f.normal * (sgn * corrMag);
+                rel = x - f.hinge;
+                n = dot3(rel, f.normal);
+            }
+        }
+        float Fn = fabsf(a.slabPrev[c]) / (h * h);
+        float aU = a.anchor[2 * c], aW = a.anchor[2 * c + 1];
+        f3 anchorWorld = f.hinge + f.axis * aU + f.w * aW + f.normal * n;
+        f3 tangential = x - anchorWorld;
+        float dist = len3(tangential);
+        float coneRadius = a.mu * Fn * (h * h * wc);
+        if (dist <= coneRadius + 1e-9f) {  // stick: compliant, capped pull to the anchor
+            f3 corr = tangential * a.omega;
+            float corrLen = len3(corr);
+            if (corrLen > a.maxDisp) corr = corr * (a.maxDisp / corrLen);
+            x = x - corr;
+        } else {  // slip: move by the cone radius and drag the anchor along
+            f3 dir = tangential * (1.f / dist);
+            x = x - dir * coneRadius;
+            f3 newRel = (anchorWorld + dir * (dist - coneRadius)) - f.hinge;
+            a.anchor[2 * c] = dot3(newRel, f.axis);
+            a.anchor[2 * c + 1] = dot3(newRel, f.w);
+        }
+    } else {
+        f3 target = f.hinge + f.axis * a.off[3 * c] + f.normal * a.off[3 * c + 1] + f.w * a.off[3 * c + 2];
+        float4 l4 = a.coreLam[c];
+        f3 lam = mk3(l4.x, l4.y, l4.z);
+        f3 applied = ((x - target) + lam * a.coreAlpha) * (-a.omega / (wc + a.coreAlpha));
+        float corrLen = len3(applied) * wc;
+        if (corrLen > a.maxDisp) applied = applied * (a.maxDisp / corrLen);
+        lam = lam + applied;
+        a.coreLam[c] = make_float4(lam.x, lam.y, lam.z, 0.f);
+        x = x + applied * wc;
+    }
+    pos[c] = make_float4(x.x, x.y, x.z, wc);
+}
+
+// After the coupling passes: k_CoreBreakCheck (xpbd_solver.cu:982-1000) while grasping, k_SolveCCDContact (:1049-1123)
+// while the tool is armed, k_update_velocity, the slab multiplier kept for the next substep (:3710-3711) and, unless
+// perpendicular edge damping must run in between, k_apply_velscale (:871-919). Per corner, in that order.
+template <bool SCALE>
+__global__ void k_legacy_finish(int N, const GraspFrame* __restrict__ frame, int grasping, int armed, float h,
+                                float invH, float velScale, GraspArgs a, float4* __restrict__ pos,
+                                const float4* __restrict__ prev, float4* __restrict__ vel) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float4 p4 = pos[c];
+    const float wc = p4.w;
+    f3 x = xyz(p4);
+    if (grasping && a.core[c] != 0) {
+        float raw = len3(xyz(a.coreLam[c])) / (h * h);
+        float e = a.ema[c];
+        e += (raw - e) * a.kEMA;
+        a.ema[c] = e;
+        if (e > a.breakForce) {
+            a.core[c] = 0;
+            a.grasped[c] = 0;
+        }
+    }
+    if (armed && wc > 0.f) {
+        const GraspFrame f = *frame;
+        f3 vp = xyz(vel[c]);
+        {  // shaft capsule, every corner
+            f3 ab = f.shaftB - f.shaftA;
+            float ab2 = dot3(ab, ab);
+            float tt = ab2 > 1e-12f ? dot3(x - f.shaftA, ab) / ab2 : 0.f;
+            tt = fmaxf(0.f, fminf(1.f, tt));
+            f3 dvec = x - (f.shaftA + ab * tt);
+            float dist = len3(dvec);
+            float C = dist - f.shaftR;
+            if (C < 0.f) {
+                f3 n = dist > 1e-9f ? dvec * (1.f / dist) : mk3(0.f, 1.f, 0.f);
+                float vApproach = -dot3(vp, n);
+                if (vApproach > a.ccdVel) {
+                    float denom = wc + a.contactAlpha;
+                    float dLambda = (-C - a.contactAlpha * a.shaftLam[c]) / denom;
+                    float applied = a.ccdOmega * dLambda;
+                    f3 corr = n * (wc * applied);
+                    float corrLen = len3(corr);
+                    if (corrLen > a.ccdMaxDisp) {
+                        float capScale = a.ccdMaxDisp / corrLen;
+                        applied *= capScale;
+                        corr = corr * capScale;
+                    }
+                    a.shaftLam[c] += applied;
+                    x = x + corr;
+                }
+            }
+        }
+        if (a.grasped[c] == 0) {  // jaw slab as a contact box, corners not grasped
+            f3 rel = x - f.hinge;
+            float u = dot3(rel, f.axis), v = dot3(rel, f.w), n = dot3(rel, f.normal);
+            if (u >= 0.f && u <= f.jawLen && fabsf(v) <= f.jawHalfWidth) {
+                float absN = fabsf(n);
+                if (absN > f.gapHalf) {
+                    float Ctrue = f.gapHalf - absN;
+                    float sgn = n >= 0.f ? 1.f : -1.f;
+                    float vApproach = -dot3(vp, f.normal * sgn);
+                    if (vApproach > a.ccdVel) {
+                        float denom = wc + a.contactAlpha;
+                        float dLambda = (-Ctrue - a.contactAlpha * a.jawLam[c]) / denom;
+                        float applied = a.ccdOmega * dLambda;
+                        float corrMag = wc * applied;
+                        if (fabsf(corrMag) > a.ccdMaxDisp) {
+                            float capScale = a.ccdMaxDisp / fabsf(corrMag);
+                            applied *= capScale;
+                            corrMag *= capScale;
+                        }
+                        a.jawLam[c] += applied;
+                        x = x - f.normal * (sgn * corrMag);
+                    }
+                }
+            }
+        }
+        pos[c] = make_float4(x.x, x.y, x.z, wc);
+    }
+    f3 v = wc > 0.f ? (x - xyz(prev[c])) * invH : mk3(0, 0, 0);
+    if (SCALE) v = wc > 0.f ? v * velScale : mk3(0, 0, 0);
+    vel[c] = make_float4(v.x, v.y, v.z, 0.f);
+    if (grasping) a.slabPrev[c] = a.slab[c];
+}
+
+// k_MarkGraspCandidates (xpbd_solver.cu:930-951): live, unpinned corners inside the jaw box widened by the capture skin,
+// with their jaw-local (u, v, n). scratch: [N] flags, then [3N] (u, v, n).
+__global__ void k_mark_candidates(int N, f3 hinge, f3 axis, f3 normal, f3 w, float jawLen, float jawHalfWidth,
+                                  float band, const int* active, const int* pinned, const float4* pos, float* scratch) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    int* cand = reinterpret_cast<int*>(scratch);
+    float* uvn = scratch + N;
+    cand[c] = 0;
+    if (active[c] == 0 || pinned[c] != 0) return;
+    f3 rel = xyz(pos[c]) - hinge;
+    float u = dot3(rel, axis), v = dot3(rel, w), n = dot3(rel, normal);
+    if (u < 0.f || u > jawLen) return;
+    if (fabsf(v) > jawHalfWidth) return;
+    if (fabsf(n) > band) return;
+    cand[c] = 1;
+    uvn[3 * c] = u;
+    uvn[3 * c + 1] = v;
+    uvn[3 * c + 2] = n;
+}
+
+// Release (xpbd_solver.cu:2456-2468): k_ReleaseDampVelocity on the grasped corners, then both sets cleared.
+__global__ void k_release(int N, int32_t* grasped, int32_t* core, float4* vel) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    if (grasped[c] != 0) {
+        float4 v = vel[c];
+        vel[c] = make_float4(v.x * 0.1f, v.y * 0.1f, v.z * 0.1f, v.w);
+    }
+    grasped[c] = 0;
+    core[c] = 0;
+}
+
 // ------------------------------------------------------------------------------------------------ cut kernels
 
 struct CutToolD {
@@ -908,40 +1126,89 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     float mu_s, mu_d, radius;
     bool strain, fallback, edges, contact, perp;
     uint32_t armed;
+    // legacy contact model: K material passes (2 while grasping), the grasp and CCD gate kernels
+    bool legacy, grasping, gate;
+    int K;
+    GraspArgs grasp;
 };
+
+// The constants gts_step and gts_step_legacy share (xpbd_begin_frame, xpbd_solver.cu:2481-2514).
+int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
+    std::memset(&P, 0, sizeof(P));
+    P.Nsub = p->substeps > 0 ? p->substeps : 1;
+    if (!v->step_consts || v->step_const_capacity < P.Nsub) return GTS_ERR_CAPACITY;
+    P.h = p->dt / (float)P.Nsub;
+    if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
+    const float ks = p->edge_ks, h = P.h;
+    P.at = ks > 0.f ? 1.f / (ks * h * h) : 0.f;
+    P.velScale = (1.f - p->damping) / (1.f + p->rayleigh * h);
+    P.betaGammaH = P.at * p->beta_s * h;
+    P.invH2 = 1.f / (h * h);
+    P.strain = v->n_tets > 0;
+    P.fallback = P.strain && ks <= 0.f;
+    P.edges = ks > 0.f || P.fallback;
+    P.fb = P.fallback ? 1.f / (p->young * h * h) : 0.f;
+    P.dOverH = fmaxf(0.f, p->strain_damping) / h;
+    P.bOverH = h > 0.f ? p->bulk_damping / h : 0.f;
+    P.iters = p->iterations > 0 ? p->iterations : 1;
+    P.mode = p->tet_mode;
+    P.omega = p->omega;
+    P.beta_s = p->beta_s;
+    P.perp = p->beta_s > 0.f;
+    P.invH = 1.f / h;
+    P.K = 1;
+    return GTS_OK;
+}
 
 // Every launch of one outer step, in the order of the reference's substep (xpbd_solver.cu:3460-3721).
 void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hipStream_t s) {
     const int T = v->n_tets, N = v->n_nodes;
     launch_refresh(v, L, s);
     if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, P.invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
-    if (P.contact) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
+    if (P.contact || P.legacy) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
     const StepShapes* shapes = reinterpret_cast<const StepShapes*>(v->step_consts);
     dim3 g(grid_for(N)), b(GTS_BLOCK);
+    const GraspFrame* frames = reinterpret_cast<const GraspFrame*>(v->step_consts);
+    // the multipliers are reset once per substep, so the K passes of a substep count as K * iters iterations
+    const int passes = P.K * P.iters;
     for (int n = 0; n < P.Nsub; n++) {
         k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, (const float4*)v->force);
-        for (int it = 0; it < P.iters; it++) {
-            if (P.edges)
-                for (int c = 0; c < v->n_edge_colours; c++) {
-                    int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
-                    if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, P.iters, P.fallback, P.at, P.fb, P.omega,
-                                                    P.betaGammaH);
-                }
-            if (P.strain)
-                for (int c = 0; c < v->n_tet_colours; c++) {
-                    int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
-                    if (cnt <= 0) continue;
-                    if (P.mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
-                    else if (P.mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
-                    else launch_tet_colour<2>(v, L, s, off, cnt, it, P.iters, P.omega, P.dOverH, P.bOverH);
-                }
+        if (P.legacy && (P.gate || P.grasping)) k_grasp_reset<<<g, b, 0, s>>>(N, P.gate, P.grasping, P.grasp);
+        for (int k = 0; k < P.K; k++) {
+            for (int i = 0; i < P.iters; i++) {
+                const int it = k * P.iters + i;
+                if (P.edges)
+                    for (int c = 0; c < v->n_edge_colours; c++) {
+                        int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
+                        if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, passes, P.fallback, P.at, P.fb, P.omega,
+                                                        P.betaGammaH);
+                    }
+                if (P.strain)
+                    for (int c = 0; c < v->n_tet_colours; c++) {
+                        int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
+                        if (cnt <= 0) continue;
+                        if (P.mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH);
+                        else if (P.mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH);
+                        else launch_tet_colour<2>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH);
+                    }
+            }
+            if (P.legacy && P.grasping) k_grasp_pass<<<g, b, 0, s>>>(N, frames + n, P.h, P.grasp, (float4*)v->pos);
+        }
+        if (P.legacy) {
+            if (P.perp) k_legacy_finish<false><<<g, b, 0, s>>>(N, frames + n, P.grasping, P.gate, P.h, P.invH, P.velScale,
+                                                               P.grasp, (float4*)v->pos, (const float4*)v->prev,
+                                                               (float4*)v->vel);
+            else k_legacy_finish<true><<<g, b, 0, s>>>(N, frames + n, P.grasping, P.gate, P.h, P.invH, P.velScale,
+                                                       P.grasp, (float4*)v->pos, (const float4*)v->prev,
+                                                       (float4*)v->vel);
         }
         const StepShapes* sh = shapes + n;
 #define GTS_FINISH(C, SC)                                                                                        \
     k_finish_nodes<C, SC><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR, P.invH,   \
                                           P.velScale, (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,  \
                                           v->reaction)
-        if (P.contact) { if (P.perp) GTS_FINISH(true, false); else GTS_FINISH(true, true); }
+        if (P.legacy) {}
+        else if (P.contact) { if (P.perp) GTS_FINISH(true, false); else GTS_FINISH(true, true); }
         else { if (P.perp) GTS_FINISH(false, false); else GTS_FINISH(false, true); }
 #undef GTS_FINISH
         if (P.perp) {
@@ -1054,32 +1321,12 @@ extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const Gt
     hipStream_t s = S(stream);
     Lattice L = lattice_of(v);
     StepPlan P;
-    std::memset(&P, 0, sizeof(P));
-    P.Nsub = p->substeps > 0 ? p->substeps : 1;
-    if (!v->step_consts || v->step_const_capacity < P.Nsub) return GTS_ERR_CAPACITY;
-    P.h = p->dt / (float)P.Nsub;
-    if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
-    const float ks = p->edge_ks, h = P.h;
-    P.at = ks > 0.f ? 1.f / (ks * h * h) : 0.f;
-    P.velScale = (1.f - p->damping) / (1.f + p->rayleigh * h);
-    P.betaGammaH = P.at * p->beta_s * h;
-    P.invH2 = 1.f / (h * h);
-    P.strain = v->n_tets > 0;
-    P.fallback = P.strain && ks <= 0.f;
-    P.edges = ks > 0.f || P.fallback;
-    P.fb = P.fallback ? 1.f / (p->young * h * h) : 0.f;
-    P.dOverH = fmaxf(0.f, p->strain_damping) / h;
-    P.bOverH = h > 0.f ? p->bulk_damping / h : 0.f;
-    P.iters = p->iterations > 0 ? p->iterations : 1;
-    P.mode = p->tet_mode;
-    P.omega = p->omega;
-    P.beta_s = p->beta_s;
-    P.perp = p->beta_s > 0.f;
+    { int rc_ = make_plan(v, p, P); if (rc_) return rc_; }
+    const float h = P.h;
     P.contact = p->contact_enabled && (tool_armed || blade_armed);
     P.armed = (tool_armed ? 7u : 0u) | (blade_armed ? 8u : 0u);
     P.alphaC = p->contact_compliance / (h * h);
     P.scaleR = 1.f / (h * h * fmaxf(1.f, (float)P.Nsub));
-    P.invH = 1.f / h;
     P.mu_s = p->mu_s;
     P.mu_d = p->mu_d;
     P.radius = p->contact_radius;
@@ -1098,6 +1345,171 @@ extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const Gt
             }
         }
         HIP_TRY(hipMemcpyAsync(v->step_consts, host, sizeof(StepShapes) * P.Nsub, hipMemcpyHostToDevice, s));
+        { int rc_ = staging_done(s); if (rc_) return rc_; }
+    }
+    if (v->use_graph && !g_profile.on) return launch_step_graph(v, L, P, s);
+    enqueue_step(v, L, P, s);
+    return -(int)hipGetLastError();
+}
+
+namespace {
+
+// The jaw frame xpbd_substep derives from a substep's pose (xpbd_solver.cu:3572-3580).
+GraspFrame grasp_frame(const ToolPose& t, float closedGap) {
+    GraspFrame f;
+    f.axis = len3(t.jawAxis) > 1e-9f ? t.jawAxis * (1.f / len3(t.jawAxis)) : mk3(0.f, 0.f, 1.f);
+    f3 n = len3(t.jawNormal) > 1e-9f ? t.jawNormal * (1.f / len3(t.jawNormal)) : mk3(0.f, 1.f, 0.f);
+    n = n - f.axis * dot3(n, f.axis);
+    if (len3(n) < 1e-7f) n = cross3(f.axis, fabsf(f.axis.x) < 0.9f ? mk3(1.f, 0.f, 0.f) : mk3(0.f, 1.f, 0.f));
+    f.normal = n * (1.f / len3(n));
+    f.w = cross3(f.axis, f.normal);
+    f.gapHalf = 0.5f * fmaxf(t.jawGap, closedGap);
+    f.hinge = t.jawHinge;
+    f.shaftA = t.shaftA;
+    f.shaftB = t.shaftB;
+    f.shaftR = t.shaftR;
+    f.jawLen = t.jawLen;
+    f.jawHalfWidth = t.jawHalfWidth;
+    return f;
+}
+
+bool grasp_state_ok(const GtsGraspState* g) {
+    return g && g->grasped && g->core && g->core_offset && g->anchor && g->slab_lambda && g->core_lambda &&
+           g->core_force_ema && g->gate_lambda && g->capture_scratch;
+}
+
+}  // namespace
+
+extern "C" int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, const GtsGraspParams* gp,
+                                  const GtsToolState* tool, int* event, void* stream) {
+    if (!v || !gp || !tool || !event || !grasp_state_ok(g)) return GTS_ERR_BAD_ARG;
+    *event = 0;
+    hipStream_t s = S(stream);
+    const int N = v->n_nodes;
+    // the jaw frame of the stored pose (xpbd_solver.cu:2326-2336)
+    f3 axis = ld3(tool->jawAxis), normal = ld3(tool->jawNormal);
+    float alen = len3(axis);
+    axis = alen > 1e-9f ? axis * (1.f / alen) : mk3(0.f, 0.f, 1.f);
+    float nlen = len3(normal);
+    normal = nlen > 1e-9f ? normal * (1.f / nlen) : mk3(0.f, 1.f, 0.f);
+    f3 w = cross3(axis, normal);
+    float wlen = len3(w);
+    w = wlen > 1e-9f ? w * (1.f / wlen) : mk3(1.f, 0.f, 0.f);
+    const float gapHalf = 0.5f * fmaxf(tool->jawGap, gp->closed_gap);
+    const bool nowAbove = tool->grasp01 > gp->capture, nowBelow = tool->grasp01 < gp->release;
+    if (!g->grasping && nowAbove) {
+        k_mark_candidates<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, ld3(tool->jawHinge), axis, normal, w, tool->jawLen,
+                                                            tool->jawHalfWidth, gapHalf + gp->skin, v->active,
+                                                            v->pinned, (const float4*)v->pos, g->capture_scratch);
+        HIP_TRY(hipGetLastError());
+        std::vector<float> scratch(4 * (size_t)N);
+        HIP_TRY(hipMemcpyAsync(scratch.data(), g->capture_scratch, scratch.size() * sizeof(float),
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const int* cand = reinterpret_cast<const int*>(scratch.data());
+        const float* uvn = scratch.data() + N;
+        std::vector<int> idxs;
+        for (int c = 0; c < N; c++)
+            if (cand[c]) idxs.push_back(c);
+        std::vector<int32_t> grasped(N, 0), core(N, 0);
+        std::vector<float> off(3 * (size_t)N, 0.f), anchor(2 * (size_t)N, 0.f);
+        if (!idxs.empty()) {
+            // normalised-ellipse core over the captured set's own u / v extents (xpbd_solver.cu:2405-2431)
+            float uMin = 1e30f, uMax = -1e30f, vMin = 1e30f, vMax = -1e30f;
+            for (int c : idxs) {
+                uMin = fminf(uMin, uvn[3 * c]); uMax = fmaxf(uMax, uvn[3 * c]);
+                vMin = fminf(vMin, uvn[3 * c + 1]); vMax = fmaxf(vMax, uvn[3 * c + 1]);
+            }
+            float uMid = 0.5f * (uMin + uMax), uHalf = fmaxf(0.5f * (uMax - uMin), 1e-6f);
+            float vMid = 0.5f * (vMin + vMax), vHalf = fmaxf(0.5f * (vMax - vMin), 1e-6f);
+            float frac = gp->core_fraction;
+            int bestIdx = -1;
+            float bestD2 = 1e30f;
+            bool anyCore = false;
+            for (int c : idxs) {
+                grasped[c] = 1;
+                float nu = (uvn[3 * c] - uMid) / uHalf, nv = (uvn[3 * c + 1] - vMid) / vHalf;
+                float d2 = nu * nu + nv * nv;
+                if (d2 < frac * frac) { core[c] = 1; anyCore = true; }
+                if (d2 < bestD2) { bestD2 = d2; bestIdx = c; }
+            }
+            if (!anyCore && bestIdx >= 0) core[bestIdx] = 1;  // a grasp is never core-less
+            for (int c : idxs) {
+                off[3 * c] = uvn[3 * c];          // u
+                off[3 * c + 1] = uvn[3 * c + 2];  // n
+                off[3 * c + 2] = uvn[3 * c + 1];  // w
+                anchor[2 * c] = uvn[3 * c];
+                anchor[2 * c + 1] = uvn[3 * c + 1];
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(g->grasped, grasped.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(g->core, core.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(g->core_offset, off.data(), off.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(g->anchor, anchor.data(), anchor.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        k_fill_float<<<grid_for(2 * N), GTS_BLOCK, 0, s>>>(2 * N, g->slab_lambda, 0.f);  // slab and its previous
+        k_fill_float<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, g->core_force_ema, 0.f);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(s));  // the host arrays above are released on return
+        g->grasping = 1;
+        *event = 1;
+    } else if (g->grasping && nowBelow) {
+        k_release<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, g->grasped, g->core, (float4*)v->vel);
+        HIP_TRY(hipGetLastError());
+        g->grasping = 0;
+        *event = 2;
+    }
+    return GTS_OK;
+}
+
+extern "C" int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp,
+                               const GtsGraspState* g, const GtsToolState* from, const GtsToolState* to, int tool_armed,
+                               void* stream) {
+    if (!v || !p || !gp || !from || !to || !grasp_state_ok(g)) return GTS_ERR_BAD_ARG;
+    if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
+    if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 64) return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    Lattice L = lattice_of(v);
+    StepPlan P;
+    { int rc_ = make_plan(v, p, P); if (rc_) return rc_; }
+    if ((size_t)P.Nsub * sizeof(GraspFrame) > (size_t)v->step_const_capacity * GTS_STEP_CONST_FLOATS * sizeof(float))
+        return GTS_ERR_CAPACITY;
+    const float h = P.h;
+    const int N = v->n_nodes;
+    P.legacy = true;
+    P.grasping = g->grasping != 0;
+    P.gate = tool_armed != 0;
+    P.K = P.grasping ? 2 : 1;  // a second material pass only while genuinely grasping (xpbd_solver.cu:3571)
+    GraspArgs& a = P.grasp;
+    a.grasped = g->grasped;
+    a.core = g->core;
+    a.off = g->core_offset;
+    a.anchor = g->anchor;
+    a.slab = g->slab_lambda;
+    a.slabPrev = g->slab_lambda + N;
+    a.coreLam = reinterpret_cast<float4*>(g->core_lambda);
+    a.ema = g->core_force_ema;
+    a.jawLam = g->gate_lambda;
+    a.shaftLam = g->gate_lambda + N;
+    a.omega = gp->grasp_omega;
+    a.contactAlpha = gp->contact_compliance / (h * h);
+    a.mu = gp->friction;
+    a.maxDisp = gp->grasp_displacement;
+    a.coreAlpha = gp->core_compliance / (h * h);
+    a.breakForce = gp->break_force;
+    a.kEMA = -expm1f(-h);  // one-second time constant, independent of the substep count (xpbd_solver.cu:992)
+    a.ccdOmega = gp->ccd_omega;
+    a.ccdVel = gp->ccd_velocity;
+    a.ccdMaxDisp = gp->ccd_displacement;
+    // the jaw frame of every substep's interpolated pose (xpbd_solver.cu:3467-3483, :3572-3580)
+    if (P.grasping || P.gate) {
+        GraspFrame* host = nullptr;
+        { int rc_ = staging_slot(sizeof(GraspFrame) * P.Nsub, reinterpret_cast<void**>(&host)); if (rc_) return rc_; }
+        const ToolPose pf = pose_of(from), pt = pose_of(to);
+        for (int n = 0; n < P.Nsub; n++) {
+            float t = fminf(1.f, (float)(n + 1) / (float)P.Nsub);
+            host[n] = grasp_frame(lerp_tool(pf, pt, t), gp->closed_gap);
+        }
+        HIP_TRY(hipMemcpyAsync(v->step_consts, host, sizeof(GraspFrame) * P.Nsub, hipMemcpyHostToDevice, s));
         { int rc_ = staging_done(s); if (rc_) return rc_; }
     }
     if (v->use_graph && !g_profile.on) return launch_step_graph(v, L, P, s);

@@ -79,6 +79,8 @@ extern "C" int gts_struct_sizes(int* out) {
     out[6] = (int)sizeof(GtsBatchInstance);
     out[7] = (int)sizeof(GtsBatchView);
     out[8] = (int)sizeof(GtsAdjointView);
+    out[9] = (int)sizeof(GtsGraspParams);
+    out[10] = (int)sizeof(GtsGraspState);
     return GTS_OK;
 }
 

@@ -40,6 +40,40 @@ class SolverParams:
         return replace(self, **changes)
 
 
+# contact models of xpbd_set_contact_model (`xpbd/xpbd_solver.h:291-293`)
+CONTACT_LEGACY = 0    # grasp capture with slab ring, core pins and break check; CCD gate against shaft and jaws
+CONTACT_FRICTION = 1  # swept friction contact of every node with the tool shapes
+
+
+@dataclass(frozen=True)
+class GraspParams:
+    """XpbdGraspParams (`xpbd/xpbd_solver.h:250-289`), named as `native_backend.py`'s GraspParams fields."""
+    capture: float = 0.5              # grasp01 above which the jaws capture
+    release: float = 0.3              # grasp01 below which they release
+    closed_gap: float = 0.0           # floor of the jaw gap
+    core_fraction: float = 0.55       # normalised-ellipse core fraction
+    friction: float = 0.6             # Coulomb mu of the captured ring
+    contact_compliance: float = 1e-8  # slab and CCD gate compliance (scaled by 1/h^2)
+    skin: float = 0.0                 # capture band widening
+    ccd_velocity: float = 0.5         # approach speed above which the CCD gate fires
+    ccd_displacement: float = 0.0     # per-substep cap of a CCD correction
+    ccd_omega: float = 0.25
+    grasp_omega: float = 0.25         # under-relaxation of the ring and core pins
+    core_compliance: float = 1e-5     # core pin compliance (scaled by 1/h^2)
+    break_force: float = 15000.0      # core force EMA above which a core corner lets go
+    grasp_displacement: float = 0.0   # per-substep cap of a grasp correction
+    exempt_rings: int = 1             # tear-law ring dilation (no ring on this path)
+
+    def with_(self, **changes):
+        return replace(self, **changes)
+
+
+def legacy_grasp_params(spacing):
+    """The grasp parameters `NativeTissue.initialize` sends for a lattice of `spacing` (`native_backend.py:314-317`)."""
+    s = float(spacing)
+    return GraspParams(skin=s * 0.5, ccd_displacement=s * 0.2, grasp_displacement=s * 0.2)
+
+
 def volume_stiffness(compliance):
     """Per-tet stiffness k = 1/compliance (float32), so that alpha~ = (1/h^2)/k as in k_ComputeTetAlpha
     (`xpbd/xpbd_solver.cu:578-587`)."""

@@ -198,7 +198,39 @@ class Tissue:
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
             self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
         self._tool_from = None
+        self.contact_model = 1
+        self.grasp_event = 0
         check("gts_init_state", self.lib.gts_init_state(ct.byref(self.view), current_stream()))
+
+    def set_contact_model(self, model, grasp=None):
+        """xpbd_set_contact_model: 0 = legacy grasp (capture, slab ring, core pins, break check) with the CCD gate,
+        1 = friction contact. `grasp` is a params.GraspParams; the grasp state starts cleared."""
+        from .params import GraspParams
+        if model not in (0, 1):
+            raise ValueError("contact model must be 0 (legacy grasp) or 1 (friction contact)")
+        self.contact_model = int(model)
+        gp = grasp or GraspParams()
+        self.grasp_params = _lib.GtsGraspParams(*[float(getattr(gp, n)) for n in _lib.GRASP_FIELDS],
+                                                int(gp.exempt_rings))
+        if model == 0:
+            dev, N = self.device, self.N
+            z = lambda n, dt=torch.float32: torch.zeros(n, dtype=dt, device=dev)
+            self.grasp_buffers = dict(grasped=z(N, torch.int32), core=z(N, torch.int32), core_offset=z(3 * N),
+                                      anchor=z(2 * N), slab_lambda=z(2 * N), core_lambda=z(4 * N),
+                                      core_force_ema=z(N), gate_lambda=z(2 * N), capture_scratch=z(4 * N))
+            b = self.grasp_buffers
+            self.grasp = _lib.GtsGraspState(*[b[k].data_ptr() for k in (
+                "grasped", "core", "core_offset", "anchor", "slab_lambda", "core_lambda", "core_force_ema",
+                "gate_lambda", "capture_scratch")], 0)
+
+    def grasp_state(self):
+        """The legacy grasp state: the latch, grasped and core flags, core force EMA, ring anchors (u, w) and the
+        previous substep's slab multiplier."""
+        b = self.grasp_buffers
+        return dict(grasping=bool(self.grasp.grasping), grasped=b["grasped"].cpu().numpy(),
+                    core=b["core"].cpu().numpy(), force_ema=b["core_force_ema"].cpu().numpy(),
+                    anchor=b["anchor"].cpu().numpy().reshape(-1, 2),
+                    slab_lambda_prev=b["slab_lambda"][self.N:].cpu().numpy())
 
     def set_colour_block(self, threads):
         """Workgroup size of the edge / tet colour kernels (a multiple of 64 up to 1024; 0: the default)."""
@@ -214,8 +246,23 @@ class Tissue:
         p = step_params(params or self.params, dt)
         to = tool_state(tool)
         frm = tool_state(tool_from) if tool_from is not None else (self._tool_from or to)
-        check("gts_step", self.lib.gts_step(ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to),
-                                            int(tool_armed), int(blade_armed), current_stream()))
+        if self.contact_model == 0:
+            # xpbd_set_tool's legacy grasp latch for the new pose, then the legacy step
+            self.grasp_event = 0
+            if tool is not None:
+                event = ct.c_int(0)
+                check("gts_grasp_set_tool", self.lib.gts_grasp_set_tool(
+                    ct.byref(self.view), ct.byref(self.grasp), ct.byref(self.grasp_params), ct.byref(to),
+                    ct.byref(event), current_stream()))
+                self.grasp_event = event.value
+                if event.value == 1:
+                    frm = to  # the capture offsets belong to this pose
+            check("gts_step_legacy", self.lib.gts_step_legacy(
+                ct.byref(self.view), ct.byref(p), ct.byref(self.grasp_params), ct.byref(self.grasp), ct.byref(frm),
+                ct.byref(to), int(tool_armed), current_stream()))
+        else:
+            check("gts_step", self.lib.gts_step(ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to),
+                                                int(tool_armed), int(blade_armed), current_stream()))
         self._tool_from = to
 
     def detect_cut(self, cut_tool):

@@ -143,6 +143,40 @@ typedef struct GtsToolState {
     float bladeA[3], bladeB[3], bladeR;
 } GtsToolState;
 
+/* XpbdGraspParams (xpbd_solver.h:250-289) in the field order of native_backend.py's GraspParams: the legacy contact
+ * model's capture / release thresholds, captured-ring friction, core pins, break force and CCD gate. */
+typedef struct GtsGraspParams {
+    float capture, release;     /* grasp01 above which the jaws capture, below which they release */
+    float closed_gap;           /* floor of the jaw gap */
+    float core_fraction;        /* normalised-ellipse core fraction */
+    float friction;             /* Coulomb mu of the captured ring */
+    float contact_compliance;   /* slab and CCD gate compliance, scaled by 1/h^2 */
+    float skin;                 /* capture band widening */
+    float ccd_velocity;         /* approach speed above which the CCD gate fires */
+    float ccd_displacement;     /* per-substep cap of a CCD correction */
+    float ccd_omega;            /* CCD gate under-relaxation */
+    float grasp_omega;          /* under-relaxation of the ring and the core pins */
+    float core_compliance;      /* core pin compliance, scaled by 1/h^2 */
+    float break_force;          /* core force (one-second EMA of |lambda| / h^2) above which a core corner lets go */
+    float grasp_displacement;   /* per-substep cap of a grasp correction */
+    int exempt_rings;           /* tear-law ring dilation; no ring on this path */
+} GtsGraspParams;
+
+/* Legacy grasp state of one tissue (xpbd_solver.cu:250-275). Device arrays of N entries unless marked; the caller
+ * allocates them and zeroes them before the first use. */
+typedef struct GtsGraspState {
+    int32_t* grasped;           /* captured corners */
+    int32_t* core;              /* core corners, pinned to their jaw-local anchor */
+    float* core_offset;         /* [3N] jaw-local (u, n, w) of each captured corner at capture */
+    float* anchor;              /* [2N] Coulomb anchor (u, w) of each ring corner */
+    float* slab_lambda;         /* [2N] slab multiplier of this substep, then of the previous substep */
+    float* core_lambda;         /* [4N] vector core multiplier, float4 rows */
+    float* core_force_ema;      /* break-check EMA */
+    float* gate_lambda;         /* [2N] CCD gate multipliers: jaw slab, then shaft */
+    float* capture_scratch;     /* [4N] candidate flags, then their (u, v, n) */
+    int grasping;               /* host latch, 1 while grasping; gts_grasp_set_tool maintains it */
+} GtsGraspState;
+
 /* CutToolDesc (cut.h:30-38): swept quad T1 = (Sprev, Eprev, E), T2 = (Sprev, E, S). */
 typedef struct GtsCutTool {
     float t1v0[3], t1v1[3], t1v2[3];
@@ -155,8 +189,8 @@ typedef struct GtsCutTool {
 
 int gts_abi_version(void);
 /* Sizes of the structs above as this library sees them (ABI guard like LCS_PhysicsAbiVersion,
- * plugin_api.cpp:102). out[0..8] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
- * GtsBatchInstance, GtsBatchView, GtsAdjointView. */
+ * plugin_api.cpp:102). out[0..10] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
+ * GtsBatchInstance, GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState. */
 int gts_struct_sizes(int* out);
 
 /* Host: derive edges, Kuhn tets and both greedy colourings from the lattice. Needs no GPU. */
@@ -177,6 +211,21 @@ int gts_refresh_liveness(const GtsTissueView* v, void* stream);
 /* Device: one outer step of p->substeps substeps. The tool pose moves from `from` to `to` over the step. */
 int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from, const GtsToolState* to,
              int tool_armed, int blade_armed, void* stream);
+/* Host + device: the legacy contact model's part of xpbd_set_tool (xpbd_solver.cu:2318-2341, :2372-2470) for the
+ * pose `tool`. grasp01 rising past gp->capture captures: live unpinned corners inside the jaw box widened by the skin
+ * are marked on the device (k_MarkGraspCandidates), read back (the stream is synchronised), the normalised-ellipse core
+ * is chosen on the host and the grasp state is uploaded. grasp01 falling below gp->release releases: the grasped
+ * corners' velocity is scaled by 0.1 and the sets are cleared. *event = 1 on capture, 2 on release, else 0. After a
+ * capture the next step must start at `tool` (the reference sets g_toolFrom = g_tool). */
+int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, const GtsGraspParams* gp, const GtsToolState* tool,
+                       int* event, void* stream);
+/* Device: one outer step under the legacy contact model (xpbd_substep with g_frictionContact false,
+ * xpbd_solver.cu:3460-3721): while grasping, K = 2 material passes, each followed by the slab ring with Coulomb
+ * anchors (k_SolveSlabCoulomb) and the core pins (k_SolveCorePin), then the core break check (k_CoreBreakCheck);
+ * with the tool armed, the CCD gate against the shaft capsule and the jaw slab (k_SolveCCDContact). The jaw frame
+ * follows the pose from `from` to `to` over the substeps. The contact fields of *p are not used, nor the reaction. */
+int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp, const GtsGraspState* g,
+                    const GtsToolState* from, const GtsToolState* to, int tool_armed, void* stream);
 /* Device: swept-quad cut of the current positions, then sever the cut links and refresh liveness. */
 int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream);
 /* Device: CCD of the edges moving from prev_positions ([3N] device) to the current positions against blade S->E. */

@@ -34,6 +34,15 @@ class OracleTool(ct.Structure):
                 ("grasp01", F), ("bladeA", F * 3), ("bladeB", F * 3), ("bladeR", F)]
 
 
+GRASP_FIELDS = ("capture", "release", "closed_gap", "core_fraction", "friction", "contact_compliance", "skin",
+                "ccd_velocity", "ccd_displacement", "ccd_omega", "grasp_omega", "core_compliance", "break_force",
+                "grasp_displacement")
+
+
+class OracleGrasp(ct.Structure):
+    _fields_ = [(name, F) for name in GRASP_FIELDS] + [("exempt_rings", I)]
+
+
 class OracleCutTool(ct.Structure):
     _fields_ = [("t1v0", F * 3), ("t1v1", F * 3), ("t1v2", F * 3), ("t2v0", F * 3), ("t2v1", F * 3),
                 ("t2v2", F * 3), ("ncut", F * 3), ("d", F), ("valid", I), ("aabbMin", F * 3), ("aabbMax", F * 3)]
@@ -75,6 +84,10 @@ def lib():
             getattr(L, f"oracle_detect_cut_ccd_{suf}").argtypes = [P, P, P, P, F, P]
             getattr(L, f"oracle_refresh_{suf}").argtypes = [P]
             getattr(L, f"oracle_tear_{suf}").argtypes = [P, F, F]
+            getattr(L, f"oracle_set_contact_model_{suf}").argtypes = [P, I, ct.POINTER(OracleGrasp)]
+            getattr(L, f"oracle_get_grasp_{suf}").argtypes = [P] * 6
+            getattr(L, f"oracle_get_grasp_{suf}").restype = I
+            getattr(L, f"oracle_set_tool_{suf}").restype = I
             getattr(L, f"oracle_get_state_{suf}").argtypes = [P, P, P]
             getattr(L, f"oracle_set_state_{suf}").argtypes = [P, P, P]
             getattr(L, f"oracle_get_reaction_{suf}").argtypes = [P, P]
@@ -182,8 +195,28 @@ class Oracle:
         self._call("set_gravity", self.h, *(float(v) for v in g))
 
     def set_tool(self, tool, tool_armed=True, blade_armed=False, reset=False):
+        """Stores the pose the next step moves to; under the legacy contact model, also the grasp capture / release
+        (returns 1 on capture, 2 on release, else 0)."""
         self._tool = tool_struct(tool)
-        self._call("set_tool", self.h, ct.byref(self._tool), int(tool_armed), int(blade_armed), int(reset))
+        return self._call("set_tool", self.h, ct.byref(self._tool), int(tool_armed), int(blade_armed), int(reset))
+
+    def set_contact_model(self, model, grasp):
+        """xpbd_set_contact_model: 0 legacy grasp and CCD gate, 1 friction contact; `grasp` has the GraspParams
+        fields. Clears the grasp state."""
+        self._grasp = OracleGrasp(*[float(getattr(grasp, n)) for n in GRASP_FIELDS], int(grasp.exempt_rings))
+        self._call("set_contact_model", self.h, int(model), ct.byref(self._grasp))
+
+    def grasp_state(self):
+        grasped = np.zeros(self.N, dtype=np.int32)
+        core = np.zeros(self.N, dtype=np.int32)
+        ema = np.zeros(self.N, dtype=self.real)
+        anchor = np.zeros((self.N, 2), dtype=self.real)
+        slab_prev = np.zeros(self.N, dtype=self.real)
+        grasping = self._call("get_grasp", self.h, *[_ptr(a) for a in (grasped, core, ema, anchor, slab_prev)])
+        if grasping < 0:
+            raise RuntimeError("no contact model was set")
+        return dict(grasping=bool(grasping), grasped=grasped, core=core, force_ema=ema, anchor=anchor,
+                    slab_lambda_prev=slab_prev)
 
     def step(self, dt):
         rc = self._call("step", self.h, float(dt))

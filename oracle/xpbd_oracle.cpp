@@ -129,6 +129,23 @@ struct OracleTool {
     float bladeA[3], bladeB[3], bladeR;
 };
 
+// XpbdGraspParams (xpbd_solver.h:250-289), in the field order of native_backend.py's GraspParams: the legacy contact
+// model's capture/release thresholds, ring friction, core pins, break force and CCD gate.
+struct OracleGrasp {
+    float capture, release;       // grasp01 hysteresis
+    float closed_gap;             // floor of the jaw gap
+    float core_fraction;          // normalised-ellipse core fraction
+    float friction;               // Coulomb mu of the ring
+    float contact_compliance;     // slab and CCD gate compliance, scaled by 1/h^2
+    float skin;                   // capture band widening
+    float ccd_velocity, ccd_displacement, ccd_omega;
+    float grasp_omega;            // under-relaxation of the slab ring and the core pins
+    float core_compliance;        // core pin compliance, scaled by 1/h^2
+    float break_force;            // threshold of the core force EMA
+    float grasp_displacement;     // per-substep cap of the grasp corrections
+    int exempt_rings;             // tear-law ring dilation (not used on this path)
+};
+
 // Swept cut plane (CutToolDesc, cut.h:30-38).
 struct OracleCutTool {
     float t1v0[3], t1v1[3], t1v2[3];
@@ -406,6 +423,13 @@ template <class R> struct Tissue {
     std::vector<CutPoint> cutPoints;
     uint32_t cutPointCounter = 0, revision = 0, refreshedRevision = 0xffffffffu;
     uint32_t tearCount = 0;  // edges the tear law marked (the reference's dbg[15])
+    // legacy grasp (contact model 0, xpbd_solver.cu:250-275): the host latch and the per-node state
+    int contactModel = 1;         // 0: legacy grasp and CCD gate; 1: friction contact
+    OracleGrasp gp{};
+    bool grasping = false;
+    std::vector<int> grasped, core;
+    std::vector<R> offU, offN, offW, anchU, anchW, slab, slabPrev, ema, jawLam, shaftLam;
+    std::vector<V3<R>> coreLam;
     int status = 0;
 
     int cid(int x, int y, int z) const { return x + nx * (y + ny * z); }
@@ -741,8 +765,260 @@ template <class R> ToolR<R> lerpTool(const ToolR<R>& from, const ToolR<R>& to, R
     return r;
 }
 
-// One substep of the legacy branch (xpbd_solver.cu:3460-3721 with K = 1, no grasp, no bending, no guard passes),
-// friction tool contact of lattice nodes (:3659-3665, friction_contact.cuh:143-169).
+// ---- legacy grasp (contact model 0): capture/release, slab ring with Coulomb anchors, core pins, break check, CCD gate
+
+// The jaw frame of a substep's interpolated pose as xpbd_substep derives it for the grasp kernels
+// (xpbd_solver.cu:3572-3580): unit axis, normal made orthogonal to it, w = axis x normal, half the floored gap.
+template <class R> struct GraspFrame {
+    V3<R> hinge, axis, normal, w, shaftA, shaftB;
+    R shaftR, jawLen, jawHalfWidth, gapHalf;
+};
+
+template <class R> GraspFrame<R> graspFrame(const ToolR<R>& t, R closedGap) {
+    GraspFrame<R> f;
+    f.axis = len3(t.jawAxis) > (R)1e-9f ? t.jawAxis * ((R)1 / len3(t.jawAxis)) : mk<R>(0, 0, 1);
+    V3<R> n = len3(t.jawNormal) > (R)1e-9f ? t.jawNormal * ((R)1 / len3(t.jawNormal)) : mk<R>(0, 1, 0);
+    n = n - f.axis * dot3(n, f.axis);
+    if (len3(n) < (R)1e-7f) n = cross3(f.axis, std::fabs(f.axis.x) < (R)0.9f ? mk<R>(1, 0, 0) : mk<R>(0, 1, 0));
+    f.normal = n * ((R)1 / len3(n));
+    f.w = cross3(f.axis, f.normal);
+    f.gapHalf = (R)0.5f * fmax_<R>(t.jawGap, closedGap);
+    f.hinge = t.jawHinge;
+    f.shaftA = t.shaftA;
+    f.shaftB = t.shaftB;
+    f.shaftR = t.shaftR;
+    f.jawLen = t.jawLen;
+    f.jawHalfWidth = t.jawHalfWidth;
+    return f;
+}
+
+// xpbd_set_contact_model's reset of the grasp state (xpbd_solver.cu:2200-2207).
+template <class R> void setContactModel(Tissue<R>& s, int model, const OracleGrasp& gp) {
+    s.contactModel = model;
+    s.gp = gp;
+    s.grasping = false;
+    s.grasped.assign(s.N, 0);
+    s.core.assign(s.N, 0);
+    for (auto* v : {&s.offU, &s.offN, &s.offW, &s.anchU, &s.anchW, &s.slab, &s.slabPrev, &s.ema, &s.jawLam,
+                    &s.shaftLam})
+        v->assign(s.N, (R)0);
+    s.coreLam.assign(s.N, mk<R>(0, 0, 0));
+}
+
+// xpbd_set_tool's legacy branch (xpbd_solver.cu:2318-2341, :2372-2470), run after the new pose is stored: capture on
+// grasp01 rising past the capture threshold, release on it falling below the release threshold. Returns 1 on capture,
+// 2 on release, 0 otherwise. A capture makes the next step start at the new pose.
+template <class R> int graspSetTool(Tissue<R>& s) {
+    const ToolR<R>& t = s.tool;
+    R alen = len3(t.jawAxis);
+    V3<R> axis = alen > (R)1e-9f ? t.jawAxis * ((R)1 / alen) : mk<R>(0, 0, 1);
+    R nlen = len3(t.jawNormal);
+    V3<R> normal = nlen > (R)1e-9f ? t.jawNormal * ((R)1 / nlen) : mk<R>(0, 1, 0);
+    V3<R> w = cross3(axis, normal);
+    R wlen = len3(w);
+    w = wlen > (R)1e-9f ? w * ((R)1 / wlen) : mk<R>(1, 0, 0);
+    R gapHalf = (R)0.5f * fmax_<R>(t.jawGap, (R)s.gp.closed_gap);
+    bool nowAbove = t.grasp01 > (R)s.gp.capture, nowBelow = t.grasp01 < (R)s.gp.release;
+    if (!s.grasping && nowAbove) {
+        // k_MarkGraspCandidates (:930-951): live, unpinned corners inside the jaw box widened by the capture skin
+        std::vector<int> idxs;
+        std::vector<R> U(s.N, (R)0), Vv(s.N, (R)0), Nn(s.N, (R)0);
+        for (int c = 0; c < s.N; c++) {
+            if (s.active[c] == 0 || s.pinned[c] != 0) continue;
+            V3<R> rel = s.pos[c] - t.jawHinge;
+            R u = dot3(rel, axis), v = dot3(rel, w), n = dot3(rel, normal);
+            if (u < 0 || u > t.jawLen) continue;
+            if (std::fabs(v) > t.jawHalfWidth) continue;
+            if (std::fabs(n) > gapHalf + (R)s.gp.skin) continue;
+            idxs.push_back(c);
+            U[c] = u; Vv[c] = v; Nn[c] = n;
+        }
+        std::fill(s.grasped.begin(), s.grasped.end(), 0);
+        std::fill(s.core.begin(), s.core.end(), 0);
+        for (auto* v : {&s.offU, &s.offN, &s.offW, &s.anchU, &s.anchW}) std::fill(v->begin(), v->end(), (R)0);
+        if (!idxs.empty()) {
+            // normalised-ellipse core over the captured set's own u / v extents (:2405-2431)
+            R uMin = (R)1e30f, uMax = (R)-1e30f, vMin = (R)1e30f, vMax = (R)-1e30f;
+            for (int c : idxs) {
+                uMin = fmin_<R>(uMin, U[c]); uMax = fmax_<R>(uMax, U[c]);
+                vMin = fmin_<R>(vMin, Vv[c]); vMax = fmax_<R>(vMax, Vv[c]);
+            }
+            R uMid = (R)0.5f * (uMin + uMax), uHalf = fmax_<R>((R)0.5f * (uMax - uMin), (R)1e-6f);
+            R vMid = (R)0.5f * (vMin + vMax), vHalf = fmax_<R>((R)0.5f * (vMax - vMin), (R)1e-6f);
+            R frac = (R)s.gp.core_fraction;
+            int bestIdx = -1;
+            R bestD2 = (R)1e30f;
+            for (int c : idxs) {
+                s.grasped[c] = 1;
+                R nu = (U[c] - uMid) / uHalf, nv = (Vv[c] - vMid) / vHalf;
+                R d2 = nu * nu + nv * nv;
+                if (d2 < frac * frac) s.core[c] = 1;
+                if (d2 < bestD2) { bestD2 = d2; bestIdx = c; }
+            }
+            bool anyCore = false;
+            for (int c : idxs) anyCore = anyCore || s.core[c] != 0;
+            if (!anyCore && bestIdx >= 0) s.core[bestIdx] = 1;  // a grasp is never core-less
+            for (int c : idxs) {
+                s.offU[c] = U[c]; s.offN[c] = Nn[c]; s.offW[c] = Vv[c];
+                s.anchU[c] = U[c]; s.anchW[c] = Vv[c];
+            }
+        }
+        std::fill(s.slab.begin(), s.slab.end(), (R)0);
+        std::fill(s.slabPrev.begin(), s.slabPrev.end(), (R)0);
+        std::fill(s.ema.begin(), s.ema.end(), (R)0);
+        s.grasping = true;
+        s.toolFrom = s.tool;
+        return 1;
+    }
+    if (s.grasping && nowBelow) {
+        for (int c = 0; c < s.N; c++)  // k_ReleaseDampVelocity (:956-961), then the sets are cleared
+            if (s.grasped[c] != 0) s.vel[c] = s.vel[c] * (R)0.1f;
+        std::fill(s.grasped.begin(), s.grasped.end(), 0);
+        std::fill(s.core.begin(), s.core.end(), 0);
+        s.grasping = false;
+        return 2;
+    }
+    return 0;
+}
+
+// k_SolveSlabCoulomb (xpbd_solver.cu:1131-1199) for a ring corner, k_SolveCorePin (:960-977) for a core corner.
+template <class R> void graspPass(Tissue<R>& s, int c, const GraspFrame<R>& f, R h, R contactAlpha, R coreAlpha) {
+    R wc = s.invMass[c];
+    if (!(wc > 0)) return;
+    R omega = (R)s.gp.grasp_omega, maxDisp = (R)s.gp.grasp_displacement;
+    V3<R> x = s.pos[c];
+    if (s.grasped[c] != 0 && s.core[c] == 0) {
+        V3<R> rel = x - f.hinge;
+        R u = dot3(rel, f.axis), v = dot3(rel, f.w), n = dot3(rel, f.normal);
+        if (u >= 0 && u <= f.jawLen && std::fabs(v) <= f.jawHalfWidth) {
+            R absN = std::fabs(n);
+            if (absN > f.gapHalf) {
+                R Ctrue = f.gapHalf - absN;
+                R sgn = n >= 0 ? (R)1 : (R)-1;
+                R denom = wc + contactAlpha;
+                R dLambda = (-Ctrue - contactAlpha * s.slab[c]) / denom;
+                R applied = omega * dLambda;
+                R corrMag = wc * applied;
+                if (std::fabs(corrMag) > maxDisp) {
+                    R capScale = maxDisp / std::fabs(corrMag);
+                    applied *= capScale;
+                    corrMag *= capScale;
+                }
+                s.slab[c] += applied;
+                x = x - f.normal * (sgn * corrMag);
+                rel = x - f.hinge;
+                n = dot3(rel, f.normal);
+            }
+        }
+        R Fn = std::fabs(s.slabPrev[c]) / (h * h);
+        V3<R> anchorWorld = f.hinge + f.axis * s.anchU[c] + f.w * s.anchW[c] + f.normal * n;
+        V3<R> tangential = x - anchorWorld;
+        R dist = len3(tangential);
+        R coneRadius = (R)s.gp.friction * Fn * (h * h * wc);
+        if (dist <= coneRadius + (R)1e-9f) {  // stick: compliant, capped pull to the anchor
+            V3<R> corr = tangential * omega;
+            R corrLen = len3(corr);
+            if (corrLen > maxDisp) corr = corr * (maxDisp / corrLen);
+            x = x - corr;
+        } else {  // slip: move by the cone radius and drag the anchor along
+            V3<R> dir = tangential * ((R)1 / dist);
+            x = x - dir * coneRadius;
+            V3<R> newAnchor = anchorWorld + dir * (dist - coneRadius);
+            V3<R> newRel = newAnchor - f.hinge;
+            s.anchU[c] = dot3(newRel, f.axis);
+            s.anchW[c] = dot3(newRel, f.w);
+        }
+        s.pos[c] = x;
+    } else if (s.core[c] != 0) {
+        V3<R> target = f.hinge + f.axis * s.offU[c] + f.normal * s.offN[c] + f.w * s.offW[c];
+        V3<R> applied = ((x - target) + s.coreLam[c] * coreAlpha) * (-omega / (wc + coreAlpha));
+        R corrLen = len3(applied) * wc;
+        if (corrLen > maxDisp) applied = applied * (maxDisp / corrLen);
+        s.coreLam[c] = s.coreLam[c] + applied;
+        s.pos[c] = x + applied * wc;
+    }
+}
+
+// k_CoreBreakCheck (xpbd_solver.cu:982-1000): a core corner whose one-second EMA of |lambda| / h^2 passes the break
+// force leaves the grasp for good.
+template <class R> void coreBreak(Tissue<R>& s, int c, R h, R kEMA) {
+    if (s.core[c] == 0) return;
+    R raw = len3(s.coreLam[c]) / (h * h);
+    s.ema[c] += (raw - s.ema[c]) * kEMA;
+    if (s.ema[c] > (R)s.gp.break_force) {
+        s.core[c] = 0;
+        s.grasped[c] = 0;
+    }
+}
+
+// k_SolveCCDContact (xpbd_solver.cu:1049-1123): the anti-tunnelling gate against the shaft capsule (every corner) and
+// the jaw slab (corners not grasped), firing only for a corner approaching faster than the threshold.
+template <class R> void ccdGate(Tissue<R>& s, int c, const GraspFrame<R>& f, R contactAlpha) {
+    R wc = s.invMass[c];
+    if (!(wc > 0)) return;
+    R omega = (R)s.gp.ccd_omega, thresh = (R)s.gp.ccd_velocity, maxDisp = (R)s.gp.ccd_displacement;
+    V3<R> x = s.pos[c], vel = s.vel[c];
+    {
+        V3<R> ab = f.shaftB - f.shaftA;
+        R ab2 = dot3(ab, ab);
+        R tt = ab2 > (R)1e-12f ? dot3(x - f.shaftA, ab) / ab2 : (R)0;
+        tt = fmax_<R>((R)0, fmin_<R>((R)1, tt));
+        V3<R> closest = f.shaftA + ab * tt;
+        V3<R> dvec = x - closest;
+        R dist = len3(dvec);
+        R C = dist - f.shaftR;
+        if (C < 0) {
+            V3<R> n = dist > (R)1e-9f ? dvec * ((R)1 / dist) : mk<R>(0, 1, 0);
+            R vApproach = -dot3(vel, n);
+            if (vApproach > thresh) {
+                R denom = wc + contactAlpha;
+                R dLambda = (-C - contactAlpha * s.shaftLam[c]) / denom;
+                R applied = omega * dLambda;
+                V3<R> corr = n * (wc * applied);
+                R corrLen = len3(corr);
+                if (corrLen > maxDisp) {
+                    R capScale = maxDisp / corrLen;
+                    applied *= capScale;
+                    corr = corr * capScale;
+                }
+                s.shaftLam[c] += applied;
+                x = x + corr;
+            }
+        }
+    }
+    if (s.grasped[c] == 0) {
+        V3<R> rel = x - f.hinge;
+        R u = dot3(rel, f.axis), v = dot3(rel, f.w), n = dot3(rel, f.normal);
+        if (u >= 0 && u <= f.jawLen && std::fabs(v) <= f.jawHalfWidth) {
+            R absN = std::fabs(n);
+            if (absN > f.gapHalf) {
+                R Ctrue = f.gapHalf - absN;
+                R sgn = n >= 0 ? (R)1 : (R)-1;
+                V3<R> nOut = f.normal * sgn;
+                R vApproach = -dot3(vel, nOut);
+                if (vApproach > thresh) {
+                    R denom = wc + contactAlpha;
+                    R dLambda = (-Ctrue - contactAlpha * s.jawLam[c]) / denom;
+                    R applied = omega * dLambda;
+                    R corrMag = wc * applied;
+                    if (std::fabs(corrMag) > maxDisp) {
+                        R capScale = maxDisp / std::fabs(corrMag);
+                        applied *= capScale;
+                        corrMag *= capScale;
+                    }
+                    s.jawLam[c] += applied;
+                    x = x - f.normal * (sgn * corrMag);
+                }
+            }
+        }
+    }
+    s.pos[c] = x;
+}
+
+// One substep of the legacy branch (xpbd_solver.cu:3460-3721, no bending, no guard passes): with the legacy contact
+// model, K = 2 material passes while grasping, each followed by the grasp ring and core pins, then the break check and
+// the CCD gate; with the friction model, friction tool contact of lattice nodes (:3659-3665,
+// friction_contact.cuh:143-169).
 template <class R> int substep(Tissue<R>& s) {
     if (!s.frameReady) return -1001;
     R h = s.h;
@@ -769,23 +1045,54 @@ template <class R> int substep(Tissue<R>& s) {
     R dampingOverH = fmax_<R>((R)0, (R)s.d.strain_damping) / h;
     R bulkOverH = h > 0 ? (R)s.d.bulk_damping / h : (R)0;
     int iters = s.d.iters > 0 ? s.d.iters : 1;
-    for (int it = 0; it < iters; it++) {
-        bool useLambda = it > 0;  // the multipliers were reset at the start of the substep (:3534-3563)
-        if (edges)
-            for (size_t c = 0; c < s.ecolorCnt.size(); c++) {
-                int off = s.ecolorOff[c], cnt = s.ecolorCnt[c];
-#pragma omp parallel for schedule(static) if (cnt > 4096)
-                for (int k = 0; k < cnt; k++) solveEdge(s, off + k, s.alphaTilde, fallback, fallbackInvEh2, useLambda);
-            }
-        if (strain)
-            for (size_t c = 0; c < s.tcolorCnt.size(); c++) {
-                int off = s.tcolorOff[c], cnt = s.tcolorCnt[c];
-#pragma omp parallel for schedule(static) if (cnt > 2048)
-                for (int k = 0; k < cnt; k++) solveTet(s, s.tcolorOrder[off + k], dampingOverH, bulkOverH, useLambda);
-            }
+    const bool legacy = s.contactModel == 0;
+    const bool grasping = legacy && s.grasping;
+    // the legacy multipliers (:3538-3545): the CCD gate's while the tool is armed, the grasp's while grasping
+    if (legacy && s.toolArmed) {
+        std::fill(s.jawLam.begin(), s.jawLam.end(), (R)0);
+        std::fill(s.shaftLam.begin(), s.shaftLam.end(), (R)0);
     }
+    if (grasping) {
+        std::fill(s.slab.begin(), s.slab.end(), (R)0);
+        std::fill(s.coreLam.begin(), s.coreLam.end(), mk<R>(0, 0, 0));
+    }
+    GraspFrame<R> frame{};
+    R contactAlpha = 0, coreAlpha = 0;
+    if (legacy) {
+        frame = graspFrame(stepTool, (R)s.gp.closed_gap);
+        contactAlpha = (R)s.gp.contact_compliance / (h * h);
+        coreAlpha = (R)s.gp.core_compliance / (h * h);
+    }
+    const int K = grasping ? 2 : 1;  // a second material pass only while genuinely grasping (:3571)
+    for (int k = 0; k < K; k++) {
+        for (int it = 0; it < iters; it++) {
+            bool useLambda = k > 0 || it > 0;  // the multipliers were reset at the start of the substep (:3534-3563)
+            if (edges)
+                for (size_t c = 0; c < s.ecolorCnt.size(); c++) {
+                    int off = s.ecolorOff[c], cnt = s.ecolorCnt[c];
+#pragma omp parallel for schedule(static) if (cnt > 4096)
+                    for (int q = 0; q < cnt; q++)
+                        solveEdge(s, off + q, s.alphaTilde, fallback, fallbackInvEh2, useLambda);
+                }
+            if (strain)
+                for (size_t c = 0; c < s.tcolorCnt.size(); c++) {
+                    int off = s.tcolorOff[c], cnt = s.tcolorCnt[c];
+#pragma omp parallel for schedule(static) if (cnt > 2048)
+                    for (int q = 0; q < cnt; q++)
+                        solveTet(s, s.tcolorOrder[off + q], dampingOverH, bulkOverH, useLambda);
+                }
+        }
+        if (grasping)  // k_SolveSlabCoulomb, then k_SolveCorePin (:3603-3610); each touches only its own corner
+            for (int c = 0; c < s.N; c++) graspPass(s, c, frame, h, contactAlpha, coreAlpha);
+    }
+    if (grasping) {  // k_CoreBreakCheck once after the coupling passes (:3612-3617)
+        R kEMA = -std::expm1(-h);
+        for (int c = 0; c < s.N; c++) coreBreak(s, c, h, kEMA);
+    }
+    if (legacy && s.toolArmed)  // k_SolveCCDContact, once, last (:3618-3624)
+        for (int c = 0; c < s.N; c++) ccdGate(s, c, frame, contactAlpha);
     // friction contact: k_FindToolContacts x kinds, k_SolveFrictionContacts x2, k_NodeReaction (:3659-3665)
-    if (s.d.contact_enabled && (s.toolArmed || s.bladeArmed)) {
+    if (!legacy && s.d.contact_enabled && (s.toolArmed || s.bladeArmed)) {
         const int kinds = 4;
         Shape<R> from[kinds], to[kinds];
         for (int kind = 0; kind < kinds; kind++) {
@@ -838,6 +1145,7 @@ template <class R> int substep(Tissue<R>& s) {
     // k_update_velocity, [k_perp_damp_edges], k_apply_velscale (:871-919)
     R invH = (R)1 / h;
     for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? (s.pos[p] - s.prev[p]) * invH : mk<R>(0, 0, 0);
+    if (grasping) s.slabPrev = s.slab;  // the next substep's normal force (:3710-3711)
     if (s.d.beta_s > 0)
         for (size_t c = 0; c < s.ecolorCnt.size(); c++)
             for (int k = 0; k < s.ecolorCnt[c]; k++) {
@@ -1232,11 +1540,24 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         Tissue<R>* s = (Tissue<R>*)h; /* k_SetGravity: force = acceleration * mass (physics.cu:317-329) */            \
         for (int i = 0; i < s->N; i++) s->ext[i] = mk<R>((R)gx, (R)gy, (R)gz) * s->mass[i];                            \
     }                                                                                                                 \
-    extern "C" void oracle_set_tool_##SUF(void* h, const OracleTool* t, int toolArmed, int bladeArmed, int reset) {   \
+    extern "C" int oracle_set_tool_##SUF(void* h, const OracleTool* t, int toolArmed, int bladeArmed, int reset) {    \
         Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
         s->tool = toolFrom<R>(*t);                                                                                    \
         if (reset) s->toolFrom = s->tool;                                                                             \
         s->toolArmed = toolArmed != 0; s->bladeArmed = bladeArmed != 0;                                               \
+        return s->contactModel == 0 ? graspSetTool(*s) : 0;                                                           \
+    }                                                                                                                 \
+    extern "C" void oracle_set_contact_model_##SUF(void* h, int model, const OracleGrasp* gp) {                       \
+        setContactModel(*(Tissue<R>*)h, model, *gp);                                                                  \
+    }                                                                                                                 \
+    extern "C" int oracle_get_grasp_##SUF(void* h, int* grasped, int* core, R* ema, R* anchor, R* slabPrev) {         \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        if ((int)s->grasped.size() != s->N) return -1;                                                                \
+        for (int i = 0; i < s->N; i++) {                                                                              \
+            grasped[i] = s->grasped[i]; core[i] = s->core[i]; ema[i] = s->ema[i]; slabPrev[i] = s->slabPrev[i];       \
+            anchor[2 * i] = s->anchU[i]; anchor[2 * i + 1] = s->anchW[i];                                             \
+        }                                                                                                             \
+        return s->grasping ? 1 : 0;                                                                                   \
     }                                                                                                                 \
     extern "C" int oracle_step_##SUF(void* h, float dt) { return step(*(Tissue<R>*)h, (R)dt); }                       \
     extern "C" void oracle_detect_cut_##SUF(void* h, const OracleCutTool* t) { detectCutQuad(*(Tissue<R>*)h, *t); }   \

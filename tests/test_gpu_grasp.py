@@ -1,0 +1,75 @@
+"""Parity of the HIP legacy contact model (gts_grasp_set_tool + gts_step_legacy) with the oracle (GPU).
+
+Bar: bit-exact positions, velocities and grasp state (latch, grasped and core flags, core force EMA, ring anchors,
+previous slab multiplier) after every step of a capture / lift / hold / release sequence on the C1 block, with the
+CCD gate armed, over variants that take every branch: two iterations, a break force low enough for cores to let go,
+a frictionless ring that slips, perpendicular edge damping, and the step graph on and off.
+"""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.params import legacy_grasp_params
+from genesis_tissue_simulation_amd.workloads import block_6144
+
+pytestmark = pytest.mark.gpu
+
+
+def sequence(s, top=2.0):
+    """Jaws open above the top face, close on it (capture at step 3), lift while held, let go at step 11, then
+    move back down through the released tissue with the shaft: the CCD gate and the jaw slab see fast corners."""
+    closure = 1.0 if 3 <= s < 11 else 0.0
+    lift = 0.04 * min(max(s - 4, 0), 6)
+    drop = 0.12 * max(s - 12, 0)
+    hinge = np.array([2.0, top - 0.05 + lift - drop, 0.3])
+    return dict(shaftA=hinge - np.array([0.0, 0.0, 2.5]), shaftB=hinge, shaftR=0.12, jawHinge=hinge,
+                jawAxis=(0.0, 0.05, 1.0), jawNormal=(0.0, 1.0, 0.0), jawMid=hinge + np.array([0.0, 0.0, 0.55]),
+                jawHalfWidth=0.55, jawLen=1.1, jawGap=0.3 * (1.0 - closure), grasp01=closure)
+
+
+VARIANTS = {
+    "base": dict(),
+    "two_iterations": dict(params=dict(iterations=2)),
+    "breaking_cores": dict(grasp=dict(break_force=2e-3)),
+    "slipping_ring": dict(grasp=dict(friction=0.0)),
+    "perp_damping": dict(params=dict(beta_s=1e-4)),
+    "no_graph": dict(use_graph=False),
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_legacy_grasp_sequence_matches_oracle(variant):
+    from genesis_tissue_simulation_amd.tissue import Tissue
+    from oracle.oracle import Oracle
+
+    cfg = VARIANTS[variant]
+    w = block_6144()
+    params = w.params.with_(**cfg.get("params", {}))
+    gp = legacy_grasp_params(0.25).with_(**cfg.get("grasp", {}))
+    kd, kh = w.stiffness()
+    o = Oracle(w.grid, params, w.initial_positions)
+    o.set_tet_stiffness(kd, kh)
+    o.set_gravity(w.gravity)
+    o.set_contact_model(0, gp)
+    t = Tissue(w.grid, params, kd, kh, w.initial_positions, use_graph=cfg.get("use_graph", True))
+    t.set_gravity(w.gravity)
+    t.set_contact_model(0, gp)
+    events, cores = [], []
+    for s in range(16):
+        print(f"legacy grasp parity: {variant}, step {s}", flush=True)
+        tool = sequence(s)
+        ev = o.set_tool(tool, tool_armed=True, reset=(s == 0))
+        o.step(w.dt)
+        t.step(w.dt, tool=tool, tool_armed=True)
+        assert t.grasp_event == ev, s
+        events.append(ev)
+        np.testing.assert_array_equal(t.positions(), o.state()[0])
+        np.testing.assert_array_equal(t.velocities(), o.state()[1])
+        gt, go = t.grasp_state(), o.grasp_state()
+        assert gt["grasping"] == go["grasping"]
+        for k in ("grasped", "core", "force_ema", "anchor", "slab_lambda_prev"):
+            np.testing.assert_array_equal(gt[k], go[k], err_msg=f"{k} at step {s}")
+        cores.append(int(go["core"].sum()))
+    assert events.count(1) == 1 and events.count(2) == 1
+    assert max(cores) >= 1
+    if variant == "breaking_cores":
+        assert cores[9] < max(cores)        # some cores broke while held
