@@ -9,12 +9,16 @@ context manager keep the reference's names, argument meaning and error behaviour
 What the step does, per reference call:
   * `LCS_SetGraspTool`: the grasper (shaft capsule + two jaw plates) from the command (`native_backend.py:341-360`);
   * `LCS_SetGravity`: gravity ramped over 0.5 s (`:379-380`);
-  * `LCS_Step`: `Settings.substeps` substeps of the legacy branch with friction tool contact of the lattice nodes
-    (`xpbd/xpbd_solver.cu:3460-3721`), the material parameters `initialize` sends (`:289-325`): fallback edges only
-    (ks = 0), guarded Neo-Hookean tets, strain damping, bulk damping 0.01 s, Coulomb 0.6 / 0.4;
+  * `LCS_Step`: `Settings.substeps` substeps of the legacy branch (`xpbd/xpbd_solver.cu:3460-3721`) with the
+    material parameters `initialize` sends (`:289-325`): fallback edges only (ks = 0), guarded Neo-Hookean tets,
+    strain damping, bulk damping 0.01 s. The tool acts through the contact model `Settings.contact_model` selects
+    (`LCS_SetContactModel`, `:318`): 0 is the legacy grasp (capture / release latch, slab ring with Coulomb anchors,
+    core pins, break check) with the CCD gate, using the `GraspParams` `initialize` sends (`:314-317`); 1 and 2 are
+    the friction contact of the lattice nodes, Coulomb 0.6 / 0.4 (model 2's persistent anchors and embedded grip
+    are not part of this path);
   * the knife: a kinematic capsule whose swept surface between consecutive commands cuts the lattice
     (`LCS_SetTool` + `LCS_DetectCut`, `plugin_api.cpp:145-155`) instead of the force-driven pressure blade.
-Out of this path: the reconstructed surface (`surface()`), scissors, grasp capture and the unified VBD/PCG branch.
+Out of this path: the reconstructed surface (`surface()`), scissors and the unified VBD/PCG branch.
 """
 from __future__ import annotations
 
@@ -24,7 +28,7 @@ from pathlib import Path
 import numpy as np
 
 from .grid import METERS_PER_UNIT, TissueGrid
-from .params import TET_GUARDED, SolverParams, neohookean_stiffness
+from .params import TET_GUARDED, SolverParams, legacy_grasp_params, neohookean_stiffness
 
 # Analytic instrument dimensions (cm); the reference derives them from the instrument meshes
 # (`instrument_geometry.py`), which this path does not load.
@@ -146,6 +150,9 @@ class NativeTissue:
         tet_type = ((topo["tet_desc"] >> 1) & 7)[np.argsort(topo["tet_gen"])]
         k_dev, k_hyd = neohookean_stiffness(s.young_pa * METERS_PER_UNIT, s.poisson, topo["type_volume"], tet_type)
         self.tissue = Tissue(g, self.params, k_dev, k_hyd, g.positions, topology=topo)
+        self.legacy_grasp = int(s.contact_model) == 0
+        if self.legacy_grasp:
+            self.tissue.set_contact_model(0, legacy_grasp_params(g.spacing))
         self._tool = None
         self._blade = None
         self.command(self.initial_command)
@@ -217,7 +224,9 @@ class NativeTissue:
         gap = float(self.settings.jaw_open_gap * (1 - self._command.closure))
         if not np.isfinite(nodes).all() or not np.isfinite(reaction).all():
             raise RuntimeError("Non-finite native state; simulation stopped")
-        return Observation(nodes, self.neighbors, np.zeros(n, dtype=np.int32), hinge, gap, 0, 0.0, blade, reaction)
+        # LCS_GetGraspCoreFlags: the core corners of the legacy grasp; the friction model captures none
+        grip = t.grasp_state()["core"].astype(np.int32) if self.legacy_grasp else np.zeros(n, dtype=np.int32)
+        return Observation(nodes, self.neighbors, grip, hinge, gap, 0, 0.0, blade, reaction)
 
     def cut_edge_fractions(self):
         """First blade fraction of each directed lattice edge, f32[N, 6], -1 where uncut (`cut.cu:159-162`)."""
