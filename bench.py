@@ -112,7 +112,7 @@ def write_traffic(fetch_csv, write_csv, copy_mib, command):
     print(json.dumps(out, indent=1))
 
 
-def run_gpu(w, steps, warmup):
+def run_gpu(w, steps, warmup, tune=True):
     import ctypes as ct
 
     import torch
@@ -130,9 +130,12 @@ def run_gpu(w, steps, warmup):
         if w.blade:
             tissue.detect_cut(swept_cut_tool(w.blade(step), w.blade(step + 1), w.blade_thickness))
 
-    # colour-kernel workgroup size, tuned on scratch copies of the tissue
-    best, table = autotune.tune_plain(lambda cb: Tissue(w.grid, w.params, kd, kh, w.initial_positions,
-                                                        colour_block=cb), advance)
+    # colour-kernel workgroup size, tuned on scratch copies of the tissue (or the library default, 64)
+    if tune:
+        best, table = autotune.tune_plain(lambda cb: Tissue(w.grid, w.params, kd, kh, w.initial_positions,
+                                                            colour_block=cb), advance)
+    else:
+        best, table = {"colour_block": 64}, []
     t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, colour_block=best["colour_block"])
 
     def one(step):
@@ -166,7 +169,7 @@ def run_gpu(w, steps, warmup):
                 colour_block=best["colour_block"], tune=table)
 
 
-def run_batched(steps, warmup, count=1024):
+def run_batched(steps, warmup, count=1024, tune=True):
     """The batched workload: 1024 blocks of 2k-100k tets, each cut once, stepped in one persistent launch per step
     with the launch shape picked by the autotuner; the step kernel is timed with events on its stream."""
     import torch
@@ -182,7 +185,13 @@ def run_batched(steps, warmup, count=1024):
         b.cut(k, cut)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
-    best, table = autotune.tune(b, w.dt)
+    if tune:
+        best, table = autotune.tune(b, w.dt)
+    else:  # the shape the tuner picks on MI355X: 1024 threads, one workgroup per CU, 10240 LDS nodes, whole step
+        best = dict(block_size=1024, workgroups_per_cu=1, lds_nodes=10240, substeps_per_launch=0)
+        b.configure(**best)
+        best["n_workgroups"] = b.n_workgroups
+        table = []
     for _ in range(warmup):
         b.step(w.dt)
     torch.cuda.synchronize()
@@ -491,6 +500,9 @@ def main():
     ap.add_argument("--cutting-block", action="store_true",
                     help="measure the 2.01M-tet continuous-cutting block with 128 adjoint instances alongside and exit")
     ap.add_argument("--adjoint-iters", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuners and run their usual picks (for counter runs, so that every dispatch of a "
+                         "kernel has the launch shape the timed run uses)")
     args = ap.parse_args()
     if args.traffic:
         write_traffic(args.traffic[0], args.traffic[1], args.copy_calibration_mib, args.traffic_command)
@@ -521,7 +533,7 @@ def main():
 
     copy_gbs = stream_copy(args.copy_calibration_mib) if args.copy_calibration_mib > 0 else None
     w = kidney_250k()
-    g = run_gpu(w, args.steps, args.warmup)
+    g = run_gpu(w, args.steps, args.warmup, tune=not args.no_tune)
     substeps = w.params.substeps
     value = g["live"] * substeps * args.steps / g["elapsed"]
     b_alg = algorithmic_bytes_per_tet_substep(g["active"], g["T"], g["E"])
@@ -557,7 +569,7 @@ def main():
     }
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.skip_batched:
-        bt = run_batched(args.batched_steps, 2, args.batched_count)
+        bt = run_batched(args.batched_steps, 2, args.batched_count, tune=not args.no_tune)
         bv = bt["live"] * bt["substeps"] * args.batched_steps / bt["elapsed"]
         bb = algorithmic_bytes_per_tet_substep(bt["active"], bt["T"], bt["E"])
         bach = bb * bt["live"] * bt["substeps"] * args.batched_steps / (bt["kernel_ms"] * 1e-3) / 1e9
