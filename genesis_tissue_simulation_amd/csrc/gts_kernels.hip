@@ -317,6 +317,138 @@ __global__ void k_velscale(int N, float velScale, const float4* pos, float4* vel
     vel[i] = make_float4(v.x, v.y, v.z, 0.f);
 }
 
+// ------------------------------------------------------------------------------------------------ particle frames
+
+// Frames are stored as the three rows of R per node (RotGpu, cut.cuh:67), 9 floats.
+__global__ void k_identity_frames(int N, float* rot, float* rot_prev) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 9 * N) return;
+    float v = (i % 9) % 4 == 0 ? 1.f : 0.f;  // entries 0, 4, 8 of a row-major 3x3
+    rot[i] = v;
+    rot_prev[i] = v;
+}
+
+__global__ void k_copy_floats(int n, const float* __restrict__ src, float* __restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+struct Rows3 {
+    f3 r0, r1, r2;
+};
+
+__device__ __forceinline__ float det3(const Rows3& m) {
+    return m.r0.x * (m.r1.y * m.r2.z - m.r1.z * m.r2.y) - m.r0.y * (m.r1.x * m.r2.z - m.r1.z * m.r2.x) +
+           m.r0.z * (m.r1.x * m.r2.y - m.r1.y * m.r2.x);
+}
+
+// transpose(inverse(m)) by the adjugate (physics.cu:37-64)
+__device__ __forceinline__ Rows3 inverse_transpose3(const Rows3& m) {
+    float invDet = 1.f / det3(m);
+    f3 a0 = mk3((m.r1.y * m.r2.z - m.r1.z * m.r2.y), -(m.r0.y * m.r2.z - m.r0.z * m.r2.y),
+                (m.r0.y * m.r1.z - m.r0.z * m.r1.y));
+    f3 a1 = mk3(-(m.r1.x * m.r2.z - m.r1.z * m.r2.x), (m.r0.x * m.r2.z - m.r0.z * m.r2.x),
+                -(m.r0.x * m.r1.z - m.r0.z * m.r1.x));
+    f3 a2 = mk3((m.r1.x * m.r2.y - m.r1.y * m.r2.x), -(m.r0.x * m.r2.y - m.r0.y * m.r2.x),
+                (m.r0.x * m.r1.y - m.r0.y * m.r1.x));
+    a0 = a0 * invDet;
+    a1 = a1 * invDet;
+    a2 = a2 * invDet;
+    Rows3 t;
+    t.r0 = mk3(a0.x, a1.x, a2.x);
+    t.r1 = mk3(a0.y, a1.y, a2.y);
+    t.r2 = mk3(a0.z, a1.z, a2.z);
+    return t;
+}
+
+__device__ __forceinline__ f3 div3(f3 a, float d) { return mk3(a.x / d, a.y / d, a.z / d); }
+
+// k_ComputeParticleRot (physics.cu:109-205): per axis the through-particle edge direction (one-sided where a neighbour
+// is gone) blended with the mean previous axis of the live neighbours; a single lost axis rebuilt as the cross product
+// of the other two; R = polar(M) by 16 Higham steps R <- (R + R^-T) / 2. Pinned or inactive nodes, and degenerate,
+// reflected or unconverged frames, get the identity.
+__global__ void k_particle_frames(int N, const float4* __restrict__ pos, const int* __restrict__ nbr,
+                                  const int* __restrict__ active, const int* __restrict__ pinned,
+                                  const float* __restrict__ prev, float* __restrict__ rot) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    float* out = rot + 9 * (size_t)c;
+    Rows3 R;
+    R.r0 = mk3(1.f, 0.f, 0.f);
+    R.r1 = mk3(0.f, 1.f, 0.f);
+    R.r2 = mk3(0.f, 0.f, 1.f);
+    bool ok = pinned[c] == 0 && active[c] != 0;
+    f3 axis[3];
+    bool have[3] = {false, false, false};
+    if (ok) {
+        const f3 cPos = xyz(pos[c]);
+        for (int k = 0; k < 3; k++) {
+            int np = nbr[6 * c + 2 * k + 1], nm = nbr[6 * c + 2 * k];
+            bool okp = np >= 0 && active[np] != 0, okm = nm >= 0 && active[nm] != 0;
+            f3 e;
+            if (okp && okm) e = xyz(pos[np]) - xyz(pos[nm]);
+            else if (okp) e = xyz(pos[np]) - cPos;
+            else if (okm) e = cPos - xyz(pos[nm]);
+            else continue;
+            float len = len3(e);
+            if (len < 1e-12f) continue;
+            f3 dir = div3(e, len);
+            f3 psum = mk3(0.f, 0.f, 0.f);
+            int pc = 0;
+            for (int q = 0; q < 6; q++) {
+                int n = nbr[6 * c + q];
+                if (n >= 0 && active[n] != 0) {
+                    const float* P = prev + 9 * (size_t)n;
+                    psum = psum + mk3(P[k], P[3 + k], P[6 + k]);
+                    pc++;
+                }
+            }
+            if (pc > 0) {
+                float pl = len3(psum);
+                if (pl > 1e-6f) {
+                    f3 blended = dir + div3(psum, pl);
+                    float dl = len3(blended);
+                    if (dl > 0.25f) dir = div3(blended, dl);
+                }
+            }
+            axis[k] = dir;
+            have[k] = true;
+        }
+        int miss = -1, mc = 0;
+        for (int k = 0; k < 3; k++)
+            if (!have[k]) { miss = k; mc++; }
+        if (mc == 1) {
+            f3 cx = cross3(axis[(miss + 1) % 3], axis[(miss + 2) % 3]);
+            float l = len3(cx);
+            if (l > 1e-9f) { axis[miss] = div3(cx, l); have[miss] = true; }
+        }
+        ok = have[0] && have[1] && have[2];
+    }
+    if (ok) {
+        Rows3 M;
+        M.r0 = mk3(axis[0].x, axis[1].x, axis[2].x);
+        M.r1 = mk3(axis[0].y, axis[1].y, axis[2].y);
+        M.r2 = mk3(axis[0].z, axis[1].z, axis[2].z);
+        ok = !(det3(M) <= 1e-4f);  // the reference's rejection test, so that a NaN frame is kept as it does
+        if (ok) {
+            Rows3 H = M;
+            for (int it = 0; it < 16; it++) {
+                Rows3 t = inverse_transpose3(H);
+                H.r0 = (H.r0 + t.r0) * 0.5f;
+                H.r1 = (H.r1 + t.r1) * 0.5f;
+                H.r2 = (H.r2 + t.r2) * 0.5f;
+            }
+            f3 c0 = mk3(H.r0.x, H.r1.x, H.r2.x), c1 = mk3(H.r0.y, H.r1.y, H.r2.y), c2 = mk3(H.r0.z, H.r1.z, H.r2.z);
+            float ortho = fmaxf(fmaxf(fabsf(dot3(c0, c0) - 1.f), fabsf(dot3(c1, c1) - 1.f)), fabsf(dot3(c2, c2) - 1.f));
+            ortho = fmaxf(ortho, fmaxf(fabsf(dot3(c0, c1)), fmaxf(fabsf(dot3(c0, c2)), fabsf(dot3(c1, c2)))));
+            if (!(ortho > 1e-3f)) R = H;
+        }
+    }
+    out[0] = R.r0.x; out[1] = R.r0.y; out[2] = R.r0.z;
+    out[3] = R.r1.x; out[4] = R.r1.y; out[5] = R.r1.z;
+    out[6] = R.r2.x; out[7] = R.r2.y; out[8] = R.r2.z;
+}
+
 // ------------------------------------------------------------------------------------------------ legacy grasp kernels
 
 // The jaw frame of one substep's interpolated pose as xpbd_substep derives it for the grasp kernels
@@ -823,11 +955,12 @@ __global__ void k_write_remap(int n, Flag flag, const int* partial, int* remap) 
 }
 
 // EmitCutPoint (cut.cu:186-215) for the edges marked by the detection, in lattice edge order: slot = counter +
-// 2 * (rank of the edge among the emitting ones) for the owner A and + 1 for B; the particle rotation is identity.
+// 2 * (rank of the edge among the emitting ones) for the owner A and + 1 for B.
 __global__ void k_emit_points(int Eg, Lattice L, f3 ncutQuad, int useNormals, float toolD, float voxelL,
                               const int* __restrict__ grid_edges, const float4* __restrict__ pos,
-                              const float* __restrict__ rest, const float4* records, const float4* normals,
-                              const int* partial, uint32_t* counters, float* points, int capacity) {
+                              const float* __restrict__ rest, const float* __restrict__ rot, const float4* records,
+                              const float4* normals, const int* partial, uint32_t* counters, float* points,
+                              int capacity) {
     __shared__ int lds[GTS_BLOCK / 64];
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     bool f = e < Eg && records[e].x >= 0.f;
@@ -857,9 +990,13 @@ __global__ void k_emit_points(int Eg, Lattice L, f3 ncutQuad, int useNormals, fl
             if (side == 0.f) side = 1.f;
             uint32_t slot = slot0 + (uint32_t)k;
             if (slot >= (uint32_t)capacity) continue;
-            f3 gap = ncut * (side * (toolD * 0.5f));
+            // the gap and the normal in the owner's material frame: R^T v (mulRtv, cut.cuh:75-78)
+            const float* Rp = rot + 9 * (size_t)p;
+            const f3 r0 = ld3(Rp), r1 = ld3(Rp + 3), r2 = ld3(Rp + 6);
+            f3 g = ncut * (side * (toolD * 0.5f)), sn = ncut * side;
+            f3 gap = r0 * g.x + r1 * g.y + r2 * g.z;
             f3 off = along[k] + gap;
-            f3 nrm = ncut * side;
+            f3 nrm = r0 * sn.x + r1 * sn.y + r2 * sn.z;
             float* cp = points + 8 * (size_t)slot;
             cp[0] = off.x; cp[1] = off.y; cp[2] = off.z;
             cp[3] = __int_as_float(p);
@@ -1003,6 +1140,14 @@ int scan_blocks(int n) { return grid_for(n); }
 // scratch layout in floats: [4*Eg) detection records, [4*Eg, 8*Eg) CCD normals, then int partials
 int* partial_area(const GtsTissueView* v) { return reinterpret_cast<int*>(v->scratch + 8 * (size_t)v->n_grid_edges); }
 
+// physics_compute_rotations (physics.cu:399-411): the current frames become the previous ones, then new frames.
+void launch_frames(const GtsTissueView* v, hipStream_t s) {
+    const int N = v->n_nodes;
+    k_copy_floats<<<grid_for(9 * N), GTS_BLOCK, 0, s>>>(9 * N, v->rot, v->rot_prev);
+    k_particle_frames<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, (const float4*)v->pos, v->nbr, v->active, v->pinned,
+                                                        v->rot_prev, v->rot);
+}
+
 int launch_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
     k_topology_stale<<<1, 1, 0, s>>>(v->counters);
     k_clear_support<<<grid_for(6 * v->n_nodes), GTS_BLOCK, 0, s>>>(6 * v->n_nodes, v->slot_support, v->counters);
@@ -1082,8 +1227,8 @@ int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut,
     k_count_flags<<<nb, GTS_BLOCK, 0, s>>>(Eg, FlagRecords{records}, partial);
     k_scan_partials<<<1, GTS_BLOCK, 0, s>>>(nb, partial);
     k_emit_points<<<nb, GTS_BLOCK, 0, s>>>(Eg, L, ncut, useNormals, toolD, v->voxel_length, v->grid_edges,
-                                           (const float4*)v->pos, v->rest, records, normals, partial, v->counters,
-                                           v->cut_points, v->cut_point_capacity);
+                                           (const float4*)v->pos, v->rest, v->rot, records, normals, partial,
+                                           v->counters, v->cut_points, v->cut_point_capacity);
     k_bump_counter<<<1, 1, 0, s>>>(partial, nb, v->counters);
     k_sever<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, v->grid_edges, v->cut_mask, v->nbr, v->bend_alive, v->counters);
     return launch_refresh(v, L, s);
@@ -1096,8 +1241,10 @@ int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut,
 extern "C" int gts_init_state(const GtsTissueView* v, void* stream) {
     if (!v || v->n_nodes <= 0) return GTS_ERR_BAD_ARG;
     hipStream_t s = S(stream);
+    if (!v->rot || !v->rot_prev || !v->rest) return GTS_ERR_BAD_ARG;
     k_init_state<<<grid_for(v->n_nodes), GTS_BLOCK, 0, s>>>(v->n_nodes, v->mass, v->active, v->pinned, v->nbr,
                                                             (float4*)v->pos, v->bend_alive, v->counters);
+    k_identity_frames<<<grid_for(9 * v->n_nodes), GTS_BLOCK, 0, s>>>(v->n_nodes, v->rot, v->rot_prev);
     HIP_TRY(hipGetLastError());
     Lattice L = lattice_of(v);
     return launch_refresh(v, L, s);
@@ -1114,6 +1261,12 @@ extern "C" int gts_refresh_liveness(const GtsTissueView* v, void* stream) {
     if (!v) return GTS_ERR_BAD_ARG;
     Lattice L = lattice_of(v);
     return launch_refresh(v, L, S(stream));
+}
+
+extern "C" int gts_compute_rotations(const GtsTissueView* v, void* stream) {
+    if (!v || !v->rot || !v->rot_prev || v->n_nodes <= 0) return GTS_ERR_BAD_ARG;
+    launch_frames(v, S(stream));
+    return -(int)hipGetLastError();
 }
 
 namespace {
@@ -1221,6 +1374,7 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
             k_velscale<<<g, b, 0, s>>>(N, P.velScale, (const float4*)v->pos, (float4*)v->vel);
         }
     }
+    launch_frames(v, s);  // LCS_Step ends with physics_compute_rotations (plugin_api.cpp:133)
 }
 
 // Captured steps, one per (view, plan) pair: replaying the graph issues the same launches with the same arguments;

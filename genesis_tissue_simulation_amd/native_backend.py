@@ -180,6 +180,7 @@ class NativeTissue:
         self._command = command
 
     def _advance(self):
+        """LCS_Step (the step and its frame update), the knife's LCS_DetectCut, then LCS_Finalize's frame update."""
         self.tissue.step(self.settings.dt, tool=self._tool, tool_armed=True)
         nxt = self._blade_next
         if nxt is not None and self._blade is not None:
@@ -188,6 +189,7 @@ class NativeTissue:
             if tool["valid"]:
                 self.tissue.detect_cut(tool)
         self._blade = nxt
+        self.tissue.finalize()
 
     def step(self, command: Command):
         if self.closed:

@@ -174,6 +174,8 @@ class Tissue:
         self.edge_cut_t = torch.full((6 * N,), -1.0, dtype=torch.float32, device=dev)
         self.edge_skin_t = f32(grid.edge_skin_fraction())
         self.rest = f32(np.asarray(grid.positions, dtype=np.float32).reshape(-1))
+        self.rot = torch.zeros(9 * N, dtype=torch.float32, device=dev)
+        self.rot_prev = torch.zeros(9 * N, dtype=torch.float32, device=dev)
         self.cut_point_capacity = 4 * max(Eg, 1)
         self.cut_points = torch.zeros(8 * self.cut_point_capacity, dtype=torch.float32, device=dev)
         self.counters = torch.zeros(8, dtype=torch.int32, device=dev)
@@ -194,7 +196,8 @@ class Tissue:
                                      self.inside, self.nbr, self.bend_alive, self.edge_desc, self.edge_lambda,
                                      self.slot_support, self.tet_desc, self.tet_k_dev, self.tet_k_hyd,
                                      self.tet_alpha, self.tet_lambda, self.grid_edges, self.occupied, self.cut_mask,
-                                     self.edge_cut_t, self.edge_skin_t, self.rest, self.cut_points)],
+                                     self.edge_cut_t, self.edge_skin_t, self.rest, self.rot, self.rot_prev,
+                                     self.cut_points)],
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
             self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
         self._tool_from = None
@@ -276,6 +279,15 @@ class Tissue:
             ct.byref(self.view), *[v.ctypes.data_as(ct.c_void_p) for v in vs], float(tool_d), _ptr(pp),
             current_stream()))
         torch.cuda.current_stream().synchronize()
+
+    def finalize(self):
+        """LCS_Finalize's frame update after a cut: the particle frames from the current positions and links
+        (physics_compute_rotations). step() already ends with one, as LCS_Step does."""
+        check("gts_compute_rotations", self.lib.gts_compute_rotations(ct.byref(self.view), current_stream()))
+
+    def rotations(self):
+        """Particle frames, [N, 3, 3] rows of R."""
+        return self.rot.cpu().numpy().reshape(-1, 3, 3)
 
     def tear(self, tau, tool_d):
         """Tear law: every intact link stretched past tau times its rest length is cut at its midpoint, then the

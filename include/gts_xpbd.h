@@ -111,6 +111,9 @@ typedef struct GtsTissueView {
     const float* edge_skin_t;    /* [6N] */
     const float* rest;           /* [3N] corner rest positions (recon_corner_rest, dc_recon.cu:671): rest edge of cut
                                   * points and tear-law scale */
+    float* rot;                  /* [9N] particle frames, rows of R per node (RotGpu, cut.cuh:67): the material frame of
+                                  * the cut points' gap and normal */
+    float* rot_prev;             /* [9N] the frames before the last update (the update's temporal blend) */
     float* cut_points;           /* [8*capacity] CutPointGpu records (cut.cuh:37-46) */
     int cut_point_capacity;
     uint32_t* counters;          /* [8] 0 revision, 1 cut point counter, 2 refreshed revision, 3 stale flag,
@@ -208,7 +211,8 @@ int gts_init_state(const GtsTissueView* v, void* stream);
 int gts_set_gravity(const GtsTissueView* v, float gx, float gy, float gz, void* stream);
 /* Device: re-derive edge / tet liveness and the fallback tet support when the topology revision moved. */
 int gts_refresh_liveness(const GtsTissueView* v, void* stream);
-/* Device: one outer step of p->substeps substeps. The tool pose moves from `from` to `to` over the step. */
+/* Device: one outer step of p->substeps substeps, then the particle frames (LCS_Step, plugin_api.cpp:125-134). The
+ * tool pose moves from `from` to `to` over the step. */
 int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from, const GtsToolState* to,
              int tool_armed, int blade_armed, void* stream);
 /* Host + device: the legacy contact model's part of xpbd_set_tool (xpbd_solver.cu:2318-2341, :2372-2470) for the
@@ -226,6 +230,10 @@ int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, const GtsGraspP
  * follows the pose from `from` to `to` over the substeps. The contact fields of *p are not used, nor the reaction. */
 int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp, const GtsGraspState* g,
                     const GtsToolState* from, const GtsToolState* to, int tool_armed, void* stream);
+/* Device: physics_compute_rotations (physics.cu:399-411) as LCS_Finalize runs it after a cut (plugin_api.cpp:159-166):
+ * the current frames become the previous ones and k_ComputeParticleRot (physics.cu:109-205) derives new ones from the
+ * current positions and links. gts_step and gts_step_legacy end with the same update. */
+int gts_compute_rotations(const GtsTissueView* v, void* stream);
 /* Device: swept-quad cut of the current positions, then sever the cut links and refresh liveness. */
 int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream);
 /* Device: CCD of the edges moving from prev_positions ([3N] device) to the current positions against blade S->E. */

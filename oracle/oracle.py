@@ -84,6 +84,8 @@ def lib():
             getattr(L, f"oracle_detect_cut_ccd_{suf}").argtypes = [P, P, P, P, F, P]
             getattr(L, f"oracle_refresh_{suf}").argtypes = [P]
             getattr(L, f"oracle_tear_{suf}").argtypes = [P, F, F]
+            getattr(L, f"oracle_compute_rotations_{suf}").argtypes = [P]
+            getattr(L, f"oracle_get_rotations_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_set_contact_model_{suf}").argtypes = [P, I, ct.POINTER(OracleGrasp)]
             getattr(L, f"oracle_get_grasp_{suf}").argtypes = [P] * 6
             getattr(L, f"oracle_get_grasp_{suf}").restype = I
@@ -236,6 +238,17 @@ class Oracle:
         """Tear law (k_TearOverstretch + k_SeverLinks): links stretched past tau times their rest length part at
         their midpoint."""
         self._call("tear", self.h, float(tau), float(tool_d))
+
+    def compute_rotations(self):
+        """physics_compute_rotations: the particle frames from the current positions (LCS_Finalize's update; step()
+        already ends with one)."""
+        self._call("compute_rotations", self.h)
+
+    def rotations(self):
+        """Particle frames, [N, 3, 3] rows of R."""
+        out = np.zeros((self.N, 3, 3), dtype=self.real)
+        self._call("get_rotations", self.h, _ptr(out))
+        return out
 
     def state(self):
         pos = np.zeros((self.N, 3), dtype=self.real)

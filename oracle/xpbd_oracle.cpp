@@ -394,6 +394,8 @@ template <class R> struct Tissue {
     R sp[3] = {0, 0, 0};
     // node state (AoS, as the reference's float3 buffers)
     std::vector<V3<R>> pos, prev, vel, ext, rest;
+    // particle frames (RotGpu rows r0, r1, r2 per node) and their previous values (physics.cu:76-77)
+    std::vector<V3<R>> rot, rotPrev;
     std::vector<R> mass, invMass;
     std::vector<int> active, pinned, inside, nbr, nbr0;
     // bend pairs (grid.py BEND_DTYPE / BendPairGpuC): only (j, k, alive) are read on this path
@@ -1169,7 +1171,111 @@ template <class R> int substep(Tissue<R>& s) {
     return 0;
 }
 
-template <class R> int step(Tissue<R>& s, R dt) {  // physics_step (physics.cu:331-394), no ribbon tick
+// 3x3 matrices as three rows (physics.cu:27-64).
+template <class R> struct M3 { V3<R> r0, r1, r2; };
+template <class R> R det3(const M3<R>& m) {
+    return m.r0.x * (m.r1.y * m.r2.z - m.r1.z * m.r2.y) - m.r0.y * (m.r1.x * m.r2.z - m.r1.z * m.r2.x) +
+           m.r0.z * (m.r1.x * m.r2.y - m.r1.y * m.r2.x);
+}
+template <class R> M3<R> inverse3(const M3<R>& m) {  // adjugate times 1/det
+    R invDet = (R)1 / det3(m);
+    M3<R> a;
+    a.r0 = mk<R>((m.r1.y * m.r2.z - m.r1.z * m.r2.y), -(m.r0.y * m.r2.z - m.r0.z * m.r2.y),
+                 (m.r0.y * m.r1.z - m.r0.z * m.r1.y));
+    a.r1 = mk<R>(-(m.r1.x * m.r2.z - m.r1.z * m.r2.x), (m.r0.x * m.r2.z - m.r0.z * m.r2.x),
+                 -(m.r0.x * m.r1.z - m.r0.z * m.r1.x));
+    a.r2 = mk<R>((m.r1.x * m.r2.y - m.r1.y * m.r2.x), -(m.r0.x * m.r2.y - m.r0.y * m.r2.x),
+                 (m.r0.x * m.r1.y - m.r0.y * m.r1.x));
+    return M3<R>{a.r0 * invDet, a.r1 * invDet, a.r2 * invDet};
+}
+template <class R> M3<R> transpose3(const M3<R>& m) {
+    return M3<R>{mk<R>(m.r0.x, m.r1.x, m.r2.x), mk<R>(m.r0.y, m.r1.y, m.r2.y), mk<R>(m.r0.z, m.r1.z, m.r2.z)};
+}
+template <class R> V3<R> divs(V3<R> a, R d) { return mk<R>(a.x / d, a.y / d, a.z / d); }
+
+// k_ComputeParticleRot (physics.cu:109-205) for every node, after the current frames became the previous ones
+// (physics_compute_rotations, physics.cu:399-411): per axis the through-particle edge direction (one-sided where a
+// neighbour is gone) blended with the mean previous axis of the live neighbours, a single lost axis recovered as the
+// cross product of the other two, R = polar(M) by 16 Higham steps; identity for pinned / inactive nodes and for
+// degenerate, reflected or unconverged frames.
+template <class R> void computeRotations(Tissue<R>& s) {
+    s.rotPrev = s.rot;
+    const V3<R> I0 = mk<R>(1, 0, 0), I1 = mk<R>(0, 1, 0), I2 = mk<R>(0, 0, 1);
+    for (int c = 0; c < s.N; c++) {
+        V3<R>* out = &s.rot[3 * c];
+        out[0] = I0; out[1] = I1; out[2] = I2;
+        if (s.pinned[c] != 0 || s.active[c] == 0) continue;
+        V3<R> axis[3];
+        bool have[3] = {false, false, false};
+        for (int k = 0; k < 3; k++) {
+            int np = s.nbr[6 * c + 2 * k + 1], nm = s.nbr[6 * c + 2 * k];
+            bool okp = np >= 0 && s.active[np] != 0, okm = nm >= 0 && s.active[nm] != 0;
+            V3<R> e;
+            if (okp && okm) e = s.pos[np] - s.pos[nm];
+            else if (okp) e = s.pos[np] - s.pos[c];
+            else if (okm) e = s.pos[c] - s.pos[nm];
+            else continue;
+            R len = len3(e);
+            if (len < (R)1e-12f) continue;
+            V3<R> dir = divs(e, len);
+            V3<R> psum = mk<R>(0, 0, 0);
+            int pc = 0;
+            for (int q = 0; q < 6; q++) {
+                int n = s.nbr[6 * c + q];
+                if (n >= 0 && s.active[n] != 0) {
+                    const V3<R>* rp = &s.rotPrev[3 * n];
+                    psum = psum + mk<R>(comp(rp[0], k), comp(rp[1], k), comp(rp[2], k));
+                    pc++;
+                }
+            }
+            if (pc > 0) {
+                R pl = len3(psum);
+                if (pl > (R)1e-6f) {
+                    V3<R> blended = dir + divs(psum, pl);
+                    R dl = len3(blended);
+                    if (dl > (R)0.25f) dir = divs(blended, dl);
+                }
+            }
+            axis[k] = dir;
+            have[k] = true;
+        }
+        int miss = -1, mc = 0;
+        for (int k = 0; k < 3; k++)
+            if (!have[k]) { miss = k; mc++; }
+        if (mc == 1) {
+            V3<R> cx = cross3(axis[(miss + 1) % 3], axis[(miss + 2) % 3]);
+            R l = len3(cx);
+            if (l > (R)1e-9f) { axis[miss] = divs(cx, l); have[miss] = true; }
+        }
+        if (!(have[0] && have[1] && have[2])) continue;
+        M3<R> M{mk<R>(axis[0].x, axis[1].x, axis[2].x), mk<R>(axis[0].y, axis[1].y, axis[2].y),
+                mk<R>(axis[0].z, axis[1].z, axis[2].z)};
+        if (det3(M) <= (R)1e-4f) continue;
+        M3<R> Rm = M;
+        for (int it = 0; it < 16; it++) {
+            M3<R> t = transpose3(inverse3(Rm));
+            Rm.r0 = (Rm.r0 + t.r0) * (R)0.5f;
+            Rm.r1 = (Rm.r1 + t.r1) * (R)0.5f;
+            Rm.r2 = (Rm.r2 + t.r2) * (R)0.5f;
+        }
+        V3<R> c0 = mk<R>(Rm.r0.x, Rm.r1.x, Rm.r2.x), c1 = mk<R>(Rm.r0.y, Rm.r1.y, Rm.r2.y),
+              c2 = mk<R>(Rm.r0.z, Rm.r1.z, Rm.r2.z);
+        R ortho = fmax_<R>(fmax_<R>(std::fabs(dot3(c0, c0) - (R)1), std::fabs(dot3(c1, c1) - (R)1)),
+                           std::fabs(dot3(c2, c2) - (R)1));
+        ortho = fmax_<R>(ortho, fmax_<R>(std::fabs(dot3(c0, c1)), fmax_<R>(std::fabs(dot3(c0, c2)),
+                                                                            std::fabs(dot3(c1, c2)))));
+        if (ortho > (R)1e-3f) continue;
+        out[0] = Rm.r0; out[1] = Rm.r1; out[2] = Rm.r2;
+    }
+}
+
+// mulRtv (cut.cuh:75-78): R^T v with R as rows.
+template <class R> V3<R> mulRtv(const V3<R>* rows, V3<R> v) {
+    return rows[0] * v.x + rows[1] * v.y + rows[2] * v.z;
+}
+
+// LCS_Step (plugin_api.cpp:125-134): physics_step (physics.cu:331-394, no ribbon tick), then the particle frames.
+template <class R> int step(Tissue<R>& s, R dt) {
     int rc = beginFrame(s, dt);
     if (rc) return rc;
     int N = s.d.substeps > 0 ? s.d.substeps : 1;
@@ -1179,6 +1285,7 @@ template <class R> int step(Tissue<R>& s, R dt) {  // physics_step (physics.cu:3
         rc = substep(s);
         if (rc) return rc;
     }
+    computeRotations(s);
     return 0;
 }
 
@@ -1206,7 +1313,7 @@ inline bool cutHitsMaterial(float t, float skinT, int insideA, int insideB) {  /
     return insideA ? t <= skinT : t >= skinT;
 }
 
-// MarkAndEmit + EmitCutPoint (cut.cu:186-254); the particle rotation is identity on this path.
+// MarkAndEmit + EmitCutPoint (cut.cu:186-254), with the owners' particle frames of the last computeRotations.
 template <class R> void markAndEmit(Tissue<R>& s, int e, int axis, const int bc[3], int idA, int idB, R t, V3<R> hit,
                                     V3<R> ncut, R toolD) {
     bool firstCut = false, sawAny = false, occupiedEdge = false;
@@ -1240,9 +1347,10 @@ template <class R> void markAndEmit(Tissue<R>& s, int e, int axis, const int bc[
         if (side == 0) side = 1;
         uint32_t slot = s.cutPointCounter++;
         if (slot >= (uint32_t)s.d.cut_point_capacity) continue;
-        V3<R> gap = ncut * (side * (toolD * (R)0.5f));
+        // the gap and the normal in the owner's material frame, R^T applied (EmitCutPoint, cut.cu:201-208)
+        V3<R> gap = mulRtv(&s.rot[3 * p], ncut * (side * (toolD * (R)0.5f)));
         V3<R> off = along[k] + gap;
-        V3<R> nrm = ncut * side;
+        V3<R> nrm = mulRtv(&s.rot[3 * p], ncut * side);
         CutPoint cp;
         cp.off[0] = (float)off.x; cp.off[1] = (float)off.y; cp.off[2] = (float)off.z;
         cp.owner = p;
@@ -1457,6 +1565,11 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
     s->V = d->vx * d->vy * d->vz;
     int N = s->N;
     s->pos.resize(N); s->prev.resize(N); s->vel.resize(N); s->ext.resize(N); s->rest.resize(N);
+    s->rot.resize(3 * (size_t)N); s->rotPrev.resize(3 * (size_t)N);
+    for (int i = 0; i < N; i++) {  // identity frames (physics.cu:260-268)
+        s->rot[3 * i] = mk<R>(1, 0, 0); s->rot[3 * i + 1] = mk<R>(0, 1, 0); s->rot[3 * i + 2] = mk<R>(0, 0, 1);
+    }
+    s->rotPrev = s->rot;
     s->mass.resize(N); s->invMass.resize(N);
     for (int i = 0; i < N; i++) {
         s->rest[i] = v3<R>(rest + 3 * i);
@@ -1569,6 +1682,13 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         detectCutCCD(*s, v3<R>(S), v3<R>(E), v3<R>(ncut), (R)toolD, pp);                                              \
     }                                                                                                                 \
     extern "C" void oracle_tear_##SUF(void* h, float tau, float toolD) { tearOverstretch(*(Tissue<R>*)h, tau, (R)toolD); } \
+    extern "C" void oracle_compute_rotations_##SUF(void* h) { computeRotations(*(Tissue<R>*)h); }                    \
+    extern "C" void oracle_get_rotations_##SUF(void* h, R* out) {                                                     \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        for (size_t i = 0; i < s->rot.size(); i++) {                                                                  \
+            out[3 * i] = s->rot[i].x; out[3 * i + 1] = s->rot[i].y; out[3 * i + 2] = s->rot[i].z;                     \
+        }                                                                                                             \
+    }                                                                                                                 \
     extern "C" void oracle_refresh_##SUF(void* h) { refreshLiveness(*(Tissue<R>*)h); }                                \
     extern "C" void oracle_get_state_##SUF(void* h, R* pos, R* vel) {                                                 \
         Tissue<R>* s = (Tissue<R>*)h;                                                                                 \

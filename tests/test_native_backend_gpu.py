@@ -70,6 +70,7 @@ def test_step_matches_oracle():
         o.set_tool(tissue_tool, tool_armed=True, reset=True)
         o.set_gravity((0, 0, 0))
         o.step(s.dt)
+        o.compute_rotations()
         blade = tissue._blade
         for k, cmd in enumerate(commands(10)):
             obs = tissue.step(cmd)
@@ -81,6 +82,7 @@ def test_step_matches_oracle():
             tool = swept_cut_tool(blade, nxt, 2 * s.blade_radius)
             if tool["valid"]:
                 o.detect_cut(tool)
+            o.compute_rotations()
             blade = nxt
         po = o.state()[0]
         err = float(np.abs(obs.nodes - po).max() / np.abs(po).max())
@@ -88,6 +90,8 @@ def test_step_matches_oracle():
         assert err <= 1e-5
         np.testing.assert_array_equal(obs.nodes, po)
         np.testing.assert_array_equal(obs.neighbors, o.cut_state()["neighbors"])
+        np.testing.assert_array_equal(tissue.tissue.cut_state()["cut_points"], o.cut_state()["cut_points"])
+        np.testing.assert_array_equal(tissue.tissue.rotations(), o.rotations())
 
 
 def test_legacy_contact_model_grasps_like_the_oracle():
@@ -108,6 +112,7 @@ def test_legacy_contact_model_grasps_like_the_oracle():
         o.set_tool(tissue._tool, tool_armed=True, reset=True)
         o.set_gravity((0, 0, 0))
         o.step(s.dt)
+        o.compute_rotations()
         held = 0
         for k in range(14):
             closure = 0.0 if k < 3 else (1.0 if k < 10 else 0.0)
@@ -116,6 +121,7 @@ def test_legacy_contact_model_grasps_like_the_oracle():
             o.set_gravity((0, -981.0 * min(1.0, (k + 1) * s.dt / 0.5), 0))
             o.set_tool(tissue._tool, tool_armed=True)
             o.step(s.dt)
+            o.compute_rotations()
             np.testing.assert_array_equal(obs.nodes, o.state()[0])
             np.testing.assert_array_equal(obs.grip_flags, o.grasp_state()["core"])
             held = max(held, int(obs.grip_flags.sum()))
