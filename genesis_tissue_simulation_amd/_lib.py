@@ -76,7 +76,8 @@ class GtsBatchView(ct.Structure):
                 ("n_workgroups", I), ("block_size", I), ("lds_nodes", I), ("use_graph", I),
                 ("substeps_per_launch", I), ("gravity", F * 3),
                 ("pos", P), ("prev", P), ("vel", P), ("mass", P), ("edge_desc", P), ("tet_desc", P), ("k_dev", P),
-                ("k_hyd", P), ("lambda_edge", P), ("lambda_tet", P), ("alpha", P)]
+                ("k_hyd", P), ("lambda_edge", P), ("lambda_tet", P), ("alpha", P), ("total_nodes", ct.c_longlong),
+                ("nbr", P), ("active", P), ("pinned", P), ("node_base", P), ("rot", P), ("rot_prev", P)]
 
 
 class GtsAdjointView(ct.Structure):

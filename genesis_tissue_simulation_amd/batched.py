@@ -51,6 +51,13 @@ class Batch:
         self.lambda_edge = torch.zeros(max(sE, 1), dtype=torch.float32, device=dev)
         self.lambda_tet = torch.zeros(max(2 * sT, 1), dtype=torch.float32, device=dev)
         self.alpha = torch.zeros(max(2 * sT, 1), dtype=torch.float32, device=dev)
+        # links, flags and particle frames of every instance, for the frame update that ends each step
+        self.nbr = torch.zeros(6 * sN, dtype=torch.int32, device=dev)
+        self.active = torch.zeros(sN, dtype=torch.int32, device=dev)
+        self.pinned = torch.zeros(sN, dtype=torch.int32, device=dev)
+        self.rot = torch.zeros(9 * sN, dtype=torch.float32, device=dev)
+        self.rot_prev = torch.zeros(9 * sN, dtype=torch.float32, device=dev)
+        self.node_base = torch.as_tensor(np.repeat(self.node_off[:-1], Ns).astype(np.int32)).to(dev)
         self.tissues = []
         records = (GtsBatchInstance * len(instances))()
         colour_rows = []
@@ -62,7 +69,9 @@ class Batch:
             storage = dict(pos=self.pos[n0:n1], prev=self.prev[n0:n1], vel=self.vel[n0:n1], mass=self.mass[n0:n1],
                            edge_desc=self.edge_desc[e0:e1], edge_lambda=self.lambda_edge[e0:e1],
                            tet_desc=self.tet_desc[t0:t1], tet_k_dev=self.k_dev[t0:t1],
-                           tet_k_hyd=self.k_hyd[t0:t1], tet_lambda=self.lambda_tet[2 * t0:2 * t1])
+                           tet_k_hyd=self.k_hyd[t0:t1], tet_lambda=self.lambda_tet[2 * t0:2 * t1],
+                           nbr=self.nbr[6 * n0:6 * n1], active=self.active[n0:n1], pinned=self.pinned[n0:n1],
+                           rot=self.rot[9 * n0:9 * n1], rot_prev=self.rot_prev[9 * n0:9 * n1])
             self.tissues.append(Tissue(grid, p_k, kd, kh, init, device=device, topology=topo, storage=storage,
                                        use_graph=False))
             r = records[k]
@@ -113,7 +122,9 @@ class Batch:
             self.work_items.data_ptr(), n_workgroups, self.block_size, self.lds_nodes, int(bool(self.use_graph)),
             self.substeps_per_launch, (ct.c_float * 3)(*self.gravity), self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(),
             self.mass.data_ptr(), self.edge_desc.data_ptr(), self.tet_desc.data_ptr(), self.k_dev.data_ptr(),
-            self.k_hyd.data_ptr(), self.lambda_edge.data_ptr(), self.lambda_tet.data_ptr(), self.alpha.data_ptr())
+            self.k_hyd.data_ptr(), self.lambda_edge.data_ptr(), self.lambda_tet.data_ptr(), self.alpha.data_ptr(),
+            int(self.node_off[-1]), self.nbr.data_ptr(), self.active.data_ptr(), self.pinned.data_ptr(),
+            self.node_base.data_ptr(), self.rot.data_ptr(), self.rot_prev.data_ptr())
         return self
 
     def step(self, dt, params=None):

@@ -22,6 +22,7 @@
 #include "../../include/gts_xpbd.h"
 #include "gts_common.h"
 #include "gts_solver.h"
+#include "gts_frames.h"
 
 #define HIP_TRY(call)                                   \
     do {                                                \
@@ -228,6 +229,20 @@ int launch_range(const GtsBatchView* v, const BatchStep& P, bool damped, hipStre
 }
 
 // The step as persistent launches of substeps_per_launch substeps each (one launch when 0).
+__global__ void k_batch_copy(long long n, const float* __restrict__ src, float* __restrict__ dst) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// k_ComputeParticleRot (physics.cu:109-205) over the pooled nodes of every instance.
+__global__ void k_batched_frames(long long n, const float4* __restrict__ pos, const int* __restrict__ nbr,
+                                 const int* __restrict__ active, const int* __restrict__ pinned,
+                                 const int* __restrict__ base, const float* __restrict__ prev,
+                                 float* __restrict__ rot) {
+    long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) particle_frame((int)g, base[g], pos, nbr, active, pinned, prev, rot);
+}
+
 int enqueue(const GtsBatchView* v, BatchStep P, bool damped, hipStream_t s) {
     const int per = v->substeps_per_launch > 0 ? v->substeps_per_launch : P.substeps;
     for (int b = 0; b < P.substeps; b += per) {
@@ -235,6 +250,13 @@ int enqueue(const GtsBatchView* v, BatchStep P, bool damped, hipStream_t s) {
         P.sub_end = b + per < P.substeps ? b + per : P.substeps;
         int rc = launch_range(v, P, damped, s);
         if (rc) return rc;
+    }
+    if (v->rot) {  // the frame update that ends LCS_Step (plugin_api.cpp:133), for every instance at once
+        const long long n = v->total_nodes;
+        k_batch_copy<<<(unsigned)((9 * n + 255) / 256), 256, 0, s>>>(9 * n, v->rot, v->rot_prev);
+        k_batched_frames<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (const float4*)v->pos, v->nbr, v->active,
+                                                                     v->pinned, v->node_base, v->rot_prev, v->rot);
+        HIP_TRY(hipGetLastError());
     }
     return GTS_OK;
 }
@@ -281,6 +303,9 @@ extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, v
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
     if ((long long)v->lds_nodes * 16 > (long long)gts_batched_lds_bytes()) return GTS_ERR_BAD_ARG;
     if (v->substeps_per_launch < 0) return GTS_ERR_BAD_ARG;
+    if (v->rot && (!v->rot_prev || !v->nbr || !v->active || !v->pinned || !v->node_base || v->total_nodes <= 0 ||
+                   v->total_nodes > (long long)INT32_MAX))
+        return GTS_ERR_BAD_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     BatchStep P;
     std::memset(&P, 0, sizeof(P));

@@ -23,6 +23,7 @@
 #include "gts_common.h"
 #include "gts_contact.h"
 #include "gts_solver.h"
+#include "gts_frames.h"
 
 #define GTS_BLOCK 256
 
@@ -333,120 +334,12 @@ __global__ void k_copy_floats(int n, const float* __restrict__ src, float* __res
     if (i < n) dst[i] = src[i];
 }
 
-struct Rows3 {
-    f3 r0, r1, r2;
-};
-
-__device__ __forceinline__ float det3(const Rows3& m) {
-    return m.r0.x * (m.r1.y * m.r2.z - m.r1.z * m.r2.y) - m.r0.y * (m.r1.x * m.r2.z - m.r1.z * m.r2.x) +
-           m.r0.z * (m.r1.x * m.r2.y - m.r1.y * m.r2.x);
-}
-
-// transpose(inverse(m)) by the adjugate (physics.cu:37-64)
-__device__ __forceinline__ Rows3 inverse_transpose3(const Rows3& m) {
-    float invDet = 1.f / det3(m);
-    f3 a0 = mk3((m.r1.y * m.r2.z - m.r1.z * m.r2.y), -(m.r0.y * m.r2.z - m.r0.z * m.r2.y),
-                (m.r0.y * m.r1.z - m.r0.z * m.r1.y));
-    f3 a1 = mk3(-(m.r1.x * m.r2.z - m.r1.z * m.r2.x), (m.r0.x * m.r2.z - m.r0.z * m.r2.x),
-                -(m.r0.x * m.r1.z - m.r0.z * m.r1.x));
-    f3 a2 = mk3((m.r1.x * m.r2.y - m.r1.y * m.r2.x), -(m.r0.x * m.r2.y - m.r0.y * m.r2.x),
-                (m.r0.x * m.r1.y - m.r0.y * m.r1.x));
-    a0 = a0 * invDet;
-    a1 = a1 * invDet;
-    a2 = a2 * invDet;
-    Rows3 t;
-    t.r0 = mk3(a0.x, a1.x, a2.x);
-    t.r1 = mk3(a0.y, a1.y, a2.y);
-    t.r2 = mk3(a0.z, a1.z, a2.z);
-    return t;
-}
-
-__device__ __forceinline__ f3 div3(f3 a, float d) { return mk3(a.x / d, a.y / d, a.z / d); }
-
-// k_ComputeParticleRot (physics.cu:109-205): per axis the through-particle edge direction (one-sided where a neighbour
-// is gone) blended with the mean previous axis of the live neighbours; a single lost axis rebuilt as the cross product
-// of the other two; R = polar(M) by 16 Higham steps R <- (R + R^-T) / 2. Pinned or inactive nodes, and degenerate,
-// reflected or unconverged frames, get the identity.
+// k_ComputeParticleRot (physics.cu:109-205) over one tissue's nodes.
 __global__ void k_particle_frames(int N, const float4* __restrict__ pos, const int* __restrict__ nbr,
                                   const int* __restrict__ active, const int* __restrict__ pinned,
                                   const float* __restrict__ prev, float* __restrict__ rot) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= N) return;
-    float* out = rot + 9 * (size_t)c;
-    Rows3 R;
-    R.r0 = mk3(1.f, 0.f, 0.f);
-    R.r1 = mk3(0.f, 1.f, 0.f);
-    R.r2 = mk3(0.f, 0.f, 1.f);
-    bool ok = pinned[c] == 0 && active[c] != 0;
-    f3 axis[3];
-    bool have[3] = {false, false, false};
-    if (ok) {
-        const f3 cPos = xyz(pos[c]);
-        for (int k = 0; k < 3; k++) {
-            int np = nbr[6 * c + 2 * k + 1], nm = nbr[6 * c + 2 * k];
-            bool okp = np >= 0 && active[np] != 0, okm = nm >= 0 && active[nm] != 0;
-            f3 e;
-            if (okp && okm) e = xyz(pos[np]) - xyz(pos[nm]);
-            else if (okp) e = xyz(pos[np]) - cPos;
-            else if (okm) e = cPos - xyz(pos[nm]);
-            else continue;
-            float len = len3(e);
-            if (len < 1e-12f) continue;
-            f3 dir = div3(e, len);
-            f3 psum = mk3(0.f, 0.f, 0.f);
-            int pc = 0;
-            for (int q = 0; q < 6; q++) {
-                int n = nbr[6 * c + q];
-                if (n >= 0 && active[n] != 0) {
-                    const float* P = prev + 9 * (size_t)n;
-                    psum = psum + mk3(P[k], P[3 + k], P[6 + k]);
-                    pc++;
-                }
-            }
-            if (pc > 0) {
-                float pl = len3(psum);
-                if (pl > 1e-6f) {
-                    f3 blended = dir + div3(psum, pl);
-                    float dl = len3(blended);
-                    if (dl > 0.25f) dir = div3(blended, dl);
-                }
-            }
-            axis[k] = dir;
-            have[k] = true;
-        }
-        int miss = -1, mc = 0;
-        for (int k = 0; k < 3; k++)
-            if (!have[k]) { miss = k; mc++; }
-        if (mc == 1) {
-            f3 cx = cross3(axis[(miss + 1) % 3], axis[(miss + 2) % 3]);
-            float l = len3(cx);
-            if (l > 1e-9f) { axis[miss] = div3(cx, l); have[miss] = true; }
-        }
-        ok = have[0] && have[1] && have[2];
-    }
-    if (ok) {
-        Rows3 M;
-        M.r0 = mk3(axis[0].x, axis[1].x, axis[2].x);
-        M.r1 = mk3(axis[0].y, axis[1].y, axis[2].y);
-        M.r2 = mk3(axis[0].z, axis[1].z, axis[2].z);
-        ok = !(det3(M) <= 1e-4f);  // the reference's rejection test, so that a NaN frame is kept as it does
-        if (ok) {
-            Rows3 H = M;
-            for (int it = 0; it < 16; it++) {
-                Rows3 t = inverse_transpose3(H);
-                H.r0 = (H.r0 + t.r0) * 0.5f;
-                H.r1 = (H.r1 + t.r1) * 0.5f;
-                H.r2 = (H.r2 + t.r2) * 0.5f;
-            }
-            f3 c0 = mk3(H.r0.x, H.r1.x, H.r2.x), c1 = mk3(H.r0.y, H.r1.y, H.r2.y), c2 = mk3(H.r0.z, H.r1.z, H.r2.z);
-            float ortho = fmaxf(fmaxf(fabsf(dot3(c0, c0) - 1.f), fabsf(dot3(c1, c1) - 1.f)), fabsf(dot3(c2, c2) - 1.f));
-            ortho = fmaxf(ortho, fmaxf(fabsf(dot3(c0, c1)), fmaxf(fabsf(dot3(c0, c2)), fabsf(dot3(c1, c2)))));
-            if (!(ortho > 1e-3f)) R = H;
-        }
-    }
-    out[0] = R.r0.x; out[1] = R.r0.y; out[2] = R.r0.z;
-    out[3] = R.r1.x; out[4] = R.r1.y; out[5] = R.r1.z;
-    out[6] = R.r2.x; out[7] = R.r2.y; out[8] = R.r2.z;
+    if (c < N) particle_frame(c, 0, pos, nbr, active, pinned, prev, rot);
 }
 
 // ------------------------------------------------------------------------------------------------ legacy grasp kernels

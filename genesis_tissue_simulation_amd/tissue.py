@@ -104,8 +104,9 @@ class Tissue:
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
                  device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None, colour_block=0):
-        """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, edge_desc, edge_lambda,
-        tet_desc, tet_k_dev, tet_k_hyd, tet_lambda) -- views into the pooled buffers of a batch. `colour_block` is
+        """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, mass, active, pinned, nbr,
+        edge_desc, edge_lambda, tet_desc, tet_k_dev, tet_k_hyd, tet_lambda, rot, rot_prev) -- views into the pooled
+        buffers of a batch. `colour_block` is
         the workgroup size of the edge / tet colour kernels (0: the library default)."""
         self.lib = load()
         if not torch.cuda.is_available():
@@ -155,10 +156,10 @@ class Tissue:
         self.vel = take("vel", lambda: torch.zeros((N, 4), dtype=torch.float32, device=dev))
         self.force = torch.zeros((N, 4), dtype=torch.float32, device=dev)
         self.mass = take("mass", lambda: f32(grid.mass))
-        self.active = i32(grid.active)
-        self.pinned = i32(grid.pinned)
+        self.active = take("active", lambda: i32(grid.active))
+        self.pinned = take("pinned", lambda: i32(grid.pinned))
         self.inside = i32(grid.inside)
-        self.nbr = i32(grid.neighbors.reshape(-1))
+        self.nbr = take("nbr", lambda: i32(grid.neighbors.reshape(-1)))
         self.bend_alive = torch.zeros(N, dtype=torch.int32, device=dev)
         self.edge_desc = take("edge_desc", lambda: i32(topo["edge_desc"]))
         self.edge_lambda = take("edge_lambda", lambda: torch.zeros(max(E, 1), dtype=torch.float32, device=dev))
@@ -174,8 +175,8 @@ class Tissue:
         self.edge_cut_t = torch.full((6 * N,), -1.0, dtype=torch.float32, device=dev)
         self.edge_skin_t = f32(grid.edge_skin_fraction())
         self.rest = f32(np.asarray(grid.positions, dtype=np.float32).reshape(-1))
-        self.rot = torch.zeros(9 * N, dtype=torch.float32, device=dev)
-        self.rot_prev = torch.zeros(9 * N, dtype=torch.float32, device=dev)
+        self.rot = take("rot", lambda: torch.zeros(9 * N, dtype=torch.float32, device=dev))
+        self.rot_prev = take("rot_prev", lambda: torch.zeros(9 * N, dtype=torch.float32, device=dev))
         self.cut_point_capacity = 4 * max(Eg, 1)
         self.cut_points = torch.zeros(8 * self.cut_point_capacity, dtype=torch.float32, device=dev)
         self.counters = torch.zeros(8, dtype=torch.int32, device=dev)

@@ -294,6 +294,15 @@ typedef struct GtsBatchView {
     float* lambda_edge;           /* [sum E] */
     float* lambda_tet;            /* [2 sum T], dev then hyd per instance slice */
     float* alpha;                 /* [2 sum T] work: alpha~ of the step, dev then hyd per instance slice */
+    /* particle frames: when rot is set, every step ends with the frame update of every instance
+     * (physics_compute_rotations, as LCS_Step does) */
+    long long total_nodes;        /* sum N */
+    const int32_t* nbr;           /* [6 sum N] each instance's link table, node ids relative to its node_off */
+    const int32_t* active;        /* [sum N] */
+    const int32_t* pinned;        /* [sum N] */
+    const int32_t* node_base;     /* [sum N] node_off of the instance each node belongs to */
+    float* rot;                   /* [9 sum N] frames, rows of R */
+    float* rot_prev;              /* [9 sum N] */
 } GtsBatchView;
 
 /* Host: LPT packing of instances (by cost) onto workgroups. out: offsets[n_workgroups+1] then items[n]. */

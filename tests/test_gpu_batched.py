@@ -1,5 +1,6 @@
 """The batched persistent launch computes, for every instance, exactly what the CPU oracle computes for that instance
-alone: LDS-staged and global-memory instances, uneven sizes, per-instance compliance, a cut between steps."""
+alone: LDS-staged and global-memory instances, uneven sizes, per-instance compliance, a cut between steps, the
+particle frames every step ends with and the cut points recorded in them."""
 import numpy as np
 import pytest
 
@@ -63,5 +64,9 @@ def test_batched_matches_oracle(lds_nodes, iterations, substeps_per_launch):
         assert err <= 1e-5, (k, err)
         np.testing.assert_array_equal(pb, po)
         np.testing.assert_array_equal(vb, vo)
+        # the frame update that ends every step, and the cut points recorded in those frames
+        np.testing.assert_array_equal(batch.tissues[k].rotations(), o.rotations())
     assert oracles[2].cut_state()["revision"] > 0
     np.testing.assert_array_equal(batch.tissues[2].neighbors(), oracles[2].cut_state()["neighbors"])
+    np.testing.assert_array_equal(batch.tissues[2].cut_state()["cut_points"], oracles[2].cut_state()["cut_points"])
+    assert np.abs(oracles[2].rotations() - np.eye(3)).max() > 0
