@@ -42,7 +42,7 @@ class GtsTissueView(ct.Structure):
                 ("active", P), ("pinned", P), ("inside", P), ("nbr", P), ("bend_alive", P), ("edge_desc", P),
                 ("edge_lambda", P), ("slot_support", P), ("tet_desc", P), ("tet_k_dev", P), ("tet_k_hyd", P),
                 ("tet_alpha", P), ("tet_lambda", P), ("grid_edges", P), ("occupied", P), ("cut_mask", P),
-                ("edge_cut_t", P), ("edge_skin_t", P), ("rest", P), ("rot", P), ("rot_prev", P), ("cut_points", P), ("cut_point_capacity", I),
+                ("edge_cut_t", P), ("edge_skin_t", P), ("rest", P), ("rot", P), ("rot_prev", P), ("prev_corner", P), ("cut_points", P), ("cut_point_capacity", I),
                 ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
                 ("use_graph", I), ("colour_block", I)]
 
@@ -121,6 +121,8 @@ SIGNATURES = {
     "gts_detect_cut_ccd": ((P, P, P, P, F, P, P), I),
     "gts_tear_overstretch": ((P, F, F, P), I),
     "gts_compute_rotations": ((P, P), I),
+    "gts_step_cutting": ((P, P, P, P, I, I, P, P, F, P), I),
+    "gts_snapshot_corners": ((P, P), I),
     "gts_grasp_set_tool": ((P, P, P, P, P, P), I),
     "gts_step_legacy": ((P, P, P, P, P, P, I, P), I),
     "gts_compact_topology": ((P, P, P, P, P, P, P), I),

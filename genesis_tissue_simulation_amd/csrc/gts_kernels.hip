@@ -334,12 +334,33 @@ __global__ void k_copy_floats(int n, const float* __restrict__ src, float* __res
     if (i < n) dst[i] = src[i];
 }
 
+// With a gate, the frame update runs only when the cut point counter moved during the tick (counters[1] against the
+// value saved in counters[5] before it).
+__device__ __forceinline__ bool frames_gated_off(const uint32_t* gate) { return gate && gate[1] == gate[5]; }
+
+__global__ void k_copy_frames(int n, const float* __restrict__ src, float* __restrict__ dst, const uint32_t* gate) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !frames_gated_off(gate)) dst[i] = src[i];
+}
+
 // k_ComputeParticleRot (physics.cu:109-205) over one tissue's nodes.
 __global__ void k_particle_frames(int N, const float4* __restrict__ pos, const int* __restrict__ nbr,
                                   const int* __restrict__ active, const int* __restrict__ pinned,
-                                  const float* __restrict__ prev, float* __restrict__ rot) {
+                                  const float* __restrict__ prev, float* __restrict__ rot, const uint32_t* gate) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < N) particle_frame(c, 0, pos, nbr, active, pinned, prev, rot);
+    if (c < N && !frames_gated_off(gate)) particle_frame(c, 0, pos, nbr, active, pinned, prev, rot);
+}
+
+__global__ void k_save_cut_counter(uint32_t* counters) { counters[5] = counters[1]; }
+
+// The ribbon tick's corner snapshot: the current positions as float triples.
+__global__ void k_snapshot(int N, const float4* __restrict__ pos, float* __restrict__ snap) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float4 p = pos[i];
+    snap[3 * i] = p.x;
+    snap[3 * i + 1] = p.y;
+    snap[3 * i + 2] = p.z;
 }
 
 // ------------------------------------------------------------------------------------------------ legacy grasp kernels
@@ -1034,11 +1055,11 @@ int scan_blocks(int n) { return grid_for(n); }
 int* partial_area(const GtsTissueView* v) { return reinterpret_cast<int*>(v->scratch + 8 * (size_t)v->n_grid_edges); }
 
 // physics_compute_rotations (physics.cu:399-411): the current frames become the previous ones, then new frames.
-void launch_frames(const GtsTissueView* v, hipStream_t s) {
+void launch_frames(const GtsTissueView* v, hipStream_t s, const uint32_t* gate = nullptr) {
     const int N = v->n_nodes;
-    k_copy_floats<<<grid_for(9 * N), GTS_BLOCK, 0, s>>>(9 * N, v->rot, v->rot_prev);
+    k_copy_frames<<<grid_for(9 * N), GTS_BLOCK, 0, s>>>(9 * N, v->rot, v->rot_prev, gate);
     k_particle_frames<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, (const float4*)v->pos, v->nbr, v->active, v->pinned,
-                                                        v->rot_prev, v->rot);
+                                                        v->rot_prev, v->rot, gate);
 }
 
 int launch_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
@@ -1111,7 +1132,8 @@ void launch_edge_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s,
     else launch_edge_colour_d<false>(v, L, s, off, cnt, it, iters, fallback, at, fb, omega, bgh);
 }
 
-int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut, int useNormals, float toolD) {
+// The ordered emission of the cut points of the edges the last detection kernel recorded (records at v->scratch).
+void emit_records(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut, int useNormals, float toolD) {
     int Eg = v->n_grid_edges;
     int nb = scan_blocks(Eg);
     int* partial = partial_area(v);
@@ -1123,8 +1145,18 @@ int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut,
                                            (const float4*)v->pos, v->rest, v->rot, records, normals, partial,
                                            v->counters, v->cut_points, v->cut_point_capacity);
     k_bump_counter<<<1, 1, 0, s>>>(partial, nb, v->counters);
+}
+
+// k_SeverLinks, then the liveness refresh.
+int sever_and_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
+    int Eg = v->n_grid_edges;
     k_sever<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, v->grid_edges, v->cut_mask, v->nbr, v->bend_alive, v->counters);
     return launch_refresh(v, L, s);
+}
+
+int finish_cut(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut, int useNormals, float toolD) {
+    emit_records(v, L, s, ncut, useNormals, toolD);
+    return sever_and_refresh(v, L, s);
 }
 
 }  // namespace
@@ -1176,7 +1208,37 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     bool legacy, grasping, gate;
     int K;
     GraspArgs grasp;
+    // the CCD ribbon tick after every substep while a cut tool is set: rod S -> E, fallback normal, gap, tear limit
+    bool tick;
+    f3 tickS, tickE, tickN;
+    float tickD, tearLen2;
 };
+
+// cut_ribbon_tick without the pressure blade (cut.cu:1303-1431) after substep n: CCD of the edges moving from the
+// snapshot to the current positions, the tear law, one sever and liveness refresh, the snapshot rolled; a tick that
+// emitted cut points, but the last of the step, updates the frames (physics.cu:366-380).
+void launch_tick(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hipStream_t s, int n) {
+    const int Eg = v->n_grid_edges, N = v->n_nodes;
+    if (Eg <= 0) return;
+    float4* records = reinterpret_cast<float4*>(v->scratch);
+    float4* normals = records + Eg;
+    k_save_cut_counter<<<1, 1, 0, s>>>(v->counters);
+    k_cut_ccd<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, P.tickS, P.tickE, P.tickN, v->voxel_length, v->grid_edges,
+                                                 v->nbr, v->prev_corner, (const float4*)v->pos, v->occupied,
+                                                 v->cut_mask, v->edge_cut_t, v->edge_skin_t, v->inside, records,
+                                                 normals);
+    emit_records(v, L, s, P.tickN, 1, P.tickD);
+    if (P.tearLen2 > 0.f) {
+        k_tear<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, P.tearLen2, v->voxel_length, v->grid_edges, v->nbr, v->active,
+                                                  (const float4*)v->pos, v->rest, v->occupied, v->cut_mask,
+                                                  v->edge_cut_t, v->edge_skin_t, v->inside, records, normals,
+                                                  v->counters);
+        emit_records(v, L, s, P.tickN, 1, P.tickD);
+    }
+    sever_and_refresh(v, L, s);
+    k_snapshot<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, (const float4*)v->pos, v->prev_corner);
+    if (n + 1 < P.Nsub) launch_frames(v, s, v->counters);
+}
 
 // The constants gts_step and gts_step_legacy share (xpbd_begin_frame, xpbd_solver.cu:2481-2514).
 int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
@@ -1266,6 +1328,7 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
             }
             k_velscale<<<g, b, 0, s>>>(N, P.velScale, (const float4*)v->pos, (float4*)v->vel);
         }
+        if (P.tick) launch_tick(v, L, P, s, n);
     }
     launch_frames(v, s);  // LCS_Step ends with physics_compute_rotations (plugin_api.cpp:133)
 }
@@ -1360,8 +1423,12 @@ int launch_step_graph(const GtsTissueView* v, const Lattice& L, const StepPlan& 
 
 }  // namespace
 
-extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from,
-                        const GtsToolState* to, int tool_armed, int blade_armed, void* stream) {
+namespace {
+
+// gts_step, with the ribbon tick of a set cut tool when `cut` is given.
+int step_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from, const GtsToolState* to,
+              int tool_armed, int blade_armed, const GtsCutTool* cut, const float* emit_ncut, float tear_tau,
+              void* stream) {
     if (!v || !p || !from || !to) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
     if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 64) return GTS_ERR_BAD_ARG;
@@ -1370,6 +1437,16 @@ extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const Gt
     StepPlan P;
     { int rc_ = make_plan(v, p, P); if (rc_) return rc_; }
     const float h = P.h;
+    if (cut) {
+        if (!emit_ncut || !v->prev_corner) return GTS_ERR_BAD_ARG;
+        P.tick = true;
+        P.tickS = ld3(cut->t2v2);  // the current rod: S = t2v2, E = t1v2 (cut.cu:1382-1383)
+        P.tickE = ld3(cut->t1v2);
+        P.tickN = ld3(emit_ncut);
+        P.tickD = cut->d;
+        const float tl = tear_tau * v->voxel_length;
+        P.tearLen2 = tear_tau >= 1.f ? tl * tl : 0.f;  // tau < 1 or NaN: the tear law is off (cut.cu:1199-1202)
+    }
     P.contact = p->contact_enabled && (tool_armed || blade_armed);
     P.armed = (tool_armed ? 7u : 0u) | (blade_armed ? 8u : 0u);
     P.alphaC = p->contact_compliance / (h * h);
@@ -1396,6 +1473,26 @@ extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const Gt
     }
     if (v->use_graph && !g_profile.on) return launch_step_graph(v, L, P, s);
     enqueue_step(v, L, P, s);
+    return -(int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from,
+                        const GtsToolState* to, int tool_armed, int blade_armed, void* stream) {
+    return step_impl(v, p, from, to, tool_armed, blade_armed, nullptr, nullptr, 0.f, stream);
+}
+
+extern "C" int gts_step_cutting(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from,
+                                const GtsToolState* to, int tool_armed, int blade_armed, const GtsCutTool* tool,
+                                const float* emit_ncut, float tear_tau, void* stream) {
+    if (!tool) return GTS_ERR_BAD_ARG;
+    return step_impl(v, p, from, to, tool_armed, blade_armed, tool, emit_ncut, tear_tau, stream);
+}
+
+extern "C" int gts_snapshot_corners(const GtsTissueView* v, void* stream) {
+    if (!v || !v->prev_corner || v->n_nodes <= 0) return GTS_ERR_BAD_ARG;
+    k_snapshot<<<grid_for(v->n_nodes), GTS_BLOCK, 0, S(stream)>>>(v->n_nodes, (const float4*)v->pos, v->prev_corner);
     return -(int)hipGetLastError();
 }
 

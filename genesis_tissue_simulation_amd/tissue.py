@@ -177,6 +177,7 @@ class Tissue:
         self.rest = f32(np.asarray(grid.positions, dtype=np.float32).reshape(-1))
         self.rot = take("rot", lambda: torch.zeros(9 * N, dtype=torch.float32, device=dev))
         self.rot_prev = take("rot_prev", lambda: torch.zeros(9 * N, dtype=torch.float32, device=dev))
+        self.prev_corner = torch.zeros(3 * N, dtype=torch.float32, device=dev)
         self.cut_point_capacity = 4 * max(Eg, 1)
         self.cut_points = torch.zeros(8 * self.cut_point_capacity, dtype=torch.float32, device=dev)
         self.counters = torch.zeros(8, dtype=torch.int32, device=dev)
@@ -198,10 +199,13 @@ class Tissue:
                                      self.slot_support, self.tet_desc, self.tet_k_dev, self.tet_k_hyd,
                                      self.tet_alpha, self.tet_lambda, self.grid_edges, self.occupied, self.cut_mask,
                                      self.edge_cut_t, self.edge_skin_t, self.rest, self.rot, self.rot_prev,
-                                     self.cut_points)],
+                                     self.prev_corner, self.cut_points)],
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
             self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
         self._tool_from = None
+        self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep
+        self.emit_ncut = np.array([0.0, 1.0, 0.0], dtype=np.float32)
+        self.tear_tau = 0.0
         self.contact_model = 1
         self.grasp_event = 0
         check("gts_init_state", self.lib.gts_init_state(ct.byref(self.view), current_stream()))
@@ -264,10 +268,30 @@ class Tissue:
             check("gts_step_legacy", self.lib.gts_step_legacy(
                 ct.byref(self.view), ct.byref(p), ct.byref(self.grasp_params), ct.byref(self.grasp), ct.byref(frm),
                 ct.byref(to), int(tool_armed), current_stream()))
+        elif self.cut_tool is not None:
+            check("gts_step_cutting", self.lib.gts_step_cutting(
+                ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to), int(tool_armed), int(blade_armed),
+                ct.byref(self.cut_tool), self.emit_ncut.ctypes.data_as(ct.c_void_p), float(self.tear_tau),
+                current_stream()))
         else:
             check("gts_step", self.lib.gts_step(ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to),
                                                 int(tool_armed), int(blade_armed), current_stream()))
         self._tool_from = to
+
+    def set_cut_tool(self, cut_tool, tear_tau=0.0):
+        """LCS_SetTool (cut_set_tool, cut.cu:1208-1226): from the next step on, every substep ends with the CCD ribbon
+        tick against this tool's current rod (and the tear law when tear_tau >= 1); the corner snapshot is taken when a
+        tool is first set. None disarms it."""
+        if cut_tool is None:
+            self.cut_tool = None
+            return
+        t = cut_tool_struct(cut_tool)
+        if t.valid:
+            self.emit_ncut = np.ascontiguousarray(np.asarray(cut_tool["ncut"], dtype=np.float32))
+        if self.cut_tool is None:
+            check("gts_snapshot_corners", self.lib.gts_snapshot_corners(ct.byref(self.view), current_stream()))
+        self.cut_tool = t
+        self.tear_tau = float(tear_tau)
 
     def detect_cut(self, cut_tool):
         t = cut_tool_struct(cut_tool)

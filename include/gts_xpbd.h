@@ -114,10 +114,11 @@ typedef struct GtsTissueView {
     float* rot;                  /* [9N] particle frames, rows of R per node (RotGpu, cut.cuh:67): the material frame of
                                   * the cut points' gap and normal */
     float* rot_prev;             /* [9N] the frames before the last update (the update's temporal blend) */
+    float* prev_corner;          /* [3N] the CCD ribbon tick's corner snapshot (d_prevCorner, cut.cu:1303-1431) */
     float* cut_points;           /* [8*capacity] CutPointGpu records (cut.cuh:37-46) */
     int cut_point_capacity;
     uint32_t* counters;          /* [8] 0 revision, 1 cut point counter, 2 refreshed revision, 3 stale flag,
-                                  * 4 edges marked by the tear law */
+                                  * 4 edges marked by the tear law, 5 the cut point counter before a ribbon tick */
     float* scratch;              /* [4*Eg + 2*nblocks + 64] detection records and scan partials */
     float* reaction;             /* [4] force of the tissue on the tools, contacts */
     float* step_consts;          /* [step_const_capacity * GTS_STEP_CONST_FLOATS] per-substep tool shapes */
@@ -230,6 +231,17 @@ int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, const GtsGraspP
  * follows the pose from `from` to `to` over the substeps. The contact fields of *p are not used, nor the reaction. */
 int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp, const GtsGraspState* g,
                     const GtsToolState* from, const GtsToolState* to, int tool_armed, void* stream);
+/* Device: LCS_Step while LCS_SetTool's cut tool is set (physics_step, physics.cu:331-394): gts_step with the CCD
+ * ribbon tick after every substep (cut_ribbon_tick without the pressure blade, cut.cu:1303-1431): k_DetectCutCCD of the
+ * edges moving from v->prev_corner to the current positions against the tool's current rod S = t2v2, E = t1v2 (emit_ncut,
+ * the last valid tool normal, where the hit has no normal of its own), then the tear law when tear_tau >= 1, one
+ * k_SeverLinks and the snapshot rolled to the current positions; a tick that emitted cut points, but the last, is
+ * followed by a frame update (physics.cu:366-380). Seed v->prev_corner with gts_snapshot_corners when the tool is set. */
+int gts_step_cutting(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from, const GtsToolState* to,
+                     int tool_armed, int blade_armed, const GtsCutTool* tool, const float* emit_ncut, float tear_tau,
+                     void* stream);
+/* Device: the corner snapshot cut_set_tool takes when a cut tool is first set (cut.cu:1213-1220). */
+int gts_snapshot_corners(const GtsTissueView* v, void* stream);
 /* Device: physics_compute_rotations (physics.cu:399-411) as LCS_Finalize runs it after a cut (plugin_api.cpp:159-166):
  * the current frames become the previous ones and k_ComputeParticleRot (physics.cu:109-205) derives new ones from the
  * current positions and links. gts_step and gts_step_legacy end with the same update. */

@@ -85,6 +85,7 @@ def lib():
             getattr(L, f"oracle_refresh_{suf}").argtypes = [P]
             getattr(L, f"oracle_tear_{suf}").argtypes = [P, F, F]
             getattr(L, f"oracle_compute_rotations_{suf}").argtypes = [P]
+            getattr(L, f"oracle_set_cut_tool_{suf}").argtypes = [P, P, F]
             getattr(L, f"oracle_get_rotations_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_set_contact_model_{suf}").argtypes = [P, I, ct.POINTER(OracleGrasp)]
             getattr(L, f"oracle_get_grasp_{suf}").argtypes = [P] * 6
@@ -238,6 +239,15 @@ class Oracle:
         """Tear law (k_TearOverstretch + k_SeverLinks): links stretched past tau times their rest length part at
         their midpoint."""
         self._call("tear", self.h, float(tau), float(tool_d))
+
+    def set_cut_tool(self, tool, tear_tau=0.0):
+        """LCS_SetTool: arms the CCD ribbon tick after every substep of the following steps against the tool's current
+        rod, with the tear law when tear_tau >= 1; None disarms it."""
+        if tool is None:
+            self._call("set_cut_tool", self.h, None, 0.0)
+            return
+        self._cut_tool = cut_tool_struct(tool)
+        self._call("set_cut_tool", self.h, ct.byref(self._cut_tool), float(tear_tau))
 
     def compute_rotations(self):
         """physics_compute_rotations: the particle frames from the current positions (LCS_Finalize's update; step()

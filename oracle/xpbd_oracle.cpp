@@ -425,6 +425,12 @@ template <class R> struct Tissue {
     std::vector<CutPoint> cutPoints;
     uint32_t cutPointCounter = 0, revision = 0, refreshedRevision = 0xffffffffu;
     uint32_t tearCount = 0;  // edges the tear law marked (the reference's dbg[15])
+    // the cut tool of LCS_SetTool and the CCD ribbon tick it arms (cut.cu:1208-1226, :1303-1431)
+    bool cutToolSet = false, havePrevCorner = false;
+    OracleCutTool cutTool{};
+    V3<R> emitNcut = V3<R>{0, 1, 0};
+    float tearTau = 0.f;
+    std::vector<V3<R>> prevCorner;
     // legacy grasp (contact model 0, xpbd_solver.cu:250-275): the host latch and the per-node state
     int contactModel = 1;         // 0: legacy grasp and CCD gate; 1: friction contact
     OracleGrasp gp{};
@@ -1274,7 +1280,11 @@ template <class R> V3<R> mulRtv(const V3<R>* rows, V3<R> v) {
     return rows[0] * v.x + rows[1] * v.y + rows[2] * v.z;
 }
 
-// LCS_Step (plugin_api.cpp:125-134): physics_step (physics.cu:331-394, no ribbon tick), then the particle frames.
+template <class R> void ribbonTick(Tissue<R>& s);
+
+// LCS_Step (plugin_api.cpp:125-134): physics_step (physics.cu:331-394) with the CCD ribbon tick after every substep
+// while a cut tool is set, the frames recomputed after a tick that emitted cut points (but the last), then the
+// particle frames.
 template <class R> int step(Tissue<R>& s, R dt) {
     int rc = beginFrame(s, dt);
     if (rc) return rc;
@@ -1284,6 +1294,11 @@ template <class R> int step(Tissue<R>& s, R dt) {
     for (int n = 0; n < N; n++) {
         rc = substep(s);
         if (rc) return rc;
+        if (s.cutToolSet) {
+            uint32_t before = s.cutPointCounter;
+            ribbonTick(s);
+            if (s.cutPointCounter != before && n + 1 < N) computeRotations(s);
+        }
     }
     computeRotations(s);
     return 0;
@@ -1396,7 +1411,7 @@ template <class R> void severLinks(Tissue<R>& s) {  // k_SeverLinks (cut.cu:732-
 // The grasp-ring exemption (cut.cu:655-673) is left out: only the legacy grasp fills the ring, so on this path the
 // ring is empty, edgeWasExempt never becomes 1 and both exemption branches are unreachable. Each edge reads and sets
 // only its own mask bits, so the serial edge order decides nothing but the cut point slots.
-template <class R> void tearOverstretch(Tissue<R>& s, float tau, R toolD) {
+template <class R> void tearOverstretch(Tissue<R>& s, float tau, R toolD, bool sever = true) {
     if (!(tau >= 1.0f)) return;
     float voxelL = s.d.tool_voxel;
     float tearLen2 = (tau * voxelL) * (tau * voxelL);
@@ -1429,7 +1444,7 @@ template <class R> void tearOverstretch(Tissue<R>& s, float tau, R toolD) {
         s.tearCount++;
         markAndEmit(s, e, axis, bc, idA, idB, (R)0.5f, hit, ncut, toolD);
     }
-    severLinks(s);
+    if (sever) severLinks(s);
 }
 
 // k_DetectCutQuad (cut.cu:421-474) + k_SeverLinks: cut_detect (cut.cu:1235-1254).
@@ -1464,7 +1479,7 @@ template <class R> void detectCutQuad(Tissue<R>& s, const OracleCutTool& tl) {
 
 // k_DetectCutCCD (cut.cu:486-592) of the moving edges against a static blade segment S->E, then k_SeverLinks.
 template <class R> void detectCutCCD(Tissue<R>& s, V3<R> S, V3<R> E, V3<R> emitNcut, R toolD,
-                                     const std::vector<V3<R>>& prevPos) {
+                                     const std::vector<V3<R>>& prevPos, bool sever = true) {
     R pad = (R)s.d.tool_voxel;
     int Eg = s.d.grid_edge_count;
     for (int e = 0; e < Eg; e++) {
@@ -1525,7 +1540,34 @@ template <class R> void detectCutCCD(Tissue<R>& s, V3<R> S, V3<R> E, V3<R> emitN
             break;
         }
     }
+    if (sever) severLinks(s);
+}
+
+// cut_ribbon_tick without the pressure blade (cut.cu:1303-1431): the first tick after the tool was set only takes the
+// corner snapshot; every later one runs the CCD of the edges moving from the snapshot to the current positions against
+// the tool's current rod S = t2v2, E = t1v2, then the tear law, one sever, and rolls the snapshot.
+template <class R> void ribbonTick(Tissue<R>& s) {
+    if (!s.havePrevCorner) {
+        s.prevCorner = s.pos;
+        s.havePrevCorner = true;
+        return;
+    }
+    V3<R> S = v3<R>(s.cutTool.t2v2), E = v3<R>(s.cutTool.t1v2);
+    detectCutCCD(s, S, E, s.emitNcut, (R)s.cutTool.d, s.prevCorner, false);
+    tearOverstretch(s, s.tearTau, (R)s.cutTool.d, false);
     severLinks(s);
+    s.prevCorner = s.pos;
+}
+
+// cut_set_tool (cut.cu:1208-1226): the tool, the last valid cut normal, and the corner snapshot on activation.
+template <class R> void setCutTool(Tissue<R>& s, const OracleCutTool& t) {
+    s.cutTool = t;
+    if (t.valid != 0) s.emitNcut = v3<R>(t.ncut);
+    if (!s.cutToolSet) {
+        s.prevCorner = s.pos;
+        s.havePrevCorner = true;
+    }
+    s.cutToolSet = true;
 }
 
 // Stable compaction of the live topology: old -> new index, -1 for removed (build side, no reference counterpart).
@@ -1682,6 +1724,12 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         detectCutCCD(*s, v3<R>(S), v3<R>(E), v3<R>(ncut), (R)toolD, pp);                                              \
     }                                                                                                                 \
     extern "C" void oracle_tear_##SUF(void* h, float tau, float toolD) { tearOverstretch(*(Tissue<R>*)h, tau, (R)toolD); } \
+    extern "C" void oracle_set_cut_tool_##SUF(void* h, const OracleCutTool* t, float tearTau) {                      \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        if (!t) { s->cutToolSet = false; s->havePrevCorner = false; return; }                                          \
+        setCutTool(*s, *t);                                                                                           \
+        s->tearTau = tearTau;                                                                                         \
+    }                                                                                                                 \
     extern "C" void oracle_compute_rotations_##SUF(void* h) { computeRotations(*(Tissue<R>*)h); }                    \
     extern "C" void oracle_get_rotations_##SUF(void* h, R* out) {                                                     \
         Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
