@@ -1242,6 +1242,10 @@ void launch_tick(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hi
 
 // The constants gts_step and gts_step_legacy share (xpbd_begin_frame, xpbd_solver.cu:2481-2514).
 int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
+    if (v->n_nodes <= 0 || v->n_edge_colours < 0 || v->n_edge_colours > GTS_MAX_COLOURS || v->n_tet_colours < 0 ||
+        v->n_tet_colours > GTS_MAX_COLOURS || !v->edge_colour_offsets || !v->tet_colour_offsets || !v->rot ||
+        !v->rot_prev)
+        return GTS_ERR_BAD_ARG;
     std::memset(&P, 0, sizeof(P));
     P.Nsub = p->substeps > 0 ? p->substeps : 1;
     if (!v->step_consts || v->step_const_capacity < P.Nsub) return GTS_ERR_CAPACITY;
@@ -1379,10 +1383,19 @@ struct GraphEntry {
 std::vector<GraphEntry> g_graphs;
 hipStream_t g_capture = nullptr;
 
+// The key holds the view, the plan and the colour offsets the launches were sized from (the view only points at them,
+// and a host array freed and reallocated at the same address must not replay a graph built for other contents).
 std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
-    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan));
-    std::memcpy(k.data(), v, sizeof(GtsTissueView));
-    std::memcpy(k.data() + sizeof(GtsTissueView), &P, sizeof(StepPlan));
+    const size_t ne = (size_t)v->n_edge_colours + 1, nt = (size_t)v->n_tet_colours + 1;
+    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) + (ne + nt) * sizeof(int32_t));
+    unsigned char* d = k.data();
+    std::memcpy(d, v, sizeof(GtsTissueView));
+    d += sizeof(GtsTissueView);
+    std::memcpy(d, &P, sizeof(StepPlan));
+    d += sizeof(StepPlan);
+    std::memcpy(d, v->edge_colour_offsets, ne * sizeof(int32_t));
+    d += ne * sizeof(int32_t);
+    std::memcpy(d, v->tet_colour_offsets, nt * sizeof(int32_t));
     return k;
 }
 

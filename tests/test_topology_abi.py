@@ -157,3 +157,35 @@ def test_plan_packs_every_instance_once():
     loads = [cost[items[offsets[w]:offsets[w + 1]]].sum() for w in range(W)]
     # LPT is within 4/3 of the optimum; the optimum is at least the mean load and the largest item
     assert max(loads) <= 4 / 3 * max(cost.sum() / W, cost.max()) + 1e-9
+
+
+def test_entry_points_reject_bad_arguments_before_touching_the_device():
+    """Null descriptors and out-of-range parameters return the bad-argument status (-1004) from the host-side checks,
+    which run before any launch (no GPU is needed to reach them)."""
+    lib = _lib.load()
+    BAD = -1004
+    null = ct.c_void_p(0)
+    assert lib.gts_step(null, null, null, null, 0, 0, null) == BAD
+    assert lib.gts_step_cutting(null, null, null, null, 0, 0, null, null, ct.c_float(0.0), null) == BAD
+    assert lib.gts_step_legacy(null, null, null, null, null, null, 0, null) == BAD
+    assert lib.gts_grasp_set_tool(null, null, null, null, null, null) == BAD
+    assert lib.gts_detect_cut(null, null, null) == BAD
+    assert lib.gts_detect_cut_ccd(null, null, null, null, ct.c_float(0.0), null, null) == BAD
+    assert lib.gts_tear_overstretch(null, ct.c_float(2.0), ct.c_float(0.0), null) == BAD
+    assert lib.gts_compute_rotations(null, null) == BAD
+    assert lib.gts_snapshot_corners(null, null) == BAD
+    assert lib.gts_init_state(null, null) == BAD
+    assert lib.gts_batched_step(null, null, null) == BAD
+    assert lib.gts_adjoint_forward(null, null, 10, 5, null) == BAD
+    assert lib.gts_adjoint_backward(null, null, 10, 5, null) == BAD
+    assert lib.gts_batched_plan(null, 4, 0, null) == BAD
+    # a step with a view but an invalid tet mode
+    view = _lib.GtsTissueView()
+    params = _lib.GtsStepParams()
+    params.tet_mode = 7
+    tool = _lib.GtsToolState()
+    assert lib.gts_step(ct.byref(view), ct.byref(params), ct.byref(tool), ct.byref(tool), 0, 0, null) == BAD
+    # a view whose colour counts are out of range is refused before its offsets are read
+    params.tet_mode = 1
+    view.n_nodes, view.n_tet_colours = 8, 1000
+    assert lib.gts_step(ct.byref(view), ct.byref(params), ct.byref(tool), ct.byref(tool), 0, 0, null) == BAD
