@@ -229,9 +229,16 @@ int launch_range(const GtsBatchView* v, const BatchStep& P, bool damped, hipStre
 }
 
 // The step as persistent launches of substeps_per_launch substeps each (one launch when 0).
-__global__ void k_batch_copy(long long n, const float* __restrict__ src, float* __restrict__ dst) {
+// dst[0, n) = src[0, n); with both pointers 16-byte aligned, 16 bytes a thread and the n % 4 tail after them.
+__global__ void k_batch_copy(long long n, const float* __restrict__ src, float* __restrict__ dst, int vec) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
+    if (!vec) {
+        if (i < n) dst[i] = src[i];
+        return;
+    }
+    const long long n4 = n >> 2;
+    if (i < n4) reinterpret_cast<float4*>(dst)[i] = reinterpret_cast<const float4*>(src)[i];
+    else if (i < n4 + (n & 3)) dst[4 * n4 + (i - n4)] = src[4 * n4 + (i - n4)];
 }
 
 // k_ComputeParticleRot (physics.cu:109-205) over the pooled nodes of every instance.
@@ -253,7 +260,9 @@ int enqueue(const GtsBatchView* v, BatchStep P, bool damped, hipStream_t s) {
     }
     if (v->rot) {  // the frame update that ends LCS_Step (plugin_api.cpp:133), for every instance at once
         const long long n = v->total_nodes;
-        k_batch_copy<<<(unsigned)((9 * n + 255) / 256), 256, 0, s>>>(9 * n, v->rot, v->rot_prev);
+        const int vec = ((reinterpret_cast<uintptr_t>(v->rot) | reinterpret_cast<uintptr_t>(v->rot_prev)) & 15) == 0;
+        const long long threads = vec ? (9 * n >> 2) + 3 : 9 * n;
+        k_batch_copy<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(9 * n, v->rot, v->rot_prev, vec);
         k_batched_frames<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, (const float4*)v->pos, v->nbr, v->active,
                                                                      v->pinned, v->node_base, v->rot_prev, v->rot);
         HIP_TRY(hipGetLastError());

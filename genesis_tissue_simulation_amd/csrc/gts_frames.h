@@ -49,39 +49,54 @@ __device__ inline void particle_frame(int c, int base, const float4* __restrict_
     R.r0 = mk3(1.f, 0.f, 0.f);
     R.r1 = mk3(0.f, 1.f, 0.f);
     R.r2 = mk3(0.f, 0.f, 1.f);
-    bool ok = pinned[c] == 0 && active[c] != 0;
+    const int pin = pinned[c], act = active[c];
+    bool ok = pin == 0 && act != 0;
     f3 axis[3];
     bool have[3] = {false, false, false};
     if (ok) {
+        // Every gather of the node is issued up front and unconditionally (a missing link reads node c itself), so
+        // the loads are in flight together instead of one dependent round trip per neighbour; the masks then apply
+        // the reference's per-axis choices, and the previous axes of the live neighbours are summed in link order.
+        int safe[6];
+        bool live[6];
+        float4 np4[6];
+        for (int q = 0; q < 6; q++) {
+            const int n = nbr[6 * c + q];
+            safe[q] = n >= 0 ? base + n : c;
+            live[q] = n >= 0;
+        }
+        int act_q[6];
+        for (int q = 0; q < 6; q++) act_q[q] = active[safe[q]];
+        for (int q = 0; q < 6; q++) np4[q] = pos[safe[q]];
+        float r[6][9];
+        for (int q = 0; q < 6; q++)
+            for (int j = 0; j < 9; j++) r[q][j] = prev[9 * (size_t)safe[q] + j];
+        for (int q = 0; q < 6; q++) live[q] = live[q] & (act_q[q] != 0);
+        f3 psum[3] = {mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+        int pc = 0;
+        for (int q = 0; q < 6; q++) {
+            for (int k = 0; k < 3; k++) {
+                const f3 s = psum[k] + mk3(r[q][k], r[q][3 + k], r[q][6 + k]);
+                psum[k] = live[q] ? s : psum[k];
+            }
+            pc += live[q] ? 1 : 0;
+        }
         const f3 cPos = frame_xyz(pos[c]);
         for (int k = 0; k < 3; k++) {
-            int np = nbr[6 * c + 2 * k + 1], nm = nbr[6 * c + 2 * k];
-            np = np >= 0 ? base + np : np;
-            nm = nm >= 0 ? base + nm : nm;
-            bool okp = np >= 0 && active[np] != 0, okm = nm >= 0 && active[nm] != 0;
+            const bool okp = live[2 * k + 1], okm = live[2 * k];
+            const f3 xp = frame_xyz(np4[2 * k + 1]), xm = frame_xyz(np4[2 * k]);
             f3 e;
-            if (okp && okm) e = frame_xyz(pos[np]) - frame_xyz(pos[nm]);
-            else if (okp) e = frame_xyz(pos[np]) - cPos;
-            else if (okm) e = cPos - frame_xyz(pos[nm]);
+            if (okp && okm) e = xp - xm;
+            else if (okp) e = xp - cPos;
+            else if (okm) e = cPos - xm;
             else continue;
             float len = len3(e);
             if (len < 1e-12f) continue;
             f3 dir = div3(e, len);
-            f3 psum = mk3(0.f, 0.f, 0.f);
-            int pc = 0;
-            for (int q = 0; q < 6; q++) {
-                int n = nbr[6 * c + q];
-                n = n >= 0 ? base + n : n;
-                if (n >= 0 && active[n] != 0) {
-                    const float* P = prev + 9 * (size_t)n;
-                    psum = psum + mk3(P[k], P[3 + k], P[6 + k]);
-                    pc++;
-                }
-            }
             if (pc > 0) {
-                float pl = len3(psum);
+                float pl = len3(psum[k]);
                 if (pl > 1e-6f) {
-                    f3 blended = dir + div3(psum, pl);
+                    f3 blended = dir + div3(psum[k], pl);
                     float dl = len3(blended);
                     if (dl > 0.25f) dir = div3(blended, dl);
                 }
