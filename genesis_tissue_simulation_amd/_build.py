@@ -10,7 +10,7 @@ PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 LIB = PACKAGE / "libgts_xpbd.so"
-SOURCES = ("gts_topology.hip", "gts_kernels.hip", "gts_batched.hip", "gts_adjoint.hip")
+SOURCES = ("gts_topology.hip", "gts_kernels.hip", "gts_batched.hip", "gts_adjoint.hip", "gts_surface.hip")
 
 # -ffp-contract=off: no multiply-add fusion, so kernels round like the host code and the CPU checker;
 # correctly rounded f32 division and square root; packed FP32 instructions disabled.

@@ -102,8 +102,17 @@ class GtsGraspState(ct.Structure):
                 ("grasping", I)]
 
 
+class GtsSurfaceView(ct.Structure):
+    _fields_ = [("n_voxels", I), ("n_isect", I), ("n_surface_edges", I), ("tri_capacity", I), ("qef_iters", I),
+                ("face_capacity", I)] + [(name, P) for name in (
+                    "voxel_corners", "isect_offset", "isect_count", "isect", "surface_edges", "conn", "edge_of",
+                    "rest_fp", "isect_world", "fp", "cut_fp", "edge_points", "candidates", "candidate_ok", "tri",
+                    "tri_work", "face_table", "normal_fixed", "normals", "scan", "counts", "out_pos", "out_nrm",
+                    "out_aux")]
+
+
 STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool, GtsBatchInstance,
-           GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState)
+           GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState, GtsSurfaceView)
 
 # name -> (argtypes, restype); the exported C ABI
 SIGNATURES = {
@@ -135,6 +144,7 @@ SIGNATURES = {
     "gts_stream_copy": ((P, P, ct.c_longlong, P), I),
     "gts_adjoint_forward": ((P, P, I, I, P), I),
     "gts_adjoint_backward": ((P, P, I, I, P), I),
+    "gts_surface_build": ((P, P, I, I, P), I),
 }
 
 _lib = None

@@ -24,8 +24,7 @@
 #include "gts_contact.h"
 #include "gts_solver.h"
 #include "gts_frames.h"
-
-#define GTS_BLOCK 256
+#include "gts_lattice.h"
 
 #define HIP_TRY(call)                                   \
     do {                                                \
@@ -589,28 +588,6 @@ struct CutToolD {
     int valid;
 };
 
-// EdgeStencil (cut.cuh:126-137): the four voxels around a lattice edge and the edge's local id in each.
-__device__ __forceinline__ void stencil(int axis, int slot, int& dx, int& dy, int& dz, int& local) {
-    const int8_t off[3][4][3] = {{{0, 0, 0}, {0, -1, 0}, {0, -1, -1}, {0, 0, -1}},
-                                 {{0, 0, 0}, {0, 0, -1}, {-1, 0, -1}, {-1, 0, 0}},
-                                 {{0, 0, 0}, {-1, 0, 0}, {-1, -1, 0}, {0, -1, 0}}};
-    const int8_t loc[3][4] = {{0, 2, 6, 4}, {3, 7, 5, 1}, {8, 9, 10, 11}};
-    dx = off[axis][slot][0]; dy = off[axis][slot][1]; dz = off[axis][slot][2];
-    local = loc[axis][slot];
-}
-
-__device__ __forceinline__ int directed_slot(int a, int b, int vx) {  // DirectedEdgeSlot (cut.cuh:25-28)
-    int dl = b - a, ad = dl < 0 ? -dl : dl;
-    int axis = ad == 1 ? 0 : (ad == vx + 1 ? 1 : 2);
-    return 6 * a + 2 * axis + (dl > 0 ? 1 : 0);
-}
-
-__device__ __forceinline__ bool cut_hits_material(float t, float skinT, int insideA, int insideB) {
-    if (insideA && insideB) return true;
-    if (!insideA && !insideB) return false;
-    if (!(skinT >= 0.f && skinT <= 1.f)) return false;
-    return insideA ? t <= skinT : t >= skinT;
-}
 
 __device__ __forceinline__ bool moller_trumbore(f3 O, f3 D, f3 V0, f3 V1, f3 V2, float& t, float& u, float& v) {
     f3 L1 = V1 - V0, L2 = V2 - V0, Tv = V0 - O;
@@ -788,24 +765,6 @@ __global__ void k_tear(int Eg, Lattice L, float tearLen2, float voxelL, const in
     mark_edge(e, axis, ge.y, ge.z, ge.w, idA, idB, 0.5f, hit, L, occupied, cut_mask, edge_cut_t, skin, inside, records);
 }
 
-// ---- ordered stable compaction: wave ballot + LDS prefix inside a block, exclusive scan of the block totals
-
-__device__ __forceinline__ int block_prefix(bool flag, int* lds, int& total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint64_t m = __ballot(flag);
-    int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) lds[wave] = __popcll(m);
-    __syncthreads();
-    int waveBase = 0;
-    total = 0;
-    for (int k = 0; k < nw; k++) {
-        int c = lds[k];
-        if (k < wave) waveBase += c;
-        total += c;
-    }
-    __syncthreads();
-    return waveBase + before;
-}
 
 struct FlagRecords {
     const float4* r;
@@ -821,52 +780,8 @@ struct FlagNodeUsed {
     __device__ bool operator()(int i) const { return used[i] != 0 && active[i] != 0; }
 };
 
-template <class Flag>
-__global__ void k_count_flags(int n, Flag flag, int* partial) {
-    __shared__ int lds[GTS_BLOCK / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int total;
-    block_prefix(i < n && flag(i), lds, total);
-    if (threadIdx.x == 0) partial[blockIdx.x] = total;
-}
 
-// Exclusive scan of the per-block totals in one workgroup; partial[nblocks] receives the grand total.
-__global__ void k_scan_partials(int nblocks, int* partial) {
-    __shared__ int lds[GTS_BLOCK / 64];
-    int carry = 0;
-    for (int start = 0; start < nblocks; start += blockDim.x) {
-        int i = start + threadIdx.x;
-        int v = i < nblocks ? partial[i] : 0;
-        // inclusive scan in the wave
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        int x = v;
-        for (int o = 1; o < 64; o <<= 1) {
-            int y = __shfl_up(x, o);
-            if (lane >= o) x += y;
-        }
-        if (lane == 63) lds[wave] = x;
-        __syncthreads();
-        int waveBase = 0, total = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-            if (k < wave) waveBase += lds[k];
-            total += lds[k];
-        }
-        if (i < nblocks) partial[i] = carry + waveBase + x - v;
-        carry += total;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[nblocks] = carry;
-}
 
-template <class Flag>
-__global__ void k_write_remap(int n, Flag flag, const int* partial, int* remap) {
-    __shared__ int lds[GTS_BLOCK / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool f = i < n && flag(i);
-    int total;
-    int r = block_prefix(f, lds, total);
-    if (i < n) remap[i] = f ? partial[blockIdx.x] + r : -1;
-}
 
 // EmitCutPoint (cut.cu:186-215) for the edges marked by the detection, in lattice edge order: slot = counter +
 // 2 * (rank of the edge among the emitting ones) for the owner A and + 1 for B.

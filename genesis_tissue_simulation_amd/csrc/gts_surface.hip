@@ -1,0 +1,710 @@
+// Surface reconstruction of the deformed, cut lattice on gfx950: the dual-contouring rebuild (recon_build_static,
+// dc_recon.cu:498-551), the cut feature points and cut walls (cut.cu:761-1100) and the internal-face cleanup
+// (surface_cleanup.cuh:1-53), as a chain of element-parallel kernels on one stream.
+//
+// Layout and order:
+//   - intersections, voxels, surface edges and nodes each get one thread; everything is gathered, nothing is
+//     scattered with float atomics;
+//   - the cut points of a (voxel, component) are summed by the voxel's thread in cut point index order, from a
+//     per-edge table of the (at most two) cut points of each cut edge: MarkAndEmit (cut.cu:211-246) emits an edge's
+//     points once, on its first cut;
+//   - candidate triangles have fixed slots (two per surface edge, then twelve per node) and are compacted in slot
+//     order by a wave-ballot scan, so the triangle list is deterministic; the face cleanup keeps each face's first
+//     listed representative of the majority orientation and compacts in index order;
+//   - normals are summed in fixed point with integer atomics, as the reference does, so their order does not matter.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "../../include/gts_xpbd.h"
+#include "gts_common.h"
+#include "gts_lattice.h"
+
+#define HIP_TRY(call)                          \
+    do {                                       \
+        hipError_t _e = (call);                \
+        if (_e != hipSuccess) return -(int)_e; \
+    } while (0)
+
+namespace {
+
+constexpr float kNormalScale = 1048576.0f;  // NORMAL_SCALE (common.cuh:45)
+constexpr float kPosScale = 16384.0f;       // POS_SCALE (cut.cuh:12)
+constexpr uint32_t kCutTag = 0x80000000u;   // a component feature point (dc_recon.cu:224)
+constexpr uint32_t kWallTag = 0x40000000u;  // emitted by the cut walls (cut.cu:1077-1081)
+constexpr uint32_t kIndexMask = 0x3FFFFFFFu;
+
+inline unsigned blocks_for(long long n) { return (unsigned)std::max(1ll, (n + GTS_BLOCK - 1) / GTS_BLOCK); }
+
+struct Grid {
+    int vx, vy, vz, nx, ny;
+    __device__ int corner(int x, int y, int z) const { return x + nx * (y + ny * z); }
+    __device__ int voxel(int x, int y, int z) const { return x + vx * (y + vy * z); }
+    __device__ void coord(int id, int& x, int& y, int& z) const {
+        x = id % nx;
+        y = (id / nx) % ny;
+        z = id / (nx * ny);
+    }
+    __device__ bool vox_in(int x, int y, int z) const {
+        return x >= 0 && y >= 0 && z >= 0 && x < vx && y < vy && z < vz;
+    }
+    __device__ int slot(int a, int b) const { return directed_slot(a, b, vx); }
+};
+
+__device__ __forceinline__ int ibits(float f) { return __float_as_int(f); }
+__device__ __forceinline__ f3 lerp3(f3 a, f3 b, float t) {
+    return mk3(a.x + (b.x - a.x) * t, a.y + (b.y - a.y) * t, a.z + (b.z - a.z) * t);
+}
+__device__ __forceinline__ f3 div3s(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 mul_rv(const float* R, f3 v) {  // mulRv (cut.cuh:70-73)
+    return mk3(dot3(ld3(R), v), dot3(ld3(R + 3), v), dot3(ld3(R + 6), v));
+}
+__device__ __forceinline__ f3 xyz4(const float4* p, int i) {
+    float4 q = p[i];
+    return mk3(q.x, q.y, q.z);
+}
+// LocalCornerOf (cut.cuh:101-109): figure-order corner of voxel (vx, vy, vz) at node (cx, cy, cz), -1 if none.
+__device__ __forceinline__ int local_corner_of(int cx, int cy, int cz, int vx, int vy, int vz) {
+    int dx = cx - vx, dy = cy - vy, dz = cz - vz;
+    if ((dx | dy | dz) & ~1) return -1;
+    return 4 * dz + (dy ? (dx ? 2 : 3) : dx);
+}
+// The i-th owner of an IsectGpu record (cut.cuh:29-31 SurfaceOwner), with the cut fraction of the edge when cut-aware.
+__device__ __forceinline__ int surface_owner(const float* rec, const Grid& G, const float* cutT) {
+    int a = ibits(rec[6]), b = ibits(rec[7]);
+    float ct = cutT ? cutT[G.slot(a, b)] : -1.f;
+    return skin_owner_a(rec[8], ct, ibits(rec[9])) ? a : b;
+}
+
+// ---- k_RebuildIsectWorld (dc_recon.cu:97-142): an intact edge lerps its deformed ends; a severed one rebuilds the
+// missing end from the skin owner's frame and the rest edge.
+__global__ void k_isect_world(int I, Grid G, const float* __restrict__ isect, const float4* __restrict__ pos,
+                              const float* __restrict__ rest, const int* __restrict__ nbr, const float* __restrict__ rot,
+                              const float* __restrict__ cutT, float* __restrict__ iw) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= I) return;
+    const float* rec = isect + 10 * (size_t)e;
+    const int A = ibits(rec[6]), B = ibits(rec[7]);
+    const float t = rec[8];
+    bool severed = false;
+    if (nbr) {
+        int ax, ay, az, bx, by, bz;
+        G.coord(A, ax, ay, az);
+        G.coord(B, bx, by, bz);
+        int dxi = bx - ax, dyi = by - ay, dzi = bz - az;
+        int axis = dxi != 0 ? 0 : (dyi != 0 ? 1 : 2);
+        int sgn = (dxi + dyi + dzi) > 0 ? 1 : -1;
+        severed = nbr[6 * A + 2 * axis + (sgn > 0 ? 1 : 0)] < 0;
+    }
+    f3 w;
+    if (!severed) {
+        w = lerp3(xyz4(pos, A), xyz4(pos, B), t);
+    } else {
+        f3 dir = ld3(rest + 3 * (size_t)B) - ld3(rest + 3 * (size_t)A);
+        if (surface_owner(rec, G, cutT) == A) {
+            f3 pA = xyz4(pos, A);
+            w = lerp3(pA, pA + mul_rv(rot + 9 * (size_t)A, dir), t);
+        } else {
+            f3 pB = xyz4(pos, B);
+            w = lerp3(pB - mul_rv(rot + 9 * (size_t)B, dir), pB, t);
+        }
+    }
+    iw[3 * (size_t)e] = w.x;
+    iw[3 * (size_t)e + 1] = w.y;
+    iw[3 * (size_t)e + 2] = w.z;
+}
+
+// ---- k_FeaturePoints (dc_recon.cu:149-207) and k_AdvectIntactSurface (dc_recon.cu:477-493): a voxel whose rest
+// feature point exists and that no cut touches carries it trilinearly with its corners; any other voxel with
+// intersections runs the decaying-step QEF ascent from their mean, stopping before it leaves the corners' box.
+__global__ void k_feature_points(int V, int iters, int cut_aware, int have_rest, const int* __restrict__ corners,
+                                 const int* __restrict__ ioff, const int* __restrict__ icnt,
+                                 const float* __restrict__ isect, const float* __restrict__ iw,
+                                 const float4* __restrict__ pos, const float* __restrict__ rest,
+                                 const float4* __restrict__ rest_fp, const uint32_t* __restrict__ mask,
+                                 float4* __restrict__ fp) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const int* cv = corners + 8 * (size_t)v;
+    if (have_rest) {
+        float4 r4 = rest_fp[v];
+        if (!(r4.w < .5f) && !(cut_aware && mask[v] != 0u)) {
+            f3 a = ld3(rest + 3 * (size_t)cv[0]), b = ld3(rest + 3 * (size_t)cv[6]);
+            f3 uv = mk3(fminf(1.f, fmaxf(0.f, (r4.x - a.x) / (b.x - a.x))),
+                        fminf(1.f, fmaxf(0.f, (r4.y - a.y) / (b.y - a.y))),
+                        fminf(1.f, fmaxf(0.f, (r4.z - a.z) / (b.z - a.z))));
+            f3 p = mk3(0.f, 0.f, 0.f);
+            for (int k = 0; k < 8; k++) {
+                const int x = k == 1 || k == 2 || k == 5 || k == 6, y = k == 2 || k == 3 || k == 6 || k == 7, z = k >= 4;
+                float w = (x ? uv.x : 1 - uv.x) * (y ? uv.y : 1 - uv.y) * (z ? uv.z : 1 - uv.z);
+                p = p + xyz4(pos, cv[k]) * w;
+            }
+            fp[v] = make_float4(p.x, p.y, p.z, 1.f);
+            return;
+        }
+    }
+    const int off = ioff[v], cnt = icnt[v];
+    if (cnt == 0) {
+        fp[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    f3 x = mk3(0.f, 0.f, 0.f);
+    for (int s = 0; s < cnt; s++) x = x + ld3(iw + 3 * (size_t)(off + s));
+    x = div3s(x, (float)cnt);
+    f3 lo = xyz4(pos, cv[0]), hi = lo;
+    for (int c = 1; c < 8; c++) {
+        f3 p = xyz4(pos, cv[c]);
+        lo = mk3(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z));
+        hi = mk3(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z));
+    }
+    for (int it = 0; it < iters; it++) {
+        f3 F = mk3(0.f, 0.f, 0.f);
+        for (int s = 0; s < cnt; s++) {
+            f3 n = ld3(isect + 10 * (size_t)(off + s) + 3);
+            F = F + n * dot3(n, ld3(iw + 3 * (size_t)(off + s)) - x);
+        }
+        float a = 0.1f * (1.0f - (float)it / (float)iters);
+        f3 xn = x + F * a;
+        if (xn.x <= lo.x || xn.y <= lo.y || xn.z <= lo.z || xn.x >= hi.x || xn.y >= hi.y || xn.z >= hi.z) break;
+        x = xn;
+    }
+    fp[v] = make_float4(x.x, x.y, x.z, 1.f);
+}
+
+__global__ void k_fill_ints(long long n, int* __restrict__ a, int value) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = value;
+}
+
+// The cut points of each cut edge: slot 2e for the owner at the edge's base node, 2e + 1 for the other end.
+__global__ void k_edge_points(int capacity, const uint32_t* __restrict__ counters, Grid G,
+                              const int* __restrict__ gedges, const float* __restrict__ points,
+                              int* __restrict__ edge_points) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (int)min(counters[1], (uint32_t)capacity);
+    if (i >= n) return;
+    const int owner = ibits(points[8 * (size_t)i + 3]), e = ibits(points[8 * (size_t)i + 4]);
+    const int4 ge = reinterpret_cast<const int4*>(gedges)[e];
+    edge_points[2 * e + (owner == G.corner(ge.y, ge.z, ge.w) ? 0 : 1)] = i;
+}
+
+// ---- k_AccumulateCutFP + k_ComputeComponentFP (cut.cu:786-985), one thread per voxel: each component of a cut,
+// occupied voxel gets one feature point from its intersections and its cut points (restored from the owners' frames,
+// summed in cut point index order), refined like the voxel feature point inside the component's box, clamped to it
+// and pulled onto the plane of its own cut points. Every other slot is cleared (k_ClearCutFP, cut.cu:761-772).
+__global__ void k_component_points(int V, Grid G, int iters, const int* __restrict__ corners,
+                                   const int* __restrict__ ioff, const int* __restrict__ icnt,
+                                   const float* __restrict__ isect, const float* __restrict__ iw,
+                                   const float4* __restrict__ pos, const float* __restrict__ rest,
+                                   const float* __restrict__ rot, const int* __restrict__ active,
+                                   const uint32_t* __restrict__ occupied, const uint32_t* __restrict__ mask,
+                                   const int* __restrict__ conn, const float* __restrict__ cutT,
+                                   const int* __restrict__ edge_of, const int* __restrict__ edge_points,
+                                   const float* __restrict__ points, float4* __restrict__ cut_fp) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    float4* out = cut_fp + 8 * (size_t)v;
+    const uint32_t m = mask[v] & 0xFFFu;
+    int compCount = 0;
+    if (m != 0u && occupied[v] != 0u) compCount = conn[9 * m];
+    for (int c = compCount; c < 8; c++) out[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (compCount == 0) return;
+    const int* row = conn + 9 * m + 1;
+    const int* cv = corners + 8 * (size_t)v;
+    const int vxc = v % G.vx, vyc = (v / G.vx) % G.vy, vzc = v / (G.vx * G.vy);
+
+    // The voxel's cut points, ascending: the twelve edges of the voxel, each with at most two points.
+    const int8_t ec[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                              {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    int list[24];
+    int n = 0;
+    for (int l = 0; l < 12; l++) {
+        int p0 = cv[ec[l][0]], p1 = cv[ec[l][1]];
+        int lo = min(p0, p1), d = max(p0, p1) - lo;
+        int axis = d == 1 ? 0 : (d == G.nx ? 1 : 2);
+        int e = edge_of[3 * lo + axis];
+        if (e < 0) continue;
+        for (int side = 0; side < 2; side++) {
+            int i = edge_points[2 * e + side];
+            if (i < 0) continue;
+            int k = n++;
+            while (k > 0 && list[k - 1] > i) {
+                list[k] = list[k - 1];
+                k--;
+            }
+            list[k] = i;
+        }
+    }
+
+    const f3 ext = ld3(rest + 3 * (size_t)cv[6]) - ld3(rest + 3 * (size_t)cv[0]);
+    const f3 half = mk3(fabsf(ext.x) * .5f, fabsf(ext.y) * .5f, fabsf(ext.z) * .5f);
+    const int off = ioff[v], cnt = icnt[v];
+    for (int c = 0; c < compCount; c++) {
+        f3 bc = mk3(0.f, 0.f, 0.f), origin = mk3(0.f, 0.f, 0.f);
+        int nCorn = 0;
+        for (int lc = 0; lc < 8; lc++) {
+            if (row[lc] != c) continue;
+            f3 p = xyz4(pos, cv[lc]);
+            if (nCorn == 0) origin = p;
+            bc = bc + p;
+            nCorn++;
+        }
+        if (nCorn > 0) bc = div3s(bc, (float)nCorn);
+
+        f3 accPos = mk3(0.f, 0.f, 0.f);
+        int ccnt = 0;
+        uint32_t nacc[3] = {0u, 0u, 0u};
+        for (int k = 0; k < n; k++) {
+            const float* cp = points + 8 * (size_t)list[k];
+            const int owner = ibits(cp[3]);
+            if (active[owner] == 0) continue;
+            int ox, oy, oz;
+            G.coord(owner, ox, oy, oz);
+            int lc = local_corner_of(ox, oy, oz, vxc, vyc, vzc);
+            if (lc < 0 || row[lc] != c) continue;
+            const float* R = rot + 9 * (size_t)owner;
+            f3 nrmW = mul_rv(R, ld3(cp + 5));
+            f3 local = (xyz4(pos, owner) - origin) + mul_rv(R, ld3(cp));
+            accPos = accPos + local;
+            ccnt++;
+            nacc[0] += (uint32_t)(int)rintf(nrmW.x * kPosScale);
+            nacc[1] += (uint32_t)(int)rintf(nrmW.y * kPosScale);
+            nacc[2] += (uint32_t)(int)rintf(nrmW.z * kPosScale);
+        }
+
+        f3 sum = mk3(0.f, 0.f, 0.f);
+        int nIncl = 0;
+        for (int s = 0; s < cnt; s++) {
+            const float* rec = isect + 10 * (size_t)(off + s);
+            int ic = surface_owner(rec, G, cutT), ix, iy, iz;
+            G.coord(ic, ix, iy, iz);
+            int lc = local_corner_of(ix, iy, iz, vxc, vyc, vzc);
+            if (lc < 0 || row[lc] != c) continue;
+            sum = sum + ld3(iw + 3 * (size_t)(off + s));
+            nIncl++;
+        }
+        if (ccnt > 0) sum = sum + accPos + origin * (float)ccnt;
+        const int total = nIncl + ccnt;
+        if (total == 0) {
+            out[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
+        }
+        f3 x = div3s(sum, (float)total);
+        if (nIncl > 0) {
+            for (int it = 0; it < iters; it++) {
+                f3 F = mk3(0.f, 0.f, 0.f);
+                for (int s = 0; s < cnt; s++) {
+                    const float* rec = isect + 10 * (size_t)(off + s);
+                    int ic = surface_owner(rec, G, cutT), ix, iy, iz;
+                    G.coord(ic, ix, iy, iz);
+                    int lc = local_corner_of(ix, iy, iz, vxc, vyc, vzc);
+                    if (lc < 0 || row[lc] != c) continue;
+                    f3 nn = ld3(rec + 3);
+                    F = F + nn * dot3(nn, ld3(iw + 3 * (size_t)(off + s)) - x);
+                }
+                float a = 0.1f * (1.0f - (float)it / (float)iters);
+                f3 xn = x + F * a;
+                if (fabsf(xn.x - bc.x) >= half.x || fabsf(xn.y - bc.y) >= half.y || fabsf(xn.z - bc.z) >= half.z) break;
+                x = xn;
+            }
+        }
+        x.x = fminf(fmaxf(x.x, bc.x - half.x), bc.x + half.x);
+        x.y = fminf(fmaxf(x.y, bc.y - half.y), bc.y + half.y);
+        x.z = fminf(fmaxf(x.z, bc.z - half.z), bc.z + half.z);
+        if (ccnt > 0) {
+            f3 nAcc = div3s(mk3((float)(int)nacc[0], (float)(int)nacc[1], (float)(int)nacc[2]), kPosScale);
+            float nLenW = len3(nAcc);
+            if (nLenW > 1e-6f && nLenW > 0.25f * (float)ccnt) {
+                f3 nbar = div3s(nAcc, nLenW);
+                f3 cutC = div3s(accPos, (float)ccnt) + origin;
+                x = x - nbar * dot3(nbar, x - cutC);
+            }
+        }
+        out[c] = make_float4(x.x, x.y, x.z, 1.f);
+    }
+}
+
+// FetchPos (dc_recon.cu:228-234)
+__device__ __forceinline__ f3 fetch_pos(int raw, const float4* fp, const float4* cut_fp) {
+    uint32_t u = (uint32_t)raw;
+    int idx = (int)(u & kIndexMask);
+    float4 q = (u & kCutTag) ? cut_fp[idx] : fp[idx];
+    return mk3(q.x, q.y, q.z);
+}
+
+__device__ __forceinline__ void candidate(int slot, int a, int b, int c, const float4* fp, const float4* cut_fp,
+                                          int* cand, int* ok) {
+    f3 p0 = fetch_pos(a, fp, cut_fp), p1 = fetch_pos(b, fp, cut_fp), p2 = fetch_pos(c, fp, cut_fp);
+    f3 n = cross3(p1 - p0, p2 - p0);
+    bool keep = !(dot3(n, n) < 1e-16f);
+    ok[slot] = keep ? 1 : 0;
+    cand[3 * (size_t)slot] = a;
+    cand[3 * (size_t)slot + 1] = b;
+    cand[3 * (size_t)slot + 2] = c;
+}
+
+// ---- k_Stitch (dc_recon.cu:239-287) with StitchVertId (:213-225): the quad of the four voxels around a surface edge,
+// each voxel resolved to the feature point of the component holding the edge's skin owner when the voxel is cut.
+__global__ void k_stitch(int Se, Grid G, int cut_aware, const int* __restrict__ sedges,
+                         const float* __restrict__ skinT, const float* __restrict__ cutT,
+                         const uint32_t* __restrict__ mask, const uint32_t* __restrict__ occupied,
+                         const int* __restrict__ conn, const float4* __restrict__ fp,
+                         const float4* __restrict__ cut_fp, int* __restrict__ cand, int* __restrict__ ok) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Se) return;
+    const int* ge = sedges + 8 * (size_t)e;
+    const int axis = ge[0], bx = ge[1], by = ge[2], bz = ge[3], insideA = ge[4];
+    const int ax = axis == 0, ay = axis == 1, az = axis == 2;
+    const int aId = G.corner(bx, by, bz), bId = G.corner(bx + ax, by + ay, bz + az);
+    const int es = G.slot(aId, bId);
+    const int ownerA = skin_owner_a(skinT[es], cut_aware ? cutT[es] : -1.f, insideA);
+    const int ix = ownerA ? bx : bx + ax, iy = ownerA ? by : by + ay, iz = ownerA ? bz : bz + az;
+    int vt[4];
+    for (int s = 0; s < 4; s++) {
+        int dx, dy, dz, local;
+        stencil(axis, s, dx, dy, dz, local);
+        const int x = bx + dx, y = by + dy, z = bz + dz;
+        const int vid = G.voxel(x, y, z);
+        vt[s] = vid;
+        if (!cut_aware) continue;
+        const uint32_t m = mask[vid] & 0xFFFu;
+        if (m == 0u || occupied[vid] == 0u) continue;
+        const int lc = local_corner_of(ix, iy, iz, x, y, z);
+        if (lc < 0) continue;
+        const int slot = 8 * vid + conn[9 * m + 1 + lc];
+        if (cut_fp[slot].w > 0.5f) vt[s] = (int)(kCutTag | (uint32_t)slot);
+    }
+    int a = vt[0], b = vt[1], c = vt[2], d = vt[3];
+    if (insideA == 0) {
+        int t = b;
+        b = d;
+        d = t;
+    }
+    candidate(2 * e, a, b, c, fp, cut_fp, cand, ok);
+    candidate(2 * e + 1, a, c, d, fp, cut_fp, cand, ok);
+}
+
+// ---- k_BuildCutTriangles with EmitCutQuad (cut.cu:988-1090): around every cut edge of a node that cuts material,
+// the feature points of the node's component in the (up to four) cut voxels form the wall; a fan of at most two
+// triangles, skipping faces made only of points the two ends share. Slots base + 12 node + 2 edge + k.
+__global__ void k_walls(int N, int base, Grid G, const uint32_t* __restrict__ mask,
+                        const uint32_t* __restrict__ occupied, const int* __restrict__ conn,
+                        const float* __restrict__ cutT, const float* __restrict__ skinT,
+                        const int* __restrict__ inside, const float4* __restrict__ fp,
+                        const float4* __restrict__ cut_fp, int* __restrict__ cand, int* __restrict__ ok) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N) return;
+    int* okp = ok + base + 12 * (size_t)p;
+    for (int k = 0; k < 12; k++) okp[k] = 0;
+    int cx, cy, cz;
+    G.coord(p, cx, cy, cz);
+    for (int e = 0; e < 6; e++) {
+        const int axis = e >> 1;
+        const int ax = axis == 0, ay = axis == 1, az = axis == 2;
+        const bool basedHere = (e & 1) == 0;
+        const int bx = basedHere ? cx : cx - ax, by = basedHere ? cy : cy - ay, bz = basedHere ? cz : cz - az;
+        bool valid = true, isCut = false;
+        int vids[4];
+        for (int s = 0; s < 4; s++) {
+            int dx, dy, dz, local;
+            stencil(axis, s, dx, dy, dz, local);
+            if (!G.vox_in(bx + dx, by + dy, bz + dz)) {
+                valid = false;
+                break;
+            }
+            vids[s] = G.voxel(bx + dx, by + dy, bz + dz);
+            if ((mask[vids[s]] >> local) & 1u) isCut = true;
+        }
+        if (!valid || !isCut) continue;
+        const int qx = basedHere ? cx + ax : cx - ax, qy = basedHere ? cy + ay : cy - ay,
+                  qz = basedHere ? cz + az : cz - az;
+        const int other = G.corner(qx, qy, qz), es = G.slot(p, other);
+        if (!cut_hits_material(cutT[es], skinT[es], inside[p], inside[other])) continue;
+        int tag[4] = {0, 0, 0, 0}, n = 0;
+        bool okq[4] = {false, false, false, false}, shared[4] = {false, false, false, false};
+        for (int s = 0; s < 4; s++) {
+            int dx, dy, dz, local;
+            stencil(axis, s, dx, dy, dz, local);
+            const int x = bx + dx, y = by + dy, z = bz + dz, v = vids[s];
+            if (occupied[v] == 0u) continue;
+            const uint32_t m = mask[v] & 0xFFFu;
+            if (m == 0u) continue;
+            const int lc = local_corner_of(cx, cy, cz, x, y, z);
+            if (lc < 0) continue;
+            const int c = conn[9 * m + 1 + lc];
+            shared[s] = c == conn[9 * m + 1 + local_corner_of(qx, qy, qz, x, y, z)];
+            const int slot = 8 * v + c;
+            if (cut_fp[slot].w < 0.5f) continue;
+            tag[s] = (int)(kCutTag | kWallTag | (uint32_t)slot);
+            okq[s] = true;
+            n++;
+        }
+        if (n < 3) continue;
+        int ord[4], mm = 0;
+        for (int s = 0; s < 4; s++)
+            if (okq[s]) ord[mm++] = s;
+        if (mm == 4 && shared[ord[0]] && shared[ord[2]] && !shared[ord[1]] && !shared[ord[3]]) {
+            int first = ord[0];
+            ord[0] = ord[1];
+            ord[1] = ord[2];
+            ord[2] = ord[3];
+            ord[3] = first;
+        }
+        const bool flip = !basedHere;
+        for (int k = 1; k < mm - 1; k++) {
+            if (shared[ord[0]] && shared[ord[k]] && shared[ord[k + 1]]) continue;
+            const int slot = base + 12 * p + 2 * e + (k - 1);
+            candidate(slot, tag[ord[0]], tag[ord[flip ? k + 1 : k]], tag[ord[flip ? k : k + 1]], fp, cut_fp, cand, ok);
+        }
+    }
+}
+
+struct OkFlag {
+    const int* ok;
+    __device__ bool operator()(int i) const { return ok[i] != 0; }
+};
+
+// Stable compaction of the flagged triangles into out (at most capacity of them).
+__global__ void k_write_tris(int n, const int* __restrict__ ok, const int* __restrict__ partial,
+                             const int* __restrict__ in, int* __restrict__ out, int capacity) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool f = i < n && ok[i] != 0;
+    int total;
+    int r = block_prefix(f, lds, total);
+    if (!f) return;
+    int k = partial[blockIdx.x] + r;
+    if (k >= capacity) return;
+    out[3 * k] = in[3 * (size_t)i];
+    out[3 * k + 1] = in[3 * (size_t)i + 1];
+    out[3 * k + 2] = in[3 * (size_t)i + 2];
+}
+
+__global__ void k_set_count(const int* __restrict__ partial, int nblocks, int capacity, uint32_t* counts, int which) {
+    counts[which] = 3u * (uint32_t)min(partial[nblocks], capacity);
+}
+
+// ---- cleanInternalFaces (surface_cleanup.cuh:6-53): faces keyed by their three identities without the wall bit;
+// an even and an odd listing of one key cancel.
+__device__ __forceinline__ bool face_key(const int* t, uint32_t key[3]) {
+    for (int k = 0; k < 3; k++) key[k] = (uint32_t)t[k] & ~kWallTag;
+    bool odd = false;
+    uint32_t s;
+    if (key[0] > key[1]) { s = key[0]; key[0] = key[1]; key[1] = s; odd = !odd; }
+    if (key[1] > key[2]) { s = key[1]; key[1] = key[2]; key[2] = s; odd = !odd; }
+    if (key[0] > key[1]) { s = key[0]; key[0] = key[1]; key[1] = s; odd = !odd; }
+    return odd;
+}
+__device__ __forceinline__ uint32_t face_hash(const uint32_t key[3], int fc) {
+    return ((key[0] * 73856093u) ^ (key[1] * 19349663u) ^ (key[2] * 83492791u)) & (uint32_t)(fc - 1);
+}
+
+__global__ void k_face_clear(int fc, int4* __restrict__ table) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < fc) table[i] = make_int4(-1, 0, 0x7fffffff, 0x7fffffff);
+}
+
+// Entry: x = first triangle of the key, y = even minus odd listings, z / w = lowest even / odd listing.
+__global__ void k_face_index(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri, int fc,
+                             int4* table) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= capacity || 3 * i >= (int)counts[1]) return;
+    uint32_t key[3];
+    const bool odd = face_key(tri + 3 * i, key);
+    const uint32_t h = face_hash(key, fc);
+    for (int k = 0; k < fc; k++) {
+        int4* entry = table + ((h + (uint32_t)k) & (uint32_t)(fc - 1));
+        int old = atomicCAS(&entry->x, -1, i);
+        bool same = old < 0;
+        if (!same) {
+            uint32_t other[3];
+            face_key(tri + 3 * old, other);
+            same = key[0] == other[0] && key[1] == other[1] && key[2] == other[2];
+        }
+        if (same) {
+            atomicAdd(&entry->y, odd ? -1 : 1);
+            atomicMin(odd ? &entry->w : &entry->z, i);
+            return;
+        }
+    }
+}
+
+__global__ void k_face_keep(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri, int fc,
+                            const int4* __restrict__ table, int* __restrict__ ok) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= capacity) return;
+    ok[i] = 0;
+    if (3 * i >= (int)counts[1]) return;
+    uint32_t key[3];
+    face_key(tri + 3 * i, key);
+    const uint32_t h = face_hash(key, fc);
+    for (int k = 0; k < fc; k++) {
+        int4 entry = table[(h + (uint32_t)k) & (uint32_t)(fc - 1)];
+        if (entry.x < 0) return;  // not reached: every listed face was indexed
+        uint32_t other[3];
+        face_key(tri + 3 * entry.x, other);
+        if (key[0] != other[0] || key[1] != other[1] || key[2] != other[2]) continue;
+        ok[i] = entry.y != 0 && i == (entry.y > 0 ? entry.z : entry.w);
+        return;
+    }
+}
+
+// ---- k_NormalsScatter (dc_recon.cu:306-331): area-weighted face normals in fixed point to every corner's point.
+__global__ void k_normal_scatter(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri,
+                                 const float4* __restrict__ fp, const float4* __restrict__ cut_fp, int V,
+                                 int* __restrict__ fixed) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= capacity || 3 * t >= (int)counts[0]) return;
+    int raw[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+    f3 p0 = fetch_pos(raw[0], fp, cut_fp), p1 = fetch_pos(raw[1], fp, cut_fp), p2 = fetch_pos(raw[2], fp, cut_fp);
+    f3 fn = cross3(p1 - p0, p2 - p0);
+    const int q[3] = {(int)(fn.x * kNormalScale), (int)(fn.y * kNormalScale), (int)(fn.z * kNormalScale)};
+    for (int k = 0; k < 3; k++) {
+        uint32_t u = (uint32_t)raw[k];
+        size_t i = u & kIndexMask;
+        int* acc = (u & kCutTag) ? fixed + 3 * ((size_t)V + i) : fixed + 3 * i;
+        atomicAdd(acc, q[0]);
+        atomicAdd(acc + 1, q[1]);
+        atomicAdd(acc + 2, q[2]);
+    }
+}
+
+// k_NormalsNormalize (dc_recon.cu:441-451) and k_NormalizeCutFP (cut.cu:1093-1100).
+__global__ void k_normalize(int n, const int* __restrict__ fixed, float* __restrict__ normals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int* f = fixed + 3 * (size_t)i;
+    f3 v = mk3((float)f[0] / kNormalScale, (float)f[1] / kNormalScale, (float)f[2] / kNormalScale);
+    float l = len3(v);
+    f3 u = l > 1e-6f ? div3s(v, l) : mk3(0.f, 0.f, 0.f);
+    normals[3 * (size_t)i] = u.x;
+    normals[3 * (size_t)i + 1] = u.y;
+    normals[3 * (size_t)i + 2] = u.z;
+}
+
+// ---- k_ExpandVertices (dc_recon.cu:358-386): the indexed triangles as a soup of (position, normal, rest anchor and
+// wall flag) per index.
+__global__ void k_expand(int n3, const uint32_t* __restrict__ counts, const int* __restrict__ tri, int V,
+                         const float4* __restrict__ fp, const float4* __restrict__ cut_fp,
+                         const float* __restrict__ normals, const float4* __restrict__ rest_fp,
+                         const float* __restrict__ rest, const int* __restrict__ corners, float* __restrict__ out_pos,
+                         float* __restrict__ out_nrm, float* __restrict__ out_aux) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3 || i >= (int)counts[0]) return;
+    const uint32_t u = (uint32_t)tri[i];
+    const size_t idx = u & kIndexMask;
+    float4 p, aux;
+    const float* nrm;
+    if (u & kCutTag) {
+        p = cut_fp[idx];
+        nrm = normals + 3 * ((size_t)V + idx);
+        const size_t v = idx >> 3;
+        const float4 r4 = rest_fp[v];
+        f3 anchor = r4.w > 0.f ? mk3(r4.x, r4.y, r4.z)
+                               : (ld3(rest + 3 * (size_t)corners[8 * v]) + ld3(rest + 3 * (size_t)corners[8 * v + 6])) *
+                                     .5f;
+        aux = make_float4(anchor.x, anchor.y, anchor.z, (float)((u >> 30) & 1u));
+    } else {
+        p = fp[idx];
+        nrm = normals + 3 * idx;
+        const float4 r4 = rest_fp[idx];
+        aux = make_float4(r4.x, r4.y, r4.z, 0.f);
+    }
+    out_pos[3 * (size_t)i] = p.x;
+    out_pos[3 * (size_t)i + 1] = p.y;
+    out_pos[3 * (size_t)i + 2] = p.z;
+    out_nrm[3 * (size_t)i] = nrm[0];
+    out_nrm[3 * (size_t)i + 1] = nrm[1];
+    out_nrm[3 * (size_t)i + 2] = nrm[2];
+    reinterpret_cast<float4*>(out_aux)[i] = aux;
+}
+
+}  // namespace
+
+extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s, int cut_aware, int have_rest,
+                                 void* stream) {
+    if (!v || !s || v->n_nodes <= 0 || v->vx <= 0 || v->vy <= 0 || v->vz <= 0) return GTS_ERR_BAD_ARG;
+    const long long V = (long long)v->vx * v->vy * v->vz;
+    if (s->n_voxels != V || s->n_isect < 0 || s->n_surface_edges < 0 || s->tri_capacity <= 0 || s->qef_iters < 0)
+        return GTS_ERR_BAD_ARG;
+    if (s->face_capacity < 2 * s->tri_capacity || (s->face_capacity & (s->face_capacity - 1)) != 0)
+        return GTS_ERR_BAD_ARG;
+    if (!v->pos || !v->rest || !s->voxel_corners || !s->isect_offset || !s->isect_count ||
+        (s->n_isect > 0 && (!s->isect || !s->isect_world)) || (s->n_surface_edges > 0 && !s->surface_edges) ||
+        !v->edge_skin_t || !s->rest_fp || !s->fp || !s->candidates || !s->candidate_ok || !s->tri ||
+        !s->normal_fixed || !s->normals || !s->scan || !s->counts || !s->out_pos || !s->out_nrm || !s->out_aux)
+        return GTS_ERR_BAD_ARG;
+    if (cut_aware && (!v->nbr || !v->rot || !v->active || !v->inside || !v->occupied || !v->cut_mask ||
+                      !v->edge_cut_t || !v->grid_edges || !v->counters || !s->conn || !s->edge_of ||
+                      !s->edge_points || !s->cut_fp || !s->tri_work || !s->face_table ||
+                      (v->cut_point_capacity > 0 && !v->cut_points)))
+        return GTS_ERR_BAD_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const int N = v->n_nodes, Se = s->n_surface_edges, cap = s->tri_capacity;
+    const Grid G{v->vx, v->vy, v->vz, v->vx + 1, v->vy + 1};
+    const float4* pos = (const float4*)v->pos;
+    const float4* fp = (const float4*)s->fp;
+    const float4* cfp = (const float4*)s->cut_fp;
+
+    if (s->n_isect > 0)
+        k_isect_world<<<blocks_for(s->n_isect), GTS_BLOCK, 0, st>>>(
+            s->n_isect, G, s->isect, pos, v->rest, cut_aware ? v->nbr : nullptr, v->rot,
+            cut_aware ? v->edge_cut_t : nullptr, s->isect_world);
+    k_feature_points<<<blocks_for(V), GTS_BLOCK, 0, st>>>((int)V, s->qef_iters, cut_aware, have_rest, s->voxel_corners,
+                                                          s->isect_offset, s->isect_count, s->isect, s->isect_world,
+                                                          pos, v->rest, (const float4*)s->rest_fp, v->cut_mask,
+                                                          (float4*)s->fp);
+    if (cut_aware) {
+        k_fill_ints<<<blocks_for(2ll * v->n_grid_edges), GTS_BLOCK, 0, st>>>(2ll * v->n_grid_edges, s->edge_points, -1);
+        if (v->cut_point_capacity > 0)
+            k_edge_points<<<blocks_for(v->cut_point_capacity), GTS_BLOCK, 0, st>>>(
+                v->cut_point_capacity, v->counters, G, v->grid_edges, v->cut_points, s->edge_points);
+        k_component_points<<<blocks_for(V), GTS_BLOCK, 0, st>>>(
+            (int)V, G, s->qef_iters, s->voxel_corners, s->isect_offset, s->isect_count, s->isect, s->isect_world, pos,
+            v->rest, v->rot, v->active, v->occupied, v->cut_mask, s->conn, v->edge_cut_t, s->edge_of, s->edge_points,
+            v->cut_points, (float4*)s->cut_fp);
+    }
+
+    // candidates, then their ordered compaction
+    const long long nc = 2ll * Se + (cut_aware ? 12ll * N : 0ll);
+    if (nc >= (1ll << 31) / 3) return GTS_ERR_BAD_ARG;
+    if (Se > 0)
+        k_stitch<<<blocks_for(Se), GTS_BLOCK, 0, st>>>(Se, G, cut_aware, s->surface_edges, v->edge_skin_t,
+                                                       v->edge_cut_t, v->cut_mask, v->occupied, s->conn, fp, cfp,
+                                                       s->candidates, s->candidate_ok);
+    if (cut_aware)
+        k_walls<<<blocks_for(N), GTS_BLOCK, 0, st>>>(N, 2 * Se, G, v->cut_mask, v->occupied, s->conn, v->edge_cut_t,
+                                                     v->edge_skin_t, v->inside, fp, cfp, s->candidates,
+                                                     s->candidate_ok);
+    const unsigned nbc = blocks_for(nc);
+    int* listed = cut_aware ? s->tri_work : s->tri;
+    k_count_flags<<<nbc, GTS_BLOCK, 0, st>>>((int)nc, OkFlag{s->candidate_ok}, s->scan);
+    k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nbc, s->scan);
+    k_write_tris<<<nbc, GTS_BLOCK, 0, st>>>((int)nc, s->candidate_ok, s->scan, s->candidates, listed, cap);
+    k_set_count<<<1, 1, 0, st>>>(s->scan, (int)nbc, cap, s->counts, 1);
+
+    if (cut_aware) {
+        const unsigned nbt = blocks_for(cap);
+        k_face_clear<<<blocks_for(s->face_capacity), GTS_BLOCK, 0, st>>>(s->face_capacity, (int4*)s->face_table);
+        k_face_index<<<nbt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, s->face_capacity, (int4*)s->face_table);
+        k_face_keep<<<nbt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, s->face_capacity, (const int4*)s->face_table,
+                                               s->candidate_ok);
+        k_count_flags<<<nbt, GTS_BLOCK, 0, st>>>(cap, OkFlag{s->candidate_ok}, s->scan);
+        k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nbt, s->scan);
+        k_write_tris<<<nbt, GTS_BLOCK, 0, st>>>(cap, s->candidate_ok, s->scan, listed, s->tri, cap);
+        k_set_count<<<1, 1, 0, st>>>(s->scan, (int)nbt, cap, s->counts, 0);
+    } else {
+        k_set_count<<<1, 1, 0, st>>>(s->scan, (int)nbc, cap, s->counts, 0);
+    }
+
+    // normals of the voxel points and of the component points, then the soup
+    k_fill_ints<<<blocks_for(27 * V), GTS_BLOCK, 0, st>>>(27 * V, s->normal_fixed, 0);
+    k_normal_scatter<<<blocks_for(cap), GTS_BLOCK, 0, st>>>(cap, s->counts, s->tri, fp, cfp, (int)V, s->normal_fixed);
+    k_normalize<<<blocks_for(9 * V), GTS_BLOCK, 0, st>>>((int)(9 * V), s->normal_fixed, s->normals);
+    k_expand<<<blocks_for(3ll * cap), GTS_BLOCK, 0, st>>>(3 * cap, s->counts, s->tri, (int)V, fp, cfp, s->normals,
+                                                          (const float4*)s->rest_fp, v->rest, s->voxel_corners,
+                                                          s->out_pos, s->out_nrm, s->out_aux);
+    HIP_TRY(hipGetLastError());
+    return GTS_OK;
+}

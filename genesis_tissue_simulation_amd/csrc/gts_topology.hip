@@ -81,6 +81,7 @@ extern "C" int gts_struct_sizes(int* out) {
     out[8] = (int)sizeof(GtsAdjointView);
     out[9] = (int)sizeof(GtsGraspParams);
     out[10] = (int)sizeof(GtsGraspState);
+    out[11] = (int)sizeof(GtsSurfaceView);
     return GTS_OK;
 }
 

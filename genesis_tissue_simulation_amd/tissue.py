@@ -320,6 +320,69 @@ class Tissue:
         check("gts_tear_overstretch", self.lib.gts_tear_overstretch(ct.byref(self.view), float(tau), float(tool_d),
                                                                     current_stream()))
 
+    # ---------------------------------------------------------------- surface
+    def init_surface(self, tri_capacity=None, qef_iters=8):
+        """recon_init (dc_recon.cu:555-658): upload the grid's intersections, surface edges and voxel corners, build
+        the surface once on the rest lattice with no cut state, and keep its feature points as the rest snapshot
+        that later rebuilds advect. `tri_capacity` defaults to NativeTissue's (native_backend.py:291)."""
+        g, dev, N, V = self.grid, self.device, self.N, self.V
+        Se, I = len(g.surface_edges), len(g.intersections)
+        cap = int(tri_capacity or 6 * Se + 1024 + 2 * self.Eg)
+        fc = 1
+        while fc < 2 * cap:
+            fc *= 2
+        nc = 2 * Se + 12 * N
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
+        zf = lambda n: torch.zeros(max(int(n), 1), dtype=torch.float32, device=dev)
+        zi = lambda n: torch.zeros(max(int(n), 1), dtype=torch.int32, device=dev)
+        edge_of = np.full(3 * N, -1, dtype=np.int32)
+        if self.Eg:
+            e = g.edges.astype(np.int64)
+            nx, ny = g.dims[0] + 1, g.dims[1] + 1
+            edge_of[3 * (e[:, 1] + nx * (e[:, 2] + ny * e[:, 3])) + e[:, 0]] = np.arange(self.Eg, dtype=np.int32)
+        isect = np.ascontiguousarray(g.intersections).view(np.float32).reshape(-1) if I else np.zeros(10, np.float32)
+        self.surf = dict(
+            voxel_corners=i32(g.corners.reshape(-1)), isect_offset=i32(g.offsets), isect_count=i32(g.counts),
+            isect=torch.as_tensor(isect.copy()).to(dev), surface_edges=i32(g.surface_edges.reshape(-1) if Se else [0]),
+            conn=i32(g.connectivity.reshape(-1)), edge_of=i32(edge_of), rest_fp=zf(4 * V),
+            isect_world=zf(3 * I), fp=zf(4 * V), cut_fp=zf(32 * V), edge_points=zi(2 * self.Eg),
+            candidates=zi(3 * nc), candidate_ok=zi(max(nc, cap)), tri=zi(3 * cap), tri_work=zi(3 * cap),
+            face_table=zi(4 * fc), normal_fixed=zi(27 * V), normals=zf(27 * V), scan=zi(-(-max(nc, cap) // 256) + 1),
+            counts=zi(4), out_pos=zf(9 * cap), out_nrm=zf(9 * cap), out_aux=zf(12 * cap))
+        names = [f[0] for f in _lib.GtsSurfaceView._fields_[6:]]
+        self.surface_view = _lib.GtsSurfaceView(V, I, Se, cap, int(qef_iters), fc,
+                                                *[self.surf[n].data_ptr() for n in names])
+        # the rest build reads the rest lattice, whatever the tissue's initial positions
+        rest4 = torch.zeros((N, 4), dtype=torch.float32, device=dev)
+        rest4[:, :3] = self.rest.reshape(N, 3)
+        at_rest = GtsTissueView.from_buffer_copy(self.view)
+        at_rest.pos = rest4.data_ptr()
+        check("gts_surface_build", self.lib.gts_surface_build(ct.byref(at_rest), ct.byref(self.surface_view), 0, 0,
+                                                              current_stream()))
+        self.surf["rest_fp"].copy_(self.surf["fp"])
+        torch.cuda.current_stream().synchronize()
+
+    def rebuild_surface(self):
+        """The surface rebuild of LCS_Finalize (recon_rebuild, dc_recon.cu:736-741) from the current positions,
+        frames, links and cut state."""
+        if getattr(self, "surface_view", None) is None:
+            self.init_surface()
+        check("gts_surface_build", self.lib.gts_surface_build(ct.byref(self.view), ct.byref(self.surface_view), 1, 1,
+                                                              current_stream()))
+
+    def surface(self):
+        """LCS_GetSurfaceCounts / LCS_GetSurface / LCS_GetSurfaceAux (recon_get_counts / recon_get_surface /
+        recon_get_surface_aux, dc_recon.cu:660-712) of the last rebuild: the triangle soup's positions [K, 3],
+        normals [K, 3], indices [K / 3, 3] (0..K-1) and aux [K, 4] (rest anchor, wall flag), plus the vertex tags."""
+        s = self.surf
+        k = int(s["counts"][0].item())
+        return dict(pos=s["out_pos"][:3 * k].reshape(-1, 3).cpu().numpy(),
+                    nrm=s["out_nrm"][:3 * k].reshape(-1, 3).cpu().numpy(),
+                    indices=np.arange(k, dtype=np.int32).reshape(-1, 3),
+                    aux=s["out_aux"][:4 * k].reshape(-1, 4).cpu().numpy(),
+                    tags=s["tri"][:k].cpu().numpy(), fp=s["fp"].reshape(-1, 4).cpu().numpy(),
+                    cut_fp=s["cut_fp"].reshape(-1, 4).cpu().numpy(), listed=int(s["counts"][1].item()))
+
     # ---------------------------------------------------------------- readback
     def positions(self):
         return self.pos[:, :3].cpu().numpy()

@@ -193,8 +193,8 @@ typedef struct GtsCutTool {
 
 int gts_abi_version(void);
 /* Sizes of the structs above as this library sees them (ABI guard like LCS_PhysicsAbiVersion,
- * plugin_api.cpp:102). out[0..10] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
- * GtsBatchInstance, GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState. */
+ * plugin_api.cpp:102). out[0..11] = GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool,
+ * GtsBatchInstance, GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState, GtsSurfaceView. */
 int gts_struct_sizes(int* out);
 
 /* Host: derive edges, Kuhn tets and both greedy colourings from the lattice. Needs no GPU. */
@@ -364,6 +364,49 @@ int gts_adjoint_forward(const GtsAdjointView* a, const GtsStepParams* p, int nsu
  * staged in LDS when it has at most batch.lds_nodes nodes. Backward calls must not overlap one another. */
 #define GTS_ADJOINT_BACKWARD_MAX_THREADS 512
 int gts_adjoint_backward(const GtsAdjointView* a, const GtsStepParams* p, int nsub, int ckpt, void* stream);
+
+/* Surface reconstruction of the deformed, cut lattice: the dual-contouring rebuild (recon_build_static,
+ * dc_recon.cu:498-551) with the cut feature points and cut walls (cut_fp_chain / cut_build_triangles /
+ * cut_normalize_fp, cut.cu:1434-1505) and the internal-face cleanup (surface_cleanup.cuh:35-53). What LCS_Finalize
+ * rebuilds (plugin_api.cpp:161-169) and LCS_GetSurfaceCounts / LCS_GetSurface / LCS_GetSurfaceAux read
+ * (plugin_api.cpp:82-99). Grid inputs are the ones LCS_Init and LCS_InitCut upload (dc_recon.h:29-36). */
+typedef struct GtsSurfaceView {
+    int n_voxels, n_isect, n_surface_edges;
+    int tri_capacity;               /* triangles the surface holds (ReconInitDesc.triCapacity) */
+    int qef_iters;                  /* feature point iterations (ReconInitDesc.qefIters) */
+    int face_capacity;              /* entries of face_table: a power of two >= 2 * tri_capacity */
+    const int32_t* voxel_corners;   /* [8V] node of each voxel corner, figure order (cut.cuh:90-99) */
+    const int32_t* isect_offset;    /* [V] first intersection of the voxel */
+    const int32_t* isect_count;     /* [V] */
+    const float* isect;             /* [10I] IsectGpu records (common.cuh:23-33): rest xyz, normal xyz, a, b, t, insideA */
+    const int32_t* surface_edges;   /* [8Se] GridEdgeGpu records (common.cuh:36-42): axis, base xyz, insideA, pad */
+    const int32_t* conn;            /* [9 * 4096] Conn4096Gpu (cut.cuh:51-56): component count, component of each corner */
+    const int32_t* edge_of;         /* [3N] cut edge id of the lattice edge (axis, base node), -1 if it is none */
+    const float* rest_fp;           /* [4V] feature points of the rest surface (the snapshot recon_init takes) */
+    float* isect_world;             /* [3I] work: deformed intersections */
+    float* fp;                      /* [4V] out: voxel feature points, w = 1 where the voxel has one */
+    float* cut_fp;                  /* [32V] out: component feature points, slot 8 * voxel + component */
+    int32_t* edge_points;           /* [2Eg] work: the cut points of each cut edge, owner A then B */
+    int32_t* candidates;            /* [3 * (2Se + 12N)] work: triangles before compaction */
+    int32_t* candidate_ok;          /* [max(2Se + 12N, tri_capacity)] work */
+    int32_t* tri;                   /* [3 * tri_capacity] out: triangle vertex tags (bit 31 component point, bit 30 wall) */
+    int32_t* tri_work;              /* [3 * tri_capacity] work */
+    int32_t* face_table;            /* [4 * face_capacity] work */
+    int32_t* normal_fixed;          /* [27V] work: fixed-point normal sums, voxels then component slots */
+    float* normals;                 /* [27V] work: unit normals, voxels then component slots */
+    int32_t* scan;                  /* [ceil(max(2Se + 12N, tri_capacity) / 256) + 1] work */
+    uint32_t* counts;               /* [4] out: 0 triangle indices, 1 indices before the cleanup */
+    float* out_pos;                 /* [9 * tri_capacity] out: the triangle soup, xyz per index */
+    float* out_nrm;                 /* [9 * tri_capacity] out */
+    float* out_aux;                 /* [12 * tri_capacity] out: rest anchor xyz, wall flag */
+} GtsSurfaceView;
+
+/* Device: rebuild the surface of the tissue in `v` (positions, frames, live links and cut state) into `s`.
+ * cut_aware = 1 is the rebuild after LCS_Finalize (the cut buffers are live, dc_recon.cu:662-673); 0 is recon_init's
+ * build before any cut state exists. have_rest = 1 advects the intact voxels' rest feature points
+ * (k_AdvectIntactSurface, dc_recon.cu:477-493). Triangles are ordered deterministically: the stitch's two per surface
+ * edge in edge order, then the cut walls in node order; the cleanup keeps each face's first listed representative. */
+int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s, int cut_aware, int have_rest, void* stream);
 
 /* Device: copy n_float4 16-byte elements (no reference counterpart): the known-size kernel against which HBM byte
  * counters are scaled, and the achievable copy bandwidth. */

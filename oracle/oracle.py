@@ -55,7 +55,8 @@ _lib = None
 
 def build(force=False):
     """Compile liboracle.so from oracle/xpbd_oracle.cpp with the committed Makefile."""
-    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "xpbd_oracle.cpp").stat().st_mtime:
+    newest = max((HERE / f).stat().st_mtime for f in ("xpbd_oracle.cpp", "surface_oracle.cpp"))
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < newest:
         subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "liboracle.so"], check=True)
     return LIB_PATH
 
@@ -326,3 +327,42 @@ def set_threads(n):
 
 if os.environ.get("GTS_ORACLE_BUILD"):
     build()
+
+
+class OracleSurfaceDesc(ct.Structure):
+    _fields_ = [("n_nodes", I), ("n_voxels", I), ("n_isect", I), ("n_surface_edges", I), ("n_grid_edges", I),
+                ("tri_capacity", I), ("vx", I), ("vy", I), ("vz", I), ("qef_iters", I), ("cut_aware", I),
+                ("have_rest", I), ("n_cut_points", I)]
+
+
+def surface(grid, pos, rot, nbr, cut_mask, edge_cut_t, cut_points, rest_fp, *, cut_aware, have_rest, tri_capacity,
+            qef_iters=8):
+    """One surface rebuild (oracle_surface in surface_oracle.cpp) of the lattice state `pos` [N, 3] with frames
+    `rot` [N, 3, 3], live links `nbr`, the cut state and the rest snapshot `rest_fp` [V, 4] of the feature points.
+
+    Returns dict(pos [K, 3], nrm [K, 3], aux [K, 4], tags [K], fp [V, 4], cut_fp [8V, 4], listed) for K triangle
+    indices, `listed` being the index count before the face cleanup."""
+    L = lib()
+    f = L.oracle_surface
+    f.restype = I
+    f.argtypes = [ct.POINTER(OracleSurfaceDesc)] + [P] * 26
+    N, V = len(grid.positions), int(np.prod(grid.dims))
+    c32 = lambda a, t=np.float32: np.ascontiguousarray(a, dtype=t)
+    points = np.ascontiguousarray(cut_points) if len(cut_points) else np.zeros(1, dtype=CUT_POINT_DTYPE)
+    d = OracleSurfaceDesc(N, V, len(grid.intersections), len(grid.surface_edges), len(grid.edges), int(tri_capacity),
+                          *grid.dims, int(qef_iters), int(cut_aware), int(have_rest), len(cut_points))
+    K = 3 * int(tri_capacity)
+    out = dict(pos=np.zeros((K, 3), np.float32), nrm=np.zeros((K, 3), np.float32), aux=np.zeros((K, 4), np.float32),
+               tags=np.zeros(K, np.int32), fp=np.zeros((V, 4), np.float32), cut_fp=np.zeros((8 * V, 4), np.float32))
+    args = [c32(pos), c32(grid.positions), c32(rot), c32(nbr, np.int32), c32(grid.active, np.int32),
+            c32(grid.inside, np.int32), c32(grid.corners, np.int32), c32(grid.offsets, np.int32),
+            c32(grid.counts, np.int32), np.ascontiguousarray(grid.intersections), c32(grid.surface_edges, np.int32),
+            c32(grid.connectivity, np.int32), c32(grid.edges, np.int32), c32(grid.occupied, np.uint32),
+            c32(cut_mask, np.uint32), c32(edge_cut_t), c32(grid.edge_skin_fraction()), points, c32(rest_fp)]
+    listed = np.zeros(1, np.int32)
+    n = f(ct.byref(d), *[_ptr(a) for a in args],
+          *[_ptr(out[k]) for k in ("pos", "nrm", "aux", "tags", "fp", "cut_fp")], _ptr(listed))
+    for k in ("pos", "nrm", "aux", "tags"):
+        out[k] = out[k][:n]
+    out["listed"] = int(listed[0])
+    return out

@@ -1,0 +1,116 @@
+"""Parity of the HIP surface reconstruction (gts_surface_build) with the oracle (GPU).
+
+Bar: bit-exact triangle tags, soup positions, normals and aux, voxel and component feature points, and the index
+count before the face cleanup, for the rest build and for the rebuilds of a deforming tissue being cut. The oracle is
+fed the tissue's own state (positions, frames, links, cut state) at each rebuild, so the comparison isolates the
+surface kernels (the step and the cut have parity tests of their own). Cases: a block in its air shell and an
+ellipsoid; a partial cut, a crossing cut (whose duplicate wall faces the cleanup removes), a cut that separates the
+block, and a separated piece moved and turned, whose severed surface edges are rebuilt from its frames.
+"""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.grid import Ellipsoid, build_grid
+from genesis_tissue_simulation_amd.params import TET_NEOHOOKEAN, SolverParams
+from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+
+pytestmark = pytest.mark.gpu
+
+D = 0.02
+GRIDS = {"block": lambda: build_grid(spacing=0.6),
+         "ellipsoid": lambda: build_grid(spacing=0.3, shape=Ellipsoid((2.5, 1.5, 1.2)))}
+
+
+def blade_x(level, depth_y, z=4.0, top=2.0):
+    return swept_cut_tool(((level, top, -z), (level, depth_y, -z)), ((level, top, z), (level, depth_y, z)), D)
+
+
+def blade_z(level, depth_y, x=4.0, top=2.0):
+    return swept_cut_tool(((-x, top, level), (-x, depth_y, level)), ((x, top, level), (x, depth_y, level)), D)
+
+
+def tissue(grid):
+    from genesis_tissue_simulation_amd.tissue import Tissue
+    from genesis_tissue_simulation_amd.workloads import tet_count
+    T = tet_count(grid)
+    k = np.full(T, 2e5, dtype=np.float32)
+    p = SolverParams(substeps=5, iterations=1, tet_mode=TET_NEOHOOKEAN, edge_ks=2e5)
+    t = Tissue(grid, p, k, 10 * k, grid.positions)
+    t.set_gravity((0.0, -981.0, 0.0))
+    return t
+
+
+def oracle_surface(t, cut_aware=True):
+    from oracle import oracle
+    g = t.grid
+    cap = t.surface_view.tri_capacity
+    if not cut_aware:
+        N, V = len(g.positions), int(np.prod(g.dims))
+        return oracle.surface(g, g.positions, np.broadcast_to(np.eye(3, dtype=np.float32), (N, 3, 3)), g.neighbors,
+                              np.zeros(V, np.uint32), np.full((N, 6), -1, np.float32),
+                              np.zeros(0, oracle.CUT_POINT_DTYPE), np.zeros((V, 4), np.float32), cut_aware=0,
+                              have_rest=0, tri_capacity=cap)
+    cs = t.cut_state()
+    rest_fp = t.surf["rest_fp"].reshape(-1, 4).cpu().numpy()
+    return oracle.surface(g, t.positions(), t.rotations(), cs["neighbors"], cs["cut_mask"], cs["edge_cut_t"],
+                          cs["cut_points"], rest_fp, cut_aware=1, have_rest=1, tri_capacity=cap)
+
+
+def same(gpu, ora, what):
+    assert gpu["listed"] == ora["listed"], what
+    for k in ("tags", "pos", "nrm", "aux", "fp", "cut_fp"):
+        np.testing.assert_array_equal(gpu[k], ora[k], err_msg=f"{k}: {what}")
+    return len(ora["tags"]) // 3
+
+
+@pytest.mark.parametrize("shape", list(GRIDS))
+def test_rest_surface_matches_oracle(shape):
+    t = tissue(GRIDS[shape]())
+    print(f"surface parity: rest build, {shape}", flush=True)
+    t.init_surface()
+    n = same(t.surface(), oracle_surface(t, cut_aware=False), f"rest {shape}")
+    assert n == 2 * len(t.grid.surface_edges)
+
+
+@pytest.mark.parametrize("shape", list(GRIDS))
+def test_rebuilds_of_a_tissue_being_cut_match_oracle(shape):
+    t = tissue(GRIDS[shape]())
+    t.init_surface()
+    depth = -0.1 if shape == "block" else -0.5
+    cuts = {2: [blade_x(0.1, depth)], 4: [blade_z(0.05, depth)], 6: [blade_x(-0.9, -3.0)]}
+    removed = 0
+    for s in range(8):
+        print(f"surface parity: {shape}, step {s}", flush=True)
+        t.step(1 / 60)
+        for c in cuts.get(s, []):
+            t.detect_cut(c)
+        t.finalize()
+        t.rebuild_surface()
+        gpu, ora = t.surface(), oracle_surface(t)
+        same(gpu, ora, f"{shape} step {s}")
+        removed = max(removed, ora["listed"] - len(ora["tags"]))
+    assert t.revision() > 0
+    assert int((gpu["aux"][:, 3] > 0).sum()) > 0
+    if shape == "ellipsoid":
+        assert removed > 0          # the crossing cut's duplicate faces went through the cleanup
+
+
+def test_a_separated_piece_moved_and_turned_matches_oracle():
+    t = tissue(GRIDS["block"]())
+    t.init_surface()
+    t.detect_cut(blade_x(0.1, -3.0))
+    t.finalize()
+    x = t.positions().astype(np.float64)
+    right = x[:, 0] > 0.1
+    c = x[right].mean(axis=0)
+    a = 0.3
+    R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    x[right] = (x[right] - c) @ R.T + c + np.array([0.8, 0.3, -0.2])
+    t.set_state(x.astype(np.float32), np.zeros_like(x, dtype=np.float32))
+    for k in range(3):
+        print(f"surface parity: separated piece, frame update {k}", flush=True)
+        t.finalize()
+        t.rebuild_surface()
+        same(t.surface(), oracle_surface(t), f"frame update {k}")
+    rot = t.rotations()
+    assert np.abs(rot[right & (t.grid.active != 0)] - np.eye(3)).max() > 0.05

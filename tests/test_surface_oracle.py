@@ -1,0 +1,166 @@
+"""Pins of the oracle's surface reconstruction (oracle/surface_oracle.cpp: recon_build_static, the cut feature point
+chain, the cut walls and the face cleanup of dc_recon.cu / cut.cu / surface_cleanup.cuh), no GPU.
+
+The reference holds no fixtures for this path, so the pins are what the geometry fixes:
+  - the rest surface of a box is a closed, outward-facing 2-manifold of two triangles per surface edge whose flat
+    faces lie on the box's planes, and an ellipsoid's feature points lie on the ellipsoid to a fraction of a cell;
+  - the rebuild of an uncut lattice advects the rest feature points: at rest it reproduces the rest surface, under a
+    rigid translation it translates it;
+  - after a cut the surface stays a closed manifold, the wall points of split voxels lie on the kerf planes (the cut
+    plane +- half the blade thickness) with normals pointing into the kerf, and only wall triangles carry the wall
+    flag;
+  - once a cut separates two pieces, moving one piece rigidly moves exactly its part of the surface: a severed
+    surface edge is rebuilt from its owner's frame, not stretched across the gap.
+"""
+from collections import Counter
+
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd.grid import Ellipsoid, build_grid
+from genesis_tissue_simulation_amd.params import SolverParams
+from genesis_tissue_simulation_amd.tissue import swept_cut_tool
+
+D = 0.02
+
+
+def capacity(g):
+    return 6 * len(g.surface_edges) + 1024 + 2 * len(g.edges)
+
+
+def rest_surface(oracle_lib, g):
+    N, V = len(g.positions), int(np.prod(g.dims))
+    rot = np.broadcast_to(np.eye(3, dtype=np.float32), (N, 3, 3))
+    return oracle_lib.surface(g, g.positions, rot, g.neighbors, np.zeros(V, np.uint32),
+                              np.full((N, 6), -1, np.float32), np.zeros(0, oracle_lib.CUT_POINT_DTYPE),
+                              np.zeros((V, 4), np.float32), cut_aware=0, have_rest=0, tri_capacity=capacity(g))
+
+
+def rebuild(oracle_lib, g, o, rest_fp, pos=None):
+    cs = o.cut_state()
+    pos = o.state()[0] if pos is None else pos
+    return oracle_lib.surface(g, pos, o.rotations(), cs["neighbors"], cs["cut_mask"], cs["edge_cut_t"],
+                              cs["cut_points"], rest_fp, cut_aware=1, have_rest=1, tri_capacity=capacity(g))
+
+
+def manifold_defects(tags):
+    """Directed edges that are not matched by exactly one opposite edge (identity without the wall bit)."""
+    t = (tags.view(np.uint32) & np.uint32(0xBFFFFFFF)).reshape(-1, 3).astype(np.int64)
+    directed = Counter()
+    for a, b, c in t:
+        for e in ((a, b), (b, c), (c, a)):
+            directed[e] += 1
+    return [e for e, n in directed.items() if n != 1 or directed.get((e[1], e[0]), 0) != 1], t
+
+
+def euler(t):
+    edges = {tuple(sorted(e)) for a, b, c in t for e in ((a, b), (b, c), (c, a))}
+    return len(set(t.ravel())) - len(edges) + len(t)
+
+
+@pytest.fixture(scope="module")
+def box():
+    return build_grid(spacing=0.6)      # a 6 x 1.4 x 4 block in a two-cell air shell
+
+
+def test_rest_box_surface_is_a_closed_outward_manifold_on_the_box_planes(oracle_lib, box):
+    s = rest_surface(oracle_lib, box)
+    bad, t = manifold_defects(s["tags"])
+    assert not bad and euler(t) == 2
+    assert len(t) == 2 * len(box.surface_edges)
+    p, n = s["pos"].astype(np.float64), s["nrm"].astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)
+    assert (np.sum(n * (p - p.mean(axis=0)), axis=1) > 0).all()
+    # a voxel with crossings on one face only: its feature point is the mean of coplanar points, on the plane
+    half = np.array([3.0, 0.7, 2.0])
+    for axis in range(3):
+        on = np.abs(np.abs(p[:, axis]) - half[axis]) < 1e-5
+        others = [k for k in range(3) if k != axis]
+        interior = on & np.all(np.abs(p[:, others]) < half[others] - 0.6, axis=1)
+        assert interior.sum() >= 4, axis
+    # every feature point lies in its voxel's corner box
+    fp = s["fp"]
+    have = fp[:, 3] > 0
+    lo = box.positions[box.corners[:, 0]]
+    hi = box.positions[box.corners[:, 6]]
+    assert (fp[have, :3] > lo[have] - 1e-6).all() and (fp[have, :3] < hi[have] + 1e-6).all()
+
+
+def test_ellipsoid_feature_points_lie_on_the_ellipsoid(oracle_lib):
+    radii = np.array([2.5, 1.5, 1.2])
+    g = build_grid(spacing=0.3, shape=Ellipsoid(tuple(radii)))
+    s = rest_surface(oracle_lib, g)
+    bad, t = manifold_defects(s["tags"])
+    assert not bad and euler(t) == 2
+    fp = s["fp"][s["fp"][:, 3] > 0, :3].astype(np.float64)
+    level = np.linalg.norm(fp / radii, axis=1)             # 1 on the ellipsoid (centred on the origin)
+    assert np.abs(level - 1).max() < 0.3 * 0.3 / radii.min()
+
+
+def test_uncut_rebuild_advects_the_rest_surface(oracle_lib, box):
+    o = oracle_lib.Oracle(box, SolverParams(), box.positions)
+    s0 = rest_surface(oracle_lib, box)
+    s1 = rebuild(oracle_lib, box, o, s0["fp"])
+    np.testing.assert_array_equal(s1["tags"], s0["tags"])
+    np.testing.assert_allclose(s1["pos"], s0["pos"], atol=1e-6)
+    np.testing.assert_allclose(s1["nrm"], s0["nrm"], atol=1e-6)
+    np.testing.assert_array_equal(s1["aux"][:, :3], s0["fp"][s0["tags"], :3])
+    shift = np.array([0.37, -1.1, 0.25], dtype=np.float32)
+    moved = rebuild(oracle_lib, box, o, s0["fp"], pos=box.positions + shift)
+    np.testing.assert_array_equal(moved["tags"], s0["tags"])
+    np.testing.assert_allclose(moved["pos"], s0["pos"] + shift, atol=2e-6)
+    np.testing.assert_allclose(moved["nrm"], s0["nrm"], atol=1e-5)
+
+
+def blade_x(level, depth_y, z=3.0, top=1.5):
+    """A vertical blade in the plane x = level from y = top down to depth_y, swept along z across the block."""
+    return swept_cut_tool(((level, top, -z), (level, depth_y, -z)), ((level, top, z), (level, depth_y, z)), D)
+
+
+def test_a_cut_keeps_the_surface_closed_with_walls_on_the_kerf_planes(oracle_lib, box):
+    o = oracle_lib.Oracle(box, SolverParams(), box.positions)
+    s0 = rest_surface(oracle_lib, box)
+    o.detect_cut(blade_x(0.1, -0.1))
+    o.compute_rotations()
+    s = rebuild(oracle_lib, box, o, s0["fp"])
+    bad, t = manifold_defects(s["tags"])
+    assert not bad
+    tags = s["tags"].view(np.uint32)
+    wall = (tags & np.uint32(0x40000000)) != 0
+    np.testing.assert_array_equal(s["aux"][:, 3], wall.astype(np.float32))
+    assert wall.reshape(-1, 3).all(axis=1).sum() >= 8 and (wall.reshape(-1, 3).any(axis=1) ==
+                                                           wall.reshape(-1, 3).all(axis=1)).all()
+    # the wall points are component points of cut voxels
+    assert ((tags[wall] & np.uint32(0x80000000)) != 0).all()
+    # where the cut has split a voxel (two components), each side's point sits on its kerf plane with its normal
+    # pointing into the kerf; at the crack tip a voxel stays whole and its point averages both sides' cut points
+    voxel = (tags & np.uint32(0x3FFFFFFF)) >> 3
+    split = wall & (box.connectivity[o.cut_state()["cut_mask"][voxel] & 0xFFF, 0] >= 2)
+    assert split.sum() >= 12
+    p, n = s["pos"][split], s["nrm"][split]
+    side = np.sign(p[:, 0] - 0.1)
+    np.testing.assert_allclose(p[:, 0], 0.1 + side * D / 2, atol=2e-6)
+    assert (n[:, 0] * side < -0.2).all()       # shared with the skin quads, so a blend of wall and skin normals
+
+
+def test_a_separated_piece_moved_rigidly_carries_its_own_surface(oracle_lib, box):
+    o = oracle_lib.Oracle(box, SolverParams(), box.positions)
+    s0 = rest_surface(oracle_lib, box)
+    o.detect_cut(blade_x(0.1, -1.5))                  # through the whole height: two pieces
+    o.compute_rotations()
+    before = rebuild(oracle_lib, box, o, s0["fp"])
+    bad, _ = manifold_defects(before["tags"])
+    assert not bad
+    x = box.positions.astype(np.float64)
+    right = x[:, 0] > 0.1
+    shift = np.array([0.9, 0.4, -0.3])
+    moved = x.copy()
+    moved[right] += shift
+    after = rebuild(oracle_lib, box, o, s0["fp"], pos=moved.astype(np.float32))
+    np.testing.assert_array_equal(after["tags"], before["tags"])
+    p0 = before["pos"].astype(np.float64)
+    delta = after["pos"] - p0
+    on_right = p0[:, 0] > 0.1
+    assert on_right.any() and (~on_right).any()
+    np.testing.assert_allclose(delta[on_right], np.broadcast_to(shift, delta[on_right].shape), atol=1e-5)
+    np.testing.assert_allclose(delta[~on_right], 0.0, atol=1e-5)

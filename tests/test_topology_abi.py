@@ -179,6 +179,13 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     assert lib.gts_adjoint_forward(null, null, 10, 5, null) == BAD
     assert lib.gts_adjoint_backward(null, null, 10, 5, null) == BAD
     assert lib.gts_batched_plan(null, 4, 0, null) == BAD
+    assert lib.gts_surface_build(null, null, 1, 1, null) == BAD
+    # a surface view whose face table is not a power of two at least twice the triangle capacity
+    sview = _lib.GtsSurfaceView()
+    sview.n_voxels, sview.tri_capacity, sview.face_capacity = 8, 16, 24
+    tv = _lib.GtsTissueView()
+    tv.n_nodes, tv.vx, tv.vy, tv.vz = 27, 2, 2, 2
+    assert lib.gts_surface_build(ct.byref(tv), ct.byref(sview), 0, 0, null) == BAD
     # a step with a view but an invalid tet mode
     view = _lib.GtsTissueView()
     params = _lib.GtsStepParams()
