@@ -166,7 +166,49 @@ def run_gpu(w, steps, warmup, tune=True):
         raise RuntimeError("non-finite positions: simulation stopped")
     return dict(elapsed=elapsed, live=(live0 + live1) / 2, tet_ms=total_ms.value, launches=launches.value,
                 units=units.value, E=t.E, T=t.T, active=int((w.grid.active != 0).sum()),
-                colour_block=best["colour_block"], tune=table)
+                colour_block=best["colour_block"], tune=table, tissue=t)
+
+
+def surface_bytes(N, V, I, Se, K):
+    """Compulsory HBM bytes of one surface rebuild: node positions (16 B), voxel corner ids, rest and new feature
+    points, cut mask and occupancy (32 + 16 + 16 + 8 B a voxel), intersection records read and their deformed points
+    written and read (40 + 24 B), surface edge records (32 B), and per soup index its position, normal, aux and tag
+    (12 + 12 + 16 + 4 B)."""
+    return 16 * N + 72 * V + 64 * I + 32 * Se + 44 * K
+
+
+def run_surface(t, reps=20, cpu=True):
+    """The rebuild LCS_Finalize runs (frames, then gts_surface_build) on the tissue's current, cut state, timed with
+    HIP events on the stream it is launched on; the oracle's rebuild of the same state is the CPU baseline."""
+    import torch
+
+    t.init_surface()
+    t.finalize()
+    t.rebuild_surface()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        t.rebuild_surface()
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / reps
+    s = t.surface()
+    g = t.grid
+    N, V, I, Se, K = t.N, t.V, len(g.intersections), len(g.surface_edges), len(s["tags"])
+    out = dict(ms=ms, N=N, V=V, I=I, Se=Se, K=K, walls=int((s["aux"][:, 3] > 0).sum()) // 3,
+               bytes=surface_bytes(N, V, I, Se, K), cpu_ms=None)
+    if cpu:
+        from oracle import oracle
+        cs = t.cut_state()
+        args = (g, t.positions(), t.rotations(), cs["neighbors"], cs["cut_mask"], cs["edge_cut_t"], cs["cut_points"],
+                t.surf["rest_fp"].reshape(-1, 4).cpu().numpy())
+        t0 = time.perf_counter()
+        o = oracle.surface(*args, cut_aware=1, have_rest=1, tri_capacity=t.surface_view.tri_capacity)
+        out["cpu_ms"] = 1e3 * (time.perf_counter() - t0)
+        out["bit_exact"] = bool(np.array_equal(o["tags"], s["tags"]) and np.array_equal(o["pos"], s["pos"]) and
+                                np.array_equal(o["nrm"], s["nrm"]) and np.array_equal(o["aux"], s["aux"]))
+    return out
 
 
 def run_batched(steps, warmup, count=1024, tune=True):
@@ -567,6 +609,21 @@ def main():
                      "launches": g["launches"], "bytes_per_tet_update": b_alg,
                      "algorithmic_bytes_per_launch": b_alg * g["units"] / max(g["launches"], 1)},
     }
+    sf = run_surface(g.pop("tissue"), cpu=not args.no_cpu)
+    sach = sf["bytes"] / (sf["ms"] * 1e-3) / 1e9
+    line["surface"] = {
+        "value": sf["V"] / (sf["ms"] * 1e-3), "unit": "voxels/s", "ms_per_rebuild": sf["ms"],
+        "config": {"workload": "kidney-250k lattice after the timed steps and their cuts: LCS_Finalize's frames + "
+                               "surface rebuild", "voxels": sf["V"], "nodes": sf["N"], "intersections": sf["I"],
+                   "surface_edges": sf["Se"], "triangles": sf["K"] // 3, "wall_triangles": sf["walls"]},
+        "roofline": {"bound": "hbm", "achieved": sach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": sach / HBM_PEAK_GBS, "kernel": "gts_surface_build (its chain of element kernels)",
+                     "avg_rebuild_us": 1e3 * sf["ms"], "algorithmic_bytes_per_rebuild": sf["bytes"]},
+    }
+    if sf["cpu_ms"] is not None:
+        line["surface"]["bit_exact_vs_oracle"] = sf["bit_exact"]
+        line["surface"]["cpu_baseline"] = {"value": sf["V"] / (sf["cpu_ms"] * 1e-3), "unit": "voxels/s", "cores": 1,
+                                           "kind": "port", "sample": "one rebuild of the same state by the oracle"}
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.skip_batched:
         bt = run_batched(args.batched_steps, 2, args.batched_count, tune=not args.no_tune)

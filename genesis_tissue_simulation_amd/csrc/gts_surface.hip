@@ -481,8 +481,11 @@ __global__ void k_write_tris(int n, const int* __restrict__ ok, const int* __res
     out[3 * k + 2] = in[3 * (size_t)i + 2];
 }
 
+// counts[which] = the indices kept (at most capacity triangles); the first compaction also records in counts[2] the
+// indices the candidates asked for, which exceed the capacity when it overflowed.
 __global__ void k_set_count(const int* __restrict__ partial, int nblocks, int capacity, uint32_t* counts, int which) {
     counts[which] = 3u * (uint32_t)min(partial[nblocks], capacity);
+    if (which == 1) counts[2] = 3u * (uint32_t)partial[nblocks];
 }
 
 // ---- cleanInternalFaces (surface_cleanup.cuh:6-53): faces keyed by their three identities without the wall bit;

@@ -17,8 +17,10 @@ What the step does, per reference call:
     the friction contact of the lattice nodes, Coulomb 0.6 / 0.4 (model 2's persistent anchors and embedded grip
     are not part of this path);
   * the knife: a kinematic capsule whose swept surface between consecutive commands cuts the lattice
-    (`LCS_SetTool` + `LCS_DetectCut`, `plugin_api.cpp:145-155`) instead of the force-driven pressure blade.
-Out of this path: the reconstructed surface (`surface()`), scissors and the unified VBD/PCG branch.
+    (`LCS_SetTool` + `LCS_DetectCut`, `plugin_api.cpp:145-155`) instead of the force-driven pressure blade;
+  * `LCS_Finalize`: the particle frames, then the rebuild of the dual-contouring surface with its cut walls
+    (`plugin_api.cpp:161-169`), which `surface()` reads (`LCS_GetSurface*`, `native_backend.py:434-451`).
+Out of this path: scissors and the unified VBD/PCG branch.
 """
 from __future__ import annotations
 
@@ -114,6 +116,16 @@ def _unit(v):
     return v / np.linalg.norm(v)
 
 
+@dataclass
+class Surface:
+    """Field-for-field the reference's reconstructed surface (`native_backend.py:199-204`): a triangle soup of
+    vertices and normals f32[K, 3], triangles i32[K / 3, 3] over them and anchors f32[K, 4] (rest anchor, wall flag)."""
+    vertices: np.ndarray
+    normals: np.ndarray
+    triangles: np.ndarray
+    anchors: np.ndarray
+
+
 class NativeTissue:
     """Owns one tissue on the GPU; like the reference, one session per process (close before constructing another)."""
     active_session = False
@@ -150,6 +162,9 @@ class NativeTissue:
         tet_type = ((topo["tet_desc"] >> 1) & 7)[np.argsort(topo["tet_gen"])]
         k_dev, k_hyd = neohookean_stiffness(s.young_pa * METERS_PER_UNIT, s.poisson, topo["type_volume"], tet_type)
         self.tissue = Tissue(g, self.params, k_dev, k_hyd, g.positions, topology=topo)
+        # LCS_Init's surface capacity and rest build (`native_backend.py:291-296`, dc_recon.cu:555-658)
+        self.tri_capacity = 6 * len(g.surface_edges) + 1024 + 2 * len(g.edges)
+        self.tissue.init_surface(tri_capacity=self.tri_capacity, qef_iters=8)
         self.legacy_grasp = int(s.contact_model) == 0
         if self.legacy_grasp:
             self.tissue.set_contact_model(0, legacy_grasp_params(g.spacing))
@@ -180,7 +195,8 @@ class NativeTissue:
         self._command = command
 
     def _advance(self):
-        """LCS_Step (the step and its frame update), the knife's LCS_DetectCut, then LCS_Finalize's frame update."""
+        """LCS_Step (the step and its frame update), the knife's LCS_DetectCut, then LCS_Finalize: the frame update
+        and the surface rebuild, which fails like the reference's when its triangles overflow the capacity."""
         self.tissue.step(self.settings.dt, tool=self._tool, tool_armed=True)
         nxt = self._blade_next
         if nxt is not None and self._blade is not None:
@@ -190,6 +206,9 @@ class NativeTissue:
                 self.tissue.detect_cut(tool)
         self._blade = nxt
         self.tissue.finalize()
+        self.tissue.rebuild_surface()
+        if self.tissue.surface_overflowed():
+            raise RuntimeError("LCS_Finalize failed with native status -1")
 
     def step(self, command: Command):
         if self.closed:
@@ -259,7 +278,20 @@ class NativeTissue:
                               self.settings.blade_radius) * METERS_PER_UNIT
 
     def surface(self):
-        raise NotImplementedError("the reconstructed dual-contouring surface is outside the substep path")
+        """The surface LCS_Finalize last rebuilt (`native_backend.py:434-451`), with the reference's checks: an empty
+        or ragged soup, the cut point capacity (4 per cut edge) and non-finite values raise RuntimeError."""
+        if self.closed:
+            raise RuntimeError("Native session is closed")
+        s = self.tissue.surface()
+        n = len(s["pos"])
+        if n <= 0 or n % 3:
+            raise RuntimeError("Native surface is empty or has an invalid triangle count")
+        if self.tissue.cut_state()["cut_point_counter"] > 4 * len(self.grid.edges) or \
+                s["asked"] > 3 * self.tri_capacity:
+            raise RuntimeError("Native cut/surface capacity exceeded")
+        if not np.isfinite(s["pos"]).all() or not np.isfinite(s["nrm"]).all():
+            raise RuntimeError("Non-finite reconstructed surface")
+        return Surface(s["pos"], s["nrm"], s["indices"], s["aux"])
 
     def close(self):
         if not self.closed:

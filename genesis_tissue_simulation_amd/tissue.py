@@ -370,10 +370,16 @@ class Tissue:
         check("gts_surface_build", self.lib.gts_surface_build(ct.byref(self.view), ct.byref(self.surface_view), 1, 1,
                                                               current_stream()))
 
+    def surface_overflowed(self):
+        """Whether the last rebuild's triangles exceeded the capacity (cleanInternalFaces then fails,
+        surface_cleanup.cuh:38)."""
+        return int(self.surf["counts"][2].item()) > 3 * self.surface_view.tri_capacity
+
     def surface(self):
         """LCS_GetSurfaceCounts / LCS_GetSurface / LCS_GetSurfaceAux (recon_get_counts / recon_get_surface /
         recon_get_surface_aux, dc_recon.cu:660-712) of the last rebuild: the triangle soup's positions [K, 3],
-        normals [K, 3], indices [K / 3, 3] (0..K-1) and aux [K, 4] (rest anchor, wall flag), plus the vertex tags."""
+        normals [K, 3], indices [K / 3, 3] (0..K-1) and aux [K, 4] (rest anchor, wall flag), plus the vertex tags,
+        the feature points, the index count before the cleanup and the index count the triangles asked for."""
         s = self.surf
         k = int(s["counts"][0].item())
         return dict(pos=s["out_pos"][:3 * k].reshape(-1, 3).cpu().numpy(),
@@ -381,7 +387,8 @@ class Tissue:
                     indices=np.arange(k, dtype=np.int32).reshape(-1, 3),
                     aux=s["out_aux"][:4 * k].reshape(-1, 4).cpu().numpy(),
                     tags=s["tri"][:k].cpu().numpy(), fp=s["fp"].reshape(-1, 4).cpu().numpy(),
-                    cut_fp=s["cut_fp"].reshape(-1, 4).cpu().numpy(), listed=int(s["counts"][1].item()))
+                    cut_fp=s["cut_fp"].reshape(-1, 4).cpu().numpy(), listed=int(s["counts"][1].item()),
+                    asked=int(s["counts"][2].item()))
 
     # ---------------------------------------------------------------- readback
     def positions(self):

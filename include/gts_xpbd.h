@@ -395,7 +395,8 @@ typedef struct GtsSurfaceView {
     int32_t* normal_fixed;          /* [27V] work: fixed-point normal sums, voxels then component slots */
     float* normals;                 /* [27V] work: unit normals, voxels then component slots */
     int32_t* scan;                  /* [ceil(max(2Se + 12N, tri_capacity) / 256) + 1] work */
-    uint32_t* counts;               /* [4] out: 0 triangle indices, 1 indices before the cleanup */
+    uint32_t* counts;               /* [4] out: 0 triangle indices, 1 indices before the cleanup, 2 indices asked for
+                                     * (above 3 * tri_capacity when the triangles overflowed it) */
     float* out_pos;                 /* [9 * tri_capacity] out: the triangle soup, xyz per index */
     float* out_nrm;                 /* [9 * tri_capacity] out */
     float* out_aux;                 /* [12 * tri_capacity] out: rest anchor xyz, wall flag */

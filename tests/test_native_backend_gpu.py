@@ -126,3 +126,31 @@ def test_legacy_contact_model_grasps_like_the_oracle():
             np.testing.assert_array_equal(obs.grip_flags, o.grasp_state()["core"])
             held = max(held, int(obs.grip_flags.sum()))
         assert held >= 1 and obs.grip_flags.sum() == 0
+
+
+def test_surface_after_knife_cuts_matches_oracle():
+    """surface() reads what LCS_Finalize rebuilt: after every step of a knife sweep, the soup equals the oracle's
+    rebuild of the same state bit for bit, with cut walls once the knife is in, and the reference's Surface fields."""
+    from oracle import oracle
+    grid = build_grid(0.6)
+    with NativeTissue(grid) as tissue:
+        t = tissue.tissue
+        rest_fp = t.surf["rest_fp"].reshape(-1, 4).cpu().numpy()
+        walls = 0
+        for k, cmd in enumerate(commands(12)):
+            print(f"drop-in surface: step {k}", flush=True)
+            tissue.step(cmd)
+            surf = tissue.surface()
+            cs = t.cut_state()
+            ora = oracle.surface(grid, t.positions(), t.rotations(), cs["neighbors"], cs["cut_mask"],
+                                 cs["edge_cut_t"], cs["cut_points"], rest_fp, cut_aware=1, have_rest=1,
+                                 tri_capacity=tissue.tri_capacity)
+            np.testing.assert_array_equal(surf.vertices, ora["pos"])
+            np.testing.assert_array_equal(surf.normals, ora["nrm"])
+            np.testing.assert_array_equal(surf.anchors, ora["aux"])
+            assert surf.triangles.shape == (len(ora["pos"]) // 3, 3)
+            np.testing.assert_array_equal(surf.triangles.ravel(), np.arange(len(ora["pos"])))
+            walls = int((surf.anchors[:, 3] > 0).sum())
+        assert walls > 0, "the knife must open cut walls"
+    with pytest.raises(RuntimeError):
+        tissue.surface()
