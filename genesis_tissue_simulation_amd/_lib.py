@@ -106,9 +106,8 @@ class GtsSurfaceView(ct.Structure):
     _fields_ = [("n_voxels", I), ("n_isect", I), ("n_surface_edges", I), ("tri_capacity", I), ("qef_iters", I),
                 ("face_capacity", I)] + [(name, P) for name in (
                     "voxel_corners", "isect_offset", "isect_count", "isect", "surface_edges", "conn", "edge_of",
-                    "rest_fp", "isect_world", "fp", "cut_fp", "edge_points", "candidates", "candidate_ok", "tri",
-                    "tri_work", "face_table", "normal_fixed", "normals", "scan", "counts", "out_pos", "out_nrm",
-                    "out_aux")]
+                    "rest_fp", "isect_world", "fp", "cut_fp", "edge_points", "candidates", "item_mask", "tri",
+                    "tri_work", "face_table", "normal_fixed", "scan", "counts", "out_pos", "out_nrm", "out_aux")]
 
 
 STRUCTS = (GtsLattice, GtsTopology, GtsTissueView, GtsStepParams, GtsToolState, GtsCutTool, GtsBatchInstance,

@@ -58,6 +58,28 @@ __device__ __forceinline__ int block_prefix(bool flag, int* lds, int& total) {
     return waveBase + before;
 }
 
+// Exclusive prefix of `value` over the workgroup (wave scan with lane shuffles, then the wave totals in LDS); total
+// receives the workgroup's sum. Every thread of the workgroup must call it.
+__device__ __forceinline__ int block_scan(int value, int* lds, int& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int x = value;
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    int waveBase = 0;
+    total = 0;
+    for (int k = 0; k < nw; k++) {
+        int c = lds[k];
+        if (k < wave) waveBase += c;
+        total += c;
+    }
+    __syncthreads();
+    return waveBase + x - value;
+}
+
 template <class Flag>
 __global__ void k_count_flags(int n, Flag flag, int* partial) {
     __shared__ int lds[GTS_BLOCK / 64];

@@ -78,13 +78,9 @@ __device__ __forceinline__ int surface_owner(const float* rec, const Grid& G, co
 }
 
 // ---- k_RebuildIsectWorld (dc_recon.cu:97-142): an intact edge lerps its deformed ends; a severed one rebuilds the
-// missing end from the skin owner's frame and the rest edge.
-__global__ void k_isect_world(int I, Grid G, const float* __restrict__ isect, const float4* __restrict__ pos,
-                              const float* __restrict__ rest, const int* __restrict__ nbr, const float* __restrict__ rot,
-                              const float* __restrict__ cutT, float* __restrict__ iw) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= I) return;
-    const float* rec = isect + 10 * (size_t)e;
+// missing end from the skin owner's frame and the rest edge. nbr is null outside the cut-aware rebuild.
+__device__ __forceinline__ f3 isect_world(const float* rec, const Grid& G, const float4* pos, const float* rest,
+                                          const int* nbr, const float* rot, const float* cutT) {
     const int A = ibits(rec[6]), B = ibits(rec[7]);
     const float t = rec[8];
     bool severed = false;
@@ -97,35 +93,42 @@ __global__ void k_isect_world(int I, Grid G, const float* __restrict__ isect, co
         int sgn = (dxi + dyi + dzi) > 0 ? 1 : -1;
         severed = nbr[6 * A + 2 * axis + (sgn > 0 ? 1 : 0)] < 0;
     }
-    f3 w;
-    if (!severed) {
-        w = lerp3(xyz4(pos, A), xyz4(pos, B), t);
-    } else {
-        f3 dir = ld3(rest + 3 * (size_t)B) - ld3(rest + 3 * (size_t)A);
-        if (surface_owner(rec, G, cutT) == A) {
-            f3 pA = xyz4(pos, A);
-            w = lerp3(pA, pA + mul_rv(rot + 9 * (size_t)A, dir), t);
-        } else {
-            f3 pB = xyz4(pos, B);
-            w = lerp3(pB - mul_rv(rot + 9 * (size_t)B, dir), pB, t);
-        }
+    if (!severed) return lerp3(xyz4(pos, A), xyz4(pos, B), t);
+    f3 dir = ld3(rest + 3 * (size_t)B) - ld3(rest + 3 * (size_t)A);
+    if (surface_owner(rec, G, cutT) == A) {
+        f3 pA = xyz4(pos, A);
+        return lerp3(pA, pA + mul_rv(rot + 9 * (size_t)A, dir), t);
     }
-    iw[3 * (size_t)e] = w.x;
-    iw[3 * (size_t)e + 1] = w.y;
-    iw[3 * (size_t)e + 2] = w.z;
+    f3 pB = xyz4(pos, B);
+    return lerp3(pB - mul_rv(rot + 9 * (size_t)B, dir), pB, t);
 }
 
-// ---- k_FeaturePoints (dc_recon.cu:149-207) and k_AdvectIntactSurface (dc_recon.cu:477-493): a voxel whose rest
-// feature point exists and that no cut touches carries it trilinearly with its corners; any other voxel with
-// intersections runs the decaying-step QEF ascent from their mean, stopping before it leaves the corners' box.
-__global__ void k_feature_points(int V, int iters, int cut_aware, int have_rest, const int* __restrict__ corners,
-                                 const int* __restrict__ ioff, const int* __restrict__ icnt,
-                                 const float* __restrict__ isect, const float* __restrict__ iw,
-                                 const float4* __restrict__ pos, const float* __restrict__ rest,
-                                 const float4* __restrict__ rest_fp, const uint32_t* __restrict__ mask,
-                                 float4* __restrict__ fp) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+// ---- One thread per voxel: k_FeaturePoints (dc_recon.cu:149-207) over k_RebuildIsectWorld of the voxel's own
+// intersections (each voxel owns a contiguous range of the list), and k_AdvectIntactSurface (dc_recon.cu:477-493):
+// a voxel whose rest feature point exists and that no cut touches carries it trilinearly with its corners; any other
+// voxel with intersections runs the decaying-step QEF ascent from their mean, stopping before it leaves the corners'
+// box. The voxel's fixed-point normal sum is cleared for the scatter that follows. In the cut-aware rebuild, thread i
+// also enters cut point i in the table of its edge's cut points (slot 2e for the owner at the edge's base node, 2e + 1
+// for the other end); the table persists between rebuilds, as the cut point list only grows.
+__global__ void k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Grid G, const int* __restrict__ corners,
+                             const int* __restrict__ ioff, const int* __restrict__ icnt,
+                             const float* __restrict__ isect, float* __restrict__ iw, const float4* __restrict__ pos,
+                             const float* __restrict__ rest, const int* __restrict__ nbr, const float* __restrict__ rot,
+                             const float* __restrict__ cutT, const float4* __restrict__ rest_fp,
+                             const uint32_t* __restrict__ mask, float4* __restrict__ fp, int* __restrict__ fixed,
+                             int capacity, const uint32_t* __restrict__ counters, const int* __restrict__ gedges,
+                             const float* __restrict__ points, int* __restrict__ edge_points) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (edge_points && i < (int)min(counters[1], (uint32_t)capacity)) {
+        const int owner = ibits(points[8 * (size_t)i + 3]), e = ibits(points[8 * (size_t)i + 4]);
+        const int4 ge = reinterpret_cast<const int4*>(gedges)[e];
+        edge_points[2 * e + (owner == G.corner(ge.y, ge.z, ge.w) ? 0 : 1)] = i;
+    }
+    const int v = i;
     if (v >= V) return;
+    fixed[3 * (size_t)v] = 0;
+    fixed[3 * (size_t)v + 1] = 0;
+    fixed[3 * (size_t)v + 2] = 0;
     const int* cv = corners + 8 * (size_t)v;
     if (have_rest) {
         float4 r4 = rest_fp[v];
@@ -150,7 +153,14 @@ __global__ void k_feature_points(int V, int iters, int cut_aware, int have_rest,
         return;
     }
     f3 x = mk3(0.f, 0.f, 0.f);
-    for (int s = 0; s < cnt; s++) x = x + ld3(iw + 3 * (size_t)(off + s));
+    for (int s = 0; s < cnt; s++) {
+        const f3 w = isect_world(isect + 10 * (size_t)(off + s), G, pos, rest, nbr, rot, cutT);
+        float* o = iw + 3 * (size_t)(off + s);
+        o[0] = w.x;
+        o[1] = w.y;
+        o[2] = w.z;
+        x = x + w;
+    }
     x = div3s(x, (float)cnt);
     f3 lo = xyz4(pos, cv[0]), hi = lo;
     for (int c = 1; c < 8; c++) {
@@ -172,27 +182,11 @@ __global__ void k_feature_points(int V, int iters, int cut_aware, int have_rest,
     fp[v] = make_float4(x.x, x.y, x.z, 1.f);
 }
 
-__global__ void k_fill_ints(long long n, int* __restrict__ a, int value) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = value;
-}
-
-// The cut points of each cut edge: slot 2e for the owner at the edge's base node, 2e + 1 for the other end.
-__global__ void k_edge_points(int capacity, const uint32_t* __restrict__ counters, Grid G,
-                              const int* __restrict__ gedges, const float* __restrict__ points,
-                              int* __restrict__ edge_points) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = (int)min(counters[1], (uint32_t)capacity);
-    if (i >= n) return;
-    const int owner = ibits(points[8 * (size_t)i + 3]), e = ibits(points[8 * (size_t)i + 4]);
-    const int4 ge = reinterpret_cast<const int4*>(gedges)[e];
-    edge_points[2 * e + (owner == G.corner(ge.y, ge.z, ge.w) ? 0 : 1)] = i;
-}
-
 // ---- k_AccumulateCutFP + k_ComputeComponentFP (cut.cu:786-985), one thread per voxel: each component of a cut,
 // occupied voxel gets one feature point from its intersections and its cut points (restored from the owners' frames,
 // summed in cut point index order), refined like the voxel feature point inside the component's box, clamped to it
-// and pulled onto the plane of its own cut points. Every other slot is cleared (k_ClearCutFP, cut.cu:761-772).
+// and pulled onto the plane of its own cut points. Every other slot is cleared (k_ClearCutFP, cut.cu:761-772); the
+// fixed-point normal sums of the voxel's slots are cleared for the scatter.
 __global__ void k_component_points(int V, Grid G, int iters, const int* __restrict__ corners,
                                    const int* __restrict__ ioff, const int* __restrict__ icnt,
                                    const float* __restrict__ isect, const float* __restrict__ iw,
@@ -201,7 +195,8 @@ __global__ void k_component_points(int V, Grid G, int iters, const int* __restri
                                    const uint32_t* __restrict__ occupied, const uint32_t* __restrict__ mask,
                                    const int* __restrict__ conn, const float* __restrict__ cutT,
                                    const int* __restrict__ edge_of, const int* __restrict__ edge_points,
-                                   const float* __restrict__ points, float4* __restrict__ cut_fp) {
+                                   const float* __restrict__ points, float4* __restrict__ cut_fp,
+                                   int* __restrict__ fixed) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     float4* out = cut_fp + 8 * (size_t)v;
@@ -210,6 +205,9 @@ __global__ void k_component_points(int V, Grid G, int iters, const int* __restri
     if (m != 0u && occupied[v] != 0u) compCount = conn[9 * m];
     for (int c = compCount; c < 8; c++) out[c] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (compCount == 0) return;
+    // the normal sums of the slots a triangle can name (a cut, occupied voxel's), cleared for the scatter
+    int* fx = fixed + 3 * ((size_t)V + 8 * (size_t)v);
+    for (int k = 0; k < 24; k++) fx[k] = 0;
     const int* row = conn + 9 * m + 1;
     const int* cv = corners + 8 * (size_t)v;
     const int vxc = v % G.vx, vyc = (v / G.vx) % G.vy, vzc = v / (G.vx * G.vy);
@@ -333,26 +331,26 @@ __device__ __forceinline__ f3 fetch_pos(int raw, const float4* fp, const float4*
     return mk3(q.x, q.y, q.z);
 }
 
-__device__ __forceinline__ void candidate(int slot, int a, int b, int c, const float4* fp, const float4* cut_fp,
-                                          int* cand, int* ok) {
+// A candidate triangle: kept (returns 1, and its tags go to slot) unless its three points are collinear or coincide
+// (k_Stitch / EmitCutQuad, dc_recon.cu:277-285, cut.cu:1013-1015).
+__device__ __forceinline__ int candidate(int slot, int a, int b, int c, const float4* fp, const float4* cut_fp,
+                                         int* cand) {
     f3 p0 = fetch_pos(a, fp, cut_fp), p1 = fetch_pos(b, fp, cut_fp), p2 = fetch_pos(c, fp, cut_fp);
     f3 n = cross3(p1 - p0, p2 - p0);
-    bool keep = !(dot3(n, n) < 1e-16f);
-    ok[slot] = keep ? 1 : 0;
+    if (dot3(n, n) < 1e-16f) return 0;
     cand[3 * (size_t)slot] = a;
     cand[3 * (size_t)slot + 1] = b;
     cand[3 * (size_t)slot + 2] = c;
+    return 1;
 }
 
 // ---- k_Stitch (dc_recon.cu:239-287) with StitchVertId (:213-225): the quad of the four voxels around a surface edge,
 // each voxel resolved to the feature point of the component holding the edge's skin owner when the voxel is cut.
-__global__ void k_stitch(int Se, Grid G, int cut_aware, const int* __restrict__ sedges,
-                         const float* __restrict__ skinT, const float* __restrict__ cutT,
-                         const uint32_t* __restrict__ mask, const uint32_t* __restrict__ occupied,
-                         const int* __restrict__ conn, const float4* __restrict__ fp,
-                         const float4* __restrict__ cut_fp, int* __restrict__ cand, int* __restrict__ ok) {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= Se) return;
+__device__ __forceinline__ int stitch_item(int e, const Grid& G, int cut_aware, const int* __restrict__ sedges,
+                                           const float* __restrict__ skinT, const float* __restrict__ cutT,
+                                           const uint32_t* __restrict__ mask, const uint32_t* __restrict__ occupied,
+                                           const int* __restrict__ conn, const float4* __restrict__ fp,
+                                           const float4* __restrict__ cut_fp, int* __restrict__ cand) {
     const int* ge = sedges + 8 * (size_t)e;
     const int axis = ge[0], bx = ge[1], by = ge[2], bz = ge[3], insideA = ge[4];
     const int ax = axis == 0, ay = axis == 1, az = axis == 2;
@@ -381,22 +379,19 @@ __global__ void k_stitch(int Se, Grid G, int cut_aware, const int* __restrict__ 
         b = d;
         d = t;
     }
-    candidate(2 * e, a, b, c, fp, cut_fp, cand, ok);
-    candidate(2 * e + 1, a, c, d, fp, cut_fp, cand, ok);
+    return candidate(2 * e, a, b, c, fp, cut_fp, cand) | candidate(2 * e + 1, a, c, d, fp, cut_fp, cand) << 1;
 }
 
 // ---- k_BuildCutTriangles with EmitCutQuad (cut.cu:988-1090): around every cut edge of a node that cuts material,
 // the feature points of the node's component in the (up to four) cut voxels form the wall; a fan of at most two
-// triangles, skipping faces made only of points the two ends share. Slots base + 12 node + 2 edge + k.
-__global__ void k_walls(int N, int base, Grid G, const uint32_t* __restrict__ mask,
-                        const uint32_t* __restrict__ occupied, const int* __restrict__ conn,
-                        const float* __restrict__ cutT, const float* __restrict__ skinT,
-                        const int* __restrict__ inside, const float4* __restrict__ fp,
-                        const float4* __restrict__ cut_fp, int* __restrict__ cand, int* __restrict__ ok) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N) return;
-    int* okp = ok + base + 12 * (size_t)p;
-    for (int k = 0; k < 12; k++) okp[k] = 0;
+// triangles, skipping faces made only of points the two ends share. Slots base + 12 node + 2 edge + k; returns the
+// mask of the slots filled.
+__device__ __forceinline__ int wall_item(int p, int base, const Grid& G, const uint32_t* __restrict__ mask,
+                                         const uint32_t* __restrict__ occupied, const int* __restrict__ conn,
+                                         const float* __restrict__ cutT, const float* __restrict__ skinT,
+                                         const int* __restrict__ inside, const float4* __restrict__ fp,
+                                         const float4* __restrict__ cut_fp, int* __restrict__ cand) {
+    int filled = 0;
     int cx, cy, cz;
     G.coord(p, cx, cy, cz);
     for (int e = 0; e < 6; e++) {
@@ -454,38 +449,93 @@ __global__ void k_walls(int N, int base, Grid G, const uint32_t* __restrict__ ma
         const bool flip = !basedHere;
         for (int k = 1; k < mm - 1; k++) {
             if (shared[ord[0]] && shared[ord[k]] && shared[ord[k + 1]]) continue;
-            const int slot = base + 12 * p + 2 * e + (k - 1);
-            candidate(slot, tag[ord[0]], tag[ord[flip ? k + 1 : k]], tag[ord[flip ? k : k + 1]], fp, cut_fp, cand, ok);
+            const int j = 2 * e + (k - 1);
+            filled |= candidate(base + 12 * p + j, tag[ord[0]], tag[ord[flip ? k + 1 : k]], tag[ord[flip ? k : k + 1]],
+                                fp, cut_fp, cand) << j;
         }
+    }
+    return filled;
+}
+
+// Candidates of every item (surface edge e < Se, then node Se + p in the cut-aware rebuild) into their fixed slots,
+// the mask of the filled ones per item, and the count per workgroup.
+__global__ void k_triangles(int Se, int n_items, Grid G, int cut_aware, const int* __restrict__ sedges,
+                            const float* __restrict__ skinT, const float* __restrict__ cutT,
+                            const uint32_t* __restrict__ mask, const uint32_t* __restrict__ occupied,
+                            const int* __restrict__ conn, const int* __restrict__ inside,
+                            const float4* __restrict__ fp, const float4* __restrict__ cut_fp, int* __restrict__ cand,
+                            int* __restrict__ item_mask, int* __restrict__ partial) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int m = 0;
+    if (i < Se) m = stitch_item(i, G, cut_aware, sedges, skinT, cutT, mask, occupied, conn, fp, cut_fp, cand);
+    else if (i < n_items) m = wall_item(i - Se, 2 * Se, G, mask, occupied, conn, cutT, skinT, inside, fp, cut_fp, cand);
+    if (i < n_items) item_mask[i] = m;
+    int total;
+    block_scan(__popc(m), lds, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
+}
+
+// Smallest power of two at least twice the count (at least 1): the face table the cleanup of `count` triangles uses.
+__device__ __forceinline__ int face_slots(int count) {
+    int fc = 1;
+    while (fc < 2 * count) fc <<= 1;
+    return fc;
+}
+
+// k_NormalsScatter (dc_recon.cu:306-331) of one triangle: its area-weighted face normal, in fixed point, to the sums
+// of its three points (voxel points at 3 * voxel, component points at 3 * (V + slot)).
+__device__ __forceinline__ void scatter_normal(const int* t, const float4* fp, const float4* cut_fp, int V,
+                                               int* fixed) {
+    f3 p0 = fetch_pos(t[0], fp, cut_fp), p1 = fetch_pos(t[1], fp, cut_fp), p2 = fetch_pos(t[2], fp, cut_fp);
+    f3 fn = cross3(p1 - p0, p2 - p0);
+    const int q[3] = {(int)(fn.x * kNormalScale), (int)(fn.y * kNormalScale), (int)(fn.z * kNormalScale)};
+    for (int k = 0; k < 3; k++) {
+        uint32_t u = (uint32_t)t[k];
+        size_t idx = u & kIndexMask;
+        int* acc = (u & kCutTag) ? fixed + 3 * ((size_t)V + idx) : fixed + 3 * idx;
+        atomicAdd(acc, q[0]);
+        atomicAdd(acc + 1, q[1]);
+        atomicAdd(acc + 2, q[2]);
     }
 }
 
-struct OkFlag {
-    const int* ok;
-    __device__ bool operator()(int i) const { return ok[i] != 0; }
-};
-
-// Stable compaction of the flagged triangles into out (at most capacity of them).
-__global__ void k_write_tris(int n, const int* __restrict__ ok, const int* __restrict__ partial,
-                             const int* __restrict__ in, int* __restrict__ out, int capacity) {
+// Ordered placement of the filled candidates: item i's triangles, in slot order, from the scanned workgroup counts;
+// those past the capacity are dropped. Thread 0 records the counts (counts[1] listed, counts[2] asked for; counts[0]
+// too when no cleanup follows, in which case the normals are scattered here). With a face table, the threads then
+// clear the entries the cleanup of the listed triangles will use.
+__global__ void k_place(int Se, int n_items, const int* __restrict__ item_mask, const int* __restrict__ partial,
+                        int nblocks, const int* __restrict__ cand, int* __restrict__ out, int capacity,
+                        uint32_t* __restrict__ counts, int final_list, const float4* __restrict__ fp,
+                        const float4* __restrict__ cut_fp, int V, int* __restrict__ fixed,
+                        int4* __restrict__ face_table) {
     __shared__ int lds[GTS_BLOCK / 64];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool f = i < n && ok[i] != 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = i < n_items ? item_mask[i] : 0;
     int total;
-    int r = block_prefix(f, lds, total);
-    if (!f) return;
-    int k = partial[blockIdx.x] + r;
-    if (k >= capacity) return;
-    out[3 * k] = in[3 * (size_t)i];
-    out[3 * k + 1] = in[3 * (size_t)i + 1];
-    out[3 * k + 2] = in[3 * (size_t)i + 2];
-}
-
-// counts[which] = the indices kept (at most capacity triangles); the first compaction also records in counts[2] the
-// indices the candidates asked for, which exceed the capacity when it overflowed.
-__global__ void k_set_count(const int* __restrict__ partial, int nblocks, int capacity, uint32_t* counts, int which) {
-    counts[which] = 3u * (uint32_t)min(partial[nblocks], capacity);
-    if (which == 1) counts[2] = 3u * (uint32_t)partial[nblocks];
+    int k = partial[blockIdx.x] + block_scan(__popc(m), lds, total);
+    const int base = i < Se ? 2 * i : 2 * Se + 12 * (i - Se);
+    for (int j = 0; m >> j; j++) {
+        if (!((m >> j) & 1)) continue;
+        if (k < capacity) {
+            const int* c = cand + 3 * (size_t)(base + j);
+            out[3 * k] = c[0];
+            out[3 * k + 1] = c[1];
+            out[3 * k + 2] = c[2];
+            if (final_list) scatter_normal(c, fp, cut_fp, V, fixed);
+        }
+        k++;
+    }
+    const int asked = partial[nblocks], listed = min(asked, capacity);
+    if (i == 0) {
+        counts[1] = 3u * (uint32_t)listed;
+        counts[2] = 3u * (uint32_t)asked;
+        if (final_list) counts[0] = 3u * (uint32_t)listed;
+    }
+    if (face_table) {
+        const int fc = face_slots(listed);
+        for (int t = i; t < fc; t += gridDim.x * blockDim.x) face_table[t] = make_int4(-1, 0, 0x7fffffff, 0x7fffffff);
+    }
 }
 
 // ---- cleanInternalFaces (surface_cleanup.cuh:6-53): faces keyed by their three identities without the wall bit;
@@ -503,16 +553,14 @@ __device__ __forceinline__ uint32_t face_hash(const uint32_t key[3], int fc) {
     return ((key[0] * 73856093u) ^ (key[1] * 19349663u) ^ (key[2] * 83492791u)) & (uint32_t)(fc - 1);
 }
 
-__global__ void k_face_clear(int fc, int4* __restrict__ table) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < fc) table[i] = make_int4(-1, 0, 0x7fffffff, 0x7fffffff);
-}
-
-// Entry: x = first triangle of the key, y = even minus odd listings, z / w = lowest even / odd listing.
-__global__ void k_face_index(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri, int fc,
+// Entry: x = first triangle of the key, y = even minus odd listings, z / w = lowest even / odd listing. The table
+// holds face_slots(listed) entries, cleared by k_place.
+__global__ void k_face_index(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri,
                              int4* table) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= capacity || 3 * i >= (int)counts[1]) return;
+    const int n = (int)(counts[1] / 3u);
+    if (i >= capacity || i >= n) return;
+    const int fc = face_slots(n);
     uint32_t key[3];
     const bool odd = face_key(tri + 3 * i, key);
     const uint32_t h = face_hash(key, fc);
@@ -533,64 +581,66 @@ __global__ void k_face_index(int capacity, const uint32_t* __restrict__ counts, 
     }
 }
 
-__global__ void k_face_keep(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri, int fc,
-                            const int4* __restrict__ table, int* __restrict__ ok) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= capacity) return;
-    ok[i] = 0;
-    if (3 * i >= (int)counts[1]) return;
-    uint32_t key[3];
-    face_key(tri + 3 * i, key);
-    const uint32_t h = face_hash(key, fc);
-    for (int k = 0; k < fc; k++) {
-        int4 entry = table[(h + (uint32_t)k) & (uint32_t)(fc - 1)];
-        if (entry.x < 0) return;  // not reached: every listed face was indexed
-        uint32_t other[3];
-        face_key(tri + 3 * entry.x, other);
-        if (key[0] != other[0] || key[1] != other[1] || key[2] != other[2]) continue;
-        ok[i] = entry.y != 0 && i == (entry.y > 0 ? entry.z : entry.w);
-        return;
+// Whether listed triangle i survives the cleanup (its key's net count is nonzero and it is the lowest listing of the
+// majority orientation), and the survivors per workgroup.
+__global__ void k_face_keep(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri,
+                            const int4* __restrict__ table, int* __restrict__ keep, int* __restrict__ partial) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (int)(counts[1] / 3u);
+    int kept = 0;
+    if (i < capacity && i < n) {
+        const int fc = face_slots(n);
+        uint32_t key[3];
+        face_key(tri + 3 * i, key);
+        const uint32_t h = face_hash(key, fc);
+        for (int k = 0; k < fc; k++) {
+            int4 entry = table[(h + (uint32_t)k) & (uint32_t)(fc - 1)];
+            if (entry.x < 0) break;  // not reached: every listed face was indexed
+            uint32_t other[3];
+            face_key(tri + 3 * entry.x, other);
+            if (key[0] != other[0] || key[1] != other[1] || key[2] != other[2]) continue;
+            kept = entry.y != 0 && i == (entry.y > 0 ? entry.z : entry.w);
+            break;
+        }
     }
+    if (i < capacity) keep[i] = kept;
+    int total;
+    block_scan(kept, lds, total);
+    if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// ---- k_NormalsScatter (dc_recon.cu:306-331): area-weighted face normals in fixed point to every corner's point.
-__global__ void k_normal_scatter(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri,
-                                 const float4* __restrict__ fp, const float4* __restrict__ cut_fp, int V,
-                                 int* __restrict__ fixed) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= capacity || 3 * t >= (int)counts[0]) return;
-    int raw[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
-    f3 p0 = fetch_pos(raw[0], fp, cut_fp), p1 = fetch_pos(raw[1], fp, cut_fp), p2 = fetch_pos(raw[2], fp, cut_fp);
-    f3 fn = cross3(p1 - p0, p2 - p0);
-    const int q[3] = {(int)(fn.x * kNormalScale), (int)(fn.y * kNormalScale), (int)(fn.z * kNormalScale)};
-    for (int k = 0; k < 3; k++) {
-        uint32_t u = (uint32_t)raw[k];
-        size_t i = u & kIndexMask;
-        int* acc = (u & kCutTag) ? fixed + 3 * ((size_t)V + i) : fixed + 3 * i;
-        atomicAdd(acc, q[0]);
-        atomicAdd(acc + 1, q[1]);
-        atomicAdd(acc + 2, q[2]);
+// The survivors in listing order, each scattering its face normal; thread 0 records their index count.
+__global__ void k_place_kept(int capacity, const int* __restrict__ keep, const int* __restrict__ partial, int nblocks,
+                             const int* __restrict__ listed, int* __restrict__ tri, uint32_t* __restrict__ counts,
+                             const float4* __restrict__ fp, const float4* __restrict__ cut_fp, int V,
+                             int* __restrict__ fixed) {
+    __shared__ int lds[GTS_BLOCK / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = i < capacity ? keep[i] : 0;
+    int total;
+    const int k = partial[blockIdx.x] + block_scan(f, lds, total);
+    if (f) {
+        tri[3 * k] = listed[3 * (size_t)i];
+        tri[3 * k + 1] = listed[3 * (size_t)i + 1];
+        tri[3 * k + 2] = listed[3 * (size_t)i + 2];
+        scatter_normal(listed + 3 * (size_t)i, fp, cut_fp, V, fixed);
     }
+    if (i == 0) counts[0] = 3u * (uint32_t)partial[nblocks];
 }
 
-// k_NormalsNormalize (dc_recon.cu:441-451) and k_NormalizeCutFP (cut.cu:1093-1100).
-__global__ void k_normalize(int n, const int* __restrict__ fixed, float* __restrict__ normals) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int* f = fixed + 3 * (size_t)i;
+// k_NormalsNormalize (dc_recon.cu:441-451) / k_NormalizeCutFP (cut.cu:1093-1100) of one fixed-point sum.
+__device__ __forceinline__ f3 unit_normal(const int* f) {
     f3 v = mk3((float)f[0] / kNormalScale, (float)f[1] / kNormalScale, (float)f[2] / kNormalScale);
     float l = len3(v);
-    f3 u = l > 1e-6f ? div3s(v, l) : mk3(0.f, 0.f, 0.f);
-    normals[3 * (size_t)i] = u.x;
-    normals[3 * (size_t)i + 1] = u.y;
-    normals[3 * (size_t)i + 2] = u.z;
+    return l > 1e-6f ? div3s(v, l) : mk3(0.f, 0.f, 0.f);
 }
 
-// ---- k_ExpandVertices (dc_recon.cu:358-386): the indexed triangles as a soup of (position, normal, rest anchor and
-// wall flag) per index.
+// ---- k_ExpandVertices (dc_recon.cu:358-386): the indexed triangles as a soup of (position, unit normal, rest anchor
+// and wall flag) per index.
 __global__ void k_expand(int n3, const uint32_t* __restrict__ counts, const int* __restrict__ tri, int V,
                          const float4* __restrict__ fp, const float4* __restrict__ cut_fp,
-                         const float* __restrict__ normals, const float4* __restrict__ rest_fp,
+                         const int* __restrict__ fixed, const float4* __restrict__ rest_fp,
                          const float* __restrict__ rest, const int* __restrict__ corners, float* __restrict__ out_pos,
                          float* __restrict__ out_nrm, float* __restrict__ out_aux) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -598,10 +648,10 @@ __global__ void k_expand(int n3, const uint32_t* __restrict__ counts, const int*
     const uint32_t u = (uint32_t)tri[i];
     const size_t idx = u & kIndexMask;
     float4 p, aux;
-    const float* nrm;
+    f3 nrm;
     if (u & kCutTag) {
         p = cut_fp[idx];
-        nrm = normals + 3 * ((size_t)V + idx);
+        nrm = unit_normal(fixed + 3 * ((size_t)V + idx));
         const size_t v = idx >> 3;
         const float4 r4 = rest_fp[v];
         f3 anchor = r4.w > 0.f ? mk3(r4.x, r4.y, r4.z)
@@ -610,16 +660,16 @@ __global__ void k_expand(int n3, const uint32_t* __restrict__ counts, const int*
         aux = make_float4(anchor.x, anchor.y, anchor.z, (float)((u >> 30) & 1u));
     } else {
         p = fp[idx];
-        nrm = normals + 3 * idx;
+        nrm = unit_normal(fixed + 3 * idx);
         const float4 r4 = rest_fp[idx];
         aux = make_float4(r4.x, r4.y, r4.z, 0.f);
     }
     out_pos[3 * (size_t)i] = p.x;
     out_pos[3 * (size_t)i + 1] = p.y;
     out_pos[3 * (size_t)i + 2] = p.z;
-    out_nrm[3 * (size_t)i] = nrm[0];
-    out_nrm[3 * (size_t)i + 1] = nrm[1];
-    out_nrm[3 * (size_t)i + 2] = nrm[2];
+    out_nrm[3 * (size_t)i] = nrm.x;
+    out_nrm[3 * (size_t)i + 1] = nrm.y;
+    out_nrm[3 * (size_t)i + 2] = nrm.z;
     reinterpret_cast<float4*>(out_aux)[i] = aux;
 }
 
@@ -635,77 +685,54 @@ extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s
         return GTS_ERR_BAD_ARG;
     if (!v->pos || !v->rest || !s->voxel_corners || !s->isect_offset || !s->isect_count ||
         (s->n_isect > 0 && (!s->isect || !s->isect_world)) || (s->n_surface_edges > 0 && !s->surface_edges) ||
-        !v->edge_skin_t || !s->rest_fp || !s->fp || !s->candidates || !s->candidate_ok || !s->tri ||
-        !s->normal_fixed || !s->normals || !s->scan || !s->counts || !s->out_pos || !s->out_nrm || !s->out_aux)
+        !v->edge_skin_t || !s->rest_fp || !s->fp || !s->candidates || !s->item_mask || !s->tri ||
+        !s->normal_fixed || !s->scan || !s->counts || !s->out_pos || !s->out_nrm || !s->out_aux)
         return GTS_ERR_BAD_ARG;
     if (cut_aware && (!v->nbr || !v->rot || !v->active || !v->inside || !v->occupied || !v->cut_mask ||
                       !v->edge_cut_t || !v->grid_edges || !v->counters || !s->conn || !s->edge_of ||
                       !s->edge_points || !s->cut_fp || !s->tri_work || !s->face_table ||
                       (v->cut_point_capacity > 0 && !v->cut_points)))
         return GTS_ERR_BAD_ARG;
-    hipStream_t st = (hipStream_t)stream;
     const int N = v->n_nodes, Se = s->n_surface_edges, cap = s->tri_capacity;
+    const long long n_items = (long long)Se + (cut_aware ? N : 0);
+    if (2ll * Se + 12ll * N >= (1ll << 31) / 3 || V >= (1ll << 26)) return GTS_ERR_BAD_ARG;
+    hipStream_t st = (hipStream_t)stream;
     const Grid G{v->vx, v->vy, v->vz, v->vx + 1, v->vy + 1};
     const float4* pos = (const float4*)v->pos;
     const float4* fp = (const float4*)s->fp;
     const float4* cfp = (const float4*)s->cut_fp;
+    const int points = cut_aware ? std::max(0, v->cut_point_capacity) : 0;
 
-    if (s->n_isect > 0)
-        k_isect_world<<<blocks_for(s->n_isect), GTS_BLOCK, 0, st>>>(
-            s->n_isect, G, s->isect, pos, v->rest, cut_aware ? v->nbr : nullptr, v->rot,
-            cut_aware ? v->edge_cut_t : nullptr, s->isect_world);
-    k_feature_points<<<blocks_for(V), GTS_BLOCK, 0, st>>>((int)V, s->qef_iters, cut_aware, have_rest, s->voxel_corners,
-                                                          s->isect_offset, s->isect_count, s->isect, s->isect_world,
-                                                          pos, v->rest, (const float4*)s->rest_fp, v->cut_mask,
-                                                          (float4*)s->fp);
-    if (cut_aware) {
-        k_fill_ints<<<blocks_for(2ll * v->n_grid_edges), GTS_BLOCK, 0, st>>>(2ll * v->n_grid_edges, s->edge_points, -1);
-        if (v->cut_point_capacity > 0)
-            k_edge_points<<<blocks_for(v->cut_point_capacity), GTS_BLOCK, 0, st>>>(
-                v->cut_point_capacity, v->counters, G, v->grid_edges, v->cut_points, s->edge_points);
+    k_voxel_pass<<<blocks_for(std::max<long long>(V, points)), GTS_BLOCK, 0, st>>>(
+        (int)V, s->qef_iters, cut_aware, have_rest, G, s->voxel_corners, s->isect_offset, s->isect_count, s->isect,
+        s->isect_world, pos, v->rest, cut_aware ? v->nbr : nullptr, v->rot, cut_aware ? v->edge_cut_t : nullptr,
+        (const float4*)s->rest_fp, v->cut_mask, (float4*)s->fp, s->normal_fixed, points, v->counters, v->grid_edges,
+        v->cut_points, cut_aware ? s->edge_points : nullptr);
+    if (cut_aware)
         k_component_points<<<blocks_for(V), GTS_BLOCK, 0, st>>>(
             (int)V, G, s->qef_iters, s->voxel_corners, s->isect_offset, s->isect_count, s->isect, s->isect_world, pos,
             v->rest, v->rot, v->active, v->occupied, v->cut_mask, s->conn, v->edge_cut_t, s->edge_of, s->edge_points,
-            v->cut_points, (float4*)s->cut_fp);
-    }
+            v->cut_points, (float4*)s->cut_fp, s->normal_fixed);
 
-    // candidates, then their ordered compaction
-    const long long nc = 2ll * Se + (cut_aware ? 12ll * N : 0ll);
-    if (nc >= (1ll << 31) / 3) return GTS_ERR_BAD_ARG;
-    if (Se > 0)
-        k_stitch<<<blocks_for(Se), GTS_BLOCK, 0, st>>>(Se, G, cut_aware, s->surface_edges, v->edge_skin_t,
-                                                       v->edge_cut_t, v->cut_mask, v->occupied, s->conn, fp, cfp,
-                                                       s->candidates, s->candidate_ok);
-    if (cut_aware)
-        k_walls<<<blocks_for(N), GTS_BLOCK, 0, st>>>(N, 2 * Se, G, v->cut_mask, v->occupied, s->conn, v->edge_cut_t,
-                                                     v->edge_skin_t, v->inside, fp, cfp, s->candidates,
-                                                     s->candidate_ok);
-    const unsigned nbc = blocks_for(nc);
+    // candidates in fixed slots, then their ordered placement
+    const unsigned nb = blocks_for(n_items);
     int* listed = cut_aware ? s->tri_work : s->tri;
-    k_count_flags<<<nbc, GTS_BLOCK, 0, st>>>((int)nc, OkFlag{s->candidate_ok}, s->scan);
-    k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nbc, s->scan);
-    k_write_tris<<<nbc, GTS_BLOCK, 0, st>>>((int)nc, s->candidate_ok, s->scan, s->candidates, listed, cap);
-    k_set_count<<<1, 1, 0, st>>>(s->scan, (int)nbc, cap, s->counts, 1);
-
-    if (cut_aware) {
-        const unsigned nbt = blocks_for(cap);
-        k_face_clear<<<blocks_for(s->face_capacity), GTS_BLOCK, 0, st>>>(s->face_capacity, (int4*)s->face_table);
-        k_face_index<<<nbt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, s->face_capacity, (int4*)s->face_table);
-        k_face_keep<<<nbt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, s->face_capacity, (const int4*)s->face_table,
-                                               s->candidate_ok);
-        k_count_flags<<<nbt, GTS_BLOCK, 0, st>>>(cap, OkFlag{s->candidate_ok}, s->scan);
-        k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nbt, s->scan);
-        k_write_tris<<<nbt, GTS_BLOCK, 0, st>>>(cap, s->candidate_ok, s->scan, listed, s->tri, cap);
-        k_set_count<<<1, 1, 0, st>>>(s->scan, (int)nbt, cap, s->counts, 0);
-    } else {
-        k_set_count<<<1, 1, 0, st>>>(s->scan, (int)nbc, cap, s->counts, 0);
+    k_triangles<<<nb, GTS_BLOCK, 0, st>>>(Se, (int)n_items, G, cut_aware, s->surface_edges, v->edge_skin_t,
+                                          v->edge_cut_t, v->cut_mask, v->occupied, s->conn, v->inside, fp, cfp,
+                                          s->candidates, s->item_mask, s->scan);
+    k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nb, s->scan);
+    k_place<<<nb, GTS_BLOCK, 0, st>>>(Se, (int)n_items, s->item_mask, s->scan, (int)nb, s->candidates, listed, cap,
+                                      s->counts, !cut_aware, fp, cfp, (int)V, s->normal_fixed,
+                                      cut_aware ? (int4*)s->face_table : nullptr);
+    if (cut_aware) {  // the cleanup of internal faces, then the survivors in order
+        const unsigned nt = blocks_for(cap);
+        k_face_index<<<nt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, (int4*)s->face_table);
+        k_face_keep<<<nt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, (const int4*)s->face_table, s->item_mask, s->scan);
+        k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nt, s->scan);
+        k_place_kept<<<nt, GTS_BLOCK, 0, st>>>(cap, s->item_mask, s->scan, (int)nt, listed, s->tri, s->counts, fp, cfp,
+                                               (int)V, s->normal_fixed);
     }
-
-    // normals of the voxel points and of the component points, then the soup
-    k_fill_ints<<<blocks_for(27 * V), GTS_BLOCK, 0, st>>>(27 * V, s->normal_fixed, 0);
-    k_normal_scatter<<<blocks_for(cap), GTS_BLOCK, 0, st>>>(cap, s->counts, s->tri, fp, cfp, (int)V, s->normal_fixed);
-    k_normalize<<<blocks_for(9 * V), GTS_BLOCK, 0, st>>>((int)(9 * V), s->normal_fixed, s->normals);
-    k_expand<<<blocks_for(3ll * cap), GTS_BLOCK, 0, st>>>(3 * cap, s->counts, s->tri, (int)V, fp, cfp, s->normals,
+    k_expand<<<blocks_for(3ll * cap), GTS_BLOCK, 0, st>>>(3 * cap, s->counts, s->tri, (int)V, fp, cfp, s->normal_fixed,
                                                           (const float4*)s->rest_fp, v->rest, s->voxel_corners,
                                                           s->out_pos, s->out_nrm, s->out_aux);
     HIP_TRY(hipGetLastError());

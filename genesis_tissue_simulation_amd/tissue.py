@@ -331,7 +331,7 @@ class Tissue:
         fc = 1
         while fc < 2 * cap:
             fc *= 2
-        nc = 2 * Se + 12 * N
+        nc, items = 2 * Se + 12 * N, max(Se + N, cap)
         i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
         zf = lambda n: torch.zeros(max(int(n), 1), dtype=torch.float32, device=dev)
         zi = lambda n: torch.zeros(max(int(n), 1), dtype=torch.int32, device=dev)
@@ -345,10 +345,10 @@ class Tissue:
             voxel_corners=i32(g.corners.reshape(-1)), isect_offset=i32(g.offsets), isect_count=i32(g.counts),
             isect=torch.as_tensor(isect.copy()).to(dev), surface_edges=i32(g.surface_edges.reshape(-1) if Se else [0]),
             conn=i32(g.connectivity.reshape(-1)), edge_of=i32(edge_of), rest_fp=zf(4 * V),
-            isect_world=zf(3 * I), fp=zf(4 * V), cut_fp=zf(32 * V), edge_points=zi(2 * self.Eg),
-            candidates=zi(3 * nc), candidate_ok=zi(max(nc, cap)), tri=zi(3 * cap), tri_work=zi(3 * cap),
-            face_table=zi(4 * fc), normal_fixed=zi(27 * V), normals=zf(27 * V), scan=zi(-(-max(nc, cap) // 256) + 1),
-            counts=zi(4), out_pos=zf(9 * cap), out_nrm=zf(9 * cap), out_aux=zf(12 * cap))
+            isect_world=zf(3 * I), fp=zf(4 * V), cut_fp=zf(32 * V), edge_points=zi(2 * self.Eg).fill_(-1),
+            candidates=zi(3 * nc), item_mask=zi(items), tri=zi(3 * cap), tri_work=zi(3 * cap), face_table=zi(4 * fc),
+            normal_fixed=zi(27 * V), scan=zi(-(-items // 256) + 1), counts=zi(4), out_pos=zf(9 * cap),
+            out_nrm=zf(9 * cap), out_aux=zf(12 * cap))
         names = [f[0] for f in _lib.GtsSurfaceView._fields_[6:]]
         self.surface_view = _lib.GtsSurfaceView(V, I, Se, cap, int(qef_iters), fc,
                                                 *[self.surf[n].data_ptr() for n in names])

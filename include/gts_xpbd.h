@@ -386,15 +386,15 @@ typedef struct GtsSurfaceView {
     float* isect_world;             /* [3I] work: deformed intersections */
     float* fp;                      /* [4V] out: voxel feature points, w = 1 where the voxel has one */
     float* cut_fp;                  /* [32V] out: component feature points, slot 8 * voxel + component */
-    int32_t* edge_points;           /* [2Eg] work: the cut points of each cut edge, owner A then B */
-    int32_t* candidates;            /* [3 * (2Se + 12N)] work: triangles before compaction */
-    int32_t* candidate_ok;          /* [max(2Se + 12N, tri_capacity)] work */
+    int32_t* edge_points;           /* [2Eg] state: the cut points of each cut edge, owner A then B; fill with -1
+                                     * before the first cut-aware rebuild, later rebuilds extend it */
+    int32_t* candidates;            /* [3 * (2Se + 12N)] work: triangles in fixed slots before their placement */
+    int32_t* item_mask;             /* [max(Se + N, tri_capacity)] work */
     int32_t* tri;                   /* [3 * tri_capacity] out: triangle vertex tags (bit 31 component point, bit 30 wall) */
     int32_t* tri_work;              /* [3 * tri_capacity] work */
     int32_t* face_table;            /* [4 * face_capacity] work */
     int32_t* normal_fixed;          /* [27V] work: fixed-point normal sums, voxels then component slots */
-    float* normals;                 /* [27V] work: unit normals, voxels then component slots */
-    int32_t* scan;                  /* [ceil(max(2Se + 12N, tri_capacity) / 256) + 1] work */
+    int32_t* scan;                  /* [ceil(max(Se + N, tri_capacity) / 256) + 1] work */
     uint32_t* counts;               /* [4] out: 0 triangle indices, 1 indices before the cleanup, 2 indices asked for
                                      * (above 3 * tri_capacity when the triangles overflowed it) */
     float* out_pos;                 /* [9 * tri_capacity] out: the triangle soup, xyz per index */
