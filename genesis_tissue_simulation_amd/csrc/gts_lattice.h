@@ -89,9 +89,10 @@ __global__ void k_count_flags(int n, Flag flag, int* partial) {
     if (threadIdx.x == 0) partial[blockIdx.x] = total;
 }
 
-// Exclusive scan of the per-block totals in one workgroup; partial[nblocks] receives the grand total.
-__global__ void k_scan_partials(int nblocks, int* partial) {
-    __shared__ int lds[GTS_BLOCK / 64];
+// Exclusive scan of the per-block totals in one workgroup (up to 1024 threads); partial[nblocks] receives the grand
+// total.
+__global__ void __launch_bounds__(1024) k_scan_partials(int nblocks, int* partial) {
+    __shared__ int lds[16];
     int carry = 0;
     for (int start = 0; start < nblocks; start += blockDim.x) {
         int i = start + threadIdx.x;

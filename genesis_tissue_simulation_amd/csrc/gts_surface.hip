@@ -107,10 +107,10 @@ __device__ __forceinline__ f3 isect_world(const float* rec, const Grid& G, const
 // intersections (each voxel owns a contiguous range of the list), and k_AdvectIntactSurface (dc_recon.cu:477-493):
 // a voxel whose rest feature point exists and that no cut touches carries it trilinearly with its corners; any other
 // voxel with intersections runs the decaying-step QEF ascent from their mean, stopping before it leaves the corners'
-// box. The voxel's fixed-point normal sum is cleared for the scatter that follows. In the cut-aware rebuild, thread i
-// also enters cut point i in the table of its edge's cut points (slot 2e for the owner at the edge's base node, 2e + 1
-// for the other end); the table persists between rebuilds, as the cut point list only grows.
-__global__ void k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Grid G, const int* __restrict__ corners,
+// box. The voxel's fixed-point normal sum is cleared for the scatter that follows. In the cut-aware rebuild, the
+// threads also enter every cut point in the table of its edge's cut points (slot 2e for the owner at the edge's base
+// node, 2e + 1 for the other end); the table persists between rebuilds, as the cut point list only grows.
+__global__ void __launch_bounds__(GTS_BLOCK) k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Grid G, const int* __restrict__ corners,
                              const int* __restrict__ ioff, const int* __restrict__ icnt,
                              const float* __restrict__ isect, float* __restrict__ iw, const float4* __restrict__ pos,
                              const float* __restrict__ rest, const int* __restrict__ nbr, const float* __restrict__ rot,
@@ -118,13 +118,15 @@ __global__ void k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Gri
                              const uint32_t* __restrict__ mask, float4* __restrict__ fp, int* __restrict__ fixed,
                              int capacity, const uint32_t* __restrict__ counters, const int* __restrict__ gedges,
                              const float* __restrict__ points, int* __restrict__ edge_points) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (edge_points && i < (int)min(counters[1], (uint32_t)capacity)) {
-        const int owner = ibits(points[8 * (size_t)i + 3]), e = ibits(points[8 * (size_t)i + 4]);
-        const int4 ge = reinterpret_cast<const int4*>(gedges)[e];
-        edge_points[2 * e + (owner == G.corner(ge.y, ge.z, ge.w) ? 0 : 1)] = i;
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (edge_points) {
+        const int n = (int)min(counters[1], (uint32_t)capacity);
+        for (int i = v; i < n; i += gridDim.x * blockDim.x) {
+            const int owner = ibits(points[8 * (size_t)i + 3]), e = ibits(points[8 * (size_t)i + 4]);
+            const int4 ge = reinterpret_cast<const int4*>(gedges)[e];
+            edge_points[2 * e + (owner == G.corner(ge.y, ge.z, ge.w) ? 0 : 1)] = i;
+        }
     }
-    const int v = i;
     if (v >= V) return;
     fixed[3 * (size_t)v] = 0;
     fixed[3 * (size_t)v + 1] = 0;
@@ -152,15 +154,23 @@ __global__ void k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Gri
         fp[v] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    f3 x = mk3(0.f, 0.f, 0.f);
-    for (int s = 0; s < cnt; s++) {
-        const f3 w = isect_world(isect + 10 * (size_t)(off + s), G, pos, rest, nbr, rot, cutT);
+    // the voxel's intersections (at most 12, one per edge) in registers: deformed point and rest normal
+    f3 w[12], nn[12];
+#pragma unroll
+    for (int s = 0; s < 12; s++) {
+        if (s >= cnt) continue;
+        const float* rec = isect + 10 * (size_t)(off + s);
+        w[s] = isect_world(rec, G, pos, rest, nbr, rot, cutT);
+        nn[s] = ld3(rec + 3);
         float* o = iw + 3 * (size_t)(off + s);
-        o[0] = w.x;
-        o[1] = w.y;
-        o[2] = w.z;
-        x = x + w;
+        o[0] = w[s].x;
+        o[1] = w[s].y;
+        o[2] = w[s].z;
     }
+    f3 x = mk3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int s = 0; s < 12; s++)
+        if (s < cnt) x = x + w[s];
     x = div3s(x, (float)cnt);
     f3 lo = xyz4(pos, cv[0]), hi = lo;
     for (int c = 1; c < 8; c++) {
@@ -170,10 +180,9 @@ __global__ void k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Gri
     }
     for (int it = 0; it < iters; it++) {
         f3 F = mk3(0.f, 0.f, 0.f);
-        for (int s = 0; s < cnt; s++) {
-            f3 n = ld3(isect + 10 * (size_t)(off + s) + 3);
-            F = F + n * dot3(n, ld3(iw + 3 * (size_t)(off + s)) - x);
-        }
+#pragma unroll
+        for (int s = 0; s < 12; s++)
+            if (s < cnt) F = F + nn[s] * dot3(nn[s], w[s] - x);
         float a = 0.1f * (1.0f - (float)it / (float)iters);
         f3 xn = x + F * a;
         if (xn.x <= lo.x || xn.y <= lo.y || xn.z <= lo.z || xn.x >= hi.x || xn.y >= hi.y || xn.z >= hi.z) break;
@@ -182,12 +191,34 @@ __global__ void k_voxel_pass(int V, int iters, int cut_aware, int have_rest, Gri
     fp[v] = make_float4(x.x, x.y, x.z, 1.f);
 }
 
+// Ascending sort of 32 ints held in registers (a bitonic network: every index is a compile-time constant).
+__device__ __forceinline__ void sort32(int v[32]) {
+#pragma unroll
+    for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+            for (int i = 0; i < 32; i++) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const int a = v[i], b = v[l];
+                    const bool up = (i & k) == 0;
+                    v[i] = up ? min(a, b) : max(a, b);
+                    v[l] = up ? max(a, b) : min(a, b);
+                }
+            }
+        }
+    }
+}
+
 // ---- k_AccumulateCutFP + k_ComputeComponentFP (cut.cu:786-985), one thread per voxel: each component of a cut,
 // occupied voxel gets one feature point from its intersections and its cut points (restored from the owners' frames,
 // summed in cut point index order), refined like the voxel feature point inside the component's box, clamped to it
 // and pulled onto the plane of its own cut points. Every other slot is cleared (k_ClearCutFP, cut.cu:761-772); the
-// fixed-point normal sums of the voxel's slots are cleared for the scatter.
-__global__ void k_component_points(int V, Grid G, int iters, const int* __restrict__ corners,
+// fixed-point normal sums of the voxel's slots are cleared for the scatter. The voxel's intersections (at most 12, one
+// per edge) and cut points (at most 24, two per edge) are loaded into registers once, in independent batches, so the
+// ascent and the sums run without memory round trips.
+__global__ void __launch_bounds__(GTS_BLOCK) k_component_points(int V, Grid G, int iters, const int* __restrict__ corners,
                                    const int* __restrict__ ioff, const int* __restrict__ icnt,
                                    const float* __restrict__ isect, const float* __restrict__ iw,
                                    const float4* __restrict__ pos, const float* __restrict__ rest,
@@ -209,38 +240,69 @@ __global__ void k_component_points(int V, Grid G, int iters, const int* __restri
     int* fx = fixed + 3 * ((size_t)V + 8 * (size_t)v);
     for (int k = 0; k < 24; k++) fx[k] = 0;
     const int* row = conn + 9 * m + 1;
-    const int* cv = corners + 8 * (size_t)v;
+    int cv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cv[k] = corners[8 * (size_t)v + k];
     const int vxc = v % G.vx, vyc = (v / G.vx) % G.vy, vzc = v / (G.vx * G.vy);
 
-    // The voxel's cut points, ascending: the twelve edges of the voxel, each with at most two points.
+    // The voxel's cut points, ascending: two table slots per edge of the voxel.
     const int8_t ec[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
                               {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-    int list[24];
-    int n = 0;
+    int eid[12];
+#pragma unroll
     for (int l = 0; l < 12; l++) {
-        int p0 = cv[ec[l][0]], p1 = cv[ec[l][1]];
-        int lo = min(p0, p1), d = max(p0, p1) - lo;
-        int axis = d == 1 ? 0 : (d == G.nx ? 1 : 2);
-        int e = edge_of[3 * lo + axis];
-        if (e < 0) continue;
-        for (int side = 0; side < 2; side++) {
-            int i = edge_points[2 * e + side];
-            if (i < 0) continue;
-            int k = n++;
-            while (k > 0 && list[k - 1] > i) {
-                list[k] = list[k - 1];
-                k--;
-            }
-            list[k] = i;
-        }
+        const int p0 = cv[ec[l][0]], p1 = cv[ec[l][1]];
+        const int lo = min(p0, p1), d = max(p0, p1) - lo;
+        eid[l] = edge_of[3 * lo + (d == 1 ? 0 : (d == G.nx ? 1 : 2))];
+    }
+    int list[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) list[k] = 0x7fffffff;
+#pragma unroll
+    for (int l = 0; l < 12; l++) {
+        if (eid[l] < 0) continue;
+        const int a = edge_points[2 * eid[l]], b = edge_points[2 * eid[l] + 1];
+        if (a >= 0) list[2 * l] = a;
+        if (b >= 0) list[2 * l + 1] = b;
+    }
+    sort32(list);
+    // owner, then its component in this voxel (-1: not counted), for each listed cut point
+    int own[24], cpc[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) own[k] = list[k] != 0x7fffffff ? ibits(points[8 * (size_t)list[k] + 3]) : -1;
+#pragma unroll
+    for (int k = 0; k < 24; k++) {
+        cpc[k] = -1;
+        if (own[k] < 0 || active[own[k]] == 0) continue;
+        int ox, oy, oz;
+        G.coord(own[k], ox, oy, oz);
+        const int lc = local_corner_of(ox, oy, oz, vxc, vyc, vzc);
+        if (lc >= 0) cpc[k] = row[lc];
+    }
+
+    // the intersections: deformed point, rest normal and the component of their skin owner (-1: none)
+    const int off = ioff[v], cnt = icnt[v];
+    f3 w[12], nn[12];
+    int ic[12];
+#pragma unroll
+    for (int s = 0; s < 12; s++) {
+        ic[s] = -1;
+        if (s >= cnt) continue;
+        const float* rec = isect + 10 * (size_t)(off + s);
+        w[s] = ld3(iw + 3 * (size_t)(off + s));
+        nn[s] = ld3(rec + 3);
+        int ix, iy, iz;
+        G.coord(surface_owner(rec, G, cutT), ix, iy, iz);
+        const int lc = local_corner_of(ix, iy, iz, vxc, vyc, vzc);
+        if (lc >= 0) ic[s] = row[lc];
     }
 
     const f3 ext = ld3(rest + 3 * (size_t)cv[6]) - ld3(rest + 3 * (size_t)cv[0]);
     const f3 half = mk3(fabsf(ext.x) * .5f, fabsf(ext.y) * .5f, fabsf(ext.z) * .5f);
-    const int off = ioff[v], cnt = icnt[v];
     for (int c = 0; c < compCount; c++) {
         f3 bc = mk3(0.f, 0.f, 0.f), origin = mk3(0.f, 0.f, 0.f);
         int nCorn = 0;
+#pragma unroll
         for (int lc = 0; lc < 8; lc++) {
             if (row[lc] != c) continue;
             f3 p = xyz4(pos, cv[lc]);
@@ -253,17 +315,13 @@ __global__ void k_component_points(int V, Grid G, int iters, const int* __restri
         f3 accPos = mk3(0.f, 0.f, 0.f);
         int ccnt = 0;
         uint32_t nacc[3] = {0u, 0u, 0u};
-        for (int k = 0; k < n; k++) {
+#pragma unroll
+        for (int k = 0; k < 24; k++) {
+            if (cpc[k] != c) continue;
             const float* cp = points + 8 * (size_t)list[k];
-            const int owner = ibits(cp[3]);
-            if (active[owner] == 0) continue;
-            int ox, oy, oz;
-            G.coord(owner, ox, oy, oz);
-            int lc = local_corner_of(ox, oy, oz, vxc, vyc, vzc);
-            if (lc < 0 || row[lc] != c) continue;
-            const float* R = rot + 9 * (size_t)owner;
+            const float* R = rot + 9 * (size_t)own[k];
             f3 nrmW = mul_rv(R, ld3(cp + 5));
-            f3 local = (xyz4(pos, owner) - origin) + mul_rv(R, ld3(cp));
+            f3 local = (xyz4(pos, own[k]) - origin) + mul_rv(R, ld3(cp));
             accPos = accPos + local;
             ccnt++;
             nacc[0] += (uint32_t)(int)rintf(nrmW.x * kPosScale);
@@ -273,13 +331,10 @@ __global__ void k_component_points(int V, Grid G, int iters, const int* __restri
 
         f3 sum = mk3(0.f, 0.f, 0.f);
         int nIncl = 0;
-        for (int s = 0; s < cnt; s++) {
-            const float* rec = isect + 10 * (size_t)(off + s);
-            int ic = surface_owner(rec, G, cutT), ix, iy, iz;
-            G.coord(ic, ix, iy, iz);
-            int lc = local_corner_of(ix, iy, iz, vxc, vyc, vzc);
-            if (lc < 0 || row[lc] != c) continue;
-            sum = sum + ld3(iw + 3 * (size_t)(off + s));
+#pragma unroll
+        for (int s = 0; s < 12; s++) {
+            if (ic[s] != c) continue;
+            sum = sum + w[s];
             nIncl++;
         }
         if (ccnt > 0) sum = sum + accPos + origin * (float)ccnt;
@@ -292,15 +347,9 @@ __global__ void k_component_points(int V, Grid G, int iters, const int* __restri
         if (nIncl > 0) {
             for (int it = 0; it < iters; it++) {
                 f3 F = mk3(0.f, 0.f, 0.f);
-                for (int s = 0; s < cnt; s++) {
-                    const float* rec = isect + 10 * (size_t)(off + s);
-                    int ic = surface_owner(rec, G, cutT), ix, iy, iz;
-                    G.coord(ic, ix, iy, iz);
-                    int lc = local_corner_of(ix, iy, iz, vxc, vyc, vzc);
-                    if (lc < 0 || row[lc] != c) continue;
-                    f3 nn = ld3(rec + 3);
-                    F = F + nn * dot3(nn, ld3(iw + 3 * (size_t)(off + s)) - x);
-                }
+#pragma unroll
+                for (int s = 0; s < 12; s++)
+                    if (ic[s] == c) F = F + nn[s] * dot3(nn[s], w[s] - x);
                 float a = 0.1f * (1.0f - (float)it / (float)iters);
                 f3 xn = x + F * a;
                 if (fabsf(xn.x - bc.x) >= half.x || fabsf(xn.y - bc.y) >= half.y || fabsf(xn.z - bc.z) >= half.z) break;
@@ -384,17 +433,19 @@ __device__ __forceinline__ int stitch_item(int e, const Grid& G, int cut_aware, 
 
 // ---- k_BuildCutTriangles with EmitCutQuad (cut.cu:988-1090): around every cut edge of a node that cuts material,
 // the feature points of the node's component in the (up to four) cut voxels form the wall; a fan of at most two
-// triangles, skipping faces made only of points the two ends share. Slots base + 12 node + 2 edge + k; returns the
-// mask of the slots filled.
-__device__ __forceinline__ int wall_item(int p, int base, const Grid& G, const uint32_t* __restrict__ mask,
+// triangles, skipping faces made only of points the two ends share. Item j is edge j % 6 (-x, +x, -y, +y, -z, +z of
+// the node's links, as the reference's loop visits them) of node j / 6; its slots are base + 2j + k; returns the mask
+// of the slots filled.
+__device__ __forceinline__ int wall_item(int j, int base, const Grid& G, const uint32_t* __restrict__ mask,
                                          const uint32_t* __restrict__ occupied, const int* __restrict__ conn,
                                          const float* __restrict__ cutT, const float* __restrict__ skinT,
                                          const int* __restrict__ inside, const float4* __restrict__ fp,
                                          const float4* __restrict__ cut_fp, int* __restrict__ cand) {
     int filled = 0;
+    const int p = j / 6, e = j - 6 * p;
     int cx, cy, cz;
     G.coord(p, cx, cy, cz);
-    for (int e = 0; e < 6; e++) {
+    {
         const int axis = e >> 1;
         const int ax = axis == 0, ay = axis == 1, az = axis == 2;
         const bool basedHere = (e & 1) == 0;
@@ -411,11 +462,11 @@ __device__ __forceinline__ int wall_item(int p, int base, const Grid& G, const u
             vids[s] = G.voxel(bx + dx, by + dy, bz + dz);
             if ((mask[vids[s]] >> local) & 1u) isCut = true;
         }
-        if (!valid || !isCut) continue;
+        if (!valid || !isCut) return filled;
         const int qx = basedHere ? cx + ax : cx - ax, qy = basedHere ? cy + ay : cy - ay,
                   qz = basedHere ? cz + az : cz - az;
         const int other = G.corner(qx, qy, qz), es = G.slot(p, other);
-        if (!cut_hits_material(cutT[es], skinT[es], inside[p], inside[other])) continue;
+        if (!cut_hits_material(cutT[es], skinT[es], inside[p], inside[other])) return filled;
         int tag[4] = {0, 0, 0, 0}, n = 0;
         bool okq[4] = {false, false, false, false}, shared[4] = {false, false, false, false};
         for (int s = 0; s < 4; s++) {
@@ -435,7 +486,7 @@ __device__ __forceinline__ int wall_item(int p, int base, const Grid& G, const u
             okq[s] = true;
             n++;
         }
-        if (n < 3) continue;
+        if (n < 3) return filled;
         int ord[4], mm = 0;
         for (int s = 0; s < 4; s++)
             if (okq[s]) ord[mm++] = s;
@@ -449,16 +500,15 @@ __device__ __forceinline__ int wall_item(int p, int base, const Grid& G, const u
         const bool flip = !basedHere;
         for (int k = 1; k < mm - 1; k++) {
             if (shared[ord[0]] && shared[ord[k]] && shared[ord[k + 1]]) continue;
-            const int j = 2 * e + (k - 1);
-            filled |= candidate(base + 12 * p + j, tag[ord[0]], tag[ord[flip ? k + 1 : k]], tag[ord[flip ? k : k + 1]],
-                                fp, cut_fp, cand) << j;
+            filled |= candidate(base + 2 * j + (k - 1), tag[ord[0]], tag[ord[flip ? k + 1 : k]],
+                                tag[ord[flip ? k : k + 1]], fp, cut_fp, cand) << (k - 1);
         }
     }
     return filled;
 }
 
-// Candidates of every item (surface edge e < Se, then node Se + p in the cut-aware rebuild) into their fixed slots,
-// the mask of the filled ones per item, and the count per workgroup.
+// Candidates of every item (surface edge e < Se, then node edge Se + 6p + e in the cut-aware rebuild) into their two
+// fixed slots 2 * item + k, the mask of the filled ones per item, and the count per workgroup.
 __global__ void k_triangles(int Se, int n_items, Grid G, int cut_aware, const int* __restrict__ sedges,
                             const float* __restrict__ skinT, const float* __restrict__ cutT,
                             const uint32_t* __restrict__ mask, const uint32_t* __restrict__ occupied,
@@ -514,7 +564,7 @@ __global__ void k_place(int Se, int n_items, const int* __restrict__ item_mask, 
     const int m = i < n_items ? item_mask[i] : 0;
     int total;
     int k = partial[blockIdx.x] + block_scan(__popc(m), lds, total);
-    const int base = i < Se ? 2 * i : 2 * Se + 12 * (i - Se);
+    const int base = 2 * i;
     for (int j = 0; m >> j; j++) {
         if (!((m >> j) & 1)) continue;
         if (k < capacity) {
@@ -553,13 +603,21 @@ __device__ __forceinline__ uint32_t face_hash(const uint32_t key[3], int fc) {
     return ((key[0] * 73856093u) ^ (key[1] * 19349663u) ^ (key[2] * 83492791u)) & (uint32_t)(fc - 1);
 }
 
+// Only triangles with a component point (bit 31) can share a key. A triangle of three voxel points comes from one
+// stitch quad, the four voxels around its surface edge; two distinct surface edges share at most two of those voxels,
+// so no two such triangles have one key, and their keys never equal one that holds a component point. Each of them
+// is therefore its own key's lowest listing with a net count of one, and is kept without being indexed.
+__device__ __forceinline__ bool names_component(const int* t) {
+    return (((uint32_t)t[0] | (uint32_t)t[1] | (uint32_t)t[2]) & kCutTag) != 0u;
+}
+
 // Entry: x = first triangle of the key, y = even minus odd listings, z / w = lowest even / odd listing. The table
 // holds face_slots(listed) entries, cleared by k_place.
 __global__ void k_face_index(int capacity, const uint32_t* __restrict__ counts, const int* __restrict__ tri,
                              int4* table) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = (int)(counts[1] / 3u);
-    if (i >= capacity || i >= n) return;
+    if (i >= capacity || i >= n || !names_component(tri + 3 * i)) return;
     const int fc = face_slots(n);
     uint32_t key[3];
     const bool odd = face_key(tri + 3 * i, key);
@@ -589,7 +647,9 @@ __global__ void k_face_keep(int capacity, const uint32_t* __restrict__ counts, c
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = (int)(counts[1] / 3u);
     int kept = 0;
-    if (i < capacity && i < n) {
+    if (i < capacity && i < n && !names_component(tri + 3 * i)) {
+        kept = 1;
+    } else if (i < capacity && i < n) {
         const int fc = face_slots(n);
         uint32_t key[3];
         face_key(tri + 3 * i, key);
@@ -694,7 +754,7 @@ extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s
                       (v->cut_point_capacity > 0 && !v->cut_points)))
         return GTS_ERR_BAD_ARG;
     const int N = v->n_nodes, Se = s->n_surface_edges, cap = s->tri_capacity;
-    const long long n_items = (long long)Se + (cut_aware ? N : 0);
+    const long long n_items = (long long)Se + (cut_aware ? 6ll * N : 0ll);
     if (2ll * Se + 12ll * N >= (1ll << 31) / 3 || V >= (1ll << 26)) return GTS_ERR_BAD_ARG;
     hipStream_t st = (hipStream_t)stream;
     const Grid G{v->vx, v->vy, v->vz, v->vx + 1, v->vy + 1};
@@ -703,7 +763,7 @@ extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s
     const float4* cfp = (const float4*)s->cut_fp;
     const int points = cut_aware ? std::max(0, v->cut_point_capacity) : 0;
 
-    k_voxel_pass<<<blocks_for(std::max<long long>(V, points)), GTS_BLOCK, 0, st>>>(
+    k_voxel_pass<<<blocks_for(V), GTS_BLOCK, 0, st>>>(
         (int)V, s->qef_iters, cut_aware, have_rest, G, s->voxel_corners, s->isect_offset, s->isect_count, s->isect,
         s->isect_world, pos, v->rest, cut_aware ? v->nbr : nullptr, v->rot, cut_aware ? v->edge_cut_t : nullptr,
         (const float4*)s->rest_fp, v->cut_mask, (float4*)s->fp, s->normal_fixed, points, v->counters, v->grid_edges,
@@ -720,7 +780,7 @@ extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s
     k_triangles<<<nb, GTS_BLOCK, 0, st>>>(Se, (int)n_items, G, cut_aware, s->surface_edges, v->edge_skin_t,
                                           v->edge_cut_t, v->cut_mask, v->occupied, s->conn, v->inside, fp, cfp,
                                           s->candidates, s->item_mask, s->scan);
-    k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nb, s->scan);
+    k_scan_partials<<<1, 1024, 0, st>>>((int)nb, s->scan);
     k_place<<<nb, GTS_BLOCK, 0, st>>>(Se, (int)n_items, s->item_mask, s->scan, (int)nb, s->candidates, listed, cap,
                                       s->counts, !cut_aware, fp, cfp, (int)V, s->normal_fixed,
                                       cut_aware ? (int4*)s->face_table : nullptr);
@@ -728,7 +788,7 @@ extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s
         const unsigned nt = blocks_for(cap);
         k_face_index<<<nt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, (int4*)s->face_table);
         k_face_keep<<<nt, GTS_BLOCK, 0, st>>>(cap, s->counts, listed, (const int4*)s->face_table, s->item_mask, s->scan);
-        k_scan_partials<<<1, GTS_BLOCK, 0, st>>>((int)nt, s->scan);
+        k_scan_partials<<<1, 1024, 0, st>>>((int)nt, s->scan);
         k_place_kept<<<nt, GTS_BLOCK, 0, st>>>(cap, s->item_mask, s->scan, (int)nt, listed, s->tri, s->counts, fp, cfp,
                                                (int)V, s->normal_fixed);
     }

@@ -327,11 +327,13 @@ class Tissue:
         that later rebuilds advect. `tri_capacity` defaults to NativeTissue's (native_backend.py:291)."""
         g, dev, N, V = self.grid, self.device, self.N, self.V
         Se, I = len(g.surface_edges), len(g.intersections)
+        if V and int(np.max(g.counts)) > 12:
+            raise ValueError("a voxel lists at most one surface crossing per edge (12)")
         cap = int(tri_capacity or 6 * Se + 1024 + 2 * self.Eg)
         fc = 1
         while fc < 2 * cap:
             fc *= 2
-        nc, items = 2 * Se + 12 * N, max(Se + N, cap)
+        nc, items = 2 * Se + 12 * N, max(Se + 6 * N, cap)
         i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32)).to(dev)
         zf = lambda n: torch.zeros(max(int(n), 1), dtype=torch.float32, device=dev)
         zi = lambda n: torch.zeros(max(int(n), 1), dtype=torch.int32, device=dev)

@@ -377,7 +377,7 @@ typedef struct GtsSurfaceView {
     int face_capacity;              /* entries of face_table: a power of two >= 2 * tri_capacity */
     const int32_t* voxel_corners;   /* [8V] node of each voxel corner, figure order (cut.cuh:90-99) */
     const int32_t* isect_offset;    /* [V] first intersection of the voxel */
-    const int32_t* isect_count;     /* [V] */
+    const int32_t* isect_count;     /* [V] at most 12: one crossing per voxel edge */
     const float* isect;             /* [10I] IsectGpu records (common.cuh:23-33): rest xyz, normal xyz, a, b, t, insideA */
     const int32_t* surface_edges;   /* [8Se] GridEdgeGpu records (common.cuh:36-42): axis, base xyz, insideA, pad */
     const int32_t* conn;            /* [9 * 4096] Conn4096Gpu (cut.cuh:51-56): component count, component of each corner */
@@ -389,12 +389,12 @@ typedef struct GtsSurfaceView {
     int32_t* edge_points;           /* [2Eg] state: the cut points of each cut edge, owner A then B; fill with -1
                                      * before the first cut-aware rebuild, later rebuilds extend it */
     int32_t* candidates;            /* [3 * (2Se + 12N)] work: triangles in fixed slots before their placement */
-    int32_t* item_mask;             /* [max(Se + N, tri_capacity)] work */
+    int32_t* item_mask;             /* [max(Se + 6N, tri_capacity)] work */
     int32_t* tri;                   /* [3 * tri_capacity] out: triangle vertex tags (bit 31 component point, bit 30 wall) */
     int32_t* tri_work;              /* [3 * tri_capacity] work */
     int32_t* face_table;            /* [4 * face_capacity] work */
     int32_t* normal_fixed;          /* [27V] work: fixed-point normal sums, voxels then component slots */
-    int32_t* scan;                  /* [ceil(max(Se + N, tri_capacity) / 256) + 1] work */
+    int32_t* scan;                  /* [ceil(max(Se + 6N, tri_capacity) / 256) + 1] work */
     uint32_t* counts;               /* [4] out: 0 triangle indices, 1 indices before the cleanup, 2 indices asked for
                                      * (above 3 * tri_capacity when the triangles overflowed it) */
     float* out_pos;                 /* [9 * tri_capacity] out: the triangle soup, xyz per index */
