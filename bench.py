@@ -36,6 +36,12 @@ TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
 TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_batched_step": "k_batched_step<", "k_adjoint_forward": "k_adjoint_forward",
                    "k_adjoint_tape": "k_adjoint_tape", "k_adjoint_reverse": "k_adjoint_reverse",
                    "k_stream_copy": "k_stream_copy"}
+# the surface rebuild's own kernels, one dispatch each per cut-aware rebuild (its two partial scans are shared with the
+# cut and are left out: they move a few KB)
+SURFACE_KERNELS = {"k_voxel_pass": "k_voxel_pass", "k_component_points": "k_component_points",
+                   "k_triangles": "k_triangles", "k_place": "k_place(", "k_face_index": "k_face_index",
+                   "k_face_keep": "k_face_keep", "k_place_kept": "k_place_kept", "k_expand": "k_expand"}
+TRAFFIC_KERNELS.update(SURFACE_KERNELS)
 
 
 def load_traffic(kernel):
@@ -175,6 +181,15 @@ def surface_bytes(N, V, I, Se, K):
     written and read (40 + 24 B), surface edge records (32 B), and per soup index its position, normal, aux and tag
     (12 + 12 + 16 + 4 B)."""
     return 16 * N + 72 * V + 64 * I + 32 * Se + 44 * K
+
+
+def surface_traffic():
+    """Measured HBM bytes of one cut-aware surface rebuild: the sum over its kernels of the counter summary's bytes
+    per dispatch, or None when the summary lacks one of them."""
+    per = [load_traffic(k) for k in SURFACE_KERNELS]
+    if any(p is None for p in per):
+        return None
+    return sum(p["bytes_per_launch"] for p in per)
 
 
 def run_surface(t, reps=20, cpu=True):
@@ -618,7 +633,8 @@ def main():
                    "surface_edges": sf["Se"], "triangles": sf["K"] // 3, "wall_triangles": sf["walls"]},
         "roofline": {"bound": "hbm", "achieved": sach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": sach / HBM_PEAK_GBS, "kernel": "gts_surface_build (its chain of element kernels)",
-                     "avg_rebuild_us": 1e3 * sf["ms"], "algorithmic_bytes_per_rebuild": sf["bytes"]},
+                     "avg_rebuild_us": 1e3 * sf["ms"], "algorithmic_bytes_per_rebuild": sf["bytes"],
+                     "traffic": surface_traffic()},
     }
     if sf["cpu_ms"] is not None:
         line["surface"]["bit_exact_vs_oracle"] = sf["bit_exact"]

@@ -214,8 +214,8 @@ __device__ __forceinline__ void sort32(int v[32]) {
 // ---- k_AccumulateCutFP + k_ComputeComponentFP (cut.cu:786-985), one thread per voxel: each component of a cut,
 // occupied voxel gets one feature point from its intersections and its cut points (restored from the owners' frames,
 // summed in cut point index order), refined like the voxel feature point inside the component's box, clamped to it
-// and pulled onto the plane of its own cut points. Every other slot is cleared (k_ClearCutFP, cut.cu:761-772); the
-// fixed-point normal sums of the voxel's slots are cleared for the scatter. The voxel's intersections (at most 12, one
+// and pulled onto the plane of its own cut points. The voxel's other slots are cleared (k_ClearCutFP, cut.cu:761-772);
+// the fixed-point normal sums of its slots are cleared for the scatter. The voxel's intersections (at most 12, one
 // per edge) and cut points (at most 24, two per edge) are loaded into registers once, in independent batches, so the
 // ascent and the sums run without memory round trips.
 __global__ void __launch_bounds__(GTS_BLOCK) k_component_points(int V, Grid G, int iters, const int* __restrict__ corners,
@@ -230,12 +230,13 @@ __global__ void __launch_bounds__(GTS_BLOCK) k_component_points(int V, Grid G, i
                                    int* __restrict__ fixed) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    float4* out = cut_fp + 8 * (size_t)v;
     const uint32_t m = mask[v] & 0xFFFu;
-    int compCount = 0;
-    if (m != 0u && occupied[v] != 0u) compCount = conn[9 * m];
+    // A voxel's cut mask only grows and cut_fp starts zeroed, so the slots of a voxel that is not cut and occupied
+    // have never been written: they are already the cleared slots k_ClearCutFP would write.
+    if (m == 0u || occupied[v] == 0u) return;
+    float4* out = cut_fp + 8 * (size_t)v;
+    const int compCount = conn[9 * m];
     for (int c = compCount; c < 8; c++) out[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (compCount == 0) return;
     // the normal sums of the slots a triangle can name (a cut, occupied voxel's), cleared for the scatter
     int* fx = fixed + 3 * ((size_t)V + 8 * (size_t)v);
     for (int k = 0; k < 24; k++) fx[k] = 0;

@@ -385,7 +385,8 @@ typedef struct GtsSurfaceView {
     const float* rest_fp;           /* [4V] feature points of the rest surface (the snapshot recon_init takes) */
     float* isect_world;             /* [3I] work: deformed intersections */
     float* fp;                      /* [4V] out: voxel feature points, w = 1 where the voxel has one */
-    float* cut_fp;                  /* [32V] out: component feature points, slot 8 * voxel + component */
+    float* cut_fp;                  /* [32V] out: component feature points, slot 8 * voxel + component; zero-filled
+                                     * before the first rebuild (only cut, occupied voxels' slots are rewritten) */
     int32_t* edge_points;           /* [2Eg] state: the cut points of each cut edge, owner A then B; fill with -1
                                      * before the first cut-aware rebuild, later rebuilds extend it */
     int32_t* candidates;            /* [3 * (2Se + 12N)] work: triangles in fixed slots before their placement */
