@@ -192,6 +192,27 @@ def surface_traffic():
     return sum(p["bytes_per_launch"] for p in per)
 
 
+def run_readback(t, reps=50):
+    """Observation readback on the tissue's current state: Tissue.observation (pinned staging, one round trip; the
+    neighbour table skipped while the topology revision is unchanged) against the per-field reads it replaces
+    (positions, revision, reaction, each a blocking pageable copy), in microseconds per observation."""
+    import torch
+
+    rev = t.observation()["revision"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t.observation(rev)
+    pinned = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        t.positions()
+        t.revision()
+        t.reaction_force()
+    fields = (time.perf_counter() - t0) / reps
+    return dict(pinned_us=1e6 * pinned, fields_us=1e6 * fields, bytes=12 * t.N + 32 + 16)
+
+
 def run_surface(t, reps=20, cpu=True):
     """The rebuild LCS_Finalize runs (frames, then gts_surface_build) on the tissue's current, cut state, timed with
     HIP events on the stream it is launched on; the oracle's rebuild of the same state is the CPU baseline."""
@@ -624,7 +645,15 @@ def main():
                      "launches": g["launches"], "bytes_per_tet_update": b_alg,
                      "algorithmic_bytes_per_launch": b_alg * g["units"] / max(g["launches"], 1)},
     }
-    sf = run_surface(g.pop("tissue"), cpu=not args.no_cpu)
+    tissue = g.pop("tissue")
+    rb = run_readback(tissue)
+    line["readback"] = {
+        "value": rb["bytes"] / (rb["pinned_us"] * 1e-6) / 1e9, "unit": "GB/s", "us_per_observation": rb["pinned_us"],
+        "per_field_reads_us": rb["fields_us"], "bytes_per_observation": rb["bytes"],
+        "config": {"workload": "kidney-250k tissue after the timed steps: positions (packed to xyz on the device), "
+                               "counters and reaction in one pinned round trip, the neighbour table skipped (revision "
+                               "unchanged)"}}
+    sf = run_surface(tissue, cpu=not args.no_cpu)
     sach = sf["bytes"] / (sf["ms"] * 1e-3) / 1e9
     line["surface"] = {
         "value": sf["V"] / (sf["ms"] * 1e-3), "unit": "voxels/s", "ms_per_rebuild": sf["ms"],

@@ -33,6 +33,8 @@ ISECT_DTYPE = np.dtype([("position", "<f4", 3), ("normal", "<f4", 3),
                         ("a", "<i4"), ("b", "<i4"), ("t", "<f4"), ("inside_a", "<i4")])
 BEND_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("k", "<i4"), ("theta", "<f4"),
                        ("alive", "<i4"), ("padding", "<i4", 3)])
+# CutPointGpu (`cut.cuh:37-46`), 32 bytes: owner-frame offset on the rest edge, owner node, cut edge, owner-frame normal.
+CUT_POINT_DTYPE = np.dtype([("offset", "<f4", 3), ("owner", "<i4"), ("edge", "<i4"), ("normal", "<f4", 3)])
 METERS_PER_UNIT = 0.01
 BLOCK_SIZE = (6.0, 1.4, 4.0)
 # The 15 unordered slot pairs of a node's bend elements, in lexicographic order (`grid.py:257-260`).

@@ -228,10 +228,10 @@ class NativeTissue:
 
     def observe(self):
         t = self.tissue
-        nodes = t.positions()
-        revision = t.revision()
-        if revision != self.topology_revision:
-            neighbors = t.neighbors()
+        ob = t.observation(self.topology_revision, with_core=self.legacy_grasp)
+        nodes, revision = ob["positions"], ob["revision"]
+        if ob["neighbors"] is not None:
+            neighbors = ob["neighbors"]
             neighbors.flags.writeable = False
             self.neighbors, self.topology_revision = neighbors, revision
         n = len(nodes)
@@ -239,14 +239,14 @@ class NativeTissue:
         if self._blade is not None:
             blade[0:3], blade[3:6] = self._blade[0], self._blade[1]
             blade[6] = self.settings.blade_radius
-        force, contacts = t.reaction_force()
+        force, contacts = ob["reaction"]
         reaction = (force * METERS_PER_UNIT).astype(np.float32)
         hinge = np.array(self._command.hinge, dtype=np.float32)
         gap = float(self.settings.jaw_open_gap * (1 - self._command.closure))
         if not np.isfinite(nodes).all() or not np.isfinite(reaction).all():
             raise RuntimeError("Non-finite native state; simulation stopped")
         # LCS_GetGraspCoreFlags: the core corners of the legacy grasp; the friction model captures none
-        grip = t.grasp_state()["core"].astype(np.int32) if self.legacy_grasp else np.zeros(n, dtype=np.int32)
+        grip = ob["core"] if self.legacy_grasp else np.zeros(n, dtype=np.int32)
         return Observation(nodes, self.neighbors, grip, hinge, gap, 0, 0.0, blade, reaction)
 
     def cut_edge_fractions(self):

@@ -13,6 +13,7 @@ import torch
 
 from . import _lib
 from ._lib import GtsCutTool, GtsLattice, GtsStepParams, GtsTissueView, GtsToolState, GtsTopology, check, load, v3
+from .grid import CUT_POINT_DTYPE
 from .params import SolverParams
 
 TOOL_FIELDS3 = ("shaftA", "shaftB", "jawHinge", "jawAxis", "jawMid", "jawNormal", "bladeA", "bladeB")
@@ -203,6 +204,7 @@ class Tissue:
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
             self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
         self._tool_from = None
+        self._staging = None          # pinned host buffers of observation()
         self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep
         self.emit_ncut = np.array([0.0, 1.0, 0.0], dtype=np.float32)
         self.tear_tau = 0.0
@@ -393,6 +395,40 @@ class Tissue:
                     asked=int(s["counts"][2].item()))
 
     # ---------------------------------------------------------------- readback
+    def observation(self, known_revision=None, with_core=False):
+        """The readbacks of one Observation in one round trip (LCS_GetCornerPos, LCS_GetTopologyRevision,
+        LCS_GetContactReaction and, with `with_core`, LCS_GetGraspCoreFlags; plugin_api.cpp:170-176, 211-235): copies
+        on the current stream into pinned host buffers allocated once, one wait. The neighbour table (LCS_GetNbrIdx)
+        is read in a second round trip only when the revision differs from `known_revision`, as the reference's
+        observe does (native_backend.py:396-403); otherwise `neighbors` is None. The arrays returned are fresh."""
+        st = self._staging
+        if st is None:
+            def pin(n, dtype):
+                return torch.empty(n, dtype=dtype, pin_memory=True)
+            st = self._staging = dict(pos=pin((self.N, 3), torch.float32), counters=pin(8, torch.int32),
+                                      reaction=pin(4, torch.float32), nbr=pin(6 * self.N, torch.int32),
+                                      core=pin(self.N, torch.int32),
+                                      xyz=torch.empty((self.N, 3), dtype=torch.float32, device=self.device))
+        stream = torch.cuda.current_stream()
+        # positions packed to xyz rows on the device first: one contiguous copy each way, no strided host copy
+        st["xyz"].copy_(self.pos[:, :3])
+        st["pos"].copy_(st["xyz"], non_blocking=True)
+        st["counters"].copy_(self.counters, non_blocking=True)
+        st["reaction"].copy_(self.reaction, non_blocking=True)
+        if with_core:
+            st["core"].copy_(self.grasp_buffers["core"], non_blocking=True)
+        stream.synchronize()
+        revision = int(st["counters"].numpy().view(np.uint32)[0])
+        r = st["reaction"].numpy()
+        out = dict(positions=st["pos"].numpy().copy(), revision=revision,
+                   reaction=(r[:3].astype(np.float64), int(round(float(r[3])))),
+                   core=st["core"].numpy().copy() if with_core else None, neighbors=None)
+        if revision != known_revision:
+            st["nbr"].copy_(self.nbr, non_blocking=True)
+            stream.synchronize()
+            out["neighbors"] = st["nbr"].numpy().reshape(-1, 6).copy()
+        return out
+
     def positions(self):
         return self.pos[:, :3].cpu().numpy()
 
@@ -419,13 +455,8 @@ class Tissue:
     def cut_state(self):
         counters = self.counters.cpu().numpy().view(np.uint32)
         n = int(min(counters[1], self.cut_point_capacity))
-        raw = self.cut_points[:8 * n].cpu().numpy().reshape(-1, 8) if n else np.zeros((0, 8), dtype=np.float32)
-        points = np.zeros(n, dtype=[("offset", "<f4", 3), ("owner", "<i4"), ("edge", "<i4"), ("normal", "<f4", 3)])
-        if n:
-            points["offset"] = raw[:, 0:3]
-            points["owner"] = raw[:, 3].view(np.int32)
-            points["edge"] = raw[:, 4].view(np.int32)
-            points["normal"] = raw[:, 5:8]
+        raw = self.cut_points[:8 * n].cpu().numpy() if n else np.zeros(0, dtype=np.float32)
+        points = raw.view(CUT_POINT_DTYPE).copy()     # the device records are CutPointGpu (cut.cuh:37-46)
         bend = self.bend_alive.cpu().numpy().view(np.uint32)
         bend_alive = ((bend[:, None] >> np.arange(15, dtype=np.uint32)) & 1).astype(np.int32)
         return dict(neighbors=self.neighbors(), cut_mask=self.cut_mask.cpu().numpy().view(np.uint32),

@@ -48,7 +48,7 @@ class OracleCutTool(ct.Structure):
                 ("t2v2", F * 3), ("ncut", F * 3), ("d", F), ("valid", I), ("aabbMin", F * 3), ("aabbMax", F * 3)]
 
 
-CUT_POINT_DTYPE = np.dtype([("offset", "<f4", 3), ("owner", "<i4"), ("edge", "<i4"), ("normal", "<f4", 3)])
+from genesis_tissue_simulation_amd.grid import CUT_POINT_DTYPE  # noqa: E402  (the records the cut emits)
 
 _lib = None
 

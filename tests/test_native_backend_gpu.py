@@ -154,3 +154,26 @@ def test_surface_after_knife_cuts_matches_oracle():
         assert walls > 0, "the knife must open cut walls"
     with pytest.raises(RuntimeError):
         tissue.surface()
+
+
+def test_observation_reads_once_and_gates_the_neighbour_table():
+    """Tissue.observation equals the per-field readbacks; the neighbour table comes back only when the topology
+    revision differs from the one the caller holds."""
+    from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
+    grid = build_grid(0.6)
+    t = Tissue(grid, solver_params(Settings()), None, None, grid.positions)
+    t.set_gravity((0.0, -981.0, 0.0))
+    t.step(1 / 30)
+    first = t.observation()
+    np.testing.assert_array_equal(first["positions"], t.positions())
+    np.testing.assert_array_equal(first["neighbors"], t.neighbors())
+    assert first["revision"] == t.revision() == 0
+    assert first["reaction"][1] == t.reaction_force()[1]
+    again = t.observation(first["revision"])
+    assert again["neighbors"] is None
+    again["positions"][:] = 0                      # fresh arrays: the staging buffers are not exposed
+    np.testing.assert_array_equal(t.observation(0)["positions"], t.positions())
+    t.detect_cut(swept_cut_tool(((0.1, 2.0, -4.0), (0.1, -0.1, -4.0)), ((0.1, 2.0, 4.0), (0.1, -0.1, 4.0)), 0.02))
+    cut = t.observation(first["revision"])
+    assert cut["revision"] > 0 and cut["neighbors"] is not None
+    np.testing.assert_array_equal(cut["neighbors"], t.neighbors())
