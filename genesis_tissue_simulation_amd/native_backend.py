@@ -286,8 +286,8 @@ class NativeTissue:
         n = len(s["pos"])
         if n <= 0 or n % 3:
             raise RuntimeError("Native surface is empty or has an invalid triangle count")
-        if self.tissue.cut_state()["cut_point_counter"] > 4 * len(self.grid.edges) or \
-                s["asked"] > 3 * self.tri_capacity:
+        cut_points = int(self.tissue.counters[1].item()) & 0xffffffff
+        if cut_points > 4 * len(self.grid.edges) or s["asked"] > 3 * self.tri_capacity:
             raise RuntimeError("Native cut/surface capacity exceeded")
         if not np.isfinite(s["pos"]).all() or not np.isfinite(s["nrm"]).all():
             raise RuntimeError("Non-finite reconstructed surface")

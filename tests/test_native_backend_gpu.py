@@ -177,3 +177,13 @@ def test_observation_reads_once_and_gates_the_neighbour_table():
     cut = t.observation(first["revision"])
     assert cut["revision"] > 0 and cut["neighbors"] is not None
     np.testing.assert_array_equal(cut["neighbors"], t.neighbors())
+
+
+def test_surface_of_an_air_free_block_is_refused_like_the_reference():
+    """With no skin and no cut the soup is empty, and surface() raises as the reference's does
+    (native_backend.py:437-438)."""
+    from genesis_tissue_simulation_amd.grid import block_grid
+    with NativeTissue(block_grid((9, 6, 6), 0.25)) as tissue:
+        tissue.step(Command(blade_enabled=False))
+        with pytest.raises(RuntimeError, match="empty"):
+            tissue.surface()
