@@ -346,6 +346,7 @@ def run_adjoint(iters, warmup, count=64, tets=20_000, nsub=50, ckpt=10):
         g = fit.backward_raw((x - tgt) * (2 * fit.mask[:, None]))
         checks[scale] = dict(x_hidden=x_hidden, target=tgt[:n0].cpu().numpy(), grad=g[:t1].cpu().numpy())
     return dict(elapsed=elapsed, kernel_ms=start.elapsed_time(stop), T=fit.sT, N=fit.sN, E=fit.sE, count=count,
+                tets=tets,
                 nsub=nsub, ckpt=ckpt, setup_s=setup, inst0=insts[0], hidden0=hidden[:t1].numpy(),
                 init0=c0[:t1].numpy(), checks=checks,
                 launch=dict(block_size=fit.batch.block_size, n_workgroups=fit.batch.n_workgroups))
@@ -386,6 +387,35 @@ def adjoint_fp64_checks(a):
             entry["grad_rel_l2"] = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
         out["stated" if scale == 1.0 else f"soft_x{scale:g}"] = entry
     return out
+
+
+def run_cpu_adjoint(a, threads, blocks=4):
+    """The adjoint's CPU baseline: the FP32 oracle's rollout and gradient (oracle_adjoint_f32, which keeps every
+    substep's state) of the first `blocks` blocks of the same workload, one after another with OpenMP inside each
+    colour, in tet-substeps of rollout + gradient per second."""
+    import ctypes as ct
+
+    from genesis_tissue_simulation_amd.workloads import adjoint_instance
+    from oracle import oracle as orc
+
+    used = orc.set_threads(threads)
+    f = orc.lib().oracle_adjoint_f32
+    f.argtypes = [ct.c_void_p, ct.c_float, ct.c_int, ct.c_int] + [ct.c_void_p] * 5
+    f.restype = ct.c_double
+    ptr = lambda arr: np.ascontiguousarray(arr, dtype=np.float32).ctypes.data_as(ct.c_void_p)
+    tets, elapsed = 0, 0.0
+    for k in range(blocks):
+        inst = adjoint_instance(tets=a["tets"], seed=k)       # the blocks run_adjoint builds
+        o = orc.Oracle(inst.grid, inst.params, inst.initial_positions)
+        T = len(inst.compliance_init)
+        grad = np.zeros(T, dtype=np.float32)
+        target = np.asarray(inst.initial_positions, dtype=np.float32)
+        t0 = time.perf_counter()
+        f(o.h, np.float32(inst.dt), a["nsub"], inst.params.iterations, ptr(inst.force), ptr(inst.compliance_init),
+          ptr(inst.mask), ptr(target), grad.ctypes.data_as(ct.c_void_p))
+        elapsed += time.perf_counter() - t0
+        tets += T
+    return dict(value=tets * a["nsub"] / elapsed, threads=used, blocks=blocks, tets=tets)
 
 
 def adjoint_traffic(segments):
@@ -720,6 +750,13 @@ def main():
         checks = adjoint_fp64_checks(a)
         line["adjoint"]["rel_err_vs_fp64"] = checks["stated"]["rel_err_vs_fp64"]
         line["adjoint"]["fp64_checks"] = checks
+        if not args.no_cpu:
+            ca = run_cpu_adjoint(a, threads)
+            line["adjoint"]["cpu_baseline"] = {
+                "value": ca["value"], "unit": "tet-substeps/s (rollout + gradient)", "cores": ca["threads"],
+                "kind": "port", "sample": f"the first {ca['blocks']} blocks ({ca['tets']} tets) of the same workload, "
+                                          f"{a['nsub']} substeps, one block after another (CPU oracle, FP32, every "
+                                          f"substep's state kept, OpenMP inside each colour)"}
     if not args.no_cpu:
         line["fp64_check"] = plain_fp64_error(w)
         c = run_cpu(w, args.cpu_steps, threads)
