@@ -144,6 +144,7 @@ SIGNATURES = {
     "gts_adjoint_forward": ((P, P, I, I, P), I),
     "gts_adjoint_backward": ((P, P, I, I, P), I),
     "gts_surface_build": ((P, P, I, I, P), I),
+    "gts_surface_depth": ((P, P, I, F, F, P, P), I),
 }
 
 _lib = None

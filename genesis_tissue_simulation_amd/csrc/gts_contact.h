@@ -2,7 +2,19 @@
 // Same operation sequence as the CPU checker's restatement, in single precision except the piecewise-quadratic box
 // time, which the reference evaluates in double.
 #pragma once
+#include "../../include/gts_xpbd.h"
 #include "gts_common.h"
+
+// The tool pose of a C-ABI tool state (XpbdToolState, xpbd_solver.h:223-229, plus the blade capsule).
+inline ToolPose pose_of(const GtsToolState* t) {
+    ToolPose p;
+    p.shaftA = ld3(t->shaftA); p.shaftB = ld3(t->shaftB); p.shaftR = t->shaftR;
+    p.jawHinge = ld3(t->jawHinge); p.jawAxis = ld3(t->jawAxis); p.jawMid = ld3(t->jawMid);
+    p.jawNormal = ld3(t->jawNormal);
+    p.jawHalfWidth = t->jawHalfWidth; p.jawLen = t->jawLen; p.jawGap = t->jawGap; p.grasp01 = t->grasp01;
+    p.bladeA = ld3(t->bladeA); p.bladeB = ld3(t->bladeB); p.bladeR = t->bladeR;
+    return p;
+}
 
 struct ContactRecord {
     f3 normal, target, motion;

@@ -954,16 +954,6 @@ void profiled_launch(void (*kern)(A...), dim3 g, dim3 b, hipStream_t s, long lon
     g_profile.used++;
 }
 
-ToolPose pose_of(const GtsToolState* t) {
-    ToolPose p;
-    p.shaftA = ld3(t->shaftA); p.shaftB = ld3(t->shaftB); p.shaftR = t->shaftR;
-    p.jawHinge = ld3(t->jawHinge); p.jawAxis = ld3(t->jawAxis); p.jawMid = ld3(t->jawMid);
-    p.jawNormal = ld3(t->jawNormal);
-    p.jawHalfWidth = t->jawHalfWidth; p.jawLen = t->jawLen; p.jawGap = t->jawGap; p.grasp01 = t->grasp01;
-    p.bladeA = ld3(t->bladeA); p.bladeB = ld3(t->bladeB); p.bladeR = t->bladeR;
-    return p;
-}
-
 int scan_blocks(int n) { return grid_for(n); }
 
 // scratch layout in floats: [4*Eg) detection records, [4*Eg, 8*Eg) CCD normals, then int partials

@@ -19,6 +19,7 @@
 
 #include "../../include/gts_xpbd.h"
 #include "gts_common.h"
+#include "gts_contact.h"
 #include "gts_lattice.h"
 
 #define HIP_TRY(call)                          \
@@ -734,7 +735,121 @@ __global__ void k_expand(int n3, const uint32_t* __restrict__ counts, const int*
     reinterpret_cast<float4*>(out_aux)[i] = aux;
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Tool depth into the reconstructed surface (k_SurfaceDepth, surface_contact.cuh:662-674).
+// ---------------------------------------------------------------------------------------------------------------
+
+// shapeBounds (surface_contact.cuh:229-235) with no widening: the axis-aligned box of a capsule or an oriented box.
+__device__ __forceinline__ void shape_bounds(const Shape& s, float lo[3], float hi[3]) {
+    const f3 c = s.capsule ? (s.a + s.b) * .5f : s.center;
+    f3 e;
+    if (s.capsule) {
+        e = mk3(fabsf(s.b.x - s.a.x) * .5f + s.radius, fabsf(s.b.y - s.a.y) * .5f + s.radius,
+                fabsf(s.b.z - s.a.z) * .5f + s.radius);
+    } else {
+        e = mk3(fabsf(s.u.x) * s.half.x + fabsf(s.n.x) * s.half.y + fabsf(s.w.x) * s.half.z,
+                fabsf(s.u.y) * s.half.x + fabsf(s.n.y) * s.half.y + fabsf(s.w.y) * s.half.z,
+                fabsf(s.u.z) * s.half.x + fabsf(s.n.z) * s.half.y + fabsf(s.w.z) * s.half.z);
+    }
+    e = e + mk3(0.f, 0.f, 0.f);
+    const f3 l = c - e, h = c + e;
+    lo[0] = l.x; lo[1] = l.y; lo[2] = l.z;
+    hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
+}
+
+__device__ __forceinline__ f3 corner_bar(int k) { return mk3((float)(k == 0), (float)(k == 1), (float)(k == 2)); }
+
+// triangleWitness (surface_contact.cuh:169-198): the barycentric point of a triangle deepest in a shape. The best of
+// the centroid and the corners starts it; twelve Frank-Wolfe steps move towards the corner the shape's normal favours,
+// each with eight golden-section reductions that reuse the surviving sample, keeping the best iterate.
+__device__ __forceinline__ f3 triangle_witness(const Shape& sh, f3 a, f3 b, f3 c) {
+    const f3 q[3] = {a, b, c};
+    f3 bar = mk3(1.f / 3, 1.f / 3, 1.f / 3), best = bar, n, s;
+    float bestD = shape_distance(sh, (a + b + c) * (1.f / 3), n, s);
+    for (int k = 0; k < 3; k++) {
+        const float d = shape_distance(sh, q[k], n, s);
+        if (d < bestD) { bestD = d; best = corner_bar(k); }
+    }
+    for (int it = 0; it < 12; it++) {
+        const f3 p = a * bar.x + b * bar.y + c * bar.z;
+        shape_distance(sh, p, n, s);
+        int k = dot3(n, b) < dot3(n, a) ? 1 : 0;
+        if (dot3(n, c) < dot3(n, q[k])) k = 2;
+        const f3 target = corner_bar(k), dir = q[k] - p;
+        float lo = 0.f, hi = 1.f, l = .381966f, r = .618034f;
+        float dl = shape_distance(sh, p + dir * l, n, s), dr = shape_distance(sh, p + dir * r, n, s);
+        for (int j = 0; j < 8; j++) {
+            if (dl < dr) {
+                hi = r; r = l; dr = dl; l = lo + (hi - lo) * .381966f;
+                if (j < 7) dl = shape_distance(sh, p + dir * l, n, s);
+            } else {
+                lo = l; l = r; dl = dr; r = lo + (hi - lo) * .618034f;
+                if (j < 7) dr = shape_distance(sh, p + dir * r, n, s);
+            }
+        }
+        const float t = .5f * (lo + hi);
+        bar = bar * (1.f - t) + target * t;
+        const float d = shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s);
+        if (d < bestD) { bestD = d; best = bar; }
+    }
+    return best;
+}
+
+struct DepthShapes {
+    Shape s[4];  // 0 shaft capsule, 1 and 2 the jaw boxes, 3 the blade capsule
+};
+
+__global__ void k_depth_clear(float* __restrict__ out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0.f;
+}
+
+// One thread per triangle of the soup (its count is read from the last rebuild's counts[0] on the device): for each
+// shape asked for, the bounds cull, the witness and the penetration there; the maximum over all of them is folded in
+// with an unsigned atomic max on the bits of a non-negative float, which does not depend on the order.
+__global__ void __launch_bounds__(GTS_BLOCK) k_surface_depth(int capacity, const uint32_t* __restrict__ counts,
+                                                             const float* __restrict__ soup, DepthShapes shapes,
+                                                             int kinds, float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int triangles = min(capacity, (int)(counts[0] / 3u));
+    if (i >= triangles) return;
+    const float* t = soup + 9 * (size_t)i;
+    const f3 a = mk3(t[0], t[1], t[2]), b = mk3(t[3], t[4], t[5]), c = mk3(t[6], t[7], t[8]);
+    const float va[3] = {a.x, a.y, a.z}, vb[3] = {b.x, b.y, b.z}, vc[3] = {c.x, c.y, c.z};
+    float deepest = 0.f;
+    for (int kind = 0; kind < 4; kind++) {
+        if (!((kinds >> kind) & 1)) continue;
+        const Shape& sh = shapes.s[kind];
+        if (sh.capsule ? sh.radius <= 0.f : (sh.half.x <= 0.f || sh.half.z <= 0.f)) continue;
+        float lo[3], hi[3];
+        shape_bounds(sh, lo, hi);
+        bool outside = false;
+        for (int ax = 0; ax < 3; ax++)
+            outside |= fmaxf(va[ax], fmaxf(vb[ax], vc[ax])) < lo[ax] || fminf(va[ax], fminf(vb[ax], vc[ax])) > hi[ax];
+        if (outside) continue;
+        const f3 bar = triangle_witness(sh, a, b, c);
+        f3 n, s;
+        const float depth = fmaxf(0.f, -shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
+        deepest = fmaxf(deepest, depth);
+    }
+    if (deepest > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(deepest));
+}
+
 }  // namespace
+
+extern "C" int gts_surface_depth(const GtsSurfaceView* s, const GtsToolState* tool, int kinds, float thickness,
+                                 float min_gap, float* depth, void* stream) {
+    if (!s || !tool || !depth || kinds < 0 || kinds > 15 || s->tri_capacity <= 0 || !s->counts || !s->out_pos)
+        return GTS_ERR_BAD_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    const ToolPose pose = pose_of(tool);
+    DepthShapes shapes;
+    for (int kind = 0; kind < 4; kind++) shapes.s[kind] = tool_shape(pose, kind, thickness, min_gap);
+    k_depth_clear<<<1, 64, 0, st>>>(depth);
+    if (kinds) k_surface_depth<<<blocks_for(s->tri_capacity), GTS_BLOCK, 0, st>>>(s->tri_capacity, s->counts, s->out_pos,
+                                                                                  shapes, kinds, depth);
+    HIP_TRY(hipGetLastError());
+    return GTS_OK;
+}
 
 extern "C" int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s, int cut_aware, int have_rest,
                                  void* stream) {

@@ -266,16 +266,23 @@ class NativeTissue:
         return np.array((np.abs(J - 1).mean(), J.min()), dtype=np.float32)
 
     def contact_depth_m(self):
-        """Deepest penetration of a lattice node into the grasper shaft capsule, in metres."""
-        t = self._tool
-        return _capsule_depth(self.tissue.positions(), t["shaftA"], t["shaftB"], t["shaftR"]) * METERS_PER_UNIT
+        """LCS_SurfaceDepth (`native_backend.py:453-456`): the deepest penetration of the grasper's shaft capsule and
+        jaw boxes into the reconstructed surface, in metres, 0 before the first tool command (the tool is unarmed)."""
+        if self.closed:
+            raise RuntimeError("Native session is closed")
+        if self._tool is None:
+            return 0.0
+        return float(self.tissue.surface_depth(self._tool, kinds=0b0111).item()) * METERS_PER_UNIT
 
     def knife_body_depth_m(self):
-        """Deepest penetration of a lattice node into the knife capsule, in metres."""
+        """LCS_KnifeBodyDepth (`native_backend.py:468-471`): the deepest penetration of the knife's capsule into the
+        reconstructed surface, in metres, 0 while the knife is off."""
+        if self.closed:
+            raise RuntimeError("Native session is closed")
         if self._blade is None:
             return 0.0
-        return _capsule_depth(self.tissue.positions(), self._blade[0], self._blade[1],
-                              self.settings.blade_radius) * METERS_PER_UNIT
+        knife = dict(bladeA=self._blade[0], bladeB=self._blade[1], bladeR=self.settings.blade_radius)
+        return float(self.tissue.surface_depth(knife, kinds=0b1000).item()) * METERS_PER_UNIT
 
     def surface(self):
         """The surface LCS_Finalize last rebuilt (`native_backend.py:434-451`), with the reference's checks: an empty
@@ -304,14 +311,6 @@ class NativeTissue:
 
     def __exit__(self, exc_type, exc, traceback):
         self.close()
-
-
-def _capsule_depth(points, a, b, r):
-    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
-    ab = b - a
-    t = np.clip((points - a) @ ab / max(ab @ ab, 1e-20), 0, 1)
-    d = np.linalg.norm(points - (a + t[:, None] * ab), axis=1)
-    return float(max(0.0, (r - d).max()))
 
 
 def tet_determinants(tissue):

@@ -374,6 +374,18 @@ class Tissue:
         check("gts_surface_build", self.lib.gts_surface_build(ct.byref(self.view), ct.byref(self.surface_view), 1, 1,
                                                               current_stream()))
 
+    def surface_depth(self, tool, kinds=0b0111, out=None):
+        """LCS_SurfaceDepth (xpbd_surface_depth, xpbd_solver.cu:2046-2059): the deepest penetration of the tool's
+        shapes into the last rebuilt surface, as a 0-d device tensor (0 when none). `kinds` selects them by bit: 1 the
+        shaft capsule, 2 and 4 the jaw boxes (with the step's jaw_thickness and jaw_gap_closed), 8 the blade capsule."""
+        if getattr(self, "surface_view", None) is None:
+            raise RuntimeError("the surface has not been built")
+        out = torch.empty((), dtype=torch.float32, device=self.device) if out is None else out
+        check("gts_surface_depth", self.lib.gts_surface_depth(
+            ct.byref(self.surface_view), ct.byref(tool_state(tool)), int(kinds), float(self.params.jaw_thickness),
+            float(self.params.jaw_gap_closed), out.data_ptr(), current_stream()))
+        return out
+
     def surface_overflowed(self):
         """Whether the last rebuild's triangles exceeded the capacity (cleanInternalFaces then fails,
         surface_cleanup.cuh:38)."""

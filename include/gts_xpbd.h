@@ -410,6 +410,14 @@ typedef struct GtsSurfaceView {
  * edge in edge order, then the cut walls in node order; the cleanup keeps each face's first listed representative. */
 int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s, int cut_aware, int have_rest, void* stream);
 
+/* Device: LCS_SurfaceDepth (plugin_api.cpp:32, xpbd_surface_depth in xpbd_solver.cu:2046-2059, k_SurfaceDepth in
+ * contact/surface_contact.cuh:662-674): the deepest penetration of the tool's shapes into the surface that `s` last
+ * rebuilt, written to the device float `depth` (0 when none). `kinds` selects the shapes by bit: 1 the shaft capsule,
+ * 2 and 4 the jaw boxes (thickness, min_gap: the step's jaw_thickness and jaw_gap_closed), 8 the blade capsule;
+ * LCS_SurfaceDepth's grasper part is 7. The triangle count is read on the device; nothing is synchronised. */
+int gts_surface_depth(const GtsSurfaceView* s, const GtsToolState* tool, int kinds, float thickness, float min_gap,
+                      float* depth, void* stream);
+
 /* Device: copy n_float4 16-byte elements (no reference counterpart): the known-size kernel against which HBM byte
  * counters are scaled, and the achievable copy bandwidth. */
 int gts_stream_copy(void* dst, const void* src, long long n_float4, void* stream);

@@ -366,3 +366,17 @@ def surface(grid, pos, rot, nbr, cut_mask, edge_cut_t, cut_points, rest_fp, *, c
         out[k] = out[k][:n]
     out["listed"] = int(listed[0])
     return out
+
+
+GRASPER_KINDS = 0b0111     # shaft capsule and both jaw boxes
+BLADE_KIND = 0b1000        # the blade capsule
+
+
+def surface_depth(tool, thickness, min_gap, pos, kinds=GRASPER_KINDS):
+    """Deepest penetration of the tool shapes in `kinds` (bit 0 shaft capsule, bits 1-2 jaw boxes, bit 3 blade
+    capsule) into a triangle soup `pos` [K, 3] (oracle_surface_depth: LCS_SurfaceDepth's grasper part)."""
+    f = lib().oracle_surface_depth
+    f.restype = F
+    f.argtypes = [ct.POINTER(OracleTool), I, F, F, P, I]
+    p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    return float(f(ct.byref(tool_struct(tool)), int(kinds), float(thickness), float(min_gap), _ptr(p), len(p) // 3))

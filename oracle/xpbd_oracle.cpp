@@ -242,6 +242,54 @@ template <class R> R shapeDistance(const Shape<R>& s, V3<R> x, V3<R>& normal, V3
     return q[k];
 }
 
+// shapeBounds (surface_contact.cuh:229-235): the axis-aligned box of a shape, widened by radius.
+inline void shapeBoundsF(const Shape<float>& s, float radius, V3<float>& lo, V3<float>& hi) {
+    V3<float> c = s.capsule ? (s.a + s.b) * .5f : s.center;
+    V3<float> e = s.capsule ? mk(std::fabs(s.b.x - s.a.x) * .5f + s.radius, std::fabs(s.b.y - s.a.y) * .5f + s.radius,
+                                 std::fabs(s.b.z - s.a.z) * .5f + s.radius)
+                            : mk(std::fabs(s.u.x) * s.half.x + std::fabs(s.n.x) * s.half.y + std::fabs(s.w.x) * s.half.z,
+                                 std::fabs(s.u.y) * s.half.x + std::fabs(s.n.y) * s.half.y + std::fabs(s.w.y) * s.half.z,
+                                 std::fabs(s.u.z) * s.half.x + std::fabs(s.n.z) * s.half.y + std::fabs(s.w.z) * s.half.z);
+    e = e + mk(radius, radius, radius);
+    lo = c - e;
+    hi = c + e;
+}
+
+// triangleWitness (surface_contact.cuh:169-198): the barycentric point of a triangle deepest in a shape, by
+// Frank-Wolfe steps on the simplex with an eight-reduction golden-section line search, keeping the best of the
+// centroid, the corners and the iterates.
+inline V3<float> triangleWitnessF(const Shape<float>& sh, V3<float> a, V3<float> b, V3<float> c) {
+    V3<float> q[3] = {a, b, c}, bar = mk(1.f / 3, 1.f / 3, 1.f / 3), best = bar, n, s;
+    float bestD = shapeDistance(sh, (a + b + c) * (1.f / 3), n, s);
+    for (int k = 0; k < 3; k++) {
+        float d = shapeDistance(sh, q[k], n, s);
+        if (d < bestD) { bestD = d; best = mk((float)(k == 0), (float)(k == 1), (float)(k == 2)); }
+    }
+    for (int it = 0; it < 12; it++) {
+        V3<float> p = a * bar.x + b * bar.y + c * bar.z;
+        shapeDistance(sh, p, n, s);
+        int k = dot3(n, b) < dot3(n, a) ? 1 : 0;
+        if (dot3(n, c) < dot3(n, q[k])) k = 2;
+        V3<float> target = mk((float)(k == 0), (float)(k == 1), (float)(k == 2)), dir = q[k] - p;
+        float lo = 0, hi = 1, l = .381966f, r = .618034f;
+        float dl = shapeDistance(sh, p + dir * l, n, s), dr = shapeDistance(sh, p + dir * r, n, s);
+        for (int j = 0; j < 8; j++) {
+            if (dl < dr) {
+                hi = r; r = l; dr = dl; l = lo + (hi - lo) * .381966f;
+                if (j < 7) dl = shapeDistance(sh, p + dir * l, n, s);
+            } else {
+                lo = l; l = r; dl = dr; r = lo + (hi - lo) * .618034f;
+                if (j < 7) dr = shapeDistance(sh, p + dir * r, n, s);
+            }
+        }
+        float t = .5f * (lo + hi);
+        bar = bar * (1 - t) + target * t;
+        float d = shapeDistance(sh, a * bar.x + b * bar.y + c * bar.z, n, s);
+        if (d < bestD) { bestD = d; best = bar; }
+    }
+    return best;
+}
+
 template <class R> Shape<R> shapeLerp(const Shape<R>& a, const Shape<R>& b, R t) {  // friction_contact.cuh:64-68
     Shape<R> s = b;
     s.center = a.center * ((R)1 - t) + b.center * t;
@@ -2225,3 +2273,35 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* for
 
 ORACLE_ADJ(float, f32)
 ORACLE_ADJ(double, f64)
+
+// LCS_SurfaceDepth for the grasper (xpbd_surface_depth, xpbd_solver.cu:2046-2059, with k_SurfaceDepth,
+// surface_contact.cuh:662-674): the deepest penetration of the tool shapes whose bits are set in `kinds` (0 shaft
+// capsule, 1 and 2 the jaw boxes, 3 the blade capsule) into the surface soup `pos` [3K] (K indices, three per
+// triangle), 0 when none.
+extern "C" float oracle_surface_depth(const OracleTool* tool, int kinds, float thickness, float minGap, const float* pos,
+                                      int K) {
+    ToolR<float> t = toolFrom<float>(*tool);
+    float result = 0.f;
+    for (int kind = 0; kind < 4; kind++) {
+        if (!((kinds >> kind) & 1)) continue;
+        Shape<float> sh = contactShape<float>(t, kind, thickness, minGap);
+        if (sh.capsule ? sh.radius <= 0 : (sh.half.x <= 0 || sh.half.z <= 0)) continue;
+        V3<float> lo, hi;
+        shapeBoundsF(sh, 0.f, lo, hi);
+        for (int i = 0; i < K / 3; i++) {
+            V3<float> a = v3<float>(pos + 9 * (size_t)i), b = v3<float>(pos + 9 * (size_t)i + 3),
+                      c = v3<float>(pos + 9 * (size_t)i + 6);
+            bool out = false;
+            for (int ax = 0; ax < 3; ax++) {
+                float mx = std::fmax(comp(a, ax), std::fmax(comp(b, ax), comp(c, ax)));
+                float mn = std::fmin(comp(a, ax), std::fmin(comp(b, ax), comp(c, ax)));
+                if (mx < comp(lo, ax) || mn > comp(hi, ax)) out = true;
+            }
+            if (out) continue;
+            V3<float> bar = triangleWitnessF(sh, a, b, c), n, s;
+            float depth = std::fmax(0.f, -shapeDistance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
+            result = std::fmax(result, depth);
+        }
+    }
+    return result;
+}

@@ -129,3 +129,50 @@ def test_a_block_without_air_matches_oracle_empty_then_walls_only():
     t.rebuild_surface()
     print("surface parity: air-free block, cut", flush=True)
     assert same(t.surface(), oracle_surface(t), "air-free cut") > 0
+
+
+def test_tool_depths_into_the_surface_match_oracle():
+    """gts_surface_depth (LCS_SurfaceDepth: k_SurfaceDepth's bounds cull, Frank-Wolfe witness and penetration, max over
+    triangles and shapes) equals the oracle's bit for bit on the rest surface and on the rebuilt surface of a cut,
+    sagging block, for each shape alone, for the grasper and for all four, with tools pressed in and clear of it."""
+    from oracle import oracle
+    t = tissue(GRIDS["block"]())
+    t.init_surface()
+    p = t.params
+    poses = {
+        "pressed": dict(shaftA=(-2.0, 0.95, 0.3), shaftB=(-0.5, 0.95, 0.3), shaftR=0.4,
+                        jawHinge=(0.8, 0.72, -1.2), jawAxis=(0.0, 0.0, 1.0), jawMid=(0.8, 0.72, -0.7),
+                        jawNormal=(0.0, 1.0, 0.0), jawHalfWidth=0.25, jawLen=1.0, jawGap=0.3, grasp01=0.5,
+                        bladeA=(1.7, 1.5, 0.2), bladeB=(1.9, 0.45, 0.6), bladeR=0.05),
+        "edge-on": dict(shaftA=(3.2, 0.9, -2.2), shaftB=(2.6, 0.4, -1.7), shaftR=0.3,
+                        jawHinge=(-3.1, 0.0, 0.5), jawAxis=(1.0, 0.2, 0.0), jawMid=(-2.6, 0.1, 0.5),
+                        jawNormal=(0.0, 0.0, 1.0), jawHalfWidth=0.3, jawLen=1.0, jawGap=0.6, grasp01=0.0,
+                        bladeA=(0.0, 0.0, 2.4), bladeB=(0.0, 0.0, 1.7), bladeR=0.1),
+        "clear": dict(shaftA=(-2.0, 3.0, 0.3), shaftB=(-0.5, 3.0, 0.3), shaftR=0.4,
+                      jawHinge=(0.8, 3.0, -1.2), jawAxis=(0.0, 0.0, 1.0), jawMid=(0.8, 3.0, -0.7),
+                      jawNormal=(0.0, 1.0, 0.0), jawHalfWidth=0.25, jawLen=1.0, jawGap=0.3, grasp01=0.5,
+                      bladeA=(1.7, 3.5, 0.2), bladeB=(1.9, 3.0, 0.6), bladeR=0.05)}
+
+    def check(what):
+        soup = t.surface()["pos"]
+        out = {}
+        for name, tool in poses.items():
+            for kinds in (1, 2, 4, 8, 7, 15):
+                print(f"surface depth parity: {what}, {name}, kinds {kinds}", flush=True)
+                gpu = float(t.surface_depth(tool, kinds).item())
+                ora = oracle.surface_depth(tool, p.jaw_thickness, p.jaw_gap_closed, soup, kinds)
+                assert gpu == ora, (what, name, kinds, gpu, ora)
+                out[name, kinds] = gpu
+        assert all(out["clear", k] == 0.0 for k in (1, 2, 4, 8, 7, 15))
+        assert out["pressed", 1] > 0 and out["pressed", 8] > 0 and max(out["pressed", 2], out["pressed", 4]) > 0
+        assert out["pressed", 15] == max(out["pressed", k] for k in (1, 2, 4, 8))
+        return out
+
+    check("rest")
+    for s in range(4):
+        t.step(1 / 60)
+        if s == 1:
+            t.detect_cut(blade_x(0.1, -0.1))
+        t.finalize()
+        t.rebuild_surface()
+    check("cut and sagging")

@@ -187,3 +187,32 @@ def test_surface_of_an_air_free_block_is_refused_like_the_reference():
         tissue.step(Command(blade_enabled=False))
         with pytest.raises(RuntimeError, match="empty"):
             tissue.surface()
+
+
+def test_contact_and_knife_depths_match_oracle_on_the_rebuilt_surface():
+    """contact_depth_m (LCS_SurfaceDepth) and knife_body_depth_m (LCS_KnifeBodyDepth) are the deepest penetration of
+    the grasper's shaft and jaw boxes, and of the knife capsule, into the surface the step rebuilt: bit for bit the
+    oracle's over the same soup, 0 before the first command, and positive once the tools press in."""
+    from genesis_tissue_simulation_amd.native_backend import METERS_PER_UNIT
+    from oracle import oracle
+    grid = build_grid(0.6)
+    with NativeTissue(grid) as tissue:
+        assert tissue.contact_depth_m() == 0.0 and tissue.knife_body_depth_m() == 0.0
+        pressed = knifed = 0
+        for k in range(10):
+            cmd = Command(hinge=(-1.4, 0.95 - 0.06 * k, -0.5), closure=0.2,
+                          blade_center=(-2.0 + 0.3 * k, 0.55, 0.5), blade_enabled=True)
+            print(f"drop-in depths: step {k}", flush=True)
+            tissue.step(cmd)
+            soup = tissue.surface().vertices
+            p = tissue.tissue.params
+            grasper = oracle.surface_depth(tissue._tool, p.jaw_thickness, p.jaw_gap_closed, soup)
+            knife = dict(bladeA=tissue._blade[0], bladeB=tissue._blade[1], bladeR=tissue.settings.blade_radius)
+            blade = oracle.surface_depth(knife, p.jaw_thickness, p.jaw_gap_closed, soup, kinds=oracle.BLADE_KIND)
+            assert tissue.contact_depth_m() == grasper * METERS_PER_UNIT, k
+            assert tissue.knife_body_depth_m() == blade * METERS_PER_UNIT, k
+            pressed += grasper > 0
+            knifed += blade > 0
+        assert pressed > 0 and knifed > 0
+    with pytest.raises(RuntimeError):
+        tissue.contact_depth_m()

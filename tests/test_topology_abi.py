@@ -186,6 +186,15 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     tv = _lib.GtsTissueView()
     tv.n_nodes, tv.vx, tv.vy, tv.vz = 27, 2, 2, 2
     assert lib.gts_surface_build(ct.byref(tv), ct.byref(sview), 0, 0, null) == BAD
+    # the surface depth without a view, a tool or an output, with an unknown shape bit, or over an unbuilt surface
+    depth_tool = _lib.GtsToolState()
+    one = ct.c_void_p(16)
+    assert lib.gts_surface_depth(null, ct.byref(depth_tool), 7, 0.1, 0.0, one, null) == BAD
+    assert lib.gts_surface_depth(ct.byref(sview), null, 7, 0.1, 0.0, one, null) == BAD
+    assert lib.gts_surface_depth(ct.byref(sview), ct.byref(depth_tool), 7, 0.1, 0.0, null, null) == BAD
+    assert lib.gts_surface_depth(ct.byref(sview), ct.byref(depth_tool), 16, 0.1, 0.0, one, null) == BAD
+    assert lib.gts_surface_depth(ct.byref(sview), ct.byref(depth_tool), -1, 0.1, 0.0, one, null) == BAD
+    assert lib.gts_surface_depth(ct.byref(sview), ct.byref(depth_tool), 7, 0.1, 0.0, one, null) == BAD  # no counts
     # a step with a view but an invalid tet mode
     view = _lib.GtsTissueView()
     params = _lib.GtsStepParams()
