@@ -213,9 +213,10 @@ def run_readback(t, reps=50):
     return dict(pinned_us=1e6 * pinned, fields_us=1e6 * fields, bytes=12 * t.N + 32 + 16)
 
 
-def run_surface(t, reps=20, cpu=True):
+def run_surface(t, reps=20, cpu=True, tool=None):
     """The rebuild LCS_Finalize runs (frames, then gts_surface_build) on the tissue's current, cut state, timed with
-    HIP events on the stream it is launched on; the oracle's rebuild of the same state is the CPU baseline."""
+    HIP events on the stream it is launched on; the oracle's rebuild of the same state is the CPU baseline. With a
+    tool, LCS_SurfaceDepth's query of the grasper against the rebuilt surface is timed the same way."""
     import torch
 
     t.init_surface()
@@ -244,6 +245,24 @@ def run_surface(t, reps=20, cpu=True):
         out["cpu_ms"] = 1e3 * (time.perf_counter() - t0)
         out["bit_exact"] = bool(np.array_equal(o["tags"], s["tags"]) and np.array_equal(o["pos"], s["pos"]) and
                                 np.array_equal(o["nrm"], s["nrm"]) and np.array_equal(o["aux"], s["aux"]))
+    if tool is not None:
+        depth = torch.empty((), dtype=torch.float32, device=t.device)
+        t.surface_depth(tool, 0b0111, out=depth)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(reps):
+            t.surface_depth(tool, 0b0111, out=depth)
+        end.record()
+        torch.cuda.synchronize()
+        q = dict(us=1e3 * start.elapsed_time(end) / reps, depth=float(depth.item()), triangles=K // 3)
+        if cpu:
+            from oracle import oracle
+            p = t.params
+            t0 = time.perf_counter()
+            ref = oracle.surface_depth(tool, p.jaw_thickness, p.jaw_gap_closed, s["pos"])
+            q["cpu_us"] = 1e6 * (time.perf_counter() - t0)
+            q["bit_exact"] = ref == q["depth"]
+        out["depth"] = q
     return out
 
 
@@ -683,7 +702,7 @@ def main():
         "config": {"workload": "kidney-250k tissue after the timed steps: positions (packed to xyz on the device), "
                                "counters and reaction in one pinned round trip, the neighbour table skipped (revision "
                                "unchanged)"}}
-    sf = run_surface(tissue, cpu=not args.no_cpu)
+    sf = run_surface(tissue, cpu=not args.no_cpu, tool=w.tool(w.steps) if w.tool else None)
     sach = sf["bytes"] / (sf["ms"] * 1e-3) / 1e9
     line["surface"] = {
         "value": sf["V"] / (sf["ms"] * 1e-3), "unit": "voxels/s", "ms_per_rebuild": sf["ms"],
@@ -699,6 +718,19 @@ def main():
         line["surface"]["bit_exact_vs_oracle"] = sf["bit_exact"]
         line["surface"]["cpu_baseline"] = {"value": sf["V"] / (sf["cpu_ms"] * 1e-3), "unit": "voxels/s", "cores": 1,
                                            "kind": "port", "sample": "one rebuild of the same state by the oracle"}
+    if "depth" in sf:
+        dq = sf["depth"]
+        line["surface"]["depth_query"] = {
+            "value": dq["triangles"] / (dq["us"] * 1e-6), "unit": "triangles/s", "us_per_query": dq["us"],
+            "depth": dq["depth"], "triangles": dq["triangles"],
+            "config": {"workload": "LCS_SurfaceDepth: the grasper pressed 3 mm in, against the rebuilt surface "
+                                   "(gts_surface_depth: a clear kernel and one thread per triangle)"},
+            "gbs_soup_read": 36 * dq["triangles"] / (dq["us"] * 1e-6) / 1e9}
+        if "cpu_us" in dq:
+            line["surface"]["depth_query"]["bit_exact_vs_oracle"] = dq["bit_exact"]
+            line["surface"]["depth_query"]["cpu_baseline"] = {
+                "value": dq["triangles"] / (dq["cpu_us"] * 1e-6), "unit": "triangles/s", "cores": 1, "kind": "port",
+                "sample": "one query of the same soup by the oracle"}
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if not args.skip_batched:
         bt = run_batched(args.batched_steps, 2, args.batched_count, tune=not args.no_tune)

@@ -60,6 +60,7 @@ __device__ inline void particle_frame(int c, int base, const float4* __restrict_
         int safe[6];
         bool live[6];
         float4 np4[6];
+#pragma unroll
         for (int q = 0; q < 6; q++) {
             const int n = nbr[6 * c + q];
             safe[q] = n >= 0 ? base + n : c;
@@ -72,16 +73,21 @@ __device__ inline void particle_frame(int c, int base, const float4* __restrict_
         for (int q = 0; q < 6; q++)
             for (int j = 0; j < 9; j++) r[q][j] = prev[9 * (size_t)safe[q] + j];
         for (int q = 0; q < 6; q++) live[q] = live[q] & (act_q[q] != 0);
-        f3 psum[3] = {mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f), mk3(0.f, 0.f, 0.f)};
+        // the three sums are separate variables, not an array: selects over an array element were lowered to
+        // scratch memory accesses
+        f3 ps0 = mk3(0.f, 0.f, 0.f), ps1 = ps0, ps2 = ps0;
         int pc = 0;
+#pragma unroll
         for (int q = 0; q < 6; q++) {
-            for (int k = 0; k < 3; k++) {
-                const f3 s = psum[k] + mk3(r[q][k], r[q][3 + k], r[q][6 + k]);
-                psum[k] = live[q] ? s : psum[k];
+            if (live[q]) {
+                ps0 = ps0 + mk3(r[q][0], r[q][3], r[q][6]);
+                ps1 = ps1 + mk3(r[q][1], r[q][4], r[q][7]);
+                ps2 = ps2 + mk3(r[q][2], r[q][5], r[q][8]);
+                pc++;
             }
-            pc += live[q] ? 1 : 0;
         }
         const f3 cPos = frame_xyz(pos[c]);
+#pragma unroll
         for (int k = 0; k < 3; k++) {
             const bool okp = live[2 * k + 1], okm = live[2 * k];
             const f3 xp = frame_xyz(np4[2 * k + 1]), xm = frame_xyz(np4[2 * k]);
@@ -94,9 +100,10 @@ __device__ inline void particle_frame(int c, int base, const float4* __restrict_
             if (len < 1e-12f) continue;
             f3 dir = div3(e, len);
             if (pc > 0) {
-                float pl = len3(psum[k]);
+                const f3 psum = k == 0 ? ps0 : (k == 1 ? ps1 : ps2);
+                float pl = len3(psum);
                 if (pl > 1e-6f) {
-                    f3 blended = dir + div3(psum[k], pl);
+                    f3 blended = dir + div3(psum, pl);
                     float dl = len3(blended);
                     if (dl > 0.25f) dir = div3(blended, dl);
                 }
@@ -104,13 +111,20 @@ __device__ inline void particle_frame(int c, int base, const float4* __restrict_
             axis[k] = dir;
             have[k] = true;
         }
-        int miss = -1, mc = 0;
-        for (int k = 0; k < 3; k++)
-            if (!have[k]) { miss = k; mc++; }
+        // a single lost axis is the cross product of the next two, cyclically; each case is spelled out so that the
+        // axes stay in registers (an axis array indexed at run time would live in scratch memory)
+        const int mc = (have[0] ? 0 : 1) + (have[1] ? 0 : 1) + (have[2] ? 0 : 1);
         if (mc == 1) {
-            f3 cx = cross3(axis[(miss + 1) % 3], axis[(miss + 2) % 3]);
-            float l = len3(cx);
-            if (l > 1e-9f) { axis[miss] = div3(cx, l); have[miss] = true; }
+            const f3 cx = !have[0] ? cross3(axis[1], axis[2]) : (!have[1] ? cross3(axis[2], axis[0])
+                                                                          : cross3(axis[0], axis[1]));
+            const float l = len3(cx);
+            if (l > 1e-9f) {
+                const f3 u = div3(cx, l);
+                if (!have[0]) axis[0] = u;
+                else if (!have[1]) axis[1] = u;
+                else axis[2] = u;
+                have[0] = have[1] = have[2] = true;
+            }
         }
         ok = have[0] && have[1] && have[2];
     }

@@ -234,6 +234,8 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
             ContactRecord rec[4];
             float lamN[4];
             f3 lamT[4];
+            // every loop over the four shapes is unrolled so that the records stay in registers
+#pragma unroll
             for (int kind = 0; kind < 4; kind++) {
                 rec[kind].valid = 0;
                 lamN[kind] = 0.f;
@@ -241,7 +243,9 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
                 if (((armed >> kind) & 1u) && shape_present(shapes.to[kind]))
                     rec[kind] = trace_tool_contact(shapes.from[kind], shapes.to[kind], radius, xp, x);
             }
+#pragma unroll
             for (int pass = 0; pass < 2; pass++)
+#pragma unroll
                 for (int kind = 0; kind < 4; kind++) {
                     const ContactRecord& c = rec[kind];
                     if (!c.valid) continue;
@@ -259,6 +263,7 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
                     x = x + (nextT - oldT) * w;
                     lamT[kind] = nextT;
                 }
+#pragma unroll
             for (int kind = 0; kind < 4; kind++)
                 if (rec[kind].valid) {
                     force = force + (rec[kind].normal * lamN[kind] + lamT[kind]) * (-scaleR);

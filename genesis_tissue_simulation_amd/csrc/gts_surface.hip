@@ -757,25 +757,27 @@ __device__ __forceinline__ void shape_bounds(const Shape& s, float lo[3], float 
     hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
 }
 
+// corner k of a triangle by selects (a private array indexed at run time would live in scratch)
+__device__ __forceinline__ f3 corner(int k, f3 a, f3 b, f3 c) { return k == 0 ? a : (k == 1 ? b : c); }
+
 __device__ __forceinline__ f3 corner_bar(int k) { return mk3((float)(k == 0), (float)(k == 1), (float)(k == 2)); }
 
 // triangleWitness (surface_contact.cuh:169-198): the barycentric point of a triangle deepest in a shape. The best of
 // the centroid and the corners starts it; twelve Frank-Wolfe steps move towards the corner the shape's normal favours,
 // each with eight golden-section reductions that reuse the surviving sample, keeping the best iterate.
 __device__ __forceinline__ f3 triangle_witness(const Shape& sh, f3 a, f3 b, f3 c) {
-    const f3 q[3] = {a, b, c};
     f3 bar = mk3(1.f / 3, 1.f / 3, 1.f / 3), best = bar, n, s;
     float bestD = shape_distance(sh, (a + b + c) * (1.f / 3), n, s);
     for (int k = 0; k < 3; k++) {
-        const float d = shape_distance(sh, q[k], n, s);
+        const float d = shape_distance(sh, corner(k, a, b, c), n, s);
         if (d < bestD) { bestD = d; best = corner_bar(k); }
     }
     for (int it = 0; it < 12; it++) {
         const f3 p = a * bar.x + b * bar.y + c * bar.z;
         shape_distance(sh, p, n, s);
         int k = dot3(n, b) < dot3(n, a) ? 1 : 0;
-        if (dot3(n, c) < dot3(n, q[k])) k = 2;
-        const f3 target = corner_bar(k), dir = q[k] - p;
+        if (dot3(n, c) < dot3(n, corner(k, a, b, c))) k = 2;
+        const f3 target = corner_bar(k), dir = corner(k, a, b, c) - p;
         float lo = 0.f, hi = 1.f, l = .381966f, r = .618034f;
         float dl = shape_distance(sh, p + dir * l, n, s), dr = shape_distance(sh, p + dir * r, n, s);
         for (int j = 0; j < 8; j++) {
@@ -803,50 +805,50 @@ __global__ void k_depth_clear(float* __restrict__ out) {
     if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0.f;
 }
 
-// One thread per triangle of the soup (its count is read from the last rebuild's counts[0] on the device): for each
-// shape asked for, the bounds cull, the witness and the penetration there; the maximum over all of them is folded in
-// with an unsigned atomic max on the bits of a non-negative float, which does not depend on the order.
+// One thread per (triangle, shape) of the soup, the four shapes of a triangle on neighbouring lanes, so that a
+// triangle near several shapes runs their witnesses side by side. The triangle count is read from the last rebuild's
+// counts[0] on the device. Per shape asked for: the bounds cull, the witness and the penetration there; the maximum
+// over all of them is folded in with an unsigned atomic max on the bits of a non-negative float, which does not depend
+// on the order.
 __global__ void __launch_bounds__(GTS_BLOCK) k_surface_depth(int capacity, const uint32_t* __restrict__ counts,
                                                              const float* __restrict__ soup, DepthShapes shapes,
                                                              int kinds, float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t >> 2, kind = t & 3;
     const int triangles = min(capacity, (int)(counts[0] / 3u));
-    if (i >= triangles) return;
-    const float* t = soup + 9 * (size_t)i;
-    const f3 a = mk3(t[0], t[1], t[2]), b = mk3(t[3], t[4], t[5]), c = mk3(t[6], t[7], t[8]);
+    if (i >= triangles || !((kinds >> kind) & 1)) return;
+    const Shape& sh = shapes.s[kind];  // read in place from the kernel arguments
+    if (sh.capsule ? sh.radius <= 0.f : (sh.half.x <= 0.f || sh.half.z <= 0.f)) return;
+    const float* v = soup + 9 * (size_t)i;
+    const f3 a = mk3(v[0], v[1], v[2]), b = mk3(v[3], v[4], v[5]), c = mk3(v[6], v[7], v[8]);
     const float va[3] = {a.x, a.y, a.z}, vb[3] = {b.x, b.y, b.z}, vc[3] = {c.x, c.y, c.z};
-    float deepest = 0.f;
-    for (int kind = 0; kind < 4; kind++) {
-        if (!((kinds >> kind) & 1)) continue;
-        const Shape& sh = shapes.s[kind];
-        if (sh.capsule ? sh.radius <= 0.f : (sh.half.x <= 0.f || sh.half.z <= 0.f)) continue;
-        float lo[3], hi[3];
-        shape_bounds(sh, lo, hi);
-        bool outside = false;
-        for (int ax = 0; ax < 3; ax++)
-            outside |= fmaxf(va[ax], fmaxf(vb[ax], vc[ax])) < lo[ax] || fminf(va[ax], fminf(vb[ax], vc[ax])) > hi[ax];
-        if (outside) continue;
-        const f3 bar = triangle_witness(sh, a, b, c);
-        f3 n, s;
-        const float depth = fmaxf(0.f, -shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
-        deepest = fmaxf(deepest, depth);
-    }
-    if (deepest > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(deepest));
+    float lo[3], hi[3];
+    shape_bounds(sh, lo, hi);
+    bool outside = false;
+    for (int ax = 0; ax < 3; ax++)
+        outside |= fmaxf(va[ax], fmaxf(vb[ax], vc[ax])) < lo[ax] || fminf(va[ax], fminf(vb[ax], vc[ax])) > hi[ax];
+    if (outside) return;
+    const f3 bar = triangle_witness(sh, a, b, c);
+    f3 n, s;
+    const float depth = fmaxf(0.f, -shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
+    if (depth > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(depth));
 }
 
 }  // namespace
 
 extern "C" int gts_surface_depth(const GtsSurfaceView* s, const GtsToolState* tool, int kinds, float thickness,
                                  float min_gap, float* depth, void* stream) {
-    if (!s || !tool || !depth || kinds < 0 || kinds > 15 || s->tri_capacity <= 0 || !s->counts || !s->out_pos)
+    if (!s || !tool || !depth || kinds < 0 || kinds > 15 || s->tri_capacity <= 0 || s->tri_capacity >= (1 << 29) ||
+        !s->counts || !s->out_pos)
         return GTS_ERR_BAD_ARG;
     hipStream_t st = (hipStream_t)stream;
     const ToolPose pose = pose_of(tool);
     DepthShapes shapes;
     for (int kind = 0; kind < 4; kind++) shapes.s[kind] = tool_shape(pose, kind, thickness, min_gap);
     k_depth_clear<<<1, 64, 0, st>>>(depth);
-    if (kinds) k_surface_depth<<<blocks_for(s->tri_capacity), GTS_BLOCK, 0, st>>>(s->tri_capacity, s->counts, s->out_pos,
-                                                                                  shapes, kinds, depth);
+    if (kinds) k_surface_depth<<<blocks_for(4ll * s->tri_capacity), GTS_BLOCK, 0, st>>>(s->tri_capacity, s->counts,
+                                                                                         s->out_pos, shapes, kinds,
+                                                                                         depth);
     HIP_TRY(hipGetLastError());
     return GTS_OK;
 }
