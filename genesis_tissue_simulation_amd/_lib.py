@@ -145,6 +145,7 @@ SIGNATURES = {
     "gts_adjoint_backward": ((P, P, I, I, P), I),
     "gts_surface_build": ((P, P, I, I, P), I),
     "gts_surface_depth": ((P, P, I, F, F, P, P), I),
+    "gts_material_quality": ((P, P, P), I),
 }
 
 _lib = None

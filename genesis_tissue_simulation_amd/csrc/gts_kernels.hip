@@ -1105,6 +1105,60 @@ extern "C" int gts_compute_rotations(const GtsTissueView* v, void* stream) {
 }
 
 namespace {
+// The {min J, count} accumulator starts at {1, 0} (xpbd_get_material_quality's seed, xpbd_solver.cu:2136); a fill
+// kernel rather than a copy or memset, so the call can sit in a captured graph.
+__global__ void k_quality_seed(float* __restrict__ out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[0] = 1.f;
+        out[1] = 0.f;
+    }
+}
+
+// k_MaterialQuality (xpbd_solver.cu:2122-2132) per live tet: F = [x1 - x0, x2 - x0, x3 - x0] Dm^-1, J = f0 . (f1 x f2);
+// a non-finite J is counted and goes no further; otherwise J lowers the minimum (a compare-and-swap loop on the
+// float, exact in any order) and is counted when J <= 0. Counts are float adds of 1, exact below 2^24 tets.
+__global__ void k_material_quality(int T, Lattice L, const uint32_t* __restrict__ tet_desc,
+                                   const float4* __restrict__ pos, float* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    const uint32_t d = tet_desc[e];
+    if (!(d & 1u)) return;
+    const int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+    int id[4];
+    for (int c = 0; c < 4; c++) id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);
+    const f3 a = xyz(pos[id[0]]);
+    const f3 e1 = xyz(pos[id[1]]) - a, e2 = xyz(pos[id[2]]) - a, e3 = xyz(pos[id[3]]) - a;
+    const float* B = L.inv_rest[type];
+    f3 f[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) f[k] = e1 * B[k] + e2 * B[3 + k] + e3 * B[6 + k];
+    const float j = dot3(f[0], cross3(f[1], f[2]));
+    if (!isfinite(j)) {
+        atomicAdd(out + 1, 1.f);
+        return;
+    }
+    unsigned int* address = reinterpret_cast<unsigned int*>(out);
+    unsigned int old = __hip_atomic_load(address, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (j < __uint_as_float(old)) {
+        const unsigned int assumed = old;
+        old = atomicCAS(address, assumed, __float_as_uint(j));
+        if (old == assumed) break;
+    }
+    if (j <= 0.f) atomicAdd(out + 1, 1.f);
+}
+}  // namespace
+
+extern "C" int gts_material_quality(const GtsTissueView* v, float* out, void* stream) {
+    if (!v || !out || v->n_tets < 0 || (v->n_tets > 0 && (!v->tet_desc || !v->pos))) return GTS_ERR_BAD_ARG;
+    hipStream_t s = S(stream);
+    k_quality_seed<<<1, 64, 0, s>>>(out);
+    if (v->n_tets > 0)
+        k_material_quality<<<grid_for(v->n_tets), GTS_BLOCK, 0, s>>>(v->n_tets, lattice_of(v), v->tet_desc,
+                                                                     reinterpret_cast<const float4*>(v->pos), out);
+    return -(int)hipGetLastError();
+}
+
+namespace {
 
 static_assert(sizeof(StepShapes) == GTS_STEP_CONST_FLOATS * sizeof(float), "step constant layout");
 

@@ -259,11 +259,11 @@ class NativeTissue:
         return self.tissue.compact(apply=apply)
 
     def material_quality(self):
-        """[mean |J - 1|, min J] over the live tets of the current deformation."""
-        J = tet_determinants(self.tissue)
-        if len(J) == 0:
-            return np.zeros(2, dtype=np.float32)
-        return np.array((np.abs(J - 1).mean(), J.min()), dtype=np.float32)
+        """LCS_GetMaterialQuality (`native_backend.py:463-466`): f32[2] = [the smallest det F over the live tets, at
+        most 1; the number of live tets that are inverted (det F <= 0) or whose det F is not finite]."""
+        if self.closed:
+            raise RuntimeError("Native session is closed")
+        return self.tissue.material_quality().cpu().numpy()
 
     def contact_depth_m(self):
         """LCS_SurfaceDepth (`native_backend.py:453-456`): the deepest penetration of the grasper's shaft capsule and
@@ -311,20 +311,3 @@ class NativeTissue:
 
     def __exit__(self, exc_type, exc, traceback):
         self.close()
-
-
-def tet_determinants(tissue):
-    """det F of every live tet (in the device's colour order), from the current positions."""
-    from .grid import KUHN_TETS
-    topo = tissue.topology
-    desc = tissue.tet_desc[:max(tissue.T, 1)].cpu().numpy().view(np.uint32)
-    live = (desc & 1) != 0
-    base = (desc >> 4).astype(np.int64)[live]
-    ty = ((desc >> 1) & 7).astype(np.int64)[live]
-    nx, ny, _ = (d + 1 for d in tissue.grid.dims)
-    offs = np.array([[(c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nx * ny for c in row] for row in KUHN_TETS])
-    ids = base[:, None] + offs[ty]
-    x = tissue.positions().astype(np.float64)[ids]
-    Ds = np.stack((x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]), axis=2)
-    B = topo["type_inv_rest"][ty]
-    return np.linalg.det(Ds @ B)

@@ -374,6 +374,14 @@ class Tissue:
         check("gts_surface_build", self.lib.gts_surface_build(ct.byref(self.view), ct.byref(self.surface_view), 1, 1,
                                                               current_stream()))
 
+    def material_quality(self, out=None):
+        """LCS_GetMaterialQuality (xpbd_get_material_quality, xpbd_solver.cu:2133-2138) as a device tensor [2]: the
+        smallest det F over the live tets (seeded at 1) and how many have det F <= 0 or a non-finite det F."""
+        out = torch.empty(2, dtype=torch.float32, device=self.device) if out is None else out
+        check("gts_material_quality", self.lib.gts_material_quality(ct.byref(self.view), out.data_ptr(),
+                                                                    current_stream()))
+        return out
+
     def surface_depth(self, tool, kinds=0b0111, out=None):
         """LCS_SurfaceDepth (xpbd_surface_depth, xpbd_solver.cu:2046-2059): the deepest penetration of the tool's
         shapes into the last rebuilt surface, as a 0-d device tensor (0 when none). `kinds` selects them by bit: 1 the

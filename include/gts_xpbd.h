@@ -415,6 +415,11 @@ int gts_surface_build(const GtsTissueView* v, const GtsSurfaceView* s, int cut_a
  * rebuilt, written to the device float `depth` (0 when none). `kinds` selects the shapes by bit: 1 the shaft capsule,
  * 2 and 4 the jaw boxes (thickness, min_gap: the step's jaw_thickness and jaw_gap_closed), 8 the blade capsule;
  * LCS_SurfaceDepth's grasper part is 7. The triangle count is read on the device; nothing is synchronised. */
+/* Device: LCS_GetMaterialQuality (plugin_api.cpp:207, xpbd_get_material_quality and k_MaterialQuality in
+ * xpbd_solver.cu:2122-2138): out[0] = the smallest det F over the live tets, seeded at 1 (so at most 1); out[1] = how
+ * many live tets have det F <= 0 or a non-finite det F. `out` is 2 device floats; nothing is synchronised. */
+int gts_material_quality(const GtsTissueView* v, float* out, void* stream);
+
 int gts_surface_depth(const GtsSurfaceView* s, const GtsToolState* tool, int kinds, float thickness, float min_gap,
                       float* depth, void* stream);
 

@@ -380,3 +380,19 @@ def surface_depth(tool, thickness, min_gap, pos, kinds=GRASPER_KINDS):
     f.argtypes = [ct.POINTER(OracleTool), I, F, F, P, I]
     p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
     return float(f(ct.byref(tool_struct(tool)), int(kinds), float(thickness), float(min_gap), _ptr(p), len(p) // 3))
+
+
+def material_quality(pos, ids, rows, active=None):
+    """[min det F seeded at 1, count of live tets with det F <= 0 or non-finite] (oracle_material_quality:
+    LCS_GetMaterialQuality). pos [N, 3], ids [T, 4] in each tet's corner order, rows [T, 3, 3] the Dm^-1 rows."""
+    f = lib().oracle_material_quality
+    f.restype = None
+    f.argtypes = [P, P, P, P, I, P]
+    x = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1)
+    i = np.ascontiguousarray(ids, dtype=np.int32).reshape(-1, 4)
+    r = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 9)
+    T = len(i)
+    a = np.ones(T, np.int32) if active is None else np.ascontiguousarray(active, dtype=np.int32)
+    out = np.zeros(2, np.float32)
+    f(_ptr(x), _ptr(i), _ptr(r), _ptr(a), T, _ptr(out))
+    return out

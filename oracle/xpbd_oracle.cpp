@@ -2274,6 +2274,29 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* for
 ORACLE_ADJ(float, f32)
 ORACLE_ADJ(double, f64)
 
+// LCS_GetMaterialQuality (xpbd_get_material_quality with k_MaterialQuality, xpbd_solver.cu:2122-2138): over the live
+// tets (ids [4T] in the tet's corner order, Dm^-1 rows [9T], active [T]), out[0] = min det F seeded at 1, out[1] = the
+// number with det F <= 0 or a non-finite det F, where F = [x1 - x0, x2 - x0, x3 - x0] Dm^-1 and det F = f0 . (f1 x f2).
+extern "C" void oracle_material_quality(const float* pos, const int* ids, const float* rows, const int* active, int T,
+                                        float* out) {
+    out[0] = 1.f;
+    out[1] = 0.f;
+    for (int i = 0; i < T; i++) {
+        if (!active[i]) continue;
+        const int* id = ids + 4 * (size_t)i;
+        const float* b = rows + 9 * (size_t)i;
+        V3<float> a = v3<float>(pos + 3 * (size_t)id[0]);
+        V3<float> e1 = v3<float>(pos + 3 * (size_t)id[1]) - a, e2 = v3<float>(pos + 3 * (size_t)id[2]) - a,
+                  e3 = v3<float>(pos + 3 * (size_t)id[3]) - a;
+        V3<float> f[3];
+        for (int k = 0; k < 3; k++) f[k] = e1 * b[k] + e2 * b[3 + k] + e3 * b[6 + k];
+        float j = dot3(f[0], cross3(f[1], f[2]));
+        if (!std::isfinite(j)) { out[1] += 1.f; continue; }
+        if (j < out[0]) out[0] = j;
+        if (j <= 0.f) out[1] += 1.f;
+    }
+}
+
 // LCS_SurfaceDepth for the grasper (xpbd_surface_depth, xpbd_solver.cu:2046-2059, with k_SurfaceDepth,
 // surface_contact.cuh:662-674): the deepest penetration of the tool shapes whose bits are set in `kinds` (0 shaft
 // capsule, 1 and 2 the jaw boxes, 3 the blade capsule) into the surface soup `pos` [3K] (K indices, three per

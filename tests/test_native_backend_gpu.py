@@ -43,8 +43,10 @@ def test_contract_and_cut():
         nodes, slots = np.nonzero(severed)
         other = grid.neighbors[nodes, slots]
         assert np.all(obs.neighbors[other, slots ^ 1] < 0)
-        q = tissue.material_quality()
-        assert np.isfinite(q).all()  # no minimum-J safeguard on this path: J may go negative under the knife
+        q = tissue.material_quality()   # [min det F seeded at 1, inverted or non-finite tets], as the oracle's
+        from test_gpu_material_quality import oracle_quality
+        np.testing.assert_array_equal(q, oracle_quality(tissue.tissue)[0])
+        assert q.dtype == np.float32 and q.shape == (2,) and q[0] <= 1 and q[1] == int(q[1])
     with pytest.raises(RuntimeError):
         tissue.step(Command())
     NativeTissue(grid).close()  # the session is free again

@@ -186,6 +186,8 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     tv = _lib.GtsTissueView()
     tv.n_nodes, tv.vx, tv.vy, tv.vz = 27, 2, 2, 2
     assert lib.gts_surface_build(ct.byref(tv), ct.byref(sview), 0, 0, null) == BAD
+    assert lib.gts_material_quality(null, ct.c_void_p(16), null) == BAD
+    assert lib.gts_material_quality(ct.byref(_lib.GtsTissueView()), null, null) == BAD
     # the surface depth without a view, a tool or an output, with an unknown shape bit, or over an unbuilt surface
     depth_tool = _lib.GtsToolState()
     one = ct.c_void_p(16)
