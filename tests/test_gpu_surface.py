@@ -124,11 +124,18 @@ def test_a_block_without_air_matches_oracle_empty_then_walls_only():
     print("surface parity: air-free block, rest", flush=True)
     t.init_surface()
     assert same(t.surface(), oracle_surface(t, cut_aware=False), "air-free rest") == 0
+    inside = dict(shaftA=(0.5, 0.6, 0.6), shaftB=(1.5, 0.6, 0.6), shaftR=0.3, bladeA=(1.1, 0.2, 0.5),
+                  bladeB=(1.1, 1.0, 0.5), bladeR=0.1)
+    assert float(t.surface_depth(inside, 15).item()) == 0.0      # an empty soup: nothing to press into
     t.detect_cut(swept_cut_tool(((1.1, 2.0, -1.0), (1.1, 0.4, -1.0)), ((1.1, 2.0, 3.0), (1.1, 0.4, 3.0)), D))
     t.finalize()
     t.rebuild_surface()
     print("surface parity: air-free block, cut", flush=True)
     assert same(t.surface(), oracle_surface(t), "air-free cut") > 0
+    from oracle import oracle
+    depth = float(t.surface_depth(inside, 15).item())
+    assert depth == oracle.surface_depth(inside, t.params.jaw_thickness, t.params.jaw_gap_closed, t.surface()["pos"],
+                                         15) and depth > 0    # the knife capsule sits in the kerf it opened
 
 
 def test_tool_depths_into_the_surface_match_oracle():
