@@ -86,6 +86,56 @@ class Command:
     scissors_yaw_deg: float = 0.0
 
 
+def scissors_frame(pitch_deg, roll_deg=0.0, yaw_deg=0.0):
+    """The scissors' axis and opening normal in native coordinates (`native_backend.py:140-152`): at zero angles the
+    axis points along -x and the first blade opens along +y; the roll turns the normal about the axis towards +z, the
+    pitch then tilts the axis downwards (about +z), and the yaw turns the result about +y towards +z."""
+    r, p, y = np.radians((roll_deg, pitch_deg, yaw_deg))
+
+    def about_x(v, a):
+        return np.array((v[0], np.cos(a) * v[1] - np.sin(a) * v[2], np.sin(a) * v[1] + np.cos(a) * v[2]))
+
+    def about_z(v, a):
+        return np.array((np.cos(a) * v[0] - np.sin(a) * v[1], np.sin(a) * v[0] + np.cos(a) * v[1], v[2]))
+
+    def about_y(v, a):
+        return np.array((np.cos(a) * v[0] + np.sin(a) * v[2], v[1], -np.sin(a) * v[0] + np.cos(a) * v[2]))
+
+    def turn(v):
+        return about_y(about_z(about_x(v, r), p), y)
+
+    return turn(np.array((-1.0, 0.0, 0.0))), turn(np.array((0.0, 1.0, 0.0)))
+
+
+@dataclass(frozen=True)
+class ScissorsLoads:
+    """The reference's scissors blade loads (`native_backend.py:155-166`), in newtons. The pressure-fracture scissors
+    are outside the legacy path, so observations here never carry one; the type is kept for callers that name it."""
+    left_n: float
+    right_n: float
+    fracture_left_n: float
+    fracture_right_n: float
+    fracture_total_n: float
+    fractures: int
+    is_squeezing: bool
+
+
+@dataclass(frozen=True)
+class ScissorsPose:
+    """The reference's solved scissors pose (`native_backend.py:169-180`); `Observation.scissors` is always None on the
+    legacy path."""
+    hinge: np.ndarray
+    axis: np.ndarray
+    normal: np.ndarray
+    gap: float
+    radius: float
+    edges: np.ndarray
+    pressure_n: float
+    phase: int
+    contacts: int
+    loads: ScissorsLoads
+
+
 @dataclass
 class Observation:
     """Field-for-field the reference observation (`native_backend.py:182-196`)."""
@@ -98,7 +148,7 @@ class Observation:
     grip_error_m: float
     blade: np.ndarray
     reaction_n: np.ndarray
-    scissors: object = None
+    scissors: ScissorsPose | None = None
     jaw_load_n: tuple = (0.0, 0.0)
 
 
