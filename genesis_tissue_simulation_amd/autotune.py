@@ -55,7 +55,7 @@ def tune(batch, dt, steps=3, warmup=1, space=None):
     return best, table
 
 
-def tune_plain(make_tissue, step, steps=3, warmup=2, blocks=(64, 128, 256)):
+def tune_plain(make_tissue, step, steps=3, warmup=2, blocks=(32, 64, 128, 256)):
     """Colour-kernel workgroup size of the single-instance step: each candidate runs `warmup` untimed and `steps`
     timed steps of `step(tissue, k)` on a fresh tissue from `make_tissue(colour_block)`. Returns (best, table)."""
     table = []

@@ -985,8 +985,8 @@ int launch_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
 }
 
 // Threads per workgroup of the colour kernels. A colour of the 250k-tet kidney lattice holds about 8k tets: 64-thread
-// workgroups spread it over ~125 CUs instead of ~32, which measured 5.83 ms per step against 6.39 ms at 256 threads
-// and 7.14 ms at 512.
+// workgroups spread it over ~125 CUs instead of ~32, which measured 5.81 ms per step against 6.32 ms at 256 threads
+// and 7.14 ms at 512; 32-thread workgroups (half-empty waves on every CU) measured 6.10 ms.
 #define GTS_COLOUR_BLOCK 64
 inline int colour_block(const GtsTissueView* v) { return v->colour_block > 0 ? v->colour_block : GTS_COLOUR_BLOCK; }
 
@@ -1408,7 +1408,7 @@ int step_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState
               void* stream) {
     if (!v || !p || !from || !to) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
-    if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 64) return GTS_ERR_BAD_ARG;
+    if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 32) return GTS_ERR_BAD_ARG;
     hipStream_t s = S(stream);
     Lattice L = lattice_of(v);
     StepPlan P;
@@ -1587,7 +1587,7 @@ extern "C" int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, c
                                void* stream) {
     if (!v || !p || !gp || !from || !to || !grasp_state_ok(g)) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
-    if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 64) return GTS_ERR_BAD_ARG;
+    if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 32) return GTS_ERR_BAD_ARG;
     hipStream_t s = S(stream);
     Lattice L = lattice_of(v);
     StepPlan P;

@@ -124,7 +124,7 @@ typedef struct GtsTissueView {
     float* step_consts;          /* [step_const_capacity * GTS_STEP_CONST_FLOATS] per-substep tool shapes */
     int step_const_capacity;     /* substeps the buffer holds */
     int use_graph;               /* 1: capture the step into a hipGraph once and replay it */
-    int colour_block;            /* threads per workgroup of the edge / tet colour kernels: 64..1024, multiple of 64;
+    int colour_block;            /* threads per workgroup of the edge / tet colour kernels: 32..1024, multiple of 32;
                                   * 0 = 64 */
 } GtsTissueView;
 
