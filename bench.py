@@ -724,7 +724,7 @@ def main():
             "value": dq["triangles"] / (dq["us"] * 1e-6), "unit": "triangles/s", "us_per_query": dq["us"],
             "depth": dq["depth"], "triangles": dq["triangles"],
             "config": {"workload": "LCS_SurfaceDepth: the grasper pressed 3 mm in, against the rebuilt surface "
-                                   "(gts_surface_depth: a clear kernel and one thread per triangle)"},
+                                   "(gts_surface_depth: a clear kernel, then one thread per (triangle, shape) on a bounded, striding grid)"},
             "gbs_soup_read": 36 * dq["triangles"] / (dq["us"] * 1e-6) / 1e9}
         if "cpu_us" in dq:
             line["surface"]["depth_query"]["bit_exact_vs_oracle"] = dq["bit_exact"]

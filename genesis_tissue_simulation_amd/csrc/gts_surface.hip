@@ -807,38 +807,41 @@ __global__ void k_depth_clear(float* __restrict__ out) {
 
 // One thread per (triangle, shape) of the soup, the four shapes of a triangle on neighbouring lanes, so that a
 // triangle near several shapes runs their witnesses side by side. The triangle count is read from the last rebuild's
-// counts[0] on the device. Per shape asked for: the bounds cull, the witness and the penetration there; the maximum
-// over all of them is folded in with an unsigned atomic max on the bits of a non-negative float, which does not depend
-// on the order.
+// counts[0] on the device, and a bounded grid strides over it, so a soup far below the capacity launches no empty
+// workgroups. Per shape asked for: the bounds cull, the witness and the penetration there; the maximum over all of
+// them is folded in with an unsigned atomic max on the bits of a non-negative float, which does not depend on the
+// order.
 __global__ void __launch_bounds__(GTS_BLOCK) k_surface_depth(int capacity, const uint32_t* __restrict__ counts,
                                                              const float* __restrict__ soup, DepthShapes shapes,
                                                              int kinds, float* __restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = t >> 2, kind = t & 3;
     const int triangles = min(capacity, (int)(counts[0] / 3u));
-    if (i >= triangles || !((kinds >> kind) & 1)) return;
-    const Shape& sh = shapes.s[kind];  // read in place from the kernel arguments
-    if (sh.capsule ? sh.radius <= 0.f : (sh.half.x <= 0.f || sh.half.z <= 0.f)) return;
-    const float* v = soup + 9 * (size_t)i;
-    const f3 a = mk3(v[0], v[1], v[2]), b = mk3(v[3], v[4], v[5]), c = mk3(v[6], v[7], v[8]);
-    const float va[3] = {a.x, a.y, a.z}, vb[3] = {b.x, b.y, b.z}, vc[3] = {c.x, c.y, c.z};
-    float lo[3], hi[3];
-    shape_bounds(sh, lo, hi);
-    bool outside = false;
-    for (int ax = 0; ax < 3; ax++)
-        outside |= fmaxf(va[ax], fmaxf(vb[ax], vc[ax])) < lo[ax] || fminf(va[ax], fminf(vb[ax], vc[ax])) > hi[ax];
-    if (outside) return;
-    const f3 bar = triangle_witness(sh, a, b, c);
-    f3 n, s;
-    const float depth = fmaxf(0.f, -shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
-    if (depth > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(depth));
+    const int items = 4 * triangles;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < items; t += gridDim.x * blockDim.x) {
+        const int i = t >> 2, kind = t & 3;
+        if (!((kinds >> kind) & 1)) continue;
+        const Shape& sh = shapes.s[kind];  // read in place from the kernel arguments
+        if (sh.capsule ? sh.radius <= 0.f : (sh.half.x <= 0.f || sh.half.z <= 0.f)) continue;
+        const float* v = soup + 9 * (size_t)i;
+        const f3 a = mk3(v[0], v[1], v[2]), b = mk3(v[3], v[4], v[5]), c = mk3(v[6], v[7], v[8]);
+        const float va[3] = {a.x, a.y, a.z}, vb[3] = {b.x, b.y, b.z}, vc[3] = {c.x, c.y, c.z};
+        float lo[3], hi[3];
+        shape_bounds(sh, lo, hi);
+        bool outside = false;
+        for (int ax = 0; ax < 3; ax++)
+            outside |= fmaxf(va[ax], fmaxf(vb[ax], vc[ax])) < lo[ax] || fminf(va[ax], fminf(vb[ax], vc[ax])) > hi[ax];
+        if (outside) continue;
+        const f3 bar = triangle_witness(sh, a, b, c);
+        f3 n, s;
+        const float depth = fmaxf(0.f, -shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
+        if (depth > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(depth));
+    }
 }
 
 }  // namespace
 
 extern "C" int gts_surface_depth(const GtsSurfaceView* s, const GtsToolState* tool, int kinds, float thickness,
                                  float min_gap, float* depth, void* stream) {
-    if (!s || !tool || !depth || kinds < 0 || kinds > 15 || s->tri_capacity <= 0 || s->tri_capacity >= (1 << 29) ||
+    if (!s || !tool || !depth || kinds < 0 || kinds > 15 || s->tri_capacity <= 0 || s->tri_capacity >= (1 << 28) ||
         !s->counts || !s->out_pos)
         return GTS_ERR_BAD_ARG;
     hipStream_t st = (hipStream_t)stream;
@@ -846,9 +849,9 @@ extern "C" int gts_surface_depth(const GtsSurfaceView* s, const GtsToolState* to
     DepthShapes shapes;
     for (int kind = 0; kind < 4; kind++) shapes.s[kind] = tool_shape(pose, kind, thickness, min_gap);
     k_depth_clear<<<1, 64, 0, st>>>(depth);
-    if (kinds) k_surface_depth<<<blocks_for(4ll * s->tri_capacity), GTS_BLOCK, 0, st>>>(s->tri_capacity, s->counts,
-                                                                                         s->out_pos, shapes, kinds,
-                                                                                         depth);
+    // at most 1024 workgroups (4 per CU); they stride over the (triangle, shape) items the rebuild left
+    const unsigned grid = (unsigned)std::min<long long>(blocks_for(4ll * s->tri_capacity), 1024ll);
+    if (kinds) k_surface_depth<<<grid, GTS_BLOCK, 0, st>>>(s->tri_capacity, s->counts, s->out_pos, shapes, kinds, depth);
     HIP_TRY(hipGetLastError());
     return GTS_OK;
 }
