@@ -51,7 +51,7 @@ class GtsStepParams(ct.Structure):
     _fields_ = [("dt", F), ("substeps", I), ("iterations", I), ("tet_mode", I), ("omega", F), ("edge_ks", F),
                 ("young", F), ("beta_s", F), ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F),
                 ("damping", F), ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F),
-                ("contact_radius", F), ("jaw_gap_closed", F), ("contact_compliance", F)]
+                ("contact_radius", F), ("jaw_gap_closed", F), ("contact_compliance", F), ("precise_strain", I)]
 
 
 class GtsToolState(ct.Structure):

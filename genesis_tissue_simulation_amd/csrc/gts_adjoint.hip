@@ -204,8 +204,10 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P,
                         ltt[T + k] = lamH;
                     }
                 }
-                if (tet_project<1, false>(S, s.prev, id, s.rec->inv_rest[type], in.ad, in.ah, lamD, lamH, 1.f, 0.f,
-                                          0.f) &&
+                f3 rk[3];
+                for (int q = 0; q < 3; q++) rk[q] = kuhn_rest_edge(type, q + 1, s.rec->rest_len);
+                if (tet_project<1, false, true>(S, s.prev, id, s.rec->inv_rest[type], in.ad, in.ah, lamD, lamH, 1.f,
+                                                0.f, 0.f, rk) &&
                     write) {
                     s.lam_t[k] = lamD;
                     s.lam_t[T + k] = lamH;
@@ -223,10 +225,18 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P,
     __syncthreads();
 }
 
-// Reverse of the tet projection (mode 1, no damping, omega = 1) at input positions p and multipliers lamD / lamH.
-// lDb / lHb hold the adjoints of the output multipliers on entry and of the input multipliers on return.
-__device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, float ad, float ah, float lamD,
-                            float lamH, const f3 ob[4], f3 pb[4], float& adb, float& ahb, float& lDb, float& lHb) {
+// Reverse of the tet projection (mode 1, no damping, omega = 1, precise invariants with rest edges rk) at input
+// positions p and multipliers lamD / lamH. lDb / lHb hold the adjoints of the output multipliers on entry and of the
+// input multipliers on return.
+//
+// The deviatoric row at its clamp: where q = J^(-2/3) I1 - 3 is at most the 1e-12 floor of C_D = sqrt(max(q, 1e-12)),
+// the forward divides the direction Ghat Dm^-T by the constant 1e-6, so an O(1) change of F moves the output by
+// O(1e6) times the (tiny) step. That derivative is exact but meaningless, and in FP32 it overflows within a few
+// substeps. The subgradient taken there holds the row's value and direction constant: no adjoint flows from C_D or
+// from the direction into F. Above the floor, the q path enters F as 2 qb Ghat, Ghat evaluated precisely.
+__device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, const f3 rk[3], float ad, float ah,
+                            float lamD, float lamH, const f3 ob[4], f3 pb[4], float& adb, float& ahb, float& lDb,
+                            float& lHb) {
     f3 br[3] = {mk3(Bm[0], Bm[1], Bm[2]), mk3(Bm[3], Bm[4], Bm[5]), mk3(Bm[6], Bm[7], Bm[8])};
     f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
     f3 F[3];
@@ -241,17 +251,21 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, fl
     float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
     float pJ = fmaxf(J, 0.05f);
     float iso = iso_pow(pJ);
-    float qraw = iso * I1 - 3.f;
+    bool jlive = J > 0.05f;
+    f3 d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+    PreciseStrain st = precise_strain(d, br);
+    f3 Gh[3];
+    float qraw = precise_dev(st, F, cof, jlive, pJ, iso, Gh);
     float q = fmaxf(qraw, 0.f);
     float CD = sqrtf(fmaxf(q, 1e-12f));
     float ds = 1.f / CD;
-    bool jlive = J > 0.05f;
+    const bool frozen = !(q > 1e-12f);
     float vg = jlive ? iso * I1 / (3.f * pJ) : 0.f;
-    float CH = J - 1.f;
+    float CH = st.a;
     for (int k = 1; k < 4; k++) {
         f3 b = br[k - 1];
         Fb[k] = F[0] * b.x + F[1] * b.y + F[2] * b.z;
-        G[k] = Fb[k] * iso - gh[k] * vg;
+        G[k] = Gh[0] * b.x + Gh[1] * b.y + Gh[2] * b.z;
         gd[k] = G[k] * ds;
     }
     gd[0] = (gd[1] + gd[2] + gd[3]) * -1.f;
@@ -313,15 +327,16 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, fl
         ghb[k] = ghb[k] - ghb[0];
     }
     float dsb = 0.f, isob = 0.f, vgb = 0.f;
-    f3 Fbb[4];
-    for (int k = 1; k < 4; k++) {
-        f3 Gb = gdb[k] * ds;
-        dsb += dot3(gdb[k], G[k]);
-        Fbb[k] = Gb * iso;
-        isob += dot3(Gb, Fb[k]);
-        ghb[k] = ghb[k] - Gb * vg;
-        vgb += -dot3(Gb, gh[k]);
-    }
+    f3 Fbb[4] = {mk3(0, 0, 0), mk3(0, 0, 0), mk3(0, 0, 0), mk3(0, 0, 0)};
+    if (!frozen)
+        for (int k = 1; k < 4; k++) {
+            f3 Gb = gdb[k] * ds;
+            dsb += dot3(gdb[k], G[k]);
+            Fbb[k] = Gb * iso;
+            isob += dot3(Gb, Fb[k]);
+            ghb[k] = ghb[k] - Gb * vg;
+            vgb += -dot3(Gb, gh[k]);
+        }
     f3 Fbar[3] = {mk3(0, 0, 0), mk3(0, 0, 0), mk3(0, 0, 0)};
     for (int k = 1; k < 4; k++) {
         f3 b = br[k - 1];
@@ -330,12 +345,10 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, fl
         Fbar[2] = Fbar[2] + Fbb[k] * b.z;
     }
     CDb += -dsb / (CD * CD);
-    float qb = q > 1e-12f ? CDb / (2.f * CD) : 0.f;
+    float qb = frozen ? 0.f : CDb / (2.f * CD);
+    if (!frozen)
+        for (int c = 0; c < 3; c++) Fbar[c] = Fbar[c] + Gh[c] * (2.f * qb);
     float I1b = 0.f;
-    if (qraw > 0.f) {
-        isob += qb * I1;
-        I1b += qb * iso;
-    }
     float pJb = 0.f;
     if (jlive) {
         isob += vgb * I1 / (3.f * pJ);
@@ -459,7 +472,9 @@ __device__ void substep_reverse(const Inst& s, const float4* __restrict__ wsrc, 
                 }
                 float lamD = tapedLam ? ltt[k] : 0.f, lamH = tapedLam ? ltt[T + k] : 0.f;
                 float adb = s.abar[k], ahb = s.abar[T + k], lDb = s.lamb_t[k], lHb = s.lamb_t[T + k];
-                tet_reverse(p, w, s.rec->inv_rest[type], in.ad, in.ah, lamD, lamH, ob, pb, adb, ahb, lDb, lHb);
+                f3 rk[3];
+                for (int q = 0; q < 3; q++) rk[q] = kuhn_rest_edge(type, q + 1, s.rec->rest_len);
+                tet_reverse(p, w, s.rec->inv_rest[type], rk, in.ad, in.ah, lamD, lamH, ob, pb, adb, ahb, lDb, lHb);
                 for (int q = 0; q < 4; q++) X.st(id[q], make_float4(pb[q].x, pb[q].y, pb[q].z, 0.f));
                 s.abar[k] = adb;
                 s.abar[T + k] = ahb;

@@ -39,13 +39,13 @@ struct TetInputs {
 };
 
 struct BatchStep {
-    int substeps, iters, mode, n_items;
+    int substeps, iters, mode, n_items, precise;
     int sub_begin, sub_end;  // the substeps this launch runs
     float h, invH, invH2, omega, velScale, gx, gy, gz, dOverH, bOverH;
     int lds_nodes;
 };
 
-template <int MODE, bool DAMPED, class Store>
+template <int MODE, bool DAMPED, bool PRECISE, class Store>
 __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const BatchStep& P, const int* colours,
                              const uint32_t* __restrict__ edge_desc, const uint32_t* __restrict__ tet_desc,
                              const float* __restrict__ k_dev, const float* __restrict__ k_hyd,
@@ -128,9 +128,12 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                     int base = (int)(t.d >> 4), type = (int)((t.d >> 1) & 7u);
                     int id[4];
                     for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), nx, nxy);
+                    f3 rk[3];
+                    if (PRECISE)
+                        for (int q = 0; q < 3; q++) rk[q] = kuhn_rest_edge(type, q + 1, I.rest_len);
                     float lamD = t.lamD, lamH = t.lamH;
-                    if (tet_project<MODE, DAMPED>(S, prev, id, I.inv_rest[type], t.ad, t.ah, lamD, lamH, P.omega,
-                                                  P.dOverH, P.bOverH) &&
+                    if (tet_project<MODE, DAMPED, PRECISE>(S, prev, id, I.inv_rest[type], t.ad, t.ah, lamD, lamH,
+                                                           P.omega, P.dOverH, P.bOverH, rk) &&
                         write) {
                         lam_tet[k] = lamD;
                         lam_tet[T + k] = lamH;
@@ -150,7 +153,7 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
     }
 }
 
-template <int MODE, bool DAMPED>
+template <int MODE, bool DAMPED, bool PRECISE>
 __global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBatchInstance* __restrict__ instances,
                                                        const int* __restrict__ colour_table,
                                                        const int* __restrict__ work_offsets,
@@ -183,14 +186,14 @@ __global__ void __launch_bounds__(1024) k_batched_step(BatchStep P, const GtsBat
         for (int i = threadIdx.x; i < staged; i += blockDim.x) lds_nodes[i] = gpos[i];
         __syncthreads();
         if (I.n_nodes <= P.lds_nodes)
-            run_instance<MODE, DAMPED>(LdsNodes{lds_nodes}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass, gprev,
-                                       gvel);
+            run_instance<MODE, DAMPED, PRECISE>(LdsNodes{lds_nodes}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass,
+                                                gprev, gvel);
         else if (staged > 0)
-            run_instance<MODE, DAMPED>(SplitNodes{lds_nodes, gpos, staged}, I, P, colours, ed, td, kd, kh, le, lt, al,
-                                       gmass, gprev, gvel);
+            run_instance<MODE, DAMPED, PRECISE>(SplitNodes{lds_nodes, gpos, staged}, I, P, colours, ed, td, kd, kh, le,
+                                                lt, al, gmass, gprev, gvel);
         else
-            run_instance<MODE, DAMPED>(GlobalNodes{gpos}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass, gprev,
-                                       gvel);
+            run_instance<MODE, DAMPED, PRECISE>(GlobalNodes{gpos}, I, P, colours, ed, td, kd, kh, le, lt, al, gmass,
+                                                gprev, gvel);
         for (int i = threadIdx.x; i < staged; i += blockDim.x) gpos[i] = lds_nodes[i];
         __syncthreads();
     }
@@ -203,10 +206,10 @@ struct BatchGraph {
 std::vector<BatchGraph> g_batch_graphs;
 hipStream_t g_batch_capture = nullptr;
 
-template <int MODE, bool DAMPED>
+template <int MODE, bool DAMPED, bool PRECISE>
 int launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
     size_t lds = (size_t)v->lds_nodes * sizeof(float4);
-    auto kern = k_batched_step<MODE, DAMPED>;
+    auto kern = k_batched_step<MODE, DAMPED, PRECISE>;
     hipError_t e =
         hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return -(int)e;
@@ -217,15 +220,20 @@ int launch(const GtsBatchView* v, const BatchStep& P, hipStream_t s) {
     return -(int)hipGetLastError();
 }
 
-int launch_range(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
+template <bool PRECISE>
+int launch_mode(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
     if (damped) {
-        if (P.mode == 0) return launch<0, true>(v, P, s);
-        if (P.mode == 1) return launch<1, true>(v, P, s);
-        return launch<2, true>(v, P, s);
+        if (P.mode == 0) return launch<0, true, PRECISE>(v, P, s);
+        if (P.mode == 1) return launch<1, true, PRECISE>(v, P, s);
+        return launch<2, true, PRECISE>(v, P, s);
     }
-    if (P.mode == 0) return launch<0, false>(v, P, s);
-    if (P.mode == 1) return launch<1, false>(v, P, s);
-    return launch<2, false>(v, P, s);
+    if (P.mode == 0) return launch<0, false, PRECISE>(v, P, s);
+    if (P.mode == 1) return launch<1, false, PRECISE>(v, P, s);
+    return launch<2, false, PRECISE>(v, P, s);
+}
+
+int launch_range(const GtsBatchView* v, const BatchStep& P, bool damped, hipStream_t s) {
+    return P.precise ? launch_mode<true>(v, P, damped, s) : launch_mode<false>(v, P, damped, s);
 }
 
 // The step as persistent launches of substeps_per_launch substeps each (one launch when 0).
@@ -323,6 +331,7 @@ extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, v
     if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;
     P.iters = p->iterations > 0 ? p->iterations : 1;
     P.mode = p->tet_mode;
+    P.precise = p->precise_strain != 0;
     P.invH = 1.f / P.h;
     P.invH2 = 1.f / (P.h * P.h);
     P.omega = p->omega;

@@ -69,6 +69,112 @@ GTS_HD float own_log(float x) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Precise strain invariants (no reference counterpart; selected by GtsStepParams.precise_strain).
+//
+// The reference forms J^(-2/3) I1 - 3 and J - 1 from F itself. In FP32 both then carry a rounding noise of about one
+// ulp of 3 (2.4e-7) whatever the strain, which is larger than the deviatoric q = O(strain^2) below strains of ~5e-4:
+// C_D = sqrt(q) and its gradient direction become noise. The same quantities are formed here from the displacement
+// gradient H = F - I = (e_k - r_k) Dm^-1, r_k the rest edges of the Kuhn type (e_k - r_k is exact where an edge is
+// near its rest vector), with every cancellation of lower-order terms carried out symbolically:
+//   a = J - 1 = tr H + e,  e = tr cof H + det H          b = I1 - 3 = 2 tr H + |H|^2
+//   u = J^(-2/3) - 1 = -2a/3 + u2,  u2 = 5a^2/9 + u3     (u3: the binomial series from a^3 on)
+//   q = 3u + b + ub = 2 |dev sym H|^2 + 2 tr H e + 5e^2/3 - 2 det H + 3 u3 - 2a|H|^2/3 + u2 b
+// (the O(H) terms of 3u + b and the O(H^2) terms besides 2 |dev sym H|^2 cancel identically), and
+//   Ghat = J^(-2/3) F - (J^(-2/3) I1 / 3J) cof F = J^(-2/3) [H + H^T - tr H I - cof H - m cof F],  m = (b - 3a) / 3J
+// using cof(I + H) = (1 + tr H) I - H^T + cof H. Ghat is half the gradient of q in F; the deviatoric constraint
+// gradient of corner k is Ghat Dm^-T row k over C_D. The residual Dm Dm^-1 - I of the float Dm^-1 (~1e-8) is left
+// out: it is below the resolution of F in FP32.
+// ---------------------------------------------------------------------------------------------------------------
+
+// u = (1 + a)^(-2/3) - 1 with its tails u2 = u + 2a/3 and u3 = u2 - 5a^2/9: the binomial series for |a| <= 1/16 (to
+// a^8; the first omitted term is below 2e-10 of u), else -a r^3 (1 + r) / (1 + r + r^2) with r = (1 + a)^(-1/3) =
+// sqrt(iso), from r^2 - 1 = (r - 1)(r + 1) and 1 - r^3 = a r^3. `iso` is J^(-2/3) of the same J = 1 + a.
+GTS_HD float iso_minus_one(float a, float iso, float& u2, float& u3) {
+    if (fabsf(a) <= 0.0625f) {
+        float s = 0.36418906f;           // 21505/59049: binomial(-2/3, 8), then down to binomial(-2/3, 3)
+        s = -0.38002336f + a * s;        // -7480/19683
+        s = 0.39902455f + a * s;         // 2618/6561
+        s = -0.42249656f + a * s;        // -308/729
+        s = 0.4526749f + a * s;          // 110/243
+        s = -0.49382716f + a * s;        // -40/81
+        u3 = a * a * a * s;
+        u2 = 0.5555556f * a * a + u3;    // 5/9
+        return -0.6666667f * a + u2;     // -2/3
+    }
+    float r = sqrtf(iso);
+    float u = -a * iso * r * (1.f + r) / (1.f + r + iso);
+    u2 = u + 0.6666667f * a;
+    u3 = u2 - 0.5555556f * a * a;
+    return u;
+}
+
+// log(1 + a) without the rounding of 1 + a: 2 atanh(s), s = a / (2 + a), for |a| < 1/4 (|s| < 1/7, series to s^11),
+// else own_log(1 + a).
+GTS_HD float own_log1p(float a) {
+    if (fabsf(a) < 0.25f) {
+        float s = a / (2.f + a);
+        float s2 = s * s;
+        float p = s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * (0.111111112f + s2 * 0.0909090936f))));
+        return 2.f * s + 2.f * s * p;
+    }
+    return own_log(1.f + a);
+}
+
+struct PreciseStrain {
+    f3 H[3], cH[3];                 // columns of H and of cof H
+    float trH, e, det, nH2, a, b;   // tr H, tr cof H + det H, det H, |H|^2, J - 1, I1 - 3
+};
+
+// H = d Dm^-1 from the edge displacements d_k = e_k - r_k (columns built as F is: H[c] = sum_k d_k (Dm^-1)_kc).
+GTS_HD PreciseStrain precise_strain(const f3 d[3], const f3 br[3]) {
+    PreciseStrain s;
+    for (int c = 0; c < 3; c++) s.H[c] = d[0] * comp3(br[0], c) + d[1] * comp3(br[1], c) + d[2] * comp3(br[2], c);
+    s.trH = s.H[0].x + s.H[1].y + s.H[2].z;
+    s.cH[0] = cross3(s.H[1], s.H[2]);
+    s.cH[1] = cross3(s.H[2], s.H[0]);
+    s.cH[2] = cross3(s.H[0], s.H[1]);
+    s.det = dot3(s.H[0], s.cH[0]);
+    s.e = (s.cH[0].x + s.cH[1].y + s.cH[2].z) + s.det;
+    s.a = s.trH + s.e;
+    s.nH2 = dot3(s.H[0], s.H[0]) + dot3(s.H[1], s.H[1]) + dot3(s.H[2], s.H[2]);
+    s.b = 2.f * s.trH + s.nH2;
+    return s;
+}
+
+// q = J^(-2/3) I1 - 3 (unclamped) and the columns of Ghat at pJ = max(J, .05), iso = pJ^(-2/3). Below the J clamp
+// (jlive false) pJ is a constant and iso - 1 is far from 0: q = 3u + b + ub with u = iso - 1, Ghat = iso F as the
+// reference's gradient there.
+GTS_HD float precise_dev(const PreciseStrain& s, const f3 F[3], const f3 cof[3], bool jlive, float pJ, float iso,
+                         f3 Gh[3]) {
+    if (!jlive) {
+        float u = iso - 1.f;
+        for (int c = 0; c < 3; c++) Gh[c] = F[c] * iso;
+        return 3.f * u + s.b + u * s.b;
+    }
+    float u2, u3;
+    iso_minus_one(s.a, iso, u2, u3);
+    const float t3 = s.trH / 3.f;
+    float d0 = s.H[0].x - t3, d1 = s.H[1].y - t3, d2 = s.H[2].z - t3;
+    float s01 = s.H[1].x + s.H[0].y, s02 = s.H[2].x + s.H[0].z, s12 = s.H[2].y + s.H[1].z;
+    float q2 = 2.f * (d0 * d0 + d1 * d1 + d2 * d2) + (s01 * s01 + s02 * s02 + s12 * s12);
+    float rest = 2.f * s.trH * s.e + 1.6666666f * s.e * s.e - 2.f * s.det + 3.f * u3 - 0.6666667f * s.a * s.nH2 +
+                 u2 * s.b;
+    float m = (s.b - 3.f * s.a) / (3.f * pJ);
+    for (int c = 0; c < 3; c++) {
+        f3 HT = mk3(comp3(s.H[0], c), comp3(s.H[1], c), comp3(s.H[2], c));
+        f3 ec = mk3(c == 0 ? 1.f : 0.f, c == 1 ? 1.f : 0.f, c == 2 ? 1.f : 0.f);
+        Gh[c] = (s.H[c] + HT - ec * s.trH - s.cH[c] - cof[c] * m) * iso;
+    }
+    return q2 + rest;
+}
+
+// Rest edge k (1..3) of a Kuhn type: the bit offsets of its corner times the axis rest lengths.
+GTS_HD f3 kuhn_rest_edge(int type, int k, const float* rest_len) {
+    int c = kuhn_corner(type, k);
+    return mk3((c & 1) ? rest_len[0] : 0.f, ((c >> 1) & 1) ? rest_len[1] : 0.f, ((c >> 2) & 1) ? rest_len[2] : 0.f);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Tool shapes and the swept node contact (reference: contact/friction_contact.cuh:17-169).
 // ---------------------------------------------------------------------------------------------------------------
 struct Shape {

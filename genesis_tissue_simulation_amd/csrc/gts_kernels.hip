@@ -193,7 +193,7 @@ __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fal
 
 // k_solve_tet (xpbd_solver.cu:749-829) over one colour. MODE 0 keeps the hydrostatic row alone; 1 and 2 are the
 // coupled 2x2 block, legacy and guarded.
-template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED>
+template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED, bool PRECISE>
 __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float dampingOverH, float bulkDampingOverH,
                        const uint32_t* __restrict__ tet_desc, const float* __restrict__ alpha,
                        float* __restrict__ lambda, float4* __restrict__ pos, const float4* __restrict__ prev) {
@@ -205,9 +205,12 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
     int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
     int id[4];
     for (int c = 0; c < 4; c++) id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);
+    f3 rk[3];
+    if (PRECISE)
+        for (int c = 0; c < 3; c++) rk[c] = kuhn_rest_edge(type, c + 1, L.rest_len);
     float lamD = READ_LAMBDA ? lambda[e] : 0.f, lamH = READ_LAMBDA ? lambda[T + e] : 0.f;
-    if (tet_project<MODE, DAMPED>(GlobalNodes{pos}, prev, id, L.inv_rest[type], alpha[e], alpha[T + e], lamD, lamH,
-                                  omega, dampingOverH, bulkDampingOverH) &&
+    if (tet_project<MODE, DAMPED, PRECISE>(GlobalNodes{pos}, prev, id, L.inv_rest[type], alpha[e], alpha[T + e], lamD,
+                                           lamH, omega, dampingOverH, bulkDampingOverH, rk) &&
         WRITE_LAMBDA) {
         lambda[e] = lamD;
         lambda[T + e] = lamH;
@@ -990,15 +993,15 @@ int launch_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
 #define GTS_COLOUR_BLOCK 64
 inline int colour_block(const GtsTissueView* v) { return v->colour_block > 0 ? v->colour_block : GTS_COLOUR_BLOCK; }
 
-template <int MODE, bool DAMPED>
+template <int MODE, bool DAMPED, bool PRECISE>
 void launch_tet_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
                          float omega, float dOverH, float bOverH) {
     const int cb = colour_block(v);
     dim3 g(grid_for(cnt, cb)), b(cb);
     const bool read = it > 0, write = iters > 1;
-#define GTS_TET(R, W)                                                                                            \
-    profiled_launch(k_tets<MODE, R, W, DAMPED>, g, b, s, cnt, off, cnt, v->n_tets, L, omega, dOverH, bOverH,     \
-                    v->tet_desc, v->tet_alpha, v->tet_lambda, (float4*)v->pos, (const float4*)v->prev)
+#define GTS_TET(R, W)                                                                                             \
+    profiled_launch(k_tets<MODE, R, W, DAMPED, PRECISE>, g, b, s, cnt, off, cnt, v->n_tets, L, omega, dOverH,     \
+                    bOverH, v->tet_desc, v->tet_alpha, v->tet_lambda, (float4*)v->pos, (const float4*)v->prev)
     if (read && write) GTS_TET(true, true);
     else if (read) GTS_TET(true, false);
     else if (write) GTS_TET(false, true);
@@ -1008,9 +1011,12 @@ void launch_tet_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s
 
 template <int MODE>
 void launch_tet_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
-                       float omega, float dOverH, float bOverH) {
-    if (dOverH != 0.f || bOverH != 0.f) launch_tet_colour_d<MODE, true>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
-    else launch_tet_colour_d<MODE, false>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
+                       float omega, float dOverH, float bOverH, bool precise) {
+    const bool damped = dOverH != 0.f || bOverH != 0.f;
+    if (damped && precise) launch_tet_colour_d<MODE, true, true>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
+    else if (damped) launch_tet_colour_d<MODE, true, false>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
+    else if (precise) launch_tet_colour_d<MODE, false, true>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
+    else launch_tet_colour_d<MODE, false, false>(v, L, s, off, cnt, it, iters, omega, dOverH, bOverH);
 }
 
 template <bool DAMPED>
@@ -1166,7 +1172,7 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     int Nsub, iters, mode;
     float h, at, velScale, betaGammaH, invH2, fb, dOverH, bOverH, alphaC, scaleR, invH, omega, beta_s;
     float mu_s, mu_d, radius;
-    bool strain, fallback, edges, contact, perp;
+    bool strain, fallback, edges, contact, perp, precise;
     uint32_t armed;
     // legacy contact model: K material passes (2 while grasping), the grasp and CCD gate kernels
     bool legacy, grasping, gate;
@@ -1228,6 +1234,7 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
     P.bOverH = h > 0.f ? p->bulk_damping / h : 0.f;
     P.iters = p->iterations > 0 ? p->iterations : 1;
     P.mode = p->tet_mode;
+    P.precise = p->precise_strain != 0;
     P.omega = p->omega;
     P.beta_s = p->beta_s;
     P.perp = p->beta_s > 0.f;
@@ -1263,9 +1270,12 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
                     for (int c = 0; c < v->n_tet_colours; c++) {
                         int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
                         if (cnt <= 0) continue;
-                        if (P.mode == 0) launch_tet_colour<0>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH);
-                        else if (P.mode == 1) launch_tet_colour<1>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH);
-                        else launch_tet_colour<2>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH);
+                        if (P.mode == 0)
+                            launch_tet_colour<0>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH, P.precise);
+                        else if (P.mode == 1)
+                            launch_tet_colour<1>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH, P.precise);
+                        else
+                            launch_tet_colour<2>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH, P.precise);
                     }
             }
             if (P.legacy && P.grasping) k_grasp_pass<<<g, b, 0, s>>>(N, frames + n, P.h, P.grasp, (float4*)v->pos);

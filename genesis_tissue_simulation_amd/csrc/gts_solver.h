@@ -58,11 +58,14 @@ __device__ __forceinline__ bool edge_project(const Store& S, const float4* __res
 }
 
 // k_solve_tet (xpbd/xpbd_solver.cu:749-829) on one tet with nodes id[4]. MODE 0 keeps the hydrostatic row alone;
-// 1 and 2 are the coupled 2x2 block, legacy and guarded. lamD / lamH are read and updated.
-template <int MODE, bool DAMPED, class Store>
+// 1 and 2 are the coupled 2x2 block, legacy and guarded. lamD / lamH are read and updated. PRECISE evaluates J - 1,
+// log J, J^(-2/3) I1 - 3 and the deviatoric gradient from the displacement gradient (precise_strain, precise_dev)
+// with rk the tet type's rest edges; otherwise the reference's direct forms, and rk is not read.
+template <int MODE, bool DAMPED, bool PRECISE = false, class Store>
 __device__ __forceinline__ bool tet_project(const Store& S, const float4* __restrict__ prev, const int id[4],
                                             const float* B, float ad, float ah, float& lamD, float& lamH,
-                                            float omega, float dampingOverH, float bulkDampingOverH) {
+                                            float omega, float dampingOverH, float bulkDampingOverH,
+                                            const f3* rk = nullptr) {
     float4 q[4];
     for (int c = 0; c < 4; c++) q[c] = S.ld(id[c]);
     f3 p[4] = {xyz4(q[0]), xyz4(q[1]), xyz4(q[2]), xyz4(q[3])};
@@ -81,8 +84,13 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
     // Undamped instantiations run only with both damping terms zero: the scales are then exactly 1 and the divisions
     // by them and the damping terms drop out without changing a bit.
     float scale = DAMPED ? 1.f + dampingOverH : 1.f;
+    PreciseStrain st;
+    if (PRECISE) {
+        f3 d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+        st = precise_strain(d, br);
+    }
     if (MODE == 0) {
-        float CH = J - 1.f;
+        float CH = PRECISE ? st.a : J - 1.f;
         gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
         float bulkScale = scale;
         float effectiveAh = ah / bulkScale;
@@ -106,14 +114,21 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
     float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
     float positiveJ = fmaxf(J, 0.05f);
     float iso = iso_pow(positiveJ);
-    float qq = fmaxf(iso * I1 - 3.f, 0.f);
+    const bool jlive = J > 0.05f;
+    f3 Gh[3];
+    float qq = fmaxf(PRECISE ? precise_dev(st, F, cof, jlive, positiveJ, iso, Gh) : iso * I1 - 3.f, 0.f);
     float CD = sqrtf(fmaxf(qq, 1e-12f));
     float devScale = 1.f / CD;
-    float volumeGradient = J > 0.05f ? iso * I1 / (3.f * positiveJ) : 0.f;
-    float CH = safeguarded ? own_log(positiveJ) + fminf(0.f, (J - 0.05f) / 0.05f) : J - 1.f;
+    float volumeGradient = jlive ? iso * I1 / (3.f * positiveJ) : 0.f;
+    float CH;
+    if (safeguarded)
+        CH = (PRECISE && jlive ? own_log1p(st.a) : own_log(positiveJ)) + fminf(0.f, (J - 0.05f) / 0.05f);
+    else
+        CH = PRECISE ? st.a : J - 1.f;
     for (int c = 1; c < 4; c++) {
         f3 b = br[c - 1];
-        gd[c] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[c] * volumeGradient) * devScale;
+        if (PRECISE) gd[c] = (Gh[0] * b.x + Gh[1] * b.y + Gh[2] * b.z) * devScale;
+        else gd[c] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[c] * volumeGradient) * devScale;
         if (safeguarded) gh[c] = gh[c] * (1.f / positiveJ);
     }
     gd[0] = (gd[1] + gd[2] + gd[3]) * -1.f;

@@ -35,6 +35,7 @@ class SolverParams:
     contact_radius: float = 0.01
     jaw_gap_closed: float = 0.0
     contact_compliance: float = 1e-8
+    precise_strain: bool = False  # tet invariants from F - I (closer to FP64); False: the reference's direct forms
 
     def with_(self, **changes):
         return replace(self, **changes)

@@ -60,7 +60,7 @@ def step_params(params: SolverParams, dt):
                          float(p.edge_ks), float(p.young), float(p.beta_s), float(p.strain_damping),
                          float(p.bulk_damping), float(p.rayleigh), float(p.damping), int(bool(p.contact_enabled)),
                          float(p.mu_s), float(p.mu_d), float(p.jaw_thickness), float(p.contact_radius),
-                         float(p.jaw_gap_closed), float(p.contact_compliance))
+                         float(p.jaw_gap_closed), float(p.contact_compliance), int(bool(p.precise_strain)))
 
 
 def build_topology(grid):

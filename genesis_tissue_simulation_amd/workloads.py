@@ -70,14 +70,16 @@ def jittered(grid, rng, sigma):
     return pos
 
 
-def block_6144(seed=0, spacing=0.25, steps=200, substeps=10, iterations=1):
+def block_6144(seed=0, spacing=0.25, steps=200, substeps=10, iterations=1, precise=False):
     """The CPU-runnable block: 17x9x9 nodes, 6144 tets, edge compliance 1e-7, volume compliance 1e-8, bottom face
     pinned, dt = 1/60 with 10 substeps of 1 iteration, 200 steps; a planar blade of half-width 2 cells at mid-height
-    sweeps +x at half a cell per step. Initial positions carry seeded N(0, 1e-4 cell) jitter."""
+    sweeps +x at half a cell per step. Initial positions carry seeded N(0, 1e-4 cell) jitter. `precise`: the tet
+    invariants formed from F - I instead of the reference's direct forms."""
     rng = np.random.default_rng(seed)
     grid = block_grid((17, 9, 9), spacing)
     L = spacing
-    params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7)
+    params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7,
+                          precise_strain=precise)
     T = tet_count(grid)
     zc = 4 * L
 
@@ -102,11 +104,12 @@ def ellipsoid_spacing_for(tets, half_extents=(6.0, 3.0, 2.5)):
     return 0.5 * (lo + hi)
 
 
-def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets=250_000):
+def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets=250_000, precise=False):
     """Kidney-scale single instance: an ellipsoid of semi-axes 6 x 3 x 2.5 cm rasterised at the spacing that gives
     about 250k tets (1.72 mm; a 0.5 mm lattice would give 9M); per-tet compliance log-normal (median 1e-7,
     sigma 0.5); a capsule grasper of radius 2 mm pressing 3 mm into the top; a blade on a seeded cubic Bezier over
-    the top surface cutting for 300 steps; 20 substeps of 2 iterations. Gravity ramps in over 0.5 s as in
+    the top surface cutting for 300 steps; 20 substeps of 2 iterations (`precise`: the tet invariants formed from
+    F - I instead of the reference's direct forms). Gravity ramps in over 0.5 s as in
     NativeTissue.step: applied at once, the 30-cell-tall lattice falls onto its pinned bed faster than two
     Gauss-Seidel sweeps per substep can carry the support up, and the bottom tets invert (in FP64 as in FP32)."""
     rng = np.random.default_rng(seed)
@@ -116,7 +119,7 @@ def kidney_250k(seed=0, spacing=None, steps=300, substeps=20, iterations=2, tets
     T = tet_count(grid)
     compliance = np.exp(rng.normal(np.log(1e-7), 0.5, T))
     params = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7,
-                          contact_enabled=True, contact_radius=0.0, jaw_thickness=0.1)
+                          contact_enabled=True, contact_radius=0.0, jaw_thickness=0.1, precise_strain=precise)
     top = 3.0
     # grasper capsule along x above the top, lowered 3 mm into the surface over the first 60 steps
     r = 0.2
@@ -243,12 +246,14 @@ def straight_cut(grid, rng, depth_fraction=0.4, thickness=0.02):
     return swept_cut_tool(seg(start), seg(end), thickness)
 
 
-def batched_blocks(count=1024, seed=0, tmin=2000, tmax=100_000, spacing=0.25, substeps=10, iterations=1):
+def batched_blocks(count=1024, seed=0, tmin=2000, tmax=100_000, spacing=0.25, substeps=10, iterations=1,
+                   precise=False):
     """Many independent blocks: tet counts log-uniform in [tmin, tmax], 2:1:1 cells, per-instance compliance
     log-uniform in [1e-8, 1e-6] (edges and tets), one seeded straight cut per instance across the top face; a
-    10-substep step of one iteration at dt = 1/60."""
+    10-substep step of one iteration at dt = 1/60 (`precise`: the tet invariants formed from F - I)."""
     rng = np.random.default_rng(seed)
-    base = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7)
+    base = SolverParams(substeps=substeps, iterations=iterations, tet_mode=TET_NEOHOOKEAN, edge_ks=1e7,
+                        precise_strain=precise)
     instances, cuts = [], []
     for _ in range(count):
         t = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))

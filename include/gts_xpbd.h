@@ -137,6 +137,11 @@ typedef struct GtsStepParams {
     float omega, edge_ks, young, beta_s, strain_damping, bulk_damping, rayleigh, damping;
     int contact_enabled;
     float mu_s, mu_d, jaw_thickness, contact_radius, jaw_gap_closed, contact_compliance;
+    /* 0: the tet constraints are evaluated in the reference's direct forms (k_solve_tet, xpbd_solver.cu:749-829), so
+     * results match an FP32 restatement of the reference bit for bit. 1: J - 1, log J, J^(-2/3) I1 - 3 and the
+     * deviatoric gradient are formed from the displacement gradient F - I (no reference counterpart), which removes
+     * their FP32 cancellation at small strain: the same constraints, closer to FP64. */
+    int precise_strain;
 } GtsStepParams;
 
 /* XpbdToolState (xpbd_solver.h:223-229) plus a blade capsule. */

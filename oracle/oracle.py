@@ -25,7 +25,8 @@ class OracleDesc(ct.Structure):
                 ("tet_mode", I), ("omega", F), ("edge_ks", F), ("young", F), ("beta_s", F),
                 ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F), ("damping", F),
                 ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F), ("contact_radius", F),
-                ("jaw_gap_closed", F), ("contact_compliance", F), ("tool_voxel", F), ("cut_point_capacity", I)]
+                ("jaw_gap_closed", F), ("contact_compliance", F), ("tool_voxel", F), ("cut_point_capacity", I),
+                ("precise_strain", I)]
 
 
 class OracleTool(ct.Structure):
@@ -149,7 +150,7 @@ class Oracle:
                        p.young, p.beta_s, p.strain_damping, p.bulk_damping, p.rayleigh, p.damping,
                        int(p.contact_enabled), p.mu_s, p.mu_d, p.jaw_thickness, p.contact_radius,
                        p.jaw_gap_closed, p.contact_compliance, float(grid.cell_size[0]),
-                       4 * len(grid.edges))
+                       4 * len(grid.edges), int(bool(getattr(p, "precise_strain", False))))
         self.desc = d
         rest = np.ascontiguousarray(grid.positions, dtype=np.float32)
         pos = rest if initial_positions is None else np.ascontiguousarray(initial_positions, dtype=np.float32)

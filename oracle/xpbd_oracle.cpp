@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -119,6 +120,7 @@ struct OracleDesc {
     float contact_compliance;  // contact compliance, scaled by 1/h^2
     float tool_voxel;          // cut: lattice spacing used as AABB pad and rest edge length (voxelL)
     int cut_point_capacity;
+    int precise_strain;        // FP32: tet invariants from F - I (the kernels' precise_strain); 0: the direct forms
 };
 
 // Tool pose (XpbdToolState, xpbd_solver.h:223-229) plus the optional blade capsule; 27 floats.
@@ -439,7 +441,6 @@ struct CutPoint {  // CutPointGpu (cut.cuh:37-46): 32 bytes, rotation taken as i
 template <class R> struct Tissue {
     OracleDesc d{};
     int N = 0, V = 0, E = 0, T = 0, nx = 0, ny = 0, nz = 0;
-    R sp[3] = {0, 0, 0};
     // node state (AoS, as the reference's float3 buffers)
     std::vector<V3<R>> pos, prev, vel, ext, rest;
     // particle frames (RotGpu rows r0, r1, r2 per node) and their previous values (physics.cu:76-77)
@@ -456,6 +457,7 @@ template <class R> struct Tissue {
     std::vector<int> tids, ttype, towner, tactive, tcolor, tcolorOff, tcolorCnt, tcolorOrder;
     std::vector<R> kDev, kHyd, alphaDev, alphaHyd, lamDev, lamHyd;
     R typeB[6][3][3];
+    V3<R> typeRk[6][3];
     double typeV[6];
     // frame constants (xpbd_begin_frame)
     R h = 0, alphaTilde = 0, velScale = 1, betaGammaH = 0;
@@ -704,14 +706,93 @@ template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallba
     s.pos[j] = s.pos[j] - n * (wj * applied);
 }
 
+
+
+// ---- precise tet invariants (no reference counterpart; OracleDesc.precise_strain). The FP32 path restates the
+// kernels' sequence (gts_common.h: iso_minus_one, own_log1p, precise_strain, precise_dev) op for op; the FP64 oracle
+// keeps the reference's direct forms, which FP64 resolves, and is the accuracy reference for both.
+// J - 1, I1 - 3, J^(-2/3) I1 - 3 and the deviatoric gradient Ghat = J^(-2/3) F - (J^(-2/3) I1 / 3J) cof F are formed
+// from H = F - I = (e_k - r_k) Dm^-1 so that their O(1) parts never cancel (see the kernels' header for the identities).
+inline float isoMinusOne(float a, float iso, float& u2, float& u3) {  // u = (1 + a)^(-2/3) - 1, u2, u3 its tails
+    if (std::fabs(a) <= 0.0625f) {  // binomial series to a^8
+        float s = 0.36418906f;
+        s = -0.38002336f + a * s;
+        s = 0.39902455f + a * s;
+        s = -0.42249656f + a * s;
+        s = 0.4526749f + a * s;
+        s = -0.49382716f + a * s;
+        u3 = a * a * a * s;
+        u2 = 0.5555556f * a * a + u3;
+        return -0.6666667f * a + u2;
+    }
+    float r = std::sqrt(iso);  // (1 + a)^(-1/3); r^2 - 1 = -a r^3 (1 + r) / (1 + r + r^2)
+    float u = -a * iso * r * (1.f + r) / (1.f + r + iso);
+    u2 = u + 0.6666667f * a;
+    u3 = u2 - 0.5555556f * a * a;
+    return u;
+}
+inline float log1pOwn(float a) {  // 2 atanh(a / (2 + a)) near 0, else the guarded log of 1 + a
+    if (std::fabs(a) < 0.25f) {
+        float s = a / (2.f + a);
+        float s2 = s * s;
+        float p = s2 * (0.333333343f + s2 * (0.2f + s2 * (0.142857149f + s2 * (0.111111112f + s2 * 0.0909090936f))));
+        return 2.f * s + 2.f * s * p;
+    }
+    return guardLog(1.f + a);
+}
+struct Strain32 {
+    V3<float> H[3], cH[3];
+    float trH, e, det, nH2, a, b;  // tr H, tr cof H + det H, det H, |H|^2, J - 1, I1 - 3
+};
+inline Strain32 preciseStrain(const V3<float> d[3], const V3<float> br[3]) {
+    Strain32 s;
+    for (int c = 0; c < 3; c++) s.H[c] = d[0] * comp(br[0], c) + d[1] * comp(br[1], c) + d[2] * comp(br[2], c);
+    s.trH = s.H[0].x + s.H[1].y + s.H[2].z;
+    s.cH[0] = cross3(s.H[1], s.H[2]);
+    s.cH[1] = cross3(s.H[2], s.H[0]);
+    s.cH[2] = cross3(s.H[0], s.H[1]);
+    s.det = dot3(s.H[0], s.cH[0]);
+    s.e = (s.cH[0].x + s.cH[1].y + s.cH[2].z) + s.det;
+    s.a = s.trH + s.e;
+    s.nH2 = dot3(s.H[0], s.H[0]) + dot3(s.H[1], s.H[1]) + dot3(s.H[2], s.H[2]);
+    s.b = 2.f * s.trH + s.nH2;
+    return s;
+}
+// q = J^(-2/3) I1 - 3 (unclamped) and the columns of Ghat, at pJ = max(J, .05) and iso = pJ^(-2/3):
+// q = 2 |dev sym H|^2 + 2 tr H e + 5e^2/3 - 2 det H + 3 u3 - 2a|H|^2/3 + u2 b above the J clamp.
+inline float preciseDev(const Strain32& s, const V3<float> F[3], const V3<float> cof[3], bool jlive, float pJ,
+                        float iso, V3<float> Gh[3]) {
+    if (!jlive) {
+        float u = iso - 1.f;
+        for (int c = 0; c < 3; c++) Gh[c] = F[c] * iso;
+        return 3.f * u + s.b + u * s.b;
+    }
+    float u2, u3;
+    isoMinusOne(s.a, iso, u2, u3);
+    const float t3 = s.trH / 3.f;
+    float d0 = s.H[0].x - t3, d1 = s.H[1].y - t3, d2 = s.H[2].z - t3;
+    float s01 = s.H[1].x + s.H[0].y, s02 = s.H[2].x + s.H[0].z, s12 = s.H[2].y + s.H[1].z;
+    float q2 = 2.f * (d0 * d0 + d1 * d1 + d2 * d2) + (s01 * s01 + s02 * s02 + s12 * s12);
+    float rest = 2.f * s.trH * s.e + 1.6666666f * s.e * s.e - 2.f * s.det + 3.f * u3 - 0.6666667f * s.a * s.nH2 +
+                 u2 * s.b;
+    float m = (s.b - 3.f * s.a) / (3.f * pJ);
+    for (int c = 0; c < 3; c++) {
+        V3<float> HT = mk(comp(s.H[0], c), comp(s.H[1], c), comp(s.H[2], c));
+        V3<float> ec = mk(c == 0 ? 1.f : 0.f, c == 1 ? 1.f : 0.f, c == 2 ? 1.f : 0.f);
+        Gh[c] = (s.H[c] + HT - ec * s.trH - s.cH[c] - cof[c] * m) * iso;
+    }
+    return q2 + rest;
+}
+
 // k_solve_tet (xpbd_solver.cu:749-829) on one tet. Mode 0 keeps the hydrostatic row alone (the deviatoric row and
 // its multiplier are absent, which is the 2x2 block with gd = 0); modes 1/2 are the coupled block, legacy / guarded.
 // k_solve_tet (xpbd_solver.cu:749-829) on one tet as a pure function of its inputs. Mode 0 keeps the hydrostatic row
 // alone (the deviatoric row and its multiplier are absent: the 2x2 block with gd = 0); modes 1/2 are the coupled block,
-// legacy / guarded. Returns false where the reference returns without writing.
+// legacy / guarded. Returns false where the reference returns without writing. With the rest edges rk of the tet's
+// type (FP32 only), the invariants take the precise forms above; without, the reference's direct forms.
 template <class R>
 bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], const V3<R> br[3], R ad, R ah,
-                R& lamD, R& lamH, R dampingOverH, R bulkDampingOverH, R omega, V3<R> out[4]) {
+                R& lamD, R& lamH, R dampingOverH, R bulkDampingOverH, R omega, V3<R> out[4], const V3<R>* rk = nullptr) {
     V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
     V3<R> F[3];
     for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
@@ -724,8 +805,20 @@ bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], con
         gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
     }
     R scale = (R)1 + dampingOverH;
+    // precise invariants: FP32 with the type's rest edges rk (the FP64 oracle passes none)
+    bool precise = false;
+    Strain32 st{};
+    if constexpr (std::is_same<R, float>::value) {
+        if (rk) {
+            precise = true;
+            V3<float> d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+            st = preciseStrain(d, br);
+        }
+    }
     if (mode == 0) {
         R CH = J - (R)1;
+        if constexpr (std::is_same<R, float>::value)
+            if (precise) CH = st.a;
         gh[0] = (gh[1] + gh[2] + gh[3]) * (R)-1;
         R bulkScale = scale;
         R effectiveAh = ah / bulkScale;
@@ -744,14 +837,23 @@ bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], con
     R I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
     R positiveJ = fmax_<R>(J, (R)0.05f);
     R iso = isoPow(positiveJ);
+    const bool jlive = J > (R)0.05f;
     R q = fmax_<R>(iso * I1 - (R)3, (R)0);
+    V3<R> Gh[3];
+    if constexpr (std::is_same<R, float>::value)
+        if (precise) q = fmax_<R>(preciseDev(st, F, cof, jlive, positiveJ, iso, Gh), (R)0);
     R CD = std::sqrt(fmax_<R>(q, (R)1e-12f));
     R devScale = (R)1 / CD;
-    R volumeGradient = J > (R)0.05f ? iso * I1 / ((R)3 * positiveJ) : (R)0;
+    R volumeGradient = jlive ? iso * I1 / ((R)3 * positiveJ) : (R)0;
     R CH = safeguarded ? guardLog(positiveJ) + fmin_<R>((R)0, (J - (R)0.05f) / (R)0.05f) : J - (R)1;
+    if constexpr (std::is_same<R, float>::value)
+        if (precise)
+            CH = safeguarded ? (jlive ? log1pOwn(st.a) : guardLog(positiveJ)) + fmin_<R>(0.f, (J - 0.05f) / 0.05f)
+                             : st.a;
     for (int k = 1; k < 4; k++) {
         V3<R> b = br[k - 1];
-        gd[k] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[k] * volumeGradient) * devScale;
+        if (precise) gd[k] = (Gh[0] * b.x + Gh[1] * b.y + Gh[2] * b.z) * devScale;
+        else gd[k] = ((F[0] * b.x + F[1] * b.y + F[2] * b.z) * iso - gh[k] * volumeGradient) * devScale;
         if (safeguarded) gh[k] = gh[k] * ((R)1 / positiveJ);
     }
     gd[0] = (gd[1] + gd[2] + gd[3]) * (R)-1;
@@ -801,7 +903,7 @@ template <class R> void solveTet(Tissue<R>& s, int e, R dampingOverH, R bulkDamp
     V3<R> br[3] = {mk<R>(B[0][0], B[0][1], B[0][2]), mk<R>(B[1][0], B[1][1], B[1][2]), mk<R>(B[2][0], B[2][1], B[2][2])};
     R lamD = useLambda ? s.lamDev[e] : (R)0, lamH = useLambda ? s.lamHyd[e] : (R)0;
     if (!tetProject<R>(s.d.tet_mode, p, pv, w, br, s.alphaDev[e], s.alphaHyd[e], lamD, lamH, dampingOverH,
-                       bulkDampingOverH, (R)s.d.omega, out))
+                       bulkDampingOverH, (R)s.d.omega, out, s.d.precise_strain ? s.typeRk[s.ttype[e]] : nullptr))
         return;
     s.lamDev[e] = lamD;
     s.lamHyd[e] = lamH;
@@ -1694,6 +1796,22 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         const float* r = restNbr + 3 * s->eowner[e];
         s->erest[e] = (R)std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
     }
+    // rest edges of the Kuhn types for the precise invariants: bit offsets times the rest length of each axis (the
+    // first edge's L0 along it, as the library's topology reports axis_rest_length; the lattice spacing if none)
+    float axisLen[3] = {(float)(((double)s->rest[s->cid(s->d.vx, 0, 0)].x - (double)s->rest[0].x) / s->d.vx),
+                        (float)(((double)s->rest[s->cid(0, s->d.vy, 0)].y - (double)s->rest[0].y) / s->d.vy),
+                        (float)(((double)s->rest[s->cid(0, 0, s->d.vz)].z - (double)s->rest[0].z) / s->d.vz)};
+    bool seen[3] = {false, false, false};
+    for (int e = 0; e < s->E; e++) {
+        int axis = (s->eowner[e] % 6) / 2;
+        if (!seen[axis]) { axisLen[axis] = (float)s->erest[e]; seen[axis] = true; }
+    }
+    for (int ti = 0; ti < 6; ti++)
+        for (int k = 0; k < 3; k++) {
+            int c = KUHN[ti][k + 1];
+            s->typeRk[ti][k] = mk<R>((c & 1) ? (R)axisLen[0] : (R)0, ((c >> 1) & 1) ? (R)axisLen[1] : (R)0,
+                                     ((c >> 2) & 1) ? (R)axisLen[2] : (R)0);
+        }
     return s;
 }
 
@@ -1868,6 +1986,48 @@ extern "C" int oracle_tet_project_f64(int mode, const double* p12, const double*
     for (int k = 0; k < 4; k++) { out12[3 * k] = out[k].x; out12[3 * k + 1] = out[k].y; out12[3 * k + 2] = out[k].z; }
     return ok ? 1 : 0;
 }
+// One tet projection in FP32 (rk9: the type's rest edges for the precise invariants, or null for the direct forms).
+extern "C" int oracle_tet_project_f32(int mode, const float* p12, const float* prev12, const float* w4,
+                                      const float* B9, const float* rk9, float ad, float ah, float* lam2,
+                                      float dampingOverH, float bulkOverH, float omega, float* out12) {
+    V3<float> p[4], pv[4], out[4], br[3], rk[3];
+    float w[4];
+    for (int k = 0; k < 4; k++) {
+        p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);
+        pv[k] = mk(prev12[3 * k], prev12[3 * k + 1], prev12[3 * k + 2]);
+        w[k] = w4[k];
+    }
+    for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);
+    if (rk9)
+        for (int r = 0; r < 3; r++) rk[r] = mk(rk9[3 * r], rk9[3 * r + 1], rk9[3 * r + 2]);
+    bool ok = tetProject<float>(mode, p, pv, w, br, ad, ah, lam2[0], lam2[1], dampingOverH, bulkOverH, omega, out,
+                                rk9 ? rk : nullptr);
+    for (int k = 0; k < 4; k++) { out12[3 * k] = out[k].x; out12[3 * k + 1] = out[k].y; out12[3 * k + 2] = out[k].z; }
+    return ok ? 1 : 0;
+}
+// The precise invariants of one tet in FP32: out = [J - 1, I1 - 3, q, Ghat columns (9)], the tet's corners p12, its
+// Dm^-1 rows B9 and rest edges rk9.
+extern "C" void oracle_precise_invariants_f32(const float* p12, const float* B9, const float* rk9, float* out) {
+    V3<float> p[4], br[3], d[3], F[3];
+    for (int k = 0; k < 4; k++) p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);
+    for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);
+    for (int k = 0; k < 3; k++) d[k] = (p[k + 1] - p[0]) - mk(rk9[3 * k], rk9[3 * k + 1], rk9[3 * k + 2]);
+    V3<float> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
+    for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
+    V3<float> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
+    float J = dot3(F[0], cof[0]), pJ = std::fmax(J, 0.05f);
+    Strain32 st = preciseStrain(d, br);
+    V3<float> Gh[3];
+    float q = preciseDev(st, F, cof, J > 0.05f, pJ, isoPow(pJ), Gh);
+    out[0] = st.a;
+    out[1] = st.b;
+    out[2] = q;
+    for (int c = 0; c < 3; c++) { out[3 + 3 * c] = Gh[c].x; out[4 + 3 * c] = Gh[c].y; out[5 + 3 * c] = Gh[c].z; }
+}
+extern "C" float oracle_iso_minus_one_f32(float a, float iso, float* tails) {
+    return isoMinusOne(a, iso, tails[0], tails[1]);
+}
+extern "C" float oracle_log1p_f32(float a) { return log1pOwn(a); }
 extern "C" int oracle_moller_trumbore_f64(const double* O, const double* D, const double* V, double* tuv) {
     double t, u, v;
     bool hit = mollerTrumbore<double>(mk(O[0], O[1], O[2]), mk(D[0], D[1], D[2]), mk(V[0], V[1], V[2]),
@@ -1912,6 +2072,9 @@ extern "C" double oracle_shape_distance_f64(const double* center, const double* 
 namespace {
 
 template <class R> struct TetTape { V3<R> p[4]; R lamD, lamH; };
+// The subgradient rule at the floor of C_D (the kernels always apply it). Off: the exact derivative through the
+// clamp (the direction G / 1e-6 differentiated as it stands), which central differences of the FP64 forward see.
+bool g_clampRule = true;
 template <class R> struct EdgeTape { V3<R> xi, xj; R lam; };
 
 // Reverse of tetProject (mode 1, no damping, omega = 1) at input positions p and multipliers lamD / lamH: given the
@@ -1919,7 +2082,7 @@ template <class R> struct EdgeTape { V3<R> xi, xj; R lam; };
 // multipliers), accumulates the adjoint of the input positions (pb) and of alpha~_D / alpha~_H.
 template <class R>
 void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, R ah, R lamD, R lamH,
-                       const V3<R> ob[4], V3<R> pb[4], R& adb, R& ahb, R& lDb, R& lHb) {
+                       const V3<R> ob[4], V3<R> pb[4], R& adb, R& ahb, R& lDb, R& lHb, const V3<R>* rk = nullptr) {
     V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
     V3<R> F[3];
     for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
@@ -1934,16 +2097,30 @@ void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, 
     R pJ = fmax_<R>(J, (R)0.05f);
     R iso = isoPow(pJ);
     R qraw = iso * I1 - (R)3;
+    bool jlive = J > (R)0.05f;
+    R CH = J - (R)1;
+    // FP32: the precise invariants of the adjoint's forward (the FP64 oracle keeps the direct forms)
+    V3<R> Gh[3];
+    if constexpr (std::is_same<R, float>::value) {
+        V3<float> d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+        Strain32 st = preciseStrain(d, br);
+        qraw = preciseDev(st, F, cof, jlive, pJ, iso, Gh);
+        CH = st.a;
+    } else {
+        R vgd = jlive ? iso * I1 / ((R)3 * pJ) : (R)0;
+        for (int c = 0; c < 3; c++) Gh[c] = F[c] * iso - cof[c] * vgd;
+    }
     R q = fmax_<R>(qraw, (R)0);
     R CD = std::sqrt(fmax_<R>(q, (R)1e-12f));
     R ds = (R)1 / CD;
-    bool jlive = J > (R)0.05f;
+    // at or below the floor of C_D the row's value and direction are held constant (no adjoint into F through them)
+    const bool atFloor = !(q > (R)1e-12f), frozen = atFloor && g_clampRule;
     R vg = jlive ? iso * I1 / ((R)3 * pJ) : (R)0;
-    R CH = J - (R)1;
     for (int k = 1; k < 4; k++) {
         V3<R> b = br[k - 1];
         Fb[k] = F[0] * b.x + F[1] * b.y + F[2] * b.z;
-        G[k] = Fb[k] * iso - gh[k] * vg;
+        if constexpr (std::is_same<R, float>::value) G[k] = Gh[0] * b.x + Gh[1] * b.y + Gh[2] * b.z;
+        else G[k] = Fb[k] * iso - gh[k] * vg;
         gd[k] = G[k] * ds;
     }
     gd[0] = (gd[1] + gd[2] + gd[3]) * (R)-1;
@@ -2013,15 +2190,16 @@ void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, 
     }
     // gd_k = G_k ds, G_k = Fb_k iso - gh_k vg
     R dsb = 0, isob = 0, vgb = 0;
-    V3<R> Fbb[4];
-    for (int k = 1; k < 4; k++) {
-        V3<R> Gb = gdb[k] * ds;
-        dsb += dot3(gdb[k], G[k]);
-        Fbb[k] = Gb * iso;
-        isob += dot3(Gb, Fb[k]);
-        ghb[k] = ghb[k] - Gb * vg;
-        vgb += -dot3(Gb, gh[k]);
-    }
+    V3<R> Fbb[4] = {mk<R>(0, 0, 0), mk<R>(0, 0, 0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
+    if (!frozen)
+        for (int k = 1; k < 4; k++) {
+            V3<R> Gb = gdb[k] * ds;
+            dsb += dot3(gdb[k], G[k]);
+            Fbb[k] = Gb * iso;
+            isob += dot3(Gb, Fb[k]);
+            ghb[k] = ghb[k] - Gb * vg;
+            vgb += -dot3(Gb, gh[k]);
+        }
     V3<R> Fbar[3] = {mk<R>(0, 0, 0), mk<R>(0, 0, 0), mk<R>(0, 0, 0)};
     for (int k = 1; k < 4; k++) {
         V3<R> b = br[k - 1];
@@ -2029,14 +2207,12 @@ void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, 
         Fbar[1] = Fbar[1] + Fbb[k] * b.y;
         Fbar[2] = Fbar[2] + Fbb[k] * b.z;
     }
-    // ds = 1 / CD; CD = sqrt(max(q, 1e-12)); q = max(iso I1 - 3, 0)
+    // ds = 1 / CD; CD = sqrt(q) above the floor; dq/dF = 2 Ghat
     CDb += -dsb / (CD * CD);
-    R qb = q > (R)1e-12f ? CDb / ((R)2 * CD) : (R)0;
+    R qb = atFloor ? (R)0 : CDb / ((R)2 * CD);
+    if (!atFloor)
+        for (int c = 0; c < 3; c++) Fbar[c] = Fbar[c] + Gh[c] * ((R)2 * qb);
     R I1b = 0;
-    if (qraw > 0) {
-        isob += qb * I1;
-        I1b += qb * iso;
-    }
     R pJb = 0;
     if (jlive) {
         isob += vgb * I1 / ((R)3 * pJ);
@@ -2144,7 +2320,9 @@ void adjointSubstep(Tissue<R>& s, const std::vector<V3<R>>& force, R h, int iter
 template <class R>
 double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* force, const float* compliance,
                   const float* mask, const float* target, R* grad) {
-    // frame constants as xpbd_begin_frame with substeps = d.substeps (h = dt / substeps)
+    // frame constants as xpbd_begin_frame with substeps = d.substeps (h = dt / substeps); the FP32 forward takes the
+    // precise invariants, as the reverse needs them
+    s.d.precise_strain = 1;
     int Nsub = s.d.substeps > 0 ? s.d.substeps : 1;
     R h = dt / (R)Nsub;
     R ks = (R)s.d.edge_ks;
@@ -2215,7 +2393,7 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* for
                                        mk<R>(B[2][0], B[2][1], B[2][2])};
                         const TetTape<R>& r = tt[(size_t)it * s.T + t];
                         tetProjectReverse<R>(r.p, w, br, s.alphaDev[t], s.alphaHyd[t], r.lamD, r.lamH, ob, pb, adb[t],
-                                             ahb[t], lDb[t], lHb[t]);
+                                             ahb[t], lDb[t], lHb[t], s.typeRk[s.ttype[t]]);
                         for (int j = 0; j < 4; j++) xb[id[j]] = pb[j];
                     }
                 for (int c = (int)s.ecolorCnt.size() - 1; c >= 0; c--)
@@ -2254,9 +2432,9 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* for
                                            const float* compliance, const float* mask, const float* target, R* grad) { \
         return adjointRun(*(Tissue<R>*)h, (R)dt, nsub, iterations, force, compliance, mask, target, grad);            \
     }                                                                                                                  \
-    extern "C" void oracle_tet_reverse_##SUF(const R* p12, const R* w4, const R* B9, R ad, R ah, const R* lam2,        \
-                                             const R* ob12, R* lb2, R* pb12, R* abar2) {                               \
-        V3<R> p[4], ob[4], pb[4], br[3];                                                                               \
+    extern "C" void oracle_tet_reverse_##SUF(const R* p12, const R* w4, const R* B9, const R* rk9, R ad, R ah,      \
+                                             const R* lam2, const R* ob12, R* lb2, R* pb12, R* abar2) {                \
+        V3<R> p[4], ob[4], pb[4], br[3], rk[3];                                                                        \
         R w[4];                                                                                                        \
         for (int k = 0; k < 4; k++) {                                                                                  \
             p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);                                                     \
@@ -2264,13 +2442,15 @@ double adjointRun(Tissue<R>& s, R dt, int nsub, int iterations, const float* for
             w[k] = w4[k];                                                                                              \
         }                                                                                                              \
         for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);                              \
+        for (int r = 0; r < 3; r++) rk[r] = rk9 ? mk(rk9[3 * r], rk9[3 * r + 1], rk9[3 * r + 2]) : mk<R>(0, 0, 0);     \
         R adb = 0, ahb = 0;                                                                                            \
-        tetProjectReverse<R>(p, w, br, ad, ah, lam2[0], lam2[1], ob, pb, adb, ahb, lb2[0], lb2[1]);                    \
+        tetProjectReverse<R>(p, w, br, ad, ah, lam2[0], lam2[1], ob, pb, adb, ahb, lb2[0], lb2[1], rk);                \
         for (int k = 0; k < 4; k++) { pb12[3 * k] = pb[k].x; pb12[3 * k + 1] = pb[k].y; pb12[3 * k + 2] = pb[k].z; }   \
         abar2[0] = adb;                                                                                                \
         abar2[1] = ahb;                                                                                                \
     }
 
+extern "C" void oracle_adjoint_clamp_rule(int on) { g_clampRule = on != 0; }
 ORACLE_ADJ(float, f32)
 ORACLE_ADJ(double, f64)
 

@@ -13,14 +13,14 @@ pytestmark = pytest.mark.gpu
 SIZES = [(5, 4, 4), (9, 5, 5), (17, 9, 9), (12, 7, 6), (21, 11, 11), (3, 2, 2)]
 
 
-def instances(seed=0, iterations=1):
+def instances(seed=0, iterations=1, precise=False):
     rng = np.random.default_rng(seed)
     out = []
     for k, nodes in enumerate(SIZES):
         g = block_grid(nodes, 0.25)
         c = float(np.exp(rng.uniform(np.log(1e-8), np.log(1e-6))))
         p = SolverParams(substeps=10, iterations=iterations, tet_mode=TET_NEOHOOKEAN,
-                         edge_ks=float(np.float32(1.0 / c)))
+                         edge_ks=float(np.float32(1.0 / c)), precise_strain=precise)
         T = tet_count(g)
         kd = volume_stiffness(np.full(T, c))
         kh = volume_stiffness(np.full(T, c / 10))
@@ -29,15 +29,16 @@ def instances(seed=0, iterations=1):
 
 
 # lds_nodes 200 stages only part of every instance but the smallest; substeps_per_launch 3 splits the 10-substep step
-# into launches of 3, 3, 3 and 1 substeps.
-@pytest.mark.parametrize("lds_nodes,iterations,substeps_per_launch", [(None, 1, 0), (200, 1, 0), (None, 2, 0),
-                                                                      (200, 1, 3)])
-def test_batched_matches_oracle(lds_nodes, iterations, substeps_per_launch):
+# into launches of 3, 3, 3 and 1 substeps; precise: the tet invariants formed from F - I.
+@pytest.mark.parametrize("lds_nodes,iterations,substeps_per_launch,precise", [
+    (None, 1, 0, False), (200, 1, 0, False), (None, 2, 0, False), (200, 1, 3, False), (None, 1, 0, True),
+    (200, 2, 3, True)])
+def test_batched_matches_oracle(lds_nodes, iterations, substeps_per_launch, precise):
     from genesis_tissue_simulation_amd.batched import Batch
     from genesis_tissue_simulation_amd.tissue import swept_cut_tool
     from oracle.oracle import Oracle
 
-    insts = instances(iterations=iterations)
+    insts = instances(iterations=iterations, precise=precise)
     base = insts[0][1]
     batch = Batch(insts, base, block_size=256, lds_nodes=lds_nodes)
     batch.configure(256, lds_nodes, 1, substeps_per_launch=substeps_per_launch)

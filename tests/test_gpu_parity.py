@@ -100,19 +100,24 @@ def test_cut_unit_parity_on_identical_positions():
     assert_cut_equal(t.cut_state(), o.cut_state())
 
 
+@pytest.mark.parametrize("precise", [False, True])
 @pytest.mark.parametrize("mode", [TET_VOLUME, TET_NEOHOOKEAN, TET_GUARDED])
 @pytest.mark.parametrize("iterations", [1, 2])
-def test_tet_modes_and_iterations(mode, iterations):
+def test_tet_modes_and_iterations(mode, iterations, precise):
+    """Every tet form, undamped and damped (bulk damping with the guarded form), in the reference's direct forms and
+    in the precise ones: bit-exact with the oracle (the guarded log is the same own sequence on both sides)."""
     w = block_6144()
-    p = w.params.with_(tet_mode=mode, iterations=iterations, strain_damping=0.01 if iterations == 2 else 0.0)
+    damped = iterations == 2
+    p = w.params.with_(tet_mode=mode, iterations=iterations, strain_damping=0.01 if damped else 0.0,
+                       bulk_damping=0.005 if damped and mode == TET_GUARDED else 0.0, precise_strain=precise)
     o, t = make_pair(w, params=p)
-    advance([o, t], w, 3, cut=False)
+    o64, _ = make_pair(w, params=p, double=True)
+    advance([o, t, o64], w, 3, cut=False)
     err = rel_err(t.positions(), o.state()[0])
-    print(f"mode {mode} iterations {iterations}: rel err {err:.3e}")
-    if mode == TET_GUARDED:
-        assert err <= REL_TOL  # logf is the device library's; tolerance only
-    else:
-        np.testing.assert_array_equal(t.positions(), o.state()[0])
+    print(f"mode {mode} iterations {iterations} precise {precise}: rel err {err:.3e}, "
+          f"vs FP64 {rel_err(t.positions(), o64.state()[0]):.3e}")
+    np.testing.assert_array_equal(t.positions(), o.state()[0])
+    np.testing.assert_array_equal(t.velocities(), o.state()[1])
 
 
 def test_fallback_edges_and_damping():
