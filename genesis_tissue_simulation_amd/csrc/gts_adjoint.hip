@@ -239,21 +239,20 @@ __device__ void tet_reverse(const f3 p[4], const float w[4], const float* Bm, co
                             float& lHb) {
     f3 br[3] = {mk3(Bm[0], Bm[1], Bm[2]), mk3(Bm[3], Bm[4], Bm[5]), mk3(Bm[6], Bm[7], Bm[8])};
     f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
-    f3 F[3];
-    for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
-    f3 cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
-    float J = dot3(F[0], cof[0]);
+    f3 d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+    PreciseStrain st = precise_strain(d, br);
+    f3 F[3], cof[3];
+    float J;
+    precise_frame(st, F, cof, J);
     f3 gh[4], gd[4], Fb[4], G[4];
     for (int k = 1; k < 4; k++) {
         f3 b = br[k - 1];
         gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
     }
-    float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    float I1 = 3.f + st.b;
     float pJ = fmaxf(J, 0.05f);
     float iso = iso_pow(pJ);
     bool jlive = J > 0.05f;
-    f3 d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
-    PreciseStrain st = precise_strain(d, br);
     f3 Gh[3];
     float qraw = precise_dev(st, F, cof, jlive, pJ, iso, Gh);
     float q = fmaxf(qraw, 0.f);

@@ -141,6 +141,20 @@ GTS_HD PreciseStrain precise_strain(const f3 d[3], const f3 br[3]) {
     return s;
 }
 
+// F = I + H, cof F = (1 + tr H) I - H^T + cof H and J = 1 + (J - 1) from the precise strain: the same functions of the
+// corners as the direct forms, without forming F, its cofactors and determinant a second time.
+GTS_HD void precise_frame(const PreciseStrain& s, f3 F[3], f3 cof[3], float& J) {
+    const f3* H = s.H;
+    const float c1 = 1.f + s.trH;
+    F[0] = mk3(H[0].x + 1.f, H[0].y, H[0].z);
+    F[1] = mk3(H[1].x, H[1].y + 1.f, H[1].z);
+    F[2] = mk3(H[2].x, H[2].y, H[2].z + 1.f);
+    cof[0] = mk3(s.cH[0].x + c1 - H[0].x, s.cH[0].y - H[1].x, s.cH[0].z - H[2].x);
+    cof[1] = mk3(s.cH[1].x - H[0].y, s.cH[1].y + c1 - H[1].y, s.cH[1].z - H[2].y);
+    cof[2] = mk3(s.cH[2].x - H[0].z, s.cH[2].y - H[1].z, s.cH[2].z + c1 - H[2].z);
+    J = 1.f + s.a;
+}
+
 // q = J^(-2/3) I1 - 3 (unclamped) and the columns of Ghat at pJ = max(J, .05), iso = pJ^(-2/3). Below the J clamp
 // (jlive false) pJ is a constant and iso - 1 is far from 0: q = 3u + b + ub with u = iso - 1, Ghat = iso F as the
 // reference's gradient there.
@@ -153,18 +167,21 @@ GTS_HD float precise_dev(const PreciseStrain& s, const f3 F[3], const f3 cof[3],
     }
     float u2, u3;
     iso_minus_one(s.a, iso, u2, u3);
-    const float t3 = s.trH / 3.f;
+    const float t3 = s.trH * 0.333333343f;  // FP32 1/3
     float d0 = s.H[0].x - t3, d1 = s.H[1].y - t3, d2 = s.H[2].z - t3;
     float s01 = s.H[1].x + s.H[0].y, s02 = s.H[2].x + s.H[0].z, s12 = s.H[2].y + s.H[1].z;
     float q2 = 2.f * (d0 * d0 + d1 * d1 + d2 * d2) + (s01 * s01 + s02 * s02 + s12 * s12);
     float rest = 2.f * s.trH * s.e + 1.6666666f * s.e * s.e - 2.f * s.det + 3.f * u3 - 0.6666667f * s.a * s.nH2 +
                  u2 * s.b;
     float m = (s.b - 3.f * s.a) / (3.f * pJ);
-    for (int c = 0; c < 3; c++) {
-        f3 HT = mk3(comp3(s.H[0], c), comp3(s.H[1], c), comp3(s.H[2], c));
-        f3 ec = mk3(c == 0 ? 1.f : 0.f, c == 1 ? 1.f : 0.f, c == 2 ? 1.f : 0.f);
-        Gh[c] = (s.H[c] + HT - ec * s.trH - s.cH[c] - cof[c] * m) * iso;
-    }
+    // column c of H + H^T - tr H I - cof H - m cof F, tr H taken off the diagonal entry only
+    const f3* H = s.H;
+    f3 S0 = mk3(H[0].x + H[0].x - s.trH, H[0].y + H[1].x, H[0].z + H[2].x);
+    f3 S1 = mk3(H[1].x + H[0].y, H[1].y + H[1].y - s.trH, H[1].z + H[2].y);
+    f3 S2 = mk3(H[2].x + H[0].z, H[2].y + H[1].z, H[2].z + H[2].z - s.trH);
+    Gh[0] = (S0 - s.cH[0] - cof[0] * m) * iso;
+    Gh[1] = (S1 - s.cH[1] - cof[1] * m) * iso;
+    Gh[2] = (S2 - s.cH[2] - cof[2] * m) * iso;
     return q2 + rest;
 }
 

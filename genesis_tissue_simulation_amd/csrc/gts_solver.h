@@ -72,10 +72,20 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
     float w[4] = {q[0].w, q[1].w, q[2].w, q[3].w};
     f3 e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
     f3 br[3] = {mk3(B[0], B[1], B[2]), mk3(B[3], B[4], B[5]), mk3(B[6], B[7], B[8])};
-    f3 F[3];
-    for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
-    f3 cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
-    float J = dot3(F[0], cof[0]);
+    f3 F[3], cof[3];
+    float J;
+    PreciseStrain st;
+    if (PRECISE) {
+        f3 d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+        st = precise_strain(d, br);
+        precise_frame(st, F, cof, J);
+    } else {
+        for (int c = 0; c < 3; c++) F[c] = e1 * comp3(br[0], c) + e2 * comp3(br[1], c) + e3 * comp3(br[2], c);
+        cof[0] = cross3(F[1], F[2]);
+        cof[1] = cross3(F[2], F[0]);
+        cof[2] = cross3(F[0], F[1]);
+        J = dot3(F[0], cof[0]);
+    }
     f3 gh[4], gd[4];
     for (int c = 1; c < 4; c++) {
         f3 b = br[c - 1];
@@ -84,11 +94,6 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
     // Undamped instantiations run only with both damping terms zero: the scales are then exactly 1 and the divisions
     // by them and the damping terms drop out without changing a bit.
     float scale = DAMPED ? 1.f + dampingOverH : 1.f;
-    PreciseStrain st;
-    if (PRECISE) {
-        f3 d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
-        st = precise_strain(d, br);
-    }
     if (MODE == 0) {
         float CH = PRECISE ? st.a : J - 1.f;
         gh[0] = (gh[1] + gh[2] + gh[3]) * -1.f;
@@ -111,7 +116,7 @@ __device__ __forceinline__ bool tet_project(const Store& S, const float4* __rest
         return true;
     }
     const bool safeguarded = MODE == 2;
-    float I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    float I1 = PRECISE ? 0.f : dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);  // the direct forms' only
     float positiveJ = fmaxf(J, 0.05f);
     float iso = iso_pow(positiveJ);
     const bool jlive = J > 0.05f;

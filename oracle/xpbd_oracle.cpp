@@ -758,6 +758,18 @@ inline Strain32 preciseStrain(const V3<float> d[3], const V3<float> br[3]) {
     s.b = 2.f * s.trH + s.nH2;
     return s;
 }
+// F = I + H, cof F = (1 + tr H) I - H^T + cof H, J = 1 + (J - 1): the precise path's F, its cofactors and determinant.
+inline void preciseFrame(const Strain32& s, V3<float> F[3], V3<float> cof[3], float& J) {
+    const V3<float>* H = s.H;
+    const float c1 = 1.f + s.trH;
+    F[0] = mk(H[0].x + 1.f, H[0].y, H[0].z);
+    F[1] = mk(H[1].x, H[1].y + 1.f, H[1].z);
+    F[2] = mk(H[2].x, H[2].y, H[2].z + 1.f);
+    cof[0] = mk(s.cH[0].x + c1 - H[0].x, s.cH[0].y - H[1].x, s.cH[0].z - H[2].x);
+    cof[1] = mk(s.cH[1].x - H[0].y, s.cH[1].y + c1 - H[1].y, s.cH[1].z - H[2].y);
+    cof[2] = mk(s.cH[2].x - H[0].z, s.cH[2].y - H[1].z, s.cH[2].z + c1 - H[2].z);
+    J = 1.f + s.a;
+}
 // q = J^(-2/3) I1 - 3 (unclamped) and the columns of Ghat, at pJ = max(J, .05) and iso = pJ^(-2/3):
 // q = 2 |dev sym H|^2 + 2 tr H e + 5e^2/3 - 2 det H + 3 u3 - 2a|H|^2/3 + u2 b above the J clamp.
 inline float preciseDev(const Strain32& s, const V3<float> F[3], const V3<float> cof[3], bool jlive, float pJ,
@@ -769,18 +781,21 @@ inline float preciseDev(const Strain32& s, const V3<float> F[3], const V3<float>
     }
     float u2, u3;
     isoMinusOne(s.a, iso, u2, u3);
-    const float t3 = s.trH / 3.f;
+    const float t3 = s.trH * 0.333333343f;  // FP32 1/3
     float d0 = s.H[0].x - t3, d1 = s.H[1].y - t3, d2 = s.H[2].z - t3;
     float s01 = s.H[1].x + s.H[0].y, s02 = s.H[2].x + s.H[0].z, s12 = s.H[2].y + s.H[1].z;
     float q2 = 2.f * (d0 * d0 + d1 * d1 + d2 * d2) + (s01 * s01 + s02 * s02 + s12 * s12);
     float rest = 2.f * s.trH * s.e + 1.6666666f * s.e * s.e - 2.f * s.det + 3.f * u3 - 0.6666667f * s.a * s.nH2 +
                  u2 * s.b;
     float m = (s.b - 3.f * s.a) / (3.f * pJ);
-    for (int c = 0; c < 3; c++) {
-        V3<float> HT = mk(comp(s.H[0], c), comp(s.H[1], c), comp(s.H[2], c));
-        V3<float> ec = mk(c == 0 ? 1.f : 0.f, c == 1 ? 1.f : 0.f, c == 2 ? 1.f : 0.f);
-        Gh[c] = (s.H[c] + HT - ec * s.trH - s.cH[c] - cof[c] * m) * iso;
-    }
+    // column c of H + H^T - tr H I - cof H - m cof F, tr H taken off the diagonal entry only
+    const V3<float>* H = s.H;
+    V3<float> S0 = mk(H[0].x + H[0].x - s.trH, H[0].y + H[1].x, H[0].z + H[2].x);
+    V3<float> S1 = mk(H[1].x + H[0].y, H[1].y + H[1].y - s.trH, H[1].z + H[2].y);
+    V3<float> S2 = mk(H[2].x + H[0].z, H[2].y + H[1].z, H[2].z + H[2].z - s.trH);
+    Gh[0] = (S0 - s.cH[0] - cof[0] * m) * iso;
+    Gh[1] = (S1 - s.cH[1] - cof[1] * m) * iso;
+    Gh[2] = (S2 - s.cH[2] - cof[2] * m) * iso;
     return q2 + rest;
 }
 
@@ -794,18 +809,10 @@ template <class R>
 bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], const V3<R> br[3], R ad, R ah,
                 R& lamD, R& lamH, R dampingOverH, R bulkDampingOverH, R omega, V3<R> out[4], const V3<R>* rk = nullptr) {
     V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
-    V3<R> F[3];
-    for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
-    V3<R> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
-    R J = dot3(F[0], cof[0]);
-    const bool safeguarded = mode == 2;
-    V3<R> gh[4], gd[4];
-    for (int k = 1; k < 4; k++) {
-        V3<R> b = br[k - 1];
-        gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
-    }
-    R scale = (R)1 + dampingOverH;
-    // precise invariants: FP32 with the type's rest edges rk (the FP64 oracle passes none)
+    V3<R> F[3], cof[3];
+    R J;
+    // precise invariants: FP32 with the type's rest edges rk (the FP64 oracle passes none); F, cof F and J follow
+    // from H there
     bool precise = false;
     Strain32 st{};
     if constexpr (std::is_same<R, float>::value) {
@@ -813,8 +820,23 @@ bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], con
             precise = true;
             V3<float> d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
             st = preciseStrain(d, br);
+            preciseFrame(st, F, cof, J);
         }
     }
+    if (!precise) {
+        for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
+        cof[0] = cross3(F[1], F[2]);
+        cof[1] = cross3(F[2], F[0]);
+        cof[2] = cross3(F[0], F[1]);
+        J = dot3(F[0], cof[0]);
+    }
+    const bool safeguarded = mode == 2;
+    V3<R> gh[4], gd[4];
+    for (int k = 1; k < 4; k++) {
+        V3<R> b = br[k - 1];
+        gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
+    }
+    R scale = (R)1 + dampingOverH;
     if (mode == 0) {
         R CH = J - (R)1;
         if constexpr (std::is_same<R, float>::value)
@@ -834,7 +856,7 @@ bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], con
         for (int k = 0; k < 4; k++) out[k] = p[k] + gh[k] * dlH * w[k];
         return true;
     }
-    R I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    R I1 = precise ? (R)0 : dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);  // the direct forms' only
     R positiveJ = fmax_<R>(J, (R)0.05f);
     R iso = isoPow(positiveJ);
     const bool jlive = J > (R)0.05f;
@@ -2012,11 +2034,11 @@ extern "C" void oracle_precise_invariants_f32(const float* p12, const float* B9,
     for (int k = 0; k < 4; k++) p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);
     for (int r = 0; r < 3; r++) br[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);
     for (int k = 0; k < 3; k++) d[k] = (p[k + 1] - p[0]) - mk(rk9[3 * k], rk9[3 * k + 1], rk9[3 * k + 2]);
-    V3<float> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
-    for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
-    V3<float> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
-    float J = dot3(F[0], cof[0]), pJ = std::fmax(J, 0.05f);
     Strain32 st = preciseStrain(d, br);
+    V3<float> cof[3];
+    float J;
+    preciseFrame(st, F, cof, J);
+    float pJ = std::fmax(J, 0.05f);
     V3<float> Gh[3];
     float q = preciseDev(st, F, cof, J > 0.05f, pJ, isoPow(pJ), Gh);
     out[0] = st.a;
@@ -2084,26 +2106,36 @@ template <class R>
 void tetProjectReverse(const V3<R> p[4], const R w[4], const V3<R> br[3], R ad, R ah, R lamD, R lamH,
                        const V3<R> ob[4], V3<R> pb[4], R& adb, R& ahb, R& lDb, R& lHb, const V3<R>* rk = nullptr) {
     V3<R> e1 = p[1] - p[0], e2 = p[2] - p[0], e3 = p[3] - p[0];
-    V3<R> F[3];
-    for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
-    V3<R> cof[3] = {cross3(F[1], F[2]), cross3(F[2], F[0]), cross3(F[0], F[1])};
-    R J = dot3(F[0], cof[0]);
+    V3<R> F[3], cof[3];
+    R J, I1;
+    // FP32: the precise invariants of the adjoint's forward, F, cof F and J from H (the FP64 oracle keeps the direct
+    // forms)
+    Strain32 st{};
+    if constexpr (std::is_same<R, float>::value) {
+        V3<float> d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
+        st = preciseStrain(d, br);
+        preciseFrame(st, F, cof, J);
+        I1 = 3.f + st.b;
+    } else {
+        for (int c = 0; c < 3; c++) F[c] = e1 * comp(br[0], c) + e2 * comp(br[1], c) + e3 * comp(br[2], c);
+        cof[0] = cross3(F[1], F[2]);
+        cof[1] = cross3(F[2], F[0]);
+        cof[2] = cross3(F[0], F[1]);
+        J = dot3(F[0], cof[0]);
+        I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
+    }
     V3<R> gh[4], gd[4], Fb[4], G[4];
     for (int k = 1; k < 4; k++) {
         V3<R> b = br[k - 1];
         gh[k] = cof[0] * b.x + cof[1] * b.y + cof[2] * b.z;
     }
-    R I1 = dot3(F[0], F[0]) + dot3(F[1], F[1]) + dot3(F[2], F[2]);
     R pJ = fmax_<R>(J, (R)0.05f);
     R iso = isoPow(pJ);
     R qraw = iso * I1 - (R)3;
     bool jlive = J > (R)0.05f;
     R CH = J - (R)1;
-    // FP32: the precise invariants of the adjoint's forward (the FP64 oracle keeps the direct forms)
     V3<R> Gh[3];
     if constexpr (std::is_same<R, float>::value) {
-        V3<float> d[3] = {e1 - rk[0], e2 - rk[1], e3 - rk[2]};
-        Strain32 st = preciseStrain(d, br);
         qraw = preciseDev(st, F, cof, jlive, pJ, iso, Gh);
         CH = st.a;
     } else {
