@@ -126,6 +126,8 @@ SIGNATURES = {
     "gts_refresh_liveness": ((P, P), I),
     "gts_step": ((P, P, P, P, I, I, P), I),
     "gts_detect_cut": ((P, P, P), I),
+    "gts_detect_cut_static": ((P, P, P), I),
+    "gts_detect_cut_static_floats": ((I,), ct.c_longlong),
     "gts_detect_cut_ccd": ((P, P, P, P, F, P, P), I),
     "gts_tear_overstretch": ((P, F, F, P), I),
     "gts_compute_rotations": ((P, P), I),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gts_xpbd.h"
@@ -1665,6 +1666,50 @@ extern "C" int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, vo
                                                   records);
     HIP_TRY(hipGetLastError());
     return finish_cut(v, L, s, t.ncut, 0, t.d);
+}
+
+// ------------------------------------------------------------------------------------------------ allocation-free cut
+// gts_detect_cut with its workspace (the per-edge detection records and the emission scan's partials) in a file-scope
+// device array instead of the view's scratch buffer, for callers that cannot allocate device memory. The array is
+// GTS_STATIC_SCRATCH_FLOATS floats (32 MiB) on every device the library runs on, held until the process ends; each
+// device's copy sits at its own address, which hipGetSymbolAddress gives on the entry point's first call on that
+// device. Host code never passes the array's name to a kernel, only that address. Every entry the cut reads is first
+// written by the same call, so nothing is cleared between calls; calls that overlap on one device race on the array.
+static __device__ __align__(256) float scratch[GTS_STATIC_SCRATCH_FLOATS];
+
+namespace {
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[64] = {};  // per device ordinal
+
+int static_scratch(float** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return GTS_ERR_BAD_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return GTS_ERR_NOT_READY;
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return GTS_OK;
+}
+}  // namespace
+
+extern "C" long long gts_detect_cut_static_floats(int n_grid_edges) {
+    return n_grid_edges <= 0 ? 0 : 8ll * n_grid_edges + scan_blocks(n_grid_edges) + 1;
+}
+
+extern "C" int gts_detect_cut_static(const GtsTissueView* v, const GtsCutTool* tool, void* stream) {
+    if (!v || !tool) return GTS_ERR_BAD_ARG;
+    if (v->n_grid_edges <= 0) return GTS_OK;
+    if (gts_detect_cut_static_floats(v->n_grid_edges) > (long long)GTS_STATIC_SCRATCH_FLOATS) return GTS_ERR_CAPACITY;
+    float* ptr = nullptr;
+    { int rc_ = static_scratch(&ptr); if (rc_) return rc_; }
+    GtsTissueView w = *v;  // a host copy of the descriptor: the kernels' workspace is the device array
+    w.scratch = ptr;
+    return gts_detect_cut(&w, tool, stream);
 }
 
 extern "C" int gts_detect_cut_ccd(const GtsTissueView* v, const float* Sb, const float* Eb, const float* emit_ncut,

@@ -253,6 +253,16 @@ int gts_snapshot_corners(const GtsTissueView* v, void* stream);
 int gts_compute_rotations(const GtsTissueView* v, void* stream);
 /* Device: swept-quad cut of the current positions, then sever the cut links and refresh liveness. */
 int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream);
+/* Device: gts_detect_cut without the caller's workspace (no reference counterpart: for callers that cannot allocate
+ * device memory -- code under graph capture, tight serving loops, harnesses that forbid hipMalloc). The detection
+ * records and scan partials live in a library-owned static device array of GTS_STATIC_SCRATCH_FLOATS floats (32 MiB,
+ * held on every device the library runs on until the process ends); v->scratch is not used. The call allocates
+ * nothing. It needs gts_detect_cut_static_floats(n_grid_edges) <= GTS_STATIC_SCRATCH_FLOATS (about 1.04M cut edges)
+ * and returns GTS_ERR_CAPACITY otherwise. Calls on one device share the array: calls that overlap on a device
+ * race. Results equal gts_detect_cut's. */
+#define GTS_STATIC_SCRATCH_FLOATS (8u << 20)
+long long gts_detect_cut_static_floats(int n_grid_edges);
+int gts_detect_cut_static(const GtsTissueView* v, const GtsCutTool* tool, void* stream);
 /* Device: CCD of the edges moving from prev_positions ([3N] device) to the current positions against blade S->E. */
 int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, const float* emit_ncut, float tool_d,
                        const float* prev_positions, void* stream);

@@ -207,3 +207,21 @@ def test_entry_points_reject_bad_arguments_before_touching_the_device():
     params.tet_mode = 1
     view.n_nodes, view.n_tet_colours = 8, 1000
     assert lib.gts_step(ct.byref(view), ct.byref(params), ct.byref(tool), ct.byref(tool), 0, 0, null) == BAD
+
+
+def test_static_workspace_cut_refuses_sizes_beyond_its_array():
+    """The allocation-free cut checks its workspace need against the static array before any device work: a lattice
+    with more cut edges than the array holds is refused with the capacity status, and an empty one is a no-op."""
+    lib = _lib.load()
+    cap = 8 << 20
+    assert lib.gts_detect_cut_static_floats(0) == 0
+    assert lib.gts_detect_cut_static_floats(1000) == 8 * 1000 + 4 + 1
+    view, tool = _lib.GtsTissueView(), _lib.GtsCutTool()
+    largest = max(n for n in range(1_040_000, 1_050_000) if lib.gts_detect_cut_static_floats(n) <= cap)
+    view.n_grid_edges = largest + 1
+    assert lib.gts_detect_cut_static(ct.byref(view), ct.byref(tool), None) == -3004
+    view.n_grid_edges = 4_000_000
+    assert lib.gts_detect_cut_static(ct.byref(view), ct.byref(tool), None) == -3004
+    view.n_grid_edges = 0
+    assert lib.gts_detect_cut_static(ct.byref(view), ct.byref(tool), None) == 0
+    assert lib.gts_detect_cut_static(None, ct.byref(tool), None) == -1004
