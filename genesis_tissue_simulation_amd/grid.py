@@ -1,15 +1,22 @@
 """Voxel lattice of a tissue: node positions, flags, lumped mass, six-neighbour links and cut edges.
 
-This is the host-side input generator of the XPBD substep. It restates the behaviour of the reference lattice
-builder (`endoscopic_tissue/grid.py:176-281`, `TissueGrid` at `:41-61`) so that a grid built here can be handed to
-`NativeTissue` exactly as the reference's is, and adds the regular no-shell blocks and the ellipsoid the benchmark
-workloads are defined on.
+The host-side input of the XPBD substep. `build_grid` takes the place of the reference lattice builder
+(`endoscopic_tissue/grid.py:176-281`; the record `TissueGrid` at `:41-61`): a grid built here can be handed to
+`NativeTissue` as the reference's is. `block_grid` and `Ellipsoid` add the shell-free blocks and the ellipsoidal organ
+the benchmark workloads are defined on.
 
-Conventions (shared with the reference):
+The lattice is handled as a 3-D array of nodes (z, y, x; x fastest in the flat numbering) and a 3-D array of voxels;
+occupancy, activity, links, cut edges and the lumped mass are shifted-slice reductions over those arrays.
+
+Formats kept from the reference, because other code reads them (paths relative to endoscopic_tissue/):
   * native units are cm, y is up; node id = x + nx * (y + ny * z) (`grid.py:64-66`);
-  * neighbour slots 0..5 are -x, +x, -y, +y, -z, +z (`xpbd_solver.h:132-133`);
-  * the eight voxel corners follow the figure order of `CORNER_OFFSETS` below; the Kuhn tetrahedra use the bit order
-    `i = dx | dy << 1 | dz << 2` (`xpbd_solver.cu:1624`).
+  * neighbour slots 0..5 are -x, +x, -y, +y, -z, +z (`native/vendor/cuda/xpbd/xpbd_solver.h:132-133`);
+  * voxel corners in the figure order of CORNER_OFFSETS and voxel edges in EDGE_CORNERS order (`grid.py:11-16`), the
+    order of the dual-contouring tables; the Kuhn tets use the bit order i = dx | dy << 1 | dz << 2
+    (`xpbd_solver.cu:1624`);
+  * the 40-byte intersection, 32-byte bend and 32-byte cut-point records; cut edges listed axis by axis in node order,
+    intersections voxel by voxel in edge order;
+  * the lattice placement: the tissue's bounds rounded up to whole cells, two air cells added on every side, centred.
 """
 from __future__ import annotations
 
@@ -39,11 +46,16 @@ METERS_PER_UNIT = 0.01
 BLOCK_SIZE = (6.0, 1.4, 4.0)
 # The 15 unordered slot pairs of a node's bend elements, in lexicographic order (`grid.py:257-260`).
 BEND_SLOT_PAIRS = np.array(list(combinations(range(6), 2)), dtype=np.int32)
+# Air cells kept around the tissue's bounding box on every side.
+SHELL_CELLS = 2
+# How many of a cell's six Kuhn tets hold each bit-order corner: the diagonal ends 0 and 7 are in all six, every other
+# corner in two. Each tet lumps a quarter of its mass (the cell's sixth) on each of its corners.
+KUHN_MULTIPLICITY = np.bincount(KUHN_TETS.ravel(), minlength=8).astype(np.int64)
 
 
 @dataclass
 class TissueGrid:
-    """Field-for-field the reference lattice record (`grid.py:41-61`)."""
+    """The lattice record, field for field the reference's (`grid.py:41-61`)."""
     spacing: float
     cell_size: np.ndarray
     dims: tuple
@@ -128,50 +140,34 @@ def connectivity_table():
     return table
 
 
+_CONNECTIVITY = None
+
+
+def connectivity_table_cached():
+    global _CONNECTIVITY
+    if _CONNECTIVITY is None:
+        _CONNECTIVITY = connectivity_table()
+    return _CONNECTIVITY.copy()
+
+
+# ------------------------------------------------------------------------------------------------------ tissue shapes
+# A shape gives its bounds `low` and `high`, `inside(positions)`, `boundary(positions, a, b, inside)` (for lattice
+# edges a -> b with one end inside: the fraction along a -> b where the edge leaves the tissue, and the outward unit
+# normal there) and `bed(positions)`, the nodes pinned where they are active -- the protocol of `grid.py:176-186`.
+
 @dataclass(frozen=True)
 class Tab:
-    """Box grown from a positive face of the block (`grid.py:27-38`)."""
+    """Box of tissue grown from a positive face of the block, in cm (`grid.py:27-38`): corners `low` and `high`,
+    its `low` face along axis `attached` on the block's positive face of that axis."""
     low: tuple
     high: tuple
     attached: int
 
 
-def _exit_along_axis(points, axis, direction, lows, highs):
-    """Where rays from interior points along +-axis leave a union of open boxes (`grid.py:85-106`).
-
-    Boxes whose open cross-section contains the ray and that overlap along it chain into one interval, so the exit is
-    the far end of the chain that contains the start.
-    """
-    n = len(points)
-    exits = np.empty(n, dtype=np.float64)
-    for r in range(n):
-        p = points[r]
-        s = float(direction[r]) * float(p[axis])
-        spans = []
-        for lo, hi in zip(lows, highs):
-            others = [k for k in range(3) if k != axis]
-            if all(lo[k] < p[k] < hi[k] for k in others):
-                if direction[r] > 0:
-                    spans.append((float(lo[axis]), float(hi[axis])))
-                else:
-                    spans.append((-float(hi[axis]), -float(lo[axis])))
-        reach = -np.inf
-        for near, far in spans:
-            if near < s < far:
-                reach = max(reach, far)
-        for _ in range(len(spans)):
-            for near, far in spans:
-                if near < reach < far:
-                    reach = max(reach, far)
-        if not np.isfinite(reach):
-            raise AssertionError("an interior point lies outside every box")
-        exits[r] = direction[r] * reach
-    return exits
-
-
 @dataclass(frozen=True)
 class BoxUnion:
-    """Axis-aligned boxes in cm: the block centred on the origin plus an optional tab (`grid.py:112-160`)."""
+    """Tissue made of axis-aligned open boxes in cm: the block centred on the origin, then an optional tab; the bed is
+    the block's bottom face (`half` is the block's half extent)."""
     lows: np.ndarray
     highs: np.ndarray
     half: np.ndarray
@@ -184,22 +180,48 @@ class BoxUnion:
     def high(self):
         return self.highs.max(axis=0)
 
+    def _contains(self, points, box):
+        return np.all((self.lows[box] < points) & (points < self.highs[box]), axis=-1)
+
     def inside(self, positions):
-        p = positions[:, None, :]
-        return ((self.lows < p) & (p < self.highs)).all(axis=2).any(axis=1)
+        result = np.zeros(len(positions), dtype=bool)
+        for box in range(len(self.lows)):
+            result |= self._contains(positions, box)
+        return result
 
     def boundary(self, positions, a, b, inside):
+        """Each edge runs along one lattice axis from its inside end. Along that line the union is open, so the first
+        point where it leaves the tissue is a box bound: the nearest bound ahead of the inside end at which the line is
+        in no box's interior. The normal is the edge axis, pointing out."""
+        a, b = np.asarray(a), np.asarray(b)
         axis = np.abs(positions[b] - positions[a]).argmax(axis=1)
-        interior = np.where(inside[a] != 0, a, b)
-        exterior = np.where(inside[a] != 0, b, a)
-        rows = np.arange(len(a))
-        direction = np.sign(positions[exterior, axis] - positions[interior, axis])
-        starts = positions[interior]
-        exit_coord = np.array([_exit_along_axis(starts[r:r + 1], int(axis[r]), direction[r:r + 1],
-                                                self.lows, self.highs)[0] for r in rows])
-        t = (exit_coord - positions[a, axis]) / (positions[b, axis] - positions[a, axis])
+        inner = np.where(inside[a] != 0, a, b)
+        outer = np.where(inside[a] != 0, b, a)
+        sign = np.sign(positions[outer, axis] - positions[inner, axis])
+        exit = np.empty(len(a), dtype=np.float32)
+        for ax in range(3):
+            rows = np.flatnonzero(axis == ax)
+            if len(rows) == 0:
+                continue
+            start = positions[inner[rows]]
+            bounds = np.unique(np.concatenate((self.lows[:, ax], self.highs[:, ax]))).astype(np.float32)
+            others = [k for k in range(3) if k != ax]
+            # the line's cross-section lies in box j's open face; bound c lies in box j's open span along the axis
+            section = np.stack([np.all((self.lows[j, others] < start[:, others]) &
+                                       (start[:, others] < self.highs[j, others]), axis=1)
+                                for j in range(len(self.lows))], axis=1)
+            span = (self.lows[:, ax, None] < bounds[None, :]) & (bounds[None, :] < self.highs[:, ax, None])
+            covered = (section[:, :, None] & span[None, :, :]).any(axis=1)
+            s = sign[rows, None]
+            ahead = s * bounds[None, :] > s * start[:, ax, None]
+            key = np.where(ahead & ~covered, s * bounds[None, :], np.inf)
+            if not np.isfinite(key.min(axis=1)).all():
+                raise AssertionError("an inside node lies in no box")
+            exit[rows] = bounds[key.argmin(axis=1)]
+        pa, pb = positions[a, axis], positions[b, axis]
+        t = (exit - pa) / (pb - pa)
         normals = np.zeros((len(a), 3), dtype=np.float32)
-        normals[rows, axis] = direction
+        normals[np.arange(len(a)), axis] = sign
         return t, normals
 
     def bed(self, positions):
@@ -207,24 +229,36 @@ class BoxUnion:
 
 
 def implicit_boundary(distance, positions, a, b):
-    """Zero crossing of a signed distance on each lattice edge a->b by 40 bisections, plus the outward normal from a
-    central-difference gradient (`grid.py:163-173`)."""
+    """Crossing of a signed distance (negative inside) on each lattice edge a -> b, for shapes given only by their
+    distance: a bracketed Illinois (modified regula falsi) iteration to the float64 resolution of the fraction, then
+    the outward unit normal from central differences of the distance over a thousandth of the edge."""
     pa, pb = positions[a].astype(np.float64), positions[b].astype(np.float64)
+    d = pb - pa
+    f = lambda t: distance(pa + d * t[:, None])
     lo, hi = np.zeros(len(a)), np.ones(len(a))
-    sign_a = np.sign(distance(pa))
-    for _ in range(40):
-        mid = (lo + hi) / 2
-        same = np.sign(distance(pa + (pb - pa) * mid[:, None])) == sign_a
-        lo, hi = np.where(same, mid, lo), np.where(same, hi, mid)
-    t = (lo + hi) / 2
-    hit = pa + (pb - pa) * t[:, None]
-    step = 1e-4
+    flo, fhi = f(lo), f(hi)
+    for _ in range(200):
+        t = np.where(fhi != flo, (lo * fhi - hi * flo) / (fhi - flo), 0.5 * (lo + hi))
+        t = np.where((t > lo) & (t < hi), t, 0.5 * (lo + hi))
+        ft = f(t)
+        left = np.sign(ft) == np.sign(flo)
+        # Illinois: halve the value at the end that stays, so the bracket shrinks from both sides
+        fhi = np.where(left, fhi * 0.5, ft)
+        flo = np.where(left, ft, flo * 0.5)
+        lo, hi = np.where(left, t, lo), np.where(left, hi, t)
+        done = (hi - lo <= 4 * np.finfo(np.float64).eps) | (ft == 0)
+        if done.all():
+            break
+    t = 0.5 * (lo + hi)
+    hit = pa + d * t[:, None]
+    step = 1e-3 * np.linalg.norm(d, axis=1, keepdims=True)
     gradient = np.stack([distance(hit + step * e) - distance(hit - step * e) for e in np.eye(3)], axis=1)
     return t, (gradient / np.linalg.norm(gradient, axis=1, keepdims=True)).astype(np.float32)
 
 
 def ellipsoid_distance(points, half_extents):
-    """Signed distance bound to an origin-centred ellipsoid, exact on its surface (`kidney.py:15-20`)."""
+    """Signed distance estimate to an origin-centred ellipsoid, exact on its surface: k0 (k0 - 1) / k1 with
+    k0 = |p / r| and k1 = |p / r^2|."""
     k0 = np.linalg.norm(points / half_extents, axis=1)
     k1 = np.maximum(np.linalg.norm(points / np.square(half_extents), axis=1), 1e-12)
     return k0 * (k0 - 1) / k1
@@ -233,8 +267,9 @@ def ellipsoid_distance(points, half_extents):
 @dataclass(frozen=True)
 class Ellipsoid:
     """Ellipsoidal organ of `half_extents` cm centred on the origin, resting on a bed: nodes below `bed_height` are
-    pinned. This is the kidney-scale benchmark body; the reference's kidney (`kidney.py:23-81`) is an ellipsoid plus
-    a tumour lobe over the same implicit-shape protocol."""
+    pinned. This is the kidney-scale benchmark body (the reference's kidney, `kidney.py:23-81`, is an ellipsoid plus a
+    tumour lobe over the same shape protocol). Inside is |p / r| < 1; an edge's crossing solves the quadratic
+    |(pa + t d) / r|^2 = 1 and its normal is the ellipsoid's, p / r^2 normalised."""
     half_extents: tuple = (6.0, 3.0, 2.5)
     bed_fraction: float = 0.15
 
@@ -250,143 +285,237 @@ class Ellipsoid:
     def bed_height(self):
         return -self.half_extents[1] * (1 - 2 * self.bed_fraction)
 
+    def _level(self, points):
+        r = np.array(self.half_extents, dtype=np.float64)
+        return np.square(np.asarray(points, dtype=np.float64) / r).sum(axis=1)
+
     def distance(self, points):
         return ellipsoid_distance(points, np.array(self.half_extents, dtype=np.float64))
 
     def inside(self, positions):
-        return self.distance(positions.astype(np.float64)) < 0
+        return self._level(positions) < 1.0
 
     def boundary(self, positions, a, b, inside):
-        return implicit_boundary(self.distance, positions, a, b)
+        r = np.array(self.half_extents, dtype=np.float64)
+        pa = positions[a].astype(np.float64) / r
+        pd = (positions[b].astype(np.float64) - positions[a].astype(np.float64)) / r
+        qa, qb, qc = (pd * pd).sum(axis=1), 2 * (pa * pd).sum(axis=1), (pa * pa).sum(axis=1) - 1
+        root = np.sqrt(np.maximum(qb * qb - 4 * qa * qc, 0.0))
+        w = -0.5 * (qb + np.copysign(root, qb))  # the two roots are w / qa and qc / w, without cancellation
+        t1 = w / qa
+        t2 = np.divide(qc, w, out=np.full_like(w, np.inf), where=w != 0)
+        t = np.where((t1 >= 0) & (t1 <= 1), t1, t2)
+        hit = positions[a].astype(np.float64) + (positions[b].astype(np.float64) - positions[a]) * t[:, None]
+        n = hit / np.square(r)
+        return t, (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
 
     def bed(self, positions):
         return positions[:, 1] < self.bed_height
 
 
-def _assemble(spacing, cell_size, dims, origin, positions, inside, bed_mask, density, shape=None,
-              keep_all_active=False):
-    """Everything of a TissueGrid that follows from the lattice geometry and the inside flags."""
-    nc = tuple(int(d) + 1 for d in dims)
-    ijk = lattice_coordinates(nc)
-    strides = np.array((1, nc[0], nc[0] * nc[1]), dtype=np.int64)
-    voxel_ijk = lattice_coordinates(tuple(int(d) for d in dims))
-    corners = ((voxel_ijk[:, None, :].astype(np.int64) + CORNER_OFFSETS) @ strides).astype(np.int32)
-    occupied = inside[corners].any(axis=1).astype(np.int32)
-    active = np.zeros(len(positions), dtype=np.int32)
-    active[corners[occupied != 0]] = 1
-    if keep_all_active:
-        active[:] = 1
-    pinned = (bed_mask & (active != 0)).astype(np.int32)
+# ------------------------------------------------------------------------------------------------------ lattice build
 
-    # Voxel edges whose ends disagree on inside carry a surface crossing (`grid.py:224-236`).
-    ca = corners[:, EDGE_CORNERS[:, 0]]
-    cb = corners[:, EDGE_CORNERS[:, 1]]
-    crossing = inside[ca] != inside[cb]
-    counts = crossing.sum(axis=1, dtype=np.int32)
-    offsets = (np.cumsum(counts, dtype=np.int32) - counts).astype(np.int32)
-    a, b = ca[crossing], cb[crossing]
-    intersections = np.empty(len(a), dtype=ISECT_DTYPE)
-    if len(a):
-        t, normals = shape.boundary(positions, a, b, inside)
-        hit = positions[a] + (positions[b] - positions[a]) * t[:, None]
-        intersections["position"], intersections["normal"] = hit, normals
-        intersections["a"], intersections["b"] = a, b
-        intersections["t"], intersections["inside_a"] = t, inside[a]
+class _Blocks:
+    """3-D views of a lattice of `dims` voxels: node and voxel arrays in (z, y, x) order, x fastest when flat."""
 
-    neighbors = np.full((len(positions), 6), -1, dtype=np.int32)
-    rest = np.zeros((len(positions), 6, 3), dtype=np.float32)
-    edge_parts, surface_parts = [], []
+    def __init__(self, dims):
+        self.dims = tuple(int(d) for d in dims)
+        vx, vy, vz = self.dims
+        self.nodes3 = (vz + 1, vy + 1, vx + 1)
+        self.voxels3 = (vz, vy, vx)
+        self.strides = np.array((1, vx + 1, (vx + 1) * (vy + 1)), dtype=np.int64)
+        self.ids = np.arange(int(np.prod(self.nodes3)), dtype=np.int64).reshape(self.nodes3)
+
+    def corner_view(self, nodes3, bit):
+        """The nodes at bit-order corner `bit` of every voxel, as a voxel-shaped view of a node array."""
+        vz, vy, vx = self.voxels3
+        dx, dy, dz = bit & 1, (bit >> 1) & 1, (bit >> 2) & 1
+        return nodes3[dz:dz + vz, dy:dy + vy, dx:dx + vx]
+
+    def pairs(self, axis):
+        """Slices of the lower and upper node of every lattice edge along `axis` (axis 0 = x = last array axis)."""
+        k = 2 - axis
+        lower = [slice(None)] * 3
+        upper = [slice(None)] * 3
+        lower[k], upper[k] = slice(0, -1), slice(1, None)
+        return tuple(lower), tuple(upper)
+
+
+def _occupancy(blocks, inside3):
+    """Occupied voxels (a corner inside), active nodes (a corner of an occupied voxel) and supported voxels (all eight
+    corners active)."""
+    occupied = np.zeros(blocks.voxels3, dtype=bool)
+    for bit in range(8):
+        occupied |= blocks.corner_view(inside3, bit)
+    active = np.zeros(blocks.nodes3, dtype=bool)
+    for bit in range(8):
+        blocks.corner_view(active, bit)[...] |= occupied
+    supported = np.ones(blocks.voxels3, dtype=bool)
+    for bit in range(8):
+        supported &= blocks.corner_view(active, bit)
+    return occupied, active, supported
+
+
+def _links(blocks, active3, cell_size):
+    """Six-neighbour table and rest vectors of the links between active nodes."""
+    n = blocks.ids.size
+    neighbors = np.full((n, 6), -1, dtype=np.int32)
+    rest = np.zeros((n, 6, 3), dtype=np.float32)
     for axis in range(3):
-        below = ijk[:, axis] < dims[axis]
-        lower = np.flatnonzero(below)
-        upper = lower + strides[axis]
-        linked = (active[lower] != 0) & (active[upper] != 0)
-        neighbors[lower[linked], 2 * axis + 1] = upper[linked]
-        neighbors[upper[linked], 2 * axis] = lower[linked]
-        rest[lower[linked], 2 * axis + 1, axis] = cell_size[axis]
-        rest[upper[linked], 2 * axis, axis] = -cell_size[axis]
-        # Cut edges need all four incident voxels in bounds (`grid.py:249-255`, `cut.cuh:140-144`).
-        interior = below.copy()
+        lo, hi = blocks.pairs(axis)
+        linked = active3[lo] & active3[hi]
+        lower, upper = blocks.ids[lo][linked], blocks.ids[hi][linked]
+        neighbors[lower, 2 * axis + 1] = upper
+        neighbors[upper, 2 * axis] = lower
+        rest[lower, 2 * axis + 1, axis] = cell_size[axis]
+        rest[upper, 2 * axis, axis] = -cell_size[axis]
+    return neighbors, rest
+
+
+def _cut_edges(blocks, inside):
+    """Lattice edges that may be cut -- those with all four incident voxels in the lattice (`cut.cuh:140-144`) -- as
+    (axis, x, y, z) records axis by axis in node order, and the ones that cross the skin as 32-byte records with the
+    inside flag of their lower end."""
+    edge_parts, surface_parts = [], []
+    vx, vy, vz = blocks.dims
+    nx, ny = vx + 1, vy + 1
+    for axis in range(3):
+        keep = np.ones(blocks.nodes3, dtype=bool)
         for other in range(3):
-            if other != axis:
-                interior &= (ijk[:, other] > 0) & (ijk[:, other] < dims[other])
-        base = np.flatnonzero(interior)
-        edge = np.column_stack((np.full(len(base), axis), ijk[base])).astype(np.int32)
+            k = 2 - other
+            sl = [slice(None)] * 3
+            if other == axis:
+                sl[k] = slice(blocks.dims[other], None)  # no upper end
+                keep[tuple(sl)] = False
+            else:
+                sl[k] = 0
+                keep[tuple(sl)] = False
+                sl[k] = blocks.dims[other]
+                keep[tuple(sl)] = False
+        base = np.flatnonzero(keep.ravel())
+        x, y, z = base % nx, (base // nx) % ny, base // (nx * ny)
+        edge = np.column_stack((np.full(len(base), axis), x, y, z)).astype(np.int32)
         edge_parts.append(edge)
-        cross = inside[base] != inside[base + strides[axis]]
-        surface = np.zeros((int(np.count_nonzero(cross)), 8), dtype=np.int32)
+        cross = inside[base] != inside[base + blocks.strides[axis]]
+        surface = np.zeros((int(cross.sum()), 8), dtype=np.int32)
         surface[:, :4] = edge[cross]
         surface[:, 4] = inside[base[cross]]
         surface_parts.append(surface)
+    return np.concatenate(edge_parts), np.concatenate(surface_parts)
 
-    bends = np.zeros((len(positions), 15), dtype=BEND_DTYPE)
-    bends["i"] = np.arange(len(positions), dtype=np.int32)[:, None]
+
+def _bends(neighbors):
+    """Fifteen bend records per node over its slot pairs: j, k the neighbours, alive where both exist, rest angle pi
+    for the two slots of one axis and pi/2 otherwise (`grid.py:257-268`)."""
+    n = len(neighbors)
+    bends = np.zeros((n, 15), dtype=BEND_DTYPE)
+    bends["i"] = np.arange(n, dtype=np.int32)[:, None]
     bends["j"] = neighbors[:, BEND_SLOT_PAIRS[:, 0]]
     bends["k"] = neighbors[:, BEND_SLOT_PAIRS[:, 1]]
     bends["alive"] = (bends["j"] >= 0) & (bends["k"] >= 0)
-    straight = BEND_SLOT_PAIRS[:, 0] // 2 == BEND_SLOT_PAIRS[:, 1] // 2
-    bends["theta"] = np.where(straight, np.pi, np.pi / 2)
+    same_axis = (BEND_SLOT_PAIRS[:, 0] >> 1) == (BEND_SLOT_PAIRS[:, 1] >> 1)
+    bends["theta"] = np.where(same_axis, np.pi, np.pi / 2)
+    return bends
 
-    # Each Kuhn tet of a supported cell lumps a quarter of its mass (cell/6) on each corner: rho*h^3/24 per
-    # tet corner (`grid.py:269-277`).
-    bit_corners = corners[:, BIT_TO_FIGURE]
-    supported = np.all(active[bit_corners] != 0, axis=1)
-    tet_nodes = bit_corners[supported][:, KUHN_TETS].reshape(-1)
-    mass = (np.bincount(tet_nodes, minlength=len(positions)) * density *
-            np.prod(cell_size * METERS_PER_UNIT) / 24).astype(np.float32)
+
+def _lumped_mass(blocks, supported, cell_size, density):
+    """Mass of each node: every supported cell's six Kuhn tets lump rho V_cell / 24 on each of their corners
+    (`grid.py:269-277`), so a cell gives its diagonal ends 6/24 and its other corners 2/24 of its mass."""
+    count = np.zeros(blocks.nodes3, dtype=np.int64)
+    for bit in range(8):
+        blocks.corner_view(count, bit)[...] += KUHN_MULTIPLICITY[bit] * supported
+    cell_mass = density * float(np.prod(cell_size.astype(np.float64) * METERS_PER_UNIT))
+    return (count.ravel() * (cell_mass / 24.0)).astype(np.float32)
+
+
+def _lattice(spacing, cell_size, dims, origin, positions, inside, bed, density, shape=None):
+    """The TissueGrid of a placed lattice from its inside flags."""
+    blocks = _Blocks(dims)
+    inside = np.asarray(inside, dtype=np.int32)
+    occupied3, active3, supported = _occupancy(blocks, inside.reshape(blocks.nodes3) != 0)
+    active = active3.ravel().astype(np.int32)
+    pinned = (np.asarray(bed, dtype=bool) & (active != 0)).astype(np.int32)
+    vz, vy, vx = blocks.voxels3
+    z, y, x = np.meshgrid(np.arange(vz), np.arange(vy), np.arange(vx), indexing="ij")
+    base = (x + (vx + 1) * (y + (vy + 1) * z)).ravel()
+    corners = (base[:, None] + CORNER_OFFSETS.astype(np.int64) @ blocks.strides).astype(np.int32)
+
+    # the skin crossings of every voxel's edges, voxel by voxel in EDGE_CORNERS order
+    end_a, end_b = corners[:, EDGE_CORNERS[:, 0]], corners[:, EDGE_CORNERS[:, 1]]
+    crossing = inside[end_a] != inside[end_b]
+    counts = crossing.sum(axis=1).astype(np.int32)
+    offsets = np.concatenate(([0], np.cumsum(counts)[:-1])).astype(np.int32)
+    a, b = end_a[crossing], end_b[crossing]
+    intersections = np.zeros(len(a), dtype=ISECT_DTYPE)
+    if len(a):
+        t, normals = shape.boundary(positions, a, b, inside)
+        intersections["position"] = positions[a] + (positions[b] - positions[a]) * t[:, None]
+        intersections["normal"] = normals
+        intersections["a"], intersections["b"], intersections["t"] = a, b, t
+        intersections["inside_a"] = inside[a]
+
+    neighbors, rest = _links(blocks, active3, cell_size)
+    edges, surface_edges = _cut_edges(blocks, inside)
+    mass = _lumped_mass(blocks, supported, cell_size, density)
     if not np.all(mass[active != 0] > 0):
         raise ValueError("an active node belongs to no supported cell")
     return TissueGrid(spacing, cell_size, tuple(int(d) for d in dims), origin, positions, inside, active, pinned, mass,
-                      neighbors, rest, bends, corners, occupied, intersections, offsets, counts,
-                      np.concatenate(surface_parts), np.concatenate(edge_parts), connectivity_table_cached())
+                      neighbors, rest, _bends(neighbors), corners, occupied3.ravel().astype(np.int32), intersections,
+                      offsets, counts, surface_edges, edges, connectivity_table_cached())
 
 
-_CONNECTIVITY = None
+def _placement(low, high, cell_size):
+    """Voxel dims and origin of the lattice around bounds low..high: the extent in whole cells plus SHELL_CELLS of air
+    on each side, centred on the bounds' centre."""
+    dims = tuple(int(v) for v in np.ceil((high - low) / cell_size).astype(int) + 2 * SHELL_CELLS)
+    origin = (low + high) / 2 - np.array(dims, dtype=np.float32) * cell_size / 2
+    return dims, origin
 
 
-def connectivity_table_cached():
-    global _CONNECTIVITY
-    if _CONNECTIVITY is None:
-        _CONNECTIVITY = connectivity_table()
-    return _CONNECTIVITY.copy()
+def _node_positions(origin, dims, cell_size):
+    index = lattice_coordinates(tuple(d + 1 for d in dims))
+    return np.ascontiguousarray(origin + index * cell_size, dtype=np.float32)
+
+
+def _tab_box(tab, half, cell_size):
+    """The tab's box: its faces within 1e-6 cm of a block face moved onto that face (one node layer for both), and its
+    attached face pushed one cell into the block, so that the nodes on the shared face lie inside the union."""
+    low = np.array(tab.low, dtype=np.float32)
+    high = np.array(tab.high, dtype=np.float32)
+    ok = (np.isfinite(low).all() and np.isfinite(high).all() and tab.attached in (0, 1, 2)
+          and np.all(high - low >= 2 * cell_size - 1e-6) and abs(low[tab.attached] - half[tab.attached]) < 1e-6)
+    if not ok:
+        raise ValueError(f"invalid tab {tab}: it must be at least two cells thick along every axis and start on a "
+                         "positive face of the block")
+    faces = np.stack((-half, half))
+    for bound in (low, high):
+        near = np.abs(bound[None, :] - faces) < 1e-6
+        for side in range(2):
+            bound[near[side]] = faces[side][near[side]]
+    low[tab.attached] = half[tab.attached] - cell_size[tab.attached]
+    return low, high
 
 
 def build_grid(spacing=0.6, size=BLOCK_SIZE, density=1000.0, vertical_spacing=None, tab=None, shape=None):
-    """Lattice of the block of `size` (with an optional `tab`) or of an implicit `shape`, in a two-cell air shell.
-
-    Same contract as the reference builder (`grid.py:176-281`): a shape gives `low`, `high`, `inside(positions)`,
-    `boundary(positions, a, b, inside)` and `bed(positions)`.
-    """
+    """Lattice of the block of `size` (with an optional `tab`) or of an implicit `shape`, in a two-cell air shell
+    (the contract of `grid.py:176-186`: lattice spacing 1..7 mm, a vertical spacing between 0.1 mm and the spacing)."""
     if not np.isfinite(spacing) or not 0.1 <= spacing <= 0.7:
-        raise ValueError("spacing must be 0.1..0.7 native units (1..7 mm)")
+        raise ValueError(f"lattice spacing {spacing} cm is outside 0.1..0.7 cm (1..7 mm)")
     cell_size = np.full(3, spacing, dtype=np.float32)
     if vertical_spacing is not None:
         if not np.isfinite(vertical_spacing) or not .01 <= vertical_spacing <= spacing:
-            raise ValueError("vertical spacing must be 0.01..spacing native units")
+            raise ValueError(f"vertical spacing {vertical_spacing} cm is outside 0.01 cm..the lattice spacing")
         cell_size[1] = vertical_spacing
     if shape is None:
         half = np.array(size, dtype=np.float32) / 2
-        lows, highs = [-half], [half]
+        boxes = [(-half, half)]
         if tab is not None:
-            low, high = np.array(tab.low, dtype=np.float32), np.array(tab.high, dtype=np.float32)
-            if not (np.isfinite((low, high)).all() and tab.attached in (0, 1, 2)
-                    and np.all(high - low >= 2 * cell_size - 1e-6) and abs(low[tab.attached] - half[tab.attached]) < 1e-6):
-                raise ValueError("tab must span two cells per axis and grow from a positive face of the block")
-            for bound, face in ((low, -half), (low, half), (high, -half), (high, half)):
-                flush = np.abs(bound - face) < 1e-6
-                bound[flush] = face[flush]
-            low[tab.attached] = half[tab.attached] - cell_size[tab.attached]
-            lows.append(low)
-            highs.append(high)
-        shape = BoxUnion(np.array(lows), np.array(highs), half)
-    low, high = shape.low, shape.high
-    dims = tuple(int(v) for v in np.ceil((high - low) / cell_size).astype(int) + 4)
-    origin = (low + high) / 2 - np.array(dims, dtype=np.float32) * cell_size / 2
-    nc = tuple(d + 1 for d in dims)
-    ijk = lattice_coordinates(nc)
-    positions = np.ascontiguousarray(origin + ijk * cell_size, dtype=np.float32)
+            boxes.append(_tab_box(tab, half, cell_size))
+        shape = BoxUnion(np.array([lo for lo, _ in boxes]), np.array([hi for _, hi in boxes]), half)
+    dims, origin = _placement(shape.low, shape.high, cell_size)
+    positions = _node_positions(origin, dims, cell_size)
     inside = shape.inside(positions).astype(np.int32)
-    return _assemble(spacing, cell_size, dims, origin, positions, inside, shape.bed(positions), density, shape)
+    return _lattice(spacing, cell_size, dims, origin, positions, inside, shape.bed(positions), density, shape)
 
 
 def block_grid(nodes=(17, 9, 9), spacing=0.25, density=1000.0, origin=(0.0, 0.0, 0.0)):
@@ -398,10 +527,10 @@ def block_grid(nodes=(17, 9, 9), spacing=0.25, density=1000.0, origin=(0.0, 0.0,
     cell_size = np.full(3, spacing, dtype=np.float32)
     dims = tuple(n - 1 for n in nodes)
     origin = np.array(origin, dtype=np.float32)
-    ijk = lattice_coordinates(nodes)
-    positions = np.ascontiguousarray(origin + ijk * cell_size, dtype=np.float32)
+    positions = _node_positions(origin, dims, cell_size)
     inside = np.ones(len(positions), dtype=np.int32)
-    return _assemble(float(spacing), cell_size, dims, origin, positions, inside, ijk[:, 1] == 0, density)
+    return _lattice(float(spacing), cell_size, dims, origin, positions, inside, lattice_coordinates(nodes)[:, 1] == 0,
+                    density)
 
 
 def block_nodes_for_tets(tets, aspect=(2, 1, 1)):

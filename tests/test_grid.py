@@ -150,3 +150,45 @@ def test_kidney_scale_lattice():
     w = kidney_250k()
     T = tet_count(w.grid)
     assert 240_000 <= T <= 260_000
+
+
+def test_ellipsoid_crossing_matches_the_implicit_root_finder():
+    """The ellipsoid's closed-form crossings (a quadratic along each edge) and analytic normals against the generic
+    bracketed root finder on its signed distance, edge by edge."""
+    from genesis_tissue_simulation_amd.grid import implicit_boundary
+    shape = Ellipsoid((6.0, 3.0, 2.5))
+    g = build_grid(0.3, shape=shape)
+    a, b = g.intersections["a"], g.intersections["b"]
+    t_closed, n_closed = shape.boundary(g.positions, a, b, g.inside)
+    t_root, n_root = implicit_boundary(shape.distance, g.positions, a, b)
+    np.testing.assert_allclose(t_closed, t_root, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(n_closed, n_root, rtol=0, atol=2e-6)
+    assert np.all((t_closed > 0) & (t_closed < 1))
+
+
+def test_box_union_exit_against_a_march():
+    """Where an axis ray from an inside node leaves a union of open boxes: boxes that only touch do not chain (a
+    point on the shared face is in neither), overlapping ones do. Checked against a fine march along each ray."""
+    from genesis_tissue_simulation_amd.grid import BoxUnion
+    half = np.array((1.0, 1.0, 1.0), dtype=np.float32)
+    for second_low_x in (1.0, 0.5):  # touching the block's +x face, then reaching into it
+        lows = np.array(((-1.0, -1.0, -1.0), (second_low_x, -0.5, -0.5)), dtype=np.float32)
+        highs = np.array(((1.0, 1.0, 1.0), (2.5, 0.5, 0.5)), dtype=np.float32)
+        u = BoxUnion(lows, highs, half)
+        pos = np.array(((0.25, 0.25, 0.25), (3.0, 0.25, 0.25), (0.25, 0.75, 0.25), (0.25, 3.0, 0.25),
+                        (0.25, 0.25, -0.25), (0.25, 0.25, -3.0)), dtype=np.float32)
+        ins = u.inside(pos).astype(np.int32)
+        a, b = np.array((0, 2, 4)), np.array((1, 3, 5))
+        t, n = u.boundary(pos, a, b, ins)
+        for k in range(3):
+            pa, pb = pos[a[k]].astype(np.float64), pos[b[k]].astype(np.float64)
+            ax = int(np.argmax(np.abs(pb - pa)))
+            # dense samples plus every box bound on the ray: a face shared by touching boxes is a single outside point
+            at_bounds = (np.concatenate((lows[:, ax], highs[:, ax])) - pa[ax]) / (pb[ax] - pa[ax])
+            s = np.unique(np.concatenate((np.linspace(0, 1, 400001), at_bounds[(at_bounds >= 0) & (at_bounds <= 1)])))
+            inside = u.inside((pa + (pb - pa) * s[:, None]).astype(np.float32))
+            first_out = s[np.argmax(~inside)]
+            assert abs(t[k] - first_out) <= 1e-5, (second_low_x, k, t[k], first_out)
+            assert np.dot(n[k], pb - pa) > 0
+        expect_x = 1.0 if second_low_x == 1.0 else 2.5
+        np.testing.assert_allclose(pos[0, 0] + t[0] * (pos[1, 0] - pos[0, 0]), expect_x, atol=1e-6)
