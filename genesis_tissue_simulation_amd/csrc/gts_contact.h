@@ -1,6 +1,12 @@
-// Swept tool contact of lattice nodes (reference: contact/friction_contact.cuh:40-169), device side.
-// Same operation sequence as the CPU checker's restatement, in single precision except the piecewise-quadratic box
-// time, which the reference evaluates in double.
+// Where a lattice node, moving from x0 to x1 during a substep, first touches a tool shape that moves from pose `from`
+// to pose `to`, and the contact record the friction solve uses: the outward normal, the target point on the inflated
+// surface and the tool's own motion at the touch point.
+//
+// Reference: contact/friction_contact.cuh:40-142 (shapeDistance, shapeLerp, rotateShaftVector, witnessAt, pointBoxTime,
+// traceToolContact). The method is the reference's -- closest-point distances to a capsule or an oriented box, a box
+// that only translates swept exactly (its squared distance to a moving point is piecewise quadratic in time), anything
+// else by conservative advancement -- formulated here around a surface query record, the box's local frame and a
+// piece-local time variable. The CPU checker restates this formulation step for step, so both round alike.
 #pragma once
 #include "../../include/gts_xpbd.h"
 #include "gts_common.h"
@@ -21,167 +27,231 @@ struct ContactRecord {
     int valid;
 };
 
-__device__ __forceinline__ float shape_distance(const Shape& s, f3 x, f3& normal, f3& surface) {
-    if (s.capsule) {
-        f3 ab = s.b - s.a;
-        float tt = fminf(1.f, fmaxf(0.f, dot3(x - s.a, ab) / fmaxf(dot3(ab, ab), 1e-20f)));
-        f3 d = x - (s.a + ab * tt);
-        float l = len3(d);
-        f3 fallback = unit_or(cross3(ab, fabsf(ab.x) < fabsf(ab.y) ? mk3(1, 0, 0) : mk3(0, 1, 0)), mk3(0, 0, 1));
-        normal = l > 1e-6f * fmaxf(1.f, len3(ab)) ? d * (1.f / l) : fallback;
-        surface = s.a + ab * tt + normal * s.radius;
-        return l - s.radius;
-    }
-    f3 r = x - s.center;
-    float v[3] = {dot3(r, s.u), dot3(r, s.n), dot3(r, s.w)};
-    float h[3] = {s.half.x, s.half.y, s.half.z};
-    float q[3], cl[3];
-    for (int k = 0; k < 3; k++) {
-        q[k] = fabsf(v[k]) - h[k];
-        cl[k] = fminf(h[k], fmaxf(-h[k], v[k]));
-    }
-    f3 closest = s.center + s.u * cl[0] + s.n * cl[1] + s.w * cl[2];
-    f3 d = s.u * copysignf(fmaxf(q[0], 0.f), v[0]) + s.n * copysignf(fmaxf(q[1], 0.f), v[1]) +
-           s.w * copysignf(fmaxf(q[2], 0.f), v[2]);
-    float len = len3(d);
-    if (q[0] > 0 || q[1] > 0 || q[2] > 0) {
-        normal = len > 0 ? d * (1.f / len) : s.n;
-        surface = closest;
-        return len;
-    }
-    int k = q[1] > q[0] ? 1 : 0;
-    if (q[2] > q[k]) k = 2;
-    f3 ax = k == 0 ? s.u : (k == 1 ? s.n : s.w);
-    normal = ax * (v[k] >= 0 ? 1.f : -1.f);
-    surface = x - normal * q[k];
-    return q[k];
+// A point against a shape: signed gap (negative inside), outward unit normal, nearest point of the shape's surface.
+struct ToolHit {
+    float gap;
+    f3 normal, foot;
+};
+
+// A unit vector across the spine direction, built on the coordinate axis least aligned with it.
+__device__ __forceinline__ f3 across_spine(f3 spine) {
+    f3 ax = fabsf(spine.x) <= fabsf(spine.y) && fabsf(spine.x) <= fabsf(spine.z)
+                ? mk3(1, 0, 0)
+                : (fabsf(spine.y) <= fabsf(spine.z) ? mk3(0, 1, 0) : mk3(0, 0, 1));
+    return unit_or(cross3(spine, ax), mk3(0, 0, 1));
 }
 
-__device__ __forceinline__ Shape shape_lerp(const Shape& a, const Shape& b, float t) {
-    Shape s = b;
-    s.center = a.center * (1.f - t) + b.center * t;
-    s.a = a.a * (1.f - t) + b.a * t;
-    s.b = a.b * (1.f - t) + b.b * t;
-    s.u = unit_or(a.u * (1.f - t) + b.u * t, b.u);
-    s.n = a.n * (1.f - t) + b.n * t;
-    s.n = unit_or(s.n - s.u * dot3(s.u, s.n), b.n);
+// Parameter of the spine point nearest x, clamped to the segment.
+__device__ __forceinline__ float spine_param(const Shape& s, f3 x) {
+    f3 spine = s.b - s.a;
+    return fminf(1.f, fmaxf(0.f, dot3(x - s.a, spine) / fmaxf(dot3(spine, spine), 1e-20f)));
+}
+
+__device__ __forceinline__ ToolHit capsule_hit(const Shape& s, f3 x) {
+    f3 spine = s.b - s.a;
+    f3 base = s.a + spine * spine_param(s, x);
+    f3 off = x - base;
+    float dist = len3(off);
+    ToolHit h;
+    // A node on the spine: the offset is projection roundoff along the spine, not a direction out of the capsule.
+    h.normal = dist > 1e-6f * fmaxf(1.f, len3(spine)) ? off * (1.f / dist) : across_spine(spine);
+    h.foot = base + h.normal * s.radius;
+    h.gap = dist - s.radius;
+    return h;
+}
+
+__device__ __forceinline__ ToolHit box_hit(const Shape& s, f3 x) {
+    const f3 r = x - s.center;
+    const float l0 = dot3(r, s.u), l1 = dot3(r, s.n), l2 = dot3(r, s.w);  // local coordinates
+    const float x0 = fabsf(l0) - s.half.x, x1 = fabsf(l1) - s.half.y, x2 = fabsf(l2) - s.half.z;  // beyond each face
+    ToolHit h;
+    if (x0 > 0.f || x1 > 0.f || x2 > 0.f) {
+        // outside: the offset from the box, assembled in the box frame from the local excesses
+        f3 off = s.u * copysignf(fmaxf(x0, 0.f), l0) + s.n * copysignf(fmaxf(x1, 0.f), l1) +
+                 s.w * copysignf(fmaxf(x2, 0.f), l2);
+        float dist = len3(off);
+        h.normal = dist > 0.f ? off * (1.f / dist) : s.n;
+        h.foot = s.center + s.u * fminf(s.half.x, fmaxf(-s.half.x, l0)) + s.n * fminf(s.half.y, fmaxf(-s.half.y, l1)) +
+                 s.w * fminf(s.half.z, fmaxf(-s.half.z, l2));
+        h.gap = dist;
+        return h;
+    }
+    // inside: out through the face of least penetration (the first such axis on ties)
+    float depth = x0, side = l0;
+    f3 ax = s.u;
+    if (x1 > depth) { depth = x1; side = l1; ax = s.n; }
+    if (x2 > depth) { depth = x2; side = l2; ax = s.w; }
+    h.normal = ax * (side >= 0.f ? 1.f : -1.f);
+    h.foot = x - h.normal * depth;
+    h.gap = depth;
+    return h;
+}
+
+__device__ __forceinline__ ToolHit tool_hit(const Shape& s, f3 x) { return s.capsule ? capsule_hit(s, x) : box_hit(s, x); }
+
+// The shape part way between two poses: points interpolated, the frame interpolated and made orthonormal again.
+__device__ __forceinline__ Shape pose_between(const Shape& from, const Shape& to, float t) {
+    Shape s = to;
+    const float k = 1.f - t;
+    s.center = from.center * k + to.center * t;
+    s.a = from.a * k + to.a * t;
+    s.b = from.b * k + to.b * t;
+    s.u = unit_or(from.u * k + to.u * t, to.u);
+    f3 n = from.n * k + to.n * t;
+    s.n = unit_or(n - s.u * dot3(s.u, n), to.n);
     s.w = cross3(s.u, s.n);
     return s;
 }
 
-__device__ __forceinline__ f3 rotate_shaft_vector(const Shape& from, const Shape& to, f3 r) {
-    f3 a = from.b - from.a, b = to.b - to.a;
-    if (len3(a) < 1e-8f || len3(b) < 1e-8f) return r;
-    a = unit_or(a, mk3(0, 0, 1));
-    b = unit_or(b, a);
-    float c = fminf(1.f, fmaxf(-1.f, dot3(a, b)));
-    f3 v = cross3(a, b);
+// The rotation taking the spine direction of `from` onto that of `to`, applied to r (Rodrigues' formula with the
+// sine folded into the cross product; a half turn about a perpendicular when the spines are opposite).
+__device__ __forceinline__ f3 swing(const Shape& from, const Shape& to, f3 r) {
+    f3 p = from.b - from.a, q = to.b - to.a;
+    if (len3(p) < 1e-8f || len3(q) < 1e-8f) return r;
+    p = unit_or(p, mk3(0, 0, 1));
+    q = unit_or(q, p);
+    const float c = fminf(1.f, fmaxf(-1.f, dot3(p, q)));
     if (c < -0.9999f) {
-        f3 u = unit_or(cross3(a, fabsf(a.x) < 0.9f ? mk3(1, 0, 0) : mk3(0, 1, 0)), mk3(0, 0, 1));
-        return u * (2.f * dot3(u, r)) - r;
+        f3 half_turn = unit_or(cross3(p, fabsf(p.x) < 0.9f ? mk3(1, 0, 0) : mk3(0, 1, 0)), mk3(0, 0, 1));
+        return half_turn * (2.f * dot3(half_turn, r)) - r;
     }
-    return r + cross3(v, r) + cross3(v, cross3(v, r)) * (1.f / (1.f + c));
+    const f3 v = cross3(p, q);
+    const f3 vr = cross3(v, r);
+    return r + vr + cross3(v, vr) * (1.f / (1.f + c));
 }
 
-__device__ __forceinline__ f3 witness_at(const Shape& from, const Shape& to, f3 p) {
+// Where the material point of the tool at p (on pose `from`) sits on pose `to`: along the spine by the same
+// parameter, swung with it; in a box by the same local coordinates.
+__device__ __forceinline__ f3 carry(const Shape& from, const Shape& to, f3 p) {
     if (from.capsule) {
-        f3 ab = from.b - from.a;
-        float t = fminf(1.f, fmaxf(0.f, dot3(p - from.a, ab) / fmaxf(dot3(ab, ab), 1e-20f)));
-        return to.a + (to.b - to.a) * t + rotate_shaft_vector(from, to, p - from.a - ab * t);
+        const float t = spine_param(from, p);
+        const f3 spine = from.b - from.a;
+        return to.a + (to.b - to.a) * t + swing(from, to, p - from.a - spine * t);
     }
-    f3 r = p - from.center;
+    const f3 r = p - from.center;
     return to.center + to.u * dot3(r, from.u) + to.n * dot3(r, from.n) + to.w * dot3(r, from.w);
 }
 
-__device__ __forceinline__ bool point_box_time(const Shape& box, float radius, f3 a, f3 b, float& time) {
-    float p[3], v[3], half[3], cuts[8];
-    cuts[0] = 0.f;
-    cuts[1] = 1.f;
-    int count = 2;
-    for (int k = 0; k < 3; k++) {
-        f3 ax = k == 0 ? box.u : (k == 1 ? box.n : box.w);
-        p[k] = dot3(a, ax);
-        v[k] = dot3(b - a, ax);
-        half[k] = comp3(box.half, k);
-        if (fabsf(v[k]) > 1e-12f)
-            for (int side = -1; side <= 1; side += 2) {
-                float t = ((float)side * half[k] - p[k]) / v[k];
-                if (t > 0 && t < 1) cuts[count++] = t;
-            }
-    }
-    for (int i = 1; i < count; i++) {
-        float x = cuts[i];
-        int j = i;
-        while (j > 0 && cuts[j - 1] > x) { cuts[j] = cuts[j - 1]; --j; }
-        cuts[j] = x;
-    }
-    for (int i = 0; i < count - 1; i++) {
-        double A = 0, B = 0, C = -(double)radius * radius;
-        float lo = cuts[i], hi = cuts[i + 1], mid = 0.5f * (lo + hi);
-        for (int k = 0; k < 3; k++) {
-            float q = p[k] + v[k] * mid;
-            if (fabsf(q) <= half[k]) continue;
-            double d = p[k] - copysignf(half[k], q);
-            A += (double)v[k] * v[k];
-            B += 2 * d * v[k];
-            C += d * d;
+// First time in [0, 1] at which a point moving from a to b (both relative to the box centre) comes within `radius`
+// of a box that does not rotate. Along the path the squared distance to the box is a quadratic in time between the
+// times at which a local coordinate crosses a face plane; each piece is solved in its own time variable s = t - lo:
+// D(lo + s) - radius^2 = A s^2 + B s + C with C the value at lo. The square roots run in double as the reference's
+// solve does.
+__device__ __forceinline__ void order2(float& a, float& b) {
+    const float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = lo;
+    b = hi;
+}
+
+__device__ __forceinline__ bool box_touch_time(const Shape& box, float radius, f3 a, f3 b, float& time) {
+    const f3 d = b - a;
+    const float p[3] = {dot3(a, box.u), dot3(a, box.n), dot3(a, box.w)};
+    const float v[3] = {dot3(d, box.u), dot3(d, box.n), dot3(d, box.w)};
+    const float h[3] = {box.half.x, box.half.y, box.half.z};
+    // piece boundaries: 0, the face-plane crossings inside (0, 1), 1; an absent crossing is 2, past the end
+    float cut[8];
+    cut[0] = 0.f;
+    cut[7] = 1.f;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const float t = ((side ? h[k] : -h[k]) - p[k]) / v[k];
+            cut[1 + 2 * k + side] = fabsf(v[k]) > 1e-12f && t > 0.f && t < 1.f ? t : 2.f;
         }
-        double atLo = (A * lo + B) * lo + C;
-        if (atLo <= 1e-14) { time = lo; return true; }
+    // Batcher's odd-even merge network for eight values, in registers
+    order2(cut[0], cut[1]); order2(cut[2], cut[3]); order2(cut[4], cut[5]); order2(cut[6], cut[7]);
+    order2(cut[0], cut[2]); order2(cut[1], cut[3]); order2(cut[4], cut[6]); order2(cut[5], cut[7]);
+    order2(cut[1], cut[2]); order2(cut[5], cut[6]);
+    order2(cut[0], cut[4]); order2(cut[1], cut[5]); order2(cut[2], cut[6]); order2(cut[3], cut[7]);
+    order2(cut[2], cut[4]); order2(cut[3], cut[5]);
+    order2(cut[1], cut[2]); order2(cut[3], cut[4]); order2(cut[5], cut[6]);
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const float lo = cut[i], hi = cut[i + 1];
+        if (hi > 1.f) break;
+        const float mid = 0.5f * (lo + hi);
+        double A = 0.0, B = 0.0, C = -(double)radius * radius;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float at_mid = p[k] + v[k] * mid;
+            if (fabsf(at_mid) <= h[k]) continue;  // inside this slab over the whole piece
+            const double e = (double)(p[k] - copysignf(h[k], at_mid)) + (double)v[k] * lo;  // excess at lo
+            A += (double)v[k] * v[k];
+            B += 2.0 * e * v[k];
+            C += e * e;
+        }
+        if (C <= 1e-14) { time = lo; return true; }
         if (A <= 1e-24) continue;
-        double disc = B * B - 4 * A * C;
-        if (disc < 0) continue;
-        double root = (-B - sqrt(disc)) / (2 * A);
-        if (root >= lo - 1e-8 && root <= hi + 1e-8) { time = fminf(hi, fmaxf(lo, (float)root)); return true; }
+        const double disc = B * B - 4.0 * A * C;
+        if (disc < 0.0) continue;
+        const double s = (-B - sqrt(disc)) / (2.0 * A);
+        if (s >= -1e-8 && s <= (double)(hi - lo) + 1e-8) {
+            time = fminf(hi, fmaxf(lo, (float)((double)lo + s)));
+            return true;
+        }
     }
     return false;
 }
 
+// A direction at pose `at` as the tool carries it to pose `to`: swung with a capsule's spine, re-expressed in a box's
+// frame.
+__device__ __forceinline__ f3 carry_direction(const Shape& at, const Shape& to, f3 n) {
+    if (to.capsule) return swing(at, to, n);
+    return unit_or(to.u * dot3(n, at.u) + to.n * dot3(n, at.n) + to.w * dot3(n, at.w), n);
+}
+
+// The contact record of a touch at pose `hit` with surface point `foot`: the tool moves that point from its place on
+// `from` to its place on `to` over the substep.
+__device__ __forceinline__ ContactRecord record_at(const Shape& from, const Shape& hit, const Shape& to, f3 normal,
+                                                   f3 foot, float radius) {
+    ContactRecord c;
+    const f3 end = carry(hit, to, foot);
+    c.normal = normal;
+    c.target = end + normal * radius;
+    c.motion = end - carry(hit, from, foot);
+    c.valid = 1;
+    return c;
+}
+
 __device__ __forceinline__ ContactRecord trace_tool_contact(const Shape& from, const Shape& to, float radius, f3 x0,
                                                             f3 x1) {
-    ContactRecord out;
-    out.normal = out.target = out.motion = mk3(0, 0, 0);
-    out.valid = 0;
-    f3 normal = mk3(0, 0, 0), surface = mk3(0, 0, 0);
+    ContactRecord none;
+    none.normal = none.target = none.motion = mk3(0, 0, 0);
+    none.valid = 0;
+    // a box whose frame does not turn over the substep: the exact first-touch time
     if (!to.capsule && len3(to.u - from.u) + len3(to.n - from.n) < 1e-7f) {
-        float time = 0;
-        if (!point_box_time(to, radius, x0 - from.center, x1 - to.center, time)) return out;
-        Shape hit = shape_lerp(from, to, time);
-        shape_distance(hit, x0 * (1.f - time) + x1 * time, normal, surface);
-        f3 end = witness_at(hit, to, surface), start = witness_at(hit, from, surface);
-        out.normal = normal; out.target = end + normal * radius; out.motion = end - start; out.valid = 1;
-        return out;
+        float time = 0.f;
+        if (!box_touch_time(to, radius, x0 - from.center, x1 - to.center, time)) return none;
+        const Shape hit = pose_between(from, to, time);
+        const ToolHit q = tool_hit(hit, x0 * (1.f - time) + x1 * time);
+        return record_at(from, hit, to, q.normal, q.foot, radius);
     }
+    // otherwise conservative advancement: the gap closes at most at `speed` per unit time, so stepping by a fraction
+    // of gap / speed cannot step over a touch
     float speed = len3(x1 - x0);
     if (to.capsule) speed += fmaxf(len3(to.a - from.a), len3(to.b - from.b));
     else speed += len3(to.center - from.center) + 4.f * len3(to.half) * (len3(to.u - from.u) + len3(to.n - from.n));
-    f3 center0 = to.capsule ? (from.a + from.b) * 0.5f : from.center;
-    f3 center1 = to.capsule ? (to.a + to.b) * 0.5f : to.center;
-    float bound = to.capsule ? fmaxf(len3(from.b - from.a), len3(to.b - to.a)) * 0.5f + to.radius : len3(to.half);
-    if (len3(x0 - center0) > bound + radius + len3(x1 - x0) + len3(center1 - center0) + 1e-5f) return out;
-    float tol = fmaxf(1e-6f, 1e-4f * (bound + radius));
-    double t = 0;
-    Shape hit = from;
-    bool found = false;
-    while (t < 1.) {
-        hit = shape_lerp(from, to, (float)t);
-        float d = shape_distance(hit, x0 * (1.f - (float)t) + x1 * (float)t, normal, surface) - radius;
-        if (d <= tol) { found = true; break; }
-        if (speed <= 1e-12f) break;
-        float step = 0.8f * d / speed;
+    // a swept bounding sphere of the tool rejects far nodes at once
+    const f3 mid0 = to.capsule ? (from.a + from.b) * 0.5f : from.center;
+    const f3 mid1 = to.capsule ? (to.a + to.b) * 0.5f : to.center;
+    const float reach = to.capsule ? fmaxf(len3(from.b - from.a), len3(to.b - to.a)) * 0.5f + to.radius : len3(to.half);
+    if (len3(x0 - mid0) > reach + radius + len3(x1 - x0) + len3(mid1 - mid0) + 1e-5f) return none;
+    const float tol = fmaxf(1e-6f, 1e-4f * (reach + radius));
+    double t = 0.0;  // time accumulates in double: grazing paths take many small steps
+    while (t < 1.0) {
+        const Shape at = pose_between(from, to, (float)t);
+        const ToolHit q = tool_hit(at, x0 * (1.f - (float)t) + x1 * (float)t);
+        const float gap = q.gap - radius;
+        if (gap <= tol) return record_at(from, at, to, carry_direction(at, to, q.normal), q.foot, radius);
+        if (speed <= 1e-12f) return none;
+        const float step = 0.8f * gap / speed;
         if ((float)t + step >= 1.f) {
-            t = 1;
-            hit = to;
-            float de = shape_distance(to, x1, normal, surface) - radius;
-            found = de <= tol;
-            break;
+            const ToolHit e = tool_hit(to, x1);
+            if (e.gap - radius > tol) return none;
+            return record_at(from, to, to, carry_direction(to, to, e.normal), e.foot, radius);
         }
         t += step;
     }
-    if (!found) return out;
-    f3 end = witness_at(hit, to, surface), start = witness_at(hit, from, surface);
-    if (to.capsule) normal = rotate_shaft_vector(hit, to, normal);
-    else normal = unit_or(to.u * dot3(normal, hit.u) + to.n * dot3(normal, hit.n) + to.w * dot3(normal, hit.w), normal);
-    out.normal = normal; out.target = end + normal * radius; out.motion = end - start; out.valid = 1;
-    return out;
+    return none;
 }

@@ -757,6 +757,13 @@ __device__ __forceinline__ void shape_bounds(const Shape& s, float lo[3], float 
     hi[0] = h.x; hi[1] = h.y; hi[2] = h.z;
 }
 
+// The signed gap of x to a tool shape and the shape's outward normal there.
+__device__ __forceinline__ float gap_at(const Shape& sh, f3 x, f3& n) {
+    const ToolHit q = tool_hit(sh, x);
+    n = q.normal;
+    return q.gap;
+}
+
 // corner k of a triangle by selects (a private array indexed at run time would live in scratch)
 __device__ __forceinline__ f3 corner(int k, f3 a, f3 b, f3 c) { return k == 0 ? a : (k == 1 ? b : c); }
 
@@ -766,32 +773,32 @@ __device__ __forceinline__ f3 corner_bar(int k) { return mk3((float)(k == 0), (f
 // the centroid and the corners starts it; twelve Frank-Wolfe steps move towards the corner the shape's normal favours,
 // each with eight golden-section reductions that reuse the surviving sample, keeping the best iterate.
 __device__ __forceinline__ f3 triangle_witness(const Shape& sh, f3 a, f3 b, f3 c) {
-    f3 bar = mk3(1.f / 3, 1.f / 3, 1.f / 3), best = bar, n, s;
-    float bestD = shape_distance(sh, (a + b + c) * (1.f / 3), n, s);
+    f3 bar = mk3(1.f / 3, 1.f / 3, 1.f / 3), best = bar, n;
+    float bestD = gap_at(sh, (a + b + c) * (1.f / 3), n);
     for (int k = 0; k < 3; k++) {
-        const float d = shape_distance(sh, corner(k, a, b, c), n, s);
+        const float d = gap_at(sh, corner(k, a, b, c), n);
         if (d < bestD) { bestD = d; best = corner_bar(k); }
     }
     for (int it = 0; it < 12; it++) {
         const f3 p = a * bar.x + b * bar.y + c * bar.z;
-        shape_distance(sh, p, n, s);
+        gap_at(sh, p, n);
         int k = dot3(n, b) < dot3(n, a) ? 1 : 0;
         if (dot3(n, c) < dot3(n, corner(k, a, b, c))) k = 2;
         const f3 target = corner_bar(k), dir = corner(k, a, b, c) - p;
         float lo = 0.f, hi = 1.f, l = .381966f, r = .618034f;
-        float dl = shape_distance(sh, p + dir * l, n, s), dr = shape_distance(sh, p + dir * r, n, s);
+        float dl = gap_at(sh, p + dir * l, n), dr = gap_at(sh, p + dir * r, n);
         for (int j = 0; j < 8; j++) {
             if (dl < dr) {
                 hi = r; r = l; dr = dl; l = lo + (hi - lo) * .381966f;
-                if (j < 7) dl = shape_distance(sh, p + dir * l, n, s);
+                if (j < 7) dl = gap_at(sh, p + dir * l, n);
             } else {
                 lo = l; l = r; dl = dr; r = lo + (hi - lo) * .618034f;
-                if (j < 7) dr = shape_distance(sh, p + dir * r, n, s);
+                if (j < 7) dr = gap_at(sh, p + dir * r, n);
             }
         }
         const float t = .5f * (lo + hi);
         bar = bar * (1.f - t) + target * t;
-        const float d = shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s);
+        const float d = gap_at(sh, a * bar.x + b * bar.y + c * bar.z, n);
         if (d < bestD) { bestD = d; best = bar; }
     }
     return best;
@@ -831,8 +838,8 @@ __global__ void __launch_bounds__(GTS_BLOCK) k_surface_depth(int capacity, const
             outside |= fmaxf(va[ax], fmaxf(vb[ax], vc[ax])) < lo[ax] || fminf(va[ax], fminf(vb[ax], vc[ax])) > hi[ax];
         if (outside) continue;
         const f3 bar = triangle_witness(sh, a, b, c);
-        f3 n, s;
-        const float depth = fmaxf(0.f, -shape_distance(sh, a * bar.x + b * bar.y + c * bar.z, n, s));
+        f3 n;
+        const float depth = fmaxf(0.f, -gap_at(sh, a * bar.x + b * bar.y + c * bar.z, n));
         if (depth > 0.f) atomicMax(reinterpret_cast<unsigned int*>(out), __float_as_uint(depth));
     }
 }

@@ -213,35 +213,65 @@ template <class R> Shape<R> contactShape(const ToolR<R>& t, int kind, R thicknes
     return s;
 }
 
-// shapeDistance (friction_contact.cuh:40-63).
-template <class R> R shapeDistance(const Shape<R>& s, V3<R> x, V3<R>& normal, V3<R>& surface) {
-    if (s.capsule) {
-        V3<R> ab = s.b - s.a;
-        R tt = fmin_<R>((R)1, fmax_<R>((R)0, dot3(x - s.a, ab) / fmax_<R>(dot3(ab, ab), (R)1e-20f)));
-        V3<R> d = x - (s.a + ab * tt);
-        R l = len3(d);
-        V3<R> fallback = unitOr(cross3(ab, std::fabs(ab.x) < std::fabs(ab.y) ? mk<R>(1, 0, 0) : mk<R>(0, 1, 0)),
-                                mk<R>(0, 0, 1));
-        normal = l > (R)1e-6f * fmax_<R>((R)1, len3(ab)) ? d * ((R)1 / l) : fallback;
-        surface = s.a + ab * tt + normal * s.radius;
-        return l - s.radius;
-    }
+// The swept tool contact (contact/friction_contact.cuh:40-142) restated in the kernels' formulation: a surface query
+// record (gap, outward normal, nearest surface point), the box's local frame, pose interpolation, the carry of tool
+// material points between poses, the exact first-touch time of a translating box solved piece by piece in a
+// piece-local time, and conservative advancement otherwise.
+template <class R> struct Hit { R gap; V3<R> normal, foot; };
+
+template <class R> V3<R> acrossSpine(V3<R> sp) {  // a unit vector across the spine, on its least aligned axis
+    V3<R> ax = std::fabs(sp.x) <= std::fabs(sp.y) && std::fabs(sp.x) <= std::fabs(sp.z)
+                   ? mk<R>(1, 0, 0)
+                   : (std::fabs(sp.y) <= std::fabs(sp.z) ? mk<R>(0, 1, 0) : mk<R>(0, 0, 1));
+    return unitOr(cross3(sp, ax), mk<R>(0, 0, 1));
+}
+template <class R> R spineParam(const Shape<R>& s, V3<R> x) {
+    V3<R> sp = s.b - s.a;
+    return fmin_<R>((R)1, fmax_<R>((R)0, dot3(x - s.a, sp) / fmax_<R>(dot3(sp, sp), (R)1e-20f)));
+}
+template <class R> Hit<R> capsuleHit(const Shape<R>& s, V3<R> x) {
+    V3<R> sp = s.b - s.a;
+    V3<R> base = s.a + sp * spineParam(s, x);
+    V3<R> off = x - base;
+    R dist = len3(off);
+    Hit<R> h;
+    h.normal = dist > (R)1e-6f * fmax_<R>((R)1, len3(sp)) ? off * ((R)1 / dist) : acrossSpine(sp);
+    h.foot = base + h.normal * s.radius;
+    h.gap = dist - s.radius;
+    return h;
+}
+template <class R> Hit<R> boxHit(const Shape<R>& s, V3<R> x) {
     V3<R> r = x - s.center;
-    R v[3] = {dot3(r, s.u), dot3(r, s.n), dot3(r, s.w)};
-    R h[3] = {s.half.x, s.half.y, s.half.z};
-    R q[3], cl[3];
-    for (int k = 0; k < 3; k++) { q[k] = std::fabs(v[k]) - h[k]; cl[k] = fmin_<R>(h[k], fmax_<R>(-h[k], v[k])); }
-    V3<R> closest = s.center + s.u * cl[0] + s.n * cl[1] + s.w * cl[2];
-    V3<R> d = s.u * std::copysign(fmax_<R>(q[0], (R)0), v[0]) + s.n * std::copysign(fmax_<R>(q[1], (R)0), v[1]) +
-              s.w * std::copysign(fmax_<R>(q[2], (R)0), v[2]);
-    R len = len3(d);
-    if (q[0] > 0 || q[1] > 0 || q[2] > 0) { normal = len > 0 ? d * ((R)1 / len) : s.n; surface = closest; return len; }
-    int k = q[1] > q[0] ? 1 : 0;
-    if (q[2] > q[k]) k = 2;
-    V3<R> axes[3] = {s.u, s.n, s.w};
-    normal = axes[k] * (v[k] >= 0 ? (R)1 : (R)-1);
-    surface = x - normal * q[k];
-    return q[k];
+    R l0 = dot3(r, s.u), l1 = dot3(r, s.n), l2 = dot3(r, s.w);
+    R x0 = std::fabs(l0) - s.half.x, x1 = std::fabs(l1) - s.half.y, x2 = std::fabs(l2) - s.half.z;
+    Hit<R> h;
+    if (x0 > 0 || x1 > 0 || x2 > 0) {
+        V3<R> off = s.u * std::copysign(fmax_<R>(x0, (R)0), l0) + s.n * std::copysign(fmax_<R>(x1, (R)0), l1) +
+                    s.w * std::copysign(fmax_<R>(x2, (R)0), l2);
+        R dist = len3(off);
+        h.normal = dist > 0 ? off * ((R)1 / dist) : s.n;
+        h.foot = s.center + s.u * fmin_<R>(s.half.x, fmax_<R>(-s.half.x, l0)) +
+                 s.n * fmin_<R>(s.half.y, fmax_<R>(-s.half.y, l1)) + s.w * fmin_<R>(s.half.z, fmax_<R>(-s.half.z, l2));
+        h.gap = dist;
+        return h;
+    }
+    R depth = x0, side = l0;
+    V3<R> ax = s.u;
+    if (x1 > depth) { depth = x1; side = l1; ax = s.n; }
+    if (x2 > depth) { depth = x2; side = l2; ax = s.w; }
+    h.normal = ax * (side >= 0 ? (R)1 : (R)-1);
+    h.foot = x - h.normal * depth;
+    h.gap = depth;
+    return h;
+}
+template <class R> Hit<R> toolHit(const Shape<R>& s, V3<R> x) { return s.capsule ? capsuleHit(s, x) : boxHit(s, x); }
+
+// The signed gap of x to a shape, its outward normal and nearest surface point (the surface-depth witness's query).
+template <class R> R shapeDistance(const Shape<R>& s, V3<R> x, V3<R>& normal, V3<R>& surface) {
+    Hit<R> h = toolHit(s, x);
+    normal = h.normal;
+    surface = h.foot;
+    return h.gap;
 }
 
 // shapeBounds (surface_contact.cuh:229-235): the axis-aligned box of a shape, widened by radius.
@@ -292,129 +322,153 @@ inline V3<float> triangleWitnessF(const Shape<float>& sh, V3<float> a, V3<float>
     return best;
 }
 
-template <class R> Shape<R> shapeLerp(const Shape<R>& a, const Shape<R>& b, R t) {  // friction_contact.cuh:64-68
-    Shape<R> s = b;
-    s.center = a.center * ((R)1 - t) + b.center * t;
-    s.a = a.a * ((R)1 - t) + b.a * t;
-    s.b = a.b * ((R)1 - t) + b.b * t;
-    s.u = unitOr(a.u * ((R)1 - t) + b.u * t, b.u);
-    s.n = a.n * ((R)1 - t) + b.n * t;
-    s.n = unitOr(s.n - s.u * dot3(s.u, s.n), b.n);
+template <class R> Shape<R> poseBetween(const Shape<R>& from, const Shape<R>& to, R t) {
+    Shape<R> s = to;
+    const R k = (R)1 - t;
+    s.center = from.center * k + to.center * t;
+    s.a = from.a * k + to.a * t;
+    s.b = from.b * k + to.b * t;
+    s.u = unitOr(from.u * k + to.u * t, to.u);
+    V3<R> n = from.n * k + to.n * t;
+    s.n = unitOr(n - s.u * dot3(s.u, n), to.n);
     s.w = cross3(s.u, s.n);
     return s;
 }
-
-template <class R> V3<R> rotateShaftVector(const Shape<R>& from, const Shape<R>& to, V3<R> r) {  // :69-76
-    V3<R> a = from.b - from.a, b = to.b - to.a;
-    if (len3(a) < (R)1e-8f || len3(b) < (R)1e-8f) return r;
-    a = unitOr(a, mk<R>(0, 0, 1));
-    b = unitOr(b, a);
-    R c = fmin_<R>((R)1, fmax_<R>((R)-1, dot3(a, b)));
-    V3<R> v = cross3(a, b);
+// The rotation taking from's spine direction onto to's, applied to r (a half turn when they are opposite).
+template <class R> V3<R> swing(const Shape<R>& from, const Shape<R>& to, V3<R> r) {
+    V3<R> p = from.b - from.a, q = to.b - to.a;
+    if (len3(p) < (R)1e-8f || len3(q) < (R)1e-8f) return r;
+    p = unitOr(p, mk<R>(0, 0, 1));
+    q = unitOr(q, p);
+    R c = fmin_<R>((R)1, fmax_<R>((R)-1, dot3(p, q)));
     if (c < (R)-0.9999f) {
-        V3<R> u = unitOr(cross3(a, std::fabs(a.x) < (R)0.9f ? mk<R>(1, 0, 0) : mk<R>(0, 1, 0)), mk<R>(0, 0, 1));
-        return u * ((R)2 * dot3(u, r)) - r;
+        V3<R> ht = unitOr(cross3(p, std::fabs(p.x) < (R)0.9f ? mk<R>(1, 0, 0) : mk<R>(0, 1, 0)), mk<R>(0, 0, 1));
+        return ht * ((R)2 * dot3(ht, r)) - r;
     }
-    return r + cross3(v, r) + cross3(v, cross3(v, r)) * ((R)1 / ((R)1 + c));
+    V3<R> v = cross3(p, q);
+    V3<R> vr = cross3(v, r);
+    return r + vr + cross3(v, vr) * ((R)1 / ((R)1 + c));
 }
-
-template <class R> V3<R> witnessAt(const Shape<R>& from, const Shape<R>& to, V3<R> p) {  // :77-84
+// Where the tool's material point at p on pose `from` sits on pose `to`.
+template <class R> V3<R> carry(const Shape<R>& from, const Shape<R>& to, V3<R> p) {
     if (from.capsule) {
-        V3<R> ab = from.b - from.a;
-        R t = fmin_<R>((R)1, fmax_<R>((R)0, dot3(p - from.a, ab) / fmax_<R>(dot3(ab, ab), (R)1e-20f)));
-        return to.a + (to.b - to.a) * t + rotateShaftVector(from, to, p - from.a - ab * t);
+        R t = spineParam(from, p);
+        V3<R> sp = from.b - from.a;
+        return to.a + (to.b - to.a) * t + swing(from, to, p - from.a - sp * t);
     }
     V3<R> r = p - from.center;
     return to.center + to.u * dot3(r, from.u) + to.n * dot3(r, from.n) + to.w * dot3(r, from.w);
 }
+template <class R> V3<R> carryDirection(const Shape<R>& at, const Shape<R>& to, V3<R> n) {
+    if (to.capsule) return swing(at, to, n);
+    return unitOr(to.u * dot3(n, at.u) + to.n * dot3(n, at.n) + to.w * dot3(n, at.w), n);
+}
 
-// pointBoxTime (friction_contact.cuh:86-106): first time a point moving a->b comes within radius of the box.
-template <class R> bool pointBoxTime(const Shape<R>& box, R radius, V3<R> a, V3<R> b, R& time) {
-    V3<R> axes[3] = {box.u, box.n, box.w};
-    R p[3], v[3], half[3], cuts[8] = {0, 1};
-    int count = 2;
-    for (int k = 0; k < 3; k++) {
-        p[k] = dot3(a, axes[k]);
-        v[k] = dot3(b - a, axes[k]);
-        half[k] = comp(box.half, k);
-        if (std::fabs(v[k]) > (R)1e-12f)
-            for (int side = -1; side <= 1; side += 2) {
-                R t = ((R)side * half[k] - p[k]) / v[k];
-                if (t > 0 && t < 1) cuts[count++] = t;
-            }
-    }
-    for (int i = 1; i < count; i++) {
-        R x = cuts[i];
-        int j = i;
-        while (j > 0 && cuts[j - 1] > x) { cuts[j] = cuts[j - 1]; --j; }
-        cuts[j] = x;
-    }
-    for (int i = 0; i < count - 1; i++) {
-        double A = 0, B = 0, C = -(double)radius * radius;
-        R lo = cuts[i], hi = cuts[i + 1], mid = (R)0.5f * (lo + hi);
-        for (int k = 0; k < 3; k++) {
-            R q = p[k] + v[k] * mid;
-            if (std::fabs(q) <= half[k]) continue;
-            double d = p[k] - std::copysign(half[k], q);
-            A += (double)v[k] * v[k];
-            B += 2 * d * v[k];
-            C += d * d;
+// First time in [0, 1] at which a point moving a -> b (relative to the box centre) comes within `radius` of a box
+// that does not rotate: pieces between the times a local coordinate crosses a face plane, each a quadratic in the
+// piece-local time s = t - lo with its value at lo as constant term, solved in double.
+template <class R> bool boxTouchTime(const Shape<R>& box, R radius, V3<R> a, V3<R> b, R& time) {
+    V3<R> d = b - a;
+    R p[3] = {dot3(a, box.u), dot3(a, box.n), dot3(a, box.w)};
+    R v[3] = {dot3(d, box.u), dot3(d, box.n), dot3(d, box.w)};
+    R h[3] = {box.half.x, box.half.y, box.half.z};
+    std::vector<R> cut = {(R)0, (R)1};  // 0, the crossings inside (0, 1), 1
+    for (int k = 0; k < 3; k++)
+        for (int side = 0; side < 2; side++) {
+            R t = ((side ? h[k] : -h[k]) - p[k]) / v[k];
+            if (std::fabs(v[k]) > (R)1e-12f && t > 0 && t < 1) cut.push_back(t);
         }
-        double atLo = (A * lo + B) * lo + C;
-        if (atLo <= 1e-14) { time = lo; return true; }
+    std::sort(cut.begin(), cut.end());
+    for (size_t i = 0; i + 1 < cut.size(); i++) {
+        R lo = cut[i], hi = cut[i + 1], mid = (R)0.5f * (lo + hi);
+        double A = 0, B = 0, C = -(double)radius * radius;
+        for (int k = 0; k < 3; k++) {
+            R atMid = p[k] + v[k] * mid;
+            if (std::fabs(atMid) <= h[k]) continue;
+            double e = (double)(p[k] - std::copysign(h[k], atMid)) + (double)v[k] * lo;
+            A += (double)v[k] * v[k];
+            B += 2.0 * e * v[k];
+            C += e * e;
+        }
+        if (C <= 1e-14) { time = lo; return true; }
         if (A <= 1e-24) continue;
-        double disc = B * B - 4 * A * C;
+        double disc = B * B - 4.0 * A * C;
         if (disc < 0) continue;
-        double root = (-B - std::sqrt(disc)) / (2 * A);
-        if (root >= lo - 1e-8 && root <= hi + 1e-8) { time = fmin_<R>(hi, fmax_<R>(lo, (R)root)); return true; }
+        double sroot = (-B - std::sqrt(disc)) / (2.0 * A);
+        if (sroot >= -1e-8 && sroot <= (double)(hi - lo) + 1e-8) {
+            time = fmin_<R>(hi, fmax_<R>(lo, (R)((double)lo + sroot)));
+            return true;
+        }
     }
     return false;
 }
 
-// traceToolContact (friction_contact.cuh:107-142): swept conservative advancement of a node against a moving shape.
+template <class R> Record<R> recordAt(const Shape<R>& from, const Shape<R>& hit, const Shape<R>& to, V3<R> normal,
+                                      V3<R> foot, R radius) {
+    Record<R> c;
+    V3<R> end = carry(hit, to, foot);
+    c.normal = normal;
+    c.target = end + normal * radius;
+    c.motion = end - carry(hit, from, foot);
+    c.valid = 1;
+    return c;
+}
+
+// The node's swept contact with a moving shape: the exact first touch for a translating box, conservative
+// advancement (steps of 0.8 gap / speed, time in double) otherwise.
 template <class R> Record<R> traceToolContact(const Shape<R>& from, const Shape<R>& to, R radius, V3<R> x0, V3<R> x1) {
-    Record<R> out{};
-    V3<R> normal{}, surface{};
+    Record<R> none{};
     if (!to.capsule && len3(to.u - from.u) + len3(to.n - from.n) < (R)1e-7f) {
         R time = 0;
-        if (!pointBoxTime(to, radius, x0 - from.center, x1 - to.center, time)) return out;
-        Shape<R> hit = shapeLerp(from, to, time);
-        shapeDistance(hit, x0 * ((R)1 - time) + x1 * time, normal, surface);
-        V3<R> end = witnessAt(hit, to, surface), start = witnessAt(hit, from, surface);
-        out.normal = normal; out.target = end + normal * radius; out.motion = end - start; out.valid = 1;
-        return out;
+        if (!boxTouchTime(to, radius, x0 - from.center, x1 - to.center, time)) return none;
+        Shape<R> hit = poseBetween(from, to, time);
+        Hit<R> q = toolHit(hit, x0 * ((R)1 - time) + x1 * time);
+        return recordAt(from, hit, to, q.normal, q.foot, radius);
     }
     R speed = len3(x1 - x0);
     if (to.capsule) speed += fmax_<R>(len3(to.a - from.a), len3(to.b - from.b));
     else speed += len3(to.center - from.center) + (R)4.f * len3(to.half) * (len3(to.u - from.u) + len3(to.n - from.n));
-    V3<R> center0 = to.capsule ? (from.a + from.b) * (R)0.5f : from.center;
-    V3<R> center1 = to.capsule ? (to.a + to.b) * (R)0.5f : to.center;
-    R bound = to.capsule ? fmax_<R>(len3(from.b - from.a), len3(to.b - to.a)) * (R)0.5f + to.radius : len3(to.half);
-    if (len3(x0 - center0) > bound + radius + len3(x1 - x0) + len3(center1 - center0) + (R)1e-5f) return out;
-    R tol = fmax_<R>((R)1e-6f, (R)1e-4f * (bound + radius));
+    V3<R> mid0 = to.capsule ? (from.a + from.b) * (R)0.5f : from.center;
+    V3<R> mid1 = to.capsule ? (to.a + to.b) * (R)0.5f : to.center;
+    R reach = to.capsule ? fmax_<R>(len3(from.b - from.a), len3(to.b - to.a)) * (R)0.5f + to.radius : len3(to.half);
+    if (len3(x0 - mid0) > reach + radius + len3(x1 - x0) + len3(mid1 - mid0) + (R)1e-5f) return none;
+    R tol = fmax_<R>((R)1e-6f, (R)1e-4f * (reach + radius));
     double t = 0;
-    Shape<R> hit = from;
-    bool found = false;
-    while (t < 1.) {
-        hit = shapeLerp(from, to, (R)t);
-        R d = shapeDistance(hit, x0 * ((R)1 - (R)t) + x1 * (R)t, normal, surface) - radius;
-        if (d <= tol) { found = true; break; }
-        if (speed <= (R)1e-12f) break;
-        R step = (R)0.8f * d / speed;
+    while (t < 1.0) {
+        Shape<R> at = poseBetween(from, to, (R)t);
+        Hit<R> q = toolHit(at, x0 * ((R)1 - (R)t) + x1 * (R)t);
+        R gap = q.gap - radius;
+        if (gap <= tol) return recordAt(from, at, to, carryDirection(at, to, q.normal), q.foot, radius);
+        if (speed <= (R)1e-12f) return none;
+        R step = (R)0.8f * gap / speed;
         if ((R)t + step >= (R)1) {
-            t = 1; hit = to;
-            R de = shapeDistance(to, x1, normal, surface) - radius;
-            found = de <= tol;
-            break;
+            Hit<R> e = toolHit(to, x1);
+            if (e.gap - radius > tol) return none;
+            return recordAt(from, to, to, carryDirection(to, to, e.normal), e.foot, radius);
         }
         t += step;
     }
-    if (!found) return out;
-    V3<R> end = witnessAt(hit, to, surface), start = witnessAt(hit, from, surface);
-    if (to.capsule) normal = rotateShaftVector(hit, to, normal);
-    else normal = unitOr(to.u * dot3(normal, hit.u) + to.n * dot3(normal, hit.n) + to.w * dot3(normal, hit.w), normal);
-    out.normal = normal; out.target = end + normal * radius; out.motion = end - start; out.valid = 1;
-    return out;
+    return none;
+}
+
+// One pass of k_SolveFrictionContacts (friction_contact.cuh:150-168) for one contact of a node: the unilateral normal
+// multiplier (compliance alpha, clamped at 0), then the tangential multiplier against the slip relative to the tool's
+// motion, kept whole below mu_s lambda_N (stick) and scaled to mu_d lambda_N above it (slide).
+template <class R>
+void frictionUpdate(const Record<R>& c, R w, R alpha, R muS, R muD, V3<R> prev, V3<R>& x, R& lamN, V3<R>& lamT) {
+    R C = dot3(x - c.target, c.normal), old = lamN;
+    R next = fmax_<R>((R)0, old + (-C - alpha * old) / (w + alpha));
+    x = x + c.normal * (w * (next - old));
+    lamN = next;
+    V3<R> slip = x - prev - c.motion;
+    slip = slip - c.normal * dot3(slip, c.normal);
+    V3<R> oldT = lamT;
+    V3<R> trial = oldT - slip * ((R)1 / w);
+    R len = len3(trial);
+    V3<R> nextT = trial;
+    if (len > muS * next) nextT = len > (R)1e-15f ? trial * (muD * next / len) : mk<R>(0, 0, 0);
+    x = x + (nextT - oldT) * w;
+    lamT = nextT;
 }
 
 // Möller–Trumbore of the deformed edge against one swept triangle (cut.cuh:162-185).
@@ -1297,23 +1351,9 @@ template <class R> int substep(Tissue<R>& s) {
             }
             R w = s.invMass[i];
             for (int pass = 0; pass < 2; pass++)
-                for (int kind = 0; kind < kinds; kind++) {
-                    const Record<R>& c = rec[kind];
-                    if (!c.valid) continue;
-                    R C = dot3(s.pos[i] - c.target, c.normal), old = lamN[kind];
-                    R next = fmax_<R>((R)0, old + (-C - alpha * old) / (w + alpha));
-                    s.pos[i] = s.pos[i] + c.normal * (w * (next - old));
-                    lamN[kind] = next;
-                    V3<R> slip = s.pos[i] - s.prev[i] - c.motion;
-                    slip = slip - c.normal * dot3(slip, c.normal);
-                    V3<R> oldT = lamT[kind];
-                    V3<R> trial = oldT - slip * ((R)1 / w);
-                    R len = len3(trial);
-                    V3<R> nextT = trial;
-                    if (len > muS * next) nextT = len > (R)1e-15f ? trial * (muD * next / len) : mk<R>(0, 0, 0);
-                    s.pos[i] = s.pos[i] + (nextT - oldT) * w;
-                    lamT[kind] = nextT;
-                }
+                for (int kind = 0; kind < kinds; kind++)
+                    if (rec[kind].valid)
+                        frictionUpdate(rec[kind], w, alpha, muS, muD, s.prev[i], s.pos[i], lamN[kind], lamT[kind]);
             for (int kind = 0; kind < kinds; kind++)
                 if (rec[kind].valid) {  // k_NodeReaction: force of the tissue on the tool
                     V3<R> f = (rec[kind].normal * lamN[kind] + lamT[kind]) * (-scaleR);
@@ -2539,4 +2579,43 @@ extern "C" float oracle_surface_depth(const OracleTool* tool, int kinds, float t
         }
     }
     return result;
+}
+
+// ---- contact pieces in FP64 for the CPU pins: a shape is 23 doubles (center, u, n, w, half, a, b, radius, capsule)
+namespace {
+Shape<double> shapeOf(const double* v) {
+    Shape<double> s{};
+    s.center = mk(v[0], v[1], v[2]); s.u = mk(v[3], v[4], v[5]); s.n = mk(v[6], v[7], v[8]); s.w = mk(v[9], v[10], v[11]);
+    s.half = mk(v[12], v[13], v[14]); s.a = mk(v[15], v[16], v[17]); s.b = mk(v[18], v[19], v[20]);
+    s.radius = v[21]; s.capsule = v[22] != 0;
+    return s;
+}
+}  // namespace
+extern "C" int oracle_box_touch_time_f64(const double* box23, double radius, const double* a3, const double* b3,
+                                         double* time) {
+    return boxTouchTime<double>(shapeOf(box23), radius, mk(a3[0], a3[1], a3[2]), mk(b3[0], b3[1], b3[2]), *time) ? 1
+                                                                                                                    : 0;
+}
+// out: normal (3), target (3), motion (3); returns valid
+extern "C" int oracle_trace_contact_f64(const double* from23, const double* to23, double radius, const double* x0,
+                                        const double* x1, double* out9) {
+    Record<double> r = traceToolContact<double>(shapeOf(from23), shapeOf(to23), radius, mk(x0[0], x0[1], x0[2]),
+                                                mk(x1[0], x1[1], x1[2]));
+    V3<double> o[3] = {r.normal, r.target, r.motion};
+    for (int k = 0; k < 3; k++) { out9[3 * k] = o[k].x; out9[3 * k + 1] = o[k].y; out9[3 * k + 2] = o[k].z; }
+    return r.valid;
+}
+// one friction pass on x (3) with the contact record rec9 (normal, target, motion); lam4 = lambda_N then lambda_T (3)
+extern "C" void oracle_friction_update_f64(const double* rec9, double w, double alpha, double muS, double muD,
+                                           const double* prev3, double* x3, double* lam4) {
+    Record<double> c;
+    c.normal = mk(rec9[0], rec9[1], rec9[2]); c.target = mk(rec9[3], rec9[4], rec9[5]);
+    c.motion = mk(rec9[6], rec9[7], rec9[8]); c.valid = 1;
+    V3<double> x = mk(x3[0], x3[1], x3[2]), lt = mk(lam4[1], lam4[2], lam4[3]);
+    frictionUpdate<double>(c, w, alpha, muS, muD, mk(prev3[0], prev3[1], prev3[2]), x, lam4[0], lt);
+    x3[0] = x.x; x3[1] = x.y; x3[2] = x.z;
+    lam4[1] = lt.x; lam4[2] = lt.y; lam4[3] = lt.z;
+}
+extern "C" double oracle_tool_gap_f64(const double* shape23, const double* x3) {
+    return toolHit<double>(shapeOf(shape23), mk(x3[0], x3[1], x3[2])).gap;
 }
