@@ -735,7 +735,30 @@ template <class R> int beginFrame(Tissue<R>& s, R dt) {
     return 0;
 }
 
-// k_solve_edges (xpbd_solver.cu:484-524) on one edge.
+// k_solve_edges (xpbd_solver.cu:484-524) as a pure function of one edge: positions xi, xj (updated), their substep
+// starts pi, pj, inverse masses, rest length, compliance alphaTilde, Kelvin-Voigt factor gamma, relaxation omega and
+// the multiplier lam (updated). Returns false where the reference returns without writing.
+template <class R>
+bool edgeProject(V3<R>& xi, V3<R>& xj, const V3<R>& pi, const V3<R>& pj, R wi, R wj, R rest, R alphaTilde, R gamma,
+                 R omega, R& lam) {
+    R wsum = wi + wj;
+    if (wsum <= 0) return false;
+    V3<R> dd = xi - xj;
+    R L = len3(dd);
+    if (L < (R)1e-9f) return false;
+    V3<R> n = dd * ((R)1 / L);
+    R C = L - rest;
+    R gradDx = dot3(n, (xi - pi) - (xj - pj));
+    R dl = (-C - alphaTilde * lam - gamma * gradDx) / (((R)1 + gamma) * wsum + alphaTilde);
+    R applied = omega * dl;
+    lam = lam + applied;
+    xi = xi + n * (wi * applied);
+    xj = xj - n * (wj * applied);
+    return true;
+}
+
+// k_solve_edges on edge e of the tissue; under the fallback (no edge stiffness, :469-479) an edge whose owner cell
+// keeps a live tet is skipped and the others take alphaTilde = 1 / (E L h^2).
 template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallback, R fallbackInvEh2, bool useLambda) {
     if (s.eactive[e] == 0) return;
     if (fallback) {
@@ -743,24 +766,27 @@ template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallba
         alphaTilde = fallbackInvEh2 / s.erest[e];
     }
     int i = s.ei[e], j = s.ej[e];
-    R wi = s.invMass[i], wj = s.invMass[j], wsum = wi + wj;
-    if (wsum <= 0) return;
-    V3<R> dd = s.pos[i] - s.pos[j];
+    R lam = useLambda ? s.elambda[e] : (R)0;
+    if (edgeProject(s.pos[i], s.pos[j], s.prev[i], s.prev[j], s.invMass[i], s.invMass[j], s.erest[e], alphaTilde,
+                    s.betaGammaH, (R)s.d.omega, lam))
+        s.elambda[e] = lam;
+}
+
+// k_perp_damp_edges (xpbd_solver.cu:890-911) on one edge: the part of the endpoints' relative velocity across the
+// edge, times hBeta = h beta_s, is taken off i and given to j in proportion to their inverse masses.
+template <class R>
+void perpDamp(V3<R>& vi, V3<R>& vj, const V3<R>& xi, const V3<R>& xj, R wi, R wj, R hBeta) {
+    if (wi + wj <= 0) return;
+    V3<R> dd = xi - xj;
     R L = len3(dd);
     if (L < (R)1e-9f) return;
     V3<R> n = dd * ((R)1 / L);
-    R C = L - s.erest[e];
-    R gamma = s.betaGammaH;
-    R gradDx = dot3(n, (s.pos[i] - s.prev[i]) - (s.pos[j] - s.prev[j]));
-    R lam = useLambda ? s.elambda[e] : (R)0;
-    R dl = (-C - alphaTilde * lam - gamma * gradDx) / (((R)1 + gamma) * wsum + alphaTilde);
-    R applied = (R)s.d.omega * dl;
-    s.elambda[e] = lam + applied;
-    s.pos[i] = s.pos[i] + n * (wi * applied);
-    s.pos[j] = s.pos[j] - n * (wj * applied);
+    V3<R> vrel = vi - vj;
+    V3<R> vp = vrel - n * dot3(vrel, n);
+    V3<R> imp = vp * hBeta;
+    if (wi > 0) vi = vi - imp * wi;
+    if (wj > 0) vj = vj + imp * wj;
 }
-
-
 
 // ---- precise tet invariants (no reference counterpart; OracleDesc.precise_strain). The FP32 path restates the
 // kernels' sequence (gts_common.h: iso_minus_one, own_log1p, precise_strain, precise_dev) op for op; the FP64 oracle
@@ -1372,17 +1398,7 @@ template <class R> int substep(Tissue<R>& s) {
                 int e = s.ecolorOff[c] + k;
                 if (s.eactive[e] == 0) continue;
                 int i = s.ei[e], j = s.ej[e];
-                R wi = s.invMass[i], wj = s.invMass[j];
-                if (wi + wj <= 0) continue;
-                V3<R> dd = s.pos[i] - s.pos[j];
-                R L = len3(dd);
-                if (L < (R)1e-9f) continue;
-                V3<R> n = dd * ((R)1 / L);
-                V3<R> vrel = s.vel[i] - s.vel[j];
-                V3<R> vp = vrel - n * dot3(vrel, n);
-                V3<R> imp = vp * (h * (R)s.d.beta_s);
-                if (wi > 0) s.vel[i] = s.vel[i] - imp * wi;
-                if (wj > 0) s.vel[j] = s.vel[j] + imp * wj;
+                perpDamp(s.vel[i], s.vel[j], s.pos[i], s.pos[j], s.invMass[i], s.invMass[j], h * (R)s.d.beta_s);
             }
     for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? s.vel[p] * s.velScale : mk<R>(0, 0, 0);
     if (s.substepIndex >= Nsub) s.toolFrom = s.tool;
@@ -2047,6 +2063,21 @@ extern "C" int oracle_tet_project_f64(int mode, const double* p12, const double*
     bool ok = tetProject<double>(mode, p, pv, w, br, ad, ah, lam2[0], lam2[1], dampingOverH, bulkOverH, omega, out);
     for (int k = 0; k < 4; k++) { out12[3 * k] = out[k].x; out12[3 * k + 1] = out[k].y; out12[3 * k + 2] = out[k].z; }
     return ok ? 1 : 0;
+}
+// One edge projection: x6 = (xi, xj) updated, p6 their substep starts, lam updated; returns 0 where it does not write.
+extern "C" int oracle_edge_project_f64(double* x6, const double* p6, double wi, double wj, double rest,
+                                       double alphaTilde, double gamma, double omega, double* lam) {
+    V3<double> xi = mk(x6[0], x6[1], x6[2]), xj = mk(x6[3], x6[4], x6[5]);
+    V3<double> pi = mk(p6[0], p6[1], p6[2]), pj = mk(p6[3], p6[4], p6[5]);
+    bool ok = edgeProject(xi, xj, pi, pj, wi, wj, rest, alphaTilde, gamma, omega, *lam);
+    x6[0] = xi.x; x6[1] = xi.y; x6[2] = xi.z; x6[3] = xj.x; x6[4] = xj.y; x6[5] = xj.z;
+    return ok ? 1 : 0;
+}
+// One edge's perpendicular velocity damping: v6 = (vi, vj) updated at positions x6.
+extern "C" void oracle_perp_damp_f64(double* v6, const double* x6, double wi, double wj, double hBeta) {
+    V3<double> vi = mk(v6[0], v6[1], v6[2]), vj = mk(v6[3], v6[4], v6[5]);
+    perpDamp(vi, vj, mk(x6[0], x6[1], x6[2]), mk(x6[3], x6[4], x6[5]), wi, wj, hBeta);
+    v6[0] = vi.x; v6[1] = vi.y; v6[2] = vi.z; v6[3] = vj.x; v6[4] = vj.y; v6[5] = vj.z;
 }
 // One tet projection in FP32 (rk9: the type's rest edges for the precise invariants, or null for the direct forms).
 extern "C" int oracle_tet_project_f32(int mode, const float* p12, const float* prev12, const float* w4,
