@@ -44,14 +44,16 @@ class GtsTissueView(ct.Structure):
                 ("tet_alpha", P), ("tet_lambda", P), ("grid_edges", P), ("occupied", P), ("cut_mask", P),
                 ("edge_cut_t", P), ("edge_skin_t", P), ("rest", P), ("rot", P), ("rot_prev", P), ("prev_corner", P), ("cut_points", P), ("cut_point_capacity", I),
                 ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
-                ("use_graph", I), ("colour_block", I)]
+                ("use_graph", I), ("colour_block", I), ("fiber_lambda", P), ("feasible_target", P),
+                ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P)]
 
 
 class GtsStepParams(ct.Structure):
     _fields_ = [("dt", F), ("substeps", I), ("iterations", I), ("tet_mode", I), ("omega", F), ("edge_ks", F),
                 ("young", F), ("beta_s", F), ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F),
                 ("damping", F), ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F),
-                ("contact_radius", F), ("jaw_gap_closed", F), ("contact_compliance", F), ("precise_strain", I)]
+                ("contact_radius", F), ("jaw_gap_closed", F), ("contact_compliance", F), ("precise_strain", I),
+                ("guard_min_volume", F), ("guard_passes", I), ("tensile_strength", F)]
 
 
 class GtsToolState(ct.Structure):

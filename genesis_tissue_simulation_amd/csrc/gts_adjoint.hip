@@ -625,8 +625,9 @@ __global__ void __launch_bounds__(GTS_ADJOINT_BACKWARD_MAX_THREADS) k_adjoint_re
 int plan_step(const GtsStepParams* p, int nsub, int ckpt, AdjStep& P) {
     if (!p || nsub <= 0 || ckpt <= 0) return GTS_ERR_BAD_ARG;
     if (p->tet_mode != GTS_TET_NEOHOOKEAN || p->iterations < 1 || p->strain_damping != 0.f ||
-        p->bulk_damping != 0.f || p->beta_s != 0.f || p->omega != 1.f || p->damping != 0.f || p->rayleigh != 0.f)
-        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped Neo-Hookean substep
+        p->bulk_damping != 0.f || p->beta_s != 0.f || p->omega != 1.f || p->damping != 0.f || p->rayleigh != 0.f ||
+        p->guard_passes != 0)
+        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped, unguarded Neo-Hookean substep
     int N = p->substeps > 0 ? p->substeps : 1;
     P.h = p->dt / (float)N;
     if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;

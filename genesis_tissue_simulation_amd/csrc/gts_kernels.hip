@@ -219,9 +219,10 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 }
 
 // Friction contact of each node with the tools (friction_contact.cuh:143-169, two solve passes as at
-// xpbd_solver.cu:3663-3665), then k_update_velocity and, unless perpendicular edge damping must run between them,
-// k_apply_velscale (xpbd_solver.cu:871-919). Nodes are independent here, so one pass does all of it.
-template <bool CONTACT, bool SCALE>
+// xpbd_solver.cu:3663-3665), then (VEL) k_update_velocity and, unless perpendicular edge damping must run between
+// them, k_apply_velscale (xpbd_solver.cu:871-919). Nodes are independent here, so one pass does all of it; with the
+// tissue guard, whose safeguard runs between contact and velocity, VEL is off and k_velocity follows the safeguard.
+template <bool CONTACT, bool SCALE, bool VEL = true>
 __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, uint32_t armed, float alpha, float muS, float muD,
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
                                const float4* __restrict__ prev, float4* __restrict__ vel, float* reaction) {
@@ -275,9 +276,11 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
                 }
             pos[i] = make_float4(x.x, x.y, x.z, w);
         }
-        f3 v = w > 0.f ? (x - xp) * invH : mk3(0, 0, 0);
-        if (SCALE) v = w > 0.f ? v * velScale : mk3(0, 0, 0);
-        vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+        if (VEL) {
+            f3 v = w > 0.f ? (x - xp) * invH : mk3(0, 0, 0);
+            if (SCALE) v = w > 0.f ? v * velScale : mk3(0, 0, 0);
+            vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+        }
     }
     if (CONTACT) {
         // reaction sum: wave reduction, one atomic per wave
@@ -324,6 +327,203 @@ __global__ void k_velscale(int N, float velScale, const float4* pos, float4* vel
     if (i >= N) return;
     f3 v = pos[i].w > 0.f ? xyz(vel[i]) * velScale : mk3(0, 0, 0);
     vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+}
+
+// k_update_velocity and, unless perpendicular edge damping runs in between, k_apply_velscale (xpbd_solver.cu:871-919),
+// where the tissue guard's safeguard separates them from the node passes above.
+template <bool SCALE>
+__global__ void k_velocity(int N, float invH, float velScale, const float4* __restrict__ pos,
+                           const float4* __restrict__ prev, float4* __restrict__ vel) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float4 p4 = pos[i];
+    f3 v = p4.w > 0.f ? (xyz(p4) - xyz(prev[i])) * invH : mk3(0, 0, 0);
+    if (SCALE) v = p4.w > 0.f ? v * velScale : mk3(0, 0, 0);
+    vel[i] = make_float4(v.x, v.y, v.z, 0.f);
+}
+
+// ------------------------------------------------------------------------------------------------ tissue guard
+// Tensile reinforcement after every tet sweep and the minimum-volume safeguard at the end of every substep
+// (xpbd_solver.cu:588-741). Every corner of a tet lies in one lattice cell, so a tet colour and a lattice parity class
+// never hold two nodes of one tet: each launch below is race-free and order-independent.
+
+__device__ __forceinline__ int tet_corners(const Lattice& L, uint32_t d, int id[4]) {
+    int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
+    for (int c = 0; c < 4; c++) id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);
+    return type;
+}
+
+__device__ __forceinline__ void type_rows(const Lattice& L, int type, f3 b[3]) {
+    const float* B = L.inv_rest[type];
+    for (int r = 0; r < 3; r++) b[r] = mk3(B[3 * r], B[3 * r + 1], B[3 * r + 2]);
+}
+
+// projectTensile (xpbd_solver.cu:591-626) over one colour: on the tet's six edges in order, one XPBD step on
+// C = s^3, s = max(L / L0 - 1, 0), of compliance alphaDev / strength; the rest edges are the columns of Dm = B^-1.
+__global__ void k_tensile(int off, int cnt, Lattice L, float strength, const uint32_t* __restrict__ tet_desc,
+                          const float* __restrict__ alpha, float* __restrict__ fiber, float4* __restrict__ pos) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    int e = off + k;
+    uint32_t d = tet_desc[e];
+    if (!(d & 1u)) return;
+    int id[4];
+    int type = tet_corners(L, d, id);
+    float4 q[4];
+    f3 p[4];
+    for (int c = 0; c < 4; c++) { q[c] = pos[id[c]]; p[c] = xyz(q[c]); }
+    if (strength > 0.f) {
+        f3 b[3];
+        type_rows(L, type, b);
+        float detB = dot3(b[0], cross3(b[1], b[2]));
+        f3 rest[4] = {mk3(0, 0, 0), cross3(b[1], b[2]) * (1.f / detB), cross3(b[2], b[0]) * (1.f / detB),
+                      cross3(b[0], b[1]) * (1.f / detB)};
+        const float a = alpha[e] / fmaxf(strength, 1e-10f);
+        int f = 0;
+        for (int i = 0; i < 4; i++)
+            for (int j = i + 1; j < 4; j++, f++) {
+                f3 edge = p[j] - p[i];
+                float length = len3(edge), restLength = len3(rest[j] - rest[i]);
+                float strain = fmaxf(length / restLength - 1.f, 0.f), gradient = 3.f * strain * strain / restLength;
+                if (strain > 0.f && length > 0.f) {
+                    float wi = q[i].w, wj = q[j].w, lam = fiber[6 * e + f];
+                    float dl = (-strain * strain * strain - a * lam) / ((wi + wj) * gradient * gradient + a);
+                    f3 g = edge * (gradient / length);
+                    fiber[6 * e + f] = lam + dl;
+                    p[i] = p[i] - g * (wi * dl);
+                    p[j] = p[j] + g * (wj * dl);
+                }
+            }
+    }
+    for (int c = 0; c < 4; c++) pos[id[c]] = make_float4(p[c].x, p[c].y, p[c].z, q[c].w);
+}
+
+// k_CheckCandidateVolume (:711-717): flag when a live tet's J is below the minimum or not finite.
+__global__ void k_guard_check(int T, Lattice L, float minimum, const uint32_t* __restrict__ tet_desc,
+                              const float4* __restrict__ pos, int* flag) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    uint32_t d = tet_desc[e];
+    if (!(d & 1u)) return;
+    int id[4];
+    f3 b[3];
+    type_rows(L, tet_corners(L, d, id), b);
+    f3 p = xyz(pos[id[0]]), u = xyz(pos[id[1]]) - p, v = xyz(pos[id[2]]) - p, w = xyz(pos[id[3]]) - p;
+    float J = dot3(u, cross3(v, w)) * dot3(b[0], cross3(b[1], b[2]));
+    if (!isfinite(J) || J < minimum) atomicExch(flag, 1);
+}
+
+// k_RepairMinimumVolume (:694-709) over one colour, while the flag is up: one Newton step on J toward
+// minimum + 2e-4 for a live tet below minimum + 1e-4.
+__global__ void k_guard_repair(int off, int cnt, Lattice L, float minimum, const uint32_t* __restrict__ tet_desc,
+                               float4* __restrict__ pos, const int* __restrict__ flag) {
+    if (!*flag) return;
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    uint32_t d = tet_desc[off + k];
+    if (!(d & 1u)) return;
+    int id[4];
+    f3 b[3];
+    type_rows(L, tet_corners(L, d, id), b);
+    float4 q[4];
+    for (int c = 0; c < 4; c++) q[c] = pos[id[c]];
+    f3 p0 = xyz(q[0]), u = xyz(q[1]) - p0, v = xyz(q[2]) - p0, x = xyz(q[3]) - p0;
+    float detB = dot3(b[0], cross3(b[1], b[2]));
+    float J = dot3(u, cross3(v, x)) * detB;
+    if (!isfinite(J) || J >= minimum + 1e-4f) return;
+    f3 g[4];
+    g[1] = cross3(v, x) * detB;
+    g[2] = cross3(x, u) * detB;
+    g[3] = cross3(u, v) * detB;
+    g[0] = (g[1] + g[2] + g[3]) * -1.f;
+    float den = 0.f;
+    for (int c = 0; c < 4; c++) den += q[c].w * dot3(g[c], g[c]);
+    if (den < 1e-20f) return;
+    float dl = (minimum + 2e-4f - J) / den;
+    for (int c = 0; c < 4; c++) {
+        f3 y = xyz(q[c]) + g[c] * (q[c].w * dl);
+        pos[id[c]] = make_float4(y.x, y.y, y.z, q[c].w);
+    }
+}
+
+// k_NeedFeasible (:663-680), while the flag is up: J is multi-affine in the corners' interpolation parameters between
+// the substep start and the candidate, so a Lipschitz bound or the 16 corner configurations certify a live tet; the
+// corners of an uncertified one are marked.
+__global__ void k_guard_need(int T, Lattice L, float minimum, const uint32_t* __restrict__ tet_desc,
+                             const float4* __restrict__ start, const float4* __restrict__ pos, int* needed,
+                             const int* __restrict__ flag) {
+    if (!*flag) return;
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T) return;
+    uint32_t d = tet_desc[e];
+    if (!(d & 1u)) return;
+    int id[4];
+    f3 b[3];
+    type_rows(L, tet_corners(L, d, id), b);
+    f3 x[4], v[4];
+    for (int c = 0; c < 4; c++) { x[c] = xyz(start[id[c]]); v[c] = xyz(pos[id[c]]) - x[c]; }
+    float detB = dot3(b[0], cross3(b[1], b[2]));
+    f3 u = x[1] - x[0], vv = x[2] - x[0], w = x[3] - x[0];
+    float r0 = len3(v[0]), r1 = r0 + len3(v[1]), r2 = r0 + len3(v[2]), r3 = r0 + len3(v[3]);
+    float bound = r1 * len3(cross3(vv, w)) + r2 * len3(cross3(w, u)) + r3 * len3(cross3(u, vv)) + r1 * r2 * len3(w) +
+                  r1 * r3 * len3(vv) + r2 * r3 * len3(u) + r1 * r2 * r3;
+    if (dot3(u, cross3(vv, w)) * detB - bound * fabsf(detB) >= minimum + 1e-5f) return;
+    bool unsafe = false;
+    for (int mask = 0; mask < 16 && !unsafe; mask++) {
+        f3 p[4];
+        for (int c = 0; c < 4; c++) p[c] = x[c] + v[c] * (float)((mask >> c) & 1);
+        float j = dot3(p[1] - p[0], cross3(p[2] - p[0], p[3] - p[0])) * detB;
+        unsafe = !isfinite(j) || j < minimum + 1e-5f;
+    }
+    if (unsafe)
+        for (int c = 0; c < 4; c++) atomicExch(needed + id[c], 1);
+}
+
+// k_PrepareFeasible (:681-683): a marked node keeps its candidate as the target and restarts from the substep start.
+__global__ void k_guard_prepare(int N, const int* __restrict__ needed, const float4* __restrict__ start,
+                                float4* __restrict__ target, float4* __restrict__ pos) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || !needed[i]) return;
+    target[i] = pos[i];
+    pos[i] = start[i];
+}
+
+// k_FeasibleVertex (:634-661) over one lattice parity class: a marked node moves toward its target, cut back so that
+// no live incident tet drops below min(minimum, max(0, J)) (J is linear in this node's move), with a 0.95 margin.
+__global__ void k_guard_vertex(int N, int ny, Lattice L, int colour, float minimum, const int* __restrict__ offsets,
+                               const int* __restrict__ incident, const uint32_t* __restrict__ tet_desc,
+                               const float4* __restrict__ target, float4* __restrict__ pos,
+                               const int* __restrict__ needed) {
+    const int nx = L.nx, nz = N / (nx * ny), cx = (nx + 1) / 2, cy = (ny + 1) / 2;
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    int x = 2 * (lane % cx) + (colour & 1), y = 2 * ((lane / cx) % cy) + ((colour >> 1) & 1),
+        z = 2 * (lane / (cx * cy)) + ((colour >> 2) & 1);
+    if (x >= nx || y >= ny || z >= nz) return;
+    int i = x + nx * (y + ny * z);
+    if (!needed[i]) return;
+    float4 pi = pos[i];
+    f3 delta = xyz(target[i]) - xyz(pi);
+    float accepted = 1.f;
+    if (!isfinite(delta.x + delta.y + delta.z)) return;
+    for (int k = offsets[i]; k < offsets[i + 1]; k++) {
+        uint32_t d = tet_desc[incident[k]];
+        if (!(d & 1u)) continue;
+        int id[4];
+        f3 b[3];
+        type_rows(L, tet_corners(L, d, id), b);
+        f3 a = xyz(pos[id[0]]), u = xyz(pos[id[1]]) - a, v = xyz(pos[id[2]]) - a, w = xyz(pos[id[3]]) - a;
+        float detB = dot3(b[0], cross3(b[1], b[2]));
+        float j = dot3(u, cross3(v, w)) * detB;
+        f3 grad[3] = {cross3(v, w), cross3(w, u), cross3(u, v)};
+        f3 g = i == id[0] ? (grad[0] + grad[1] + grad[2]) * -1.f : (i == id[1] ? grad[0] : (i == id[2] ? grad[1] : grad[2]));
+        float dj = dot3(g, delta) * detB;
+        if (dj < 0.f) {
+            float floor_ = fminf(minimum, fmaxf(0.f, j));
+            if (j + dj < floor_) accepted = fminf(accepted, fmaxf(0.f, 0.95f * (j - floor_) / (-dj)));
+        }
+    }
+    f3 moved = xyz(pi) + delta * accepted;
+    pos[i] = make_float4(moved.x, moved.y, moved.z, pi.w);
 }
 
 // ------------------------------------------------------------------------------------------------ particle frames
@@ -472,9 +672,10 @@ __global__ void k_grasp_pass(int N, const GraspFrame* __restrict__ frame, float 
 }
 
 // After the coupling passes: k_CoreBreakCheck (xpbd_solver.cu:982-1000) while grasping, k_SolveCCDContact (:1049-1123)
-// while the tool is armed, k_update_velocity, the slab multiplier kept for the next substep (:3710-3711) and, unless
-// perpendicular edge damping must run in between, k_apply_velscale (:871-919). Per corner, in that order.
-template <bool SCALE>
+// while the tool is armed, (VEL) k_update_velocity, the slab multiplier kept for the next substep (:3710-3711) and,
+// unless perpendicular edge damping must run in between, k_apply_velscale (:871-919). Per corner, in that order; with
+// the tissue guard VEL is off and k_velocity follows its safeguard.
+template <bool SCALE, bool VEL = true>
 __global__ void k_legacy_finish(int N, const GraspFrame* __restrict__ frame, int grasping, int armed, float h,
                                 float invH, float velScale, GraspArgs a, float4* __restrict__ pos,
                                 const float4* __restrict__ prev, float4* __restrict__ vel) {
@@ -550,9 +751,11 @@ __global__ void k_legacy_finish(int N, const GraspFrame* __restrict__ frame, int
         }
         pos[c] = make_float4(x.x, x.y, x.z, wc);
     }
-    f3 v = wc > 0.f ? (x - xyz(prev[c])) * invH : mk3(0, 0, 0);
-    if (SCALE) v = wc > 0.f ? v * velScale : mk3(0, 0, 0);
-    vel[c] = make_float4(v.x, v.y, v.z, 0.f);
+    if (VEL) {
+        f3 v = wc > 0.f ? (x - xyz(prev[c])) * invH : mk3(0, 0, 0);
+        if (SCALE) v = wc > 0.f ? v * velScale : mk3(0, 0, 0);
+        vel[c] = make_float4(v.x, v.y, v.z, 0.f);
+    }
     if (grasping) a.slabPrev[c] = a.slab[c];
 }
 
@@ -1183,6 +1386,9 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     bool tick;
     f3 tickS, tickE, tickN;
     float tickD, tearLen2;
+    // the tissue guard: tensile sweeps per tet sweep, their strength, the safeguard's minimum J
+    int guardPasses;
+    float strength, guardMin;
 };
 
 // cut_ribbon_tick without the pressure blade (cut.cu:1303-1431) after substep n: CCD of the edges moving from the
@@ -1241,7 +1447,51 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
     P.perp = p->beta_s > 0.f;
     P.invH = 1.f / h;
     P.K = 1;
+    // xpbd_set_tissue_guard / xpbd_set_tissue_material's argument checks (xpbd_solver.cu:1895, :2165-2167)
+    if (p->guard_passes < 0 || p->guard_passes > 4) return GTS_ERR_BAD_ARG;
+    if (p->guard_passes > 0) {
+        if (!std::isfinite(p->guard_min_volume) || p->guard_min_volume < 0.f || p->guard_min_volume > .95f ||
+            !std::isfinite(p->tensile_strength) || p->tensile_strength < 0.f || p->tensile_strength > 1000.f)
+            return GTS_ERR_BAD_ARG;
+        if (P.strain && (!v->fiber_lambda || !v->feasible_target || !v->feasible_needed || !v->incident_offsets ||
+                         !v->incident_tets))
+            return GTS_ERR_BAD_ARG;
+        P.guardPasses = P.strain ? p->guard_passes : 0;
+        P.strength = p->tensile_strength;
+        P.guardMin = p->guard_min_volume;
+    }
     return GTS_OK;
+}
+
+// finishFeasible (xpbd_solver.cu:718-741) after substep n (the reference's substep index n + 1 picks the phase):
+// flags cleared, candidate check, two colour sweeps of the repair, a second check, the uncertified tets' corners sent
+// back to the substep start (prev), then eight lattice parity classes of the feasible filter.
+void launch_safeguard(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hipStream_t s, int n) {
+    const int T = v->n_tets, N = v->n_nodes, ny = L.nxy / L.nx;
+    int* needed = v->feasible_needed;
+    int* flag = needed + N;
+    k_fill_int<<<grid_for(N + 1), GTS_BLOCK, 0, s>>>(N + 1, needed, 0);
+    k_guard_check<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, L, P.guardMin, v->tet_desc, (const float4*)v->pos, flag);
+    for (int pass = 0; pass < 2; pass++)
+        for (int c = 0; c < v->n_tet_colours; c++) {
+            int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
+            if (cnt > 0)
+                k_guard_repair<<<grid_for(cnt), GTS_BLOCK, 0, s>>>(off, cnt, L, P.guardMin, v->tet_desc,
+                                                                   (float4*)v->pos, flag);
+        }
+    k_fill_int<<<1, 64, 0, s>>>(1, flag, 0);
+    k_guard_check<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, L, P.guardMin, v->tet_desc, (const float4*)v->pos, flag);
+    k_guard_need<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, L, P.guardMin, v->tet_desc, (const float4*)v->prev,
+                                                   (const float4*)v->pos, needed, flag);
+    k_guard_prepare<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, needed, (const float4*)v->prev,
+                                                      (float4*)v->feasible_target, (float4*)v->pos);
+    const int sites = ((L.nx + 1) / 2) * ((ny + 1) / 2) * ((v->vz + 2) / 2);
+    const int phase = (n + 1) & 7;
+    for (int k = 0; k < 8; k++)
+        k_guard_vertex<<<grid_for(sites), GTS_BLOCK, 0, s>>>(N, ny, L, (k + phase) & 7, P.guardMin,
+                                                             v->incident_offsets, v->incident_tets, v->tet_desc,
+                                                             (const float4*)v->feasible_target, (float4*)v->pos,
+                                                             needed);
 }
 
 // Every launch of one outer step, in the order of the reference's substep (xpbd_solver.cu:3460-3721).
@@ -1258,6 +1508,7 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
     for (int n = 0; n < P.Nsub; n++) {
         k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, (const float4*)v->force);
         if (P.legacy && (P.gate || P.grasping)) k_grasp_reset<<<g, b, 0, s>>>(N, P.gate, P.grasping, P.grasp);
+        if (P.guardPasses) k_fill_float<<<grid_for(6 * T), GTS_BLOCK, 0, s>>>(6 * T, v->fiber_lambda, 0.f);
         for (int k = 0; k < P.K; k++) {
             for (int i = 0; i < P.iters; i++) {
                 const int it = k * P.iters + i;
@@ -1278,26 +1529,44 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
                         else
                             launch_tet_colour<2>(v, L, s, off, cnt, it, passes, P.omega, P.dOverH, P.bOverH, P.precise);
                     }
+                // k_TensileReinforcement after the tet sweep, inside solveTetGraph (xpbd_solver.cu:2531-2533)
+                for (int pass = 0; pass < P.guardPasses; pass++)
+                    for (int c = 0; c < v->n_tet_colours; c++) {
+                        int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
+                        if (cnt > 0)
+                            k_tensile<<<grid_for(cnt), GTS_BLOCK, 0, s>>>(off, cnt, L, P.strength, v->tet_desc,
+                                                                          v->tet_alpha, v->fiber_lambda,
+                                                                          (float4*)v->pos);
+                    }
             }
             if (P.legacy && P.grasping) k_grasp_pass<<<g, b, 0, s>>>(N, frames + n, P.h, P.grasp, (float4*)v->pos);
         }
+#define GTS_LEGACY(SC, VL)                                                                                       \
+    k_legacy_finish<SC, VL><<<g, b, 0, s>>>(N, frames + n, P.grasping, P.gate, P.h, P.invH, P.velScale, P.grasp, \
+                                            (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel)
         if (P.legacy) {
-            if (P.perp) k_legacy_finish<false><<<g, b, 0, s>>>(N, frames + n, P.grasping, P.gate, P.h, P.invH, P.velScale,
-                                                               P.grasp, (float4*)v->pos, (const float4*)v->prev,
-                                                               (float4*)v->vel);
-            else k_legacy_finish<true><<<g, b, 0, s>>>(N, frames + n, P.grasping, P.gate, P.h, P.invH, P.velScale,
-                                                       P.grasp, (float4*)v->pos, (const float4*)v->prev,
-                                                       (float4*)v->vel);
+            if (P.guardPasses) GTS_LEGACY(false, false);
+            else if (P.perp) GTS_LEGACY(false, true);
+            else GTS_LEGACY(true, true);
         }
+#undef GTS_LEGACY
         const StepShapes* sh = shapes + n;
-#define GTS_FINISH(C, SC)                                                                                        \
-    k_finish_nodes<C, SC><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR, P.invH,   \
-                                          P.velScale, (float4*)v->pos, (const float4*)v->prev, (float4*)v->vel,  \
-                                          v->reaction)
+#define GTS_FINISH(C, SC, VL)                                                                                    \
+    k_finish_nodes<C, SC, VL><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR,       \
+                                              P.invH, P.velScale, (float4*)v->pos, (const float4*)v->prev,       \
+                                              (float4*)v->vel, v->reaction)
         if (P.legacy) {}
-        else if (P.contact) { if (P.perp) GTS_FINISH(true, false); else GTS_FINISH(true, true); }
-        else { if (P.perp) GTS_FINISH(false, false); else GTS_FINISH(false, true); }
+        else if (P.guardPasses) { if (P.contact) GTS_FINISH(true, false, false); }
+        else if (P.contact) { if (P.perp) GTS_FINISH(true, false, true); else GTS_FINISH(true, true, true); }
+        else { if (P.perp) GTS_FINISH(false, false, true); else GTS_FINISH(false, true, true); }
 #undef GTS_FINISH
+        if (P.guardPasses) {  // the safeguard between the node passes and the velocity update (:3678)
+            launch_safeguard(v, L, P, s, n);
+            if (P.perp) k_velocity<false><<<g, b, 0, s>>>(N, P.invH, P.velScale, (const float4*)v->pos,
+                                                          (const float4*)v->prev, (float4*)v->vel);
+            else k_velocity<true><<<g, b, 0, s>>>(N, P.invH, P.velScale, (const float4*)v->pos,
+                                                  (const float4*)v->prev, (float4*)v->vel);
+        }
         if (P.perp) {
             for (int c = 0; c < v->n_edge_colours; c++) {
                 int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;

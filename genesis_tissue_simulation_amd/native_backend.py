@@ -153,12 +153,14 @@ class Observation:
 
 
 def solver_params(settings: Settings) -> SolverParams:
-    """The solver parameters `NativeTissue.initialize` sends (`native_backend.py:289-325`)."""
+    """The solver parameters `NativeTissue.initialize` sends (`native_backend.py:289-325`), with the tissue guard of
+    LCS_SetTissueGuard(0.15, 1, ...) and the material of LCS_SetTissueMaterial(80, 0.01) (`native_backend.py:308-309`)."""
     return SolverParams(substeps=int(settings.substeps), iterations=int(settings.iterations), tet_mode=TET_GUARDED,
                         omega=1.0, edge_ks=0.0, young=float(settings.young_pa * METERS_PER_UNIT), beta_s=0.0,
                         strain_damping=float(settings.strain_damping_s), bulk_damping=0.01, rayleigh=0.0,
                         damping=0.0, contact_enabled=True, mu_s=0.6, mu_d=0.4, jaw_thickness=0.1,
-                        contact_radius=0.01, jaw_gap_closed=0.0, contact_compliance=1e-8)
+                        contact_radius=0.01, jaw_gap_closed=0.0, contact_compliance=1e-8, guard_min_volume=0.15,
+                        guard_passes=1, tensile_strength=80.0)
 
 
 def _unit(v):

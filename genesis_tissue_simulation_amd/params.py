@@ -36,6 +36,10 @@ class SolverParams:
     jaw_gap_closed: float = 0.0
     contact_compliance: float = 1e-8
     precise_strain: bool = False  # tet invariants from F - I (closer to FP64); False: the reference's direct forms
+    # tissue guard and material (LCS_SetTissueGuard / LCS_SetTissueMaterial, xpbd_solver.cu:1895-1915, :2165-2171)
+    guard_min_volume: float = 0.0  # minimum J of a live tet kept by the end-of-substep safeguard, 0..0.95
+    guard_passes: int = 0          # tensile reinforcement passes after every tet sweep, 0..4; 0 turns the guard off
+    tensile_strength: float = 80.0  # strength of the s^6 fibre energy (the reference's default)
 
     def with_(self, **changes):
         return replace(self, **changes)

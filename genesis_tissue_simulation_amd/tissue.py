@@ -60,7 +60,8 @@ def step_params(params: SolverParams, dt):
                          float(p.edge_ks), float(p.young), float(p.beta_s), float(p.strain_damping),
                          float(p.bulk_damping), float(p.rayleigh), float(p.damping), int(bool(p.contact_enabled)),
                          float(p.mu_s), float(p.mu_d), float(p.jaw_thickness), float(p.contact_radius),
-                         float(p.jaw_gap_closed), float(p.contact_compliance), int(bool(p.precise_strain)))
+                         float(p.jaw_gap_closed), float(p.contact_compliance), int(bool(p.precise_strain)),
+                         float(p.guard_min_volume), int(p.guard_passes), float(p.tensile_strength))
 
 
 def build_topology(grid):
@@ -251,9 +252,43 @@ class Tissue:
         check("gts_set_gravity", self.lib.gts_set_gravity(ct.byref(self.view), *(float(x) for x in g),
                                                           current_stream()))
 
+    def _guard_buffers(self):
+        """The tissue guard's device buffers (xpbd_set_tissue_guard's, xpbd_solver.cu:1900-1912), allocated on the
+        first guarded step: fibre multipliers, candidate targets, the node flags and each node's incident tet slots."""
+        if getattr(self, "fiber_lambda", None) is not None:
+            return
+        dev, N, T = self.device, self.N, self.T
+        self.fiber_lambda = torch.zeros(6 * max(T, 1), dtype=torch.float32, device=dev)
+        self.feasible_target = torch.zeros((N, 4), dtype=torch.float32, device=dev)
+        self.feasible_needed = torch.zeros(N + 1, dtype=torch.int32, device=dev)
+        self._incidence()
+        v = self.view
+        v.fiber_lambda, v.feasible_target = self.fiber_lambda.data_ptr(), self.feasible_target.data_ptr()
+        v.feasible_needed = self.feasible_needed.data_ptr()
+
+    def _incidence(self):
+        """CSR lists of the tet slots around each node, from the current tet descriptors (corner-0 node << 4 | Kuhn
+        type << 1)."""
+        from .grid import KUHN_TETS
+        dev, N, T = self.device, self.N, self.T
+        nx, ny = self.grid.dims[0] + 1, self.grid.dims[1] + 1
+        bits = torch.as_tensor(KUHN_TETS, dtype=torch.int64, device=dev)
+        corner = (bits & 1) + ((bits >> 1) & 1) * nx + ((bits >> 2) & 1) * nx * ny   # [6, 4] node offsets
+        desc = self.tet_desc[:T].to(torch.int64) & 0xFFFFFFFF
+        ids = ((desc >> 4)[:, None] + corner[(desc >> 1) & 7]).reshape(-1)
+        order = torch.sort(ids, stable=True).indices
+        self.incident_tets = (order // 4).to(torch.int32) if T else torch.zeros(1, dtype=torch.int32, device=dev)
+        counts = torch.bincount(ids, minlength=N)
+        self.incident_offsets = torch.cat((torch.zeros(1, dtype=torch.int64, device=dev),
+                                           torch.cumsum(counts, 0))).to(torch.int32)
+        self.view.incident_offsets = self.incident_offsets.data_ptr()
+        self.view.incident_tets = self.incident_tets.data_ptr()
+
     def step(self, dt, tool=None, tool_from=None, tool_armed=False, blade_armed=False, params=None):
         """One outer step; the tool pose moves from `tool_from` (default: the previous step's `tool`) to `tool`."""
         p = step_params(params or self.params, dt)
+        if p.guard_passes > 0:
+            self._guard_buffers()
         to = tool_state(tool)
         frm = tool_state(tool_from) if tool_from is not None else (self._tool_from or to)
         if self.contact_model == 0:
@@ -545,3 +580,5 @@ class Tissue:
         v.tet_colour_offsets, v.edge_colour_offsets = self.tet_offsets.ctypes.data, self.edge_offsets.ctypes.data
         v.tet_desc, v.tet_k_dev, v.tet_k_hyd = tet_desc.data_ptr(), k_dev.data_ptr(), k_hyd.data_ptr()
         v.edge_desc = edge_desc.data_ptr()
+        if getattr(self, "fiber_lambda", None) is not None:  # the guard's lists follow the new tet slots
+            self._incidence()

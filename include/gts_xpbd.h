@@ -126,6 +126,14 @@ typedef struct GtsTissueView {
     int use_graph;               /* 1: capture the step into a hipGraph once and replay it */
     int colour_block;            /* threads per workgroup of the edge / tet colour kernels: 32..1024, multiple of 32;
                                   * 0 = 64 */
+    /* tissue guard (GtsStepParams.guard_passes > 0; xpbd_set_tissue_guard, xpbd_solver.cu:1895-1915); null
+     * otherwise. The substep's start positions are `prev` (k_predict writes them; nothing on this path rewrites
+     * them before the safeguard reads them, so they equal the reference's d_feasibleStart copy). */
+    float* fiber_lambda;             /* [6T] tensile reinforcement multipliers, per tet slot */
+    float* feasible_target;          /* [4N] candidate positions of the nodes the feasible filter moves */
+    int32_t* feasible_needed;        /* [N + 1] nodes needing the filter, then the candidate-unsafe flag */
+    const int32_t* incident_offsets; /* [N + 1] CSR offsets of each node's tets */
+    const int32_t* incident_tets;    /* [4T] tet slots incident to each node (any order) */
 } GtsTissueView;
 
 /* Floats of one substep's tool shapes in GtsTissueView.step_consts (8 shapes of 23 floats). */
@@ -142,6 +150,14 @@ typedef struct GtsStepParams {
      * deviatoric gradient are formed from the displacement gradient F - I (no reference counterpart), which removes
      * their FP32 cancellation at small strain: the same constraints, closer to FP64. */
     int precise_strain;
+    /* Tissue guard and material (LCS_SetTissueGuard / LCS_SetTissueMaterial, plugin_api.cpp:212-214 ->
+     * xpbd_set_tissue_guard / xpbd_set_tissue_material, xpbd_solver.cu:1895-1915, :2165-2171). guard_passes > 0
+     * (at most 4) runs that many tensile reinforcement sweeps (fibre energy of strength tensile_strength, 0..1000)
+     * after every tet sweep and, at the end of every substep, the safeguard that keeps every live tet's J at or
+     * above guard_min_volume (0..0.95); 0 turns both off. Needs the view's guard buffers. */
+    float guard_min_volume;
+    int guard_passes;
+    float tensile_strength;
 } GtsStepParams;
 
 /* XpbdToolState (xpbd_solver.h:223-229) plus a blade capsule. */

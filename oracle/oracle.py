@@ -26,7 +26,7 @@ class OracleDesc(ct.Structure):
                 ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F), ("damping", F),
                 ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F), ("contact_radius", F),
                 ("jaw_gap_closed", F), ("contact_compliance", F), ("tool_voxel", F), ("cut_point_capacity", I),
-                ("precise_strain", I)]
+                ("precise_strain", I), ("guard_min_volume", F), ("guard_passes", I), ("tensile_strength", F)]
 
 
 class OracleTool(ct.Structure):
@@ -101,6 +101,8 @@ def lib():
             getattr(L, f"oracle_get_cut_points_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_get_liveness_{suf}").argtypes = [P, P, P]
             getattr(L, f"oracle_compact_{suf}").argtypes = [P, P, P, P, P]
+            getattr(L, f"oracle_get_feasible_{suf}").argtypes = [P, P]
+            getattr(L, f"oracle_get_feasible_{suf}").restype = I
         L.oracle_threads.argtypes = [I]
         L.oracle_threads.restype = I
         _lib = L
@@ -150,7 +152,8 @@ class Oracle:
                        p.young, p.beta_s, p.strain_damping, p.bulk_damping, p.rayleigh, p.damping,
                        int(p.contact_enabled), p.mu_s, p.mu_d, p.jaw_thickness, p.contact_radius,
                        p.jaw_gap_closed, p.contact_compliance, float(grid.cell_size[0]),
-                       4 * len(grid.edges), int(bool(getattr(p, "precise_strain", False))))
+                       4 * len(grid.edges), int(bool(getattr(p, "precise_strain", False))),
+                       float(p.guard_min_volume), int(p.guard_passes), float(p.tensile_strength))
         self.desc = d
         rest = np.ascontiguousarray(grid.positions, dtype=np.float32)
         pos = rest if initial_positions is None else np.ascontiguousarray(initial_positions, dtype=np.float32)
@@ -292,6 +295,12 @@ class Oracle:
         return dict(neighbors=nbr, cut_mask=mask, edge_cut_t=cut_t, bend_alive=bend_alive,
                     revision=int(counters[0]), cut_point_counter=int(counters[1]), tear_count=int(counters[2]),
                     cut_points=points)
+
+    def feasible(self):
+        """The nodes the last minimum-volume safeguard sent through the feasible filter (flags [N])."""
+        needed = np.zeros(self.N, dtype=np.int32)
+        self._call("get_feasible", self.h, _ptr(needed))
+        return needed
 
     def liveness(self):
         tet = np.zeros(self.T, dtype=np.int32)

@@ -121,6 +121,12 @@ struct OracleDesc {
     float tool_voxel;          // cut: lattice spacing used as AABB pad and rest edge length (voxelL)
     int cut_point_capacity;
     int precise_strain;        // FP32: tet invariants from F - I (the kernels' precise_strain); 0: the direct forms
+    // tissue guard and material (xpbd_set_tissue_guard / xpbd_set_tissue_material, xpbd_solver.cu:1895-1915,
+    // :2165-2171): with guard_passes > 0, tensile reinforcement after every tet sweep and the minimum-volume
+    // safeguard at the end of every substep
+    float guard_min_volume;    // minimum J of a live tet, 0 .. 0.95
+    int guard_passes;          // tensile reinforcement passes per tet sweep, 0 .. 4 (0: no guard)
+    float tensile_strength;    // fibre energy strength (the reference's default 80)
 };
 
 // Tool pose (XpbdToolState, xpbd_solver.h:223-229) plus the optional blade capsule; 27 floats.
@@ -542,6 +548,11 @@ template <class R> struct Tissue {
     std::vector<int> grasped, core;
     std::vector<R> offU, offN, offW, anchU, anchW, slab, slabPrev, ema, jawLam, shaftLam;
     std::vector<V3<R>> coreLam;
+    // tissue guard (xpbd_solver.cu:588-741): fibre multipliers, the substep's start and candidate positions, the
+    // nodes that need the feasible filter and each node's incident tets
+    std::vector<R> fiberLambda;
+    std::vector<V3<R>> feasStart, feasTarget;
+    std::vector<int> needed, incOff, incTets;
     int status = 0;
 
     int cid(int x, int y, int z) const { return x + nx * (y + ny * z); }
@@ -694,6 +705,18 @@ template <class R> int buildTopology(Tissue<R>& s) {
     s.kDev.assign(s.T, (R)1); s.kHyd.assign(s.T, (R)1);
     s.alphaDev.assign(s.T, 0); s.alphaHyd.assign(s.T, 0);
     s.lamDev.assign(s.T, 0); s.lamHyd.assign(s.T, 0);
+    // the guard's buffers and incidence lists (xpbd_set_tissue_guard, xpbd_solver.cu:1900-1912): tets in index order
+    s.fiberLambda.assign(6 * (size_t)s.T, 0);
+    s.feasStart.assign(s.N, V3<R>{0, 0, 0});
+    s.feasTarget.assign(s.N, V3<R>{0, 0, 0});
+    s.needed.assign(s.N, 0);
+    s.incOff.assign(s.N + 1, 0);
+    for (int i : s.tids) s.incOff[i + 1]++;
+    for (int i = 1; i <= s.N; i++) s.incOff[i] += s.incOff[i - 1];
+    s.incTets.assign(4 * (size_t)s.T, 0);
+    std::vector<int> slot(s.incOff.begin(), s.incOff.end() - 1);
+    for (int t = 0; t < s.T; t++)
+        for (int k = 0; k < 4; k++) s.incTets[slot[s.tids[4 * t + k]]++] = t;
     return 0;
 }
 
@@ -1012,6 +1035,176 @@ template <class R> void solveTet(Tissue<R>& s, int e, R dampingOverH, R bulkDamp
     for (int k = 0; k < 4; k++) s.pos[id[k]] = out[k];
 }
 
+// ---- tissue guard passes (xpbd_solver.cu:588-741) ------------------------------------------------------------------
+template <class R> void typeRows(const Tissue<R>& s, int e, V3<R> b[3]) {
+    const R(*B)[3] = s.typeB[s.ttype[e]];
+    for (int r = 0; r < 3; r++) b[r] = mk<R>(B[r][0], B[r][1], B[r][2]);
+}
+
+// projectTensile (xpbd_solver.cu:591-620) as a pure function of one tet: on its six edges (i < k, in order) the
+// finite tensile energy of strength S, C = s^3 with s = max(L / L0 - 1, 0) and compliance alphaDev / S; the rest
+// edges are the columns of Dm = B^-1, formed from the Dm^-1 rows b. lam: the six fibre multipliers, updated.
+template <class R> void tensileProject(V3<R> p[4], const R w[4], const V3<R> b[3], R alphaDev, R strength, R lam[6]) {
+    if (!(strength > 0)) return;
+    R detB = dot3(b[0], cross3(b[1], b[2]));
+    V3<R> rest[4] = {mk<R>(0, 0, 0), cross3(b[1], b[2]) * ((R)1 / detB), cross3(b[2], b[0]) * ((R)1 / detB),
+                     cross3(b[0], b[1]) * ((R)1 / detB)};
+    int fiber = 0;
+    for (int i = 0; i < 4; i++)
+        for (int k = i + 1; k < 4; k++, fiber++) {
+            V3<R> edge = p[k] - p[i];
+            R length = len3(edge), restLength = len3(rest[k] - rest[i]);
+            R strain = fmax_<R>(length / restLength - (R)1, (R)0), gradient = (R)3 * strain * strain / restLength;
+            R alpha = alphaDev / fmax_<R>(strength, (R)1e-10f);
+            if (strain > 0 && length > 0) {
+                R dl = (-strain * strain * strain - alpha * lam[fiber]) / ((w[i] + w[k]) * gradient * gradient + alpha);
+                V3<R> g = edge * (gradient / length);
+                lam[fiber] += dl;
+                p[i] = p[i] - g * (w[i] * dl);
+                p[k] = p[k] + g * (w[k] * dl);
+            }
+        }
+}
+
+template <class R> void tensilePass(Tissue<R>& s, int e, R strength) {  // k_TensileReinforcement on tet e
+    if (!s.tactive[e]) return;
+    const int* id = &s.tids[4 * e];
+    V3<R> p[4], b[3];
+    R w[4];
+    for (int k = 0; k < 4; k++) { p[k] = s.pos[id[k]]; w[k] = s.invMass[id[k]]; }
+    typeRows(s, e, b);
+    tensileProject(p, w, b, s.alphaDev[e], strength, &s.fiberLambda[6 * (size_t)e]);
+    for (int k = 0; k < 4; k++) s.pos[id[k]] = p[k];
+}
+
+template <class R> R tetJ(const Tissue<R>& s, int e, const std::vector<V3<R>>& x) {  // k_CheckCandidateVolume's J
+    const int* id = &s.tids[4 * e];
+    V3<R> b[3];
+    typeRows(s, e, b);
+    V3<R> p = x[id[0]], u = x[id[1]] - p, v = x[id[2]] - p, w = x[id[3]] - p;
+    return dot3(u, cross3(v, w)) * dot3(b[0], cross3(b[1], b[2]));
+}
+
+// k_RepairMinimumVolume (:694-709) as a pure function of one tet: below minimum + 1e-4 it is pushed to minimum +
+// 2e-4 by one rigid Newton step on J along w-weighted gradients. Returns false where it does not write.
+template <class R> bool repairProject(V3<R> p[4], const R w[4], const V3<R> b[3], R minimum) {
+    V3<R> u = p[1] - p[0], v = p[2] - p[0], x = p[3] - p[0];
+    R detB = dot3(b[0], cross3(b[1], b[2]));
+    R J = dot3(u, cross3(v, x)) * detB;
+    if (!std::isfinite(J) || J >= minimum + (R)1e-4f) return false;
+    V3<R> g[4];
+    g[1] = cross3(v, x) * detB;
+    g[2] = cross3(x, u) * detB;
+    g[3] = cross3(u, v) * detB;
+    g[0] = (g[1] + g[2] + g[3]) * (R)-1;
+    R den = 0;
+    for (int k = 0; k < 4; k++) den += w[k] * dot3(g[k], g[k]);
+    if (den < (R)1e-20f) return false;
+    R dl = (minimum + (R)2e-4f - J) / den;
+    for (int k = 0; k < 4; k++) p[k] = p[k] + g[k] * (w[k] * dl);
+    return true;
+}
+
+template <class R> void repairVolume(Tissue<R>& s, int e, R minimum) {
+    if (!s.tactive[e]) return;
+    const int* id = &s.tids[4 * e];
+    V3<R> p[4], b[3];
+    R w[4];
+    for (int k = 0; k < 4; k++) { p[k] = s.pos[id[k]]; w[k] = s.invMass[id[k]]; }
+    typeRows(s, e, b);
+    if (repairProject(p, w, b, minimum))
+        for (int k = 0; k < 4; k++) s.pos[id[k]] = p[k];
+}
+
+// k_NeedFeasible (:663-680): J is multi-affine in the four corners' interpolation parameters, so the 16 corner
+// configurations between the substep start and the candidate bound every partial move; a cheap Lipschitz bound
+// certifies most tets first. Uncertified tets mark their corners.
+template <class R> void needFeasible(Tissue<R>& s, int e, R minimum) {
+    if (!s.tactive[e]) return;
+    const int* id = &s.tids[4 * e];
+    V3<R> x[4], v[4];
+    for (int k = 0; k < 4; k++) { x[k] = s.feasStart[id[k]]; v[k] = s.pos[id[k]] - x[k]; }
+    V3<R> b[3];
+    typeRows(s, e, b);
+    R detB = dot3(b[0], cross3(b[1], b[2]));
+    V3<R> u = x[1] - x[0], vv = x[2] - x[0], w = x[3] - x[0];
+    R r0 = len3(v[0]), r1 = r0 + len3(v[1]), r2 = r0 + len3(v[2]), r3 = r0 + len3(v[3]);
+    R bound = r1 * len3(cross3(vv, w)) + r2 * len3(cross3(w, u)) + r3 * len3(cross3(u, vv)) + r1 * r2 * len3(w) +
+              r1 * r3 * len3(vv) + r2 * r3 * len3(u) + r1 * r2 * r3;
+    if (dot3(u, cross3(vv, w)) * detB - bound * std::fabs(detB) >= minimum + (R)1e-5f) return;
+    bool unsafe = false;
+    for (int mask = 0; mask < 16; mask++) {
+        V3<R> p[4];
+        for (int k = 0; k < 4; k++) p[k] = x[k] + v[k] * (R)((mask >> k) & 1);
+        R j = dot3(p[1] - p[0], cross3(p[2] - p[0], p[3] - p[0])) * detB;
+        if (!std::isfinite(j) || j < minimum + (R)1e-5f) { unsafe = true; break; }
+    }
+    if (unsafe)
+        for (int k = 0; k < 4; k++) s.needed[id[k]] = 1;
+}
+
+// k_FeasibleVertex (:634-661) on node i: its move from the substep start to the candidate, cut back so that no
+// live incident tet drops below min(minimum, max(0, J)) -- with the other corners fixed, J is linear in this
+// node's displacement -- with a 0.95 margin.
+template <class R> void feasibleVertex(Tissue<R>& s, int i, R minimum) {
+    if (!s.needed[i]) return;
+    V3<R> delta = s.feasTarget[i] - s.pos[i];
+    R accepted = 1;
+    if (!std::isfinite(delta.x + delta.y + delta.z)) return;
+    for (int k = s.incOff[i]; k < s.incOff[i + 1]; k++) {
+        int t = s.incTets[k];
+        if (!s.tactive[t]) continue;
+        const int* id = &s.tids[4 * t];
+        V3<R> a = s.pos[id[0]], u = s.pos[id[1]] - a, v = s.pos[id[2]] - a, w = s.pos[id[3]] - a;
+        V3<R> b[3];
+        typeRows(s, t, b);
+        R detB = dot3(b[0], cross3(b[1], b[2]));
+        R j = dot3(u, cross3(v, w)) * detB;
+        V3<R> grad[3] = {cross3(v, w), cross3(w, u), cross3(u, v)};
+        V3<R> g = i == id[0] ? (grad[0] + grad[1] + grad[2]) * (R)-1
+                             : (i == id[1] ? grad[0] : (i == id[2] ? grad[1] : grad[2]));
+        R dj = dot3(g, delta) * detB;
+        if (dj < 0) {
+            R floor = fmin_<R>(minimum, fmax_<R>((R)0, j));
+            if (j + dj < floor) accepted = fmin_<R>(accepted, fmax_<R>((R)0, (R)0.95f * (j - floor) / (-dj)));
+        }
+    }
+    s.pos[i] = s.pos[i] + delta * accepted;
+}
+
+// finishFeasible (:718-741) at the end of a substep: the candidate's live tets are checked against the minimum J;
+// if any fails, two colour sweeps of the volume repair, a second check, and if still unsafe the corners of the
+// uncertified tets go back to the substep start and move toward their candidate one parity colour at a time (eight
+// lattice parity classes, so no two nodes of a class share a tet), starting from the substep's phase.
+template <class R> void finishFeasible(Tissue<R>& s) {
+    if (s.d.guard_passes <= 0 || s.T <= 0) return;
+    const R minimum = (R)s.d.guard_min_volume;
+    const int phase = s.substepIndex & 7;
+    std::fill(s.needed.begin(), s.needed.end(), 0);
+    auto check = [&]() {
+        for (int t = 0; t < s.T; t++) {
+            if (!s.tactive[t]) continue;
+            R J = tetJ(s, t, s.pos);
+            if (!std::isfinite(J) || J < minimum) return true;
+        }
+        return false;
+    };
+    if (check())
+        for (int pass = 0; pass < 2; pass++)
+            for (size_t c = 0; c < s.tcolorCnt.size(); c++)
+                for (int q = 0; q < s.tcolorCnt[c]; q++) repairVolume(s, s.tcolorOrder[s.tcolorOff[c] + q], minimum);
+    if (!check()) return;
+    for (int t = 0; t < s.T; t++) needFeasible(s, t, minimum);
+    for (int i = 0; i < s.N; i++)
+        if (s.needed[i]) { s.feasTarget[i] = s.pos[i]; s.pos[i] = s.feasStart[i]; }
+    for (int k = 0; k < 8; k++) {
+        int colour = (k + phase) & 7;
+        for (int z = (colour >> 2) & 1; z < s.nz; z += 2)
+            for (int y = (colour >> 1) & 1; y < s.ny; y += 2)
+                for (int x = colour & 1; x < s.nx; x += 2) feasibleVertex(s, s.cid(x, y, z), minimum);
+    }
+}
+
 template <class R> ToolR<R> lerpTool(const ToolR<R>& from, const ToolR<R>& to, R t) {  // xpbd_solver.cu:3467-3483
     ToolR<R> r = to;
     r.jawHinge = from.jawHinge * ((R)1 - t) + to.jawHinge * t;
@@ -1275,10 +1468,11 @@ template <class R> void ccdGate(Tissue<R>& s, int c, const GraspFrame<R>& f, R c
     s.pos[c] = x;
 }
 
-// One substep of the legacy branch (xpbd_solver.cu:3460-3721, no bending, no guard passes): with the legacy contact
-// model, K = 2 material passes while grasping, each followed by the grasp ring and core pins, then the break check and
-// the CCD gate; with the friction model, friction tool contact of lattice nodes (:3659-3665,
-// friction_contact.cuh:143-169).
+// One substep of the legacy branch (xpbd_solver.cu:3460-3721, no bending): with the legacy contact model, K = 2
+// material passes while grasping, each followed by the grasp ring and core pins, then the break check and the CCD
+// gate; with the friction model, friction tool contact of lattice nodes (:3659-3665, friction_contact.cuh:143-169);
+// with the tissue guard, tensile reinforcement after every tet sweep and the minimum-volume safeguard before the
+// velocity update.
 template <class R> int substep(Tissue<R>& s) {
     if (!s.frameReady) return -1001;
     R h = s.h;
@@ -1288,6 +1482,8 @@ template <class R> int substep(Tissue<R>& s) {
     ToolR<R> previousTool = lerpTool(s.toolFrom, s.tool, previousT);
     ToolR<R> stepTool = lerpTool(s.toolFrom, s.tool, t);
     refreshLiveness(s);
+    const int guardPasses = s.d.guard_passes > 0 ? s.d.guard_passes : 0;
+    if (guardPasses) s.feasStart = s.pos;  // beginFeasible (:684-686)
     // k_predict (:393-412)
     for (int p = 0; p < s.N; p++) {
         s.prev[p] = s.pos[p];
@@ -1316,6 +1512,7 @@ template <class R> int substep(Tissue<R>& s) {
         std::fill(s.slab.begin(), s.slab.end(), (R)0);
         std::fill(s.coreLam.begin(), s.coreLam.end(), mk<R>(0, 0, 0));
     }
+    if (guardPasses) std::fill(s.fiberLambda.begin(), s.fiberLambda.end(), (R)0);  // (:3557)
     GraspFrame<R> frame{};
     R contactAlpha = 0, coreAlpha = 0;
     if (legacy) {
@@ -1341,6 +1538,15 @@ template <class R> int substep(Tissue<R>& s) {
                     for (int q = 0; q < cnt; q++)
                         solveTet(s, s.tcolorOrder[off + q], dampingOverH, bulkOverH, useLambda);
                 }
+            // k_TensileReinforcement after the tet sweep, inside solveTetGraph (:2531-2533)
+            if (strain)
+                for (int pass = 0; pass < guardPasses; pass++)
+                    for (size_t c = 0; c < s.tcolorCnt.size(); c++) {
+                        int off = s.tcolorOff[c], cnt = s.tcolorCnt[c];
+#pragma omp parallel for schedule(static) if (cnt > 2048)
+                        for (int q = 0; q < cnt; q++)
+                            tensilePass(s, s.tcolorOrder[off + q], (R)s.d.tensile_strength);
+                    }
         }
         if (grasping)  // k_SolveSlabCoulomb, then k_SolveCorePin (:3603-3610); each touches only its own corner
             for (int c = 0; c < s.N; c++) graspPass(s, c, frame, h, contactAlpha, coreAlpha);
@@ -1388,6 +1594,7 @@ template <class R> int substep(Tissue<R>& s) {
                 }
         }
     }
+    finishFeasible(s);  // (:3678)
     // k_update_velocity, [k_perp_damp_edges], k_apply_velscale (:871-919)
     R invH = (R)1 / h;
     for (int p = 0; p < s.N; p++) s.vel[p] = s.invMass[p] > 0 ? (s.pos[p] - s.prev[p]) * invH : mk<R>(0, 0, 0);
@@ -2017,6 +2224,13 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         std::memcpy(tetActiveGen, s->tactive.data(), sizeof(int) * s->T);                                             \
         std::memcpy(edgeActive, s->eactive.data(), sizeof(int) * s->E);                                               \
     }                                                                                                                 \
+    /* the nodes the last minimum-volume safeguard sent through the feasible filter; returns their count */         \
+    extern "C" int oracle_get_feasible_##SUF(void* h, int* needed) {                                                  \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        int n = 0;                                                                                                    \
+        for (int i = 0; i < s->N; i++) { if (needed) needed[i] = s->needed[i]; n += s->needed[i] ? 1 : 0; }           \
+        return n;                                                                                                     \
+    }                                                                                                                 \
     extern "C" void oracle_compact_##SUF(void* h, int* tetRemap, int* edgeRemap, int* nodeRemap, int* counts) {       \
         compact(*(Tissue<R>*)h, tetRemap, edgeRemap, nodeRemap, counts);                                              \
     }
@@ -2078,6 +2292,25 @@ extern "C" void oracle_perp_damp_f64(double* v6, const double* x6, double wi, do
     V3<double> vi = mk(v6[0], v6[1], v6[2]), vj = mk(v6[3], v6[4], v6[5]);
     perpDamp(vi, vj, mk(x6[0], x6[1], x6[2]), mk(x6[3], x6[4], x6[5]), wi, wj, hBeta);
     v6[0] = vi.x; v6[1] = vi.y; v6[2] = vi.z; v6[3] = vj.x; v6[4] = vj.y; v6[5] = vj.z;
+}
+// One tet's tensile reinforcement: p12 corners (updated), w4 inverse masses, B9 Dm^-1 rows, lam6 fibre multipliers
+// (updated).
+extern "C" void oracle_tensile_project_f64(double* p12, const double* w4, const double* B9, double alphaDev,
+                                           double strength, double* lam6) {
+    V3<double> p[4], b[3];
+    for (int k = 0; k < 4; k++) p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);
+    for (int r = 0; r < 3; r++) b[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);
+    tensileProject(p, w4, b, alphaDev, strength, lam6);
+    for (int k = 0; k < 4; k++) { p12[3 * k] = p[k].x; p12[3 * k + 1] = p[k].y; p12[3 * k + 2] = p[k].z; }
+}
+// One tet's minimum-volume repair; returns 0 where it does not write.
+extern "C" int oracle_repair_project_f64(double* p12, const double* w4, const double* B9, double minimum) {
+    V3<double> p[4], b[3];
+    for (int k = 0; k < 4; k++) p[k] = mk(p12[3 * k], p12[3 * k + 1], p12[3 * k + 2]);
+    for (int r = 0; r < 3; r++) b[r] = mk(B9[3 * r], B9[3 * r + 1], B9[3 * r + 2]);
+    bool ok = repairProject(p, w4, b, minimum);
+    for (int k = 0; k < 4; k++) { p12[3 * k] = p[k].x; p12[3 * k + 1] = p[k].y; p12[3 * k + 2] = p[k].z; }
+    return ok ? 1 : 0;
 }
 // One tet projection in FP32 (rk9: the type's rest edges for the precise invariants, or null for the direct forms).
 extern "C" int oracle_tet_project_f32(int mode, const float* p12, const float* prev12, const float* w4,

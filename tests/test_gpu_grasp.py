@@ -3,7 +3,8 @@
 Bar: bit-exact positions, velocities and grasp state (latch, grasped and core flags, core force EMA, ring anchors,
 previous slab multiplier) after every step of a capture / lift / hold / release sequence on the C1 block, with the
 CCD gate armed, over variants that take every branch: two iterations, a break force low enough for cores to let go,
-a frictionless ring that slips, perpendicular edge damping, and the step graph on and off.
+a frictionless ring that slips, perpendicular edge damping, the step graph on and off, and the guarded form with the
+tissue guard's tensile sweeps and safeguard (NativeTissue's settings).
 """
 import numpy as np
 import pytest
@@ -33,6 +34,7 @@ VARIANTS = {
     "slipping_ring": dict(grasp=dict(friction=0.0)),
     "perp_damping": dict(params=dict(beta_s=1e-4)),
     "no_graph": dict(use_graph=False),
+    "tissue_guard": dict(params=dict(tet_mode=2, bulk_damping=0.01, guard_min_volume=0.15, guard_passes=1)),
 }
 
 
