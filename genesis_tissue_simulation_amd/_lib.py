@@ -45,7 +45,8 @@ class GtsTissueView(ct.Structure):
                 ("edge_cut_t", P), ("edge_skin_t", P), ("rest", P), ("rot", P), ("rot_prev", P), ("prev_corner", P), ("cut_points", P), ("cut_point_capacity", I),
                 ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
                 ("use_graph", I), ("colour_block", I), ("fiber_lambda", P), ("feasible_target", P),
-                ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P)]
+                ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P), ("grasp_ring", P),
+                ("edge_exempt", P), ("nbr_init", P)]
 
 
 class GtsStepParams(ct.Structure):
@@ -137,6 +138,7 @@ SIGNATURES = {
     "gts_snapshot_corners": ((P, P), I),
     "gts_grasp_set_tool": ((P, P, P, P, P, P), I),
     "gts_step_legacy": ((P, P, P, P, P, P, I, P), I),
+    "gts_step_legacy_cutting": ((P, P, P, P, P, P, I, P, P, F, P), I),
     "gts_compact_topology": ((P, P, P, P, P, P, P), I),
     "gts_apply_compaction": ((P, P, P, P, P, P, P, P), I),
     "gts_profile_begin": ((), I),

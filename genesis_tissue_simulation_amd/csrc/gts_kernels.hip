@@ -792,6 +792,31 @@ __global__ void k_release(int N, int32_t* grasped, int32_t* core, float4* vel) {
     core[c] = 0;
 }
 
+// The tear law's grasp ring (cut_recompute_grasp_ring, cut.cu:1513-1528): one hop of the dilation over live links
+// (k_DilateRingHop, :687-698), then the cut-rim carve-out (k_CarveRimBridge, :703-710).
+__global__ void k_ring_hop(int N, const int* __restrict__ nbr, const int* __restrict__ ring, int* __restrict__ next) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N) return;
+    int v = ring[c] != 0 ? 1 : 0;
+    for (int k = 0; k < 6 && !v; k++) {
+        int n = nbr[6 * c + k];
+        if (n >= 0 && ring[n] != 0) v = 1;
+    }
+    next[c] = v;
+}
+
+__global__ void k_ring_carve(int N, const int* __restrict__ nbr, const int* __restrict__ nbr_init, int* ring) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= N || ring[c] == 0) return;
+    for (int k = 0; k < 6; k++)
+        if (nbr_init[6 * c + k] >= 0 && nbr[6 * c + k] < 0) { ring[c] = 0; return; }
+}
+
+__global__ void k_copy_ints(int n, const int* __restrict__ src, int* __restrict__ dst) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------------------------------------ cut kernels
 
 struct CutToolD {
@@ -937,13 +962,15 @@ __global__ void k_cut_ccd(int Eg, Lattice L, f3 S, f3 E, f3 emitN, float pad, co
 
 // k_TearOverstretch (cut.cu:607-681): an intact link between active corners, not yet marked and with an occupied
 // stencil voxel, longer than tau times its rest length (tearLen2 scaled by |rest edge|^2 / L^2) is marked at its
-// midpoint with the plane normal to it. Each edge reads and sets only its own mask bits, so the edges are independent.
-// The grasp-ring exemption (cut.cu:655-673) is absent: only the legacy grasp fills the ring, and with an empty ring
-// neither of its branches can be taken. counters[4] counts the marked edges (the reference's dbg[15]).
+// midpoint with the plane normal to it. A link with a grasp-ring corner is exempt (and restarts its release decay);
+// out of the ring it stays exempt while longer than 0.8 tau L, for up to GTS_TEAR_DECAY_TICKS ticks. Each edge reads
+// and sets only its own mask bits and exemption state, so the edges are independent. counters[4] counts the marked
+// edges (the reference's dbg[15]).
 __global__ void k_tear(int Eg, Lattice L, float tearLen2, float voxelL, const int* __restrict__ grid_edges,
                        const int* nbr, const int* active, const float4* __restrict__ pos, const float* __restrict__ rest,
                        const uint32_t* occupied, uint32_t* cut_mask, float* edge_cut_t, const float* skin,
-                       const int* inside, float4* records, float4* normals, uint32_t* counters) {
+                       const int* inside, float4* records, float4* normals, uint32_t* counters,
+                       const int* __restrict__ ring, int* __restrict__ exempt) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= Eg) return;
     records[e] = make_float4(-1.f, 0.f, 0.f, 0.f);
@@ -968,6 +995,23 @@ __global__ void k_tear(int Eg, Lattice L, float tearLen2, float voxelL, const in
     float len2 = dot3(d, d);
     f3 r = ld3(rest + 3 * idB) - ld3(rest + 3 * idA);
     float limit = tearLen2 * (dot3(r, r) / (voxelL * voxelL));
+    if (ring && exempt) {  // the grasp-ring exemption and its release decay (cut.cu:647-673); exempt: flags, ticks
+        if (ring[idA] != 0 || ring[idB] != 0) {
+            exempt[e] = 1;
+            exempt[Eg + e] = 0;
+            return;
+        }
+        if (exempt[e] != 0) {
+            float decayLen2 = 0.64f * limit;
+            if (len2 > decayLen2) {
+                int ticks = exempt[Eg + e] + 1;
+                exempt[Eg + e] = ticks;
+                if (ticks < GTS_TEAR_DECAY_TICKS) return;
+            }
+            exempt[e] = 0;
+            exempt[Eg + e] = 0;
+        }
+    }
     if (len2 <= limit) return;
     float len = sqrtf(len2);
     f3 ncut = mk3(d.x / len, d.y / len, d.z / len);
@@ -1409,7 +1453,7 @@ void launch_tick(const GtsTissueView* v, const Lattice& L, const StepPlan& P, hi
         k_tear<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, P.tearLen2, v->voxel_length, v->grid_edges, v->nbr, v->active,
                                                   (const float4*)v->pos, v->rest, v->occupied, v->cut_mask,
                                                   v->edge_cut_t, v->edge_skin_t, v->inside, records, normals,
-                                                  v->counters);
+                                                  v->counters, v->grasp_ring, v->edge_exempt);
         emit_records(v, L, s, P.tickN, 1, P.tickD);
     }
     sever_and_refresh(v, L, s);
@@ -1682,6 +1726,19 @@ int launch_step_graph(const GtsTissueView* v, const Lattice& L, const StepPlan& 
 
 namespace {
 
+// The CCD ribbon tick after every substep while a cut tool is set (cut_ribbon_tick, cut.cu:1303-1431).
+int plan_tick(const GtsTissueView* v, const GtsCutTool* cut, const float* emit_ncut, float tear_tau, StepPlan& P) {
+    if (!emit_ncut || !v->prev_corner) return GTS_ERR_BAD_ARG;
+    P.tick = true;
+    P.tickS = ld3(cut->t2v2);  // the current rod: S = t2v2, E = t1v2 (cut.cu:1382-1383)
+    P.tickE = ld3(cut->t1v2);
+    P.tickN = ld3(emit_ncut);
+    P.tickD = cut->d;
+    const float tl = tear_tau * v->voxel_length;
+    P.tearLen2 = tear_tau >= 1.f ? tl * tl : 0.f;  // tau < 1 or NaN: the tear law is off (cut.cu:1199-1202)
+    return GTS_OK;
+}
+
 // gts_step, with the ribbon tick of a set cut tool when `cut` is given.
 int step_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState* from, const GtsToolState* to,
               int tool_armed, int blade_armed, const GtsCutTool* cut, const float* emit_ncut, float tear_tau,
@@ -1694,16 +1751,7 @@ int step_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState
     StepPlan P;
     { int rc_ = make_plan(v, p, P); if (rc_) return rc_; }
     const float h = P.h;
-    if (cut) {
-        if (!emit_ncut || !v->prev_corner) return GTS_ERR_BAD_ARG;
-        P.tick = true;
-        P.tickS = ld3(cut->t2v2);  // the current rod: S = t2v2, E = t1v2 (cut.cu:1382-1383)
-        P.tickE = ld3(cut->t1v2);
-        P.tickN = ld3(emit_ncut);
-        P.tickD = cut->d;
-        const float tl = tear_tau * v->voxel_length;
-        P.tearLen2 = tear_tau >= 1.f ? tl * tl : 0.f;  // tau < 1 or NaN: the tear law is off (cut.cu:1199-1202)
-    }
+    if (cut) { int rc_ = plan_tick(v, cut, emit_ncut, tear_tau, P); if (rc_) return rc_; }
     P.contact = p->contact_enabled && (tool_armed || blade_armed);
     P.armed = (tool_armed ? 7u : 0u) | (blade_armed ? 8u : 0u);
     P.alphaC = p->contact_compliance / (h * h);
@@ -1859,12 +1907,28 @@ extern "C" int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, cons
         g->grasping = 0;
         *event = 2;
     }
+    // physics_set_tool then recomputes the tear law's ring from the current captured set (physics.cu:436-447)
+    if (v->grasp_ring) {
+        if (!v->nbr_init) return GTS_ERR_BAD_ARG;
+        int* a = v->grasp_ring;
+        int* b = a + N;
+        k_copy_ints<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, g->grasped, a);
+        for (int hop = 0; hop < gp->exempt_rings; hop++) {
+            k_ring_hop<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, v->nbr, a, b);
+            std::swap(a, b);
+        }
+        if (a != v->grasp_ring) k_copy_ints<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, a, v->grasp_ring);
+        k_ring_carve<<<grid_for(N), GTS_BLOCK, 0, s>>>(N, v->nbr, v->nbr_init, v->grasp_ring);
+        HIP_TRY(hipGetLastError());
+    }
     return GTS_OK;
 }
 
-extern "C" int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp,
-                               const GtsGraspState* g, const GtsToolState* from, const GtsToolState* to, int tool_armed,
-                               void* stream) {
+namespace {
+
+int legacy_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp, const GtsGraspState* g,
+                const GtsToolState* from, const GtsToolState* to, int tool_armed, const GtsCutTool* cut,
+                const float* emit_ncut, float tear_tau, void* stream) {
     if (!v || !p || !gp || !from || !to || !grasp_state_ok(g)) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
     if (v->colour_block < 0 || v->colour_block > 1024 || v->colour_block % 32) return GTS_ERR_BAD_ARG;
@@ -1876,6 +1940,7 @@ extern "C" int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, c
         return GTS_ERR_CAPACITY;
     const float h = P.h;
     const int N = v->n_nodes;
+    if (cut) { int rc_ = plan_tick(v, cut, emit_ncut, tear_tau, P); if (rc_) return rc_; }
     P.legacy = true;
     P.grasping = g->grasping != 0;
     P.gate = tool_armed != 0;
@@ -1916,6 +1981,22 @@ extern "C" int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, c
     if (v->use_graph && !g_profile.on) return launch_step_graph(v, L, P, s);
     enqueue_step(v, L, P, s);
     return -(int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp,
+                               const GtsGraspState* g, const GtsToolState* from, const GtsToolState* to, int tool_armed,
+                               void* stream) {
+    return legacy_impl(v, p, gp, g, from, to, tool_armed, nullptr, nullptr, 0.f, stream);
+}
+
+extern "C" int gts_step_legacy_cutting(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp,
+                                       const GtsGraspState* g, const GtsToolState* from, const GtsToolState* to,
+                                       int tool_armed, const GtsCutTool* tool, const float* emit_ncut, float tear_tau,
+                                       void* stream) {
+    if (!tool) return GTS_ERR_BAD_ARG;
+    return legacy_impl(v, p, gp, g, from, to, tool_armed, tool, emit_ncut, tear_tau, stream);
 }
 
 extern "C" int gts_detect_cut(const GtsTissueView* v, const GtsCutTool* tool, void* stream) {
@@ -2011,7 +2092,7 @@ extern "C" int gts_tear_overstretch(const GtsTissueView* v, float tau, float too
     k_tear<<<grid_for(Eg), GTS_BLOCK, 0, s>>>(Eg, L, tl * tl, v->voxel_length, v->grid_edges, v->nbr, v->active,
                                               (const float4*)v->pos, v->rest, v->occupied, v->cut_mask,
                                               v->edge_cut_t, v->edge_skin_t, v->inside, records, normals,
-                                              v->counters);
+                                              v->counters, v->grasp_ring, v->edge_exempt);
     HIP_TRY(hipGetLastError());
     return finish_cut(v, L, s, mk3(0.f, 0.f, 0.f), 1, tool_d);
 }

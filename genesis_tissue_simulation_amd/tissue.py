@@ -204,6 +204,14 @@ class Tissue:
                                      self.prev_corner, self.cut_points)],
             self.cut_point_capacity, self.counters.data_ptr(), self.scratch.data_ptr(), self.reaction.data_ptr(),
             self.step_consts.data_ptr(), self.max_substeps, int(bool(use_graph)), int(colour_block))
+        # the tear law's grasp ring [2N] (flags, dilation scratch), per cut edge exemption flags and decay ticks, and
+        # the neighbour table as built (cut.cu:68-83, :1143-1147, :1190)
+        self.grasp_ring = torch.zeros(2 * N, dtype=torch.int32, device=dev)
+        self.edge_exempt = torch.zeros(2 * max(Eg, 1), dtype=torch.int32, device=dev)
+        self.nbr_init = self.nbr.clone()
+        v = self.view
+        v.grasp_ring, v.edge_exempt = self.grasp_ring.data_ptr(), self.edge_exempt.data_ptr()
+        v.nbr_init = self.nbr_init.data_ptr()
         self._tool_from = None
         self._staging = None          # pinned host buffers of observation()
         self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep
@@ -223,6 +231,7 @@ class Tissue:
         gp = grasp or GraspParams()
         self.grasp_params = _lib.GtsGraspParams(*[float(getattr(gp, n)) for n in _lib.GRASP_FIELDS],
                                                 int(gp.exempt_rings))
+        self.grasp_ring.zero_()  # a new contact model starts with no captured corners, so no ring
         if model == 0:
             dev, N = self.device, self.N
             z = lambda n, dt=torch.float32: torch.zeros(n, dtype=dt, device=dev)
@@ -302,9 +311,15 @@ class Tissue:
                 self.grasp_event = event.value
                 if event.value == 1:
                     frm = to  # the capture offsets belong to this pose
-            check("gts_step_legacy", self.lib.gts_step_legacy(
-                ct.byref(self.view), ct.byref(p), ct.byref(self.grasp_params), ct.byref(self.grasp), ct.byref(frm),
-                ct.byref(to), int(tool_armed), current_stream()))
+            if self.cut_tool is not None:  # physics_step's ribbon tick runs whatever the contact model
+                check("gts_step_legacy_cutting", self.lib.gts_step_legacy_cutting(
+                    ct.byref(self.view), ct.byref(p), ct.byref(self.grasp_params), ct.byref(self.grasp),
+                    ct.byref(frm), ct.byref(to), int(tool_armed), ct.byref(self.cut_tool),
+                    self.emit_ncut.ctypes.data_as(ct.c_void_p), float(self.tear_tau), current_stream()))
+            else:
+                check("gts_step_legacy", self.lib.gts_step_legacy(
+                    ct.byref(self.view), ct.byref(p), ct.byref(self.grasp_params), ct.byref(self.grasp),
+                    ct.byref(frm), ct.byref(to), int(tool_armed), current_stream()))
         elif self.cut_tool is not None:
             check("gts_step_cutting", self.lib.gts_step_cutting(
                 ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to), int(tool_armed), int(blade_armed),
@@ -350,6 +365,12 @@ class Tissue:
     def rotations(self):
         """Particle frames, [N, 3, 3] rows of R."""
         return self.rot.cpu().numpy().reshape(-1, 3, 3)
+
+    def tear_exempt(self):
+        """The tear law's grasp ring [N] and each cut edge's exemption flag and release-decay ticks [Eg]."""
+        Eg = self.Eg
+        ex = self.edge_exempt.cpu().numpy()
+        return self.grasp_ring[:self.N].cpu().numpy(), ex[:Eg], ex[Eg:2 * Eg]
 
     def tear(self, tau, tool_d):
         """Tear law: every intact link stretched past tau times its rest length is cut at its midpoint, then the

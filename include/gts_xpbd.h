@@ -134,7 +134,15 @@ typedef struct GtsTissueView {
     int32_t* feasible_needed;        /* [N + 1] nodes needing the filter, then the candidate-unsafe flag */
     const int32_t* incident_offsets; /* [N + 1] CSR offsets of each node's tets */
     const int32_t* incident_tets;    /* [4T] tet slots incident to each node (any order) */
+    /* the tear law's grasp-ring exemption (cut.cu:68-83, :647-673); null: no exemption */
+    int32_t* grasp_ring;             /* [2N] ring flags (the first N), then the dilation's ping-pong half; zeroed */
+    int32_t* edge_exempt;            /* [2 Eg] per cut edge: was-exempt flag, then release-decay ticks; zeroed */
+    const int32_t* nbr_init;         /* [6N] the neighbour table as built (cut-rim carve-out of the ring) */
 } GtsTissueView;
+
+/* Ribbon ticks a released grasp-ring edge stays exempt from the tear law while still longer than 0.8 tau L
+ * (TEAR_DECAY_TIMEOUT_TICKS, cut.cu:83). */
+#define GTS_TEAR_DECAY_TICKS 2000
 
 /* Floats of one substep's tool shapes in GtsTissueView.step_consts (8 shapes of 23 floats). */
 #define GTS_STEP_CONST_FLOATS 184
@@ -242,7 +250,10 @@ int gts_step(const GtsTissueView* v, const GtsStepParams* p, const GtsToolState*
  * are marked on the device (k_MarkGraspCandidates), read back (the stream is synchronised), the normalised-ellipse core
  * is chosen on the host and the grasp state is uploaded. grasp01 falling below gp->release releases: the grasped
  * corners' velocity is scaled by 0.1 and the sets are cleared. *event = 1 on capture, 2 on release, else 0. After a
- * capture the next step must start at `tool` (the reference sets g_toolFrom = g_tool). */
+ * capture the next step must start at `tool` (the reference sets g_toolFrom = g_tool). Then, with v->grasp_ring,
+ * the tear law's ring is recomputed from the grasped set as physics_set_tool does (physics.cu:436-447 ->
+ * cut_recompute_grasp_ring, cut.cu:1513-1528): gp->exempt_rings hops of dilation over live links, minus the corners
+ * that have lost a link since v->nbr_init. */
 int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, const GtsGraspParams* gp, const GtsToolState* tool,
                        int* event, void* stream);
 /* Device: one outer step under the legacy contact model (xpbd_substep with g_frictionContact false,
@@ -252,6 +263,12 @@ int gts_grasp_set_tool(const GtsTissueView* v, GtsGraspState* g, const GtsGraspP
  * follows the pose from `from` to `to` over the substeps. The contact fields of *p are not used, nor the reaction. */
 int gts_step_legacy(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp, const GtsGraspState* g,
                     const GtsToolState* from, const GtsToolState* to, int tool_armed, void* stream);
+/* Device: gts_step_legacy while LCS_SetTool's cut tool is set: the CCD ribbon tick after every substep as in
+ * gts_step_cutting (physics_step runs it whatever the contact model, physics.cu:352-377), with the tear law's
+ * grasp-ring exemption when tear_tau >= 1. */
+int gts_step_legacy_cutting(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspParams* gp,
+                            const GtsGraspState* g, const GtsToolState* from, const GtsToolState* to, int tool_armed,
+                            const GtsCutTool* tool, const float* emit_ncut, float tear_tau, void* stream);
 /* Device: LCS_Step while LCS_SetTool's cut tool is set (physics_step, physics.cu:331-394): gts_step with the CCD
  * ribbon tick after every substep (cut_ribbon_tick without the pressure blade, cut.cu:1303-1431): k_DetectCutCCD of the
  * edges moving from v->prev_corner to the current positions against the tool's current rod S = t2v2, E = t1v2 (emit_ncut,
@@ -286,7 +303,8 @@ int gts_detect_cut_ccd(const GtsTissueView* v, const float* S, const float* E, c
  * cut.cu:1397-1405): every intact link between active corners that is not yet cut, has an occupied stencil voxel and
  * is longer than tau times its rest length is cut at its midpoint (cut points with tool gap tool_d); then the links are
  * severed and liveness refreshed. tau < 1 or NaN: no-op (cut.cu:1199-1202). Needs v->rest. Call it after the tick's
- * detection; the grasp-ring exemption of the reference is inert on this path (no grasp ring). */
+ * detection. With v->grasp_ring and v->edge_exempt, links with a ring corner are exempt, and released ones decay
+ * (cut.cu:647-673). */
 int gts_tear_overstretch(const GtsTissueView* v, float tau, float tool_d, void* stream);
 /* Device: stable remap of the live topology. tet_remap [T] (colour order), edge_remap [E], node_remap [N],
  * counts [3] (live tets, edges, nodes); all device. scratch as in the view. */

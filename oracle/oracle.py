@@ -101,6 +101,8 @@ def lib():
             getattr(L, f"oracle_get_cut_points_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_get_liveness_{suf}").argtypes = [P, P, P]
             getattr(L, f"oracle_compact_{suf}").argtypes = [P, P, P, P, P]
+            getattr(L, f"oracle_set_tear_exempt_{suf}").argtypes = [P, P, P, P]
+            getattr(L, f"oracle_get_tear_exempt_{suf}").argtypes = [P, P, P, P]
             getattr(L, f"oracle_get_feasible_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_get_feasible_{suf}").restype = I
         L.oracle_threads.argtypes = [I]
@@ -295,6 +297,18 @@ class Oracle:
         return dict(neighbors=nbr, cut_mask=mask, edge_cut_t=cut_t, bend_alive=bend_alive,
                     revision=int(counters[0]), cut_point_counter=int(counters[1]), tear_count=int(counters[2]),
                     cut_points=points)
+
+    def tear_exempt(self):
+        """The tear law's grasp ring [N] and each grid edge's exemption flag and release-decay ticks [Eg]."""
+        Eg = len(self.grid.edges)
+        ring, was, ticks = np.zeros(self.N, np.int32), np.zeros(Eg, np.int32), np.zeros(Eg, np.int32)
+        self._call("get_tear_exempt", self.h, _ptr(ring), _ptr(was), _ptr(ticks))
+        return ring, was, ticks
+
+    def set_tear_exempt(self, ring=None, was=None, ticks=None):
+        args = [None if a is None else np.ascontiguousarray(a, dtype=np.int32) for a in (ring, was, ticks)]
+        self._keep_exempt = args
+        self._call("set_tear_exempt", self.h, *[None if a is None else _ptr(a) for a in args])
 
     def feasible(self):
         """The nodes the last minimum-volume safeguard sent through the feasible filter (flags [N])."""

@@ -553,6 +553,8 @@ template <class R> struct Tissue {
     std::vector<R> fiberLambda;
     std::vector<V3<R>> feasStart, feasTarget;
     std::vector<int> needed, incOff, incTets;
+    // the tear law's grasp ring and release decay (cut.cu:68-83): per node, per grid edge
+    std::vector<int> graspRing, edgeWasExempt, edgeDecayTicks;
     int status = 0;
 
     int cid(int x, int y, int z) const { return x + nx * (y + ny * z); }
@@ -1843,9 +1845,38 @@ template <class R> void severLinks(Tissue<R>& s) {  // k_SeverLinks (cut.cu:732-
 // k_TearOverstretch (cut.cu:607-681), then k_SeverLinks, as cut_ribbon_tick runs them (cut.cu:1397-1405): an intact
 // link between active corners, not yet marked and with an occupied stencil voxel, that is longer than tau times its
 // rest length is marked at its midpoint with the plane normal to it. tau < 1 or NaN disables the law (cut.cu:1199-1202).
-// The grasp-ring exemption (cut.cu:655-673) is left out: only the legacy grasp fills the ring, so on this path the
-// ring is empty, edgeWasExempt never becomes 1 and both exemption branches are unreachable. Each edge reads and sets
-// only its own mask bits, so the serial edge order decides nothing but the cut point slots.
+// A link with a grasp-ring corner is exempt, and a released one stays exempt while it decays (above). Each edge reads
+// and sets only its own mask bits and exemption state, so the serial edge order decides nothing but the cut point
+// slots.
+// cut_recompute_grasp_ring (cut.cu:1513-1528), after every tool update (physics_set_tool, physics.cu:436-447): the
+// captured corners, dilated by exempt_rings hops over live links (k_DilateRingHop), minus every corner that has lost a
+// link since the lattice was built (k_CarveRimBridge): a cut-rim corner keeps the tear law.
+template <class R> void recomputeGraspRing(Tissue<R>& s) {
+    const int hops = s.gp.exempt_rings > 0 ? s.gp.exempt_rings : 0;
+    std::vector<int> ring(s.N, 0), next(s.N, 0);
+    if ((int)s.grasped.size() == s.N) ring = s.grasped;
+    for (int hop = 0; hop < hops; hop++) {
+        for (int c = 0; c < s.N; c++) {
+            int v = ring[c] != 0 ? 1 : 0;
+            for (int k = 0; k < 6 && !v; k++) {
+                int n = s.nbr[6 * c + k];
+                if (n >= 0 && ring[n] != 0) v = 1;
+            }
+            next[c] = v;
+        }
+        ring.swap(next);
+    }
+    for (int c = 0; c < s.N; c++) {
+        if (ring[c] == 0) continue;
+        for (int k = 0; k < 6; k++)
+            if (s.nbr0[6 * c + k] >= 0 && s.nbr[6 * c + k] < 0) { ring[c] = 0; break; }
+    }
+    s.graspRing = ring;
+}
+
+// The tear law's exemption (cut.cu:647-673): an edge with a ring corner is exempt and (re)starts its release
+// decay; once out of the ring it stays exempt while longer than 0.8 tau L, for up to 2000 ticks.
+constexpr int TEAR_DECAY_TICKS = 2000;
 template <class R> void tearOverstretch(Tissue<R>& s, float tau, R toolD, bool sever = true) {
     if (!(tau >= 1.0f)) return;
     float voxelL = s.d.tool_voxel;
@@ -1872,6 +1903,21 @@ template <class R> void tearOverstretch(Tissue<R>& s, float tau, R toolD, bool s
         V3<float> rb = mk<float>((float)s.rest[idB].x, (float)s.rest[idB].y, (float)s.rest[idB].z);
         V3<float> r = rb - ra;
         float limit = tearLen2 * (dot3(r, r) / (voxelL * voxelL));
+        if (s.graspRing[idA] != 0 || s.graspRing[idB] != 0) {
+            s.edgeWasExempt[e] = 1;
+            s.edgeDecayTicks[e] = 0;
+            continue;
+        }
+        if (s.edgeWasExempt[e] != 0) {
+            float decayLen2 = 0.64f * limit;
+            if (len2 > (R)decayLen2) {
+                int ticks = s.edgeDecayTicks[e] + 1;
+                s.edgeDecayTicks[e] = ticks;
+                if (ticks < TEAR_DECAY_TICKS) continue;
+            }
+            s.edgeWasExempt[e] = 0;
+            s.edgeDecayTicks[e] = 0;
+        }
         if (len2 <= (R)limit) continue;
         R len = std::sqrt(len2);
         V3<R> ncut = mk<R>(d.x / len, d.y / len, d.z / len);
@@ -2076,6 +2122,9 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
     s->cutMask.assign(s->V, 0u);
     s->edgeCutT.assign(6 * N, -1.f);
     s->edgeSkinT.assign(edgeSkin, edgeSkin + 6 * N);
+    s->graspRing.assign(N, 0);
+    s->edgeWasExempt.assign(d->grid_edge_count, 0);
+    s->edgeDecayTicks.assign(d->grid_edge_count, 0);
     s->status = buildTopology(*s);
     for (int e = 0; e < s->E; e++) {  // L0 = sqrtf(|restNbr[6p+s]|^2) in float (xpbd_solver.cu:1458)
         const float* r = restNbr + 3 * s->eowner[e];
@@ -2151,7 +2200,9 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         s->tool = toolFrom<R>(*t);                                                                                    \
         if (reset) s->toolFrom = s->tool;                                                                             \
         s->toolArmed = toolArmed != 0; s->bladeArmed = bladeArmed != 0;                                               \
-        return s->contactModel == 0 ? graspSetTool(*s) : 0;                                                           \
+        int event = s->contactModel == 0 ? graspSetTool(*s) : 0;                                                      \
+        recomputeGraspRing(*s);                                                                                       \
+        return event;                                                                                                 \
     }                                                                                                                 \
     extern "C" void oracle_set_contact_model_##SUF(void* h, int model, const OracleGrasp* gp) {                       \
         setContactModel(*(Tissue<R>*)h, model, *gp);                                                                  \
@@ -2223,6 +2274,19 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         refreshLiveness(*s);                                                                                          \
         std::memcpy(tetActiveGen, s->tactive.data(), sizeof(int) * s->T);                                             \
         std::memcpy(edgeActive, s->eactive.data(), sizeof(int) * s->E);                                               \
+    }                                                                                                                 \
+    /* the tear law's grasp ring [N] and per grid edge exemption flag and decay ticks [Eg]; null arrays are skipped */ \
+    extern "C" void oracle_set_tear_exempt_##SUF(void* h, const int* ring, const int* was, const int* ticks) {       \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        if (ring) s->graspRing.assign(ring, ring + s->N);                                                             \
+        if (was) s->edgeWasExempt.assign(was, was + s->d.grid_edge_count);                                            \
+        if (ticks) s->edgeDecayTicks.assign(ticks, ticks + s->d.grid_edge_count);                                     \
+    }                                                                                                                 \
+    extern "C" void oracle_get_tear_exempt_##SUF(void* h, int* ring, int* was, int* ticks) {                         \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        if (ring) std::copy(s->graspRing.begin(), s->graspRing.end(), ring);                                          \
+        if (was) std::copy(s->edgeWasExempt.begin(), s->edgeWasExempt.end(), was);                                    \
+        if (ticks) std::copy(s->edgeDecayTicks.begin(), s->edgeDecayTicks.end(), ticks);                              \
     }                                                                                                                 \
     /* the nodes the last minimum-volume safeguard sent through the feasible filter; returns their count */         \
     extern "C" int oracle_get_feasible_##SUF(void* h, int* needed) {                                                  \
