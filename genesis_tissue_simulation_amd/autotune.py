@@ -31,7 +31,9 @@ def candidates(max_lds_bytes):
 
 def tune(batch, dt, steps=3, warmup=1, space=None):
     """Time every candidate launch shape on `batch` and leave the fastest configured. Returns (best, table)."""
-    saved = {name: getattr(batch, name).clone() for name in ("pos", "prev", "vel", "lambda_edge", "lambda_tet")}
+    # everything a step advances, the particle frames included (each step's frame update blends the previous ones)
+    saved = {name: getattr(batch, name).clone()
+             for name in ("pos", "prev", "vel", "lambda_edge", "lambda_tet", "rot", "rot_prev")}
     space = space or candidates(int(batch.lib.gts_batched_lds_bytes()))
     table = []
     for cand in space:

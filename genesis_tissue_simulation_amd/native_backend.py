@@ -318,23 +318,31 @@ class NativeTissue:
         return self.tissue.material_quality().cpu().numpy()
 
     def contact_depth_m(self):
-        """LCS_SurfaceDepth (`native_backend.py:453-456`): the deepest penetration of the grasper's shaft capsule and
-        jaw boxes into the reconstructed surface, in metres, 0 before the first tool command (the tool is unarmed)."""
+        """LCS_SurfaceDepth (`native_backend.py:453-456` -> xpbd_surface_depth, xpbd_solver.cu:2046-2059): the deepest
+        penetration of the grasper's shaft capsule and jaw boxes and, while the knife is enabled, of its edge capsule
+        (queryBladeDepth) into the reconstructed surface, in metres; 0 before the first tool command (the tool is
+        unarmed). The knife capsule sits at the commanded pose: the reference's pressure-blade body follows the command
+        through its drive dynamics, which this path does not simulate, so the two agree once the body has caught up
+        with the command."""
         if self.closed:
             raise RuntimeError("Native session is closed")
         if self._tool is None:
             return 0.0
-        return float(self.tissue.surface_depth(self._tool, kinds=0b0111).item()) * METERS_PER_UNIT
+        tool, kinds = dict(self._tool), 0b0111
+        if self._blade is not None:
+            tool.update(bladeA=self._blade[0], bladeB=self._blade[1], bladeR=self.settings.blade_radius)
+            kinds |= 0b1000
+        return float(self.tissue.surface_depth(tool, kinds=kinds).item()) * METERS_PER_UNIT
 
     def knife_body_depth_m(self):
-        """LCS_KnifeBodyDepth (`native_backend.py:468-471`): the deepest penetration of the knife's capsule into the
-        reconstructed surface, in metres, 0 while the knife is off."""
+        """LCS_KnifeBodyDepth (`native_backend.py:468-471` -> xpbd_knife_body_depth, pressure_blade.cuh:50-57) measures
+        the knife's two back boxes on the pressure-blade body, a body this path does not simulate (its dynamics and
+        shapes belong to the pressure blade). Rather than report another quantity under the reference's name, this
+        raises."""
         if self.closed:
             raise RuntimeError("Native session is closed")
-        if self._blade is None:
-            return 0.0
-        knife = dict(bladeA=self._blade[0], bladeB=self._blade[1], bladeR=self.settings.blade_radius)
-        return float(self.tissue.surface_depth(knife, kinds=0b1000).item()) * METERS_PER_UNIT
+        raise NotImplementedError("knife_body_depth_m needs the pressure-blade body (knife back boxes), which this "
+                                  "backend does not simulate; contact_depth_m includes the knife's edge capsule")
 
     def surface(self):
         """The surface LCS_Finalize last rebuilt (`native_backend.py:434-451`), with the reference's checks: an empty
