@@ -46,7 +46,7 @@ class GtsTissueView(ct.Structure):
                 ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
                 ("use_graph", I), ("colour_block", I), ("fiber_lambda", P), ("feasible_target", P),
                 ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P), ("grasp_ring", P),
-                ("edge_exempt", P), ("nbr_init", P)]
+                ("edge_exempt", P), ("nbr_init", P), ("xcd_order", I)]
 
 
 class GtsStepParams(ct.Structure):

@@ -39,6 +39,7 @@ inline int grid_for(int n, int block = GTS_BLOCK) { return std::max(1, (n + bloc
 
 struct Lattice {  // strides and per-type rest data passed by value to every kernel
     int nx, nxy, vx, vy, vz;
+    int xcd_order;  // the colour kernels' workgroup -> item mapping (colour_item)
     float inv_rest[6][9];
     float rest_len[3];
 };
@@ -48,6 +49,7 @@ Lattice lattice_of(const GtsTissueView* v) {
     L.vx = v->vx; L.vy = v->vy; L.vz = v->vz;
     L.nx = v->vx + 1;
     L.nxy = (v->vx + 1) * (v->vy + 1);
+    L.xcd_order = v->xcd_order;
     std::memcpy(L.inv_rest, v->inv_rest, sizeof(L.inv_rest));
     for (int a = 0; a < 3; a++) L.rest_len[a] = v->axis_rest_length[a];
     return L;
@@ -56,6 +58,20 @@ Lattice lattice_of(const GtsTissueView* v) {
 __device__ __forceinline__ int axis_stride(const Lattice& L, int axis) { return axis == 0 ? 1 : (axis == 1 ? L.nx : L.nxy); }
 
 __device__ __forceinline__ f3 xyz(float4 p) { return mk3(p.x, p.y, p.z); }
+
+// The item a thread of a colour kernel takes. Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on
+// XCD b % 8). With L.xcd_order the workgroups of one XCD take one contiguous range of the colour; colours list their
+// items in lattice order, so an XCD then meets the same slab of the lattice in every colour, and that slab's nodes can
+// stay in the XCD's L2 between launches. The mapping is a bijection, so results do not depend on it.
+__device__ __forceinline__ int ordered_item(int xcd_order) {
+    int b = blockIdx.x;
+    if (xcd_order) {
+        const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, slot = b >> 3;
+        b = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + slot;
+    }
+    return b * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ int colour_item(const Lattice& L) { return ordered_item(L.xcd_order); }
 
 // ------------------------------------------------------------------------------------------------ set-up kernels
 
@@ -153,8 +169,8 @@ __global__ void k_tet_alpha(int T, float invH2, const float* kd, const float* kh
 
 // k_predict (xpbd_solver.cu:393-412): prev = x; for w > 0, v += h F w and x += h v.
 __global__ void k_predict(int N, float h, float4* __restrict__ pos, float4* __restrict__ prev,
-                          float4* __restrict__ vel, const float4* __restrict__ force) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+                          float4* __restrict__ vel, const float4* __restrict__ force, int xcd_order) {
+    int i = ordered_item(xcd_order);  // nodes in lattice order: the XCD that owns their slab in the colour kernels
     if (i >= N) return;
     float4 p = pos[i];
     prev[i] = p;
@@ -174,7 +190,7 @@ template <bool READ_LAMBDA, bool WRITE_LAMBDA, bool FALLBACK, bool DAMPED>
 __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fallbackInvEh2, float omega,
                         float betaGammaH, const uint32_t* __restrict__ edge_desc, float* __restrict__ lambda,
                         float4* __restrict__ pos, const float4* __restrict__ prev) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = colour_item(L);
     if (k >= cnt) return;
     int e = off + k;
     uint32_t d = edge_desc[e];
@@ -198,7 +214,7 @@ template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED, bool PRECI
 __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float dampingOverH, float bulkDampingOverH,
                        const uint32_t* __restrict__ tet_desc, const float* __restrict__ alpha,
                        float* __restrict__ lambda, float4* __restrict__ pos, const float4* __restrict__ prev) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = colour_item(L);
     if (k >= cnt) return;
     int e = off + k;
     uint32_t d = tet_desc[e];
@@ -225,8 +241,9 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 template <bool CONTACT, bool SCALE, bool VEL = true>
 __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, uint32_t armed, float alpha, float muS, float muD,
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
-                               const float4* __restrict__ prev, float4* __restrict__ vel, float* reaction) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+                               const float4* __restrict__ prev, float4* __restrict__ vel, float* reaction,
+                               int xcd_order) {
+    int i = ordered_item(xcd_order);
     f3 force = mk3(0, 0, 0);
     float contacts = 0.f;
     if (i < N) {
@@ -302,7 +319,7 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
 // k_perp_damp_edges (xpbd_solver.cu:890-910) over one colour.
 __global__ void k_perp_damp(int off, int cnt, Lattice L, float cs, float h, const uint32_t* edge_desc,
                             const float4* pos, float4* vel) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = colour_item(L);
     if (k >= cnt) return;
     uint32_t d = edge_desc[off + k];
     if (!(d & 1u)) return;
@@ -362,7 +379,7 @@ __device__ __forceinline__ void type_rows(const Lattice& L, int type, f3 b[3]) {
 // C = s^3, s = max(L / L0 - 1, 0), of compliance alphaDev / strength; the rest edges are the columns of Dm = B^-1.
 __global__ void k_tensile(int off, int cnt, Lattice L, float strength, const uint32_t* __restrict__ tet_desc,
                           const float* __restrict__ alpha, float* __restrict__ fiber, float4* __restrict__ pos) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = colour_item(L);
     if (k >= cnt) return;
     int e = off + k;
     uint32_t d = tet_desc[e];
@@ -418,7 +435,7 @@ __global__ void k_guard_check(int T, Lattice L, float minimum, const uint32_t* _
 __global__ void k_guard_repair(int off, int cnt, Lattice L, float minimum, const uint32_t* __restrict__ tet_desc,
                                float4* __restrict__ pos, const int* __restrict__ flag) {
     if (!*flag) return;
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = colour_item(L);
     if (k >= cnt) return;
     uint32_t d = tet_desc[off + k];
     if (!(d & 1u)) return;
@@ -1550,7 +1567,8 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
     // the multipliers are reset once per substep, so the K passes of a substep count as K * iters iterations
     const int passes = P.K * P.iters;
     for (int n = 0; n < P.Nsub; n++) {
-        k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, (const float4*)v->force);
+        k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, (const float4*)v->force,
+                                  L.xcd_order);
         if (P.legacy && (P.gate || P.grasping)) k_grasp_reset<<<g, b, 0, s>>>(N, P.gate, P.grasping, P.grasp);
         if (P.guardPasses) k_fill_float<<<grid_for(6 * T), GTS_BLOCK, 0, s>>>(6 * T, v->fiber_lambda, 0.f);
         for (int k = 0; k < P.K; k++) {
@@ -1598,7 +1616,7 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
 #define GTS_FINISH(C, SC, VL)                                                                                    \
     k_finish_nodes<C, SC, VL><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR,       \
                                               P.invH, P.velScale, (float4*)v->pos, (const float4*)v->prev,       \
-                                              (float4*)v->vel, v->reaction)
+                                              (float4*)v->vel, v->reaction, L.xcd_order)
         if (P.legacy) {}
         else if (P.guardPasses) { if (P.contact) GTS_FINISH(true, false, false); }
         else if (P.contact) { if (P.perp) GTS_FINISH(true, false, true); else GTS_FINISH(true, true, true); }

@@ -105,11 +105,13 @@ class Tissue:
     """Device state of one tissue and the HIP entry points that advance and cut it."""
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
-                 device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None, colour_block=0):
+                 device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None, colour_block=0,
+                 xcd_order=True):
         """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, mass, active, pinned, nbr,
         edge_desc, edge_lambda, tet_desc, tet_k_dev, tet_k_hyd, tet_lambda, rot, rot_prev) -- views into the pooled
         buffers of a batch. `colour_block` is
-        the workgroup size of the edge / tet colour kernels (0: the library default)."""
+        the workgroup size of the edge / tet colour kernels (0: the library default); `xcd_order` gives each XCD one
+        contiguous range of every colour (same results)."""
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -212,6 +214,7 @@ class Tissue:
         v = self.view
         v.grasp_ring, v.edge_exempt = self.grasp_ring.data_ptr(), self.edge_exempt.data_ptr()
         v.nbr_init = self.nbr_init.data_ptr()
+        v.xcd_order = int(bool(xcd_order))
         self._tool_from = None
         self._staging = None          # pinned host buffers of observation()
         self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep

@@ -138,6 +138,9 @@ typedef struct GtsTissueView {
     int32_t* grasp_ring;             /* [2N] ring flags (the first N), then the dilation's ping-pong half; zeroed */
     int32_t* edge_exempt;            /* [2 Eg] per cut edge: was-exempt flag, then release-decay ticks; zeroed */
     const int32_t* nbr_init;         /* [6N] the neighbour table as built (cut-rim carve-out of the ring) */
+    int xcd_order;                   /* 1: the colour kernels give each XCD one contiguous range of a colour (the
+                                      * workgroups of an XCD meet the same lattice slab in every colour); 0: the
+                                      * plain workgroup order. Results are the same either way. */
 } GtsTissueView;
 
 /* Ribbon ticks a released grasp-ring edge stays exempt from the tear law while still longer than 0.8 tau L

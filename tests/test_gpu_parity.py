@@ -268,3 +268,28 @@ def test_particle_frames_and_cut_points_in_the_owner_frame():
     assert np.abs(rot - np.eye(3)).max() > 0.3
     assert_cut_equal(t.cut_state(), o.cut_state())
     assert o.cut_state()["cut_point_counter"] > 0
+
+
+def test_xcd_order_changes_nothing():
+    """The XCD-aware workgroup order of the colour and node kernels is a bijection of the items: with it on and off
+    the step (tool contact, damping, cuts) gives the same bits."""
+    from genesis_tissue_simulation_amd.tissue import Tissue
+    w = block_6144()
+    p = w.params.with_(contact_enabled=True, strain_damping=0.01, beta_s=0.5, iterations=2)
+    kd, kh = w.stiffness()
+    sims = []
+    for order in (False, True):
+        t = Tissue(w.grid, p, kd, kh, w.initial_positions, xcd_order=order)
+        t.set_gravity(w.gravity)
+        sims.append(t)
+    L = 0.25
+    for s in range(6):
+        tool = dict(shaftA=(-L, 8 * L - 0.1 * s * L, 4 * L), shaftB=(17 * L, 8 * L - 0.1 * s * L, 4 * L),
+                    shaftR=0.3 * L)
+        for t in sims:
+            t.step(w.dt, tool=tool, tool_armed=True)
+        advance(sims, w, 1, cut=True, start=s)
+    a, b = sims
+    np.testing.assert_array_equal(a.positions(), b.positions())
+    np.testing.assert_array_equal(a.velocities(), b.velocities())
+    assert_cut_equal(a.cut_state(), b.cut_state())
