@@ -112,7 +112,16 @@ def write_traffic(fetch_csv, write_csv, copy_mib, command):
         if k in fetch and k in write:
             rb, wb = fetch[k][0] * f_fetch * 1024.0, write[k][0] * f_write * 1024.0
             out["kernels"][k] = {"dispatches": fetch[k][1], "read_bytes_per_launch": rb,
-                                 "write_bytes_per_launch": wb, "bytes_per_launch": rb + wb}
+                                 "write_bytes_per_launch": wb, "bytes_per_launch": rb + wb, "command": command}
+    # kernels the runs did not reach keep their earlier figures, each with the command that measured it
+    if TRAFFIC_FILE.exists():
+        try:
+            old = json.loads(TRAFFIC_FILE.read_text())
+        except ValueError:
+            old = {}
+        for k, v in old.get("kernels", {}).items():
+            if k not in out["kernels"]:
+                out["kernels"][k] = dict(v, command=v.get("command", old.get("command", "")))
     TRAFFIC_FILE.parent.mkdir(exist_ok=True)
     TRAFFIC_FILE.write_text(json.dumps(out, indent=1) + "\n")
     print(json.dumps(out, indent=1))
@@ -691,6 +700,9 @@ def main():
     ap.add_argument("--skip-extra-configs", action="store_true",
                     help="skip the reference-forms kidney leg, the 6144-tet block and the 2.01M-tet cutting block")
     ap.add_argument("--cutting-steps", type=int, default=8)
+    ap.add_argument("--plain-only", action="store_true",
+                    help="time the headline kidney-250k leg alone and print its line (for kernel traces of the "
+                         "graph-replayed step)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuners and run their usual picks (for counter runs, so that every dispatch of a "
                          "kernel has the launch shape the timed run uses)")
@@ -764,6 +776,9 @@ def main():
                      "algorithmic_bytes_per_launch": b_alg * g["units"] / max(g["launches"], 1)},
     }
     tissue = g.pop("tissue")
+    if args.plain_only:
+        print(json.dumps(line), flush=True)
+        return
     rb = run_readback(tissue)
     line["readback"] = {
         "value": rb["bytes"] / (rb["pinned_us"] * 1e-6) / 1e9, "unit": "GB/s", "us_per_observation": rb["pinned_us"],
