@@ -163,8 +163,8 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P,
                     et[2 * k + 1] = S.ld(j);
                     if (write) lte[k] = lam;
                 }
-                if (edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, lam, out) && write)
-                    s.lam_e[k] = out;
+                const bool ok = edge_project<false>(S, s.prev, i, j, s.rec->rest_len[axis], at, 1.f, 0.f, lam, out);
+                if (write && (ok || !read)) s.lam_e[k] = ok ? out : 0.f;  // 0: the substep's reset
             }
             __syncthreads();
         }
@@ -206,11 +206,11 @@ __device__ void substep_forward(const Inst& s, const Store& S, const AdjStep& P,
                 }
                 f3 rk[3];
                 for (int q = 0; q < 3; q++) rk[q] = kuhn_rest_edge(type, q + 1, s.rec->rest_len);
-                if (tet_project<1, false, true>(S, s.prev, id, s.rec->inv_rest[type], in.ad, in.ah, lamD, lamH, 1.f,
-                                                0.f, 0.f, rk) &&
-                    write) {
-                    s.lam_t[k] = lamD;
-                    s.lam_t[T + k] = lamH;
+                const bool ok = tet_project<1, false, true>(S, s.prev, id, s.rec->inv_rest[type], in.ad, in.ah, lamD,
+                                                            lamH, 1.f, 0.f, 0.f, rk);
+                if (write && (ok || !read)) {  // 0: the substep's reset
+                    s.lam_t[k] = ok ? lamD : 0.f;
+                    s.lam_t[T + k] = ok ? lamH : 0.f;
                 }
             }
             __syncthreads();

@@ -93,9 +93,9 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                         int axis = (int)((d >> 1) & 3u);
                         int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? nx : nxy));
                         float lam = read ? lam_edge[k] : 0.f, out;
-                        if (edge_project<DAMPED>(S, prev, i, j, I.rest_len[axis], at, P.omega, betaGammaH, lam, out) &&
-                            write)
-                            lam_edge[k] = out;
+                        const bool ok =
+                            edge_project<DAMPED>(S, prev, i, j, I.rest_len[axis], at, P.omega, betaGammaH, lam, out);
+                        if (write && (ok || !read)) lam_edge[k] = ok ? out : 0.f;  // 0: the substep's reset
                     }
                     __syncthreads();
                 }
@@ -132,11 +132,11 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                     if (PRECISE)
                         for (int q = 0; q < 3; q++) rk[q] = kuhn_rest_edge(type, q + 1, I.rest_len);
                     float lamD = t.lamD, lamH = t.lamH;
-                    if (tet_project<MODE, DAMPED, PRECISE>(S, prev, id, I.inv_rest[type], t.ad, t.ah, lamD, lamH,
-                                                           P.omega, P.dOverH, P.bOverH, rk) &&
-                        write) {
-                        lam_tet[k] = lamD;
-                        lam_tet[T + k] = lamH;
+                    const bool ok = tet_project<MODE, DAMPED, PRECISE>(S, prev, id, I.inv_rest[type], t.ad, t.ah,
+                                                                       lamD, lamH, P.omega, P.dOverH, P.bOverH, rk);
+                    if (write && (ok || !read)) {  // 0: the substep's reset
+                        lam_tet[k] = ok ? lamD : 0.f;
+                        lam_tet[T + k] = ok ? lamH : 0.f;
                     }
                 }
                 __syncthreads();

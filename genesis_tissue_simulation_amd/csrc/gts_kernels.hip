@@ -203,9 +203,9 @@ __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fal
     }
     int i = (int)(d >> 4), j = i + axis_stride(L, axis);
     float lam = READ_LAMBDA ? lambda[e] : 0.f, out;
-    if (edge_project<DAMPED>(GlobalNodes{pos}, prev, i, j, restLen, alphaTilde, omega, betaGammaH, lam, out) &&
-        WRITE_LAMBDA)
-        lambda[e] = out;
+    const bool ok = edge_project<DAMPED>(GlobalNodes{pos}, prev, i, j, restLen, alphaTilde, omega, betaGammaH, lam, out);
+    // a row that writes nothing in the first iteration keeps the substep's reset multiplier, 0 (xpbd_solver.cu:3534)
+    if (WRITE_LAMBDA && (ok || !READ_LAMBDA)) lambda[e] = ok ? out : 0.f;
 }
 
 // k_solve_tet (xpbd_solver.cu:749-829) over one colour. MODE 0 keeps the hydrostatic row alone; 1 and 2 are the
@@ -226,11 +226,13 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
     if (PRECISE)
         for (int c = 0; c < 3; c++) rk[c] = kuhn_rest_edge(type, c + 1, L.rest_len);
     float lamD = READ_LAMBDA ? lambda[e] : 0.f, lamH = READ_LAMBDA ? lambda[T + e] : 0.f;
-    if (tet_project<MODE, DAMPED, PRECISE>(GlobalNodes{pos}, prev, id, L.inv_rest[type], alpha[e], alpha[T + e], lamD,
-                                           lamH, omega, dampingOverH, bulkDampingOverH, rk) &&
-        WRITE_LAMBDA) {
-        lambda[e] = lamD;
-        lambda[T + e] = lamH;
+    const bool ok = tet_project<MODE, DAMPED, PRECISE>(GlobalNodes{pos}, prev, id, L.inv_rest[type], alpha[e],
+                                                       alpha[T + e], lamD, lamH, omega, dampingOverH,
+                                                       bulkDampingOverH, rk);
+    // a row that writes nothing in the first iteration keeps the substep's reset multipliers, 0 (xpbd_solver.cu:3536)
+    if (WRITE_LAMBDA && (ok || !READ_LAMBDA)) {
+        lambda[e] = ok ? lamD : 0.f;
+        lambda[T + e] = ok ? lamH : 0.f;
     }
 }
 

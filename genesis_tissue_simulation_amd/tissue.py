@@ -552,6 +552,17 @@ class Tissue:
         edge[self.edge_slot_orig] = self.edge_desc[:self.E] & 1
         return tet[:self.T0].cpu().numpy(), edge[:self.E0].cpu().numpy()
 
+    def multipliers(self):
+        """(edge multipliers, tet deviatoric, tet hydrostatic) after the last substep, in the orders liveness()
+        uses; removed entries read 0."""
+        edge = torch.zeros(max(self.E0, 1), dtype=torch.float32, device=self.device)
+        edge[self.edge_slot_orig] = self.edge_lambda[:self.E]
+        dev = torch.zeros(max(self.T0, 1), dtype=torch.float32, device=self.device)
+        hyd = torch.zeros_like(dev)
+        dev[self.tet_slot_gen] = self.tet_lambda[:self.T]
+        hyd[self.tet_slot_gen] = self.tet_lambda[self.T:2 * self.T]
+        return edge[:self.E0].cpu().numpy(), dev[:self.T0].cpu().numpy(), hyd[:self.T0].cpu().numpy()
+
     def compact(self, apply=False, readback=True):
         """Stable remap of the live topology: (tet_remap in colour order, edge_remap, node_remap, counts), relative
         to the current lists. With `apply`, the dead tets and edges are removed: the live ones move to their remapped

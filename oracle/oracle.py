@@ -105,6 +105,7 @@ def lib():
             getattr(L, f"oracle_get_tear_exempt_{suf}").argtypes = [P, P, P, P]
             getattr(L, f"oracle_get_feasible_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_get_feasible_{suf}").restype = I
+            getattr(L, f"oracle_get_multipliers_{suf}").argtypes = [P, P, P, P]
         L.oracle_threads.argtypes = [I]
         L.oracle_threads.restype = I
         _lib = L
@@ -315,6 +316,15 @@ class Oracle:
         needed = np.zeros(self.N, dtype=np.int32)
         self._call("get_feasible", self.h, _ptr(needed))
         return needed
+
+    def multipliers(self):
+        """(edge multipliers [E], tet deviatoric [T], tet hydrostatic [T]) after the last substep's last
+        iteration; tets in generation order."""
+        edge = np.zeros(self.E, dtype=self.real)
+        dev = np.zeros(self.T, dtype=self.real)
+        hyd = np.zeros(self.T, dtype=self.real)
+        self._call("get_multipliers", self.h, _ptr(edge), _ptr(dev), _ptr(hyd))
+        return edge, dev, hyd
 
     def liveness(self):
         tet = np.zeros(self.T, dtype=np.int32)

@@ -795,6 +795,8 @@ template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallba
     if (edgeProject(s.pos[i], s.pos[j], s.prev[i], s.prev[j], s.invMass[i], s.invMass[j], s.erest[e], alphaTilde,
                     s.betaGammaH, (R)s.d.omega, lam))
         s.elambda[e] = lam;
+    else if (!useLambda)
+        s.elambda[e] = 0;  // the substep's reset (xpbd_solver.cu:3534): a row that writes nothing keeps lambda = 0
 }
 
 // k_perp_damp_edges (xpbd_solver.cu:890-911) on one edge: the part of the endpoints' relative velocity across the
@@ -1030,8 +1032,13 @@ template <class R> void solveTet(Tissue<R>& s, int e, R dampingOverH, R bulkDamp
     V3<R> br[3] = {mk<R>(B[0][0], B[0][1], B[0][2]), mk<R>(B[1][0], B[1][1], B[1][2]), mk<R>(B[2][0], B[2][1], B[2][2])};
     R lamD = useLambda ? s.lamDev[e] : (R)0, lamH = useLambda ? s.lamHyd[e] : (R)0;
     if (!tetProject<R>(s.d.tet_mode, p, pv, w, br, s.alphaDev[e], s.alphaHyd[e], lamD, lamH, dampingOverH,
-                       bulkDampingOverH, (R)s.d.omega, out, s.d.precise_strain ? s.typeRk[s.ttype[e]] : nullptr))
+                       bulkDampingOverH, (R)s.d.omega, out, s.d.precise_strain ? s.typeRk[s.ttype[e]] : nullptr)) {
+        if (!useLambda) {  // the substep's reset (xpbd_solver.cu:3536-3539): a row that writes nothing keeps 0
+            s.lamDev[e] = 0;
+            s.lamHyd[e] = 0;
+        }
         return;
+    }
     s.lamDev[e] = lamD;
     s.lamHyd[e] = lamH;
     for (int k = 0; k < 4; k++) s.pos[id[k]] = out[k];
@@ -2294,6 +2301,13 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         int n = 0;                                                                                                    \
         for (int i = 0; i < s->N; i++) { if (needed) needed[i] = s->needed[i]; n += s->needed[i] ? 1 : 0; }           \
         return n;                                                                                                     \
+    }                                                                                                                 \
+    /* the edge multipliers [E] and the tets' deviatoric and hydrostatic multipliers [T] after the last substep */     \
+    extern "C" void oracle_get_multipliers_##SUF(void* h, R* edge, R* dev, R* hyd) {                                  \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        std::memcpy(edge, s->elambda.data(), sizeof(R) * s->E);                                                       \
+        std::memcpy(dev, s->lamDev.data(), sizeof(R) * s->T);                                                         \
+        std::memcpy(hyd, s->lamHyd.data(), sizeof(R) * s->T);                                                         \
     }                                                                                                                 \
     extern "C" void oracle_compact_##SUF(void* h, int* tetRemap, int* edgeRemap, int* nodeRemap, int* counts) {       \
         compact(*(Tissue<R>*)h, tetRemap, edgeRemap, nodeRemap, counts);                                              \

@@ -118,6 +118,34 @@ def test_tet_modes_and_iterations(mode, iterations, precise):
           f"vs FP64 {rel_err(t.positions(), o64.state()[0]):.3e}")
     np.testing.assert_array_equal(t.positions(), o.state()[0])
     np.testing.assert_array_equal(t.velocities(), o.state()[1])
+    if iterations > 1:  # with one iteration nothing reads the multipliers and the kernels store none
+        for a, b in zip(t.multipliers(), o.multipliers()):
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("iterations", [2, 3])
+def test_multipliers_of_rows_that_cannot_move(iterations):
+    """Rows whose every node is pinned project nothing; the substep's reset leaves their multipliers at 0 whatever
+    the buffers held before (xpbd_solver.cu:3534-3539), and every other multiplier matches the oracle bit for bit."""
+    import torch
+    w = block_6144()
+    grid = w.grid
+    pinned = grid.pinned.astype(bool)
+    o, t = make_pair(w, params=w.params.with_(iterations=iterations))
+    with torch.no_grad():
+        t.edge_lambda.fill_(7.0)
+        t.tet_lambda.fill_(-3.0)
+    advance([o, t], w, 1, cut=False)
+    ge, gd, gh = t.multipliers()
+    oe, od, oh = o.multipliers()
+    np.testing.assert_array_equal(ge, oe)
+    np.testing.assert_array_equal(gd, od)
+    np.testing.assert_array_equal(gh, oh)
+    e_top = o.topology()
+    both_pinned = pinned[e_top["edge_i"]] & pinned[e_top["edge_j"]]
+    assert both_pinned.any(), "the block has no edge between two pinned nodes"
+    assert (ge[both_pinned] == 0).all()
+    np.testing.assert_array_equal(t.positions(), o.state()[0])
 
 
 def test_fallback_edges_and_damping():
