@@ -16,7 +16,7 @@ I = ct.c_int
 P = ct.c_void_p
 V3 = F * 3
 
-GTS_ABI_VERSION = 1
+GTS_ABI_VERSION = 2
 GTS_MAX_COLOURS = 64
 GTS_STEP_CONST_FLOATS = 184
 STATUS = {0: "ok", -1000: "not ready", -1002: "bad dt", -1003: "colour overflow", -1004: "bad argument",
@@ -31,7 +31,8 @@ class GtsTopology(ct.Structure):
     _fields_ = [("edge_desc", P), ("edge_gen_index", P), ("edge_colour_offsets", P), ("tet_desc", P),
                 ("tet_gen_index", P), ("tet_colour_offsets", P), ("type_inv_rest", ct.c_double * 54),
                 ("type_volume", ct.c_double * 6), ("axis_rest_length", F * 3), ("n_edges", I), ("n_tets", I),
-                ("n_edge_colours", I), ("n_tet_colours", I)]
+                ("n_edge_colours", I), ("n_tet_colours", I), ("bend_desc", P), ("bend_gen_index", P),
+                ("bend_colour_offsets", P), ("n_bends", I), ("n_bend_colours", I)]
 
 
 class GtsTissueView(ct.Structure):
@@ -46,7 +47,8 @@ class GtsTissueView(ct.Structure):
                 ("counters", P), ("scratch", P), ("reaction", P), ("step_consts", P), ("step_const_capacity", I),
                 ("use_graph", I), ("colour_block", I), ("fiber_lambda", P), ("feasible_target", P),
                 ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P), ("grasp_ring", P),
-                ("edge_exempt", P), ("nbr_init", P), ("xcd_order", I)]
+                ("edge_exempt", P), ("nbr_init", P), ("xcd_order", I), ("n_bends", I), ("n_bend_colours", I),
+                ("bend_colour_offsets", P), ("bend_desc", P), ("bend_lambda", P)]
 
 
 class GtsStepParams(ct.Structure):
@@ -54,7 +56,8 @@ class GtsStepParams(ct.Structure):
                 ("young", F), ("beta_s", F), ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F),
                 ("damping", F), ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F),
                 ("contact_radius", F), ("jaw_gap_closed", F), ("contact_compliance", F), ("precise_strain", I),
-                ("guard_min_volume", F), ("guard_passes", I), ("tensile_strength", F)]
+                ("guard_min_volume", F), ("guard_passes", I), ("tensile_strength", F), ("bend_kb", F),
+                ("beta_b", F)]
 
 
 class GtsToolState(ct.Structure):
@@ -123,6 +126,7 @@ SIGNATURES = {
     "gts_build_topology": ((P, P), I),
     "gts_iso_pow": ((F,), F),
     "gts_log": ((F,), F),
+    "gts_acos": ((F,), F),
     "gts_scratch_floats": ((I, I, I, I), ct.c_longlong),
     "gts_init_state": ((P, P), I),
     "gts_set_gravity": ((P, F, F, F, P), I),

@@ -32,7 +32,7 @@ class Batch:
         self.params = params
         self.gravity = tuple(float(g) for g in gravity)
         dev = self.device
-        topos = [build_topology(inst[0]) for inst in instances]
+        topos = [build_topology(inst[0], bends=False) for inst in instances]  # the batched step has no bending
         Ns = [len(inst[0].positions) for inst in instances]
         Es = [len(t["edge_desc"]) for t in topos]
         Ts = [len(t["tet_desc"]) for t in topos]

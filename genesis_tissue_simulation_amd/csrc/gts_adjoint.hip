@@ -626,8 +626,8 @@ int plan_step(const GtsStepParams* p, int nsub, int ckpt, AdjStep& P) {
     if (!p || nsub <= 0 || ckpt <= 0) return GTS_ERR_BAD_ARG;
     if (p->tet_mode != GTS_TET_NEOHOOKEAN || p->iterations < 1 || p->strain_damping != 0.f ||
         p->bulk_damping != 0.f || p->beta_s != 0.f || p->omega != 1.f || p->damping != 0.f || p->rayleigh != 0.f ||
-        p->guard_passes != 0)
-        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped, unguarded Neo-Hookean substep
+        p->guard_passes != 0 || p->bend_kb != 0.f)
+        return GTS_ERR_BAD_ARG;  // the adjoint covers the undamped, unguarded Neo-Hookean substep without bending
     int N = p->substeps > 0 ? p->substeps : 1;
     P.h = p->dt / (float)N;
     if (!(P.h > 0.f) || !std::isfinite(P.h)) return GTS_ERR_BAD_DT;

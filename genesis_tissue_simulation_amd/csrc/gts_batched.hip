@@ -318,7 +318,8 @@ extern "C" int gts_batched_plan(const double* cost, int n, int n_workgroups, int
 extern "C" int gts_batched_step(const GtsBatchView* v, const GtsStepParams* p, void* stream) {
     if (!v || !p || v->n_workgroups <= 0 || v->block_size <= 0 || v->block_size > 1024) return GTS_ERR_BAD_ARG;
     if (p->tet_mode < 0 || p->tet_mode > 2) return GTS_ERR_BAD_ARG;
-    if (p->guard_passes != 0) return GTS_ERR_BAD_ARG;  // the tissue guard runs on the single-tissue step only
+    // the tissue guard and bending run on the single-tissue step only
+    if (p->guard_passes != 0 || p->bend_kb != 0.f) return GTS_ERR_BAD_ARG;
     if ((long long)v->lds_nodes * 16 > (long long)gts_batched_lds_bytes()) return GTS_ERR_BAD_ARG;
     if (v->substeps_per_launch < 0) return GTS_ERR_BAD_ARG;
     if (v->rot && (!v->rot_prev || !v->nbr || !v->active || !v->pinned || !v->node_base || v->total_nodes <= 0 ||

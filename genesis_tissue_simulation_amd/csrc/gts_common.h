@@ -68,6 +68,38 @@ GTS_HD float own_log(float x) {
     return fe * 0.693145752f + (fe * 1.42860677e-06f + lm);
 }
 
+// arccos for the bend angle (the reference calls acosf, xpbd/xpbd_solver.cu:553), from the Maclaurin series of
+// arcsin, asin x = x + sum_n (2n)! / (4^n (n!)^2 (2n + 1)) x^(2n+1), summed to n = 11 for |x| <= 1/2:
+//   |c| <= 1/2:  acos c = pi/2 - asin c;
+//   c > 1/2:     acos c = 2 asin sqrt((1 - c)/2);   c < -1/2: acos c = pi - 2 asin sqrt((1 + c)/2)
+// (1 -+ c and the halving are exact there). pi and pi/2 enter as a float head plus the float of their residual. A
+// fixed IEEE sequence the CPU checker repeats; within 1.1 ulp of the exact arccos on [-1, 1].
+GTS_HD float own_acos(float c) {
+    const float a = fabsf(c);
+    const bool mid = a <= 0.5f;
+    const float x = mid ? c : sqrtf((1.f - a) * 0.5f);
+    const float x2 = mid ? c * c : (1.f - a) * 0.5f;
+    float p = 0.007312526f;           // n = 11: 88179/12058624
+    p = 0.008390335f + x2 * p;        // 46189/5505024
+    p = 0.009761609f + x2 * p;        // 12155/1245184
+    p = 0.011551801f + x2 * p;        // 6435/557056
+    p = 0.013964844f + x2 * p;        // 143/10240
+    p = 0.017352764f + x2 * p;        // 231/13312
+    p = 0.022372158f + x2 * p;        // 63/2816
+    p = 0.030381944f + x2 * p;        // 35/1152
+    p = 0.044642858f + x2 * p;        // 5/112
+    p = 0.075f + x2 * p;              // 3/40
+    p = 0.16666667f + x2 * p;         // n = 1: 1/6
+    const float tail = x * x2 * p;    // asin x - x
+    if (mid) return 1.57079637f - (x + (tail - -4.37113883e-08f));  // pi/2 = 1.57079637f + -4.37113883e-08f
+    const float twice = 2.f * (x + tail);
+    return c > 0.f ? twice : 3.14159274f - (twice - -8.74227766e-08f);  // pi = 3.14159274f + -8.74227766e-08f
+}
+
+// Bend slot pair p = 0..14 of a node: slots (a, b), a < b, in lexicographic order (grid.py:257-260).
+GTS_HD int bend_slot_a(int p) { return (int)((0x11B491248000ull >> (3 * p)) & 7ull); }
+GTS_HD int bend_slot_b(int p) { return (int)((0x16CB1D8D58D1ull >> (3 * p)) & 7ull); }
+
 // ---------------------------------------------------------------------------------------------------------------
 // Precise strain invariants (no reference counterpart; selected by GtsStepParams.precise_strain).
 //

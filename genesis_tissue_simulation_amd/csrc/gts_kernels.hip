@@ -208,6 +208,29 @@ __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fal
     if (WRITE_LAMBDA && (ok || !READ_LAMBDA)) lambda[e] = ok ? out : 0.f;
 }
 
+// k_solve_bending (xpbd_solver.cu:534-573) over one colour of bend elements. An element is live while its apex keeps
+// the pair's bit in bend_alive (the refresh's bendActive, :446; SeverLink clears it); its arms are the lattice
+// neighbours in the pair's two slots and its rest angle pi for opposite slots, else pi/2 (grid.py:267-268).
+template <bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED>
+__global__ void k_bends(int off, int cnt, Lattice L, float alphaB, float omega, float betaGammaH,
+                        const uint32_t* __restrict__ bend_desc, const uint32_t* __restrict__ bend_alive,
+                        float* __restrict__ lambda, float4* __restrict__ pos, const float4* __restrict__ prev) {
+    int q = colour_item(L);
+    if (q >= cnt) return;
+    int e = off + q;
+    uint32_t d = bend_desc[e];
+    int i = (int)(d >> 4), pair = (int)(d & 15u);
+    if (!((bend_alive[i] >> pair) & 1u)) return;
+    const int sa = bend_slot_a(pair), sb = bend_slot_b(pair);
+    const int da = axis_stride(L, sa >> 1), db = axis_stride(L, sb >> 1);
+    const int j = i + ((sa & 1) ? da : -da), k = i + ((sb & 1) ? db : -db);
+    const float theta0 = (sa >> 1) == (sb >> 1) ? 3.14159274f : 1.57079637f;
+    float lam = READ_LAMBDA ? lambda[e] : 0.f, out;
+    const bool ok = bend_project<DAMPED>(GlobalNodes{pos}, prev, i, j, k, theta0, alphaB, omega, betaGammaH, lam, out);
+    // a row that writes nothing in the first iteration keeps the substep's reset multiplier, 0 (xpbd_solver.cu:3535)
+    if (WRITE_LAMBDA && (ok || !READ_LAMBDA)) lambda[e] = ok ? out : 0.f;
+}
+
 // k_solve_tet (xpbd_solver.cu:749-829) over one colour. MODE 0 keeps the hydrostatic row alone; 1 and 2 are the
 // coupled 2x2 block, legacy and guarded.
 template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED, bool PRECISE>
@@ -1315,6 +1338,28 @@ void launch_edge_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s,
     else launch_edge_colour_d<false>(v, L, s, off, cnt, it, iters, fallback, at, fb, omega, bgh);
 }
 
+template <bool DAMPED>
+void launch_bend_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
+                          float alphaB, float omega, float bgh) {
+    const int cb = colour_block(v);
+    dim3 g(grid_for(cnt, cb)), b(cb);
+    const bool read = it > 0, write = iters > 1;
+#define GTS_BEND(R, W)                                                                                          \
+    k_bends<R, W, DAMPED><<<g, b, 0, s>>>(off, cnt, L, alphaB, omega, bgh, v->bend_desc, v->bend_alive,       \
+                                          v->bend_lambda, (float4*)v->pos, (const float4*)v->prev)
+    if (read && write) GTS_BEND(true, true);
+    else if (read) GTS_BEND(true, false);
+    else if (write) GTS_BEND(false, true);
+    else GTS_BEND(false, false);
+#undef GTS_BEND
+}
+
+void launch_bend_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
+                        float alphaB, float omega, float bgh) {
+    if (bgh != 0.f) launch_bend_colour_d<true>(v, L, s, off, cnt, it, iters, alphaB, omega, bgh);
+    else launch_bend_colour_d<false>(v, L, s, off, cnt, it, iters, alphaB, omega, bgh);
+}
+
 // The ordered emission of the cut points of the edges the last detection kernel recorded (records at v->scratch).
 void emit_records(const GtsTissueView* v, const Lattice& L, hipStream_t s, f3 ncut, int useNormals, float toolD) {
     int Eg = v->n_grid_edges;
@@ -1452,6 +1497,9 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     // the tissue guard: tensile sweeps per tet sweep, their strength, the safeguard's minimum J
     int guardPasses;
     float strength, guardMin;
+    // bending: compliance 1/(kb h^2) and its Kelvin-Voigt factor (xpbd_begin_frame, xpbd_solver.cu:2496-2499)
+    bool bend;
+    float alphaB, betaGammaHB;
 };
 
 // cut_ribbon_tick without the pressure blade (cut.cu:1303-1431) after substep n: CCD of the edges moving from the
@@ -1523,6 +1571,16 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
         P.strength = p->tensile_strength;
         P.guardMin = p->guard_min_volume;
     }
+    // g_doBendF = B > 0 && kb > 0 (xpbd_solver.cu:2496)
+    if (!(p->bend_kb >= 0.f) || !(p->beta_b >= 0.f)) return GTS_ERR_BAD_ARG;
+    P.bend = p->bend_kb > 0.f && v->n_bends > 0;
+    if (P.bend) {
+        if (!v->bend_desc || !v->bend_lambda || !v->bend_colour_offsets || v->n_bend_colours < 0 ||
+            v->n_bend_colours > GTS_MAX_COLOURS)
+            return GTS_ERR_BAD_ARG;
+        P.alphaB = 1.f / (p->bend_kb * h * h);
+        P.betaGammaHB = P.alphaB * p->beta_b * h;
+    }
     return GTS_OK;
 }
 
@@ -1581,6 +1639,12 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
                         int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
                         if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, passes, P.fallback, P.at, P.fb, P.omega,
                                                         P.betaGammaH);
+                    }
+                if (P.bend)  // then the bending angle constraints (xpbd_solver.cu:3590-3596)
+                    for (int c = 0; c < v->n_bend_colours; c++) {
+                        int off = v->bend_colour_offsets[c], cnt = v->bend_colour_offsets[c + 1] - off;
+                        if (cnt > 0) launch_bend_colour(v, L, s, off, cnt, it, passes, P.alphaB, P.omega,
+                                                        P.betaGammaHB);
                     }
                 if (P.strain)
                     for (int c = 0; c < v->n_tet_colours; c++) {
@@ -1695,7 +1759,8 @@ hipStream_t g_capture = nullptr;
 // and a host array freed and reallocated at the same address must not replay a graph built for other contents).
 std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
     const size_t ne = (size_t)v->n_edge_colours + 1, nt = (size_t)v->n_tet_colours + 1;
-    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) + (ne + nt) * sizeof(int32_t));
+    const size_t nb = P.bend ? (size_t)v->n_bend_colours + 1 : 0;
+    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) + (ne + nt + nb) * sizeof(int32_t));
     unsigned char* d = k.data();
     std::memcpy(d, v, sizeof(GtsTissueView));
     d += sizeof(GtsTissueView);
@@ -1704,6 +1769,8 @@ std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) 
     std::memcpy(d, v->edge_colour_offsets, ne * sizeof(int32_t));
     d += ne * sizeof(int32_t);
     std::memcpy(d, v->tet_colour_offsets, nt * sizeof(int32_t));
+    d += nt * sizeof(int32_t);
+    if (nb) std::memcpy(d, v->bend_colour_offsets, nb * sizeof(int32_t));
     return k;
 }
 
@@ -2176,6 +2243,7 @@ extern "C" int gts_apply_compaction(const GtsTissueView* v, const int32_t* tet_r
 
 extern "C" float gts_iso_pow(float J) { return iso_pow(J); }
 extern "C" float gts_log(float x) { return own_log(x); }
+extern "C" float gts_acos(float c) { return own_acos(c); }
 
 namespace {
 // Streaming copy of n float4: the known-size reference for HBM byte counters and the achievable copy bandwidth.

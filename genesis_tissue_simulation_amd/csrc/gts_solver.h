@@ -57,6 +57,46 @@ __device__ __forceinline__ bool edge_project(const Store& S, const float4* __res
     return true;
 }
 
+// k_solve_bending (xpbd/xpbd_solver.cu:534-573) on one bend element: apex i, arms j and k, rest angle theta0. The
+// constraint is C = theta - theta0 with theta the apex angle between x_i - x_j and x_i - x_k; the corner directions
+// are the unit arms' mutual perpendicular parts g_j = n_v - c n_u, g_k = n_u - c n_v, g_i = -(g_j + g_k) (|g_j| =
+// |g_k| = sin theta), the reference's choice of the angle gradient scaled by arm length times sin theta. Returns
+// false where the reference returns without writing (all three pinned, a degenerate arm, arms within 1e-4 of
+// collinear, a vanishing denominator).
+template <bool DAMPED, class Store>
+__device__ __forceinline__ bool bend_project(const Store& S, const float4* __restrict__ prev, int i, int j, int k,
+                                             float theta0, float alphaB, float omega, float betaGammaH, float lam,
+                                             float& lamOut) {
+    float4 pi = S.ld(i), pj = S.ld(j), pk = S.ld(k);
+    float wi = pi.w, wj = pj.w, wk = pk.w;
+    if (wi + wj + wk <= 0.f) return false;
+    f3 u = xyz4(pi) - xyz4(pj), v = xyz4(pi) - xyz4(pk);
+    float Lu = len3(u), Lv = len3(v);
+    if (Lu < 1e-9f || Lv < 1e-9f) return false;
+    f3 nu = u * (1.f / Lu), nv = v * (1.f / Lv);
+    float c = fmaxf(-1.f, fminf(1.f, dot3(nu, nv)));
+    float s = sqrtf(fmaxf(1.f - c * c, 0.f));
+    if (s < 1e-4f) return false;
+    float C = own_acos(c) - theta0;
+    f3 gj = nv - nu * c, gk = nu - nv * c;
+    f3 gi = (gj + gk) * -1.f;
+    float wg = wi * dot3(gi, gi) + wj * dot3(gj, gj) + wk * dot3(gk, gk);
+    float gradDx = 0.f;
+    if (DAMPED)
+        gradDx = dot3(gi, xyz4(pi) - xyz4(prev[i])) + dot3(gj, xyz4(pj) - xyz4(prev[j])) +
+                 dot3(gk, xyz4(pk) - xyz4(prev[k]));
+    float denom = (1.f + betaGammaH) * wg + alphaB;
+    if (denom < 1e-12f) return false;
+    float dl = (-C - alphaB * lam - betaGammaH * gradDx) / denom;
+    float applied = omega * dl;
+    lamOut = lam + applied;
+    f3 xi = xyz4(pi) + gi * (wi * applied), xj = xyz4(pj) + gj * (wj * applied), xk = xyz4(pk) + gk * (wk * applied);
+    S.st(i, make_float4(xi.x, xi.y, xi.z, wi));
+    S.st(j, make_float4(xj.x, xj.y, xj.z, wj));
+    S.st(k, make_float4(xk.x, xk.y, xk.z, wk));
+    return true;
+}
+
 // k_solve_tet (xpbd/xpbd_solver.cu:749-829) on one tet with nodes id[4]. MODE 0 keeps the hydrostatic row alone;
 // 1 and 2 are the coupled 2x2 block, legacy and guarded. lamD / lamH are read and updated. PRECISE evaluates J - 1,
 // log J, J^(-2/3) I1 - 3 and the deviatoric gradient from the displacement gradient (precise_strain, precise_dev)

@@ -5,7 +5,9 @@
 //                  (xpbd/xpbd_solver.cu:1445-1463);
 //   xpbd_init      greedy first-free edge colouring on a 64-bit per-node mask (:1213-1230);
 //   xpbd_build_tets Kuhn split of every all-active cell, Dm^-1 per type from the rest spacing in double, each rest
-//                  tet checked against its type, greedy 4-node tet colouring (:1600-1768).
+//                  tet checked against its type, greedy 4-node tet colouring (:1600-1768);
+//   xpbd_bind + xpbd_set_bending  one bend element per live bend pair (apex ascending, pair 0..14; :1466-1478),
+//                  greedy 3-node colouring (:1344-1356).
 // The device order is colour-major with generation order inside a colour; the descriptors are packed for the
 // kernels (see gts_kernels.hip).
 #include <cmath>
@@ -189,6 +191,40 @@ extern "C" int gts_build_topology(const GtsLattice* lat, GtsTopology* out) {
     out->n_tets = T;
     out->n_edge_colours = enc;
     out->n_tet_colours = tnc;
+
+    // ---- bend elements: the pairs of a node's live links, in the order of the lattice's bend records
+    out->n_bends = 0;
+    out->n_bend_colours = 0;
+    if (!out->bend_desc) return GTS_OK;
+    if (!out->bend_gen_index || !out->bend_colour_offsets) return GTS_ERR_BAD_ARG;
+    std::vector<int> bids, bdesc;
+    for (int p = 0; p < N; p++)
+        for (int q = 0; q < 15; q++) {
+            const int sa = bend_slot_a(q), sb = bend_slot_b(q);
+            const int j = lat->nbr[6 * p + sa], k = lat->nbr[6 * p + sb];
+            if (j < 0 || k < 0) continue;
+            // the kernels find the arms on the lattice: slot s links p to p +- the stride of axis s / 2
+            const int strideA = sa < 2 ? 1 : (sa < 4 ? nx : nx * ny), strideB = sb < 2 ? 1 : (sb < 4 ? nx : nx * ny);
+            if (j != p + ((sa & 1) ? strideA : -strideA) || k != p + ((sb & 1) ? strideB : -strideB))
+                return GTS_ERR_BAD_ARG;
+            bids.push_back(p);
+            bids.push_back(j);
+            bids.push_back(k);
+            bdesc.push_back((p << 4) | q);
+        }
+    const int B = (int)bdesc.size();
+    std::vector<int> bcol;
+    int bnc = 0;
+    rc = colour<3>(B, N, bids, bcol, bnc);
+    if (rc) return rc;
+    std::vector<int> border;
+    colour_major(bcol, bnc, border, out->bend_colour_offsets);
+    for (int k = 0; k < B; k++) {
+        out->bend_desc[k] = (uint32_t)bdesc[border[k]];
+        out->bend_gen_index[k] = border[k];
+    }
+    out->n_bends = B;
+    out->n_bend_colours = bnc;
     return GTS_OK;
 }
 

@@ -40,6 +40,9 @@ class SolverParams:
     guard_min_volume: float = 0.0  # minimum J of a live tet kept by the end-of-substep safeguard, 0..0.95
     guard_passes: int = 0          # tensile reinforcement passes after every tet sweep, 0..4; 0 turns the guard off
     tensile_strength: float = 80.0  # strength of the s^6 fibre energy (the reference's default)
+    # bending angle constraints (PhysicsDesc kb / betaB, xpbd_solver.cu:2496-2499, :534-573); kb <= 0 turns them off
+    bend_kb: float = 0.0            # bend stiffness (native force units per radian)
+    beta_b: float = 0.0             # bend Kelvin-Voigt damping stiffness
 
     def with_(self, **changes):
         return replace(self, **changes)

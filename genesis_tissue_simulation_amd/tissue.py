@@ -61,11 +61,13 @@ def step_params(params: SolverParams, dt):
                          float(p.bulk_damping), float(p.rayleigh), float(p.damping), int(bool(p.contact_enabled)),
                          float(p.mu_s), float(p.mu_d), float(p.jaw_thickness), float(p.contact_radius),
                          float(p.jaw_gap_closed), float(p.contact_compliance), int(bool(p.precise_strain)),
-                         float(p.guard_min_volume), int(p.guard_passes), float(p.tensile_strength))
+                         float(p.guard_min_volume), int(p.guard_passes), float(p.tensile_strength),
+                         float(p.bend_kb), float(p.beta_b))
 
 
-def build_topology(grid):
-    """Host topology of a lattice via gts_build_topology; arrays in device order plus per-type rest data."""
+def build_topology(grid, bends=True):
+    """Host topology of a lattice via gts_build_topology; arrays in device order plus per-type rest data. With
+    `bends`, also the bend elements and their colouring (the bending constraints need them)."""
     lib = load()
     nx, ny, nz = (d + 1 for d in grid.dims)
     N = nx * ny * nz
@@ -83,9 +85,19 @@ def build_topology(grid):
     toff = np.zeros(_lib.GTS_MAX_COLOURS + 1, dtype=np.int32)
     topo = GtsTopology(edge_desc.ctypes.data, edge_gen.ctypes.data, eoff.ctypes.data, tet_desc.ctypes.data,
                        tet_gen.ctypes.data, toff.ctypes.data)
+    if bends:
+        bend_desc = np.zeros(15 * N, dtype=np.uint32)
+        bend_gen = np.zeros(15 * N, dtype=np.int32)
+        boff = np.zeros(_lib.GTS_MAX_COLOURS + 1, dtype=np.int32)
+        topo.bend_desc, topo.bend_gen_index = bend_desc.ctypes.data, bend_gen.ctypes.data
+        topo.bend_colour_offsets = boff.ctypes.data
     check("gts_build_topology", lib.gts_build_topology(ct.byref(lat), ct.byref(topo)))
-    E, T = topo.n_edges, topo.n_tets
-    return dict(edge_desc=edge_desc[:E].copy(), edge_gen=edge_gen[:E].copy(),
+    E, T, B = topo.n_edges, topo.n_tets, topo.n_bends
+    extra = {}
+    if bends:
+        extra = dict(bend_desc=bend_desc[:B].copy(), bend_gen=bend_gen[:B].copy(),
+                     bend_offsets=boff[:topo.n_bend_colours + 1].copy())
+    return dict(**extra, edge_desc=edge_desc[:E].copy(), edge_gen=edge_gen[:E].copy(),
                 edge_offsets=eoff[:topo.n_edge_colours + 1].copy(), tet_desc=tet_desc[:T].copy(),
                 tet_gen=tet_gen[:T].copy(), tet_offsets=toff[:topo.n_tet_colours + 1].copy(),
                 type_inv_rest=np.array(topo.type_inv_rest, dtype=np.float64).reshape(6, 3, 3),
@@ -215,6 +227,14 @@ class Tissue:
         v.grasp_ring, v.edge_exempt = self.grasp_ring.data_ptr(), self.edge_exempt.data_ptr()
         v.nbr_init = self.nbr_init.data_ptr()
         v.xcd_order = int(bool(xcd_order))
+        # bend elements (solved while params.bend_kb > 0)
+        self.bend_offsets = np.ascontiguousarray(topo.get("bend_offsets", np.zeros(1)), dtype=np.int32)
+        self.B = len(topo.get("bend_desc", ()))
+        self.bend_desc = i32(topo["bend_desc"]) if self.B else torch.zeros(1, dtype=torch.int32, device=dev)
+        self.bend_lambda = torch.zeros(max(self.B, 1), dtype=torch.float32, device=dev)
+        v.n_bends, v.n_bend_colours = self.B, len(self.bend_offsets) - 1
+        v.bend_colour_offsets = self.bend_offsets.ctypes.data
+        v.bend_desc, v.bend_lambda = self.bend_desc.data_ptr(), self.bend_lambda.data_ptr()
         self._tool_from = None
         self._staging = None          # pinned host buffers of observation()
         self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define GTS_ABI_VERSION 1
+#define GTS_ABI_VERSION 2
 
 #define GTS_OK 0
 #define GTS_ERR_NOT_READY (-1000)
@@ -72,6 +72,13 @@ typedef struct GtsTopology {
     double type_volume[6];
     float axis_rest_length[3];  /* L0 of the lattice edges along x, y, z */
     int n_edges, n_tets, n_edge_colours, n_tet_colours;
+    /* Bend elements (xpbd_bind's live bend pairs, xpbd_solver.cu:1466-1478, coloured by xpbd_set_bending's greedy
+     * 3-node colouring, :1344-1356): descriptor apex node i << 4 | pair p (0..14, the slot pair of grid.py:257-260);
+     * the arms are the lattice neighbours in the pair's two slots. Null bend_desc: no bend elements are built. */
+    uint32_t* bend_desc;          /* capacity 15N */
+    int32_t* bend_gen_index;      /* colour slot -> generation index (apex ascending, pair 0..14) */
+    int32_t* bend_colour_offsets; /* capacity GTS_MAX_COLOURS + 1 */
+    int n_bends, n_bend_colours;
 } GtsTopology;
 
 /* Device view of one tissue. Pointers are device pointers unless marked host. */
@@ -141,6 +148,12 @@ typedef struct GtsTissueView {
     int xcd_order;                   /* 1: the colour kernels give each XCD one contiguous range of a colour (the
                                       * workgroups of an XCD meet the same lattice slab in every colour); 0: the
                                       * plain workgroup order. Results are the same either way. */
+    /* bend elements (GtsTopology.bend_desc), solved when GtsStepParams.bend_kb > 0; their liveness is the apex's
+     * bend_alive bit of the pair (the refresh's bendActive, xpbd_solver.cu:446) */
+    int n_bends, n_bend_colours;
+    const int32_t* bend_colour_offsets; /* host, [n_bend_colours + 1] */
+    const uint32_t* bend_desc;          /* [n_bends] */
+    float* bend_lambda;                 /* [n_bends] */
 } GtsTissueView;
 
 /* Ribbon ticks a released grasp-ring edge stays exempt from the tear law while still longer than 0.8 tau L
@@ -169,6 +182,10 @@ typedef struct GtsStepParams {
     float guard_min_volume;
     int guard_passes;
     float tensile_strength;
+    /* Bending (PhysicsDesc kb / betaB, xpbd_begin_frame, xpbd_solver.cu:2491-2499): with bend_kb > 0 and bend
+     * elements in the view, every iteration solves the bend colours after the edge colours (k_solve_bending,
+     * :534-573, :3590-3596) with compliance 1 / (kb h^2) and Kelvin-Voigt factor alpha~ beta_b h. */
+    float bend_kb, beta_b;
 } GtsStepParams;
 
 /* XpbdToolState (xpbd_solver.h:223-229) plus a blade capsule. */
@@ -229,13 +246,15 @@ int gts_abi_version(void);
  * GtsBatchInstance, GtsBatchView, GtsAdjointView, GtsGraspParams, GtsGraspState, GtsSurfaceView. */
 int gts_struct_sizes(int* out);
 
-/* Host: derive edges, Kuhn tets and both greedy colourings from the lattice. Needs no GPU. */
+/* Host: derive edges, Kuhn tets, bend elements and their greedy colourings from the lattice. Needs no GPU. */
 int gts_build_topology(const GtsLattice* lattice, GtsTopology* out);
 
 /* Host: J^(-2/3) as the tet kernel evaluates it (own cube-root sequence), for checks. */
 float gts_iso_pow(float J);
 /* Host: log x as the guarded tet kernel evaluates it (own atanh-series sequence), for checks. */
 float gts_log(float x);
+/* Host: arccos as the bend kernel evaluates it (own arcsin-series sequence), for checks. */
+float gts_acos(float c);
 /* Host: floats of device scratch a tissue view needs (detection records + scan partials). */
 long long gts_scratch_floats(int n_nodes, int n_edges, int n_tets, int n_grid_edges);
 

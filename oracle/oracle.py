@@ -26,7 +26,8 @@ class OracleDesc(ct.Structure):
                 ("strain_damping", F), ("bulk_damping", F), ("rayleigh", F), ("damping", F),
                 ("contact_enabled", I), ("mu_s", F), ("mu_d", F), ("jaw_thickness", F), ("contact_radius", F),
                 ("jaw_gap_closed", F), ("contact_compliance", F), ("tool_voxel", F), ("cut_point_capacity", I),
-                ("precise_strain", I), ("guard_min_volume", F), ("guard_passes", I), ("tensile_strength", F)]
+                ("precise_strain", I), ("guard_min_volume", F), ("guard_passes", I), ("tensile_strength", F),
+                ("bend_kb", F), ("beta_b", F)]
 
 
 class OracleTool(ct.Structure):
@@ -106,6 +107,12 @@ def lib():
             getattr(L, f"oracle_get_feasible_{suf}").argtypes = [P, P]
             getattr(L, f"oracle_get_feasible_{suf}").restype = I
             getattr(L, f"oracle_get_multipliers_{suf}").argtypes = [P, P, P, P]
+            getattr(L, f"oracle_bends_{suf}").argtypes = [P, P, P, P]
+        D = ct.c_double
+        L.oracle_bend_project_f64.argtypes = [P, P, P, D, D, D, D, P]
+        L.oracle_bend_project_f64.restype = I
+        L.oracle_acos_f32.argtypes = [F]
+        L.oracle_acos_f32.restype = F
         L.oracle_threads.argtypes = [I]
         L.oracle_threads.restype = I
         _lib = L
@@ -156,7 +163,8 @@ class Oracle:
                        int(p.contact_enabled), p.mu_s, p.mu_d, p.jaw_thickness, p.contact_radius,
                        p.jaw_gap_closed, p.contact_compliance, float(grid.cell_size[0]),
                        4 * len(grid.edges), int(bool(getattr(p, "precise_strain", False))),
-                       float(p.guard_min_volume), int(p.guard_passes), float(p.tensile_strength))
+                       float(p.guard_min_volume), int(p.guard_passes), float(p.tensile_strength),
+                       float(p.bend_kb), float(p.beta_b))
         self.desc = d
         rest = np.ascontiguousarray(grid.positions, dtype=np.float32)
         pos = rest if initial_positions is None else np.ascontiguousarray(initial_positions, dtype=np.float32)
@@ -190,6 +198,15 @@ class Oracle:
         self._call("topology", self.h, *[_ptr(a) for a in (ei, ej, eo, ecc, tids, ttype, towner, tcolor, tcc)])
         return dict(edge_i=ei, edge_j=ej, edge_owner=eo, edge_colour_counts=ecc, tet_ids=tids.reshape(-1, 4),
                     tet_type=ttype, tet_owner=towner.reshape(-1, 3), tet_colour=tcolor, tet_colour_counts=tcc)
+
+    def bends(self):
+        """Bend elements in colour order: the pair index 15 i + p of each, and the element count per colour."""
+        counts = np.zeros(2, dtype=np.int32)
+        self._call("bends", self.h, _ptr(counts), None, None)
+        src = np.zeros(max(int(counts[0]), 1), dtype=np.int32)
+        per_colour = np.zeros(max(int(counts[1]), 1), dtype=np.int32)
+        self._call("bends", self.h, _ptr(counts), _ptr(src), _ptr(per_colour))
+        return src[:counts[0]], per_colour[:counts[1]]
 
     def type_data(self):
         B, V = np.zeros(54), np.zeros(6)

@@ -15,7 +15,8 @@
 //   substep      xpbd_substep legacy branch order (:3460-3721): tool lerp, k_RefreshLiveness (:429-466),
 //                fallback edge support (:469-479), k_predict (:393-412), lambda reset, k_solve_edges (:484-524),
 //                k_solve_tet (:749-829), friction tool contact (contact/friction_contact.cuh:17-169),
-//                k_update_velocity / k_perp_damp_edges / k_apply_velscale (:871-919)
+//                k_update_velocity / k_perp_damp_edges / k_apply_velscale (:871-919); k_solve_bending (:534-573)
+//                after the edges while kb > 0, over the live bend pairs coloured by xpbd_set_bending (:1344-1356)
 //   cut          k_DetectCutQuad (cut.cu:421-474), MollerTrumbore (cut.cuh:162-185), MarkAndEmit/EmitCutPoint
 //                (cut.cu:186-254), k_DetectCutCCD (cut.cu:486-592), SeverLink/k_SeverLinks (cut.cu:719-746)
 //   compaction   build-side stable remap of live tets / edges / nodes (no reference counterpart)
@@ -127,6 +128,9 @@ struct OracleDesc {
     float guard_min_volume;    // minimum J of a live tet, 0 .. 0.95
     int guard_passes;          // tensile reinforcement passes per tet sweep, 0 .. 4 (0: no guard)
     float tensile_strength;    // fibre energy strength (the reference's default 80)
+    // bending (PhysicsDesc kb / betaB; xpbd_begin_frame, xpbd_solver.cu:2496-2499): kb <= 0 turns it off
+    float bend_kb;
+    float beta_b;
 };
 
 // Tool pose (XpbdToolState, xpbd_solver.h:223-229) plus the optional blade capsule; 27 floats.
@@ -509,6 +513,12 @@ template <class R> struct Tissue {
     std::vector<int> active, pinned, inside, nbr, nbr0;
     // bend pairs (grid.py BEND_DTYPE / BendPairGpuC): only (j, k, alive) are read on this path
     std::vector<int> bendJ, bendK, bendAlive;
+    // bend elements: the pairs alive at build, colour-ordered (xpbd_bind :1466-1478, xpbd_set_bending :1344-1378);
+    // bsrc is the pair's index 15 i + p, whose alive flag gates the element (bendActive, :446)
+    std::vector<int> bi, bj, bk, bsrc, bcolorOff, bcolorCnt;
+    std::vector<R> btheta, blambda;
+    R alphaB = 0, betaGammaHB = 0;
+    bool doBend = false;
     // edges, colour-ordered (xpbd_solver.cu:1230-1249)
     std::vector<int> ei, ej, eowner, eactive, ecolorOff, ecolorCnt, egenIndex;
     std::vector<R> erest, elambda;
@@ -719,6 +729,37 @@ template <class R> int buildTopology(Tissue<R>& s) {
     std::vector<int> slot(s.incOff.begin(), s.incOff.end() - 1);
     for (int t = 0; t < s.T; t++)
         for (int k = 0; k < 4; k++) s.incTets[slot[s.tids[4 * t + k]]++] = t;
+    // Bend elements: every pair alive at bind time, apex-major, with rest angle pi for opposite slots and pi/2
+    // otherwise (grid.py:262-268), coloured over their three nodes (xpbd_set_bending, :1344-1378).
+    std::vector<int> bids, bsrc;
+    std::vector<R> bth;
+    for (int i = 0; i < s.N; i++)
+        for (int p = 0; p < 15; p++) {
+            if (!s.bendAlive[15 * i + p]) continue;
+            bids.push_back(i);
+            bids.push_back(s.bendJ[15 * i + p]);
+            bids.push_back(s.bendK[15 * i + p]);
+            bsrc.push_back(15 * i + p);
+            bool opposite = BEND_PAIRS[p][0] / 2 == BEND_PAIRS[p][1] / 2;
+            bth.push_back((R)(opposite ? (float)3.14159265358979323846 : (float)(3.14159265358979323846 / 2)));
+        }
+    const int B = (int)bsrc.size();
+    std::vector<int> bcol;
+    int bnc = 0;
+    if (greedyColour<3>(B, s.N, bids, bcol, bnc)) return -1003;
+    s.bcolorCnt.assign(bnc, 0);
+    for (int e = 0; e < B; e++) s.bcolorCnt[bcol[e]]++;
+    s.bcolorOff.assign(bnc, 0);
+    for (int c = 1; c < bnc; c++) s.bcolorOff[c] = s.bcolorOff[c - 1] + s.bcolorCnt[c - 1];
+    s.bi.resize(B); s.bj.resize(B); s.bk.resize(B); s.bsrc.resize(B); s.btheta.resize(B);
+    std::vector<int> bcur = s.bcolorOff;
+    for (int e = 0; e < B; e++) {
+        int dst = bcur[bcol[e]]++;
+        s.bi[dst] = bids[3 * e]; s.bj[dst] = bids[3 * e + 1]; s.bk[dst] = bids[3 * e + 2];
+        s.bsrc[dst] = bsrc[e];
+        s.btheta[dst] = bth[e];
+    }
+    s.blambda.assign(B, 0);
     return 0;
 }
 
@@ -750,6 +791,9 @@ template <class R> int beginFrame(Tissue<R>& s, R dt) {
     s.betaGammaH = at * (R)s.d.beta_s * h;
     s.alphaTilde = at;
     s.h = h;
+    s.doBend = !s.bi.empty() && s.d.bend_kb > 0;  // g_doBendF (:2496-2499)
+    s.alphaB = s.doBend ? (R)1 / ((R)s.d.bend_kb * h * h) : (R)0;
+    s.betaGammaHB = s.alphaB * (R)s.d.beta_b * h;
     R invH2 = (R)1 / (h * h);
     for (int t = 0; t < s.T; t++) {
         s.alphaDev[t] = invH2 / s.kDev[t];
@@ -797,6 +841,73 @@ template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallba
         s.elambda[e] = lam;
     else if (!useLambda)
         s.elambda[e] = 0;  // the substep's reset (xpbd_solver.cu:3534): a row that writes nothing keeps lambda = 0
+}
+
+// arccos of the bend angle. The reference calls acosf (xpbd_solver.cu:553); the float oracle and the kernel share one
+// fixed sequence: the Maclaurin series of arcsin to x^23 on |x| <= 1/2, with acos c = pi/2 - asin c there and
+// acos c = 2 asin sqrt((1 - c)/2), pi - 2 asin sqrt((1 + c)/2) beyond, pi and pi/2 as a float head plus the float of
+// the residual. The double oracle uses the library arccos.
+inline float bendAcos(float c) {
+    const float a = std::fabs(c);
+    const bool mid = a <= 0.5f;
+    const float x = mid ? c : std::sqrt((1.f - a) * 0.5f);
+    const float x2 = mid ? c * c : (1.f - a) * 0.5f;
+    // (2n)! / (4^n (n!)^2 (2n + 1)), n = 11 down to 1
+    const float coef[11] = {0.007312526f, 0.008390335f, 0.009761609f, 0.011551801f, 0.013964844f, 0.017352764f,
+                            0.022372158f, 0.030381944f, 0.044642858f, 0.075f, 0.16666667f};
+    float p = coef[0];
+    for (int n = 1; n < 11; n++) p = coef[n] + x2 * p;
+    const float tail = x * x2 * p;
+    if (mid) return 1.57079637f - (x + (tail - -4.37113883e-08f));
+    const float twice = 2.f * (x + tail);
+    return c > 0.f ? twice : 3.14159274f - (twice - -8.74227766e-08f);
+}
+inline double bendAcos(double c) { return std::acos(c); }
+
+// k_solve_bending (xpbd_solver.cu:534-573) as a pure function of one element: apex xi, arms xj, xk (updated), their
+// substep starts, inverse masses, rest angle theta0, compliance alphaB, Kelvin-Voigt factor gamma, relaxation omega
+// and the multiplier lam (updated). C = theta - theta0; the corner directions are the reference's g_j = n_v - c n_u,
+// g_k = n_u - c n_v, g_i = -(g_j + g_k). Returns false where the reference returns without writing.
+template <class R>
+bool bendProject(V3<R>& xi, V3<R>& xj, V3<R>& xk, const V3<R>& pi, const V3<R>& pj, const V3<R>& pk, R wi, R wj, R wk,
+                 R theta0, R alphaB, R gamma, R omega, R& lam) {
+    if (wi + wj + wk <= 0) return false;
+    V3<R> u = xi - xj;
+    R Lu = len3(u);
+    V3<R> v = xi - xk;
+    R Lv = len3(v);
+    if (Lu < (R)1e-9f || Lv < (R)1e-9f) return false;
+    V3<R> nu = u * ((R)1 / Lu), nv = v * ((R)1 / Lv);
+    R c = fmax_<R>((R)-1, fmin_<R>((R)1, dot3(nu, nv)));
+    R sn = std::sqrt(fmax_<R>((R)1 - c * c, (R)0));
+    if (sn < (R)1e-4f) return false;
+    R C = bendAcos(c) - theta0;
+    V3<R> gj = nv - nu * c;
+    V3<R> gk = nu - nv * c;
+    V3<R> gi = (gj + gk) * (R)-1;
+    R wg = wi * dot3(gi, gi) + wj * dot3(gj, gj) + wk * dot3(gk, gk);
+    R gradDx = dot3(gi, xi - pi) + dot3(gj, xj - pj) + dot3(gk, xk - pk);
+    R denom = ((R)1 + gamma) * wg + alphaB;
+    if (denom < (R)1e-12f) return false;
+    R dl = (-C - alphaB * lam - gamma * gradDx) / denom;
+    R applied = omega * dl;
+    lam = lam + applied;
+    xi = xi + gi * (wi * applied);
+    xj = xj + gj * (wj * applied);
+    xk = xk + gk * (wk * applied);
+    return true;
+}
+
+// k_solve_bending on element e of the tissue, gated by its pair's alive flag.
+template <class R> void solveBend(Tissue<R>& s, int e, bool useLambda) {
+    if (!s.bendAlive[s.bsrc[e]]) return;
+    int i = s.bi[e], j = s.bj[e], k = s.bk[e];
+    R lam = useLambda ? s.blambda[e] : (R)0;
+    if (bendProject(s.pos[i], s.pos[j], s.pos[k], s.prev[i], s.prev[j], s.prev[k], s.invMass[i], s.invMass[j],
+                    s.invMass[k], s.btheta[e], s.alphaB, s.betaGammaHB, (R)s.d.omega, lam))
+        s.blambda[e] = lam;
+    else if (!useLambda)
+        s.blambda[e] = 0;  // the substep's reset (xpbd_solver.cu:3535)
 }
 
 // k_perp_damp_edges (xpbd_solver.cu:890-911) on one edge: the part of the endpoints' relative velocity across the
@@ -1477,7 +1588,7 @@ template <class R> void ccdGate(Tissue<R>& s, int c, const GraspFrame<R>& f, R c
     s.pos[c] = x;
 }
 
-// One substep of the legacy branch (xpbd_solver.cu:3460-3721, no bending): with the legacy contact model, K = 2
+// One substep of the legacy branch (xpbd_solver.cu:3460-3721): with the legacy contact model, K = 2
 // material passes while grasping, each followed by the grasp ring and core pins, then the break check and the CCD
 // gate; with the friction model, friction tool contact of lattice nodes (:3659-3665, friction_contact.cuh:143-169);
 // with the tissue guard, tensile reinforcement after every tet sweep and the minimum-volume safeguard before the
@@ -1539,6 +1650,12 @@ template <class R> int substep(Tissue<R>& s) {
 #pragma omp parallel for schedule(static) if (cnt > 4096)
                     for (int q = 0; q < cnt; q++)
                         solveEdge(s, off + q, s.alphaTilde, fallback, fallbackInvEh2, useLambda);
+                }
+            if (s.doBend)  // then the bending angle constraints (:3590-3596)
+                for (size_t c = 0; c < s.bcolorCnt.size(); c++) {
+                    int off = s.bcolorOff[c], cnt = s.bcolorCnt[c];
+#pragma omp parallel for schedule(static) if (cnt > 4096)
+                    for (int q = 0; q < cnt; q++) solveBend(s, off + q, useLambda);
                 }
             if (strain)
                 for (size_t c = 0; c < s.tcolorCnt.size(); c++) {
@@ -2176,6 +2293,14 @@ template <class R> Tissue<R>* create(const OracleDesc* d, const float* rest, con
         out[0] = s->N; out[1] = s->E; out[2] = s->T;                                                                  \
         out[3] = (int)s->ecolorCnt.size(); out[4] = (int)s->tcolorCnt.size();                                         \
     }                                                                                                                 \
+    /* bend elements: count, colours, then per colour slot the pair index 15 i + p, and the colour counts */          \
+    extern "C" void oracle_bends_##SUF(void* h, int* counts, int* src, int* colorCnt) {                               \
+        Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
+        counts[0] = (int)s->bsrc.size(); counts[1] = (int)s->bcolorCnt.size();                                        \
+        if (src) std::memcpy(src, s->bsrc.data(), sizeof(int) * s->bsrc.size());                                      \
+        if (colorCnt)                                                                                                 \
+            for (size_t c = 0; c < s->bcolorCnt.size(); c++) colorCnt[c] = s->bcolorCnt[c];                           \
+    }                                                                                                                 \
     extern "C" void oracle_topology_##SUF(void* h, int* ei, int* ej, int* eowner, int* ecolorCnt, int* tids,          \
                                           int* ttype, int* towner, int* tcolor, int* tcolorCnt) {                     \
         Tissue<R>* s = (Tissue<R>*)h;                                                                                 \
@@ -2365,6 +2490,21 @@ extern "C" int oracle_edge_project_f64(double* x6, const double* p6, double wi, 
     x6[0] = xi.x; x6[1] = xi.y; x6[2] = xi.z; x6[3] = xj.x; x6[4] = xj.y; x6[5] = xj.z;
     return ok ? 1 : 0;
 }
+// One bend projection: x9 = (xi, xj, xk) updated, p9 their substep starts, w3 inverse masses, lam updated; returns 0
+// where it does not write.
+extern "C" int oracle_bend_project_f64(double* x9, const double* p9, const double* w3, double theta0, double alphaB,
+                                       double gamma, double omega, double* lam) {
+    V3<double> x[3], p[3];
+    for (int k = 0; k < 3; k++) {
+        x[k] = mk(x9[3 * k], x9[3 * k + 1], x9[3 * k + 2]);
+        p[k] = mk(p9[3 * k], p9[3 * k + 1], p9[3 * k + 2]);
+    }
+    bool ok = bendProject(x[0], x[1], x[2], p[0], p[1], p[2], w3[0], w3[1], w3[2], theta0, alphaB, gamma, omega, *lam);
+    for (int k = 0; k < 3; k++) { x9[3 * k] = x[k].x; x9[3 * k + 1] = x[k].y; x9[3 * k + 2] = x[k].z; }
+    return ok ? 1 : 0;
+}
+// The float oracle's arccos of the bend angle.
+extern "C" float oracle_acos_f32(float c) { return bendAcos(c); }
 // One edge's perpendicular velocity damping: v6 = (vi, vj) updated at positions x6.
 extern "C" void oracle_perp_damp_f64(double* v6, const double* x6, double wi, double wj, double hBeta) {
     V3<double> vi = mk(v6[0], v6[1], v6[2]), vj = mk(v6[3], v6[4], v6[5]);
