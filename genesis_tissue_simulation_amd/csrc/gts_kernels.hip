@@ -263,11 +263,12 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 // xpbd_solver.cu:3663-3665), then (VEL) k_update_velocity and, unless perpendicular edge damping must run between
 // them, k_apply_velscale (xpbd_solver.cu:871-919). Nodes are independent here, so one pass does all of it; with the
 // tissue guard, whose safeguard runs between contact and velocity, VEL is off and k_velocity follows the safeguard.
-template <bool CONTACT, bool SCALE, bool VEL = true>
+// PREDICT (with VEL) also runs the next substep's k_predict on the node, when nothing else runs between the two.
+template <bool CONTACT, bool SCALE, bool VEL = true, bool PREDICT = false>
 __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, uint32_t armed, float alpha, float muS, float muD,
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
-                               const float4* __restrict__ prev, float4* __restrict__ vel, float* reaction,
-                               int xcd_order) {
+                               float4* __restrict__ prev, float4* __restrict__ vel, float* reaction,
+                               int xcd_order, float h = 0.f, const float4* __restrict__ ext = nullptr) {
     int i = ordered_item(xcd_order);
     f3 force = mk3(0, 0, 0);
     float contacts = 0.f;
@@ -321,6 +322,16 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
         if (VEL) {
             f3 v = w > 0.f ? (x - xp) * invH : mk3(0, 0, 0);
             if (SCALE) v = w > 0.f ? v * velScale : mk3(0, 0, 0);
+            if (PREDICT) {  // k_predict of the next substep, as k_predict below computes it from these values
+                prev[i] = make_float4(x.x, x.y, x.z, w);
+                if (w > 0.f) {
+                    float4 F = ext[i];
+                    f3 a = mk3(F.x, F.y, F.z) * w;
+                    v = v + a * h;
+                    x = x + v * h;
+                    pos[i] = make_float4(x.x, x.y, x.z, w);
+                }
+            }
             vel[i] = make_float4(v.x, v.y, v.z, 0.f);
         }
     }
@@ -1626,9 +1637,13 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
     const GraspFrame* frames = reinterpret_cast<const GraspFrame*>(v->step_consts);
     // the multipliers are reset once per substep, so the K passes of a substep count as K * iters iterations
     const int passes = P.K * P.iters;
+    // Where nothing runs between a substep's node pass and the next substep's k_predict (no legacy grasp, guard,
+    // perpendicular damping or ribbon tick), the node pass also predicts the next substep: one launch fewer per substep.
+    const bool fuse = !P.legacy && !P.guardPasses && !P.perp && !P.tick;
     for (int n = 0; n < P.Nsub; n++) {
-        k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel, (const float4*)v->force,
-                                  L.xcd_order);
+        if (!fuse || n == 0)
+            k_predict<<<g, b, 0, s>>>(N, P.h, (float4*)v->pos, (float4*)v->prev, (float4*)v->vel,
+                                      (const float4*)v->force, L.xcd_order);
         if (P.legacy && (P.gate || P.grasping)) k_grasp_reset<<<g, b, 0, s>>>(N, P.gate, P.grasping, P.grasp);
         if (P.guardPasses) k_fill_float<<<grid_for(6 * T), GTS_BLOCK, 0, s>>>(6 * T, v->fiber_lambda, 0.f);
         for (int k = 0; k < P.K; k++) {
@@ -1679,14 +1694,23 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
         }
 #undef GTS_LEGACY
         const StepShapes* sh = shapes + n;
-#define GTS_FINISH(C, SC, VL)                                                                                    \
-    k_finish_nodes<C, SC, VL><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR,       \
-                                              P.invH, P.velScale, (float4*)v->pos, (const float4*)v->prev,       \
-                                              (float4*)v->vel, v->reaction, L.xcd_order)
+#define GTS_FINISH(C, SC, VL, PR)                                                                                \
+    k_finish_nodes<C, SC, VL, PR><<<g, b, 0, s>>>(N, sh, P.armed, P.alphaC, P.mu_s, P.mu_d, P.radius, P.scaleR,   \
+                                                  P.invH, P.velScale, (float4*)v->pos, (float4*)v->prev,         \
+                                                  (float4*)v->vel, v->reaction, L.xcd_order, P.h,                \
+                                                  (const float4*)v->force)
+        const bool predictNext = fuse && n + 1 < P.Nsub;
         if (P.legacy) {}
-        else if (P.guardPasses) { if (P.contact) GTS_FINISH(true, false, false); }
-        else if (P.contact) { if (P.perp) GTS_FINISH(true, false, true); else GTS_FINISH(true, true, true); }
-        else { if (P.perp) GTS_FINISH(false, false, true); else GTS_FINISH(false, true, true); }
+        else if (P.guardPasses) { if (P.contact) GTS_FINISH(true, false, false, false); }
+        else if (P.contact) {
+            if (P.perp) GTS_FINISH(true, false, true, false);
+            else if (predictNext) GTS_FINISH(true, true, true, true);
+            else GTS_FINISH(true, true, true, false);
+        } else {
+            if (P.perp) GTS_FINISH(false, false, true, false);
+            else if (predictNext) GTS_FINISH(false, true, true, true);
+            else GTS_FINISH(false, true, true, false);
+        }
 #undef GTS_FINISH
         if (P.guardPasses) {  // the safeguard between the node passes and the velocity update (:3678)
             launch_safeguard(v, L, P, s, n);
