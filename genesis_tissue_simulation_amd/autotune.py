@@ -14,9 +14,10 @@ import time
 import torch
 
 
-def candidates(max_lds_bytes):
+def candidates(max_lds_bytes, substeps=10):
     """Threads per workgroup x workgroups per CU (which splits the LDS) with the whole step in one launch, and the
-    best-guess shape with one launch per substep (the colours of a substep per launch)."""
+    shapes with one workgroup per CU at 512 and 1024 threads with 1, 2 and half the substeps per launch (the colours
+    per launch are that many substeps' colours)."""
     out = []
     for wpc in (1, 2, 4):
         lds_nodes = max_lds_bytes // 16 // wpc
@@ -24,17 +25,20 @@ def candidates(max_lds_bytes):
             if block * wpc > 2048:
                 continue
             out.append(dict(block_size=block, workgroups_per_cu=wpc, lds_nodes=lds_nodes, substeps_per_launch=0))
+    per_launch = sorted({k for k in (1, 2, max(1, substeps // 2)) if k < substeps})
     for block in (512, 1024):
-        out.append(dict(block_size=block, workgroups_per_cu=1, lds_nodes=max_lds_bytes // 16, substeps_per_launch=1))
+        for k in per_launch:
+            out.append(dict(block_size=block, workgroups_per_cu=1, lds_nodes=max_lds_bytes // 16,
+                            substeps_per_launch=k))
     return out
 
 
-def tune(batch, dt, steps=3, warmup=1, space=None):
+def tune(batch, dt, steps=5, warmup=1, space=None):
     """Time every candidate launch shape on `batch` and leave the fastest configured. Returns (best, table)."""
     # everything a step advances, the particle frames included (each step's frame update blends the previous ones)
     saved = {name: getattr(batch, name).clone()
              for name in ("pos", "prev", "vel", "lambda_edge", "lambda_tet", "rot", "rot_prev")}
-    space = space or candidates(int(batch.lib.gts_batched_lds_bytes()))
+    space = space or candidates(int(batch.lib.gts_batched_lds_bytes()), int(batch.params.substeps))
     table = []
     for cand in space:
         batch.configure(cand["block_size"], cand["lds_nodes"], cand["workgroups_per_cu"],
