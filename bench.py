@@ -127,7 +127,7 @@ def write_traffic(fetch_csv, write_csv, copy_mib, command):
     print(json.dumps(out, indent=1))
 
 
-def run_gpu(w, steps, warmup, tune=True):
+def run_gpu(w, steps, warmup, tune=True, persistent=False):
     import ctypes as ct
 
     import torch
@@ -146,12 +146,13 @@ def run_gpu(w, steps, warmup, tune=True):
             tissue.detect_cut(swept_cut_tool(w.blade(step), w.blade(step + 1), w.blade_thickness))
 
     # colour-kernel workgroup size, tuned on scratch copies of the tissue (or the library default, 64)
-    if tune:
+    if tune and not persistent:
         best, table = autotune.tune_plain(lambda cb: Tissue(w.grid, w.params, kd, kh, w.initial_positions,
                                                             colour_block=cb), advance)
     else:
         best, table = {"colour_block": 64}, []
-    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, colour_block=best["colour_block"])
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, colour_block=best["colour_block"],
+               persistent=persistent)
 
     def one(step):
         advance(t, step)
@@ -549,7 +550,7 @@ def batched_fp64_error(bt, k=0):
                 **position_errors(x, out[True]), bit_exact_vs_fp32_oracle=bool(np.array_equal(x, out[False])))
 
 
-def plain_fp64_error(w, steps=5):
+def plain_fp64_error(w, steps=5, persistent=False):
     """A fresh GPU run of `steps` steps of the plain workload (tool, blade cut every step) against the oracle of the
     same schedule: relative position error against FP64 and bit-exactness against FP32."""
     import torch
@@ -558,7 +559,7 @@ def plain_fp64_error(w, steps=5):
     from oracle import oracle as orc
 
     kd, kh = w.stiffness()
-    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions)
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, persistent=persistent)
     oracles = {}
     for double in (False, True):
         o = orc.Oracle(w.grid, w.params, w.initial_positions, double=double)
@@ -663,16 +664,27 @@ def plain_cpu_baseline(w, steps, threads, info):
             "cpu": info}
 
 
-def plain_leg(w, steps, warmup, tune, cpu, cpu_steps, threads, info):
-    """A plain single-instance leg: throughput, FP64 error and FP32-oracle exactness, and its CPU baseline."""
-    g = run_gpu(w, steps, warmup, tune=tune)
-    g.pop("tissue")
+def plain_leg(w, steps, warmup, tune, cpu, cpu_steps, threads, info, persistent=False):
+    """A plain single-instance leg: throughput, FP64 error and FP32-oracle exactness, and its CPU baseline. With
+    `persistent`, the tissue steps as one persistent workgroup (Tissue(persistent=True)), and the same workload
+    through the per-colour launches is reported beside it."""
+    g = run_gpu(w, steps, warmup, tune=tune, persistent=persistent)
+    t = g.pop("tissue")
+    if persistent and t._solo is None:
+        raise RuntimeError(f"{w.name} does not qualify for the persistent step")
     leg = {"value": g["live"] * w.params.substeps * steps / g["elapsed"], "unit": "tet-substeps/s",
            "ms_per_step": 1e3 * g["elapsed"] / steps, "steps": steps,
            "config": {"workload": w.name, "tets": g["T"], "live_tets": g["live"], "substeps": w.params.substeps,
-                      "iterations": w.params.iterations, "precise_strain": bool(w.params.precise_strain)}}
+                      "iterations": w.params.iterations, "precise_strain": bool(w.params.precise_strain),
+                      "path": "one persistent workgroup for the whole step" if persistent else
+                      "one launch per colour, graph-replayed"}}
+    if persistent:
+        c = run_gpu(w, steps, warmup, tune=tune)
+        c.pop("tissue")
+        leg["colour_launches"] = {"value": c["live"] * w.params.substeps * steps / c["elapsed"],
+                                  "ms_per_step": 1e3 * c["elapsed"] / steps}
     if cpu:
-        leg["fp64_check"] = plain_fp64_error(w)
+        leg["fp64_check"] = plain_fp64_error(w, persistent=persistent)
         leg["cpu_baseline"] = plain_cpu_baseline(w, cpu_steps, threads, info)
     return leg
 
@@ -890,7 +902,7 @@ def main():
         progress("reference-forms kidney leg done")
         # the CPU-runnable 6144-tet block: 200 steps of 10 substeps with its sweeping blade
         b = block_6144(precise=True)
-        line["block6144"] = plain_leg(b, b.steps, 3, False, not args.no_cpu, b.steps, threads, info)
+        line["block6144"] = plain_leg(b, b.steps, 3, False, not args.no_cpu, b.steps, threads, info, persistent=True)
         progress("6144-tet block leg done")
         r = run_cutting_block(args.cutting_steps, 2)
         progress("cutting block leg done")

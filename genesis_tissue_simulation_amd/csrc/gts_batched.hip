@@ -89,7 +89,10 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                 for (int c = 0; c < I.n_edge_colours; c++) {
                     for (int k = ecol[c] + tid; k < ecol[c + 1]; k += bs) {
                         uint32_t d = edge_desc[k];
-                        if (!(d & 1u)) continue;
+                        if (!(d & 1u)) {  // a dead row keeps the substep's reset multiplier, 0
+                            if (write && !read) lam_edge[k] = 0.f;
+                            continue;
+                        }
                         int axis = (int)((d >> 1) & 3u);
                         int i = (int)(d >> 4), j = i + (axis == 0 ? 1 : (axis == 1 ? nx : nxy));
                         float lam = read ? lam_edge[k] : 0.f, out;
@@ -124,7 +127,10 @@ __device__ void run_instance(const Store& S, const GtsBatchInstance& I, const Ba
                 if (c + 1 < I.n_tet_colours) next = fetch(tcol[c + 1] + tid, tcol[c + 2]);
                 for (int k = k0; k < end; k += bs) {
                     const TetInputs t = k == k0 ? first : fetch(k, end);
-                    if (!(t.d & 1u)) continue;
+                    if (!(t.d & 1u)) {  // a dead tet keeps the substep's reset multipliers, 0
+                        if (write && !read) lam_tet[k] = lam_tet[T + k] = 0.f;
+                        continue;
+                    }
                     int base = (int)(t.d >> 4), type = (int)((t.d >> 1) & 7u);
                     int id[4];
                     for (int q = 0; q < 4; q++) id[q] = base + corner_offset(kuhn_corner(type, q), nx, nxy);

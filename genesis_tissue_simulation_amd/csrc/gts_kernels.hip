@@ -194,13 +194,15 @@ __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fal
     if (k >= cnt) return;
     int e = off + k;
     uint32_t d = edge_desc[e];
-    if (!(d & 1u)) return;
+    // a row that does not project keeps the substep's reset multiplier, 0 (xpbd_solver.cu:3534)
+    const bool skip = !(d & 1u) || (FALLBACK && (d & 8u));
+    if (skip) {
+        if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = 0.f;
+        return;
+    }
     int axis = (int)((d >> 1) & 3u);
     float restLen = L.rest_len[axis];
-    if (FALLBACK) {
-        if (d & 8u) return;
-        alphaTilde = fallbackInvEh2 / restLen;
-    }
+    if (FALLBACK) alphaTilde = fallbackInvEh2 / restLen;
     int i = (int)(d >> 4), j = i + axis_stride(L, axis);
     float lam = READ_LAMBDA ? lambda[e] : 0.f, out;
     const bool ok = edge_project<DAMPED>(GlobalNodes{pos}, prev, i, j, restLen, alphaTilde, omega, betaGammaH, lam, out);
@@ -220,7 +222,10 @@ __global__ void k_bends(int off, int cnt, Lattice L, float alphaB, float omega, 
     int e = off + q;
     uint32_t d = bend_desc[e];
     int i = (int)(d >> 4), pair = (int)(d & 15u);
-    if (!((bend_alive[i] >> pair) & 1u)) return;
+    if (!((bend_alive[i] >> pair) & 1u)) {  // a dead element keeps the substep's reset multiplier, 0 (:3535)
+        if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = 0.f;
+        return;
+    }
     const int sa = bend_slot_a(pair), sb = bend_slot_b(pair);
     const int da = axis_stride(L, sa >> 1), db = axis_stride(L, sb >> 1);
     const int j = i + ((sa & 1) ? da : -da), k = i + ((sb & 1) ? db : -db);
@@ -241,7 +246,10 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
     if (k >= cnt) return;
     int e = off + k;
     uint32_t d = tet_desc[e];
-    if (!(d & 1u)) return;
+    if (!(d & 1u)) {  // a dead tet keeps the substep's reset multipliers, 0 (xpbd_solver.cu:3536)
+        if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = lambda[T + e] = 0.f;
+        return;
+    }
     int base = (int)(d >> 4), type = (int)((d >> 1) & 7u);
     int id[4];
     for (int c = 0; c < 4; c++) id[c] = base + corner_offset(kuhn_corner(type, c), L.nx, L.nxy);

@@ -12,7 +12,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import GtsCutTool, GtsLattice, GtsStepParams, GtsTissueView, GtsToolState, GtsTopology, check, load, v3
+from ._lib import (GtsBatchInstance, GtsBatchView, GtsCutTool, GtsLattice, GtsStepParams, GtsTissueView, GtsToolState,
+                   GtsTopology, check, load, v3)
 from .grid import CUT_POINT_DTYPE
 from .params import SolverParams
 
@@ -118,12 +119,15 @@ class Tissue:
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
                  device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None, colour_block=0,
-                 xcd_order=True):
+                 xcd_order=True, persistent=False):
         """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, mass, active, pinned, nbr,
         edge_desc, edge_lambda, tet_desc, tet_k_dev, tet_k_hyd, tet_lambda, rot, rot_prev) -- views into the pooled
         buffers of a batch. `colour_block` is
         the workgroup size of the edge / tet colour kernels (0: the library default); `xcd_order` gives each XCD one
-        contiguous range of every colour (same results)."""
+        contiguous range of every colour (same results). `persistent`: a small tissue (every node in one workgroup's
+        LDS, at most PERSISTENT_COLOUR_TETS tets per colour) steps without tool contact, guard, bending, edge damping or
+        cut tool as one persistent workgroup running the whole step (the batched kernel over this tissue alone):
+        colours are separated by workgroup barriers instead of launches, with the same results."""
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -236,6 +240,10 @@ class Tissue:
         v.bend_colour_offsets = self.bend_offsets.ctypes.data
         v.bend_desc, v.bend_lambda = self.bend_desc.data_ptr(), self.bend_lambda.data_ptr()
         self._tool_from = None
+        self.persistent = bool(persistent)
+        self._solo = None             # the one-instance batched view of the persistent step, and what it was built for
+        self._liveness_fresh = False  # liveness derived since initialisation (the persistent step does not derive it)
+        self.gravity = (0.0, 0.0, 0.0)
         self._staging = None          # pinned host buffers of observation()
         self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep
         self.emit_ncut = np.array([0.0, 1.0, 0.0], dtype=np.float32)
@@ -281,8 +289,53 @@ class Tissue:
 
     # ---------------------------------------------------------------- stepping
     def set_gravity(self, g):
+        self.gravity = tuple(float(x) for x in g)
         check("gts_set_gravity", self.lib.gts_set_gravity(ct.byref(self.view), *(float(x) for x in g),
                                                           current_stream()))
+
+    # A colour of at most this many tets takes at most two waves per SIMD in one 1024-thread workgroup, about 2.6 us of
+    # projection issue, below the 3.85 us a colour launch costs on MI355X; larger colours launch.
+    PERSISTENT_COLOUR_TETS = 512
+
+    def _persistent_view(self, p, tool_armed, blade_armed):
+        """The one-instance batched view over this tissue's own buffers when the step qualifies, else None."""
+        if not self.persistent or self.pooled or self.contact_model == 0 or self.cut_tool is not None:
+            return None
+        if p.contact_enabled and (tool_armed or blade_armed):
+            return None
+        if p.guard_passes or p.bend_kb > 0 or p.beta_s != 0 or not p.edge_ks > 0 or self.T <= 0:
+            return None
+        lds_nodes = int(self.lib.gts_batched_lds_bytes()) // 16
+        if self.N > lds_nodes or int(np.diff(self.tet_offsets).max(initial=0)) > self.PERSISTENT_COLOUR_TETS:
+            return None
+        key = (self.T, self.E, self.tet_offsets.tobytes(), self.edge_offsets.tobytes(), float(p.edge_ks),
+               self.gravity, bool(self.view.use_graph))
+        if self._solo is None or self._solo[0] != key:
+            dev, topo, grid = self.device, self.topology, self.grid
+            nx, ny = grid.dims[0] + 1, grid.dims[1] + 1
+            rec = GtsBatchInstance(0, self.N, 0, self.E, 0, self.T, nx, nx * ny, len(self.edge_offsets) - 1,
+                                   len(self.tet_offsets) - 1, 0, 0, float(p.edge_ks),
+                                   (ct.c_float * 3)(*[float(x) for x in topo["axis_rest_length"]]),
+                                   (ct.c_float * 54)(*topo["type_inv_rest"].astype(np.float32).reshape(-1)))
+            raw = np.frombuffer(bytes(rec), dtype=np.uint8).copy()
+            keep = dict(instances=torch.as_tensor(raw).to(dev),
+                        colours=torch.as_tensor(np.concatenate((self.edge_offsets, self.tet_offsets))
+                                                .astype(np.int32)).to(dev),
+                        work=torch.tensor([0, 1, 0], dtype=torch.int32, device=dev),
+                        base=torch.zeros(self.N, dtype=torch.int32, device=dev))
+            view = GtsBatchView(
+                1, keep["instances"].data_ptr(), keep["colours"].data_ptr(), keep["work"].data_ptr(),
+                keep["work"].data_ptr() + 8, 1, 1024, self.N, int(self.view.use_graph), 0,
+                (ct.c_float * 3)(*self.gravity), self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(),
+                self.mass.data_ptr(), self.edge_desc.data_ptr(), self.tet_desc.data_ptr(), self.tet_k_dev.data_ptr(),
+                self.tet_k_hyd.data_ptr(), self.edge_lambda.data_ptr(), self.tet_lambda.data_ptr(),
+                self.tet_alpha.data_ptr(), self.N, self.nbr.data_ptr(), self.active.data_ptr(),
+                self.pinned.data_ptr(), keep["base"].data_ptr(), self.rot.data_ptr(), self.rot_prev.data_ptr())
+            self._solo = (key, view, keep)
+        if not self._liveness_fresh:
+            check("gts_refresh_liveness", self.lib.gts_refresh_liveness(ct.byref(self.view), current_stream()))
+            self._liveness_fresh = True
+        return self._solo[1]
 
     def _guard_buffers(self):
         """The tissue guard's device buffers (xpbd_set_tissue_guard's, xpbd_solver.cu:1900-1912), allocated on the
@@ -349,8 +402,13 @@ class Tissue:
                 ct.byref(self.cut_tool), self.emit_ncut.ctypes.data_as(ct.c_void_p), float(self.tear_tau),
                 current_stream()))
         else:
-            check("gts_step", self.lib.gts_step(ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to),
-                                                int(tool_armed), int(blade_armed), current_stream()))
+            solo = self._persistent_view(p, tool_armed, blade_armed)
+            if solo is not None:
+                check("gts_batched_step", self.lib.gts_batched_step(ct.byref(solo), ct.byref(p), current_stream()))
+            else:
+                check("gts_step", self.lib.gts_step(ct.byref(self.view), ct.byref(p), ct.byref(frm), ct.byref(to),
+                                                    int(tool_armed), int(blade_armed), current_stream()))
+                self._liveness_fresh = True
         self._tool_from = to
 
     def set_cut_tool(self, cut_tool, tear_tau=0.0):

@@ -829,11 +829,12 @@ bool edgeProject(V3<R>& xi, V3<R>& xj, const V3<R>& pi, const V3<R>& pj, R wi, R
 // k_solve_edges on edge e of the tissue; under the fallback (no edge stiffness, :469-479) an edge whose owner cell
 // keeps a live tet is skipped and the others take alphaTilde = 1 / (E L h^2).
 template <class R> void solveEdge(Tissue<R>& s, int e, R alphaTilde, bool fallback, R fallbackInvEh2, bool useLambda) {
-    if (s.eactive[e] == 0) return;
-    if (fallback) {
-        if (s.support[s.eowner[e]]) return;
-        alphaTilde = fallbackInvEh2 / s.erest[e];
+    // the substep's reset (xpbd_solver.cu:3534) leaves every row that does not project at 0
+    if (s.eactive[e] == 0 || (fallback && s.support[s.eowner[e]])) {
+        if (!useLambda) s.elambda[e] = 0;
+        return;
     }
+    if (fallback) alphaTilde = fallbackInvEh2 / s.erest[e];
     int i = s.ei[e], j = s.ej[e];
     R lam = useLambda ? s.elambda[e] : (R)0;
     if (edgeProject(s.pos[i], s.pos[j], s.prev[i], s.prev[j], s.invMass[i], s.invMass[j], s.erest[e], alphaTilde,
@@ -900,7 +901,10 @@ bool bendProject(V3<R>& xi, V3<R>& xj, V3<R>& xk, const V3<R>& pi, const V3<R>& 
 
 // k_solve_bending on element e of the tissue, gated by its pair's alive flag.
 template <class R> void solveBend(Tissue<R>& s, int e, bool useLambda) {
-    if (!s.bendAlive[s.bsrc[e]]) return;
+    if (!s.bendAlive[s.bsrc[e]]) {
+        if (!useLambda) s.blambda[e] = 0;  // the substep's reset (:3535)
+        return;
+    }
     int i = s.bi[e], j = s.bj[e], k = s.bk[e];
     R lam = useLambda ? s.blambda[e] : (R)0;
     if (bendProject(s.pos[i], s.pos[j], s.pos[k], s.prev[i], s.prev[j], s.prev[k], s.invMass[i], s.invMass[j],
@@ -1134,7 +1138,13 @@ bool tetProject(int mode, const V3<R> p[4], const V3<R> pv[4], const R w[4], con
 }
 
 template <class R> void solveTet(Tissue<R>& s, int e, R dampingOverH, R bulkDampingOverH, bool useLambda) {
-    if (s.tactive[e] == 0) return;
+    if (s.tactive[e] == 0) {  // the substep's reset (xpbd_solver.cu:3536-3539)
+        if (!useLambda) {
+            s.lamDev[e] = 0;
+            s.lamHyd[e] = 0;
+        }
+        return;
+    }
     const int* id = &s.tids[4 * e];
     V3<R> p[4], pv[4], out[4];
     R w[4];
