@@ -294,7 +294,7 @@ class Tissue:
                                                           current_stream()))
 
     # A colour of at most this many tets takes at most two waves per SIMD in one 1024-thread workgroup, about 2.6 us of
-    # projection issue, below the 3.85 us a colour launch costs on MI355X; larger colours launch.
+    # projection issue, below the 3.6-3.9 us a colour launch costs on MI355X; larger colours launch.
     PERSISTENT_COLOUR_TETS = 512
 
     def _persistent_view(self, p, tool_armed, blade_armed):
