@@ -67,33 +67,33 @@ template <bool DAMPED, class Store>
 __device__ __forceinline__ bool bend_project(const Store& S, const float4* __restrict__ prev, int i, int j, int k,
                                              float theta0, float alphaB, float omega, float betaGammaH, float lam,
                                              float& lamOut) {
-    float4 pi = S.ld(i), pj = S.ld(j), pk = S.ld(k);
-    float wi = pi.w, wj = pj.w, wk = pk.w;
-    if (wi + wj + wk <= 0.f) return false;
-    f3 u = xyz4(pi) - xyz4(pj), v = xyz4(pi) - xyz4(pk);
-    float Lu = len3(u), Lv = len3(v);
-    if (Lu < 1e-9f || Lv < 1e-9f) return false;
-    f3 nu = u * (1.f / Lu), nv = v * (1.f / Lv);
-    float c = fmaxf(-1.f, fminf(1.f, dot3(nu, nv)));
-    float s = sqrtf(fmaxf(1.f - c * c, 0.f));
-    if (s < 1e-4f) return false;
-    float C = own_acos(c) - theta0;
-    f3 gj = nv - nu * c, gk = nu - nv * c;
-    f3 gi = (gj + gk) * -1.f;
-    float wg = wi * dot3(gi, gi) + wj * dot3(gj, gj) + wk * dot3(gk, gk);
-    float gradDx = 0.f;
+    const float4 apex = S.ld(i), endA = S.ld(j), endB = S.ld(k);
+    const float wApex = apex.w, wA = endA.w, wB = endB.w;
+    if (wApex + wA + wB <= 0.f) return false;
+    const f3 armA = xyz4(apex) - xyz4(endA), armB = xyz4(apex) - xyz4(endB);
+    const float lenA = len3(armA), lenB = len3(armB);
+    if (lenA < 1e-9f || lenB < 1e-9f) return false;
+    const f3 unitA = armA * (1.f / lenA), unitB = armB * (1.f / lenB);
+    const float cosT = fmaxf(-1.f, fminf(1.f, dot3(unitA, unitB)));
+    if (sqrtf(fmaxf(1.f - cosT * cosT, 0.f)) < 1e-4f) return false;  // arms (anti)parallel: no defined plane
+    const float err = own_acos(cosT) - theta0;
+    // each arm end moves across its arm, within the plane of the two arms; the apex takes the opposite sum
+    const f3 dirA = unitB - unitA * cosT, dirB = unitA - unitB * cosT;
+    const f3 dirApex = (dirA + dirB) * -1.f;
+    const float weighted = wApex * dot3(dirApex, dirApex) + wA * dot3(dirA, dirA) + wB * dot3(dirB, dirB);
+    float rate = 0.f;  // the angle's rate along the directions over the substep (Kelvin-Voigt term)
     if (DAMPED)
-        gradDx = dot3(gi, xyz4(pi) - xyz4(prev[i])) + dot3(gj, xyz4(pj) - xyz4(prev[j])) +
-                 dot3(gk, xyz4(pk) - xyz4(prev[k]));
-    float denom = (1.f + betaGammaH) * wg + alphaB;
-    if (denom < 1e-12f) return false;
-    float dl = (-C - alphaB * lam - betaGammaH * gradDx) / denom;
-    float applied = omega * dl;
-    lamOut = lam + applied;
-    f3 xi = xyz4(pi) + gi * (wi * applied), xj = xyz4(pj) + gj * (wj * applied), xk = xyz4(pk) + gk * (wk * applied);
-    S.st(i, make_float4(xi.x, xi.y, xi.z, wi));
-    S.st(j, make_float4(xj.x, xj.y, xj.z, wj));
-    S.st(k, make_float4(xk.x, xk.y, xk.z, wk));
+        rate = dot3(dirApex, xyz4(apex) - xyz4(prev[i])) + dot3(dirA, xyz4(endA) - xyz4(prev[j])) +
+               dot3(dirB, xyz4(endB) - xyz4(prev[k]));
+    const float den = (1.f + betaGammaH) * weighted + alphaB;
+    if (den < 1e-12f) return false;
+    const float step = omega * ((-err - alphaB * lam - betaGammaH * rate) / den);
+    lamOut = lam + step;
+    const f3 newApex = xyz4(apex) + dirApex * (wApex * step), newA = xyz4(endA) + dirA * (wA * step),
+             newB = xyz4(endB) + dirB * (wB * step);
+    S.st(i, make_float4(newApex.x, newApex.y, newApex.z, wApex));
+    S.st(j, make_float4(newA.x, newA.y, newA.z, wA));
+    S.st(k, make_float4(newB.x, newB.y, newB.z, wB));
     return true;
 }
 

@@ -308,8 +308,7 @@ class Tissue:
         lds_nodes = int(self.lib.gts_batched_lds_bytes()) // 16
         if self.N > lds_nodes or int(np.diff(self.tet_offsets).max(initial=0)) > self.PERSISTENT_COLOUR_TETS:
             return None
-        key = (self.T, self.E, self.tet_offsets.tobytes(), self.edge_offsets.tobytes(), float(p.edge_ks),
-               self.gravity, bool(self.view.use_graph))
+        key = (self.T, self.E, self.tet_offsets.tobytes(), self.edge_offsets.tobytes(), float(p.edge_ks))
         if self._solo is None or self._solo[0] != key:
             dev, topo, grid = self.device, self.topology, self.grid
             nx, ny = grid.dims[0] + 1, grid.dims[1] + 1
@@ -323,15 +322,20 @@ class Tissue:
                                                 .astype(np.int32)).to(dev),
                         work=torch.tensor([0, 1, 0], dtype=torch.int32, device=dev),
                         base=torch.zeros(self.N, dtype=torch.int32, device=dev))
+            # one kernel and the frame update per step: launched directly (a graph would be keyed on the gravity,
+            # which a caller may ramp every step)
             view = GtsBatchView(
                 1, keep["instances"].data_ptr(), keep["colours"].data_ptr(), keep["work"].data_ptr(),
-                keep["work"].data_ptr() + 8, 1, 1024, self.N, int(self.view.use_graph), 0,
-                (ct.c_float * 3)(*self.gravity), self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(),
-                self.mass.data_ptr(), self.edge_desc.data_ptr(), self.tet_desc.data_ptr(), self.tet_k_dev.data_ptr(),
+                keep["work"].data_ptr() + 8, 1, 1024, self.N, 0, 0, (ct.c_float * 3)(0.0, 0.0, 0.0),
+                self.pos.data_ptr(), self.prev.data_ptr(), self.vel.data_ptr(), self.mass.data_ptr(),
+                self.edge_desc.data_ptr(), self.tet_desc.data_ptr(), self.tet_k_dev.data_ptr(),
                 self.tet_k_hyd.data_ptr(), self.edge_lambda.data_ptr(), self.tet_lambda.data_ptr(),
                 self.tet_alpha.data_ptr(), self.N, self.nbr.data_ptr(), self.active.data_ptr(),
                 self.pinned.data_ptr(), keep["base"].data_ptr(), self.rot.data_ptr(), self.rot_prev.data_ptr())
             self._solo = (key, view, keep)
+        view = self._solo[1]
+        for a in range(3):  # the batched kernel applies gravity * mass, as k_set_gravity's force does
+            view.gravity[a] = self.gravity[a]
         if not self._liveness_fresh:
             check("gts_refresh_liveness", self.lib.gts_refresh_liveness(ct.byref(self.view), current_stream()))
             self._liveness_fresh = True

@@ -1,7 +1,7 @@
 """A small tissue stepped as one persistent workgroup (Tissue(persistent=True): the batched kernel over the tissue's
 own buffers, colours separated by workgroup barriers) against the FP32 oracle: the C1 block with the blade cutting
-every step, in the direct and the precise tet forms, with one and two iterations and strain damping, then compacted
-and stepped on; positions, velocities, multipliers, particle frames and cut topology bit-exact after every step. A
+every step and gravity ramping in, in the direct and the precise tet forms, with one and two iterations and strain
+damping, then compacted and stepped on; positions, velocities, multipliers, particle frames and cut topology bit-exact after every step. A
 tissue over the colour-size limit, or with tool contact armed, takes the per-colour launches."""
 import numpy as np
 import pytest
@@ -29,7 +29,9 @@ def test_persistent_small_tissue_matches_oracle(precise, iterations, damping):
         sim.set_gravity(w.gravity)
     for s in range(6, 20):
         tool = swept_cut_tool(w.blade(s), w.blade(s + 1), w.blade_thickness)
+        ramp = min(1.0, (s - 5) / 6.0)  # gravity ramping in, as NativeTissue.step does
         for sim in (t, o):
+            sim.set_gravity(tuple(ramp * g for g in w.gravity))
             sim.step(w.dt)
             sim.detect_cut(tool)
         assert t._solo is not None, "the block qualifies for the persistent step"
