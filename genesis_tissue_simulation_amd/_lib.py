@@ -48,7 +48,9 @@ class GtsTissueView(ct.Structure):
                 ("use_graph", I), ("colour_block", I), ("fiber_lambda", P), ("feasible_target", P),
                 ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P), ("grasp_ring", P),
                 ("edge_exempt", P), ("nbr_init", P), ("xcd_order", I), ("n_bends", I), ("n_bend_colours", I),
-                ("bend_colour_offsets", P), ("bend_desc", P), ("bend_lambda", P)]
+                ("bend_colour_offsets", P), ("bend_desc", P), ("bend_lambda", P), ("tile_groups", I),
+                ("tile_kmax", I), ("tile_box", I), ("tile_tets", I), ("tile_group_info", P), ("tile_headers", P),
+                ("tile_entries", P), ("tile_writes", P), ("pos_alt", P), ("lambda_alt", P)]
 
 
 class GtsStepParams(ct.Structure):

@@ -23,13 +23,14 @@ GTS_HD float comp3(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 GTS_HD f3 ld3(const float* p) { return mk3(p[0], p[1], p[2]); }
 
 // Kuhn split of a cell around its main diagonal 0-7, corners in bit order i = dx | dy<<1 | dz<<2
-// (reference: xpbd/xpbd_solver.cu:1624). Packed 4 x 3 bits per type.
+// (reference: xpbd/xpbd_solver.cu:1624):
+//   {0,1,3,7},{0,3,2,7},{0,2,6,7},{0,6,4,7},{0,4,5,7},{0,5,1,7}
+// Every type starts at 0 and ends at 7; its two middle corners are packed 6 bits a type into one immediate, so that
+// a kernel finds a tet's nodes without a table in memory (a load that would wait on the tet's descriptor).
 GTS_HD int kuhn_corner(int type, int k) {
-    // {0,1,3,7},{0,3,2,7},{0,2,6,7},{0,6,4,7},{0,4,5,7},{0,5,1,7}
-    const uint32_t packed[6] = {0u | 1u << 3 | 3u << 6 | 7u << 9, 0u | 3u << 3 | 2u << 6 | 7u << 9,
-                                0u | 2u << 3 | 6u << 6 | 7u << 9, 0u | 6u << 3 | 4u << 6 | 7u << 9,
-                                0u | 4u << 3 | 5u << 6 | 7u << 9, 0u | 5u << 3 | 1u << 6 | 7u << 9};
-    return (int)((packed[type] >> (3 * k)) & 7u);
+    if (k == 0) return 0;
+    if (k == 3) return 7;
+    return (int)((0x36c9b24d9ull >> (6 * type + 3 * (k - 1))) & 7u);
 }
 
 // Node offset of bit-order corner c in a lattice with nx, nxy points per row / layer.

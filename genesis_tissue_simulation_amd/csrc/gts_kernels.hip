@@ -40,12 +40,20 @@ inline int grid_for(int n, int block = GTS_BLOCK) { return std::max(1, (n + bloc
 struct Lattice {  // strides and per-type rest data passed by value to every kernel
     int nx, nxy, vx, vy, vz;
     int xcd_order;  // the colour kernels' workgroup -> item mapping (colour_item)
+    int block;      // threads per workgroup of the colour kernels, an argument rather than a read of the dispatch
     float inv_rest[6][9];
     float rest_len[3];
 };
 
+// Threads per workgroup of the colour kernels. A colour of the 250k-tet kidney lattice holds about 8k tets: 64-thread
+// workgroups spread it over ~125 CUs instead of ~32, which measured 5.81 ms per step against 6.32 ms at 256 threads
+// and 7.14 ms at 512; 32-thread workgroups (half-empty waves on every CU) measured 6.10 ms.
+#define GTS_COLOUR_BLOCK 64
+inline int colour_block(const GtsTissueView* v) { return v->colour_block > 0 ? v->colour_block : GTS_COLOUR_BLOCK; }
+
 Lattice lattice_of(const GtsTissueView* v) {
     Lattice L;
+    L.block = colour_block(v);
     L.vx = v->vx; L.vy = v->vy; L.vz = v->vz;
     L.nx = v->vx + 1;
     L.nxy = (v->vx + 1) * (v->vy + 1);
@@ -63,15 +71,30 @@ __device__ __forceinline__ f3 xyz(float4 p) { return mk3(p.x, p.y, p.z); }
 // XCD b % 8). With L.xcd_order the workgroups of one XCD take one contiguous range of the colour; colours list their
 // items in lattice order, so an XCD then meets the same slab of the lattice in every colour, and that slab's nodes can
 // stay in the XCD's L2 between launches. The mapping is a bijection, so results do not depend on it.
-__device__ __forceinline__ int ordered_item(int xcd_order) {
-    int b = blockIdx.x;
-    if (xcd_order) {
-        const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, slot = b >> 3;
-        b = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + slot;
-    }
-    return b * blockDim.x + threadIdx.x;
+// `block` is the launch's workgroup size, passed in: reading blockDim costs a load from the dispatch packet before the
+// first load that depends on the item.
+__device__ __forceinline__ void asm_use(int x) { asm volatile("" ::"s"(x)); }
+__device__ __forceinline__ void asm_use(float x) { asm volatile("" ::"s"(x)); }
+__device__ __forceinline__ void asm_use(unsigned x) { asm volatile("" ::"s"(x)); }
+template <class T>
+__device__ __forceinline__ void asm_use(T* p) { asm volatile("" ::"s"(p)); }
+
+// Marks kernel arguments as needed at this point: their scalar loads issue together before it, not one round at a time
+// before each first use. No instruction is emitted.
+template <class... A>
+__device__ __forceinline__ void args_ready(const A&... a) {
+    (asm_use(a), ...);
 }
-__device__ __forceinline__ int colour_item(const Lattice& L) { return ordered_item(L.xcd_order); }
+
+// Branch-free, so that the kernel's argument loads issue together and its first data load waits on one round of them.
+__device__ __forceinline__ int ordered_item(int xcd_order, int block) {
+    const int b = blockIdx.x;
+    const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, slot = b >> 3;
+    const int mapped = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + slot;
+    return (xcd_order ? mapped : b) * block + threadIdx.x;
+}
+// the colour kernels launched with colour_block threads
+__device__ __forceinline__ int colour_item(const Lattice& L) { return ordered_item(L.xcd_order, L.block); }
 
 // ------------------------------------------------------------------------------------------------ set-up kernels
 
@@ -170,7 +193,8 @@ __global__ void k_tet_alpha(int T, float invH2, const float* kd, const float* kh
 // k_predict (xpbd_solver.cu:393-412): prev = x; for w > 0, v += h F w and x += h v.
 __global__ void k_predict(int N, float h, float4* __restrict__ pos, float4* __restrict__ prev,
                           float4* __restrict__ vel, const float4* __restrict__ force, int xcd_order) {
-    int i = ordered_item(xcd_order);  // nodes in lattice order: the XCD that owns their slab in the colour kernels
+    // nodes in lattice order: the XCD that owns their slab in the colour kernels
+    int i = ordered_item(xcd_order, GTS_BLOCK);
     if (i >= N) return;
     float4 p = pos[i];
     prev[i] = p;
@@ -190,10 +214,12 @@ template <bool READ_LAMBDA, bool WRITE_LAMBDA, bool FALLBACK, bool DAMPED>
 __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fallbackInvEh2, float omega,
                         float betaGammaH, const uint32_t* __restrict__ edge_desc, float* __restrict__ lambda,
                         float4* __restrict__ pos, const float4* __restrict__ prev) {
-    int k = colour_item(L);
+    args_ready(off, cnt, edge_desc, L.xcd_order, L.block, (int)gridDim.x, L.nx, L.nxy, L.rest_len[0], L.rest_len[1],
+               L.rest_len[2], pos, lambda, prev);  // as in k_tets
+    const int k = colour_item(L);
+    const uint32_t d = edge_desc[off + (k < cnt ? k : cnt - 1)];  // issued before the range check, as in k_tets
     if (k >= cnt) return;
-    int e = off + k;
-    uint32_t d = edge_desc[e];
+    const int e = off + k;
     // a row that does not project keeps the substep's reset multiplier, 0 (xpbd_solver.cu:3534)
     const bool skip = !(d & 1u) || (FALLBACK && (d & 8u));
     if (skip) {
@@ -242,10 +268,15 @@ template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED, bool PRECI
 __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float dampingOverH, float bulkDampingOverH,
                        const uint32_t* __restrict__ tet_desc, const float* __restrict__ alpha,
                        float* __restrict__ lambda, float4* __restrict__ pos, const float4* __restrict__ prev) {
-    int k = colour_item(L);
+    // Every argument the tet needs is requested in one round of scalar loads here, rather than each just before its
+    // first use (the compiler's placement), which would put further rounds between the descriptor and the nodes.
+    args_ready(off, cnt, tet_desc, L.xcd_order, L.block, (int)gridDim.x, L.nx, L.nxy, L.rest_len[0], L.rest_len[1],
+               L.rest_len[2], pos, alpha, lambda, prev);
+    const int k = colour_item(L);
+    // the descriptor load issues before the range check (at a clamped item), with nothing but the arguments before it
+    const uint32_t d = tet_desc[off + (k < cnt ? k : cnt - 1)];
     if (k >= cnt) return;
-    int e = off + k;
-    uint32_t d = tet_desc[e];
+    const int e = off + k;
     if (!(d & 1u)) {  // a dead tet keeps the substep's reset multipliers, 0 (xpbd_solver.cu:3536)
         if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = lambda[T + e] = 0.f;
         return;
@@ -267,6 +298,146 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
     }
 }
 
+// Several consecutive tet colours in one launch over one lattice region per workgroup (the tiled sweep; the plan and
+// its argument are in genesis_tissue_simulation_amd/tiling.py). The region's node box is loaded into LDS from pos_in;
+// pass p projects, in LDS, the colour-(c0+p) tets within k-1-p cells of the region (the plan's entries), each pass
+// after a workgroup barrier, so every tet reads its nodes as the per-colour sweep leaves them; the region then writes
+// the nodes it owns to pos_out and the multipliers of the tets it owns to lam_out (read from lam_in). No projection
+// reads memory another workgroup of the launch writes. Undamped forms only: prev is not read.
+__device__ __forceinline__ int ordered_block(int xcd_order) {
+    int b = blockIdx.x;
+    if (xcd_order) {
+        const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, slot = b >> 3;
+        b = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + slot;
+    }
+    return b;
+}
+
+#define GTS_TILE_KMAX 4  // colours a tiled launch may hold: each thread prefetches its first tet of every pass
+#define GTS_TILE_BOX_LOADS 8  // nodes of the LDS box a thread loads: box^3 <= 8 * 512
+#define GTS_TILE_WRITE_LOADS 2  // node writes a thread requests before the passes
+
+// One tet's inputs of a projection, loaded ahead of use.
+struct TiledTetInputs {
+    uint32_t d;
+    float ad, ah, lamD, lamH;
+};
+
+// One tiled tet projection in the region's LDS box: e the plan entry, the rest its inputs as loaded.
+template <int MODE, bool WRITE_LAMBDA, bool PRECISE>
+__device__ __forceinline__ void tiled_project(int2 e, uint32_t d, float ad, float ah, float inD, float inH,
+                                              float4* nodes, int box, int box2, const Lattice& L, float omega, int T,
+                                              float* __restrict__ lam_out) {
+    const bool owned = e.x < 0;
+    const int slot = e.x & 0x7fffffff;
+    float lamD = inD, lamH = inH;
+    bool ok = false;
+    if (d & 1u) {  // a dead tet keeps its multipliers: 0 from the first iteration's reset
+        const int type = (int)((d >> 1) & 7u);
+        int id[4];
+        for (int c = 0; c < 4; c++) id[c] = e.y + corner_offset(kuhn_corner(type, c), box, box2);
+        f3 rk[3];
+        if (PRECISE)
+            for (int c = 0; c < 3; c++) rk[c] = kuhn_rest_edge(type, c + 1, L.rest_len);
+        ok = tet_project<MODE, false, PRECISE>(LdsNodes{nodes}, nullptr, id, L.inv_rest[type], ad, ah, lamD, lamH,
+                                               omega, 0.f, 0.f, rk);
+    }
+    // as k_tets stores in place: the update, else the multiplier read (the first iteration's reset 0)
+    if (owned && WRITE_LAMBDA) {
+        lam_out[slot] = ok ? lamD : inD;
+        lam_out[T + slot] = ok ? lamH : inH;
+    }
+}
+
+template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool PRECISE>
+__global__ void __launch_bounds__(512) k_tets_tiled(int k, int cap, int hdr_stride, const int* __restrict__ headers,
+                                                    const int2* __restrict__ entries, const int2* __restrict__ writes,
+                                                    int box, Lattice L, float omega, int T,
+                                                    const uint32_t* __restrict__ tet_desc,
+                                                    const float* __restrict__ alpha, const float* __restrict__ lam_in,
+                                                    float* __restrict__ lam_out, const float4* __restrict__ pos_in,
+                                                    float4* __restrict__ pos_out) {
+    extern __shared__ float4 nodes[];
+    const int rl = ordered_block(L.xcd_order);
+    const int* h = headers + rl * hdr_stride;
+    // A region's entries sit at a fixed stride, so that they are requested together with its header: the first of
+    // the two rounds of loads before the projections. Loads are unconditional, at clamped in-bounds rows, so that
+    // no branch makes the compiler wait for one before issuing the next.
+    const int2* ent = entries + (size_t)rl * k * cap;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int row = tid < cap ? tid : cap - 1;
+    int cnt[GTS_TILE_KMAX];
+    int2 en[GTS_TILE_KMAX];
+#pragma unroll
+    for (int p = 0; p < GTS_TILE_KMAX; p++) {
+        cnt[p] = p < k ? h[8 + p] : 0;
+        en[p] = ent[(p < k ? p : 0) * cap + row];
+    }
+    const int ox = h[0], oy = h[1], oz = h[2], wbeg = h[4], wcnt = h[5];
+    // The second round, all in flight together: this thread's first tet of every pass, its first node writes and
+    // its nodes of the box (box <= 16 and 512 threads: at most 8 a thread).
+    TiledTetInputs pre[GTS_TILE_KMAX];
+#pragma unroll
+    for (int p = 0; p < GTS_TILE_KMAX; p++) {
+        const int slot = en[p].x & 0x7fffffff;  // a valid slot on every row, padding included (slot 0)
+        pre[p].d = tet_desc[slot];
+        pre[p].ad = alpha[slot];
+        pre[p].ah = alpha[T + slot];
+        pre[p].lamD = READ_LAMBDA ? lam_in[slot] : 0.f;
+        pre[p].lamH = READ_LAMBDA ? lam_in[T + slot] : 0.f;
+    }
+    int2 wr[GTS_TILE_WRITE_LOADS];
+#pragma unroll
+    for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++) {
+        const int q = tid + j * bs;
+        wr[j] = writes[wbeg + (q < wcnt ? q : 0)];  // wcnt >= 1: every region writes a node
+    }
+    const int ny = L.nxy / L.nx, nz = L.vz + 1, box2 = box * box, box3 = box2 * box;
+    {
+        // i / box and i / box^2 for i < 4096 as (i * ceil(2^20 / d)) >> 20, exact for d <= 256
+        const uint32_t m1 = ((1u << 20) + box - 1) / box, m2 = ((1u << 20) + box2 - 1) / box2;
+        float4 v[GTS_TILE_BOX_LOADS];
+        bool in[GTS_TILE_BOX_LOADS];
+#pragma unroll
+        for (int j = 0; j < GTS_TILE_BOX_LOADS; j++) {
+            const int i = tid + j * bs;
+            const int lz = (int)(((uint32_t)i * m2) >> 20), rem = i - lz * box2;
+            const int ly = (int)(((uint32_t)rem * m1) >> 20), lx = rem - ly * box;
+            const int x = ox + lx, y = oy + ly, z = oz + lz;
+            in[j] = i < box3 && x >= 0 && x < L.nx && y >= 0 && y < ny && z >= 0 && z < nz;
+            const int g = in[j] ? x + L.nx * y + L.nxy * z : 0;
+            v[j] = pos_in[g];
+        }
+#pragma unroll
+        for (int j = 0; j < GTS_TILE_BOX_LOADS; j++)
+            if (in[j]) nodes[tid + j * bs] = v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < GTS_TILE_KMAX; p++) {
+        if (p >= k) break;
+        if (tid < cnt[p])
+            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(en[p], pre[p].d, pre[p].ad, pre[p].ah, pre[p].lamD,
+                                                       pre[p].lamH, nodes, box, box2, L, omega, T, lam_out);
+        for (int q = tid + bs; q < cnt[p]; q += bs) {  // passes of more tets than threads
+            const int2 e = ent[p * cap + q];
+            const int slot = e.x & 0x7fffffff;
+            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(e, tet_desc[slot], alpha[slot], alpha[T + slot],
+                                                       READ_LAMBDA ? lam_in[slot] : 0.f,
+                                                       READ_LAMBDA ? lam_in[T + slot] : 0.f, nodes, box, box2, L,
+                                                       omega, T, lam_out);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++)
+        if (tid + j * bs < wcnt) pos_out[wr[j].y] = nodes[wr[j].x];
+    for (int q = tid + GTS_TILE_WRITE_LOADS * bs; q < wcnt; q += bs) {
+        const int2 w = writes[wbeg + q];
+        pos_out[w.y] = nodes[w.x];
+    }
+}
+
 // Friction contact of each node with the tools (friction_contact.cuh:143-169, two solve passes as at
 // xpbd_solver.cu:3663-3665), then (VEL) k_update_velocity and, unless perpendicular edge damping must run between
 // them, k_apply_velscale (xpbd_solver.cu:871-919). Nodes are independent here, so one pass does all of it; with the
@@ -277,7 +448,7 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
                                float4* __restrict__ prev, float4* __restrict__ vel, float* reaction,
                                int xcd_order, float h = 0.f, const float4* __restrict__ ext = nullptr) {
-    int i = ordered_item(xcd_order);
+    int i = ordered_item(xcd_order, GTS_BLOCK);
     f3 force = mk3(0, 0, 0);
     float contacts = 0.f;
     if (i < N) {
@@ -363,7 +534,7 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
 // k_perp_damp_edges (xpbd_solver.cu:890-910) over one colour.
 __global__ void k_perp_damp(int off, int cnt, Lattice L, float cs, float h, const uint32_t* edge_desc,
                             const float4* pos, float4* vel) {
-    int k = colour_item(L);
+    int k = ordered_item(L.xcd_order, GTS_BLOCK);
     if (k >= cnt) return;
     uint32_t d = edge_desc[off + k];
     if (!(d & 1u)) return;
@@ -423,7 +594,7 @@ __device__ __forceinline__ void type_rows(const Lattice& L, int type, f3 b[3]) {
 // C = s^3, s = max(L / L0 - 1, 0), of compliance alphaDev / strength; the rest edges are the columns of Dm = B^-1.
 __global__ void k_tensile(int off, int cnt, Lattice L, float strength, const uint32_t* __restrict__ tet_desc,
                           const float* __restrict__ alpha, float* __restrict__ fiber, float4* __restrict__ pos) {
-    int k = colour_item(L);
+    int k = ordered_item(L.xcd_order, GTS_BLOCK);
     if (k >= cnt) return;
     int e = off + k;
     uint32_t d = tet_desc[e];
@@ -479,7 +650,7 @@ __global__ void k_guard_check(int T, Lattice L, float minimum, const uint32_t* _
 __global__ void k_guard_repair(int off, int cnt, Lattice L, float minimum, const uint32_t* __restrict__ tet_desc,
                                float4* __restrict__ pos, const int* __restrict__ flag) {
     if (!*flag) return;
-    int k = colour_item(L);
+    int k = ordered_item(L.xcd_order, GTS_BLOCK);
     if (k >= cnt) return;
     uint32_t d = tet_desc[off + k];
     if (!(d & 1u)) return;
@@ -1296,11 +1467,6 @@ int launch_refresh(const GtsTissueView* v, const Lattice& L, hipStream_t s) {
     return -(int)hipGetLastError();
 }
 
-// Threads per workgroup of the colour kernels. A colour of the 250k-tet kidney lattice holds about 8k tets: 64-thread
-// workgroups spread it over ~125 CUs instead of ~32, which measured 5.81 ms per step against 6.32 ms at 256 threads
-// and 7.14 ms at 512; 32-thread workgroups (half-empty waves on every CU) measured 6.10 ms.
-#define GTS_COLOUR_BLOCK 64
-inline int colour_block(const GtsTissueView* v) { return v->colour_block > 0 ? v->colour_block : GTS_COLOUR_BLOCK; }
 
 template <int MODE, bool DAMPED, bool PRECISE>
 void launch_tet_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
@@ -1371,6 +1537,36 @@ void launch_bend_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t 
     else if (write) GTS_BEND(false, true);
     else GTS_BEND(false, false);
 #undef GTS_BEND
+}
+
+// One iteration's tet sweep as the tiled launches: positions alternate between pos and pos_alt from launch to launch
+// (an even number of them, so the sweep ends in pos); the multipliers are read from the buffer the previous iteration
+// wrote and written to the other, the last iteration writing tet_lambda.
+template <int MODE, bool PRECISE>
+void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s, int it, int iters, float omega) {
+    const bool read = it > 0, write = iters > 1;
+    const int stride = 8 + v->tile_kmax;
+    float* lamW = ((iters - 1 - it) % 2 == 0) ? v->tet_lambda : v->lambda_alt;
+    float* lamR = ((iters - it) % 2 == 0) ? v->tet_lambda : v->lambda_alt;  // what iteration it - 1 wrote
+    const size_t lds = (size_t)v->tile_box * v->tile_box * v->tile_box * sizeof(float4);
+    for (int g = 0; g < v->tile_groups; g++) {
+        const int32_t* gi = v->tile_group_info + 8 * g;
+        const int k = gi[1], first = gi[2], regions = gi[3], cap = gi[6];
+        const float4* in = reinterpret_cast<const float4*>((g & 1) ? v->pos_alt : v->pos);
+        float4* out = reinterpret_cast<float4*>((g & 1) ? v->pos : v->pos_alt);
+        if (regions <= 0) continue;
+#define GTS_TILED(R, W)                                                                                           \
+    k_tets_tiled<MODE, R, W, PRECISE><<<regions, 512, lds, s>>>(                                                  \
+        k, cap, stride, v->tile_headers + (size_t)first * stride,                                                \
+        reinterpret_cast<const int2*>(v->tile_entries) + gi[7],                                                  \
+        reinterpret_cast<const int2*>(v->tile_writes), v->tile_box, L, omega, v->n_tets, v->tet_desc,            \
+        v->tet_alpha, lamR, lamW, in, out)
+        if (read && write) GTS_TILED(true, true);
+        else if (read) GTS_TILED(true, false);
+        else if (write) GTS_TILED(false, true);
+        else GTS_TILED(false, false);
+#undef GTS_TILED
+    }
 }
 
 void launch_bend_colour(const GtsTissueView* v, const Lattice& L, hipStream_t s, int off, int cnt, int it, int iters,
@@ -1519,6 +1715,8 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     // bending: compliance 1/(kb h^2) and its Kelvin-Voigt factor (xpbd_begin_frame, xpbd_solver.cu:2496-2499)
     bool bend;
     float alphaB, betaGammaHB;
+    // the tet sweep as tiled launches (GtsTissueView.tile_*)
+    bool tiled;
 };
 
 // cut_ribbon_tick without the pressure blade (cut.cu:1303-1431) after substep n: CCD of the edges moving from the
@@ -1600,6 +1798,23 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
         P.alphaB = 1.f / (p->bend_kb * h * h);
         P.betaGammaHB = P.alphaB * p->beta_b * h;
     }
+    // the tiled tet sweep: a plan for the current lists, undamped and unguarded (the legacy step turns it off)
+    P.tiled = false;
+    if (v->tile_groups > 0 && P.strain && P.dOverH == 0.f && P.bOverH == 0.f && P.guardPasses == 0 &&
+        v->tile_tets == v->n_tets && (v->tile_groups % 2) == 0 && v->tile_group_info && v->tile_headers &&
+        v->tile_entries && v->tile_writes && v->pos_alt && v->lambda_alt && v->tile_kmax >= 1 &&
+        v->tile_kmax <= GTS_TILE_KMAX &&
+        v->tile_box >= 2 && (long long)v->tile_box * v->tile_box * v->tile_box * 16 <= 65536) {
+        int next = 0;
+        bool ok = true;
+        for (int g = 0; g < v->tile_groups && ok; g++) {
+            const int32_t* gi = v->tile_group_info + 8 * g;
+            ok = gi[0] == next && gi[1] >= 1 && gi[1] <= v->tile_kmax && gi[3] >= 0 && gi[4] == v->tile_box &&
+                 gi[6] >= 1 && gi[7] >= 0;
+            next += gi[1];
+        }
+        P.tiled = ok && next == v->n_tet_colours;
+    }
     return GTS_OK;
 }
 
@@ -1669,7 +1884,14 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
                         if (cnt > 0) launch_bend_colour(v, L, s, off, cnt, it, passes, P.alphaB, P.omega,
                                                         P.betaGammaHB);
                     }
-                if (P.strain)
+                if (P.strain && P.tiled) {
+                    if (P.mode == 0 && P.precise) launch_tiled_sweep<0, true>(v, L, s, it, passes, P.omega);
+                    else if (P.mode == 0) launch_tiled_sweep<0, false>(v, L, s, it, passes, P.omega);
+                    else if (P.mode == 1 && P.precise) launch_tiled_sweep<1, true>(v, L, s, it, passes, P.omega);
+                    else if (P.mode == 1) launch_tiled_sweep<1, false>(v, L, s, it, passes, P.omega);
+                    else if (P.precise) launch_tiled_sweep<2, true>(v, L, s, it, passes, P.omega);
+                    else launch_tiled_sweep<2, false>(v, L, s, it, passes, P.omega);
+                } else if (P.strain)
                     for (int c = 0; c < v->n_tet_colours; c++) {
                         int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
                         if (cnt <= 0) continue;
@@ -1792,7 +2014,8 @@ hipStream_t g_capture = nullptr;
 std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
     const size_t ne = (size_t)v->n_edge_colours + 1, nt = (size_t)v->n_tet_colours + 1;
     const size_t nb = P.bend ? (size_t)v->n_bend_colours + 1 : 0;
-    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) + (ne + nt + nb) * sizeof(int32_t));
+    const size_t ng = P.tiled ? (size_t)8 * v->tile_groups : 0;
+    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) + (ne + nt + nb + ng) * sizeof(int32_t));
     unsigned char* d = k.data();
     std::memcpy(d, v, sizeof(GtsTissueView));
     d += sizeof(GtsTissueView);
@@ -1803,6 +2026,8 @@ std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) 
     std::memcpy(d, v->tet_colour_offsets, nt * sizeof(int32_t));
     d += nt * sizeof(int32_t);
     if (nb) std::memcpy(d, v->bend_colour_offsets, nb * sizeof(int32_t));
+    d += nb * sizeof(int32_t);
+    if (ng) std::memcpy(d, v->tile_group_info, ng * sizeof(int32_t));
     return k;
 }
 
@@ -2061,6 +2286,7 @@ int legacy_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspPa
     const int N = v->n_nodes;
     if (cut) { int rc_ = plan_tick(v, cut, emit_ncut, tear_tau, P); if (rc_) return rc_; }
     P.legacy = true;
+    P.tiled = false;  // the coupling passes of the legacy grasp keep the per-colour launches
     P.grasping = g->grasping != 0;
     P.gate = tool_armed != 0;
     P.K = P.grasping ? 2 : 1;  // a second material pass only while genuinely grasping (xpbd_solver.cu:3571)

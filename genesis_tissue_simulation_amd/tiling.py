@@ -1,0 +1,151 @@
+"""Tile plans for the temporally blocked tet sweep: several consecutive tet colours in one launch.
+
+The Gauss-Seidel sweep runs its tet colours one after another; a launch per colour costs about 4 us on MI355X whatever
+its size. A tiled launch takes K consecutive colours c0 .. c0+K-1 over cubic regions of S x S x S lattice cells, one
+workgroup per region. The workgroup loads the nodes of its region and a halo into LDS, then projects, pass by pass,
+  pass p:  the tets of colour c0+p whose cell lies within K-1-p cells of the region,
+so that every tet a later pass reads through its nodes has already been projected in this workgroup's LDS: a tet of
+colour c0+p touches nodes whose colour-(c0+p-1) tets lie within one more cell. Halo tets are projected by every region
+that needs them, from the same inputs with the same operations, so each copy equals the owner's. The owner -- the
+region holding the tet's cell -- writes the tet's multipliers; each node is written by the region holding the last
+tet of the launch's colours that touches it (whose LDS copy is final), or, untouched, by the region holding the node.
+
+Launches read positions from one buffer and write them to the other (a region's halo overlaps its neighbours' writes),
+so a sweep takes an even number of launches; the multipliers of a sweep are read from the previous iteration's buffer
+and written to the other for the same reason. The per-node sequence of updates is the per-colour sweep's, so the
+results are bit for bit those of one launch per colour.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# bit-order Kuhn corners (i = dx | dy << 1 | dz << 2) of the six types, as the kernels index them
+KUHN_BITS = ((0, 1, 3, 7), (0, 3, 2, 7), (0, 2, 6, 7), (0, 6, 4, 7), (0, 4, 5, 7), (0, 5, 1, 7))
+HDR = 8  # region header ints: box origin x, y, z, entry offset, write offset, write count, 2 unused; pass counts follow
+
+
+def group_sizes(colours, kmax):
+    """Consecutive colour groups of at most `kmax`, as few as possible but an even number of them (each launch swaps
+    the position buffers, so a sweep must end where it began)."""
+    if colours <= 0:
+        return []
+    n = -(-colours // kmax)
+    if n % 2:
+        n += 1
+    if n > colours:  # one colour: a sweep of one launch cannot end where it began
+        return None
+    base, extra = divmod(colours, n)
+    return [base + 1] * extra + [base] * (n - extra)
+
+
+def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
+    """Plan of the tiled sweep for a lattice of `dims` cells and its colour-major tet descriptors.
+
+    Returns a dict of numpy arrays:
+      groups  int32 [G, 8]: first colour, colours, first region header, regions, box width, halo of the box, pass
+              capacity (the most tets any region's pass holds), first entry
+      headers int32 [R, HDR + kmax]: box origin (x, y, z), entry offset, write offset, write count, 0, 0, then the
+              entry count of each pass
+      entries int32 [M, 2]: tet slot | owned << 31, the tet's corner-0 index in the region's LDS box. The regions of a
+              group take k * capacity rows each, in header order from the group's first entry, pass p at rows
+              p * capacity .. p * capacity + count - 1 (the rest zero), so a region finds its entries without its
+              header
+      writes  int32 [W + 1, 2]: LDS box index, node id, then one zero row
+    or None when the sweep cannot be tiled (fewer than two colours)."""
+    vx, vy, vz = (int(d) for d in dims)
+    nx, ny = vx + 1, vy + 1
+    nxy = nx * ny
+    desc = np.asarray(tet_desc, dtype=np.uint32)
+    offs = np.asarray(tet_offsets, dtype=np.int64)
+    sizes = group_sizes(len(offs) - 1, kmax)
+    if not sizes:
+        return None
+    S, H = int(region), int(kmax) - 1
+    B = S + 2 * H + 1
+    RX, RY, RZ = -(-vx // S), -(-vy // S), -(-vz // S)
+    base = (desc >> 4).astype(np.int64)
+    ttype = ((desc >> 1) & 7).astype(np.int64)
+    cx, cy, cz = base % nx, (base // nx) % ny, base // nxy
+    corner = np.array([[(c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy for c in k] for k in KUHN_BITS])
+    ids = base[:, None] + corner[ttype]
+    all_nodes = np.unique(ids)
+    ncx, ncy, ncz = all_nodes % nx, (all_nodes // nx) % ny, all_nodes // nxy
+    node_region = (np.minimum(ncx // S, RX - 1) + RX * (np.minimum(ncy // S, RY - 1) + RY * np.minimum(ncz // S, RZ - 1)))
+    offsets27 = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
+    groups, headers, entries, writes = [], [], [], []
+    n_entries = n_writes = 0
+    c0 = 0
+    for k in sizes:
+        rows = []  # (region, pass, slot | owned, local base)
+        writer = np.full(nx * ny * (vz + 1), -1, dtype=np.int64)
+        for p in range(k):
+            c = c0 + p
+            lo, hi = int(offs[c]), int(offs[c + 1])
+            if hi <= lo:
+                continue
+            h = k - 1 - p
+            slot = np.arange(lo, hi, dtype=np.int64)
+            x, y, z = cx[lo:hi], cy[lo:hi], cz[lo:hi]
+            rx, ry, rz = x // S, y // S, z // S
+            for dx, dy, dz in offsets27:
+                qx, qy, qz = rx + dx, ry + dy, rz + dz
+                ok = (qx >= 0) & (qx < RX) & (qy >= 0) & (qy < RY) & (qz >= 0) & (qz < RZ)
+                ok &= (x >= qx * S - h) & (x < (qx + 1) * S + h)
+                ok &= (y >= qy * S - h) & (y < (qy + 1) * S + h)
+                ok &= (z >= qz * S - h) & (z < (qz + 1) * S + h)
+                if not ok.any():
+                    continue
+                r = qx[ok] + RX * (qy[ok] + RY * qz[ok])
+                lb = (x[ok] - (qx[ok] * S - H)) + B * ((y[ok] - (qy[ok] * S - H)) + B * (z[ok] - (qz[ok] * S - H)))
+                owned = 1 if (dx, dy, dz) == (0, 0, 0) else 0
+                rows.append(np.stack((r, np.full(len(r), p), slot[ok] | (owned << 31), lb), axis=1))
+            # the last colour of the launch touching a node decides its writer: the region of that tet's cell
+            own = rx + RX * (ry + RY * rz)
+            for q in range(4):
+                writer[ids[lo:hi, q]] = own
+        rows = np.concatenate(rows) if rows else np.zeros((0, 4), dtype=np.int64)
+        order = np.lexsort((rows[:, 1], rows[:, 0]))
+        rows = rows[order]
+        # nodes of any tet that no tet of these colours touches are carried by the region holding them
+        wr = writer[all_nodes]
+        wr = np.where(wr < 0, node_region, wr)
+        # a region that owns no tet of these colours and writes no node computes nothing that is kept
+        owners = rows[rows[:, 2] >= (1 << 31), 0]
+        regions = np.union1d(np.unique(owners), np.unique(wr))
+        rows = rows[np.isin(rows[:, 0], regions)]
+        g_first, g_entry = len(headers), n_entries
+        per_region = []
+        for r in regions:
+            sel = rows[rows[:, 0] == r]
+            per_region.append((sel, np.bincount(sel[:, 1].astype(np.int64), minlength=k)))
+        cap = max(1, max(int(c.max()) for _, c in per_region))
+        for r, (sel, counts) in zip(regions, per_region):
+            rx, ry, rz = r % RX, (r // RX) % RY, r // (RX * RY)
+            ox, oy, oz = rx * S - H, ry * S - H, rz * S - H
+            nodes = all_nodes[wr == r]
+            lx, ly, lz = nodes % nx - ox, (nodes // nx) % ny - oy, nodes // nxy - oz
+            local = lx + B * (ly + B * lz)
+            hdr = np.zeros(HDR + kmax, dtype=np.int64)
+            hdr[:6] = (ox, oy, oz, n_entries, n_writes, len(nodes))
+            hdr[HDR:HDR + k] = counts
+            headers.append(hdr)
+            padded = np.zeros((k * cap, 2), dtype=np.int64)
+            pos = 0
+            for p in range(k):  # sel is sorted by pass
+                padded[p * cap:p * cap + counts[p]] = sel[pos:pos + counts[p], 2:4]
+                pos += counts[p]
+            entries.append(padded)
+            writes.append(np.stack((local, nodes), axis=1))
+            n_entries += k * cap
+            n_writes += len(nodes)
+        groups.append((c0, k, g_first, len(regions), B, H, cap, g_entry))
+        c0 += k
+    as32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64).astype(np.int32))
+    ent = np.concatenate(entries) if entries else np.zeros((0, 2), dtype=np.int64)
+    # slot | owned << 31 wraps to a negative int32, which the kernel reads back as the same 32 bits
+    ent32 = np.ascontiguousarray(ent.astype(np.uint64).astype(np.uint32).view(np.int32).reshape(-1, 2))
+    # one zero row after the last region's writes: a region whose write list is empty still reads a row in bounds
+    writes.append(np.zeros((1, 2), dtype=np.int64))
+    return dict(groups=as32(groups), headers=as32(np.stack(headers)), entries=ent32,
+                writes=as32(np.concatenate(writes)), region=S, kmax=int(kmax), box=B, halo=H,
+                tets=int(len(desc)))

@@ -119,7 +119,7 @@ class Tissue:
 
     def __init__(self, grid, params: SolverParams, k_dev=None, k_hyd=None, initial_positions=None,
                  device="cuda:0", topology=None, use_graph=True, max_substeps=64, storage=None, colour_block=0,
-                 xcd_order=True, persistent=False):
+                 xcd_order=True, persistent=False, tiles=None):
         """`storage` optionally supplies pre-allocated device tensors (pos, prev, vel, mass, active, pinned, nbr,
         edge_desc, edge_lambda, tet_desc, tet_k_dev, tet_k_hyd, tet_lambda, rot, rot_prev) -- views into the pooled
         buffers of a batch. `colour_block` is
@@ -127,7 +127,9 @@ class Tissue:
         contiguous range of every colour (same results). `persistent`: a small tissue (every node in one workgroup's
         LDS, at most PERSISTENT_COLOUR_TETS tets per colour) steps without tool contact, guard, bending, edge damping or
         cut tool as one persistent workgroup running the whole step (the batched kernel over this tissue alone):
-        colours are separated by workgroup barriers instead of launches, with the same results."""
+        colours are separated by workgroup barriers instead of launches, with the same results. `tiles`: (region,
+        kmax) runs each tet sweep as tiled launches of up to kmax colours over regions of region^3 cells (see
+        set_tiles); None keeps one launch per colour."""
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -244,6 +246,7 @@ class Tissue:
         self._solo = None             # the one-instance batched view of the persistent step, and what it was built for
         self._liveness_fresh = False  # liveness derived since initialisation (the persistent step does not derive it)
         self.gravity = (0.0, 0.0, 0.0)
+        self._tiles = None            # the tiled sweep's plan and its device arrays
         self._staging = None          # pinned host buffers of observation()
         self.cut_tool = None          # LCS_SetTool's tool, arming the CCD ribbon tick of every substep
         self.emit_ncut = np.array([0.0, 1.0, 0.0], dtype=np.float32)
@@ -251,6 +254,40 @@ class Tissue:
         self.contact_model = 1
         self.grasp_event = 0
         check("gts_init_state", self.lib.gts_init_state(ct.byref(self.view), current_stream()))
+        if tiles is not None:
+            self.set_tiles(*tiles)
+
+    def set_tiles(self, region=8, kmax=3):
+        """Run each tet sweep of an undamped, unguarded step as tiled launches: up to `kmax` consecutive colours per
+        launch, one workgroup per region of region^3 lattice cells (tiling.build_tile_plan); the results are the
+        per-colour sweep's. region=None turns it off. The plan holds for the current tet lists: after a compaction the
+        step takes one launch per colour again until set_tiles is called anew."""
+        from . import tiling
+        v = self.view
+        if region is None:
+            self._tiles = None
+            v.tile_groups = 0
+            return None
+        plan = tiling.build_tile_plan(self.grid.dims, self.tet_desc[:self.T].cpu().numpy().view(np.uint32),
+                                      self.tet_offsets, region=region, kmax=kmax)
+        if plan is None:
+            self._tiles = None
+            v.tile_groups = 0
+            return None
+        dev = self.device
+        keep = dict(plan=plan, groups=np.ascontiguousarray(plan["groups"].reshape(-1)),
+                    headers=torch.as_tensor(plan["headers"].reshape(-1)).to(dev),
+                    entries=torch.as_tensor(plan["entries"].reshape(-1)).to(dev),
+                    writes=torch.as_tensor(plan["writes"].reshape(-1)).to(dev),
+                    pos_alt=self.pos.clone(), lambda_alt=torch.zeros_like(self.tet_lambda))
+        self._tiles = keep
+        v.tile_groups = len(plan["groups"])
+        v.tile_kmax, v.tile_box, v.tile_tets = plan["kmax"], plan["box"], plan["tets"]
+        v.tile_group_info = keep["groups"].ctypes.data
+        v.tile_headers, v.tile_entries = keep["headers"].data_ptr(), keep["entries"].data_ptr()
+        v.tile_writes = keep["writes"].data_ptr()
+        v.pos_alt, v.lambda_alt = keep["pos_alt"].data_ptr(), keep["lambda_alt"].data_ptr()
+        return plan
 
     def set_contact_model(self, model, grasp=None):
         """xpbd_set_contact_model: 0 = legacy grasp (capture, slab ring, core pins, break check) with the CCD gate,

@@ -154,6 +154,26 @@ typedef struct GtsTissueView {
     const int32_t* bend_colour_offsets; /* host, [n_bend_colours + 1] */
     const uint32_t* bend_desc;          /* [n_bends] */
     float* bend_lambda;                 /* [n_bends] */
+    /* Tiled tet sweep (no reference counterpart; the results are the per-colour sweep's bit for bit): with
+     * tile_groups > 0 an undamped, unguarded step without the legacy grasp runs its tet colours in tile_groups launches
+     * of up to tile_kmax consecutive colours each, one workgroup per lattice region, positions in an LDS box of
+     * tile_box^3 nodes around the region (the plan: genesis_tissue_simulation_amd/tiling.py). Each launch reads the
+     * positions from one buffer and writes them to the other (pos, pos_alt alternate; tile_groups is even), and a
+     * sweep reads the multipliers of the previous iteration from one buffer and writes the other (tet_lambda and
+     * lambda_alt; the last iteration writes tet_lambda). The plan holds for tile_tets tets in the current colour
+     * lists; a view whose lists differ (a compaction) takes the per-colour launches. */
+    int tile_groups;
+    int tile_kmax, tile_box, tile_tets;
+    const int32_t* tile_group_info;     /* host [tile_groups * 8]: first colour, colours, first header, regions, box
+                                         * width, box halo, pass capacity, first entry */
+    const int32_t* tile_headers;        /* [regions * (8 + tile_kmax)]: box origin x, y, z, entry offset, write offset,
+                                         * write count, 0, 0, entries of each pass */
+    const int32_t* tile_entries;        /* [2 * entries]: tet slot | owned << 31, corner-0 index in the LDS box; a
+                                         * region's pass p holds rows p * capacity .. p * capacity + count - 1 of
+                                         * its k * capacity rows, which start at first entry + region * k * capacity */
+    const int32_t* tile_writes;         /* [2 * writes]: LDS box index, node id */
+    float* pos_alt;                     /* [4N] the second position buffer */
+    float* lambda_alt;                  /* [2T] the second multiplier buffer */
 } GtsTissueView;
 
 /* Ribbon ticks a released grasp-ring edge stays exempt from the tear law while still longer than 0.8 tau L

@@ -2,15 +2,16 @@
 per-tet compliance, the 2 mm capsule grasper pressing in with friction contact, the seeded cubic-Bezier blade cutting
 after every step, 20 substeps of 2 iterations, gravity ramping in. Three steps on the GPU against the oracle on the
 same seeded inputs: positions, velocities, the live neighbour table, cut masks, cut points and tet / edge liveness
-bit-exact, in the reference's direct tet forms and in the precise ones; the FP64 oracle's error is reported."""
+bit-exact, in the reference's direct tet forms and in the precise ones, and with the tet sweep tiled (three colours per
+launch over regions of 8^3 cells); the FP64 oracle's error is reported."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precise", [False, True])
-def test_kidney_full_size_matches_oracle(precise):
+@pytest.mark.parametrize("precise,tiles", [(False, None), (True, None), (True, (8, 3))])
+def test_kidney_full_size_matches_oracle(precise, tiles):
     import torch
 
     from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
@@ -20,8 +21,9 @@ def test_kidney_full_size_matches_oracle(precise):
     w = kidney_250k(precise=precise)
     kd, kh = w.stiffness()
     assert tet_count(w.grid) > 240_000 and np.std(np.log(1.0 / kd)) > 0.4  # full size, per-tet compliance varies
-    print(f"kidney full size: {tet_count(w.grid)} tets, precise {precise}", flush=True)
-    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions)
+    print(f"kidney full size: {tet_count(w.grid)} tets, precise {precise}, tiles {tiles}", flush=True)
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, tiles=tiles)
+    assert tiles is None or t.view.tile_groups > 0
     oracles = {}
     for double in (False, True):
         o = orc.Oracle(w.grid, w.params, w.initial_positions, double=double)

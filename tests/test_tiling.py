@@ -1,0 +1,124 @@
+"""The tiled tet sweep's plan (genesis_tissue_simulation_amd/tiling.py), checked by brute force on small lattices (no
+GPU): the properties that make a launch of several colours compute what one launch per colour computes.
+
+  * The groups cover the colours in order, at most kmax each, in an even number of launches.
+  * Every tet of a group's colours is owned by exactly one region, the one holding its cell, and listed in a region's
+    pass p exactly when its cell lies within k-1-p cells of the region.
+  * Halo completeness: for every region and pass p > 0, every tet of colour c0+p-1 that shares a node with a tet of
+    the region's pass p is in the region's pass p-1 (so every node a pass reads is already final in LDS).
+  * Every node of every tet is written by exactly one region per launch: the region of the last tet of the launch's
+    colours touching it, else the region holding the node; the LDS box indices of entries and writes are in range and
+    name the right nodes.
+"""
+import numpy as np
+import pytest
+
+from genesis_tissue_simulation_amd import tiling
+from genesis_tissue_simulation_amd.grid import block_grid
+
+
+def topo_of(g):
+    from genesis_tissue_simulation_amd.tissue import build_topology
+    return build_topology(g, bends=False)
+
+
+@pytest.mark.parametrize("nodes,region,kmax", [((10, 7, 6), 3, 2), ((10, 7, 6), 3, 3), ((13, 9, 8), 4, 4),
+                                               ((9, 9, 9), 2, 3)])
+def test_plan_properties(nodes, region, kmax):
+    g = block_grid(nodes, 0.25)
+    g.active[np.arange(len(g.active)) % 11 == 3] = 0  # holes: cells without tets
+    topo = topo_of(g)
+    desc, offs = topo["tet_desc"], topo["tet_offsets"]
+    plan = tiling.build_tile_plan(g.dims, desc, offs, region=region, kmax=kmax)
+    vx, vy, vz = g.dims
+    nx, ny = vx + 1, vy + 1
+    nxy = nx * ny
+    S, B, H = plan["region"], plan["box"], plan["halo"]
+    RX, RY = -(-vx // S), -(-vy // S)
+    base = (desc >> 4).astype(np.int64)
+    typ = ((desc >> 1) & 7).astype(np.int64)
+    cell = np.stack((base % nx, (base // nx) % ny, base // nxy), axis=1)
+    corner = np.array([[(c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy for c in k] for k in tiling.KUHN_BITS])
+    ids = base[:, None] + corner[typ]
+    colour = np.repeat(np.arange(len(offs) - 1), np.diff(offs))
+    tet_nodes = np.unique(ids)
+    groups = plan["groups"]
+    assert len(groups) % 2 == 0 and groups[0, 0] == 0
+    assert np.all(groups[1:, 0] == groups[:-1, 0] + groups[:-1, 1]) and groups[-1, 0] + groups[-1, 1] == len(offs) - 1
+    assert np.all((groups[:, 1] >= 1) & (groups[:, 1] <= kmax))
+    hdrs, ents, wrs = plan["headers"], plan["entries"].view(np.uint32), plan["writes"]
+    assert groups.shape[1] == 8
+    for c0, k, first, nreg, box, halo, cap, entry0 in groups:
+        assert box == B and halo == H
+        assert hdrs[first, 3] == entry0
+        owned_count = np.zeros(len(desc), dtype=np.int64)
+        written = np.zeros(nx * ny * (vz + 1), dtype=np.int64)
+        lists = {}
+        for r in range(first, first + nreg):
+            h = hdrs[r]
+            o = h[:3].astype(np.int64)
+            region_xyz = (o + H) // S
+            rid = region_xyz[0] + RX * (region_xyz[1] + RY * region_xyz[2])
+            e = int(h[3])
+            assert e == entry0 + (r - first) * k * cap  # found without the header
+            assert np.all(h[tiling.HDR:tiling.HDR + k] <= cap) and h[tiling.HDR:tiling.HDR + k].max() >= 0
+            per_pass = []
+            for p in range(k):
+                cnt = int(h[tiling.HDR + p])
+                rows = ents[e + p * cap:e + p * cap + cnt]
+                assert np.all(ents[e + p * cap + cnt:e + (p + 1) * cap] == 0)
+                slots = (rows[:, 0] & 0x7fffffff).astype(np.int64)
+                owned = (rows[:, 0] >> 31).astype(bool)
+                assert np.all(colour[slots] == c0 + p)
+                lo, hi = region_xyz * S - (k - 1 - p), (region_xyz + 1) * S + (k - 1 - p)
+                inside = np.all((cell >= lo) & (cell < hi), axis=1) & (colour == c0 + p)
+                np.testing.assert_array_equal(np.sort(slots), np.flatnonzero(inside))
+                np.testing.assert_array_equal(owned, np.all(cell[slots] // S == region_xyz, axis=1))
+                owned_count[slots[owned]] += 1
+                # the LDS box index of corner 0
+                lb = rows[:, 1].astype(np.int64)
+                rel = cell[slots] - o
+                np.testing.assert_array_equal(lb, rel[:, 0] + B * (rel[:, 1] + B * rel[:, 2]))
+                assert np.all((rel >= 0) & (rel + 1 < B))
+                per_pass.append(set(slots.tolist()))
+            lists[rid] = per_pass
+            # halo completeness
+            for p in range(1, k):
+                need = set()
+                nodes_p = set(ids[list(per_pass[p])].reshape(-1).tolist()) if per_pass[p] else set()
+                prev = np.flatnonzero(colour == c0 + p - 1)
+                for t in prev:
+                    if nodes_p.intersection(ids[t].tolist()):
+                        need.add(int(t))
+                assert need <= per_pass[p - 1], (rid, p)
+            w0, wn = int(h[4]), int(h[5])
+            w = wrs[w0:w0 + wn].astype(np.int64)
+            written[w[:, 1]] += 1
+            rel = np.stack((w[:, 1] % nx, (w[:, 1] // nx) % ny, w[:, 1] // nxy), axis=1) - o
+            assert np.all((rel >= 0) & (rel < B))
+            np.testing.assert_array_equal(w[:, 0], rel[:, 0] + B * (rel[:, 1] + B * rel[:, 2]))
+            # a written node touched by the launch's colours: this region projects its last toucher
+            for node in w[:, 1]:
+                touch = np.flatnonzero(np.any(ids == node, axis=1) & (colour >= c0) & (colour < c0 + k))
+                if len(touch):
+                    last = touch[np.argmax(colour[touch])]
+                    assert last in per_pass[colour[last] - c0]
+                    assert np.all(cell[last] // S == region_xyz)
+        mine = (colour >= c0) & (colour < c0 + k)
+        assert np.all(owned_count[mine] == 1) and np.all(owned_count[~mine] == 0)
+        assert np.all(written[tet_nodes] == 1)
+        others = np.ones(len(written), bool)
+        others[tet_nodes] = False
+        assert np.all(written[others] == 0)
+
+
+def test_group_sizes_are_even_and_bounded():
+    for colours in range(2, 70):
+        for kmax in (1, 2, 3, 4, 5):
+            sizes = tiling.group_sizes(colours, kmax)
+            if kmax == 1 and colours % 2:  # one colour a launch, an odd number of them: no even split
+                assert sizes is None
+                continue
+            assert sum(sizes) == colours and len(sizes) % 2 == 0 and max(sizes) <= kmax
+            assert len(sizes) <= -(-colours // kmax) + 1
+    assert tiling.group_sizes(1, 3) is None
