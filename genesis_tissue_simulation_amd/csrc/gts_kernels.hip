@@ -374,6 +374,7 @@ __global__ void __launch_bounds__(512) k_tets_tiled(int k, int cap, int hdr_stri
         en[p] = ent[(p < k ? p : 0) * cap + row];
     }
     const int ox = h[0], oy = h[1], oz = h[2], wbeg = h[4], wcnt = h[5];
+    if (wcnt == 0 && cnt[0] + cnt[1] + cnt[2] + cnt[3] == 0) return;  // idle in this launch (the whole workgroup)
     // The second round, all in flight together: this thread's first tet of every pass, its first node writes and
     // its nodes of the box (box <= 16 and 512 threads: at most 8 a thread).
     TiledTetInputs pre[GTS_TILE_KMAX];

@@ -72,8 +72,7 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
     ncx, ncy, ncz = all_nodes % nx, (all_nodes // nx) % ny, all_nodes // nxy
     node_region = (np.minimum(ncx // S, RX - 1) + RX * (np.minimum(ncy // S, RY - 1) + RY * np.minimum(ncz // S, RZ - 1)))
     offsets27 = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1)]
-    groups, headers, entries, writes = [], [], [], []
-    n_entries = n_writes = 0
+    per_group = []  # (first colour, colours, rows, node writers)
     c0 = 0
     for k in sizes:
         rows = []  # (region, pass, slot | owned, local base)
@@ -104,15 +103,23 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
             for q in range(4):
                 writer[ids[lo:hi, q]] = own
         rows = np.concatenate(rows) if rows else np.zeros((0, 4), dtype=np.int64)
-        order = np.lexsort((rows[:, 1], rows[:, 0]))
-        rows = rows[order]
+        rows = rows[np.lexsort((rows[:, 1], rows[:, 0]))]
         # nodes of any tet that no tet of these colours touches are carried by the region holding them
         wr = writer[all_nodes]
         wr = np.where(wr < 0, node_region, wr)
-        # a region that owns no tet of these colours and writes no node computes nothing that is kept
-        owners = rows[rows[:, 2] >= (1 << 31), 0]
-        regions = np.union1d(np.unique(owners), np.unique(wr))
-        rows = rows[np.isin(rows[:, 0], regions)]
+        per_group.append((c0, k, rows, wr))
+        c0 += k
+    # Every launch takes the same regions in the same order -- those owning a tet or writing a node in any launch (a
+    # region owning nothing and writing nothing computes nothing that is kept) -- so that a region lands on the same
+    # XCD in every launch and finds what it wrote last launch in that XCD's L2. A region idle in one launch has no
+    # entries and no writes there and returns at once.
+    regions = np.unique(np.concatenate([np.concatenate((rows[rows[:, 2] >= (1 << 31), 0], wr))
+                                        for _, _, rows, wr in per_group]))
+    groups, headers, entries, writes = [], [], [], []
+    n_entries = n_writes = 0
+    for c0, k, rows, wr in per_group:
+        active = np.union1d(rows[rows[:, 2] >= (1 << 31), 0], wr)
+        rows = rows[np.isin(rows[:, 0], active)]
         g_first, g_entry = len(headers), n_entries
         per_region = []
         for r in regions:
@@ -139,7 +146,6 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
             n_entries += k * cap
             n_writes += len(nodes)
         groups.append((c0, k, g_first, len(regions), B, H, cap, g_entry))
-        c0 += k
     as32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64).astype(np.int32))
     ent = np.concatenate(entries) if entries else np.zeros((0, 2), dtype=np.int64)
     # slot | owned << 31 wraps to a negative int32, which the kernel reads back as the same 32 bits

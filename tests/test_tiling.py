@@ -2,8 +2,9 @@
 GPU): the properties that make a launch of several colours compute what one launch per colour computes.
 
   * The groups cover the colours in order, at most kmax each, in an even number of launches.
-  * Every tet of a group's colours is owned by exactly one region, the one holding its cell, and listed in a region's
-    pass p exactly when its cell lies within k-1-p cells of the region.
+  * Every launch takes the same regions in the same order. Every tet of a group's colours is owned by exactly one
+    region, the one holding its cell, and listed in a region's pass p exactly when its cell lies within k-1-p cells of
+    the region, unless the region owns no tet and writes no node in that launch (then it lists nothing).
   * Halo completeness: for every region and pass p > 0, every tet of colour c0+p-1 that shares a node with a tet of
     the region's pass p is in the region's pass p-1 (so every node a pass reads is already final in LDS).
   * Every node of every tet is written by exactly one region per launch: the region of the last tet of the launch's
@@ -47,6 +48,9 @@ def test_plan_properties(nodes, region, kmax):
     assert np.all(groups[1:, 0] == groups[:-1, 0] + groups[:-1, 1]) and groups[-1, 0] + groups[-1, 1] == len(offs) - 1
     assert np.all((groups[:, 1] >= 1) & (groups[:, 1] <= kmax))
     hdrs, ents, wrs = plan["headers"], plan["entries"].view(np.uint32), plan["writes"]
+    # every launch takes the same regions in the same order
+    origins = [hdrs[f:f + n, :3] for f, n in zip(groups[:, 2], groups[:, 3])]
+    assert all(np.array_equal(o, origins[0]) for o in origins)
     assert groups.shape[1] == 8
     for c0, k, first, nreg, box, halo, cap, entry0 in groups:
         assert box == B and halo == H
@@ -54,6 +58,7 @@ def test_plan_properties(nodes, region, kmax):
         owned_count = np.zeros(len(desc), dtype=np.int64)
         written = np.zeros(nx * ny * (vz + 1), dtype=np.int64)
         lists = {}
+        regions_idle = 0
         for r in range(first, first + nreg):
             h = hdrs[r]
             o = h[:3].astype(np.int64)
@@ -63,6 +68,12 @@ def test_plan_properties(nodes, region, kmax):
             assert e == entry0 + (r - first) * k * cap  # found without the header
             assert np.all(h[tiling.HDR:tiling.HDR + k] <= cap) and h[tiling.HDR:tiling.HDR + k].max() >= 0
             per_pass = []
+            # a region that owns no tet of these colours and writes no node is idle in this launch: no entries
+            mine_here = np.all(cell // S == region_xyz, axis=1) & (colour >= c0) & (colour < c0 + k)
+            idle = not mine_here.any() and int(h[5]) == 0
+            if idle:
+                assert np.all(h[tiling.HDR:tiling.HDR + k] == 0)
+                regions_idle += 1
             for p in range(k):
                 cnt = int(h[tiling.HDR + p])
                 rows = ents[e + p * cap:e + p * cap + cnt]
@@ -72,7 +83,7 @@ def test_plan_properties(nodes, region, kmax):
                 assert np.all(colour[slots] == c0 + p)
                 lo, hi = region_xyz * S - (k - 1 - p), (region_xyz + 1) * S + (k - 1 - p)
                 inside = np.all((cell >= lo) & (cell < hi), axis=1) & (colour == c0 + p)
-                np.testing.assert_array_equal(np.sort(slots), np.flatnonzero(inside))
+                np.testing.assert_array_equal(np.sort(slots), [] if idle else np.flatnonzero(inside))
                 np.testing.assert_array_equal(owned, np.all(cell[slots] // S == region_xyz, axis=1))
                 owned_count[slots[owned]] += 1
                 # the LDS box index of corner 0
