@@ -36,18 +36,18 @@ GTS_HD int kuhn_corner(int type, int k) {
 // Node offset of bit-order corner c in a lattice with nx, nxy points per row / layer.
 GTS_HD int corner_offset(int c, int nx, int nxy) { return (c & 1) + ((c >> 1) & 1) * nx + ((c >> 2) & 1) * nxy; }
 
-// J^(-2/3) for J >= 0.05: a bit-level cube-root seed, three Newton steps c -= (c^3 - J)/(3c^2), then 1/c^2. The
-// reference evaluates powf(J, -2/3) (xpbd/xpbd_solver.cu:773); this fixed IEEE sequence stays within a few ulp of it
-// and is reproduced exactly by the CPU checker.
+// J^(-2/3) for J >= 0.05: y = J^(-1/3) from a bit-level seed (K - bits(J) / 3, within 3.5%) and three Newton steps
+// y += y (1 - J y^3) / 3 on y^-3 = J, which need no division, then y^2. The reference evaluates powf(J, -2/3)
+// (xpbd/xpbd_solver.cu:773); this fixed IEEE sequence stays within 3 ulp of the exact power and is reproduced exactly
+// by the CPU checker.
 GTS_HD float iso_pow(float J) {
     uint32_t i = __builtin_bit_cast(uint32_t, J);
-    i = i / 3u + 0x2a514067u;
-    float c = __builtin_bit_cast(float, i);
+    float y = __builtin_bit_cast(float, 0x54a23280u - i / 3u);
     for (int k = 0; k < 3; k++) {
-        float c2 = c * c;
-        c = c - (c2 * c - J) / (3.f * c2);
+        const float r = 1.f - J * (y * y * y);
+        y = y + y * r * 0.333333343f;
     }
-    return 1.f / (c * c);
+    return y * y;
 }
 
 // Natural log for the guarded bulk strain log J (the reference calls logf, xpbd/xpbd_solver.cu:780): split x = m 2^e

@@ -46,20 +46,21 @@ template <class R> inline V3<R> cross3(V3<R> a, V3<R> b) {
 template <class R> inline R comp(V3<R> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 // J^(-2/3) of k_solve_tet (the reference calls powf, xpbd_solver.cu:773). The float oracle and the HIP kernel both
-// evaluate it with this fixed sequence of IEEE operations -- a bit-level cube-root seed and three Newton steps, then
-// 1/c^2 -- so the two agree bit for bit; it stays within a few ulp of the exact power for J >= 0.05. The double
-// oracle uses the library power.
+// evaluate it with this fixed sequence of IEEE operations -- a bit-level seed of J^(-1/3), three division-free Newton
+// steps y += y (1 - J y^3) / 3, then y^2 -- so the two agree bit for bit; it stays within 3 ulp of the exact power for
+// J >= 0.05. The double oracle uses the library power.
 inline float isoPow(float J) {
     uint32_t i;
     std::memcpy(&i, &J, 4);
-    i = i / 3u + 0x2a514067u;
-    float c;
-    std::memcpy(&c, &i, 4);
+    i = 0x54a23280u - i / 3u;
+    float y;
+    std::memcpy(&y, &i, 4);
     for (int k = 0; k < 3; k++) {
-        float c2 = c * c;
-        c = c - (c2 * c - J) / (3.f * c2);
+        float y3 = y * y * y;
+        float r = 1.f - J * y3;
+        y = y + y * r * 0.333333343f;
     }
-    return 1.f / (c * c);
+    return y * y;
 }
 inline double isoPow(double J) { return std::pow(J, -2.0 / 3.0); }
 
