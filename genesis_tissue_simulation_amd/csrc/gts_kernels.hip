@@ -67,26 +67,24 @@ __device__ __forceinline__ int axis_stride(const Lattice& L, int axis) { return 
 
 __device__ __forceinline__ f3 xyz(float4 p) { return mk3(p.x, p.y, p.z); }
 
-// The item a thread of a colour kernel takes. Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on
-// XCD b % 8). With L.xcd_order the workgroups of one XCD take one contiguous range of the colour; colours list their
-// items in lattice order, so an XCD then meets the same slab of the lattice in every colour, and that slab's nodes can
-// stay in the XCD's L2 between launches. The mapping is a bijection, so results do not depend on it.
-// `block` is the launch's workgroup size, passed in: reading blockDim costs a load from the dispatch packet before the
-// first load that depends on the item.
+// Marks kernel arguments as needed at this point: their scalar loads issue together before it, not one round at a time
+// before each first use. No instruction is emitted.
 __device__ __forceinline__ void asm_use(int x) { asm volatile("" ::"s"(x)); }
 __device__ __forceinline__ void asm_use(float x) { asm volatile("" ::"s"(x)); }
 __device__ __forceinline__ void asm_use(unsigned x) { asm volatile("" ::"s"(x)); }
 template <class T>
 __device__ __forceinline__ void asm_use(T* p) { asm volatile("" ::"s"(p)); }
-
-// Marks kernel arguments as needed at this point: their scalar loads issue together before it, not one round at a time
-// before each first use. No instruction is emitted.
 template <class... A>
 __device__ __forceinline__ void args_ready(const A&... a) {
     (asm_use(a), ...);
 }
 
-// Branch-free, so that the kernel's argument loads issue together and its first data load waits on one round of them.
+// The item a thread of a colour kernel takes. Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on
+// XCD b % 8). With L.xcd_order the workgroups of one XCD take one contiguous range of the colour; colours list their
+// items in lattice order, so an XCD then meets the same slab of the lattice in every colour, and that slab's nodes can
+// stay in the XCD's L2 between launches. The mapping is a bijection, so results do not depend on it. `block` is the
+// launch's workgroup size, passed in (reading blockDim costs a load from the dispatch packet before the first load that
+// depends on the item); the computation is branch-free, so that the kernel's argument loads issue together.
 __device__ __forceinline__ int ordered_item(int xcd_order, int block) {
     const int b = blockIdx.x;
     const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, slot = b >> 3;
@@ -449,16 +447,27 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
                                float4* __restrict__ prev, float4* __restrict__ vel, float* reaction,
                                int xcd_order, float h = 0.f, const float4* __restrict__ ext = nullptr) {
-    int i = ordered_item(xcd_order, GTS_BLOCK);
+    args_ready(N, xcd_order, (int)gridDim.x, shapes_p, pos, prev);
+    const int i = ordered_item(xcd_order, GTS_BLOCK), ic = i < N ? i : N - 1;
+    const float4 p4 = pos[ic], pv = prev[ic];
+    // The substep's tool shapes, staged in LDS by one round of loads beside the node's: read in place, each field
+    // would be a scalar load issued only after the branch that needs it.
+    __shared__ StepShapes staged;
+    if (CONTACT) {
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(shapes_p);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(&staged);
+        static_assert(sizeof(StepShapes) / 4 <= GTS_BLOCK, "one word of the shapes a thread");
+        if (threadIdx.x < sizeof(StepShapes) / 4) dst[threadIdx.x] = src[threadIdx.x];
+        __syncthreads();
+    }
     f3 force = mk3(0, 0, 0);
     float contacts = 0.f;
     if (i < N) {
-        float4 p4 = pos[i];
         float w = p4.w;
         f3 x = xyz(p4);
-        f3 xp = xyz(prev[i]);
+        f3 xp = xyz(pv);
         if (CONTACT && w > 0.f) {
-            const StepShapes& shapes = *shapes_p;
+            const StepShapes& shapes = staged;
             ContactRecord rec[4];
             float lamN[4];
             f3 lamT[4];
