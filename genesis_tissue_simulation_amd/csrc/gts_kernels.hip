@@ -78,6 +78,27 @@ template <class... A>
 __device__ __forceinline__ void args_ready(const A&... a) {
     (asm_use(a), ...);
 }
+// The same for loaded values: the loads issued before this point complete here, together.
+__device__ __forceinline__ void value_use(float x) { asm volatile("" ::"v"(x)); }
+__device__ __forceinline__ void value_use(float4 x) {
+    value_use(x.x);
+    value_use(x.y);
+    value_use(x.z);
+    value_use(x.w);
+}
+template <class... A>
+__device__ __forceinline__ void values_ready(const A&... a) {
+    (value_use(a), ...);
+}
+
+// An edge's two nodes, loaded ahead: reads return them, writes go to memory.
+struct NodePair {
+    float4* p;
+    int i;
+    float4 a, b;
+    __device__ __forceinline__ float4 ld(int k) const { return k == i ? a : b; }
+    __device__ __forceinline__ void st(int k, float4 v) const { p[k] = v; }
+};
 
 // The item a thread of a colour kernel takes. Workgroups are dispatched round-robin over the 8 XCDs (workgroup b on
 // XCD b % 8). With L.xcd_order the workgroups of one XCD take one contiguous range of the colour; colours list their
@@ -224,12 +245,18 @@ __global__ void k_edges(int off, int cnt, Lattice L, float alphaTilde, float fal
         if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = 0.f;
         return;
     }
-    int axis = (int)((d >> 1) & 3u);
-    float restLen = L.rest_len[axis];
+    const int axis = (int)((d >> 1) & 3u);
+    const float restLen = axis == 0 ? L.rest_len[0] : (axis == 1 ? L.rest_len[1] : L.rest_len[2]);  // no indexed load
     if (FALLBACK) alphaTilde = fallbackInvEh2 / restLen;
-    int i = (int)(d >> 4), j = i + axis_stride(L, axis);
-    float lam = READ_LAMBDA ? lambda[e] : 0.f, out;
-    const bool ok = edge_project<DAMPED>(GlobalNodes{pos}, prev, i, j, restLen, alphaTilde, omega, betaGammaH, lam, out);
+    const int i = (int)(d >> 4), j = i + axis_stride(L, axis);
+    // Both nodes and the multiplier in one round of loads: left to the compiler, the inverse masses would be read first
+    // and the coordinates and the multiplier only past the tests that need them, one round each.
+    const float4 pi = pos[i], pj = pos[j];
+    const float lam = READ_LAMBDA ? lambda[e] : 0.f;
+    values_ready(pi, pj, lam);
+    float out;
+    const bool ok = edge_project<DAMPED>(NodePair{pos, i, pi, pj}, prev, i, j, restLen, alphaTilde, omega, betaGammaH,
+                                         lam, out);
     // a row that writes nothing in the first iteration keeps the substep's reset multiplier, 0 (xpbd_solver.cu:3534)
     if (WRITE_LAMBDA && (ok || !READ_LAMBDA)) lambda[e] = ok ? out : 0.f;
 }
