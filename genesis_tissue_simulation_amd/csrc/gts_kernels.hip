@@ -91,6 +91,15 @@ __device__ __forceinline__ void values_ready(const A&... a) {
     (value_use(a), ...);
 }
 
+// A bend element's three nodes, loaded ahead, as NodePair.
+struct NodeTriple {
+    float4* p;
+    int i, j;
+    float4 a, b, c;
+    __device__ __forceinline__ float4 ld(int k) const { return k == i ? a : (k == j ? b : c); }
+    __device__ __forceinline__ void st(int k, float4 v) const { p[k] = v; }
+};
+
 // An edge's two nodes, loaded ahead: reads return them, writes go to memory.
 struct NodePair {
     float4* p;
@@ -268,21 +277,29 @@ template <bool READ_LAMBDA, bool WRITE_LAMBDA, bool DAMPED>
 __global__ void k_bends(int off, int cnt, Lattice L, float alphaB, float omega, float betaGammaH,
                         const uint32_t* __restrict__ bend_desc, const uint32_t* __restrict__ bend_alive,
                         float* __restrict__ lambda, float4* __restrict__ pos, const float4* __restrict__ prev) {
-    int q = colour_item(L);
+    args_ready(off, cnt, bend_desc, bend_alive, L.xcd_order, L.block, (int)gridDim.x, L.nx, L.nxy, pos, lambda, prev);
+    const int q = colour_item(L);
+    const uint32_t d = bend_desc[off + (q < cnt ? q : cnt - 1)];  // issued before the range check, as in k_tets
     if (q >= cnt) return;
-    int e = off + q;
-    uint32_t d = bend_desc[e];
-    int i = (int)(d >> 4), pair = (int)(d & 15u);
-    if (!((bend_alive[i] >> pair) & 1u)) {  // a dead element keeps the substep's reset multiplier, 0 (:3535)
-        if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = 0.f;
-        return;
-    }
+    const int e = off + q;
+    const int i = (int)(d >> 4), pair = (int)(d & 15u);
     const int sa = bend_slot_a(pair), sb = bend_slot_b(pair);
     const int da = axis_stride(L, sa >> 1), db = axis_stride(L, sb >> 1);
     const int j = i + ((sa & 1) ? da : -da), k = i + ((sb & 1) ? db : -db);
+    // the liveness word, the three nodes and the multiplier in one round of loads (the nodes exist whether or not the
+    // element is live: the builder keeps only pairs whose arms are lattice neighbours)
+    const uint32_t alive = bend_alive[i];
+    const float4 pa = pos[i], pb = pos[j], pc = pos[k];
+    const float lam = READ_LAMBDA ? lambda[e] : 0.f;
+    values_ready(pa, pb, pc, lam);
+    if (!((alive >> pair) & 1u)) {  // a dead element keeps the substep's reset multiplier, 0 (:3535)
+        if (WRITE_LAMBDA && !READ_LAMBDA) lambda[e] = 0.f;
+        return;
+    }
     const float theta0 = (sa >> 1) == (sb >> 1) ? 3.14159274f : 1.57079637f;
-    float lam = READ_LAMBDA ? lambda[e] : 0.f, out;
-    const bool ok = bend_project<DAMPED>(GlobalNodes{pos}, prev, i, j, k, theta0, alphaB, omega, betaGammaH, lam, out);
+    float out;
+    const bool ok = bend_project<DAMPED>(NodeTriple{pos, i, j, pa, pb, pc}, prev, i, j, k, theta0, alphaB, omega,
+                                         betaGammaH, lam, out);
     // a row that writes nothing in the first iteration keeps the substep's reset multiplier, 0 (xpbd_solver.cu:3535)
     if (WRITE_LAMBDA && (ok || !READ_LAMBDA)) lambda[e] = ok ? out : 0.f;
 }
