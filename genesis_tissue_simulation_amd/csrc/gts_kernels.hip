@@ -346,16 +346,10 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 // after a workgroup barrier, so every tet reads its nodes as the per-colour sweep leaves them; the region then writes
 // the nodes it owns to pos_out and the multipliers of the tets it owns to lam_out (read from lam_in). No projection
 // reads memory another workgroup of the launch writes. Undamped forms only: prev is not read.
-__device__ __forceinline__ int ordered_block(int xcd_order) {
-    int b = blockIdx.x;
-    if (xcd_order) {
-        const int G = gridDim.x, per = G >> 3, rem = G & 7, x = b & 7, slot = b >> 3;
-        b = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + slot;
-    }
-    return b;
-}
+__device__ __forceinline__ int ordered_block(int xcd_order) { return ordered_item(xcd_order, 1) - (int)threadIdx.x; }
 
 #define GTS_TILE_KMAX 4  // colours a tiled launch may hold: each thread prefetches its first tet of every pass
+#define GTS_TILE_THREADS 512  // threads of a tiled workgroup
 #define GTS_TILE_BOX_LOADS 8  // nodes of the LDS box a thread loads: box^3 <= 8 * 512
 #define GTS_TILE_WRITE_LOADS 2  // node writes a thread requests before the passes
 
@@ -392,13 +386,12 @@ __device__ __forceinline__ void tiled_project(int2 e, uint32_t d, float ad, floa
 }
 
 template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool PRECISE>
-__global__ void __launch_bounds__(512) k_tets_tiled(int k, int cap, int hdr_stride, const int* __restrict__ headers,
-                                                    const int2* __restrict__ entries, const int2* __restrict__ writes,
-                                                    int box, Lattice L, float omega, int T,
-                                                    const uint32_t* __restrict__ tet_desc,
-                                                    const float* __restrict__ alpha, const float* __restrict__ lam_in,
-                                                    float* __restrict__ lam_out, const float4* __restrict__ pos_in,
-                                                    float4* __restrict__ pos_out) {
+__global__ void __launch_bounds__(GTS_TILE_THREADS)
+    k_tets_tiled(int k, int cap, int hdr_stride, const int* __restrict__ headers, const int2* __restrict__ entries,
+                 const int2* __restrict__ writes, int box, Lattice L, float omega, int T,
+                 const uint32_t* __restrict__ tet_desc, const float* __restrict__ alpha,
+                 const float* __restrict__ lam_in, float* __restrict__ lam_out, const float4* __restrict__ pos_in,
+                 float4* __restrict__ pos_out) {
     extern __shared__ float4 nodes[];
     const int rl = ordered_block(L.xcd_order);
     const int* h = headers + rl * hdr_stride;
@@ -406,7 +399,7 @@ __global__ void __launch_bounds__(512) k_tets_tiled(int k, int cap, int hdr_stri
     // the two rounds of loads before the projections. Loads are unconditional, at clamped in-bounds rows, so that
     // no branch makes the compiler wait for one before issuing the next.
     const int2* ent = entries + (size_t)rl * k * cap;
-    const int tid = threadIdx.x, bs = blockDim.x;
+    const int tid = threadIdx.x, bs = GTS_TILE_THREADS;  // the launch's workgroup size, not read from the dispatch
     const int row = tid < cap ? tid : cap - 1;
     int cnt[GTS_TILE_KMAX];
     int2 en[GTS_TILE_KMAX];
@@ -491,27 +484,16 @@ __global__ void k_finish_nodes(int N, const StepShapes* __restrict__ shapes_p, u
                                float radius, float scaleR, float invH, float velScale, float4* __restrict__ pos,
                                float4* __restrict__ prev, float4* __restrict__ vel, float* reaction,
                                int xcd_order, float h = 0.f, const float4* __restrict__ ext = nullptr) {
-    args_ready(N, xcd_order, (int)gridDim.x, shapes_p, pos, prev);
-    const int i = ordered_item(xcd_order, GTS_BLOCK), ic = i < N ? i : N - 1;
-    const float4 p4 = pos[ic], pv = prev[ic];
-    // The substep's tool shapes, staged in LDS by one round of loads beside the node's: read in place, each field
-    // would be a scalar load issued only after the branch that needs it.
-    __shared__ StepShapes staged;
-    if (CONTACT) {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(shapes_p);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(&staged);
-        static_assert(sizeof(StepShapes) / 4 <= GTS_BLOCK, "one word of the shapes a thread");
-        if (threadIdx.x < sizeof(StepShapes) / 4) dst[threadIdx.x] = src[threadIdx.x];
-        __syncthreads();
-    }
+    int i = ordered_item(xcd_order, GTS_BLOCK);
     f3 force = mk3(0, 0, 0);
     float contacts = 0.f;
     if (i < N) {
+        float4 p4 = pos[i];
         float w = p4.w;
         f3 x = xyz(p4);
-        f3 xp = xyz(pv);
+        f3 xp = xyz(prev[i]);
         if (CONTACT && w > 0.f) {
-            const StepShapes& shapes = staged;
+            const StepShapes& shapes = *shapes_p;
             ContactRecord rec[4];
             float lamN[4];
             f3 lamT[4];
@@ -1610,7 +1592,7 @@ void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s,
         float4* out = reinterpret_cast<float4*>((g & 1) ? v->pos : v->pos_alt);
         if (regions <= 0) continue;
 #define GTS_TILED(R, W)                                                                                           \
-    k_tets_tiled<MODE, R, W, PRECISE><<<regions, 512, lds, s>>>(                                                  \
+    k_tets_tiled<MODE, R, W, PRECISE><<<regions, GTS_TILE_THREADS, lds, s>>>(                                     \
         k, cap, stride, v->tile_headers + (size_t)first * stride,                                                \
         reinterpret_cast<const int2*>(v->tile_entries) + gi[7],                                                  \
         reinterpret_cast<const int2*>(v->tile_writes), v->tile_box, L, omega, v->n_tets, v->tet_desc,            \
