@@ -3,8 +3,9 @@
 `python bench.py [--steps K] [--warmup W]` runs the kidney-scale single instance (an ellipsoid of ~250k Kuhn tets,
 20 substeps x 2 iterations per step, grasper contact, a Bezier blade cutting once per step) for W untimed and K
 timed steps on cuda:0, and prints one JSON line. `value` is live tets x substeps / second over the timed steps, with
-all state resident on the GPU. The dominant kernel (the tet colour pass) is timed per launch with HIP events on its
-stream over the same steps; `roofline.achieved` is its algorithmic bytes per tet update (the per-tet-substep figure
+all state resident on the GPU. Its tet sweeps run tiled (several colours a launch, KIDNEY_TILES); the same workload
+with one launch per colour is reported beside it (plain_per_colour). The dominant kernel (the tiled tet pass) is timed
+per launch with HIP events on its stream over the same steps; `roofline.achieved` is its algorithmic bytes per tet update (the per-tet-substep figure
 (52 N + 8 T + 4 E)/T) over its measured duration. `cpu_baseline` is the CPU oracle of the same step, timed on this
 host's cores on a bounded sample of the same workload.
 """
@@ -23,6 +24,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# The kidney legs run their tet sweeps tiled: up to 4 consecutive colours per launch over regions of 7^3 lattice cells
+# (Tissue.set_tiles; the same bits as one launch per colour, which the line reports beside it). Measured on MI355X
+# against 2-4 colours over 6^3-8^3 regions, this was the fastest.
+KIDNEY_TILES = (7, 4)
 
 
 def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
@@ -33,7 +38,8 @@ def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
 
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
 # kernels whose HBM bytes per dispatch the traffic file records, by the name fragment rocprofv3 reports
-TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_batched_step": "k_batched_step<", "k_adjoint_forward": "k_adjoint_forward",
+TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_tets_tiled": "k_tets_tiled<", "k_batched_step": "k_batched_step<",
+                   "k_adjoint_forward": "k_adjoint_forward",
                    "k_adjoint_tape": "k_adjoint_tape", "k_adjoint_reverse": "k_adjoint_reverse",
                    "k_stream_copy": "k_stream_copy"}
 # the surface rebuild's own kernels, one dispatch each per cut-aware rebuild (its two partial scans are shared with the
@@ -127,7 +133,7 @@ def write_traffic(fetch_csv, write_csv, copy_mib, command):
     print(json.dumps(out, indent=1))
 
 
-def run_gpu(w, steps, warmup, tune=True, persistent=False):
+def run_gpu(w, steps, warmup, tune=True, persistent=False, tiles=None):
     import ctypes as ct
 
     import torch
@@ -152,7 +158,9 @@ def run_gpu(w, steps, warmup, tune=True, persistent=False):
     else:
         best, table = {"colour_block": 64}, []
     t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, colour_block=best["colour_block"],
-               persistent=persistent)
+               persistent=persistent, tiles=tiles)
+    if tiles is not None and t.view.tile_groups == 0:
+        raise RuntimeError(f"{w.name}: no tile plan for {tiles}")
 
     def one(step):
         advance(t, step)
@@ -550,7 +558,7 @@ def batched_fp64_error(bt, k=0):
                 **position_errors(x, out[True]), bit_exact_vs_fp32_oracle=bool(np.array_equal(x, out[False])))
 
 
-def plain_fp64_error(w, steps=5, persistent=False):
+def plain_fp64_error(w, steps=5, persistent=False, tiles=None):
     """A fresh GPU run of `steps` steps of the plain workload (tool, blade cut every step) against the oracle of the
     same schedule: relative position error against FP64 and bit-exactness against FP32."""
     import torch
@@ -559,7 +567,7 @@ def plain_fp64_error(w, steps=5, persistent=False):
     from oracle import oracle as orc
 
     kd, kh = w.stiffness()
-    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, persistent=persistent)
+    t = Tissue(w.grid, w.params, kd, kh, w.initial_positions, persistent=persistent, tiles=tiles)
     oracles = {}
     for double in (False, True):
         o = orc.Oracle(w.grid, w.params, w.initial_positions, double=double)
@@ -664,11 +672,17 @@ def plain_cpu_baseline(w, steps, threads, info):
             "cpu": info}
 
 
-def plain_leg(w, steps, warmup, tune, cpu, cpu_steps, threads, info, persistent=False):
+def tiled_path(tiles):
+    region, kmax = tiles
+    return (f"tet sweeps tiled: up to {kmax} colours a launch over {region}^3-cell regions staged in LDS, "
+            f"graph-replayed")
+
+
+def plain_leg(w, steps, warmup, tune, cpu, cpu_steps, threads, info, persistent=False, tiles=None):
     """A plain single-instance leg: throughput, FP64 error and FP32-oracle exactness, and its CPU baseline. With
-    `persistent`, the tissue steps as one persistent workgroup (Tissue(persistent=True)), and the same workload
-    through the per-colour launches is reported beside it."""
-    g = run_gpu(w, steps, warmup, tune=tune, persistent=persistent)
+    `persistent`, the tissue steps as one persistent workgroup (Tissue(persistent=True)); with `tiles`, its tet sweeps
+    run tiled; either way the same workload through one launch per colour is reported beside it."""
+    g = run_gpu(w, steps, warmup, tune=tune, persistent=persistent, tiles=tiles)
     t = g.pop("tissue")
     if persistent and t._solo is None:
         raise RuntimeError(f"{w.name} does not qualify for the persistent step")
@@ -677,14 +691,14 @@ def plain_leg(w, steps, warmup, tune, cpu, cpu_steps, threads, info, persistent=
            "config": {"workload": w.name, "tets": g["T"], "live_tets": g["live"], "substeps": w.params.substeps,
                       "iterations": w.params.iterations, "precise_strain": bool(w.params.precise_strain),
                       "path": "one persistent workgroup for the whole step" if persistent else
-                      "one launch per colour, graph-replayed"}}
-    if persistent:
+                      tiled_path(tiles) if tiles else "one launch per colour, graph-replayed"}}
+    if persistent or tiles:
         c = run_gpu(w, steps, warmup, tune=tune)
         c.pop("tissue")
         leg["colour_launches"] = {"value": c["live"] * w.params.substeps * steps / c["elapsed"],
                                   "ms_per_step": 1e3 * c["elapsed"] / steps}
     if cpu:
-        leg["fp64_check"] = plain_fp64_error(w, persistent=persistent)
+        leg["fp64_check"] = plain_fp64_error(w, persistent=persistent, tiles=tiles)
         leg["cpu_baseline"] = plain_cpu_baseline(w, cpu_steps, threads, info)
     return leg
 
@@ -751,14 +765,14 @@ def main():
     # metric's 1e-4 of FP64; the reference's direct forms are measured beside it (plain_reference_forms)
     w = kidney_250k(precise=True)
     progress("kidney-250k lattice built")
-    g = run_gpu(w, args.steps, args.warmup, tune=not args.no_tune)
+    g = run_gpu(w, args.steps, args.warmup, tune=not args.no_tune, tiles=KIDNEY_TILES)
     progress("plain leg timed")
     substeps = w.params.substeps
     value = g["live"] * substeps * args.steps / g["elapsed"]
     b_alg = algorithmic_bytes_per_tet_substep(g["active"], g["T"], g["E"])
     avg_launch_ms = g["tet_ms"] / max(g["launches"], 1)
     achieved = b_alg * g["units"] / (g["tet_ms"] * 1e-3) / 1e9 if g["tet_ms"] > 0 else 0.0
-    traffic = load_traffic("k_tets")
+    traffic = load_traffic("k_tets_tiled")
     line = {
         "metric": "tet-substeps/s plain vs batched vs adjoint @ <=1e-4 rel err vs FP64",
         "value": value,
@@ -775,7 +789,7 @@ def main():
         "config": {"workload": "kidney-250k plain single instance", "tets": g["T"], "live_tets": g["live"],
                    "active_nodes": g["active"], "substeps": substeps, "iterations": w.params.iterations,
                    "dt": w.dt, "contact": "capsule grasper", "cut": "swept-quad blade, once per step",
-                   "precise_strain": True,
+                   "precise_strain": True, "path": tiled_path(KIDNEY_TILES),
                    "colour_block": g["colour_block"],
                    "autotune": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                                 for r in g["tune"]]},
@@ -783,11 +797,19 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "stream_copy_GBs": copy_gbs,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "k_tets (one colour of the tet pass)", "avg_launch_us": 1e3 * avg_launch_ms,
+                     "kernel": "k_tets_tiled (up to 4 tet colours a launch, one region a workgroup)",
+                     "avg_launch_us": 1e3 * avg_launch_ms,
                      "launches": g["launches"], "bytes_per_tet_update": b_alg,
                      "algorithmic_bytes_per_launch": b_alg * g["units"] / max(g["launches"], 1)},
     }
     tissue = g.pop("tissue")
+    # the same workload with one launch per tet colour
+    c = run_gpu(w, args.steps, args.warmup, tune=False)
+    c.pop("tissue")
+    line["plain_per_colour"] = {"value": c["live"] * substeps * args.steps / c["elapsed"], "unit": "tet-substeps/s",
+                                "ms_per_step": 1e3 * c["elapsed"] / args.steps,
+                                "path": "one launch per colour, graph-replayed"}
+    progress("per-colour plain leg timed")
     if args.plain_only:
         print(json.dumps(line), flush=True)
         return
@@ -896,7 +918,7 @@ def main():
         # the same kidney workload in the reference's direct tet forms (bit-exact with an FP32 restatement of the
         # reference; its own FP32 error against FP64 is above the metric's 1e-4 on this scene)
         ref = plain_leg(kidney_250k(precise=False), args.steps, args.warmup, not args.no_tune, not args.no_cpu,
-                        max(2, args.cpu_steps // 4), threads, info)
+                        max(2, args.cpu_steps // 4), threads, info, tiles=KIDNEY_TILES)
         ref.pop("cpu_baseline", None)
         line["plain_reference_forms"] = ref
         progress("reference-forms kidney leg done")
@@ -917,7 +939,7 @@ def main():
                                    "iterations, checkpoint 10) on a second stream",
                        "tets": r["T0"], "live_tets_start": r["live0"], "live_tets_end": r["live1"]}}
     if not args.no_cpu:
-        line["fp64_check"] = plain_fp64_error(w)
+        line["fp64_check"] = plain_fp64_error(w, tiles=KIDNEY_TILES)
         progress("plain FP64 check done")
         line["cpu_baseline"] = plain_cpu_baseline(w, args.cpu_steps, threads, info)
         progress("plain CPU baseline done")

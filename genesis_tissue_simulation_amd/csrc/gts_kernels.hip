@@ -344,29 +344,42 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 // its argument are in genesis_tissue_simulation_amd/tiling.py). The region's node box is loaded into LDS from pos_in;
 // pass p projects, in LDS, the colour-(c0+p) tets within k-1-p cells of the region (the plan's entries), each pass
 // after a workgroup barrier, so every tet reads its nodes as the per-colour sweep leaves them; the region then writes
-// the nodes it owns to pos_out and the multipliers of the tets it owns to lam_out (read from lam_in). No projection
-// reads memory another workgroup of the launch writes. Undamped forms only: prev is not read.
+// the nodes it owns to pos_out, its copies of the multipliers to tile_lambda and those of the tets it owns to
+// tet_lambda. No projection reads memory another workgroup of the launch writes. Undamped forms only.
+
 __device__ __forceinline__ int ordered_block(int xcd_order) { return ordered_item(xcd_order, 1) - (int)threadIdx.x; }
 
-#define GTS_TILE_KMAX 4  // colours a tiled launch may hold: each thread prefetches its first tet of every pass
+#define GTS_TILE_KMAX 4  // colours a tiled launch may hold: each thread loads its first tet of every pass up front
 #define GTS_TILE_THREADS 512  // threads of a tiled workgroup
-#define GTS_TILE_BOX_LOADS 8  // nodes of the LDS box a thread loads: box^3 <= 8 * 512
-#define GTS_TILE_WRITE_LOADS 2  // node writes a thread requests before the passes
+#define GTS_TILE_BOX_LOADS 8  // nodes of the LDS box a thread loads: box^3 < 8 * 512 (box <= 15)
+#define GTS_TILE_WRITE_LOADS 2  // node writes a thread loads up front
+#define GTS_TILE_MAX_REGIONS 640  // regions of a tiled launch: their dense ids fill most of the 4 KiB argument space
 
-// One tet's inputs of a projection, loaded ahead of use.
-struct TiledTetInputs {
-    uint32_t d;
-    float ad, ah, lamD, lamH;
+// Where a launch's regions find their data: every address follows from the region's index, and its box origin from
+// its dense id, which travels with the launch (a scalar read of the argument segment, no memory round of its own).
+struct TileLaunch {
+    int k, cap, wcap, hdr_stride, box, region, rx, ry, halo, T;
+    int dense[GTS_TILE_MAX_REGIONS];  // dense id rx + rx_count (ry + ry_count rz) of each region (32-bit: a scalar load)
 };
 
-// One tiled tet projection in the region's LDS box: e the plan entry, the rest its inputs as loaded.
+// Copies each plan entry's tet inputs into the entry: alpha_D, alpha_H and the descriptor's bits (liveness, type).
+__global__ void k_tile_gather(int n, int T, const int2* __restrict__ entries, const uint32_t* __restrict__ tet_desc,
+                              const float* __restrict__ alpha, float4* __restrict__ inputs) {
+    const int q = blockIdx.x * GTS_BLOCK + threadIdx.x;
+    if (q >= n) return;
+    const int slot = entries[q].x & 0x7fffffff;  // padding rows name slot 0
+    inputs[q] = make_float4(alpha[slot], alpha[T + slot], __builtin_bit_cast(float, tet_desc[slot]), 0.f);
+}
+
+// One tiled tet projection in the region's LDS box: e the plan entry, in its gathered inputs, lam its multipliers.
 template <int MODE, bool WRITE_LAMBDA, bool PRECISE>
-__device__ __forceinline__ void tiled_project(int2 e, uint32_t d, float ad, float ah, float inD, float inH,
-                                              float4* nodes, int box, int box2, const Lattice& L, float omega, int T,
-                                              float* __restrict__ lam_out) {
+__device__ __forceinline__ void tiled_project(int2 e, float4 in, float2 lam, float4* nodes, const float* invRest,
+                                              int box, int box2, const Lattice& L, float omega, int T,
+                                              float2* __restrict__ lam_copy, float* __restrict__ lam_out) {
     const bool owned = e.x < 0;
     const int slot = e.x & 0x7fffffff;
-    float lamD = inD, lamH = inH;
+    const uint32_t d = __builtin_bit_cast(uint32_t, in.z);
+    float lamD = lam.x, lamH = lam.y;
     bool ok = false;
     if (d & 1u) {  // a dead tet keeps its multipliers: 0 from the first iteration's reset
         const int type = (int)((d >> 1) & 7u);
@@ -375,101 +388,102 @@ __device__ __forceinline__ void tiled_project(int2 e, uint32_t d, float ad, floa
         f3 rk[3];
         if (PRECISE)
             for (int c = 0; c < 3; c++) rk[c] = kuhn_rest_edge(type, c + 1, L.rest_len);
-        ok = tet_project<MODE, false, PRECISE>(LdsNodes{nodes}, nullptr, id, L.inv_rest[type], ad, ah, lamD, lamH,
-                                               omega, 0.f, 0.f, rk);
+        ok = tet_project<MODE, false, PRECISE>(LdsNodes{nodes}, nullptr, id, invRest + 9 * type, in.x, in.y, lamD,
+                                               lamH, omega, 0.f, 0.f, rk);
     }
     // as k_tets stores in place: the update, else the multiplier read (the first iteration's reset 0)
-    if (owned && WRITE_LAMBDA) {
-        lam_out[slot] = ok ? lamD : inD;
-        lam_out[T + slot] = ok ? lamH : inH;
+    if (WRITE_LAMBDA) {
+        const float2 out = make_float2(ok ? lamD : lam.x, ok ? lamH : lam.y);
+        *lam_copy = out;
+        if (owned) {
+            lam_out[slot] = out.x;
+            lam_out[T + slot] = out.y;
+        }
     }
 }
 
 template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool PRECISE>
 __global__ void __launch_bounds__(GTS_TILE_THREADS)
-    k_tets_tiled(int k, int cap, int hdr_stride, const int* __restrict__ headers, const int2* __restrict__ entries,
-                 const int2* __restrict__ writes, int box, Lattice L, float omega, int T,
-                 const uint32_t* __restrict__ tet_desc, const float* __restrict__ alpha,
-                 const float* __restrict__ lam_in, float* __restrict__ lam_out, const float4* __restrict__ pos_in,
+    k_tets_tiled(TileLaunch A, const int* __restrict__ headers, const int2* __restrict__ entries,
+                 const float4* __restrict__ inputs, float2* __restrict__ lam_copy, const int2* __restrict__ writes,
+                 Lattice L, float omega, float* __restrict__ lam_out, const float4* __restrict__ pos_in,
                  float4* __restrict__ pos_out) {
     extern __shared__ float4 nodes[];
-    const int rl = ordered_block(L.xcd_order);
-    const int* h = headers + rl * hdr_stride;
-    // A region's entries sit at a fixed stride, so that they are requested together with its header: the first of
-    // the two rounds of loads before the projections. Loads are unconditional, at clamped in-bounds rows, so that
-    // no branch makes the compiler wait for one before issuing the next.
-    const int2* ent = entries + (size_t)rl * k * cap;
+    __shared__ float invRest[54];  // the six types' Dm^-1, staged with the box (indexed by type, not from arguments)
+    const int rl = ordered_block(L.xcd_order), dense = A.dense[rl];
     const int tid = threadIdx.x, bs = GTS_TILE_THREADS;  // the launch's workgroup size, not read from the dispatch
+    const int k = A.k, cap = A.cap, box = A.box, box2 = box * box, box3 = box2 * box;
+    // One round of loads: every address below follows from rl and tid. Rows past a count are padding (in bounds).
+    const int* h = headers + rl * A.hdr_stride;
+    const size_t e0 = (size_t)rl * k * cap;
     const int row = tid < cap ? tid : cap - 1;
     int cnt[GTS_TILE_KMAX];
     int2 en[GTS_TILE_KMAX];
+    float4 in[GTS_TILE_KMAX];
+    float2 lm[GTS_TILE_KMAX];
 #pragma unroll
     for (int p = 0; p < GTS_TILE_KMAX; p++) {
+        const size_t q = e0 + (size_t)(p < k ? p : 0) * cap + row;
         cnt[p] = p < k ? h[8 + p] : 0;
-        en[p] = ent[(p < k ? p : 0) * cap + row];
+        en[p] = entries[q];
+        in[p] = inputs[q];
+        lm[p] = READ_LAMBDA ? lam_copy[q] : make_float2(0.f, 0.f);
     }
-    const int ox = h[0], oy = h[1], oz = h[2], wbeg = h[4], wcnt = h[5];
-    if (wcnt == 0 && cnt[0] + cnt[1] + cnt[2] + cnt[3] == 0) return;  // idle in this launch (the whole workgroup)
-    // The second round, all in flight together: this thread's first tet of every pass, its first node writes and
-    // its nodes of the box (box <= 16 and 512 threads: at most 8 a thread).
-    TiledTetInputs pre[GTS_TILE_KMAX];
-#pragma unroll
-    for (int p = 0; p < GTS_TILE_KMAX; p++) {
-        const int slot = en[p].x & 0x7fffffff;  // a valid slot on every row, padding included (slot 0)
-        pre[p].d = tet_desc[slot];
-        pre[p].ad = alpha[slot];
-        pre[p].ah = alpha[T + slot];
-        pre[p].lamD = READ_LAMBDA ? lam_in[slot] : 0.f;
-        pre[p].lamH = READ_LAMBDA ? lam_in[T + slot] : 0.f;
-    }
+    const int wcnt = h[5];
+    const int2* wr_row = writes + (size_t)rl * A.wcap;
     int2 wr[GTS_TILE_WRITE_LOADS];
 #pragma unroll
     for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++) {
         const int q = tid + j * bs;
-        wr[j] = writes[wbeg + (q < wcnt ? q : 0)];  // wcnt >= 1: every region writes a node
+        wr[j] = wr_row[q < A.wcap ? q : A.wcap - 1];
     }
-    const int ny = L.nxy / L.nx, nz = L.vz + 1, box2 = box * box, box3 = box2 * box;
+    // the box, at the origin of the region's dense id
+    const int ox = (dense % A.rx) * A.region - A.halo, oy = ((dense / A.rx) % A.ry) * A.region - A.halo,
+              oz = (dense / (A.rx * A.ry)) * A.region - A.halo;
+    const int ny = L.nxy / L.nx, nz = L.vz + 1;
+    const float invRestMine = reinterpret_cast<const float*>(L.inv_rest)[tid < 54 ? tid : 0];
     {
         // i / box and i / box^2 for i < 4096 as (i * ceil(2^20 / d)) >> 20, exact for d <= 256
         const uint32_t m1 = ((1u << 20) + box - 1) / box, m2 = ((1u << 20) + box2 - 1) / box2;
         float4 v[GTS_TILE_BOX_LOADS];
-        bool in[GTS_TILE_BOX_LOADS];
+        bool inb[GTS_TILE_BOX_LOADS];
 #pragma unroll
         for (int j = 0; j < GTS_TILE_BOX_LOADS; j++) {
             const int i = tid + j * bs;
             const int lz = (int)(((uint32_t)i * m2) >> 20), rem = i - lz * box2;
             const int ly = (int)(((uint32_t)rem * m1) >> 20), lx = rem - ly * box;
             const int x = ox + lx, y = oy + ly, z = oz + lz;
-            in[j] = i < box3 && x >= 0 && x < L.nx && y >= 0 && y < ny && z >= 0 && z < nz;
-            const int g = in[j] ? x + L.nx * y + L.nxy * z : 0;
-            v[j] = pos_in[g];
+            inb[j] = i < box3 && x >= 0 && x < L.nx && y >= 0 && y < ny && z >= 0 && z < nz;
+            v[j] = pos_in[inb[j] ? x + L.nx * y + L.nxy * z : 0];
         }
+        // all in flight together: left alone, the compiler moves each load into its conditional store, one round each
+#pragma unroll
+        for (int j = 0; j < GTS_TILE_BOX_LOADS; j++) value_use(v[j]);
 #pragma unroll
         for (int j = 0; j < GTS_TILE_BOX_LOADS; j++)
-            if (in[j]) nodes[tid + j * bs] = v[j];
+            if (inb[j]) nodes[tid + j * bs] = v[j];
+        if (tid < 54) invRest[tid] = invRestMine;
     }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < GTS_TILE_KMAX; p++) {
         if (p >= k) break;
+        const size_t pq = e0 + (size_t)p * cap;
         if (tid < cnt[p])
-            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(en[p], pre[p].d, pre[p].ad, pre[p].ah, pre[p].lamD,
-                                                       pre[p].lamH, nodes, box, box2, L, omega, T, lam_out);
-        for (int q = tid + bs; q < cnt[p]; q += bs) {  // passes of more tets than threads
-            const int2 e = ent[p * cap + q];
-            const int slot = e.x & 0x7fffffff;
-            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(e, tet_desc[slot], alpha[slot], alpha[T + slot],
-                                                       READ_LAMBDA ? lam_in[slot] : 0.f,
-                                                       READ_LAMBDA ? lam_in[T + slot] : 0.f, nodes, box, box2, L,
-                                                       omega, T, lam_out);
-        }
+            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(en[p], in[p], lm[p], nodes, invRest, box, box2, L, omega, A.T,
+                                                       lam_copy + pq + tid, lam_out);
+        for (int q = tid + bs; q < cnt[p]; q += bs)  // passes of more tets than threads
+            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(entries[pq + q], inputs[pq + q],
+                                                       READ_LAMBDA ? lam_copy[pq + q] : make_float2(0.f, 0.f),
+                                                       nodes, invRest, box, box2, L, omega, A.T, lam_copy + pq + q,
+                                                       lam_out);
         __syncthreads();
     }
 #pragma unroll
     for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++)
         if (tid + j * bs < wcnt) pos_out[wr[j].y] = nodes[wr[j].x];
     for (int q = tid + GTS_TILE_WRITE_LOADS * bs; q < wcnt; q += bs) {
-        const int2 w = writes[wbeg + q];
+        const int2 w = wr_row[q];
         pos_out[w.y] = nodes[w.x];
     }
 }
@@ -1458,10 +1472,10 @@ template <class T> struct same_type { using type = T; };
 // event that the runtime stamps when the kernel itself starts and ends, so the recorded duration is the kernel's own
 // (no enqueue or dispatch latency), and `units` (tets) are credited to the launch.
 template <class... A>
-void profiled_launch(void (*kern)(A...), dim3 g, dim3 b, hipStream_t s, long long units,
-                     typename same_type<A>::type... args) {
+void profiled_launch_lds(void (*kern)(A...), dim3 g, dim3 b, size_t lds, hipStream_t s, long long units,
+                         typename same_type<A>::type... args) {
     if (!g_profile.on) {
-        kern<<<g, b, 0, s>>>(args...);
+        kern<<<g, b, lds, s>>>(args...);
         return;
     }
     if (2 * g_profile.used + 2 > g_profile.events.size()) {
@@ -1470,12 +1484,17 @@ void profiled_launch(void (*kern)(A...), dim3 g, dim3 b, hipStream_t s, long lon
         for (size_t k = old; k < g_profile.events.size(); k++) profile_note(hipEventCreate(&g_profile.events[k]));
     }
     void* ptrs[] = {(void*)&args...};
-    profile_note(hipExtLaunchKernel(reinterpret_cast<const void*>(kern), g, b, ptrs, 0, s,
+    profile_note(hipExtLaunchKernel(reinterpret_cast<const void*>(kern), g, b, ptrs, lds, s,
                                     g_profile.events[2 * g_profile.used], g_profile.events[2 * g_profile.used + 1],
                                     0));
     if (g_profile.units.size() <= g_profile.used) g_profile.units.resize(g_profile.used + 1024);
     g_profile.units[g_profile.used] = units;
     g_profile.used++;
+}
+template <class... A>
+void profiled_launch(void (*kern)(A...), dim3 g, dim3 b, hipStream_t s, long long units,
+                     typename same_type<A>::type... args) {
+    profiled_launch_lds(kern, g, b, 0, s, units, args...);
 }
 
 int scan_blocks(int n) { return grid_for(n); }
@@ -1576,27 +1595,38 @@ void launch_bend_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t 
 }
 
 // One iteration's tet sweep as the tiled launches: positions alternate between pos and pos_alt from launch to launch
-// (an even number of them, so the sweep ends in pos); the multipliers are read from the buffer the previous iteration
-// wrote and written to the other, the last iteration writing tet_lambda.
+// (an even number of them, so the sweep ends in pos); each entry's multipliers are read from its copy, which the
+// previous iteration's projection of that entry wrote.
 template <int MODE, bool PRECISE>
 void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s, int it, int iters, float omega) {
     const bool read = it > 0, write = iters > 1;
-    const int stride = 8 + v->tile_kmax;
-    float* lamW = ((iters - 1 - it) % 2 == 0) ? v->tet_lambda : v->lambda_alt;
-    float* lamR = ((iters - it) % 2 == 0) ? v->tet_lambda : v->lambda_alt;  // what iteration it - 1 wrote
+    TileLaunch A;
+    A.hdr_stride = 8 + v->tile_kmax;
+    A.box = v->tile_box;
+    A.region = v->tile_region;
+    A.rx = v->tile_rx;
+    A.ry = v->tile_ry;
+    A.T = v->n_tets;
+    const int nreg = v->tile_group_info[3];  // every launch takes the same regions
+    for (int r = 0; r < nreg; r++) A.dense[r] = v->tile_dense_ids[r];
     const size_t lds = (size_t)v->tile_box * v->tile_box * v->tile_box * sizeof(float4);
     for (int g = 0; g < v->tile_groups; g++) {
-        const int32_t* gi = v->tile_group_info + 8 * g;
-        const int k = gi[1], first = gi[2], regions = gi[3], cap = gi[6];
+        const int32_t* gi = v->tile_group_info + 12 * g;
+        const int first = gi[2], regions = gi[3];
+        A.k = gi[1];
+        A.halo = gi[5];
+        A.cap = gi[6];
+        A.wcap = gi[8];
         const float4* in = reinterpret_cast<const float4*>((g & 1) ? v->pos_alt : v->pos);
         float4* out = reinterpret_cast<float4*>((g & 1) ? v->pos : v->pos_alt);
         if (regions <= 0) continue;
+        // tet updates of the launch (its colours' tets), credited while profiling
+        const long long units = (long long)v->tet_colour_offsets[gi[0] + gi[1]] - v->tet_colour_offsets[gi[0]];
 #define GTS_TILED(R, W)                                                                                           \
-    k_tets_tiled<MODE, R, W, PRECISE><<<regions, GTS_TILE_THREADS, lds, s>>>(                                     \
-        k, cap, stride, v->tile_headers + (size_t)first * stride,                                                \
-        reinterpret_cast<const int2*>(v->tile_entries) + gi[7],                                                  \
-        reinterpret_cast<const int2*>(v->tile_writes), v->tile_box, L, omega, v->n_tets, v->tet_desc,            \
-        v->tet_alpha, lamR, lamW, in, out)
+    profiled_launch_lds(k_tets_tiled<MODE, R, W, PRECISE>, dim3(regions), dim3(GTS_TILE_THREADS), lds, s, units,  \
+        A, v->tile_headers + (size_t)first * A.hdr_stride, reinterpret_cast<const int2*>(v->tile_entries) + gi[7], \
+        reinterpret_cast<const float4*>(v->tile_inputs) + gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7], \
+        reinterpret_cast<const int2*>(v->tile_writes) + gi[9], L, omega, v->tet_lambda, in, out)
         if (read && write) GTS_TILED(true, true);
         else if (read) GTS_TILED(true, false);
         else if (write) GTS_TILED(false, true);
@@ -1834,19 +1864,22 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
         P.alphaB = 1.f / (p->bend_kb * h * h);
         P.betaGammaHB = P.alphaB * p->beta_b * h;
     }
-    // the tiled tet sweep: a plan for the current lists, undamped and unguarded (the legacy step turns it off)
+    // the tiled tet sweep: a plan for the current lists, undamped and unguarded (the legacy step and the ribbon tick
+    // turn it off)
     P.tiled = false;
     if (v->tile_groups > 0 && P.strain && P.dOverH == 0.f && P.bOverH == 0.f && P.guardPasses == 0 &&
         v->tile_tets == v->n_tets && (v->tile_groups % 2) == 0 && v->tile_group_info && v->tile_headers &&
-        v->tile_entries && v->tile_writes && v->pos_alt && v->lambda_alt && v->tile_kmax >= 1 &&
-        v->tile_kmax <= GTS_TILE_KMAX &&
-        v->tile_box >= 2 && (long long)v->tile_box * v->tile_box * v->tile_box * 16 <= 65536) {
+        v->tile_entries && v->tile_writes && v->pos_alt && v->tile_inputs && v->tile_lambda && v->tile_kmax >= 1 &&
+        v->tile_kmax <= GTS_TILE_KMAX && v->tile_region >= 1 && v->tile_rx >= 1 && v->tile_ry >= 1 &&
+        v->tile_box >= 2 && (long long)v->tile_box * v->tile_box * v->tile_box * 16 + 256 <= 65536) {
         int next = 0;
-        bool ok = true;
+        const int marked = v->tile_group_info[3];
+        bool ok = v->tile_dense_ids && marked >= 1 && marked <= GTS_TILE_MAX_REGIONS;
+        for (int r = 0; r < marked && ok; r++) ok = v->tile_dense_ids[r] >= 0;
         for (int g = 0; g < v->tile_groups && ok; g++) {
-            const int32_t* gi = v->tile_group_info + 8 * g;
-            ok = gi[0] == next && gi[1] >= 1 && gi[1] <= v->tile_kmax && gi[3] >= 0 && gi[4] == v->tile_box &&
-                 gi[6] >= 1 && gi[7] >= 0;
+            const int32_t* gi = v->tile_group_info + 12 * g;
+            ok = gi[0] == next && gi[1] >= 1 && gi[1] <= v->tile_kmax && gi[3] == marked && gi[4] == v->tile_box &&
+                 gi[5] == v->tile_kmax - 1 && gi[6] >= 1 && gi[7] >= 0 && gi[8] >= 1 && gi[9] >= 0;
             next += gi[1];
         }
         P.tiled = ok && next == v->n_tet_colours;
@@ -1890,6 +1923,13 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
     const int T = v->n_tets, N = v->n_nodes;
     launch_refresh(v, L, s);
     if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, P.invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
+    if (P.tiled) {  // each plan entry's copy of its tet's inputs, after this step's liveness and compliances
+        const int32_t* last = v->tile_group_info + 12 * (v->tile_groups - 1);
+        const int entries = last[7] + last[3] * last[1] * last[6];
+        k_tile_gather<<<grid_for(entries), GTS_BLOCK, 0, s>>>(entries, T, reinterpret_cast<const int2*>(v->tile_entries),
+                                                             v->tet_desc, v->tet_alpha,
+                                                             reinterpret_cast<float4*>(v->tile_inputs));
+    }
     if (P.contact || P.legacy) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
     const StepShapes* shapes = reinterpret_cast<const StepShapes*>(v->step_consts);
     dim3 g(grid_for(N)), b(GTS_BLOCK);
@@ -2050,8 +2090,10 @@ hipStream_t g_capture = nullptr;
 std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
     const size_t ne = (size_t)v->n_edge_colours + 1, nt = (size_t)v->n_tet_colours + 1;
     const size_t nb = P.bend ? (size_t)v->n_bend_colours + 1 : 0;
-    const size_t ng = P.tiled ? (size_t)8 * v->tile_groups : 0;
-    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) + (ne + nt + nb + ng) * sizeof(int32_t));
+    const size_t ng = P.tiled ? (size_t)12 * v->tile_groups : 0;
+    const size_t nd = P.tiled ? (size_t)v->tile_group_info[3] : 0;  // the dense ids the launches carry
+    std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) +
+                                 (ne + nt + nb + ng + nd) * sizeof(int32_t));
     unsigned char* d = k.data();
     std::memcpy(d, v, sizeof(GtsTissueView));
     d += sizeof(GtsTissueView);
@@ -2064,6 +2106,8 @@ std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) 
     if (nb) std::memcpy(d, v->bend_colour_offsets, nb * sizeof(int32_t));
     d += nb * sizeof(int32_t);
     if (ng) std::memcpy(d, v->tile_group_info, ng * sizeof(int32_t));
+    d += ng * sizeof(int32_t);
+    if (nd) std::memcpy(d, v->tile_dense_ids, nd * sizeof(int32_t));
     return k;
 }
 
@@ -2110,6 +2154,7 @@ namespace {
 int plan_tick(const GtsTissueView* v, const GtsCutTool* cut, const float* emit_ncut, float tear_tau, StepPlan& P) {
     if (!emit_ncut || !v->prev_corner) return GTS_ERR_BAD_ARG;
     P.tick = true;
+    P.tiled = false;  // the tick changes liveness within the step, after the step's gather of the tiled inputs
     P.tickS = ld3(cut->t2v2);  // the current rod: S = t2v2, E = t1v2 (cut.cu:1382-1383)
     P.tickE = ld3(cut->t1v2);
     P.tickN = ld3(emit_ncut);

@@ -11,9 +11,12 @@ region holding the tet's cell -- writes the tet's multipliers; each node is writ
 tet of the launch's colours that touches it (whose LDS copy is final), or, untouched, by the region holding the node.
 
 Launches read positions from one buffer and write them to the other (a region's halo overlaps its neighbours' writes),
-so a sweep takes an even number of launches; the multipliers of a sweep are read from the previous iteration's buffer
-and written to the other for the same reason. The per-node sequence of updates is the per-colour sweep's, so the
-results are bit for bit those of one launch per colour.
+so a sweep takes an even number of launches. Every input a region needs sits at an address it can compute from its
+index, so that all its loads go out in one round: its entries, write list and header at fixed strides, its box origin
+from its dense id (passed with the launch), and per entry a copy of the tet's inputs (gathered once a step) and of its
+multipliers (each region keeps the copies of the tets it projects, halo ones included, which stay equal to the owner's).
+The per-node sequence of updates is the per-colour sweep's, so the results are bit for bit those of one launch per
+colour.
 """
 from __future__ import annotations
 
@@ -21,7 +24,9 @@ import numpy as np
 
 # bit-order Kuhn corners (i = dx | dy << 1 | dz << 2) of the six types, as the kernels index them
 KUHN_BITS = ((0, 1, 3, 7), (0, 3, 2, 7), (0, 2, 6, 7), (0, 6, 4, 7), (0, 4, 5, 7), (0, 5, 1, 7))
-HDR = 8  # region header ints: box origin x, y, z, entry offset, write offset, write count, 2 unused; pass counts follow
+HDR = 8  # region header ints: box origin x, y, z, entry offset, write offset, write count, dense region id, 0; pass
+# counts follow
+MAX_REGIONS = 640  # regions a launch may take: their dense ids travel as kernel arguments (4 KiB in all)
 
 
 def group_sizes(colours, kmax):
@@ -42,16 +47,20 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
     """Plan of the tiled sweep for a lattice of `dims` cells and its colour-major tet descriptors.
 
     Returns a dict of numpy arrays:
-      groups  int32 [G, 8]: first colour, colours, first region header, regions, box width, halo of the box, pass
-              capacity (the most tets any region's pass holds), first entry
-      headers int32 [R, HDR + kmax]: box origin (x, y, z), entry offset, write offset, write count, 0, 0, then the
-              entry count of each pass
+      groups  int32 [G, 12]: first colour, colours, first region header, regions, box width, halo of the box, pass
+              capacity (the most tets any region's pass holds), first entry, write capacity (the most nodes any region
+              writes), first write, 0, 0
+      headers int32 [R, HDR + kmax]: box origin (x, y, z), entry offset, write offset, write count, dense region id, 0,
+              then the entry count of each pass
       entries int32 [M, 2]: tet slot | owned << 31, the tet's corner-0 index in the region's LDS box. The regions of a
               group take k * capacity rows each, in header order from the group's first entry, pass p at rows
-              p * capacity .. p * capacity + count - 1 (the rest zero), so a region finds its entries without its
-              header
-      writes  int32 [W + 1, 2]: LDS box index, node id, then one zero row
-    or None when the sweep cannot be tiled (fewer than two colours)."""
+              p * capacity .. p * capacity + count - 1 (the rest zero)
+      writes  int32 [W, 2]: LDS box index, node id; each region of a group takes write-capacity rows in header order
+              from the group's first write (the rest zero)
+      dense_ids int32 [R per launch]: the dense id r = rx + RX (ry + RY rz) of each region, in header order (the
+              same in every launch)
+    with region, kmax, box, halo, RX, RY and the tet count, or None when the sweep cannot be tiled (fewer than two
+    colours, or more than MAX_REGIONS regions)."""
     vx, vy, vz = (int(d) for d in dims)
     nx, ny = vx + 1, vy + 1
     nxy = nx * ny
@@ -115,25 +124,27 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
     # entries and no writes there and returns at once.
     regions = np.unique(np.concatenate([np.concatenate((rows[rows[:, 2] >= (1 << 31), 0], wr))
                                         for _, _, rows, wr in per_group]))
+    if len(regions) > MAX_REGIONS:  # a launch's table of dense ids
+        return None
     groups, headers, entries, writes = [], [], [], []
     n_entries = n_writes = 0
     for c0, k, rows, wr in per_group:
         active = np.union1d(rows[rows[:, 2] >= (1 << 31), 0], wr)
         rows = rows[np.isin(rows[:, 0], active)]
-        g_first, g_entry = len(headers), n_entries
+        g_first, g_entry, g_write = len(headers), n_entries, n_writes
         per_region = []
         for r in regions:
             sel = rows[rows[:, 0] == r]
-            per_region.append((sel, np.bincount(sel[:, 1].astype(np.int64), minlength=k)))
-        cap = max(1, max(int(c.max()) for _, c in per_region))
-        for r, (sel, counts) in zip(regions, per_region):
+            per_region.append((sel, np.bincount(sel[:, 1].astype(np.int64), minlength=k), all_nodes[wr == r]))
+        cap = max(1, max(int(c.max()) for _, c, _ in per_region))
+        wcap = max(1, max(len(nd) for _, _, nd in per_region))
+        for r, (sel, counts, nodes) in zip(regions, per_region):
             rx, ry, rz = r % RX, (r // RX) % RY, r // (RX * RY)
             ox, oy, oz = rx * S - H, ry * S - H, rz * S - H
-            nodes = all_nodes[wr == r]
             lx, ly, lz = nodes % nx - ox, (nodes // nx) % ny - oy, nodes // nxy - oz
             local = lx + B * (ly + B * lz)
             hdr = np.zeros(HDR + kmax, dtype=np.int64)
-            hdr[:6] = (ox, oy, oz, n_entries, n_writes, len(nodes))
+            hdr[:7] = (ox, oy, oz, n_entries, n_writes, len(nodes), r)
             hdr[HDR:HDR + k] = counts
             headers.append(hdr)
             padded = np.zeros((k * cap, 2), dtype=np.int64)
@@ -142,16 +153,16 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
                 padded[p * cap:p * cap + counts[p]] = sel[pos:pos + counts[p], 2:4]
                 pos += counts[p]
             entries.append(padded)
-            writes.append(np.stack((local, nodes), axis=1))
+            wpad = np.zeros((wcap, 2), dtype=np.int64)
+            wpad[:len(nodes)] = np.stack((local, nodes), axis=1)
+            writes.append(wpad)
             n_entries += k * cap
-            n_writes += len(nodes)
-        groups.append((c0, k, g_first, len(regions), B, H, cap, g_entry))
+            n_writes += wcap
+        groups.append((c0, k, g_first, len(regions), B, H, cap, g_entry, wcap, g_write, 0, 0))
     as32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64).astype(np.int32))
     ent = np.concatenate(entries) if entries else np.zeros((0, 2), dtype=np.int64)
     # slot | owned << 31 wraps to a negative int32, which the kernel reads back as the same 32 bits
     ent32 = np.ascontiguousarray(ent.astype(np.uint64).astype(np.uint32).view(np.int32).reshape(-1, 2))
-    # one zero row after the last region's writes: a region whose write list is empty still reads a row in bounds
-    writes.append(np.zeros((1, 2), dtype=np.int64))
     return dict(groups=as32(groups), headers=as32(np.stack(headers)), entries=ent32,
-                writes=as32(np.concatenate(writes)), region=S, kmax=int(kmax), box=B, halo=H,
-                tets=int(len(desc)))
+                writes=as32(np.concatenate(writes)), dense_ids=as32(regions), region=S, kmax=int(kmax), box=B,
+                halo=H, rx=RX, ry=RY, tets=int(len(desc)))

@@ -275,18 +275,25 @@ class Tissue:
             v.tile_groups = 0
             return None
         dev = self.device
+        n_entries = len(plan["entries"])
         keep = dict(plan=plan, groups=np.ascontiguousarray(plan["groups"].reshape(-1)),
                     headers=torch.as_tensor(plan["headers"].reshape(-1)).to(dev),
                     entries=torch.as_tensor(plan["entries"].reshape(-1)).to(dev),
                     writes=torch.as_tensor(plan["writes"].reshape(-1)).to(dev),
-                    pos_alt=self.pos.clone(), lambda_alt=torch.zeros_like(self.tet_lambda))
+                    pos_alt=self.pos.clone(),
+                    inputs=torch.zeros(4 * n_entries, dtype=torch.float32, device=dev),
+                    lam=torch.zeros(2 * n_entries, dtype=torch.float32, device=dev))
         self._tiles = keep
         v.tile_groups = len(plan["groups"])
         v.tile_kmax, v.tile_box, v.tile_tets = plan["kmax"], plan["box"], plan["tets"]
         v.tile_group_info = keep["groups"].ctypes.data
         v.tile_headers, v.tile_entries = keep["headers"].data_ptr(), keep["entries"].data_ptr()
         v.tile_writes = keep["writes"].data_ptr()
-        v.pos_alt, v.lambda_alt = keep["pos_alt"].data_ptr(), keep["lambda_alt"].data_ptr()
+        v.pos_alt = keep["pos_alt"].data_ptr()
+        v.tile_inputs, v.tile_lambda = keep["inputs"].data_ptr(), keep["lam"].data_ptr()
+        v.tile_region, v.tile_rx, v.tile_ry = plan["region"], plan["rx"], plan["ry"]
+        keep["dense"] = np.ascontiguousarray(plan["dense_ids"], dtype=np.int32)
+        v.tile_dense_ids = keep["dense"].ctypes.data
         return plan
 
     def set_contact_model(self, model, grasp=None):

@@ -155,25 +155,32 @@ typedef struct GtsTissueView {
     const uint32_t* bend_desc;          /* [n_bends] */
     float* bend_lambda;                 /* [n_bends] */
     /* Tiled tet sweep (no reference counterpart; the results are the per-colour sweep's bit for bit): with
-     * tile_groups > 0 an undamped, unguarded step without the legacy grasp runs its tet colours in tile_groups launches
-     * of up to tile_kmax consecutive colours each, one workgroup per lattice region, positions in an LDS box of
-     * tile_box^3 nodes around the region (the plan: genesis_tissue_simulation_amd/tiling.py). Each launch reads the
-     * positions from one buffer and writes them to the other (pos, pos_alt alternate; tile_groups is even), and a
-     * sweep reads the multipliers of the previous iteration from one buffer and writes the other (tet_lambda and
-     * lambda_alt; the last iteration writes tet_lambda). The plan holds for tile_tets tets in the current colour
-     * lists; a view whose lists differ (a compaction) takes the per-colour launches. */
+     * tile_groups > 0 an undamped, unguarded step without the legacy grasp or a ribbon tick runs its tet colours in
+     * tile_groups launches of up to tile_kmax consecutive colours each, one workgroup per lattice region, positions in
+     * an LDS box of tile_box^3 nodes around the region (the plan: genesis_tissue_simulation_amd/tiling.py). Each launch
+     * reads the positions from one buffer and writes them to the other (pos, pos_alt alternate; tile_groups is even).
+     * Every input of a region sits at an address it computes from its index: its header, entries and writes at fixed
+     * strides, its box origin from its dense id (a kernel argument), and per entry the tet's inputs (tile_inputs, gathered at the start of
+     * each step) and its multipliers (tile_lambda: each region keeps the copies of the tets it projects; the owners
+     * also write tet_lambda). The plan holds for tile_tets tets in the current colour lists; a view whose lists
+     * differ (a compaction) takes the per-colour launches. */
     int tile_groups;
     int tile_kmax, tile_box, tile_tets;
-    const int32_t* tile_group_info;     /* host [tile_groups * 8]: first colour, colours, first header, regions, box
-                                         * width, box halo, pass capacity, first entry */
+    const int32_t* tile_group_info;     /* host [tile_groups * 12]: first colour, colours, first header, regions, box
+                                         * width, box halo, pass capacity, first entry, write capacity, first
+                                         * write, 0, 0 */
     const int32_t* tile_headers;        /* [regions * (8 + tile_kmax)]: box origin x, y, z, entry offset, write offset,
-                                         * write count, 0, 0, entries of each pass */
+                                         * write count, dense region id, 0, entries of each pass */
     const int32_t* tile_entries;        /* [2 * entries]: tet slot | owned << 31, corner-0 index in the LDS box; a
                                          * region's pass p holds rows p * capacity .. p * capacity + count - 1 of
                                          * its k * capacity rows, which start at first entry + region * k * capacity */
-    const int32_t* tile_writes;         /* [2 * writes]: LDS box index, node id */
+    const int32_t* tile_writes;         /* [2 * writes]: LDS box index, node id; write capacity rows a region */
     float* pos_alt;                     /* [4N] the second position buffer */
-    float* lambda_alt;                  /* [2T] the second multiplier buffer */
+    float* tile_inputs;                 /* [4 * entries] per entry: alpha_D, alpha_H, the descriptor's bits, 0 */
+    float* tile_lambda;                 /* [2 * entries] per entry: its copy of the tet's multipliers */
+    int tile_region, tile_rx, tile_ry;  /* cells along a region's edge; regions along x and y of the dense numbering */
+    const int32_t* tile_dense_ids;      /* host [regions per launch, at most 640]: dense id rx + tile_rx (ry +
+                                         * tile_ry rz) of each region taking part, in header order */
 } GtsTissueView;
 
 /* Ribbon ticks a released grasp-ring edge stays exempt from the tear law while still longer than 0.8 tau L

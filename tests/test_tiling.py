@@ -2,7 +2,8 @@
 GPU): the properties that make a launch of several colours compute what one launch per colour computes.
 
   * The groups cover the colours in order, at most kmax each, in an even number of launches.
-  * Every launch takes the same regions in the same order. Every tet of a group's colours is owned by exactly one
+  * Every launch takes the same regions in the same order (the plan's dense ids, from which the box origins follow),
+    each region's entries and writes at fixed strides. Every tet of a group's colours is owned by exactly one
     region, the one holding its cell, and listed in a region's pass p exactly when its cell lies within k-1-p cells of
     the region, unless the region owns no tet and writes no node in that launch (then it lists nothing).
   * Halo completeness: for every region and pass p > 0, every tet of colour c0+p-1 that shares a node with a tet of
@@ -48,11 +49,20 @@ def test_plan_properties(nodes, region, kmax):
     assert np.all(groups[1:, 0] == groups[:-1, 0] + groups[:-1, 1]) and groups[-1, 0] + groups[-1, 1] == len(offs) - 1
     assert np.all((groups[:, 1] >= 1) & (groups[:, 1] <= kmax))
     hdrs, ents, wrs = plan["headers"], plan["entries"].view(np.uint32), plan["writes"]
-    # every launch takes the same regions in the same order
+    # every launch takes the same regions in the same order: the plan's dense ids
     origins = [hdrs[f:f + n, :3] for f, n in zip(groups[:, 2], groups[:, 3])]
     assert all(np.array_equal(o, origins[0]) for o in origins)
-    assert groups.shape[1] == 8
-    for c0, k, first, nreg, box, halo, cap, entry0 in groups:
+    dense = plan["dense_ids"]
+    assert len(dense) <= tiling.MAX_REGIONS and np.all(np.diff(dense) > 0)
+    RXp, RYp = plan["rx"], plan["ry"]
+    for f, n in zip(groups[:, 2], groups[:, 3]):
+        np.testing.assert_array_equal(hdrs[f:f + n, 6], dense)
+        # the box origin follows from the dense id alone
+        d = dense.astype(np.int64)
+        o = np.stack((d % RXp * S - H, d // RXp % RYp * S - H, d // (RXp * RYp) * S - H), axis=1)
+        np.testing.assert_array_equal(hdrs[f:f + n, :3], o)
+    assert groups.shape[1] == 12
+    for c0, k, first, nreg, box, halo, cap, entry0, wcap, write0, _, _ in groups:
         assert box == B and halo == H
         assert hdrs[first, 3] == entry0
         owned_count = np.zeros(len(desc), dtype=np.int64)
@@ -103,6 +113,8 @@ def test_plan_properties(nodes, region, kmax):
                         need.add(int(t))
                 assert need <= per_pass[p - 1], (rid, p)
             w0, wn = int(h[4]), int(h[5])
+            assert w0 == write0 + (r - first) * wcap and wn <= wcap  # found without the header
+            assert np.all(wrs[w0 + wn:w0 + wcap] == 0)
             w = wrs[w0:w0 + wn].astype(np.int64)
             written[w[:, 1]] += 1
             rel = np.stack((w[:, 1] % nx, (w[:, 1] // nx) % ny, w[:, 1] // nxy), axis=1) - o
