@@ -49,7 +49,7 @@ class GtsTissueView(ct.Structure):
                 ("feasible_needed", P), ("incident_offsets", P), ("incident_tets", P), ("grasp_ring", P),
                 ("edge_exempt", P), ("nbr_init", P), ("xcd_order", I), ("n_bends", I), ("n_bend_colours", I),
                 ("bend_colour_offsets", P), ("bend_desc", P), ("bend_lambda", P), ("tile_groups", I),
-                ("tile_kmax", I), ("tile_box", I), ("tile_tets", I), ("tile_group_info", P), ("tile_headers", P),
+                ("tile_kmax", I), ("tile_box", I), ("tile_tets", I), ("tile_group_info", P),
                 ("tile_entries", P), ("tile_writes", P), ("pos_alt", P), ("tile_inputs", P),
                 ("tile_lambda", P), ("tile_region", I), ("tile_rx", I), ("tile_ry", I), ("tile_dense_ids", P)]
 

@@ -358,7 +358,8 @@ __device__ __forceinline__ int ordered_block(int xcd_order) { return ordered_ite
 // Where a launch's regions find their data: every address follows from the region's index, and its box origin from
 // its dense id, which travels with the launch (a scalar read of the argument segment, no memory round of its own).
 struct TileLaunch {
-    int k, cap, wcap, hdr_stride, box, region, rx, ry, halo, T;
+    int k, stride, wcap, box, region, rx, ry, halo, T;
+    int cap[GTS_TILE_KMAX], start[GTS_TILE_KMAX];  // each pass's rows in a region's block of `stride` entry rows
     int dense[GTS_TILE_MAX_REGIONS];  // dense id rx + rx_count (ry + ry_count rz) of each region (32-bit: a scalar load)
 };
 
@@ -367,17 +368,19 @@ __global__ void k_tile_gather(int n, int T, const int2* __restrict__ entries, co
                               const float* __restrict__ alpha, float4* __restrict__ inputs) {
     const int q = blockIdx.x * GTS_BLOCK + threadIdx.x;
     if (q >= n) return;
-    const int slot = entries[q].x & 0x7fffffff;  // padding rows name slot 0
+    const int slot = entries[q].x & 0x3fffffff;  // padding rows name slot 0
     inputs[q] = make_float4(alpha[slot], alpha[T + slot], __builtin_bit_cast(float, tet_desc[slot]), 0.f);
 }
 
 // One tiled tet projection in the region's LDS box: e the plan entry, in its gathered inputs, lam its multipliers.
+// Padding rows (bit 30 of e.x clear) do nothing.
 template <int MODE, bool WRITE_LAMBDA, bool PRECISE>
 __device__ __forceinline__ void tiled_project(int2 e, float4 in, float2 lam, float4* nodes, const float* invRest,
                                               int box, int box2, const Lattice& L, float omega, int T,
                                               float2* __restrict__ lam_copy, float* __restrict__ lam_out) {
+    if (!(e.x & (1 << 30))) return;
     const bool owned = e.x < 0;
-    const int slot = e.x & 0x7fffffff;
+    const int slot = e.x & 0x3fffffff;
     const uint32_t d = __builtin_bit_cast(uint32_t, in.z);
     float lamD = lam.x, lamH = lam.y;
     bool ok = false;
@@ -404,38 +407,42 @@ __device__ __forceinline__ void tiled_project(int2 e, float4 in, float2 lam, flo
 
 template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool PRECISE>
 __global__ void __launch_bounds__(GTS_TILE_THREADS)
-    k_tets_tiled(TileLaunch A, const int* __restrict__ headers, const int2* __restrict__ entries,
+    k_tets_tiled(TileLaunch A, const int2* __restrict__ entries,
                  const float4* __restrict__ inputs, float2* __restrict__ lam_copy, const int2* __restrict__ writes,
                  Lattice L, float omega, float* __restrict__ lam_out, const float4* __restrict__ pos_in,
                  float4* __restrict__ pos_out) {
     extern __shared__ float4 nodes[];
     __shared__ float invRest[54];  // the six types' Dm^-1, staged with the box (indexed by type, not from arguments)
+    args_ready(L.xcd_order, (int)gridDim.x, pos_in, A.k, A.stride, A.wcap, A.cap[0], A.cap[1], A.cap[2], A.cap[3],
+               A.start[0], A.start[1], A.start[2], A.start[3], entries, inputs, lam_copy, writes);
     const int rl = ordered_block(L.xcd_order), dense = A.dense[rl];
     const int tid = threadIdx.x, bs = GTS_TILE_THREADS;  // the launch's workgroup size, not read from the dispatch
-    const int k = A.k, cap = A.cap, box = A.box, box2 = box * box, box3 = box2 * box;
-    // One round of loads: every address below follows from rl and tid. Rows past a count are padding (in bounds).
-    const int* h = headers + rl * A.hdr_stride;
-    const size_t e0 = (size_t)rl * k * cap;
-    const int row = tid < cap ? tid : cap - 1;
-    int cnt[GTS_TILE_KMAX];
+    const int k = A.k, box = A.box, box2 = box * box, box3 = box2 * box;
+    // One round of loads: every address below follows from rl and tid. Each pass loads the rows of its capacity (the
+    // most any region of the launch holds; the region's rows past its own count are padding, which the rows mark, so
+    // no count is read), the write list those of its.
+    const size_t e0 = (size_t)rl * A.stride;
     int2 en[GTS_TILE_KMAX];
     float4 in[GTS_TILE_KMAX];
     float2 lm[GTS_TILE_KMAX];
 #pragma unroll
     for (int p = 0; p < GTS_TILE_KMAX; p++) {
-        const size_t q = e0 + (size_t)(p < k ? p : 0) * cap + row;
-        cnt[p] = p < k ? h[8 + p] : 0;
-        en[p] = entries[q];
-        in[p] = inputs[q];
-        lm[p] = READ_LAMBDA ? lam_copy[q] : make_float2(0.f, 0.f);
+        en[p] = make_int2(0, 0);
+        in[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        lm[p] = make_float2(0.f, 0.f);
+        if (tid < A.cap[p]) {  // A.cap[p] = 0 past the launch's colours
+            const size_t q = e0 + A.start[p] + tid;
+            en[p] = entries[q];
+            in[p] = inputs[q];
+            if (READ_LAMBDA) lm[p] = lam_copy[q];
+        }
     }
-    const int wcnt = h[5];
     const int2* wr_row = writes + (size_t)rl * A.wcap;
     int2 wr[GTS_TILE_WRITE_LOADS];
 #pragma unroll
     for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++) {
         const int q = tid + j * bs;
-        wr[j] = wr_row[q < A.wcap ? q : A.wcap - 1];
+        wr[j] = wr_row[q < A.wcap ? q : A.wcap - 1];  // clamped, not conditional: a conditional load waits here
     }
     // the box, at the origin of the region's dense id
     const int ox = (dense % A.rx) * A.region - A.halo, oy = ((dense / A.rx) % A.ry) * A.region - A.halo,
@@ -468,11 +475,11 @@ __global__ void __launch_bounds__(GTS_TILE_THREADS)
 #pragma unroll
     for (int p = 0; p < GTS_TILE_KMAX; p++) {
         if (p >= k) break;
-        const size_t pq = e0 + (size_t)p * cap;
-        if (tid < cnt[p])
+        const size_t pq = e0 + A.start[p];
+        if (tid < A.cap[p])
             tiled_project<MODE, WRITE_LAMBDA, PRECISE>(en[p], in[p], lm[p], nodes, invRest, box, box2, L, omega, A.T,
                                                        lam_copy + pq + tid, lam_out);
-        for (int q = tid + bs; q < cnt[p]; q += bs)  // passes of more tets than threads
+        for (int q = tid + bs; q < A.cap[p]; q += bs)  // passes of more rows than threads
             tiled_project<MODE, WRITE_LAMBDA, PRECISE>(entries[pq + q], inputs[pq + q],
                                                        READ_LAMBDA ? lam_copy[pq + q] : make_float2(0.f, 0.f),
                                                        nodes, invRest, box, box2, L, omega, A.T, lam_copy + pq + q,
@@ -480,11 +487,11 @@ __global__ void __launch_bounds__(GTS_TILE_THREADS)
         __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++)
-        if (tid + j * bs < wcnt) pos_out[wr[j].y] = nodes[wr[j].x];
-    for (int q = tid + GTS_TILE_WRITE_LOADS * bs; q < wcnt; q += bs) {
+    for (int j = 0; j < GTS_TILE_WRITE_LOADS; j++)  // padding rows name node -1
+        if (tid + j * bs < A.wcap && wr[j].y >= 0) pos_out[wr[j].y] = nodes[wr[j].x];
+    for (int q = tid + GTS_TILE_WRITE_LOADS * bs; q < A.wcap; q += bs) {
         const int2 w = wr_row[q];
-        pos_out[w.y] = nodes[w.x];
+        if (w.y >= 0) pos_out[w.y] = nodes[w.x];
     }
 }
 
@@ -1601,7 +1608,6 @@ template <int MODE, bool PRECISE>
 void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s, int it, int iters, float omega) {
     const bool read = it > 0, write = iters > 1;
     TileLaunch A;
-    A.hdr_stride = 8 + v->tile_kmax;
     A.box = v->tile_box;
     A.region = v->tile_region;
     A.rx = v->tile_rx;
@@ -1611,12 +1617,17 @@ void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s,
     for (int r = 0; r < nreg; r++) A.dense[r] = v->tile_dense_ids[r];
     const size_t lds = (size_t)v->tile_box * v->tile_box * v->tile_box * sizeof(float4);
     for (int g = 0; g < v->tile_groups; g++) {
-        const int32_t* gi = v->tile_group_info + 12 * g;
-        const int first = gi[2], regions = gi[3];
+        const int32_t* gi = v->tile_group_info + 16 * g;
+        const int regions = gi[3];
         A.k = gi[1];
         A.halo = gi[5];
-        A.cap = gi[6];
+        A.stride = gi[6];
         A.wcap = gi[8];
+        for (int p = 0, start = 0; p < GTS_TILE_KMAX; p++) {
+            A.cap[p] = p < A.k ? gi[10 + p] : 0;
+            A.start[p] = start;
+            start += A.cap[p];
+        }
         const float4* in = reinterpret_cast<const float4*>((g & 1) ? v->pos_alt : v->pos);
         float4* out = reinterpret_cast<float4*>((g & 1) ? v->pos : v->pos_alt);
         if (regions <= 0) continue;
@@ -1624,9 +1635,9 @@ void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s,
         const long long units = (long long)v->tet_colour_offsets[gi[0] + gi[1]] - v->tet_colour_offsets[gi[0]];
 #define GTS_TILED(R, W)                                                                                           \
     profiled_launch_lds(k_tets_tiled<MODE, R, W, PRECISE>, dim3(regions), dim3(GTS_TILE_THREADS), lds, s, units,  \
-        A, v->tile_headers + (size_t)first * A.hdr_stride, reinterpret_cast<const int2*>(v->tile_entries) + gi[7], \
-        reinterpret_cast<const float4*>(v->tile_inputs) + gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7], \
-        reinterpret_cast<const int2*>(v->tile_writes) + gi[9], L, omega, v->tet_lambda, in, out)
+        A, reinterpret_cast<const int2*>(v->tile_entries) + gi[7], reinterpret_cast<const float4*>(v->tile_inputs) + \
+        gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7], reinterpret_cast<const int2*>(v->tile_writes) +  \
+        gi[9], L, omega, v->tet_lambda, in, out)
         if (read && write) GTS_TILED(true, true);
         else if (read) GTS_TILED(true, false);
         else if (write) GTS_TILED(false, true);
@@ -1868,7 +1879,7 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
     // turn it off)
     P.tiled = false;
     if (v->tile_groups > 0 && P.strain && P.dOverH == 0.f && P.bOverH == 0.f && P.guardPasses == 0 &&
-        v->tile_tets == v->n_tets && (v->tile_groups % 2) == 0 && v->tile_group_info && v->tile_headers &&
+        v->tile_tets == v->n_tets && (v->tile_groups % 2) == 0 && v->tile_group_info &&
         v->tile_entries && v->tile_writes && v->pos_alt && v->tile_inputs && v->tile_lambda && v->tile_kmax >= 1 &&
         v->tile_kmax <= GTS_TILE_KMAX && v->tile_region >= 1 && v->tile_rx >= 1 && v->tile_ry >= 1 &&
         v->tile_box >= 2 && (long long)v->tile_box * v->tile_box * v->tile_box * 16 + 256 <= 65536) {
@@ -1877,9 +1888,12 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
         bool ok = v->tile_dense_ids && marked >= 1 && marked <= GTS_TILE_MAX_REGIONS;
         for (int r = 0; r < marked && ok; r++) ok = v->tile_dense_ids[r] >= 0;
         for (int g = 0; g < v->tile_groups && ok; g++) {
-            const int32_t* gi = v->tile_group_info + 12 * g;
+            const int32_t* gi = v->tile_group_info + 16 * g;
+            int rows = 0;
+            for (int p = 0; p < gi[1] && p < GTS_TILE_KMAX; p++) rows += gi[10 + p];
             ok = gi[0] == next && gi[1] >= 1 && gi[1] <= v->tile_kmax && gi[3] == marked && gi[4] == v->tile_box &&
-                 gi[5] == v->tile_kmax - 1 && gi[6] >= 1 && gi[7] >= 0 && gi[8] >= 1 && gi[9] >= 0;
+                 gi[5] == v->tile_kmax - 1 && gi[6] == rows && gi[7] >= 0 && gi[8] >= 1 && gi[9] >= 0;
+            for (int p = 0; p < gi[1] && p < GTS_TILE_KMAX && ok; p++) ok = gi[10 + p] >= 1;
             next += gi[1];
         }
         P.tiled = ok && next == v->n_tet_colours;
@@ -1924,8 +1938,8 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
     launch_refresh(v, L, s);
     if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, P.invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
     if (P.tiled) {  // each plan entry's copy of its tet's inputs, after this step's liveness and compliances
-        const int32_t* last = v->tile_group_info + 12 * (v->tile_groups - 1);
-        const int entries = last[7] + last[3] * last[1] * last[6];
+        const int32_t* last = v->tile_group_info + 16 * (v->tile_groups - 1);
+        const int entries = last[7] + last[3] * last[6];
         k_tile_gather<<<grid_for(entries), GTS_BLOCK, 0, s>>>(entries, T, reinterpret_cast<const int2*>(v->tile_entries),
                                                              v->tet_desc, v->tet_alpha,
                                                              reinterpret_cast<float4*>(v->tile_inputs));
@@ -2090,7 +2104,7 @@ hipStream_t g_capture = nullptr;
 std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
     const size_t ne = (size_t)v->n_edge_colours + 1, nt = (size_t)v->n_tet_colours + 1;
     const size_t nb = P.bend ? (size_t)v->n_bend_colours + 1 : 0;
-    const size_t ng = P.tiled ? (size_t)12 * v->tile_groups : 0;
+    const size_t ng = P.tiled ? (size_t)16 * v->tile_groups : 0;
     const size_t nd = P.tiled ? (size_t)v->tile_group_info[3] : 0;  // the dense ids the launches carry
     std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) +
                                  (ne + nt + nb + ng + nd) * sizeof(int32_t));

@@ -47,16 +47,18 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
     """Plan of the tiled sweep for a lattice of `dims` cells and its colour-major tet descriptors.
 
     Returns a dict of numpy arrays:
-      groups  int32 [G, 12]: first colour, colours, first region header, regions, box width, halo of the box, pass
-              capacity (the most tets any region's pass holds), first entry, write capacity (the most nodes any region
-              writes), first write, 0, 0
+      groups  int32 [G, 16]: first colour, colours, first region header, regions, box width, halo of the box, entry
+              rows a region (the sum of the pass capacities), first entry, write capacity (the most nodes any region
+              writes), first write, then the capacity of each pass p < 4 (the most tets any region's pass p holds;
+              0 past the group's colours), 0, 0
       headers int32 [R, HDR + kmax]: box origin (x, y, z), entry offset, write offset, write count, dense region id, 0,
-              then the entry count of each pass
-      entries int32 [M, 2]: tet slot | owned << 31, the tet's corner-0 index in the region's LDS box. The regions of a
-              group take k * capacity rows each, in header order from the group's first entry, pass p at rows
-              p * capacity .. p * capacity + count - 1 (the rest zero)
+              then the entry count of each pass (a description of the regions for the host; the kernel derives all
+              of it from the region's index, its dense id and the rows' padding marks)
+      entries int32 [M, 2]: tet slot | 1 << 30 | owned << 31, the tet's corner-0 index in the region's LDS box (padding
+              rows are zero: bit 30 clear). The regions of a group take (entry rows) rows each, in header order from
+              the group's first entry, pass p from row cap_0 + .. + cap_(p-1) of its block, count rows of it used
       writes  int32 [W, 2]: LDS box index, node id; each region of a group takes write-capacity rows in header order
-              from the group's first write (the rest zero)
+              from the group's first write (padding rows: node id -1)
       dense_ids int32 [R per launch]: the dense id r = rx + RX (ry + RY rz) of each region, in header order (the
               same in every launch)
     with region, kmax, box, halo, RX, RY and the tet count, or None when the sweep cannot be tiled (fewer than two
@@ -106,7 +108,7 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
                 r = qx[ok] + RX * (qy[ok] + RY * qz[ok])
                 lb = (x[ok] - (qx[ok] * S - H)) + B * ((y[ok] - (qy[ok] * S - H)) + B * (z[ok] - (qz[ok] * S - H)))
                 owned = 1 if (dx, dy, dz) == (0, 0, 0) else 0
-                rows.append(np.stack((r, np.full(len(r), p), slot[ok] | (owned << 31), lb), axis=1))
+                rows.append(np.stack((r, np.full(len(r), p), slot[ok] | (owned << 31) | (1 << 30), lb), axis=1))
             # the last colour of the launch touching a node decides its writer: the region of that tet's cell
             own = rx + RX * (ry + RY * rz)
             for q in range(4):
@@ -136,7 +138,9 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
         for r in regions:
             sel = rows[rows[:, 0] == r]
             per_region.append((sel, np.bincount(sel[:, 1].astype(np.int64), minlength=k), all_nodes[wr == r]))
-        cap = max(1, max(int(c.max()) for _, c, _ in per_region))
+        caps = [max(1, max(int(c[p]) for _, c, _ in per_region)) for p in range(k)]
+        starts = np.concatenate(([0], np.cumsum(caps)))
+        stride = int(starts[-1])
         wcap = max(1, max(len(nd) for _, _, nd in per_region))
         for r, (sel, counts, nodes) in zip(regions, per_region):
             rx, ry, rz = r % RX, (r // RX) % RY, r // (RX * RY)
@@ -147,21 +151,23 @@ def build_tile_plan(dims, tet_desc, tet_offsets, region=8, kmax=3):
             hdr[:7] = (ox, oy, oz, n_entries, n_writes, len(nodes), r)
             hdr[HDR:HDR + k] = counts
             headers.append(hdr)
-            padded = np.zeros((k * cap, 2), dtype=np.int64)
+            padded = np.zeros((stride, 2), dtype=np.int64)
             pos = 0
             for p in range(k):  # sel is sorted by pass
-                padded[p * cap:p * cap + counts[p]] = sel[pos:pos + counts[p], 2:4]
+                padded[starts[p]:starts[p] + counts[p]] = sel[pos:pos + counts[p], 2:4]
                 pos += counts[p]
             entries.append(padded)
             wpad = np.zeros((wcap, 2), dtype=np.int64)
+            wpad[:, 1] = -1  # padding: no node
             wpad[:len(nodes)] = np.stack((local, nodes), axis=1)
             writes.append(wpad)
-            n_entries += k * cap
+            n_entries += stride
             n_writes += wcap
-        groups.append((c0, k, g_first, len(regions), B, H, cap, g_entry, wcap, g_write, 0, 0))
+        groups.append((c0, k, g_first, len(regions), B, H, stride, g_entry, wcap, g_write,
+                       *(caps + [0] * (4 - k)), 0, 0))
     as32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64).astype(np.int32))
     ent = np.concatenate(entries) if entries else np.zeros((0, 2), dtype=np.int64)
-    # slot | owned << 31 wraps to a negative int32, which the kernel reads back as the same 32 bits
+    # slot | 1 << 30 | owned << 31 wraps to a negative int32, which the kernel reads back as the same 32 bits
     ent32 = np.ascontiguousarray(ent.astype(np.uint64).astype(np.uint32).view(np.int32).reshape(-1, 2))
     return dict(groups=as32(groups), headers=as32(np.stack(headers)), entries=ent32,
                 writes=as32(np.concatenate(writes)), dense_ids=as32(regions), region=S, kmax=int(kmax), box=B,

@@ -277,7 +277,6 @@ class Tissue:
         dev = self.device
         n_entries = len(plan["entries"])
         keep = dict(plan=plan, groups=np.ascontiguousarray(plan["groups"].reshape(-1)),
-                    headers=torch.as_tensor(plan["headers"].reshape(-1)).to(dev),
                     entries=torch.as_tensor(plan["entries"].reshape(-1)).to(dev),
                     writes=torch.as_tensor(plan["writes"].reshape(-1)).to(dev),
                     pos_alt=self.pos.clone(),
@@ -287,7 +286,7 @@ class Tissue:
         v.tile_groups = len(plan["groups"])
         v.tile_kmax, v.tile_box, v.tile_tets = plan["kmax"], plan["box"], plan["tets"]
         v.tile_group_info = keep["groups"].ctypes.data
-        v.tile_headers, v.tile_entries = keep["headers"].data_ptr(), keep["entries"].data_ptr()
+        v.tile_entries = keep["entries"].data_ptr()
         v.tile_writes = keep["writes"].data_ptr()
         v.pos_alt = keep["pos_alt"].data_ptr()
         v.tile_inputs, v.tile_lambda = keep["inputs"].data_ptr(), keep["lam"].data_ptr()

@@ -159,22 +159,20 @@ typedef struct GtsTissueView {
      * tile_groups launches of up to tile_kmax consecutive colours each, one workgroup per lattice region, positions in
      * an LDS box of tile_box^3 nodes around the region (the plan: genesis_tissue_simulation_amd/tiling.py). Each launch
      * reads the positions from one buffer and writes them to the other (pos, pos_alt alternate; tile_groups is even).
-     * Every input of a region sits at an address it computes from its index: its header, entries and writes at fixed
-     * strides, its box origin from its dense id (a kernel argument), and per entry the tet's inputs (tile_inputs, gathered at the start of
+     * Every input of a region sits at an address it computes from its index: its entries and writes at fixed strides, its box origin from its dense id (a kernel argument), and per entry the tet's inputs (tile_inputs, gathered at the start of
      * each step) and its multipliers (tile_lambda: each region keeps the copies of the tets it projects; the owners
      * also write tet_lambda). The plan holds for tile_tets tets in the current colour lists; a view whose lists
      * differ (a compaction) takes the per-colour launches. */
     int tile_groups;
     int tile_kmax, tile_box, tile_tets;
-    const int32_t* tile_group_info;     /* host [tile_groups * 12]: first colour, colours, first header, regions, box
-                                         * width, box halo, pass capacity, first entry, write capacity, first
-                                         * write, 0, 0 */
-    const int32_t* tile_headers;        /* [regions * (8 + tile_kmax)]: box origin x, y, z, entry offset, write offset,
-                                         * write count, dense region id, 0, entries of each pass */
-    const int32_t* tile_entries;        /* [2 * entries]: tet slot | owned << 31, corner-0 index in the LDS box; a
-                                         * region's pass p holds rows p * capacity .. p * capacity + count - 1 of
-                                         * its k * capacity rows, which start at first entry + region * k * capacity */
-    const int32_t* tile_writes;         /* [2 * writes]: LDS box index, node id; write capacity rows a region */
+    const int32_t* tile_group_info;     /* host [tile_groups * 16]: first colour, colours, first region, regions, box
+                                         * width, box halo, entry rows a region, first entry, write capacity, first
+                                         * write, the capacity of each of 4 passes, 0, 0 */
+    const int32_t* tile_entries;        /* [2 * entries]: tet slot | 1 << 30 | owned << 31, corner-0 index in the LDS
+                                         * box (padding rows: 0, 0); a region's rows start at first entry + region *
+                                         * (entry rows), pass p's after the capacities of the passes before it */
+    const int32_t* tile_writes;         /* [2 * writes]: LDS box index, node id (padding rows: node -1); write
+                                         * capacity rows a region */
     float* pos_alt;                     /* [4N] the second position buffer */
     float* tile_inputs;                 /* [4 * entries] per entry: alpha_D, alpha_H, the descriptor's bits, 0 */
     float* tile_lambda;                 /* [2 * entries] per entry: its copy of the tet's multipliers */

@@ -61,8 +61,11 @@ def test_plan_properties(nodes, region, kmax):
         d = dense.astype(np.int64)
         o = np.stack((d % RXp * S - H, d // RXp % RYp * S - H, d // (RXp * RYp) * S - H), axis=1)
         np.testing.assert_array_equal(hdrs[f:f + n, :3], o)
-    assert groups.shape[1] == 12
-    for c0, k, first, nreg, box, halo, cap, entry0, wcap, write0, _, _ in groups:
+    assert groups.shape[1] == 16
+    for c0, k, first, nreg, box, halo, stride, entry0, wcap, write0, *rest in groups:
+        caps = rest[:k]
+        starts = np.concatenate(([0], np.cumsum(caps)))
+        assert stride == starts[-1] and all(c >= 1 for c in caps) and all(c == 0 for c in rest[k:])
         assert box == B and halo == H
         assert hdrs[first, 3] == entry0
         owned_count = np.zeros(len(desc), dtype=np.int64)
@@ -75,8 +78,8 @@ def test_plan_properties(nodes, region, kmax):
             region_xyz = (o + H) // S
             rid = region_xyz[0] + RX * (region_xyz[1] + RY * region_xyz[2])
             e = int(h[3])
-            assert e == entry0 + (r - first) * k * cap  # found without the header
-            assert np.all(h[tiling.HDR:tiling.HDR + k] <= cap) and h[tiling.HDR:tiling.HDR + k].max() >= 0
+            assert e == entry0 + (r - first) * stride  # found without the header
+            assert np.all(h[tiling.HDR:tiling.HDR + k] <= caps)
             per_pass = []
             # a region that owns no tet of these colours and writes no node is idle in this launch: no entries
             mine_here = np.all(cell // S == region_xyz, axis=1) & (colour >= c0) & (colour < c0 + k)
@@ -86,9 +89,10 @@ def test_plan_properties(nodes, region, kmax):
                 regions_idle += 1
             for p in range(k):
                 cnt = int(h[tiling.HDR + p])
-                rows = ents[e + p * cap:e + p * cap + cnt]
-                assert np.all(ents[e + p * cap + cnt:e + (p + 1) * cap] == 0)
-                slots = (rows[:, 0] & 0x7fffffff).astype(np.int64)
+                rows = ents[e + starts[p]:e + starts[p] + cnt]
+                assert np.all(ents[e + starts[p] + cnt:e + starts[p + 1]] == 0)
+                assert np.all((rows[:, 0] >> 30) & 1)  # real rows carry bit 30
+                slots = (rows[:, 0] & 0x3fffffff).astype(np.int64)
                 owned = (rows[:, 0] >> 31).astype(bool)
                 assert np.all(colour[slots] == c0 + p)
                 lo, hi = region_xyz * S - (k - 1 - p), (region_xyz + 1) * S + (k - 1 - p)
@@ -114,7 +118,7 @@ def test_plan_properties(nodes, region, kmax):
                 assert need <= per_pass[p - 1], (rid, p)
             w0, wn = int(h[4]), int(h[5])
             assert w0 == write0 + (r - first) * wcap and wn <= wcap  # found without the header
-            assert np.all(wrs[w0 + wn:w0 + wcap] == 0)
+            assert np.all(wrs[w0 + wn:w0 + wcap, 1] == -1)
             w = wrs[w0:w0 + wn].astype(np.int64)
             written[w[:, 1]] += 1
             rel = np.stack((w[:, 1] % nx, (w[:, 1] // nx) % ny, w[:, 1] // nxy), axis=1) - o
