@@ -51,7 +51,8 @@ class GtsTissueView(ct.Structure):
                 ("bend_colour_offsets", P), ("bend_desc", P), ("bend_lambda", P), ("tile_groups", I),
                 ("tile_kmax", I), ("tile_box", I), ("tile_tets", I), ("tile_group_info", P),
                 ("tile_entries", P), ("tile_writes", P), ("pos_alt", P), ("tile_inputs", P),
-                ("tile_lambda", P), ("tile_region", I), ("tile_rx", I), ("tile_ry", I), ("tile_dense_ids", P)]
+                ("tile_lambda", P), ("tile_region", I), ("tile_rx", I), ("tile_ry", I), ("tile_dense_ids", P),
+                ("tile_edges", I)]
 
 
 class GtsStepParams(ct.Structure):

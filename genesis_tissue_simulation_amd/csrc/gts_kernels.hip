@@ -363,13 +363,15 @@ struct TileLaunch {
     int dense[GTS_TILE_MAX_REGIONS];  // dense id rx + rx_count (ry + ry_count rz) of each region (32-bit: a scalar load)
 };
 
-// Copies each plan entry's tet inputs into the entry: alpha_D, alpha_H and the descriptor's bits (liveness, type).
-__global__ void k_tile_gather(int n, int T, const int2* __restrict__ entries, const uint32_t* __restrict__ tet_desc,
+// Copies each plan entry's inputs into the entry: for a tet alpha_D, alpha_H and the descriptor's bits (liveness,
+// type); for an edge (alpha null) 0, 0 and the descriptor's bits (liveness, axis).
+__global__ void k_tile_gather(int n, int T, const int2* __restrict__ entries, const uint32_t* __restrict__ desc,
                               const float* __restrict__ alpha, float4* __restrict__ inputs) {
     const int q = blockIdx.x * GTS_BLOCK + threadIdx.x;
     if (q >= n) return;
     const int slot = entries[q].x & 0x3fffffff;  // padding rows name slot 0
-    inputs[q] = make_float4(alpha[slot], alpha[T + slot], __builtin_bit_cast(float, tet_desc[slot]), 0.f);
+    inputs[q] = make_float4(alpha ? alpha[slot] : 0.f, alpha ? alpha[T + slot] : 0.f,
+                            __builtin_bit_cast(float, desc[slot]), 0.f);
 }
 
 // One tiled tet projection in the region's LDS box: e the plan entry, in its gathered inputs, lam its multipliers.
@@ -405,12 +407,35 @@ __device__ __forceinline__ void tiled_project(int2 e, float4 in, float2 lam, flo
     }
 }
 
-template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool PRECISE>
+// One tiled edge projection (k_edges, undamped, without the fallback compliance): e.y is node i's index in the box,
+// node j lies one node further along the edge's axis. lam.x is the multiplier; the owner writes edge_lambda.
+template <bool WRITE_LAMBDA>
+__device__ __forceinline__ void tiled_edge_project(int2 e, float4 in, float2 lam, float4* nodes, int box, int box2,
+                                                   const Lattice& L, float alphaTilde, float omega,
+                                                   float2* __restrict__ lam_copy, float* __restrict__ lam_out) {
+    if (!(e.x & (1 << 30))) return;
+    const uint32_t d = __builtin_bit_cast(uint32_t, in.z);
+    float out = lam.x;
+    bool ok = false;
+    if (d & 1u) {  // a dead edge keeps its multiplier: 0 from the first iteration's reset
+        const int axis = (int)((d >> 1) & 3u);
+        const float restLen = axis == 0 ? L.rest_len[0] : (axis == 1 ? L.rest_len[1] : L.rest_len[2]);
+        const int i = e.y, j = i + (axis == 0 ? 1 : (axis == 1 ? box : box2));
+        ok = edge_project<false>(LdsNodes{nodes}, nullptr, i, j, restLen, alphaTilde, omega, 0.f, lam.x, out);
+    }
+    if (WRITE_LAMBDA) {
+        const float v = ok ? out : lam.x;
+        *lam_copy = make_float2(v, 0.f);
+        if (e.x < 0) lam_out[e.x & 0x3fffffff] = v;
+    }
+}
+
+template <int MODE, bool READ_LAMBDA, bool WRITE_LAMBDA, bool PRECISE, bool EDGES>
 __global__ void __launch_bounds__(GTS_TILE_THREADS)
     k_tets_tiled(TileLaunch A, const int2* __restrict__ entries,
                  const float4* __restrict__ inputs, float2* __restrict__ lam_copy, const int2* __restrict__ writes,
-                 Lattice L, float omega, float* __restrict__ lam_out, const float4* __restrict__ pos_in,
-                 float4* __restrict__ pos_out) {
+                 Lattice L, float omega, float alphaTilde, float* __restrict__ lam_out,
+                 const float4* __restrict__ pos_in, float4* __restrict__ pos_out) {
     extern __shared__ float4 nodes[];
     __shared__ float invRest[54];  // the six types' Dm^-1, staged with the box (indexed by type, not from arguments)
     args_ready(L.xcd_order, (int)gridDim.x, pos_in, A.k, A.stride, A.wcap, A.cap[0], A.cap[1], A.cap[2], A.cap[3],
@@ -476,14 +501,24 @@ __global__ void __launch_bounds__(GTS_TILE_THREADS)
     for (int p = 0; p < GTS_TILE_KMAX; p++) {
         if (p >= k) break;
         const size_t pq = e0 + A.start[p];
-        if (tid < A.cap[p])
-            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(en[p], in[p], lm[p], nodes, invRest, box, box2, L, omega, A.T,
-                                                       lam_copy + pq + tid, lam_out);
-        for (int q = tid + bs; q < A.cap[p]; q += bs)  // passes of more rows than threads
-            tiled_project<MODE, WRITE_LAMBDA, PRECISE>(entries[pq + q], inputs[pq + q],
-                                                       READ_LAMBDA ? lam_copy[pq + q] : make_float2(0.f, 0.f),
-                                                       nodes, invRest, box, box2, L, omega, A.T, lam_copy + pq + q,
-                                                       lam_out);
+        if (EDGES) {
+            if (tid < A.cap[p])
+                tiled_edge_project<WRITE_LAMBDA>(en[p], in[p], lm[p], nodes, box, box2, L, alphaTilde, omega,
+                                                 lam_copy + pq + tid, lam_out);
+            for (int q = tid + bs; q < A.cap[p]; q += bs)
+                tiled_edge_project<WRITE_LAMBDA>(entries[pq + q], inputs[pq + q],
+                                                 READ_LAMBDA ? lam_copy[pq + q] : make_float2(0.f, 0.f), nodes, box,
+                                                 box2, L, alphaTilde, omega, lam_copy + pq + q, lam_out);
+        } else {
+            if (tid < A.cap[p])
+                tiled_project<MODE, WRITE_LAMBDA, PRECISE>(en[p], in[p], lm[p], nodes, invRest, box, box2, L, omega,
+                                                           A.T, lam_copy + pq + tid, lam_out);
+            for (int q = tid + bs; q < A.cap[p]; q += bs)  // passes of more rows than threads
+                tiled_project<MODE, WRITE_LAMBDA, PRECISE>(entries[pq + q], inputs[pq + q],
+                                                           READ_LAMBDA ? lam_copy[pq + q] : make_float2(0.f, 0.f),
+                                                           nodes, invRest, box, box2, L, omega, A.T,
+                                                           lam_copy + pq + q, lam_out);
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -1601,11 +1636,12 @@ void launch_bend_colour_d(const GtsTissueView* v, const Lattice& L, hipStream_t 
 #undef GTS_BEND
 }
 
-// One iteration's tet sweep as the tiled launches: positions alternate between pos and pos_alt from launch to launch
-// (an even number of them, so the sweep ends in pos); each entry's multipliers are read from its copy, which the
-// previous iteration's projection of that entry wrote.
-template <int MODE, bool PRECISE>
-void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s, int it, int iters, float omega) {
+// One iteration's tet sweep (EDGES: edge sweep) as the tiled launches of the plan's groups of that kind: positions
+// alternate between pos and pos_alt from launch to launch (an even number of them, so the sweep ends in pos); each
+// entry's multipliers are read from its copy, which the previous iteration's projection of that entry wrote.
+template <int MODE, bool PRECISE, bool EDGES>
+void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s, int it, int iters, float omega,
+                        float alphaTilde) {
     const bool read = it > 0, write = iters > 1;
     TileLaunch A;
     A.box = v->tile_box;
@@ -1616,8 +1652,11 @@ void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s,
     const int nreg = v->tile_group_info[3];  // every launch takes the same regions
     for (int r = 0; r < nreg; r++) A.dense[r] = v->tile_dense_ids[r];
     const size_t lds = (size_t)v->tile_box * v->tile_box * v->tile_box * sizeof(float4);
-    for (int g = 0; g < v->tile_groups; g++) {
+    const int* offsets = EDGES ? v->edge_colour_offsets : v->tet_colour_offsets;
+    float* lam_out = EDGES ? v->edge_lambda : v->tet_lambda;
+    for (int g = 0, n = 0; g < v->tile_groups; g++) {
         const int32_t* gi = v->tile_group_info + 16 * g;
+        if (gi[14] != (EDGES ? 1 : 0)) continue;
         const int regions = gi[3];
         A.k = gi[1];
         A.halo = gi[5];
@@ -1628,16 +1667,17 @@ void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s,
             A.start[p] = start;
             start += A.cap[p];
         }
-        const float4* in = reinterpret_cast<const float4*>((g & 1) ? v->pos_alt : v->pos);
-        float4* out = reinterpret_cast<float4*>((g & 1) ? v->pos : v->pos_alt);
+        const float4* in = reinterpret_cast<const float4*>((n & 1) ? v->pos_alt : v->pos);
+        float4* out = reinterpret_cast<float4*>((n & 1) ? v->pos : v->pos_alt);
+        n++;
         if (regions <= 0) continue;
-        // tet updates of the launch (its colours' tets), credited while profiling
-        const long long units = (long long)v->tet_colour_offsets[gi[0] + gi[1]] - v->tet_colour_offsets[gi[0]];
+        // updates of the launch (its colours' elements), credited while profiling
+        const long long units = (long long)offsets[gi[0] + gi[1]] - offsets[gi[0]];
 #define GTS_TILED(R, W)                                                                                           \
-    profiled_launch_lds(k_tets_tiled<MODE, R, W, PRECISE>, dim3(regions), dim3(GTS_TILE_THREADS), lds, s, units,  \
-        A, reinterpret_cast<const int2*>(v->tile_entries) + gi[7], reinterpret_cast<const float4*>(v->tile_inputs) + \
-        gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7], reinterpret_cast<const int2*>(v->tile_writes) +  \
-        gi[9], L, omega, v->tet_lambda, in, out)
+    profiled_launch_lds(k_tets_tiled<MODE, R, W, PRECISE, EDGES>, dim3(regions), dim3(GTS_TILE_THREADS), lds, s,  \
+        units, A, reinterpret_cast<const int2*>(v->tile_entries) + gi[7],                                         \
+        reinterpret_cast<const float4*>(v->tile_inputs) + gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7], \
+        reinterpret_cast<const int2*>(v->tile_writes) + gi[9], L, omega, alphaTilde, lam_out, in, out)
         if (read && write) GTS_TILED(true, true);
         else if (read) GTS_TILED(true, false);
         else if (write) GTS_TILED(false, true);
@@ -1792,8 +1832,8 @@ struct StepPlan {  // host-side constants of one outer step (xpbd_begin_frame, x
     // bending: compliance 1/(kb h^2) and its Kelvin-Voigt factor (xpbd_begin_frame, xpbd_solver.cu:2496-2499)
     bool bend;
     float alphaB, betaGammaHB;
-    // the tet sweep as tiled launches (GtsTissueView.tile_*)
-    bool tiled;
+    // the tet and edge sweeps as tiled launches (GtsTissueView.tile_*)
+    bool tiled, tiledEdges;
 };
 
 // cut_ribbon_tick without the pressure blade (cut.cu:1303-1431) after substep n: CCD of the edges moving from the
@@ -1875,28 +1915,39 @@ int make_plan(const GtsTissueView* v, const GtsStepParams* p, StepPlan& P) {
         P.alphaB = 1.f / (p->bend_kb * h * h);
         P.betaGammaHB = P.alphaB * p->beta_b * h;
     }
-    // the tiled tet sweep: a plan for the current lists, undamped and unguarded (the legacy step and the ribbon tick
-    // turn it off)
-    P.tiled = false;
-    if (v->tile_groups > 0 && P.strain && P.dOverH == 0.f && P.bOverH == 0.f && P.guardPasses == 0 &&
-        v->tile_tets == v->n_tets && (v->tile_groups % 2) == 0 && v->tile_group_info &&
-        v->tile_entries && v->tile_writes && v->pos_alt && v->tile_inputs && v->tile_lambda && v->tile_kmax >= 1 &&
-        v->tile_kmax <= GTS_TILE_KMAX && v->tile_region >= 1 && v->tile_rx >= 1 && v->tile_ry >= 1 &&
-        v->tile_box >= 2 && (long long)v->tile_box * v->tile_box * v->tile_box * 16 + 256 <= 65536) {
-        int next = 0;
+    // the tiled sweeps: a plan for the current lists; tets undamped and unguarded, edges undamped without the fallback
+    // compliance (the legacy step and the ribbon tick turn both off)
+    P.tiled = P.tiledEdges = false;
+    if (v->tile_groups > 0 && (v->tile_groups % 2) == 0 && v->tile_group_info && v->tile_entries && v->tile_writes &&
+        v->pos_alt && v->tile_inputs && v->tile_lambda && v->tile_kmax >= 1 && v->tile_kmax <= GTS_TILE_KMAX &&
+        v->tile_region >= 1 && v->tile_rx >= 1 && v->tile_ry >= 1 && v->tile_box >= 2 &&
+        (long long)v->tile_box * v->tile_box * v->tile_box * 16 + 256 <= 65536) {
+        int next[2] = {0, 0}, count[2] = {0, 0};
         const int marked = v->tile_group_info[3];
         bool ok = v->tile_dense_ids && marked >= 1 && marked <= GTS_TILE_MAX_REGIONS;
         for (int r = 0; r < marked && ok; r++) ok = v->tile_dense_ids[r] >= 0;
         for (int g = 0; g < v->tile_groups && ok; g++) {
             const int32_t* gi = v->tile_group_info + 16 * g;
+            const int kind = gi[14];
+            if (kind != 0 && kind != 1) {
+                ok = false;
+                break;
+            }
             int rows = 0;
             for (int p = 0; p < gi[1] && p < GTS_TILE_KMAX; p++) rows += gi[10 + p];
-            ok = gi[0] == next && gi[1] >= 1 && gi[1] <= v->tile_kmax && gi[3] == marked && gi[4] == v->tile_box &&
-                 gi[5] == v->tile_kmax - 1 && gi[6] == rows && gi[7] >= 0 && gi[8] >= 1 && gi[9] >= 0;
+            // an edge group's colours fit the box's halo: at most tile_kmax - 1 of them
+            ok = gi[0] == next[kind] && gi[1] >= 1 && gi[1] <= v->tile_kmax - kind && gi[3] == marked &&
+                 gi[4] == v->tile_box && gi[5] == v->tile_kmax - 1 && gi[6] == rows && gi[7] >= 0 && gi[8] >= 1 &&
+                 gi[9] >= 0;
             for (int p = 0; p < gi[1] && p < GTS_TILE_KMAX && ok; p++) ok = gi[10 + p] >= 1;
-            next += gi[1];
+            next[kind] += gi[1];
+            count[kind]++;
         }
-        P.tiled = ok && next == v->n_tet_colours;
+        ok = ok && count[0] % 2 == 0 && count[1] % 2 == 0;
+        P.tiled = ok && count[0] > 0 && next[0] == v->n_tet_colours && v->tile_tets == v->n_tets && P.strain &&
+                  P.dOverH == 0.f && P.bOverH == 0.f && P.guardPasses == 0;
+        P.tiledEdges = ok && count[1] > 0 && next[1] == v->n_edge_colours && v->tile_edges == v->n_edges &&
+                       P.edges && !P.fallback && P.betaGammaH == 0.f;
     }
     return GTS_OK;
 }
@@ -1937,12 +1988,21 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
     const int T = v->n_tets, N = v->n_nodes;
     launch_refresh(v, L, s);
     if (T > 0) k_tet_alpha<<<grid_for(T), GTS_BLOCK, 0, s>>>(T, P.invH2, v->tet_k_dev, v->tet_k_hyd, v->tet_alpha);
-    if (P.tiled) {  // each plan entry's copy of its tet's inputs, after this step's liveness and compliances
-        const int32_t* last = v->tile_group_info + 16 * (v->tile_groups - 1);
-        const int entries = last[7] + last[3] * last[6];
-        k_tile_gather<<<grid_for(entries), GTS_BLOCK, 0, s>>>(entries, T, reinterpret_cast<const int2*>(v->tile_entries),
-                                                             v->tet_desc, v->tet_alpha,
-                                                             reinterpret_cast<float4*>(v->tile_inputs));
+    // each plan entry's copy of its element's inputs, after this step's liveness and compliances
+    for (int kind = 0; kind < 2; kind++) {
+        if (!(kind ? P.tiledEdges : P.tiled)) continue;
+        int first = -1, end = 0;  // a kind's entries are contiguous
+        for (int g = 0; g < v->tile_groups; g++) {
+            const int32_t* gi = v->tile_group_info + 16 * g;
+            if (gi[14] != kind) continue;
+            if (first < 0) first = gi[7];
+            end = gi[7] + gi[3] * gi[6];
+        }
+        const int n = end - first;
+        if (first < 0 || n <= 0) continue;
+        k_tile_gather<<<grid_for(n), GTS_BLOCK, 0, s>>>(
+            n, T, reinterpret_cast<const int2*>(v->tile_entries) + first, kind ? v->edge_desc : v->tet_desc,
+            kind ? nullptr : v->tet_alpha, reinterpret_cast<float4*>(v->tile_inputs) + first);
     }
     if (P.contact || P.legacy) k_fill_float<<<1, 64, 0, s>>>(4, v->reaction, 0.f);
     const StepShapes* shapes = reinterpret_cast<const StepShapes*>(v->step_consts);
@@ -1962,7 +2022,9 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
         for (int k = 0; k < P.K; k++) {
             for (int i = 0; i < P.iters; i++) {
                 const int it = k * P.iters + i;
-                if (P.edges)
+                if (P.edges && P.tiledEdges)
+                    launch_tiled_sweep<0, false, true>(v, L, s, it, passes, P.omega, P.at);
+                else if (P.edges)
                     for (int c = 0; c < v->n_edge_colours; c++) {
                         int off = v->edge_colour_offsets[c], cnt = v->edge_colour_offsets[c + 1] - off;
                         if (cnt > 0) launch_edge_colour(v, L, s, off, cnt, it, passes, P.fallback, P.at, P.fb, P.omega,
@@ -1975,12 +2037,13 @@ void enqueue_step(const GtsTissueView* v, const Lattice& L, const StepPlan& P, h
                                                         P.betaGammaHB);
                     }
                 if (P.strain && P.tiled) {
-                    if (P.mode == 0 && P.precise) launch_tiled_sweep<0, true>(v, L, s, it, passes, P.omega);
-                    else if (P.mode == 0) launch_tiled_sweep<0, false>(v, L, s, it, passes, P.omega);
-                    else if (P.mode == 1 && P.precise) launch_tiled_sweep<1, true>(v, L, s, it, passes, P.omega);
-                    else if (P.mode == 1) launch_tiled_sweep<1, false>(v, L, s, it, passes, P.omega);
-                    else if (P.precise) launch_tiled_sweep<2, true>(v, L, s, it, passes, P.omega);
-                    else launch_tiled_sweep<2, false>(v, L, s, it, passes, P.omega);
+                    if (P.mode == 0 && P.precise) launch_tiled_sweep<0, true, false>(v, L, s, it, passes, P.omega, 0.f);
+                    else if (P.mode == 0) launch_tiled_sweep<0, false, false>(v, L, s, it, passes, P.omega, 0.f);
+                    else if (P.mode == 1 && P.precise)
+                        launch_tiled_sweep<1, true, false>(v, L, s, it, passes, P.omega, 0.f);
+                    else if (P.mode == 1) launch_tiled_sweep<1, false, false>(v, L, s, it, passes, P.omega, 0.f);
+                    else if (P.precise) launch_tiled_sweep<2, true, false>(v, L, s, it, passes, P.omega, 0.f);
+                    else launch_tiled_sweep<2, false, false>(v, L, s, it, passes, P.omega, 0.f);
                 } else if (P.strain)
                     for (int c = 0; c < v->n_tet_colours; c++) {
                         int off = v->tet_colour_offsets[c], cnt = v->tet_colour_offsets[c + 1] - off;
@@ -2104,8 +2167,9 @@ hipStream_t g_capture = nullptr;
 std::vector<unsigned char> graph_key(const GtsTissueView* v, const StepPlan& P) {
     const size_t ne = (size_t)v->n_edge_colours + 1, nt = (size_t)v->n_tet_colours + 1;
     const size_t nb = P.bend ? (size_t)v->n_bend_colours + 1 : 0;
-    const size_t ng = P.tiled ? (size_t)16 * v->tile_groups : 0;
-    const size_t nd = P.tiled ? (size_t)v->tile_group_info[3] : 0;  // the dense ids the launches carry
+    const bool tiled = P.tiled || P.tiledEdges;
+    const size_t ng = tiled ? (size_t)16 * v->tile_groups : 0;
+    const size_t nd = tiled ? (size_t)v->tile_group_info[3] : 0;  // the dense ids the launches carry
     std::vector<unsigned char> k(sizeof(GtsTissueView) + sizeof(StepPlan) +
                                  (ne + nt + nb + ng + nd) * sizeof(int32_t));
     unsigned char* d = k.data();
@@ -2168,7 +2232,8 @@ namespace {
 int plan_tick(const GtsTissueView* v, const GtsCutTool* cut, const float* emit_ncut, float tear_tau, StepPlan& P) {
     if (!emit_ncut || !v->prev_corner) return GTS_ERR_BAD_ARG;
     P.tick = true;
-    P.tiled = false;  // the tick changes liveness within the step, after the step's gather of the tiled inputs
+    P.tiled = P.tiledEdges = false;  // the tick changes liveness within the step, after the step's gather of the tiled
+                                     // inputs
     P.tickS = ld3(cut->t2v2);  // the current rod: S = t2v2, E = t1v2 (cut.cu:1382-1383)
     P.tickE = ld3(cut->t1v2);
     P.tickN = ld3(emit_ncut);
@@ -2381,7 +2446,7 @@ int legacy_impl(const GtsTissueView* v, const GtsStepParams* p, const GtsGraspPa
     const int N = v->n_nodes;
     if (cut) { int rc_ = plan_tick(v, cut, emit_ncut, tear_tau, P); if (rc_) return rc_; }
     P.legacy = true;
-    P.tiled = false;  // the coupling passes of the legacy grasp keep the per-colour launches
+    P.tiled = P.tiledEdges = false;  // the coupling passes of the legacy grasp keep the per-colour launches
     P.grasping = g->grasping != 0;
     P.gate = tool_armed != 0;
     P.K = P.grasping ? 2 : 1;  // a second material pass only while genuinely grasping (xpbd_solver.cu:3571)

@@ -128,8 +128,8 @@ class Tissue:
         LDS, at most PERSISTENT_COLOUR_TETS tets per colour) steps without tool contact, guard, bending, edge damping or
         cut tool as one persistent workgroup running the whole step (the batched kernel over this tissue alone):
         colours are separated by workgroup barriers instead of launches, with the same results. `tiles`: (region,
-        kmax) runs each tet sweep as tiled launches of up to kmax colours over regions of region^3 cells (see
-        set_tiles); None keeps one launch per colour."""
+        kmax[, edges]) runs each tet sweep (and with edges the edge sweep) as tiled launches of up to kmax colours
+        over regions of region^3 cells (see set_tiles); None keeps one launch per colour."""
         self.lib = load()
         if not torch.cuda.is_available():
             raise _lib.NativeError("no GPU visible: the HIP tissue path has no CPU fallback")
@@ -257,11 +257,12 @@ class Tissue:
         if tiles is not None:
             self.set_tiles(*tiles)
 
-    def set_tiles(self, region=8, kmax=3):
+    def set_tiles(self, region=8, kmax=3, edges=False):
         """Run each tet sweep of an undamped, unguarded step as tiled launches: up to `kmax` consecutive colours per
-        launch, one workgroup per region of region^3 lattice cells (tiling.build_tile_plan); the results are the
-        per-colour sweep's. region=None turns it off. The plan holds for the current tet lists: after a compaction the
-        step takes one launch per colour again until set_tiles is called anew."""
+        launch, one workgroup per region of region^3 lattice cells (tiling.build_tile_plan), and with `edges` each
+        undamped edge sweep without the fallback compliance as launches of up to min(3, kmax - 1) edge colours over the
+        same regions; the results are the per-colour sweeps'. region=None turns it off. The plan holds for the current tet and edge
+        lists: after a compaction the step takes one launch per colour again until set_tiles is called anew."""
         from . import tiling
         v = self.view
         if region is None:
@@ -269,7 +270,9 @@ class Tissue:
             v.tile_groups = 0
             return None
         plan = tiling.build_tile_plan(self.grid.dims, self.tet_desc[:self.T].cpu().numpy().view(np.uint32),
-                                      self.tet_offsets, region=region, kmax=kmax)
+                                      self.tet_offsets, region=region, kmax=kmax,
+                                      edge_desc=self.edge_desc[:self.E].cpu().numpy().view(np.uint32) if edges else None,
+                                      edge_offsets=self.edge_offsets if edges else None)
         if plan is None:
             self._tiles = None
             v.tile_groups = 0
@@ -284,7 +287,7 @@ class Tissue:
                     lam=torch.zeros(2 * n_entries, dtype=torch.float32, device=dev))
         self._tiles = keep
         v.tile_groups = len(plan["groups"])
-        v.tile_kmax, v.tile_box, v.tile_tets = plan["kmax"], plan["box"], plan["tets"]
+        v.tile_kmax, v.tile_box, v.tile_tets, v.tile_edges = plan["kmax"], plan["box"], plan["tets"], plan["edges"]
         v.tile_group_info = keep["groups"].ctypes.data
         v.tile_entries = keep["entries"].data_ptr()
         v.tile_writes = keep["writes"].data_ptr()

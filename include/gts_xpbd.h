@@ -179,6 +179,8 @@ typedef struct GtsTissueView {
     int tile_region, tile_rx, tile_ry;  /* cells along a region's edge; regions along x and y of the dense numbering */
     const int32_t* tile_dense_ids;      /* host [regions per launch, at most 640]: dense id rx + tile_rx (ry +
                                          * tile_ry rz) of each region taking part, in header order */
+    int tile_edges;                     /* edges the plan's edge groups (kind 1, after the tet groups) hold; a view
+                                         * whose edge lists differ takes the per-colour edge launches */
 } GtsTissueView;
 
 /* Ribbon ticks a released grasp-ring edge stays exempt from the tear law while still longer than 0.8 tau L

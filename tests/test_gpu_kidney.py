@@ -3,8 +3,8 @@ per-tet compliance, the 2 mm capsule grasper pressing in with friction contact, 
 after every step, 20 substeps of 2 iterations, gravity ramping in. Three steps on the GPU against the oracle on the
 same seeded inputs: positions, velocities, the live neighbour table, cut masks, cut points and tet / edge liveness
 bit-exact, in the reference's direct tet forms and in the precise ones, one launch per colour and with the tet sweep
-tiled (three colours per launch over regions of 8^3 cells; four over 7^3, the benchmark's choice, in both forms); the
-FP64 oracle's error is reported."""
+tiled (three colours per launch over regions of 8^3 cells; four over 7^3, the benchmark's choice, in both forms; the
+same with the edge sweep tiled too, three edge colours a launch); the FP64 oracle's error is reported."""
 import numpy as np
 import pytest
 
@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("precise,tiles", [(False, None), (True, None), (True, (8, 3)), (True, (7, 4)),
-                                           (False, (7, 4))])
+                                           (False, (7, 4)), (True, (7, 4, True))])
 def test_kidney_full_size_matches_oracle(precise, tiles):
     import torch
 

@@ -2,7 +2,8 @@
 lattice region, halo tets recomputed in LDS) against the FP32 oracle, which sweeps one colour at a time: positions,
 velocities, multipliers and cut topology bit-exact after every step, for every tet form, in the direct and precise
 forms, with one and two iterations, two to four colours per launch, regions of 3 and 4 cells on the C1 block with the
-blade cutting every step; a compaction hands the step back to the per-colour launches."""
+blade cutting every step, with the edge sweep tiled too (one or two edge colours a launch) or left per colour; a
+compaction hands the step back to the per-colour launches."""
 import numpy as np
 import pytest
 
@@ -12,10 +13,10 @@ from genesis_tissue_simulation_amd.workloads import block_6144
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("mode,precise,iterations,region,kmax", [
-    (TET_NEOHOOKEAN, True, 2, 4, 3), (TET_NEOHOOKEAN, False, 1, 3, 2), (TET_VOLUME, False, 2, 4, 4),
-    (TET_GUARDED, True, 2, 3, 3), (TET_NEOHOOKEAN, True, 3, 4, 3)])
-def test_tiled_sweep_matches_oracle(mode, precise, iterations, region, kmax):
+@pytest.mark.parametrize("mode,precise,iterations,region,kmax,edges", [
+    (TET_NEOHOOKEAN, True, 2, 4, 3, True), (TET_NEOHOOKEAN, False, 1, 3, 2, True), (TET_VOLUME, False, 2, 4, 4, False),
+    (TET_GUARDED, True, 2, 3, 3, True), (TET_NEOHOOKEAN, True, 3, 4, 3, False)])
+def test_tiled_sweep_matches_oracle(mode, precise, iterations, region, kmax, edges):
     import torch
 
     from genesis_tissue_simulation_amd.tissue import Tissue, swept_cut_tool
@@ -24,10 +25,11 @@ def test_tiled_sweep_matches_oracle(mode, precise, iterations, region, kmax):
     w = block_6144(precise=precise)
     p = w.params.with_(tet_mode=mode, iterations=iterations)
     kd, kh = w.stiffness()
-    print(f"tiled sweep: mode {mode}, precise {precise}, iterations {iterations}, region {region}, kmax {kmax}",
-          flush=True)
-    t = Tissue(w.grid, p, kd, kh, w.initial_positions, tiles=(region, kmax))
+    print(f"tiled sweep: mode {mode}, precise {precise}, iterations {iterations}, region {region}, kmax {kmax}, "
+          f"edges {edges}", flush=True)
+    t = Tissue(w.grid, p, kd, kh, w.initial_positions, tiles=(region, kmax, edges))
     assert t.view.tile_groups > 0 and t.view.tile_groups % 2 == 0
+    assert (t.view.tile_edges > 0) == edges
     o = Oracle(w.grid, p, w.initial_positions)
     o.set_tet_stiffness(kd, kh)
     for sim in (t, o):

@@ -149,3 +149,86 @@ def test_group_sizes_are_even_and_bounded():
             assert sum(sizes) == colours and len(sizes) % 2 == 0 and max(sizes) <= kmax
             assert len(sizes) <= -(-colours // kmax) + 1
     assert tiling.group_sizes(1, 3) is None
+
+
+def _mix(a, b, s, side):
+    """A non-commutative integer update of an edge's node value from both its nodes and its slot (wrapping uint64)."""
+    with np.errstate(over="ignore"):
+        return (a * np.uint64(0x9E3779B97F4A7C15) + b * np.uint64(0xC2B2AE3D27D4EB4F) + s * np.uint64(2 * side + 3)
+                + np.uint64(side + 1)) ^ (a >> np.uint64(29))
+
+
+@pytest.mark.parametrize("nodes,region,kmax", [((10, 7, 6), 3, 4), ((13, 9, 8), 4, 4), ((11, 8, 9), 3, 3),
+                                               ((10, 8, 9), 2, 2), ((12, 10, 9), 5, 4)])
+def test_edge_groups_replay_the_colour_sweep(nodes, region, kmax):
+    """The plan's edge launches, replayed on the CPU with each region's box, passes and write list, leave every node
+    where the edge colours applied one after another leave it; every edge is owned once, by its cell's region."""
+    g = block_grid(nodes, 0.25)
+    g.active[np.arange(len(g.active)) % 7 == 2] = 0
+    topo = topo_of(g)
+    ed, eoffs = topo["edge_desc"].view(np.uint32), topo["edge_offsets"]
+    plan = tiling.build_tile_plan(g.dims, topo["tet_desc"], topo["tet_offsets"], region=region, kmax=kmax,
+                                  edge_desc=ed, edge_offsets=eoffs)
+    vx, vy, vz = g.dims
+    nx, ny = vx + 1, vy + 1
+    nxy = nx * ny
+    S, B, H, RX, RY = plan["region"], plan["box"], plan["halo"], plan["rx"], plan["ry"]
+    groups = plan["groups"]
+    eg = groups[groups[:, 14] == 1]
+    assert plan["edges"] == len(ed)
+    if H < 1:
+        assert len(eg) == 0
+        return
+    assert len(eg) % 2 == 0 and eg[0, 0] == 0 and eg[-1, 0] + eg[-1, 1] == len(eoffs) - 1
+    assert np.all(eg[1:, 0] == eg[:-1, 0] + eg[:-1, 1]) and np.all(eg[:, 1] <= min(3, H))
+    ei = (ed >> 4).astype(np.int64)
+    ej = ei + np.array([1, nx, nxy])[(ed >> 1) & 3]
+    rng = np.random.default_rng(5)
+    vals = rng.integers(0, 2**63, size=nx * ny * (vz + 1), dtype=np.uint64)
+    ents = plan["entries"].view(np.uint32).reshape(-1, 2)
+    wrs = plan["writes"]
+    dense = plan["dense_ids"]
+    for gi in eg:
+        c0, k, regions, stride, e0, wcap, w0 = gi[0], gi[1], gi[3], gi[6], gi[7], gi[8], gi[9]
+        caps = gi[10:10 + k]
+        starts = np.concatenate(([0], np.cumsum(caps)))
+        want = vals.copy()  # the colour sweep, one colour at a time (a colour's edges share no node)
+        for c in range(c0, c0 + k):
+            sl = np.arange(eoffs[c], eoffs[c + 1])
+            a, b = want[ei[sl]], want[ej[sl]]
+            want[ei[sl]], want[ej[sl]] = _mix(a, b, sl.astype(np.uint64), 0), _mix(b, a, sl.astype(np.uint64), 1)
+        got = np.full_like(vals, 0)
+        written = np.zeros(len(vals), dtype=np.int64)
+        owners = np.zeros(len(ed), dtype=np.int64)
+        for rl in range(regions):
+            d = int(dense[rl])
+            ox, oy, oz = (d % RX) * S - H, ((d // RX) % RY) * S - H, (d // (RX * RY)) * S - H
+            lx, ly, lz = np.meshgrid(np.arange(B), np.arange(B), np.arange(B), indexing="ij")
+            x, y, z = (ox + lx).T.ravel(), (oy + ly).T.ravel(), (oz + lz).T.ravel()  # box index lx + B (ly + B lz)
+            inb = (x >= 0) & (x < nx) & (y >= 0) & (y < ny) & (z >= 0) & (z <= vz)
+            box = np.zeros(B ** 3, dtype=np.uint64)
+            box[inb] = vals[(x + nx * y + nxy * z)[inb]]
+            for p in range(k):
+                rows = ents[e0 + rl * stride + starts[p]:e0 + rl * stride + starts[p] + caps[p]]
+                rows = rows[(rows[:, 0] >> 30) & 1 == 1]
+                sl = (rows[:, 0] & 0x3fffffff).astype(np.int64)
+                owners[sl[(rows[:, 0] >> 31) == 1]] += 1
+                assert np.all(eoffs[c0 + p] <= sl) and np.all(sl < eoffs[c0 + p + 1])
+                li = rows[:, 1].astype(np.int64)
+                lj = li + np.array([1, B, B * B])[(ed[sl] >> 1) & 3]
+                assert np.all((x[li] + nx * y[li] + nxy * z[li]) == ei[sl]) and np.all(inb[li])
+                assert np.all((x[lj] + nx * y[lj] + nxy * z[lj]) == ej[sl]) and np.all(inb[lj])
+                a, b = box[li], box[lj]
+                box[li], box[lj] = _mix(a, b, sl.astype(np.uint64), 0), _mix(b, a, sl.astype(np.uint64), 1)
+            w = wrs[w0 + rl * wcap:w0 + (rl + 1) * wcap]
+            w = w[w[:, 1] >= 0]
+            got[w[:, 1]] = box[w[:, 0]]
+            written[w[:, 1]] += 1
+        touched = np.unique(np.concatenate((ei, ej)))
+        assert np.all(written[touched] == 1)
+        np.testing.assert_array_equal(got[touched], want[touched])
+        sel = np.arange(eoffs[c0], eoffs[c0 + k])
+        assert np.all(owners[sel] == 1)
+        cx = np.minimum(ei[sel] % nx, vx - 1) // S
+        assert cx.min() >= 0
+        vals = want
