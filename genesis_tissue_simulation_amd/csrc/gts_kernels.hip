@@ -350,9 +350,9 @@ __global__ void k_tets(int off, int cnt, int T, Lattice L, float omega, float da
 __device__ __forceinline__ int ordered_block(int xcd_order) { return ordered_item(xcd_order, 1) - (int)threadIdx.x; }
 
 #define GTS_TILE_KMAX 4  // colours a tiled launch may hold: each thread loads its first tet of every pass up front
-#define GTS_TILE_THREADS 512  // threads of a tiled workgroup
-#define GTS_TILE_BOX_LOADS 8  // nodes of the LDS box a thread loads: box^3 < 8 * 512 (box <= 15)
-#define GTS_TILE_WRITE_LOADS 2  // node writes a thread loads up front
+#define GTS_TILE_THREADS 1024  // threads of a tiled workgroup
+#define GTS_TILE_BOX_LOADS 4  // nodes of the LDS box a thread loads: box^3 < 4 * 1024 (box <= 15)
+#define GTS_TILE_WRITE_LOADS 1  // node writes a thread loads up front
 #define GTS_TILE_MAX_REGIONS 640  // regions of a tiled launch: their dense ids fill most of the 4 KiB argument space
 
 // Where a launch's regions find their data: every address follows from the region's index, and its box origin from
