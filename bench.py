@@ -3,7 +3,7 @@
 `python bench.py [--steps K] [--warmup W]` runs the kidney-scale single instance (an ellipsoid of ~250k Kuhn tets,
 20 substeps x 2 iterations per step, grasper contact, a Bezier blade cutting once per step) for W untimed and K
 timed steps on cuda:0, and prints one JSON line. `value` is live tets x substeps / second over the timed steps, with
-all state resident on the GPU. Its tet sweeps run tiled (several colours a launch, KIDNEY_TILES); the same workload
+all state resident on the GPU. Its tet and edge sweeps run tiled (several colours a launch, KIDNEY_TILES); the same workload
 with one launch per colour is reported beside it (plain_per_colour). The dominant kernel (the tiled tet pass) is timed
 per launch with HIP events on its stream over the same steps; `roofline.achieved` is its algorithmic bytes per tet update (the per-tet-substep figure
 (52 N + 8 T + 4 E)/T) over its measured duration. `cpu_baseline` is the CPU oracle of the same step, timed on this
@@ -24,10 +24,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# The kidney legs run their tet sweeps tiled: up to 4 consecutive colours per launch over regions of 7^3 lattice cells
-# (Tissue.set_tiles; the same bits as one launch per colour, which the line reports beside it). Measured on MI355X
-# against 2-4 colours over 6^3-8^3 regions, this was the fastest.
-KIDNEY_TILES = (7, 4)
+# The kidney legs run their tet sweeps tiled: up to 4 consecutive colours per launch over regions of 7^3 lattice cells,
+# and their edge sweeps as two launches of 3 edge colours over the same regions (Tissue.set_tiles; the same bits as
+# one launch per colour, which the line reports beside it). Measured on MI355X against 2-4 colours over 6^3-8^3
+# regions, with and without the edges tiled, this was the fastest.
+KIDNEY_TILES = (7, 4, True)
 
 
 def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
@@ -38,7 +39,7 @@ def algorithmic_bytes_per_tet_substep(active_nodes, tets, edges):
 
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic.json"
 # kernels whose HBM bytes per dispatch the traffic file records, by the name fragment rocprofv3 reports
-TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_tets_tiled": "k_tets_tiled<", "k_batched_step": "k_batched_step<",
+TRAFFIC_KERNELS = {"k_tets": "k_tets<", "k_tets_tiled": "k_tets_tiled<1,", "k_batched_step": "k_batched_step<",
                    "k_adjoint_forward": "k_adjoint_forward",
                    "k_adjoint_tape": "k_adjoint_tape", "k_adjoint_reverse": "k_adjoint_reverse",
                    "k_stream_copy": "k_stream_copy"}
@@ -673,8 +674,9 @@ def plain_cpu_baseline(w, steps, threads, info):
 
 
 def tiled_path(tiles):
-    region, kmax = tiles
-    return (f"tet sweeps tiled: up to {kmax} colours a launch over {region}^3-cell regions staged in LDS, "
+    region, kmax, *edges = tiles
+    what = "tet and edge sweeps" if edges and edges[0] else "tet sweeps"
+    return (f"{what} tiled: up to {kmax} colours a launch over {region}^3-cell regions staged in LDS, "
             f"graph-replayed")
 
 

@@ -1673,16 +1673,26 @@ void launch_tiled_sweep(const GtsTissueView* v, const Lattice& L, hipStream_t s,
         if (regions <= 0) continue;
         // updates of the launch (its colours' elements), credited while profiling
         const long long units = (long long)offsets[gi[0] + gi[1]] - offsets[gi[0]];
-#define GTS_TILED(R, W)                                                                                           \
-    profiled_launch_lds(k_tets_tiled<MODE, R, W, PRECISE, EDGES>, dim3(regions), dim3(GTS_TILE_THREADS), lds, s,  \
-        units, A, reinterpret_cast<const int2*>(v->tile_entries) + gi[7],                                         \
-        reinterpret_cast<const float4*>(v->tile_inputs) + gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7], \
-        reinterpret_cast<const int2*>(v->tile_writes) + gi[9], L, omega, alphaTilde, lam_out, in, out)
+// the profiled launches are the tet sweep's (their units are tet updates); edge launches go out directly
+#define GTS_TILED_ARGS                                                                                            \
+    A, reinterpret_cast<const int2*>(v->tile_entries) + gi[7], reinterpret_cast<const float4*>(v->tile_inputs) + \
+        gi[7], reinterpret_cast<float2*>(v->tile_lambda) + gi[7],                                                  \
+        reinterpret_cast<const int2*>(v->tile_writes) + gi[9], L, omega, alphaTilde, lam_out, in, out
+#define GTS_TILED(R, W)                                                                                              \
+    do {                                                                                                             \
+        if (EDGES)                                                                                                   \
+            k_tets_tiled<MODE, R, W, PRECISE, EDGES><<<dim3(regions), dim3(GTS_TILE_THREADS), lds, s>>>(              \
+                GTS_TILED_ARGS);                                                                                     \
+        else                                                                                                         \
+            profiled_launch_lds(k_tets_tiled<MODE, R, W, PRECISE, EDGES>, dim3(regions), dim3(GTS_TILE_THREADS), lds, \
+                                s, units, GTS_TILED_ARGS);                                                           \
+    } while (0)
         if (read && write) GTS_TILED(true, true);
         else if (read) GTS_TILED(true, false);
         else if (write) GTS_TILED(false, true);
         else GTS_TILED(false, false);
 #undef GTS_TILED
+#undef GTS_TILED_ARGS
     }
 }
 
