@@ -154,20 +154,23 @@ typedef struct GtsTissueView {
     const int32_t* bend_colour_offsets; /* host, [n_bend_colours + 1] */
     const uint32_t* bend_desc;          /* [n_bends] */
     float* bend_lambda;                 /* [n_bends] */
-    /* Tiled tet sweep (no reference counterpart; the results are the per-colour sweep's bit for bit): with
+    /* Tiled tet and edge sweeps (no reference counterpart; the results are the per-colour sweeps' bit for bit): with
      * tile_groups > 0 an undamped, unguarded step without the legacy grasp or a ribbon tick runs its tet colours in
-     * tile_groups launches of up to tile_kmax consecutive colours each, one workgroup per lattice region, positions in
-     * an LDS box of tile_box^3 nodes around the region (the plan: genesis_tissue_simulation_amd/tiling.py). Each launch
-     * reads the positions from one buffer and writes them to the other (pos, pos_alt alternate; tile_groups is even).
-     * Every input of a region sits at an address it computes from its index: its entries and writes at fixed strides, its box origin from its dense id (a kernel argument), and per entry the tet's inputs (tile_inputs, gathered at the start of
-     * each step) and its multipliers (tile_lambda: each region keeps the copies of the tets it projects; the owners
-     * also write tet_lambda). The plan holds for tile_tets tets in the current colour lists; a view whose lists
-     * differ (a compaction) takes the per-colour launches. */
+     * launches of up to tile_kmax consecutive colours each (the groups of kind 0), one workgroup per lattice region,
+     * positions in an LDS box of tile_box^3 nodes around the region (the plan: genesis_tissue_simulation_amd/tiling.py);
+     * where the plan also holds groups of kind 1, each undamped edge sweep without the fallback compliance runs as
+     * launches of up to tile_kmax - 1 edge colours over the same regions. Each launch reads the positions from one
+     * buffer and writes them to the other (pos, pos_alt alternate; each kind has an even number of groups). Every input
+     * of a region sits at an address it computes from its index: its entries and writes at fixed strides, its box
+     * origin from its dense id (a kernel argument), and per entry the element's inputs (tile_inputs, gathered at the
+     * start of each step) and its multipliers (tile_lambda: each region keeps the copies of the elements it projects;
+     * the owners also write tet_lambda or edge_lambda). The plan holds for tile_tets tets and tile_edges edges in the
+     * current colour lists; a view whose lists differ (a compaction) takes the per-colour launches. */
     int tile_groups;
     int tile_kmax, tile_box, tile_tets;
     const int32_t* tile_group_info;     /* host [tile_groups * 16]: first colour, colours, first region, regions, box
                                          * width, box halo, entry rows a region, first entry, write capacity, first
-                                         * write, the capacity of each of 4 passes, 0, 0 */
+                                         * write, the capacity of each of 4 passes, kind (0 tets, 1 edges), 0 */
     const int32_t* tile_entries;        /* [2 * entries]: tet slot | 1 << 30 | owned << 31, corner-0 index in the LDS
                                          * box (padding rows: 0, 0); a region's rows start at first entry + region *
                                          * (entry rows), pass p's after the capacities of the passes before it */
